@@ -1,0 +1,94 @@
+// hss_internal.h -- device-side tables shared by the HIP kernels and the host
+// runtime of libhss_hip.so.  Not part of the public ABI (include/hss_hip.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hss {
+
+// Reflector block size of the compact-WY representation used by the solve.
+constexpr int NB = 16;
+// Threads of the factor kernel (one workgroup per tree node).
+constexpr int FACT_THREADS = 1024;
+// Columns of the node matrix held in registers by one factor thread.
+constexpr int FACT_CPT = 32;
+// Threads of the matvec / solve kernels.
+constexpr int SWEEP_THREADS = 256;
+// Right-hand-side columns per workgroup in the matvec / solve sweeps.
+constexpr int RHS_TILE = 32;
+
+// One tree node of a matrix, derived from hss_node_desc at hss_mat_create.
+struct DevNode {
+    int32_t m, n, ku, kv;       // generator sizes; ku/kv = basis ranks (0 at the root)
+    int32_t left, right, parent;
+    int32_t is_leaf, is_left;   // is_left: this node is its parent's first child
+    int32_t depth, height, pad;
+    int64_t row_off, col_off;
+    int64_t D, U, V, B12, B21, R, W;  // element offsets into the generator blob (-1 = none)
+    int64_t zrow, frow;         // matvec workspace rows: Z (kv rows), F (ku rows)
+};
+
+// Factorization metadata of one node (host-computed before the factor runs).
+struct FNode {
+    int32_t m, n;      // size of the node's (merged) diagonal block
+    int32_t k, nk;     // k = min(ku, m) eliminated-by-QL rank; nk = m - k LQ reflectors
+    int32_t mr, nr;    // rows / columns passed to the parent
+    int32_t pass;      // 1: k >= m, nothing is eliminated (src/ulvfactor.jl:32-36)
+    int32_t S;         // factor threads per matrix row
+    int32_t nblk;      // ceil(nk / NB)
+    int32_t pad;
+    // factor storage (elements of the factor type), -1 = none
+    int64_t Yql, Tql, Ylq, taulq, Tlq, L1, L1inv, L2a, Vt, P12, P21, LU, piv;
+    // factorization-time workspace (reduced D/U/V passed to the parent, merged D scratch)
+    int64_t rwD, rwU, rwV, rwM;
+    // solve workspace rows (times nrhs): reduced b (mr rows), u (kv rows), z (n rows)
+    int64_t brow, urow, zrow;
+};
+
+struct WorkItem {
+    int32_t inst, node;
+};
+
+// Per-instance arguments of one matvec launch sequence.
+struct MvInst {
+    const DevNode* nodes;
+    const double* blob;
+    const double* B;
+    double* C;
+    int64_t ldb, ldc;
+    double* Z;
+    double* F;
+    double alpha, beta;
+    int32_t nrhs, pad;
+};
+
+// Per-instance arguments of a factorization.
+template <typename TF>
+struct FInst {
+    const DevNode* nodes;
+    const FNode* fn;
+    const double* blob;
+    TF* fac;
+    TF* rw;
+    int32_t* ipiv;
+    int32_t* status;  // first zero pivot (1-based) of the root LU, 0 = none
+};
+
+// Per-instance arguments of a solve.
+template <typename TF, typename TV>
+struct SInst {
+    const DevNode* nodes;
+    const FNode* fn;
+    const TF* fac;
+    const int32_t* ipiv;
+    const double* blob;
+    const double* B;
+    double* X;
+    int64_t ldb, ldx;
+    TV* wb;
+    TV* wu;
+    TV* wz;
+    int32_t nrhs, pad;
+};
+
+}  // namespace hss

@@ -1,0 +1,35 @@
+// hss_launch.h -- host launchers of the kernels in hss_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hss_internal.h"
+
+namespace hss {
+
+hipError_t launch_mv_up(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                        hipStream_t st);
+hipError_t launch_mv_down(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, hipStream_t st);
+hipError_t launch_mv_leaf(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                          hipStream_t st);
+
+template <typename T>
+hipError_t launch_ulv_factor(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds, hipStream_t st);
+template <typename T>
+hipError_t launch_ulv_wy(const FInst<T>* insts, const WorkItem* work, int nwork, hipStream_t st);
+template <typename T>
+hipError_t launch_ulv_root(const FInst<T>* insts, const WorkItem* work, int nwork, hipStream_t st);
+template <typename TF, typename TV>
+hipError_t launch_ulv_sup(const SInst<TF, TV>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                          hipStream_t st);
+template <typename TF, typename TV>
+hipError_t launch_ulv_sroot(const SInst<TF, TV>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                            hipStream_t st);
+template <typename TF, typename TV>
+hipError_t launch_ulv_sdown(const SInst<TF, TV>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                            hipStream_t st);
+
+hipError_t launch_residual(const double* B, int64_t ldb, double* R, int64_t ldr, int64_t n, int nrhs, hipStream_t st);
+hipError_t launch_axpy(double* X, int64_t ldx, const double* D, int64_t ldd, int64_t n, int nrhs, hipStream_t st);
+hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, double* out, hipStream_t st);
+
+}  // namespace hss

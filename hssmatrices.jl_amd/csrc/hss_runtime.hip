@@ -1,0 +1,1094 @@
+// hss_runtime.hip -- host runtime and C ABI of libhss_hip.so (include/hss_hip.h).
+//
+// Owns device memory, builds the level-major work lists from the node table,
+// and sequences the kernel launches of hss_kernels.hip:
+//   matvec  : up-sweep by height, down-sweep by depth, leaf products
+//             (reference src/matmul.jl:18-62)
+//   factor  : one launch per height (all nodes of that height, of every matrix
+//             in a batch), then the root LU (src/ulvfactor.jl:60-96)
+//   solve   : up-sweep by height, root solve, down-sweep by depth
+//             (src/ulvfactor.jl:10-19, 98-107)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/hss_hip.h"
+#include "hss_internal.h"
+#include "hss_launch.h"
+
+using namespace hss;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+static thread_local double g_last_ms = -1.0;
+
+static hss_status fail(hss_status st, const std::string& msg) {
+    g_err = msg;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess)                                                                      \
+            return fail(_e == hipErrorOutOfMemory ? HSS_ERR_OOM : HSS_ERR_HIP,                     \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));                        \
+    } while (0)
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+struct Range {
+    int off = 0, count = 0;
+};
+
+// Device copies of per-call instance records.  A record is uploaded once and
+// reused while the same (pointers, nrhs, ...) come back, so repeated calls
+// enqueue no host-to-device copy.  Replacing a slot first waits for the
+// stream, so no kernel still reads it, and then copies synchronously from
+// host memory that stays owned by the cache.
+struct SlotCache {
+    static constexpr int SLOTS = 4;
+    size_t rec = 0;
+    char* d = nullptr;
+    std::vector<char> h;
+    int used = 0, next = 0;
+    ~SlotCache() { (void)hipFree(d); }
+    hipError_t get(const void* v, size_t bytes, hipStream_t st, const void** out) {
+        if (!d) {
+            rec = bytes;
+            hipError_t e = hipMalloc(reinterpret_cast<void**>(&d), rec * SLOTS);
+            if (e != hipSuccess) return e;
+            h.assign(rec * SLOTS, 0);
+        }
+        if (bytes != rec) return hipErrorInvalidValue;
+        for (int i = 0; i < used; ++i)
+            if (std::memcmp(h.data() + i * rec, v, rec) == 0) {
+                *out = d + i * rec;
+                return hipSuccess;
+            }
+        int slot = used < SLOTS ? used++ : next;
+        next = (slot + 1) % SLOTS;
+        hipError_t e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+        std::memcpy(h.data() + slot * rec, v, rec);
+        e = hipMemcpy(d + slot * rec, v, rec, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return e;
+        *out = d + slot * rec;
+        return hipSuccess;
+    }
+};
+
+struct hss_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct hss_mat_s {
+    hss_ctx_s* ctx = nullptr;
+    int nn = 0;
+    int64_t rows = 0, cols = 0;
+    std::vector<hss_node_desc> desc;
+    std::vector<DevNode> hn;
+    double* d_blob = nullptr;
+    int64_t blob_len = 0;
+    DevNode* d_nodes = nullptr;
+    // work lists (instance 0), concatenated in one device buffer
+    std::vector<WorkItem> hwork;
+    WorkItem* d_work = nullptr;
+    std::vector<Range> by_height;  // non-root nodes of height h
+    std::vector<Range> br_depth;   // branch nodes of depth d
+    std::vector<Range> by_depth;   // nodes of depth d (d >= 1)
+    Range leaves, root;
+    int max_height = 0, max_depth = 0;
+    int64_t zrows = 0, frows = 0;
+    size_t lds_up = 0, lds_leaf = 0;  // per RHS tile
+    // per-call scratch
+    SlotCache mvinst;
+    double *d_Z = nullptr, *d_F = nullptr;
+    size_t zf_cap = 0;  // nrhs capacity of Z/F
+    double *d_io_b = nullptr, *d_io_c = nullptr;
+    size_t io_cap = 0;  // elements
+    int64_t dev_bytes = 0;
+};
+
+struct hss_fact_s {
+    hss_mat_s* mat = nullptr;
+    hss_prec prec = HSS_PREC_FP64;
+    std::vector<FNode> hfn;
+    FNode* d_fn = nullptr;
+    void* d_fac = nullptr;
+    int64_t fac_len = 0;
+    int32_t* d_piv = nullptr;
+    int32_t* d_status = nullptr;
+    int64_t brows = 0, urows = 0, zrows = 0;
+    size_t lds_sup = 0, lds_sroot = 0, lds_sdown = 0;  // per RHS tile
+    void *d_wb = nullptr, *d_wu = nullptr, *d_wz = nullptr;
+    size_t ws_cap = 0;  // nrhs capacity
+    SlotCache sinst;
+    // single-request solve schedule (instance 0)
+    WorkItem* d_swork = nullptr;
+    std::vector<Range> s_up, s_down;
+    Range s_root;
+    // refinement scratch (Float64): residual and correction
+    double *d_r = nullptr, *d_d = nullptr, *d_nrm = nullptr;
+    size_t rf_cap = 0;
+    double *d_io_b = nullptr, *d_io_x = nullptr;
+    size_t io_cap = 0;
+    int64_t dev_bytes = 0;
+};
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" hss_status hss_ctx_create(int device, hss_ctx_t* out) {
+    if (!out) return fail(HSS_ERR_ARG, "null output handle");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(HSS_ERR_ARG, "device index out of range");
+    HIP_TRY(hipSetDevice(device));
+    auto* c = new (std::nothrow) hss_ctx_s;
+    if (!c) return fail(HSS_ERR_OOM, "host allocation");
+    c->device = device;
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&c->ev0));
+    HIP_TRY(hipEventCreate(&c->ev1));
+    *out = c;
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_ctx_destroy(hss_ctx_t c) {
+    if (!c) return HSS_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipEventDestroy(c->ev0);
+    (void)hipEventDestroy(c->ev1);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return HSS_OK;
+}
+
+extern "C" void* hss_ctx_stream(hss_ctx_t c) { return c ? (void*)c->stream : nullptr; }
+
+// ---------------------------------------------------------------------------
+// matrix
+// ---------------------------------------------------------------------------
+static hss_status validate(const hss_node_desc* nd, int nn, int64_t blob_len) {
+    if (nn <= 0) return fail(HSS_ERR_ARG, "empty node table");
+    if (nd[0].parent != -1) return fail(HSS_ERR_ARG, "node 0 must be the root");
+    auto in_blob = [&](int64_t off, int64_t count) { return count == 0 || (off >= 0 && off + count <= blob_len); };
+    for (int i = 0; i < nn; ++i) {
+        const hss_node_desc& x = nd[i];
+        if (x.m < 0 || x.n < 0 || x.ku < 0 || x.kv < 0)
+            return fail(HSS_ERR_ARG, "negative size in node " + std::to_string(i));
+        const bool leaf = x.left < 0;
+        if (leaf != (x.right < 0)) return fail(HSS_ERR_ARG, "node " + std::to_string(i) + " has one child");
+        if (i > 0) {
+            if (x.parent < 0 || x.parent >= i) return fail(HSS_ERR_ARG, "nodes must be breadth-first ordered");
+            const hss_node_desc& p = nd[x.parent];
+            if (p.left != i && p.right != i) return fail(HSS_ERR_ARG, "parent/child links disagree");
+            if (!in_blob(x.off_R, (int64_t)x.ku * p.ku) || !in_blob(x.off_W, (int64_t)x.kv * p.kv))
+                return fail(HSS_ERR_ARG, "translator out of the blob in node " + std::to_string(i));
+        } else if (x.ku != 0 || x.kv != 0) {
+            return fail(HSS_ERR_DIM, "root must have 0-column R/W (ku = kv = 0)");
+        }
+        if (leaf) {
+            if (!in_blob(x.off_D, (int64_t)x.m * x.n) || !in_blob(x.off_U, (int64_t)x.m * x.ku) ||
+                !in_blob(x.off_V, (int64_t)x.n * x.kv))
+                return fail(HSS_ERR_ARG, "leaf generator out of the blob in node " + std::to_string(i));
+            if ((int64_t)x.m * x.n > 0 && x.off_D < 0) return fail(HSS_ERR_ARG, "leaf without D");
+        } else {
+            if (x.left <= i || x.right <= i || x.left >= nn || x.right >= nn)
+                return fail(HSS_ERR_ARG, "bad child index in node " + std::to_string(i));
+            const hss_node_desc& l = nd[x.left];
+            const hss_node_desc& r = nd[x.right];
+            if (l.m + r.m != x.m || l.n + r.n != x.n)
+                return fail(HSS_ERR_DIM, "children sizes do not add up in node " + std::to_string(i));
+            if (l.row_off != x.row_off || r.row_off != x.row_off + l.m || l.col_off != x.col_off ||
+                r.col_off != x.col_off + l.n)
+                return fail(HSS_ERR_ARG, "row/column offsets inconsistent in node " + std::to_string(i));
+            if (!in_blob(x.off_B12, (int64_t)l.ku * r.kv) || !in_blob(x.off_B21, (int64_t)r.ku * l.kv))
+                return fail(HSS_ERR_ARG, "coupling out of the blob in node " + std::to_string(i));
+        }
+    }
+    return HSS_OK;
+}
+
+static void free_mat(hss_mat_s* M) {
+    if (!M) return;
+    (void)hipFree(M->d_blob);
+    (void)hipFree(M->d_nodes);
+    (void)hipFree(M->d_work);
+    (void)hipFree(M->d_Z);
+    (void)hipFree(M->d_F);
+    (void)hipFree(M->d_io_b);
+    (void)hipFree(M->d_io_c);
+    delete M;
+}
+
+extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, int32_t nn, const double* blob,
+                                     int64_t blob_len, hss_mat_t* out) {
+    if (!ctx || !nodes || !out || (blob_len > 0 && !blob)) return fail(HSS_ERR_ARG, "null argument");
+    *out = nullptr;
+    hss_status vs = validate(nodes, nn, blob_len);
+    if (vs != HSS_OK) return vs;
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto* M = new (std::nothrow) hss_mat_s;
+    if (!M) return fail(HSS_ERR_OOM, "host allocation");
+    M->ctx = ctx;
+    M->nn = nn;
+    M->rows = nodes[0].m;
+    M->cols = nodes[0].n;
+    M->desc.assign(nodes, nodes + nn);
+    M->blob_len = blob_len;
+    M->hn.resize(nn);
+    int64_t zr = 0, fr = 0;
+    for (int i = 0; i < nn; ++i) {
+        const hss_node_desc& x = nodes[i];
+        DevNode& d = M->hn[i];
+        std::memset(&d, 0, sizeof(d));
+        d.m = x.m; d.n = x.n; d.ku = x.ku; d.kv = x.kv;
+        d.left = x.left; d.right = x.right; d.parent = x.parent;
+        d.is_leaf = x.left < 0;
+        d.is_left = (x.parent >= 0 && nodes[x.parent].left == i) ? 1 : 0;
+        d.depth = x.depth; d.height = x.height;
+        d.row_off = x.row_off; d.col_off = x.col_off;
+        d.D = x.off_D; d.U = x.off_U; d.V = x.off_V; d.B12 = x.off_B12; d.B21 = x.off_B21;
+        d.R = x.off_R; d.W = x.off_W;
+        d.zrow = zr; zr += x.kv;
+        d.frow = fr; fr += x.ku;
+    }
+    // recompute heights/depths (the caller's values are advisory)
+    for (int i = nn - 1; i >= 0; --i) {
+        DevNode& d = M->hn[i];
+        d.height = d.is_leaf ? 0 : 1 + std::max(M->hn[d.left].height, M->hn[d.right].height);
+    }
+    for (int i = 0; i < nn; ++i) M->hn[i].depth = (i == 0) ? 0 : M->hn[M->hn[i].parent].depth + 1;
+    M->zrows = zr;
+    M->frows = fr;
+    for (int i = 0; i < nn; ++i) {
+        M->max_height = std::max(M->max_height, M->hn[i].height);
+        M->max_depth = std::max(M->max_depth, M->hn[i].depth);
+    }
+    // work lists
+    auto add_range = [&](std::vector<Range>& v, auto pred, int levels, auto level_of) {
+        v.assign(levels, Range{});
+        for (int L = 0; L < levels; ++L) {
+            v[L].off = (int)M->hwork.size();
+            for (int i = 0; i < nn; ++i)
+                if (pred(i) && level_of(i) == L) M->hwork.push_back(WorkItem{0, i});
+            v[L].count = (int)M->hwork.size() - v[L].off;
+        }
+    };
+    add_range(M->by_height, [&](int i) { return i != 0; }, M->max_height, [&](int i) { return M->hn[i].height; });
+    add_range(M->br_depth, [&](int i) { return !M->hn[i].is_leaf; }, M->max_depth + 1,
+              [&](int i) { return M->hn[i].depth; });
+    add_range(M->by_depth, [&](int i) { return i != 0; }, M->max_depth + 1, [&](int i) { return M->hn[i].depth; });
+    M->leaves.off = (int)M->hwork.size();
+    for (int i = 0; i < nn; ++i)
+        if (M->hn[i].is_leaf) M->hwork.push_back(WorkItem{0, i});
+    M->leaves.count = (int)M->hwork.size() - M->leaves.off;
+    M->root.off = (int)M->hwork.size();
+    M->hwork.push_back(WorkItem{0, 0});
+    M->root.count = 1;
+    size_t maxn = 0, maxleaf = 0;
+    for (int i = 0; i < nn; ++i)
+        if (M->hn[i].is_leaf) {
+            maxn = std::max(maxn, (size_t)M->hn[i].n);
+            maxleaf = std::max(maxleaf, (size_t)(M->hn[i].n + M->hn[i].ku));
+        }
+    M->lds_up = maxn * RHS_TILE * sizeof(double);
+    M->lds_leaf = maxleaf * RHS_TILE * sizeof(double);
+    if (M->lds_leaf > 160 * 1024) {
+        free_mat(M);
+        return fail(HSS_ERR_UNSUPPORTED, "leaf too large for the LDS-staged matvec");
+    }
+
+    hipError_t e;
+    if ((e = dalloc(&M->d_blob, (size_t)blob_len)) != hipSuccess ||
+        (e = dalloc(&M->d_nodes, (size_t)nn)) != hipSuccess ||
+        (e = dalloc(&M->d_work, M->hwork.size())) != hipSuccess) {
+        free_mat(M);
+        return fail(HSS_ERR_OOM, std::string("hss_mat_create: ") + hipGetErrorString(e));
+    }
+    hipStream_t st = ctx->stream;
+    if (blob_len > 0) HIP_TRY(hipMemcpyAsync(M->d_blob, blob, sizeof(double) * blob_len, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(M->d_nodes, M->hn.data(), sizeof(DevNode) * nn, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(M->d_work, M->hwork.data(), sizeof(WorkItem) * M->hwork.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    M->dev_bytes = (int64_t)(sizeof(double) * blob_len + sizeof(DevNode) * nn + sizeof(WorkItem) * M->hwork.size());
+    *out = M;
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_mat_destroy(hss_mat_t M) {
+    if (!M) return HSS_OK;
+    (void)hipSetDevice(M->ctx->device);
+    (void)hipStreamSynchronize(M->ctx->stream);
+    free_mat(M);
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_mat_size(hss_mat_t M, int64_t* rows, int64_t* cols) {
+    if (!M) return fail(HSS_ERR_ARG, "null matrix");
+    if (rows) *rows = M->rows;
+    if (cols) *cols = M->cols;
+    return HSS_OK;
+}
+
+extern "C" int64_t hss_mat_device_bytes(hss_mat_t M) { return M ? M->dev_bytes : 0; }
+
+// ---------------------------------------------------------------------------
+// matvec
+// ---------------------------------------------------------------------------
+static hss_status ensure_zf(hss_mat_s* M, int nrhs) {
+    if ((size_t)nrhs <= M->zf_cap) return HSS_OK;
+    (void)hipFree(M->d_Z);
+    (void)hipFree(M->d_F);
+    M->d_Z = M->d_F = nullptr;
+    M->zf_cap = 0;
+    HIP_TRY(dalloc(&M->d_Z, (size_t)std::max<int64_t>(M->zrows, 1) * nrhs));
+    HIP_TRY(dalloc(&M->d_F, (size_t)std::max<int64_t>(M->frows, 1) * nrhs));
+    M->zf_cap = nrhs;
+    return HSS_OK;
+}
+
+static hss_status matmul_enqueue(hss_mat_s* M, int nrhs, double alpha, const double* dB, int64_t ldb, double beta,
+                                 double* dC, int64_t ldc) {
+    if (nrhs < 0) return fail(HSS_ERR_ARG, "nrhs < 0");
+    if (nrhs == 0 || M->rows == 0) return HSS_OK;
+    if (ldb < std::max<int64_t>(M->cols, 1) || ldc < std::max<int64_t>(M->rows, 1))
+        return fail(HSS_ERR_DIM, "leading dimension smaller than the matrix size");
+    hss_status s = ensure_zf(M, nrhs);
+    if (s != HSS_OK) return s;
+    hipStream_t st = M->ctx->stream;
+    MvInst I;
+    std::memset(&I, 0, sizeof(I));
+    I.nodes = M->d_nodes;
+    I.blob = M->d_blob;
+    I.B = dB;
+    I.C = dC;
+    I.ldb = ldb;
+    I.ldc = ldc;
+    I.Z = M->d_Z;
+    I.F = M->d_F;
+    I.alpha = alpha;
+    I.beta = beta;
+    I.nrhs = nrhs;
+    const void* dv = nullptr;
+    HIP_TRY(M->mvinst.get(&I, sizeof(I), st, &dv));
+    const MvInst* di = reinterpret_cast<const MvInst*>(dv);
+    if (M->nn > 1) {
+        for (int h = 0; h < M->max_height; ++h) {
+            const Range& r = M->by_height[h];
+            HIP_TRY(launch_mv_up(di, M->d_work + r.off, r.count, nrhs, h == 0 ? M->lds_up : 0, st));
+        }
+        for (int d = 0; d <= M->max_depth; ++d) {
+            const Range& r = M->br_depth[d];
+            HIP_TRY(launch_mv_down(di, M->d_work + r.off, r.count, nrhs, st));
+        }
+    }
+    HIP_TRY(launch_mv_leaf(di, M->d_work + M->leaves.off, M->leaves.count, nrhs, M->lds_leaf, st));
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_matmul_dev(hss_mat_t M, int32_t nrhs, double alpha, const double* dB, int64_t ldb,
+                                     double beta, double* dC, int64_t ldc) {
+    if (!M) return fail(HSS_ERR_ARG, "null matrix");
+    HIP_TRY(hipSetDevice(M->ctx->device));
+    return matmul_enqueue(M, nrhs, alpha, dB, ldb, beta, dC, ldc);
+}
+
+static hss_status ensure_io(double** a, double** b, size_t* cap, size_t need) {
+    if (need <= *cap) return HSS_OK;
+    (void)hipFree(*a);
+    (void)hipFree(*b);
+    *a = *b = nullptr;
+    *cap = 0;
+    HIP_TRY(dalloc(a, need));
+    HIP_TRY(dalloc(b, need));
+    *cap = need;
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_matmul(hss_mat_t M, int32_t nrhs, double alpha, const double* B, int64_t ldb, double beta,
+                                 double* C, int64_t ldc) {
+    if (!M || (!B && nrhs > 0) || (!C && nrhs > 0)) return fail(HSS_ERR_ARG, "null argument");
+    if (nrhs < 0) return fail(HSS_ERR_ARG, "nrhs < 0");
+    if (nrhs == 0) return HSS_OK;
+    if (ldb < std::max<int64_t>(M->cols, 1) || ldc < std::max<int64_t>(M->rows, 1))
+        return fail(HSS_ERR_DIM, "leading dimension smaller than the matrix size");
+    HIP_TRY(hipSetDevice(M->ctx->device));
+    const int64_t nb = std::max<int64_t>(M->cols, 1), nc = std::max<int64_t>(M->rows, 1);
+    hss_status s = ensure_io(&M->d_io_b, &M->d_io_c, &M->io_cap, (size_t)std::max(nb, nc) * nrhs);
+    if (s != HSS_OK) return s;
+    hipStream_t st = M->ctx->stream;
+    HIP_TRY(hipMemcpy2DAsync(M->d_io_b, nb * sizeof(double), B, ldb * sizeof(double), M->cols * sizeof(double), nrhs,
+                             hipMemcpyHostToDevice, st));
+    if (beta != 0.0)
+        HIP_TRY(hipMemcpy2DAsync(M->d_io_c, nc * sizeof(double), C, ldc * sizeof(double), M->rows * sizeof(double),
+                                 nrhs, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(M->ctx->ev0, st));
+    s = matmul_enqueue(M, nrhs, alpha, M->d_io_b, nb, beta, M->d_io_c, nc);
+    if (s != HSS_OK) return s;
+    HIP_TRY(hipEventRecord(M->ctx->ev1, st));
+    HIP_TRY(hipMemcpy2DAsync(C, ldc * sizeof(double), M->d_io_c, nc * sizeof(double), M->rows * sizeof(double), nrhs,
+                             hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, M->ctx->ev0, M->ctx->ev1) == hipSuccess) g_last_ms = ms;
+    return HSS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ULV factorization
+// ---------------------------------------------------------------------------
+static int pow2_at_least(int x) {
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+template <typename TF>
+static size_t factor_lds(const FNode& f, const DevNode& d) {
+    const size_t nw = FACT_THREADS / 64;
+    const size_t kk = (size_t)std::max(f.k, f.nk) + 1;
+    return sizeof(TF) * ((size_t)f.m * d.ku + 2 * (size_t)f.n + nw * f.n + f.n + 8 + kk + (size_t)f.k * f.k);
+}
+
+// Fill the factor metadata of one matrix; offsets are relative to this
+// matrix's own factor / workspace buffers.
+template <typename TF>
+static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
+    const int nn = M->nn;
+    F->hfn.assign(nn, FNode{});
+    int64_t fac = 0, rw = 0, piv = 0, brow = 0, urow = 0, zrow = 0;
+    auto take = [](int64_t& cur, int64_t count) {
+        int64_t off = cur;
+        cur += std::max<int64_t>(count, 0);
+        return off;
+    };
+    // post-order = reverse breadth-first is enough: children have larger indices
+    for (int i = nn - 1; i >= 0; --i) {
+        const DevNode& d = M->hn[i];
+        FNode& f = F->hfn[i];
+        std::memset(&f, 0, sizeof(f));
+        f.Yql = f.Tql = f.Ylq = f.taulq = f.Tlq = f.L1 = f.L1inv = f.L2a = f.Vt = f.P12 = f.P21 = f.LU = f.piv = -1;
+        f.rwD = f.rwU = f.rwV = f.rwM = -1;
+        if (d.is_leaf) {
+            f.m = d.m;
+            f.n = d.n;
+        } else {
+            const FNode& a = F->hfn[d.left];
+            const FNode& b = F->hfn[d.right];
+            f.m = a.mr + b.mr;
+            f.n = a.nr + b.nr;
+            const DevNode& dl = M->hn[d.left];
+            const DevNode& dr = M->hn[d.right];
+            f.P12 = take(fac, (int64_t)a.mr * dr.kv);
+            f.P21 = take(fac, (int64_t)b.mr * dl.kv);
+        }
+        if (f.m != f.n)
+            return fail(HSS_ERR_DIM, "ULV needs square diagonal blocks (node " + std::to_string(i) + ": " +
+                                         std::to_string(f.m) + "x" + std::to_string(f.n) + ")");
+        if (i == 0) {
+            f.LU = take(fac, (int64_t)f.m * f.m);
+            f.piv = take(piv, f.m);
+            f.zrow = take(zrow, f.m);
+            f.mr = f.m;
+            f.nr = f.n;
+            continue;
+        }
+        f.k = std::min(d.ku, f.m);
+        f.pass = f.k >= f.m ? 1 : 0;
+        f.nk = f.pass ? 0 : std::min(f.m - f.k, f.n);
+        f.mr = f.pass ? f.m : f.k;
+        f.nr = f.n - f.nk;
+        f.nblk = (f.nk + NB - 1) / NB;
+        int S = pow2_at_least((f.n + FACT_CPT - 1) / FACT_CPT);
+        if (S < 1) S = 1;
+        f.S = S;
+        if (S > 64 || (int64_t)(f.m + d.kv) * S > FACT_THREADS)
+            return fail(HSS_ERR_UNSUPPORTED, "node " + std::to_string(i) + " (" + std::to_string(f.m) + "x" +
+                                                 std::to_string(f.n) + ", kv " + std::to_string(d.kv) +
+                                                 ") exceeds the register-resident factor kernel");
+        if (factor_lds<TF>(f, d) > 160 * 1024)
+            return fail(HSS_ERR_UNSUPPORTED, "node " + std::to_string(i) + " exceeds the LDS of the factor kernel");
+        if (!f.pass) {
+            f.Yql = take(fac, (int64_t)f.m * f.k);
+            f.Tql = take(fac, (int64_t)f.k * f.k);
+            f.Ylq = take(fac, (int64_t)f.n * f.nk);
+            f.taulq = take(fac, (int64_t)f.nk);
+            f.Tlq = take(fac, (int64_t)f.nblk * NB * NB);
+            f.L1 = take(fac, (int64_t)f.nk * f.nk);
+            f.L1inv = take(fac, (int64_t)f.nblk * NB * NB);
+            f.L2a = take(fac, (int64_t)f.k * f.nk);
+            f.Vt = take(fac, (int64_t)d.kv * f.nk);
+        }
+        if (!d.is_leaf) f.rwM = take(rw, (int64_t)(f.m + d.kv) * f.n);
+        f.rwD = take(rw, (int64_t)f.mr * f.nr);
+        f.rwU = take(rw, (int64_t)f.mr * d.ku);
+        f.rwV = take(rw, (int64_t)f.nr * d.kv);
+        f.brow = take(brow, f.mr);
+        f.urow = take(urow, d.kv);
+        f.zrow = take(zrow, f.n);
+    }
+    F->fac_len = fac;
+    F->brows = brow;
+    F->urows = urow;
+    F->zrows = zrow;
+    // per-level LDS requirements of the solve kernels (per RHS tile)
+    size_t sup = 0, sroot = 0, sdown = 0;
+    for (int i = 0; i < nn; ++i) {
+        const FNode& f = F->hfn[i];
+        const size_t mp = (size_t)std::max(f.m, NB);
+        if (i == 0) sroot = 2 * mp * RHS_TILE;
+        else {
+            sup = std::max(sup, 3 * mp * RHS_TILE);
+            sdown = std::max(sdown, ((size_t)f.n + 2 * NB) * RHS_TILE);
+        }
+    }
+    const size_t tv = sizeof(TF);
+    F->lds_sup = sup * tv;
+    F->lds_sroot = sroot * tv;
+    F->lds_sdown = sdown * tv;
+    (void)rw;
+    (void)piv;
+    return HSS_OK;
+}
+
+static void free_fact(hss_fact_s* F) {
+    if (!F) return;
+    (void)hipFree(F->d_fn);
+    (void)hipFree(F->d_fac);
+    (void)hipFree(F->d_piv);
+    (void)hipFree(F->d_status);
+    (void)hipFree(F->d_wb);
+    (void)hipFree(F->d_wu);
+    (void)hipFree(F->d_wz);
+    (void)hipFree(F->d_swork);
+    (void)hipFree(F->d_r);
+    (void)hipFree(F->d_d);
+    (void)hipFree(F->d_nrm);
+    (void)hipFree(F->d_io_b);
+    (void)hipFree(F->d_io_x);
+    delete F;
+}
+
+// Factor a batch of matrices that share one context: one launch per height
+// across all of them, then one root launch.
+template <typename TF>
+static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec prec, std::vector<hss_fact_s*>& out) {
+    const int cnt = (int)mats.size();
+    hss_ctx_s* ctx = mats[0]->ctx;
+    hipStream_t st = ctx->stream;
+    out.assign(cnt, nullptr);
+    std::vector<int64_t> rw_off(cnt), rw_len(cnt);
+    int64_t rw_total = 0;
+    int maxh = 0;
+    auto cleanup = [&]() {
+        for (auto* f : out) free_fact(f);
+        out.assign(cnt, nullptr);
+    };
+    for (int b = 0; b < cnt; ++b) {
+        hss_mat_s* M = mats[b];
+        if (M->ctx != ctx) { cleanup(); return fail(HSS_ERR_ARG, "batched matrices must share one context"); }
+        auto* F = new (std::nothrow) hss_fact_s;
+        if (!F) { cleanup(); return fail(HSS_ERR_OOM, "host allocation"); }
+        out[b] = F;
+        F->mat = M;
+        F->prec = prec;
+        hss_status s = plan_factor<TF>(M, F);
+        if (s != HSS_OK) { cleanup(); return s; }
+        int64_t rw = 0;
+        for (const FNode& f : F->hfn) {
+            if (f.rwD >= 0) rw = std::max(rw, f.rwD + (int64_t)f.mr * f.nr);
+        }
+        for (int i = 1; i < M->nn; ++i) {
+            const FNode& f = F->hfn[i];
+            if (f.rwM >= 0) rw = std::max(rw, f.rwM + (int64_t)(f.m + M->hn[i].kv) * f.n);
+            rw = std::max(rw, f.rwV + (int64_t)f.nr * M->hn[i].kv);
+            rw = std::max(rw, f.rwU + (int64_t)f.mr * M->hn[i].ku);
+        }
+        rw_off[b] = rw_total;
+        rw_len[b] = rw;
+        rw_total += rw;
+        maxh = std::max(maxh, M->max_height);
+        hipError_t e;
+        if ((e = dalloc(&F->d_fn, F->hfn.size())) != hipSuccess ||
+            (e = dalloc(reinterpret_cast<TF**>(&F->d_fac), (size_t)F->fac_len)) != hipSuccess ||
+            (e = dalloc(&F->d_piv, (size_t)F->hfn[0].m)) != hipSuccess || (e = dalloc(&F->d_status, 1)) != hipSuccess) {
+            cleanup();
+            return fail(HSS_ERR_OOM, std::string("factor allocation: ") + hipGetErrorString(e));
+        }
+        HIP_TRY(hipMemcpyAsync(F->d_fn, F->hfn.data(), sizeof(FNode) * F->hfn.size(), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemsetAsync(F->d_status, 0, sizeof(int32_t), st));
+        F->dev_bytes = (int64_t)(sizeof(TF) * F->fac_len + sizeof(FNode) * F->hfn.size() + 4 * (F->hfn[0].m + 1));
+    }
+    TF* d_rw = nullptr;
+    FInst<TF>* d_inst = nullptr;
+    WorkItem* d_work = nullptr;
+    // work lists across the batch
+    std::vector<WorkItem> work;
+    std::vector<Range> lev(maxh);
+    std::vector<size_t> lev_lds(maxh, 0);
+    for (int h = 0; h < maxh; ++h) {
+        lev[h].off = (int)work.size();
+        for (int b = 0; b < cnt; ++b) {
+            hss_mat_s* M = mats[b];
+            for (int i = 1; i < M->nn; ++i)
+                if (M->hn[i].height == h) {
+                    work.push_back(WorkItem{b, i});
+                    lev_lds[h] = std::max(lev_lds[h], factor_lds<TF>(out[b]->hfn[i], M->hn[i]));
+                }
+        }
+        lev[h].count = (int)work.size() - lev[h].off;
+    }
+    const int wy_count = (int)work.size();  // every non-root node of every matrix
+    Range roots{(int)work.size(), cnt};
+    for (int b = 0; b < cnt; ++b) work.push_back(WorkItem{b, 0});
+    std::vector<FInst<TF>> hinst(cnt);
+    hipError_t e;
+    if ((e = dalloc(&d_rw, (size_t)rw_total)) != hipSuccess || (e = dalloc(&d_inst, (size_t)cnt)) != hipSuccess ||
+        (e = dalloc(&d_work, work.size())) != hipSuccess) {
+        (void)hipFree(d_rw); (void)hipFree(d_inst); (void)hipFree(d_work);
+        cleanup();
+        return fail(HSS_ERR_OOM, std::string("factor workspace: ") + hipGetErrorString(e));
+    }
+    for (int b = 0; b < cnt; ++b) {
+        FInst<TF>& I = hinst[b];
+        I.nodes = mats[b]->d_nodes;
+        I.fn = out[b]->d_fn;
+        I.blob = mats[b]->d_blob;
+        I.fac = reinterpret_cast<TF*>(out[b]->d_fac);
+        I.rw = d_rw + rw_off[b];
+        I.ipiv = out[b]->d_piv;
+        I.status = out[b]->d_status;
+    }
+    hss_status rs = HSS_OK;
+    auto run = [&]() -> hss_status {
+        HIP_TRY(hipMemcpyAsync(d_inst, hinst.data(), sizeof(FInst<TF>) * cnt, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_work, work.data(), sizeof(WorkItem) * work.size(), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipEventRecord(ctx->ev0, st));
+        for (int h = 0; h < maxh; ++h)
+            HIP_TRY((launch_ulv_factor<TF>(d_inst, d_work + lev[h].off, lev[h].count, lev_lds[h], st)));
+        HIP_TRY((launch_ulv_wy<TF>(d_inst, d_work, wy_count, st)));
+        HIP_TRY((launch_ulv_root<TF>(d_inst, d_work + roots.off, roots.count, st)));
+        HIP_TRY(hipEventRecord(ctx->ev1, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) g_last_ms = ms;
+        return HSS_OK;
+    };
+    rs = run();
+    (void)hipFree(d_rw);
+    (void)hipFree(d_inst);
+    (void)hipFree(d_work);
+    if (rs != HSS_OK) { cleanup(); return rs; }
+    for (int b = 0; b < cnt; ++b) {
+        int32_t info = 0;
+        HIP_TRY(hipMemcpy(&info, out[b]->d_status, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (info != 0) {
+            cleanup();
+            return fail(HSS_ERR_SINGULAR, "SingularException(" + std::to_string(info) +
+                                              "): exactly zero pivot in the root block of matrix " + std::to_string(b));
+        }
+    }
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_ulv_factor_batch(int32_t count, const hss_mat_t* mats, hss_prec prec, hss_fact_t* out) {
+    if (count <= 0 || !mats || !out) return fail(HSS_ERR_ARG, "bad batch arguments");
+    std::vector<hss_mat_s*> v(mats, mats + count);
+    for (auto* m : v)
+        if (!m) return fail(HSS_ERR_ARG, "null matrix in batch");
+    HIP_TRY(hipSetDevice(v[0]->ctx->device));
+    std::vector<hss_fact_s*> f;
+    hss_status s = (prec == HSS_PREC_FP32) ? factor_batch<float>(v, prec, f) : factor_batch<double>(v, prec, f);
+    if (s != HSS_OK) return s;
+    for (int i = 0; i < count; ++i) out[i] = f[i];
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_ulv_factor(hss_mat_t M, hss_prec prec, hss_fact_t* out) {
+    if (!M || !out) return fail(HSS_ERR_ARG, "null argument");
+    if (prec != HSS_PREC_FP64 && prec != HSS_PREC_FP32) return fail(HSS_ERR_ARG, "unknown precision");
+    return hss_ulv_factor_batch(1, &M, prec, out);
+}
+
+extern "C" hss_status hss_fact_destroy(hss_fact_t F) {
+    if (!F) return HSS_OK;
+    (void)hipSetDevice(F->mat->ctx->device);
+    (void)hipStreamSynchronize(F->mat->ctx->stream);
+    free_fact(F);
+    return HSS_OK;
+}
+
+extern "C" int64_t hss_fact_device_bytes(hss_fact_t F) { return F ? F->dev_bytes : 0; }
+
+// ---------------------------------------------------------------------------
+// ULV solve
+// ---------------------------------------------------------------------------
+template <typename TV>
+static hss_status ensure_solve_ws(hss_fact_s* F, int nrhs) {
+    if ((size_t)nrhs <= F->ws_cap) return HSS_OK;
+    (void)hipFree(F->d_wb);
+    (void)hipFree(F->d_wu);
+    (void)hipFree(F->d_wz);
+    F->d_wb = F->d_wu = F->d_wz = nullptr;
+    F->ws_cap = 0;
+    HIP_TRY(dalloc(reinterpret_cast<TV**>(&F->d_wb), (size_t)std::max<int64_t>(F->brows, 1) * nrhs));
+    HIP_TRY(dalloc(reinterpret_cast<TV**>(&F->d_wu), (size_t)std::max<int64_t>(F->urows, 1) * nrhs));
+    HIP_TRY(dalloc(reinterpret_cast<TV**>(&F->d_wz), (size_t)std::max<int64_t>(F->zrows, 1) * nrhs));
+    F->ws_cap = nrhs;
+    return HSS_OK;
+}
+
+struct SolveReq {
+    hss_fact_s* F;
+    int nrhs;
+    const double* dB;
+    int64_t ldb;
+    double* dX;
+    int64_t ldx;
+};
+
+template <typename TF, typename TV>
+static SInst<TF, TV> make_sinst(const SolveReq& q) {
+    hss_fact_s* F = q.F;
+    hss_mat_s* M = F->mat;
+    SInst<TF, TV> I;
+    std::memset(&I, 0, sizeof(I));
+    I.nodes = M->d_nodes;
+    I.fn = F->d_fn;
+    I.fac = reinterpret_cast<const TF*>(F->d_fac);
+    I.ipiv = F->d_piv;
+    I.blob = M->d_blob;
+    I.B = q.dB;
+    I.X = q.dX;
+    I.ldb = q.ldb;
+    I.ldx = q.ldx;
+    I.wb = reinterpret_cast<TV*>(F->d_wb);
+    I.wu = reinterpret_cast<TV*>(F->d_wu);
+    I.wz = reinterpret_cast<TV*>(F->d_wz);
+    I.nrhs = q.nrhs;
+    return I;
+}
+
+// Level schedule of a set of requests: up-sweep by height, roots, down-sweep by depth.
+struct Schedule {
+    std::vector<WorkItem> work;
+    std::vector<Range> up, down;
+    Range roots;
+};
+
+static void build_schedule(const std::vector<hss_mat_s*>& mats, Schedule& S) {
+    const int cnt = (int)mats.size();
+    int maxh = 0, maxd = 0;
+    for (auto* M : mats) {
+        maxh = std::max(maxh, M->max_height);
+        maxd = std::max(maxd, M->max_depth);
+    }
+    S.work.clear();
+    S.up.assign(maxh, Range{});
+    S.down.assign(maxd + 1, Range{});
+    for (int h = 0; h < maxh; ++h) {
+        S.up[h].off = (int)S.work.size();
+        for (int b = 0; b < cnt; ++b) {
+            hss_mat_s* M = mats[b];
+            if (h < M->max_height) {
+                const Range& r = M->by_height[h];
+                for (int t = 0; t < r.count; ++t) S.work.push_back(WorkItem{b, M->hwork[r.off + t].node});
+            }
+        }
+        S.up[h].count = (int)S.work.size() - S.up[h].off;
+    }
+    S.roots = Range{(int)S.work.size(), cnt};
+    for (int b = 0; b < cnt; ++b) S.work.push_back(WorkItem{b, 0});
+    for (int d = 1; d <= maxd; ++d) {
+        S.down[d].off = (int)S.work.size();
+        for (int b = 0; b < cnt; ++b) {
+            hss_mat_s* M = mats[b];
+            if (d <= M->max_depth) {
+                const Range& r = M->by_depth[d];
+                for (int t = 0; t < r.count; ++t) S.work.push_back(WorkItem{b, M->hwork[r.off + t].node});
+            }
+        }
+        S.down[d].count = (int)S.work.size() - S.down[d].off;
+    }
+}
+
+template <typename TF, typename TV>
+static hss_status launch_schedule(const SInst<TF, TV>* di, const WorkItem* dw, const std::vector<Range>& up,
+                                  const Range& roots, const std::vector<Range>& down, int maxnrhs, size_t lsup,
+                                  size_t lsroot, size_t lsdown, hipStream_t st) {
+    for (size_t h = 0; h < up.size(); ++h)
+        HIP_TRY((launch_ulv_sup<TF, TV>(di, dw + up[h].off, up[h].count, maxnrhs, lsup, st)));
+    HIP_TRY((launch_ulv_sroot<TF, TV>(di, dw + roots.off, roots.count, maxnrhs, lsroot, st)));
+    for (size_t d = 1; d < down.size(); ++d)
+        HIP_TRY((launch_ulv_sdown<TF, TV>(di, dw + down[d].off, down[d].count, maxnrhs, lsdown, st)));
+    return HSS_OK;
+}
+
+static hss_status ensure_schedule(hss_fact_s* F) {
+    if (F->d_swork) return HSS_OK;
+    Schedule S;
+    build_schedule(std::vector<hss_mat_s*>{F->mat}, S);
+    HIP_TRY(dalloc(&F->d_swork, S.work.size()));
+    HIP_TRY(hipMemcpy(F->d_swork, S.work.data(), sizeof(WorkItem) * S.work.size(), hipMemcpyHostToDevice));
+    F->s_up = S.up;
+    F->s_down = S.down;
+    F->s_root = S.roots;
+    return HSS_OK;
+}
+
+template <typename TF, typename TV>
+static hss_status solve_one(hss_fact_s* F, int nrhs, const double* dB, int64_t ldb, double* dX, int64_t ldx,
+                            hipStream_t st) {
+    hss_status s = ensure_solve_ws<TV>(F, nrhs);
+    if (s != HSS_OK) return s;
+    s = ensure_schedule(F);
+    if (s != HSS_OK) return s;
+    SInst<TF, TV> I = make_sinst<TF, TV>(SolveReq{F, nrhs, dB, ldb, dX, ldx});
+    const void* dv = nullptr;
+    HIP_TRY(F->sinst.get(&I, sizeof(I), st, &dv));
+    return launch_schedule<TF, TV>(reinterpret_cast<const SInst<TF, TV>*>(dv), F->d_swork, F->s_up, F->s_root,
+                                   F->s_down, nrhs, F->lds_sup, F->lds_sroot, F->lds_sdown, st);
+}
+
+static hss_status ensure_refine(hss_fact_s* F, int nrhs) {
+    const size_t need = (size_t)std::max<int64_t>(F->mat->rows, 1) * nrhs;
+    if (need <= F->rf_cap) return HSS_OK;
+    (void)hipFree(F->d_r);
+    (void)hipFree(F->d_d);
+    (void)hipFree(F->d_nrm);
+    F->d_r = F->d_d = F->d_nrm = nullptr;
+    F->rf_cap = 0;
+    HIP_TRY(dalloc(&F->d_r, need));
+    HIP_TRY(dalloc(&F->d_d, need));
+    HIP_TRY(dalloc(&F->d_nrm, (size_t)2 * nrhs));
+    F->rf_cap = need;
+    return HSS_OK;
+}
+
+// max_j ||r_j|| / ||b_j||
+static hss_status rel_residual(hss_fact_s* F, int nrhs, const double* dB, int64_t ldb, const double* dR, int64_t ldr,
+                               hipStream_t st, double* out) {
+    const int64_t n = F->mat->rows;
+    HIP_TRY(launch_colnorm2(dB, ldb, n, nrhs, F->d_nrm, st));
+    HIP_TRY(launch_colnorm2(dR, ldr, n, nrhs, F->d_nrm + nrhs, st));
+    std::vector<double> h(2 * (size_t)nrhs);
+    HIP_TRY(hipMemcpyAsync(h.data(), F->d_nrm, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    double worst = 0.0;
+    for (int j = 0; j < nrhs; ++j) {
+        const double nb = std::sqrt(h[j]), nr = std::sqrt(h[nrhs + j]);
+        const double rel = nb > 0 ? nr / nb : nr;
+        worst = std::max(worst, rel);
+    }
+    *out = worst;
+    return HSS_OK;
+}
+
+static hss_status solve_dev_impl(hss_fact_s* F, int nrhs, const double* dB, int64_t ldb, double* dX, int64_t ldx,
+                                 const hss_refine_opts* opts, hss_solve_report* rep) {
+    hss_mat_s* M = F->mat;
+    hipStream_t st = M->ctx->stream;
+    if (nrhs < 0) return fail(HSS_ERR_ARG, "nrhs < 0");
+    if (rep) { rep->iterations = 0; rep->rel_residual = -1.0; rep->seconds = 0.0; }
+    if (nrhs == 0 || M->rows == 0) return HSS_OK;
+    if (ldb < M->rows || ldx < M->rows) return fail(HSS_ERR_DIM, "leading dimension smaller than the matrix size");
+    const bool fp32 = F->prec == HSS_PREC_FP32;
+    hss_refine_opts o;
+    if (opts) o = *opts;
+    else { o.tol = fp32 ? 1e-10 : 0.0; o.max_iter = fp32 ? 10 : 0; o.mode = fp32 ? 1 : 0; }
+    auto solve = [&](const double* b, int64_t lb, double* x, int64_t lx) -> hss_status {
+        return fp32 ? solve_one<float, float>(F, nrhs, b, lb, x, lx, st)
+                    : solve_one<double, double>(F, nrhs, b, lb, x, lx, st);
+    };
+    HIP_TRY(hipEventRecord(M->ctx->ev0, st));
+    hss_status s = solve(dB, ldb, dX, ldx);
+    if (s != HSS_OK) return s;
+    if (o.mode == 1 && o.max_iter >= 0) {
+        s = ensure_refine(F, nrhs);
+        if (s != HSS_OK) return s;
+        const int64_t n = M->rows;
+        double rel = 0.0;
+        int it = 0;
+        for (;;) {
+            // r = b - A x   (Float64 HSS matvec)
+            s = matmul_enqueue(M, nrhs, 1.0, dX, ldx, 0.0, F->d_r, n);
+            if (s != HSS_OK) return s;
+            HIP_TRY(launch_residual(dB, ldb, F->d_r, n, n, nrhs, st));
+            s = rel_residual(F, nrhs, dB, ldb, F->d_r, n, st, &rel);
+            if (s != HSS_OK) return s;
+            if (rel <= o.tol || it >= o.max_iter) break;
+            // d = A \ r with the resident factor, x += d
+            s = solve(F->d_r, n, F->d_d, n);
+            if (s != HSS_OK) return s;
+            HIP_TRY(launch_axpy(dX, ldx, F->d_d, n, n, nrhs, st));
+            ++it;
+        }
+        if (rep) { rep->iterations = it; rep->rel_residual = rel; }
+        HIP_TRY(hipEventRecord(M->ctx->ev1, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, M->ctx->ev0, M->ctx->ev1) == hipSuccess) {
+            g_last_ms = ms;
+            if (rep) rep->seconds = ms * 1e-3;
+        }
+        if (rel > o.tol) {
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "refinement stopped at relative residual %.3e > tol %.3e after %d steps",
+                          rel, o.tol, it);
+            return fail(HSS_ERR_NOCONV, buf);
+        }
+        return HSS_OK;
+    }
+    HIP_TRY(hipEventRecord(M->ctx->ev1, st));
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_ulv_solve_dev(hss_fact_t F, int32_t nrhs, const double* dB, int64_t ldb, double* dX,
+                                        int64_t ldx, const hss_refine_opts* opts, hss_solve_report* rep) {
+    if (!F) return fail(HSS_ERR_ARG, "null factor");
+    HIP_TRY(hipSetDevice(F->mat->ctx->device));
+    return solve_dev_impl(F, nrhs, dB, ldb, dX, ldx, opts, rep);
+}
+
+extern "C" hss_status hss_ulv_solve(hss_fact_t F, int32_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx,
+                                    const hss_refine_opts* opts, hss_solve_report* rep) {
+    if (!F || (nrhs > 0 && (!B || !X))) return fail(HSS_ERR_ARG, "null argument");
+    if (nrhs < 0) return fail(HSS_ERR_ARG, "nrhs < 0");
+    if (nrhs == 0) return HSS_OK;
+    hss_mat_s* M = F->mat;
+    if (ldb < M->rows || ldx < M->rows) return fail(HSS_ERR_DIM, "leading dimension smaller than the matrix size");
+    HIP_TRY(hipSetDevice(M->ctx->device));
+    const int64_t n = std::max<int64_t>(M->rows, 1);
+    hss_status s = ensure_io(&F->d_io_b, &F->d_io_x, &F->io_cap, (size_t)n * nrhs);
+    if (s != HSS_OK) return s;
+    hipStream_t st = M->ctx->stream;
+    HIP_TRY(hipMemcpy2DAsync(F->d_io_b, n * sizeof(double), B, ldb * sizeof(double), M->rows * sizeof(double), nrhs,
+                             hipMemcpyHostToDevice, st));
+    s = solve_dev_impl(F, nrhs, F->d_io_b, n, F->d_io_x, n, opts, rep);
+    if (s != HSS_OK && s != HSS_ERR_NOCONV) return s;
+    HIP_TRY(hipMemcpy2DAsync(X, ldx * sizeof(double), F->d_io_x, n * sizeof(double), M->rows * sizeof(double), nrhs,
+                             hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, M->ctx->ev0, M->ctx->ev1) == hipSuccess) {
+        g_last_ms = ms;
+        if (rep) rep->seconds = ms * 1e-3;
+    }
+    return s;
+}
+
+extern "C" hss_status hss_ulvfactsolve(hss_mat_t M, int32_t nrhs, const double* B, int64_t ldb, double* X,
+                                       int64_t ldx) {
+    if (!M) return fail(HSS_ERR_ARG, "null matrix");
+    if (M->rows != M->cols) return fail(HSS_ERR_DIM, "ulvfactsolve needs a square matrix");
+    hss_fact_t F = nullptr;
+    hss_status s = hss_ulv_factor(M, HSS_PREC_FP64, &F);
+    if (s != HSS_OK) return s;
+    hss_refine_opts none{0.0, 0, 0};
+    s = hss_ulv_solve(F, nrhs, B, ldb, X, ldx, &none, nullptr);
+    hss_fact_destroy(F);
+    return s;
+}
+
+// Batched solves: one launch per level across all requests.  Requests that
+// share a factor must not overlap (they would share its workspace).
+extern "C" hss_status hss_ulv_solve_batch_dev(int32_t count, const hss_fact_t* facts, const int32_t* nrhs,
+                                              const double* const* dB, const int64_t* ldb, double* const* dX,
+                                              const int64_t* ldx) {
+    if (count <= 0 || !facts || !nrhs || !dB || !ldb || !dX || !ldx || !facts[0])
+        return fail(HSS_ERR_ARG, "bad batch arguments");
+    hss_ctx_s* ctx = facts[0]->mat->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const hss_prec prec = facts[0]->prec;
+    std::vector<SolveReq> reqs;
+    for (int i = 0; i < count; ++i) {
+        hss_fact_s* F = facts[i];
+        if (!F || F->mat->ctx != ctx || F->prec != prec)
+            return fail(HSS_ERR_ARG, "batched factors must share one context and one precision");
+        for (int j = 0; j < i; ++j)
+            if (facts[j] == F) return fail(HSS_ERR_ARG, "a factor may appear only once per batch");
+        if (nrhs[i] < 0 || ldb[i] < F->mat->rows || ldx[i] < F->mat->rows)
+            return fail(HSS_ERR_DIM, "bad nrhs / leading dimension in batch request " + std::to_string(i));
+        if (nrhs[i] == 0) continue;
+        reqs.push_back(SolveReq{F, nrhs[i], dB[i], ldb[i], dX[i], ldx[i]});
+    }
+    if (reqs.empty()) return HSS_OK;
+    hipStream_t st = ctx->stream;
+    int maxnrhs = 0;
+    size_t lsup = 0, lsroot = 0, lsdown = 0;
+    std::vector<hss_mat_s*> mats;
+    for (const SolveReq& q : reqs) {
+        hss_status s = (prec == HSS_PREC_FP32) ? ensure_solve_ws<float>(q.F, q.nrhs) : ensure_solve_ws<double>(q.F, q.nrhs);
+        if (s != HSS_OK) return s;
+        mats.push_back(q.F->mat);
+        maxnrhs = std::max(maxnrhs, q.nrhs);
+        lsup = std::max(lsup, q.F->lds_sup);
+        lsroot = std::max(lsroot, q.F->lds_sroot);
+        lsdown = std::max(lsdown, q.F->lds_sdown);
+    }
+    Schedule S;
+    build_schedule(mats, S);
+    // tables of the previous batch may still be read: wait, then upload synchronously
+    static thread_local char* tbl = nullptr;
+    static thread_local size_t tbl_cap = 0;
+    const size_t isz = (prec == HSS_PREC_FP32 ? sizeof(SInst<float, float>) : sizeof(SInst<double, double>)) * reqs.size();
+    const size_t wsz = sizeof(WorkItem) * S.work.size();
+    const size_t need = ((isz + 255) / 256) * 256 + wsz;
+    HIP_TRY(hipStreamSynchronize(st));
+    if (need > tbl_cap) {
+        (void)hipFree(tbl);
+        tbl = nullptr;
+        tbl_cap = 0;
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&tbl), need));
+        tbl_cap = need;
+    }
+    WorkItem* dw = reinterpret_cast<WorkItem*>(tbl + ((isz + 255) / 256) * 256);
+    HIP_TRY(hipMemcpy(dw, S.work.data(), wsz, hipMemcpyHostToDevice));
+    HIP_TRY(hipEventRecord(ctx->ev0, st));
+    if (prec == HSS_PREC_FP32) {
+        std::vector<SInst<float, float>> hi;
+        for (const SolveReq& q : reqs) hi.push_back(make_sinst<float, float>(q));
+        HIP_TRY(hipMemcpy(tbl, hi.data(), isz, hipMemcpyHostToDevice));
+        hss_status s = launch_schedule<float, float>(reinterpret_cast<const SInst<float, float>*>(tbl), dw, S.up,
+                                                     S.roots, S.down, maxnrhs, lsup, lsroot, lsdown, st);
+        if (s != HSS_OK) return s;
+    } else {
+        std::vector<SInst<double, double>> hi;
+        for (const SolveReq& q : reqs) hi.push_back(make_sinst<double, double>(q));
+        HIP_TRY(hipMemcpy(tbl, hi.data(), isz, hipMemcpyHostToDevice));
+        hss_status s = launch_schedule<double, double>(reinterpret_cast<const SInst<double, double>*>(tbl), dw, S.up,
+                                                       S.roots, S.down, maxnrhs, lsup, lsroot, lsdown, st);
+        if (s != HSS_OK) return s;
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, st));
+    return HSS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics
+// ---------------------------------------------------------------------------
+extern "C" const char* hss_last_error(void) { return g_err.c_str(); }
+extern "C" double hss_last_device_ms(void) { return g_last_ms; }

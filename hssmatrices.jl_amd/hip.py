@@ -1,0 +1,370 @@
+"""ctypes binding of libhss_hip.so and the device-side HSS objects.
+
+This is the Python mirror of the reference's operator interface for the hot
+path (the Julia mirror is ``julia/HssHip.jl``):
+
+=====================================  =============================================
+reference (bonevbs/HssMatrices.jl)     here
+=====================================  =============================================
+``hssA * x`` / ``*(hssA, B)``          ``hssA @ x``  (``HipHssMatrix.__matmul__``)
+  ``src/matmul.jl:13,15``
+``mul!(C, hssA, B, a, b)``             ``mul(C, hssA, B, a, b)``
+  ``src/matmul.jl:18``
+``hssA \\ B`` (``hssmatrix.jl:234``)   ``ldiv(hssA, B)`` / ``hssA.solve(B)``
+``ulvfactsolve(hssA, b)``              ``ulvfactsolve(hssA, b)``
+  ``src/ulvfactor.jl:10``
+(no counterpart: refactors each call)  ``hssA.ulvfactor(prec)`` -> resident ``HipUlvFactor``
+=====================================  =============================================
+
+Errors follow the reference: ``DimensionMismatch`` (``matmul.jl:19-20``),
+``ValueError`` for ``ArgumentError``, ``SingularException`` for a zero pivot of
+the dense root solve (``ulvfactor.jl:83``).
+
+The shared library must have been built (``__graft_entry__.build()`` or
+``make -C hssmatrices.jl_amd/csrc``); importing this module without it raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+
+import numpy as np
+
+from .hssmatrix import NODE_DTYPE, DimensionMismatch, HssMatrix, SingularException, pack
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "libhss_hip.so"
+
+HSS_OK, HSS_ERR_DIM, HSS_ERR_ARG, HSS_ERR_SINGULAR, HSS_ERR_NOCONV = 0, 1, 2, 3, 4
+HSS_ERR_OOM, HSS_ERR_BUDGET, HSS_ERR_HIP, HSS_ERR_UNSUPPORTED = 5, 6, 7, 8
+PREC_FP64, PREC_FP32 = 0, 1
+
+# every symbol include/hss_hip.h declares
+EXPORTS = [
+    "hss_ctx_create", "hss_ctx_destroy", "hss_ctx_stream",
+    "hss_mat_create", "hss_mat_destroy", "hss_mat_size", "hss_mat_device_bytes",
+    "hss_matmul", "hss_matmul_dev",
+    "hss_ulvfactsolve", "hss_ulv_factor", "hss_fact_destroy", "hss_fact_device_bytes",
+    "hss_ulv_solve", "hss_ulv_solve_dev", "hss_ulv_solve_batch_dev", "hss_ulv_factor_batch",
+    "hss_last_error", "hss_last_device_ms",
+]
+
+
+class HssError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"[hss status {status}] {msg}")
+        self.status = status
+
+
+class NotConverged(HssError):
+    pass
+
+
+class RefineOpts(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int), ("mode", ctypes.c_int)]
+
+
+class SolveReport(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("rel_residual", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load_library(path: pathlib.Path | str | None = None):
+    """Load libhss_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = pathlib.Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise ImportError(f"{p} is missing: build it with __graft_entry__.build() "
+                          f"or `make -C hssmatrices.jl_amd/csrc`")
+    lib = ctypes.CDLL(str(p))
+    V, I32, I64, D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    PV = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "hss_ctx_create": ([ctypes.c_int, PV], ctypes.c_int),
+        "hss_ctx_destroy": ([V], ctypes.c_int),
+        "hss_ctx_stream": ([V], V),
+        "hss_mat_create": ([V, V, I32, V, I64, PV], ctypes.c_int),
+        "hss_mat_destroy": ([V], ctypes.c_int),
+        "hss_mat_size": ([V, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
+        "hss_mat_device_bytes": ([V], I64),
+        "hss_matmul": ([V, I32, D, V, I64, D, V, I64], ctypes.c_int),
+        "hss_matmul_dev": ([V, I32, D, V, I64, D, V, I64], ctypes.c_int),
+        "hss_ulvfactsolve": ([V, I32, V, I64, V, I64], ctypes.c_int),
+        "hss_ulv_factor": ([V, ctypes.c_int, PV], ctypes.c_int),
+        "hss_fact_destroy": ([V], ctypes.c_int),
+        "hss_fact_device_bytes": ([V], I64),
+        "hss_ulv_solve": ([V, I32, V, I64, V, I64, ctypes.POINTER(RefineOpts), ctypes.POINTER(SolveReport)],
+                          ctypes.c_int),
+        "hss_ulv_solve_dev": ([V, I32, V, I64, V, I64, ctypes.POINTER(RefineOpts), ctypes.POINTER(SolveReport)],
+                              ctypes.c_int),
+        "hss_ulv_solve_batch_dev": ([I32, V, V, V, V, V, V], ctypes.c_int),
+        "hss_ulv_factor_batch": ([I32, V, ctypes.c_int, V], ctypes.c_int),
+        "hss_last_error": ([], ctypes.c_char_p),
+        "hss_last_device_ms": ([], D),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _check(status: int):
+    if status == HSS_OK:
+        return
+    msg = _lib.hss_last_error().decode(errors="replace")
+    if status == HSS_ERR_DIM:
+        raise DimensionMismatch(msg)
+    if status == HSS_ERR_ARG:
+        raise ValueError(msg)
+    if status == HSS_ERR_SINGULAR:
+        raise SingularException(msg)
+    if status == HSS_ERR_NOCONV:
+        raise NotConverged(status, msg)
+    raise HssError(status, msg)
+
+
+def last_device_ms() -> float:
+    return float(load_library().hss_last_device_ms())
+
+
+class Context:
+    """One device and its HIP stream (``hss_ctx_create``)."""
+
+    def __init__(self, device: int = 0):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib.hss_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        return int(_lib.hss_ctx_stream(self._h) or 0)
+
+    def close(self):
+        if self._h:
+            _lib.hss_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _as_f64_fortran(B):
+    B = np.asarray(B, dtype=np.float64)
+    vec = B.ndim == 1
+    if vec:
+        B = B.reshape(-1, 1)
+    return np.asfortranarray(B), vec
+
+
+class HipHssMatrix:
+    """Device-resident HSS matrix (``hss_mat_create``); the drop-in for ``HssMatrix{Float64}``."""
+
+    def __init__(self, hss, ctx: Context | None = None):
+        lib = load_library()
+        self.ctx = ctx or default_context()
+        if isinstance(hss, HssMatrix):
+            nodes, blob = pack(hss)
+        else:
+            nodes, blob = hss
+        self.nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+        blob = np.ascontiguousarray(blob, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _check(lib.hss_mat_create(self.ctx.handle, self.nodes.ctypes.data, len(self.nodes), blob.ctypes.data,
+                                  blob.size, ctypes.byref(h)))
+        self._h = h
+        self.shape = (int(self.nodes["m"][0]), int(self.nodes["n"][0]))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_bytes(self) -> int:
+        return int(_lib.hss_mat_device_bytes(self._h))
+
+    def size(self, dim=None):
+        return self.shape if dim is None else self.shape[dim - 1]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.hss_mat_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- matvec ----
+    def __matmul__(self, B):
+        """``hssA * B`` (reference ``matmul.jl:13,15``)."""
+        Bf, vec = _as_f64_fortran(B)
+        if Bf.shape[0] != self.shape[1]:
+            raise DimensionMismatch(
+                f"First dimension of B does not match second dimension of A. Expected {self.shape[1]}, "
+                f"got {Bf.shape[0]}")
+        C = np.zeros((self.shape[0], Bf.shape[1]), order="F")
+        mul(C, self, Bf, 1.0, 0.0)
+        return C[:, 0] if vec else C
+
+    def matmul_dev(self, nrhs: int, alpha: float, dB: int, ldb: int, beta: float, dC: int, ldc: int):
+        """``hss_matmul_dev`` on device pointers (e.g. ``torch.Tensor.data_ptr()``)."""
+        _check(_lib.hss_matmul_dev(self._h, nrhs, alpha, dB, ldb, beta, dC, ldc))
+
+    # ---- ULV ----
+    def ulvfactor(self, prec: str = "fp64") -> "HipUlvFactor":
+        return HipUlvFactor(self, prec)
+
+    def solve(self, B):
+        """``hssA \\ B`` (reference ``hssmatrix.jl:234`` -> ``ulvfactsolve``)."""
+        return ulvfactsolve(self, B)
+
+
+def mul(C, hssA: HipHssMatrix, B, alpha: float = 1.0, beta: float = 0.0):
+    """``mul!(C, hssA, B, alpha, beta)`` (reference ``matmul.jl:18-28``); C is updated in place."""
+    if not isinstance(C, np.ndarray) or C.dtype != np.float64 or not C.flags.f_contiguous:
+        raise ValueError("C must be a Fortran-ordered float64 array")
+    Bf, _ = _as_f64_fortran(B)
+    C2 = C.reshape(C.shape[0], -1, order="F")
+    if Bf.shape[0] != hssA.shape[1]:
+        raise DimensionMismatch(
+            f"First dimension of B does not match second dimension of A. Expected {hssA.shape[1]}, got {Bf.shape[0]}")
+    if C2.shape != (hssA.shape[0], Bf.shape[1]):
+        raise DimensionMismatch("Dimensions of C don't match up with A and B.")
+    nrhs = Bf.shape[1]
+    _check(_lib.hss_matmul(hssA.handle, nrhs, float(alpha), Bf.ctypes.data, max(Bf.shape[0], 1), float(beta),
+                           C2.ctypes.data, max(C2.shape[0], 1)))
+    return C
+
+
+def ulvfactsolve(hssA: HipHssMatrix, b):
+    """``ulvfactsolve(hssA, b)`` (reference ``ulvfactor.jl:10-19``): factor + solve in one call."""
+    bf, vec = _as_f64_fortran(b)
+    n = hssA.shape[1]
+    if hssA.shape[0] != hssA.shape[1] or bf.shape[0] != hssA.shape[0]:
+        raise DimensionMismatch(f"matrix is {hssA.shape}, right-hand side has {bf.shape[0]} rows")
+    X = np.zeros((n, bf.shape[1]), order="F")
+    _check(_lib.hss_ulvfactsolve(hssA.handle, bf.shape[1], bf.ctypes.data, max(n, 1), X.ctypes.data, max(n, 1)))
+    return X[:, 0] if vec else X
+
+
+ldiv = ulvfactsolve
+
+
+class HipUlvFactor:
+    """Resident ULV factorization (``hss_ulv_factor``)."""
+
+    def __init__(self, hssA: HipHssMatrix, prec: str = "fp64"):
+        self.mat = hssA
+        self.prec = prec
+        p = PREC_FP32 if prec in ("fp32", "mixed", PREC_FP32) else PREC_FP64
+        h = ctypes.c_void_p()
+        _check(_lib.hss_ulv_factor(hssA.handle, p, ctypes.byref(h)))
+        self._h = h
+        self.factor_ms = last_device_ms()
+
+    @classmethod
+    def _wrap(cls, hssA, prec, handle):
+        obj = cls.__new__(cls)
+        obj.mat = hssA
+        obj.prec = prec
+        obj._h = handle
+        obj.factor_ms = last_device_ms()
+        return obj
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_bytes(self) -> int:
+        return int(_lib.hss_fact_device_bytes(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.hss_fact_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, b, tol: float | None = None, max_iter: int | None = None, report: bool = False):
+        """X = A \\ b with the resident factor; FP32 factors refine to ``tol`` (default 1e-10)."""
+        bf, vec = _as_f64_fortran(b)
+        n = self.mat.shape[0]
+        if bf.shape[0] != n:
+            raise DimensionMismatch(f"matrix is {self.mat.shape}, right-hand side has {bf.shape[0]} rows")
+        X = np.zeros((n, bf.shape[1]), order="F")
+        opts = None
+        if tol is not None or max_iter is not None:
+            opts = RefineOpts(tol if tol is not None else 0.0, max_iter if max_iter is not None else 10,
+                              1 if (tol is not None and tol > 0) else 0)
+        rep = SolveReport()
+        _check(_lib.hss_ulv_solve(self._h, bf.shape[1], bf.ctypes.data, max(n, 1), X.ctypes.data, max(n, 1),
+                                  ctypes.byref(opts) if opts is not None else None, ctypes.byref(rep)))
+        out = X[:, 0] if vec else X
+        if report:
+            return out, {"iterations": rep.iterations, "rel_residual": rep.rel_residual, "seconds": rep.seconds}
+        return out
+
+    def solve_dev(self, nrhs: int, dB: int, ldb: int, dX: int, ldx: int, tol: float | None = None,
+                  max_iter: int | None = None):
+        opts = None
+        if tol is not None or max_iter is not None:
+            opts = RefineOpts(tol if tol is not None else 0.0, max_iter if max_iter is not None else 10,
+                              1 if (tol is not None and tol > 0) else 0)
+        rep = SolveReport()
+        _check(_lib.hss_ulv_solve_dev(self._h, nrhs, dB, ldb, dX, ldx,
+                                      ctypes.byref(opts) if opts is not None else None, ctypes.byref(rep)))
+        return {"iterations": rep.iterations, "rel_residual": rep.rel_residual, "seconds": rep.seconds}
+
+
+def ulvfactor_batch(mats, prec: str = "fp64"):
+    """Factor many matrices with one launch per tree level (``hss_ulv_factor_batch``)."""
+    lib = load_library()
+    p = PREC_FP32 if prec in ("fp32", "mixed") else PREC_FP64
+    hs = (ctypes.c_void_p * len(mats))(*[m.handle.value for m in mats])
+    out = (ctypes.c_void_p * len(mats))()
+    _check(lib.hss_ulv_factor_batch(len(mats), ctypes.cast(hs, ctypes.c_void_p), p, ctypes.cast(out, ctypes.c_void_p)))
+    return [HipUlvFactor._wrap(m, prec, ctypes.c_void_p(out[i])) for i, m in enumerate(mats)]
+
+
+def solve_batch_dev(facts, nrhs, dB, ldb, dX, ldx):
+    """Batched solves on device pointers, one launch per tree level (``hss_ulv_solve_batch_dev``)."""
+    lib = load_library()
+    cnt = len(facts)
+    hs = (ctypes.c_void_p * cnt)(*[f.handle.value for f in facts])
+    nr = (ctypes.c_int32 * cnt)(*nrhs)
+    b = (ctypes.c_void_p * cnt)(*dB)
+    lb = (ctypes.c_int64 * cnt)(*ldb)
+    x = (ctypes.c_void_p * cnt)(*dX)
+    lx = (ctypes.c_int64 * cnt)(*ldx)
+    _check(lib.hss_ulv_solve_batch_dev(cnt, ctypes.cast(hs, ctypes.c_void_p), ctypes.cast(nr, ctypes.c_void_p),
+                                       ctypes.cast(b, ctypes.c_void_p), ctypes.cast(lb, ctypes.c_void_p),
+                                       ctypes.cast(x, ctypes.c_void_p), ctypes.cast(lx, ctypes.c_void_p)))

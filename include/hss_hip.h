@@ -1,0 +1,156 @@
+/*
+ * hss_hip.h -- C ABI of libhss_hip.so: HSS matvec and ULV factor/solve on
+ * AMD Instinct MI355X (gfx950).
+ *
+ * Drop-in boundary for bonevbs/HssMatrices.jl 0.1.6.  The reference has no
+ * FFI: its boundary is Julia dispatch on HssMatrix{T}.  Each entry point below
+ * replaces one reference method; the Julia binding a maintainer would add is
+ * in INTEGRATION.md (and hssmatrices.jl_amd/julia/HssHip.jl).
+ *
+ *   hss_matmul / hss_matmul_dev   mul!(C, hssA, B, alpha, beta)  src/matmul.jl:18
+ *                                 *(hssA, B) / *(hssA, x)        src/matmul.jl:13,15
+ *   hss_ulvfactsolve              ulvfactsolve(hssA, b)          src/ulvfactor.jl:10
+ *                                 \(hssA, B::Matrix)             src/hssmatrix.jl:234
+ *   hss_ulv_factor + hss_ulv_solve   the same, split into a factorization that
+ *                                 stays resident in HBM and a solve that reuses
+ *                                 it (the reference refactors on every call;
+ *                                 src/ulvdivide.jl:29-33 is its only split form)
+ *
+ * Conventions (mirroring the reference):
+ *   - Float64, column-major, leading dimensions in elements.
+ *   - Every call returns an hss_status; 0 = success.  HSS_ERR_DIM is the
+ *     reference's DimensionMismatch (src/matmul.jl:19-20, src/ulvfactor.jl:25-26),
+ *     HSS_ERR_ARG its ArgumentError (src/hssmatrix.jl:41-42), HSS_ERR_SINGULAR
+ *     the SingularException of the dense root solve (src/ulvfactor.jl:83).
+ *     hss_last_error() returns a thread-local message for the last failure.
+ *   - Inputs are never modified; outputs are caller-allocated.  Host buffers
+ *     are only borrowed for the duration of a call.  hss_mat_create copies the
+ *     generators into library-owned HBM.
+ *   - Handles are externally synchronized: one caller at a time per handle.
+ */
+#ifndef HSS_HIP_H
+#define HSS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hss_status {
+    HSS_OK = 0,
+    HSS_ERR_DIM = 1,      /* DimensionMismatch */
+    HSS_ERR_ARG = 2,      /* ArgumentError / malformed tree */
+    HSS_ERR_SINGULAR = 3, /* SingularException (zero pivot in the dense root solve) */
+    HSS_ERR_NOCONV = 4,   /* iterative refinement did not reach the requested residual */
+    HSS_ERR_OOM = 5,      /* device allocation failed */
+    HSS_ERR_BUDGET = 6,   /* admission refused: over the memory budget */
+    HSS_ERR_HIP = 7,      /* HIP runtime error */
+    HSS_ERR_UNSUPPORTED = 8
+} hss_status;
+
+/*
+ * One record per HSS tree node, in breadth-first order (root = 0, then
+ * depth 1 left to right, ...).  Generators live in one Float64 blob,
+ * column-major, addressed by element offsets (-1 = absent).
+ *
+ * Mirrors HssMatrix{T} (src/hssmatrix.jl:11-68):
+ *   leaf:   D (m x n) at off_D, U (m x ku) at off_U, V (n x kv) at off_V;
+ *   branch: B12 (ku(left) x kv(right)) at off_B12, B21 (ku(right) x kv(left)) at off_B21;
+ *   every non-root node: its translator into the parent, R (ku x ku(parent))
+ *   at off_R and W (kv x kv(parent)) at off_W -- i.e. the reference's R1/W1 are
+ *   stored with the left child and R2/W2 with the right child.
+ * ku/kv are gensize(node) (src/hssmatrix.jl:254-262); 0 at the root, whose
+ * R/W have no columns (src/hssmatrix.jl:46-55).
+ */
+typedef struct hss_node_desc {
+    int32_t parent, left, right; /* node indices, -1 = none */
+    int32_t depth, height;       /* root depth 0, leaf height 0 */
+    int32_t m, n;                /* block rows / columns */
+    int32_t ku, kv;              /* column / row basis ranks */
+    int32_t pad;
+    int64_t row_off, col_off;    /* first global row / column of the block */
+    int64_t off_D, off_U, off_V;
+    int64_t off_B12, off_B21;
+    int64_t off_R, off_W;
+} hss_node_desc; /* 112 bytes */
+
+typedef struct hss_ctx_s* hss_ctx_t;
+typedef struct hss_mat_s* hss_mat_t;
+typedef struct hss_fact_s* hss_fact_t;
+
+typedef enum hss_prec {
+    HSS_PREC_FP64 = 0, /* factor and solve in Float64 */
+    HSS_PREC_FP32 = 1  /* factor and inner solve in Float32; Float64 refinement */
+} hss_prec;
+
+typedef struct hss_refine_opts {
+    double tol;   /* target ||b - A x|| / ||b|| per column (max over columns); <= 0: no refinement */
+    int max_iter; /* refinement steps after the first solve */
+    int mode;     /* 0 = none, 1 = iterative refinement with the HSS matvec residual */
+} hss_refine_opts;
+
+typedef struct hss_solve_report {
+    int iterations;     /* refinement steps taken */
+    double rel_residual; /* final max_j ||b_j - A x_j|| / ||b_j|| (computed when refinement is on) */
+    double seconds;      /* device time of the call */
+} hss_solve_report;
+
+/* ---- context: owns the HIP stream used by every call on its handles ---- */
+hss_status hss_ctx_create(int device, hss_ctx_t* out);
+hss_status hss_ctx_destroy(hss_ctx_t ctx);
+/* hipStream_t of the context, for callers that enqueue around hss_*_dev calls */
+void* hss_ctx_stream(hss_ctx_t ctx);
+
+/* ---- matrix: copies the generators into HBM (level-major) ----
+ * Replaces the HssMatrix{Float64} object itself (src/hssmatrix.jl:11-68). */
+hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, int32_t num_nodes, const double* blob,
+                          int64_t blob_len, hss_mat_t* out);
+hss_status hss_mat_destroy(hss_mat_t mat);
+/* size(hssA) (src/hssmatrix.jl:94) */
+hss_status hss_mat_size(hss_mat_t mat, int64_t* rows, int64_t* cols);
+/* bytes of HBM held by the matrix (generators + tables) */
+int64_t hss_mat_device_bytes(hss_mat_t mat);
+
+/* ---- matvec: C = alpha*A*B + beta*C,  mul!(C, hssA, B, alpha, beta), src/matmul.jl:18-62 ----
+ * Host pointers; B is cols(A) x nrhs (ldb), C is rows(A) x nrhs (ldc). */
+hss_status hss_matmul(hss_mat_t mat, int32_t nrhs, double alpha, const double* B, int64_t ldb, double beta,
+                      double* C, int64_t ldc);
+/* Same with device pointers, enqueued on the context stream; returns without waiting. */
+hss_status hss_matmul_dev(hss_mat_t mat, int32_t nrhs, double alpha, const double* dB, int64_t ldb, double beta,
+                          double* dC, int64_t ldc);
+
+/* ---- ULV: ulvfactsolve(hssA, b), src/ulvfactor.jl:10-107; \(hssA, B) src/hssmatrix.jl:234 ----
+ * Factor + solve + release in one call, as the reference does on every "\". */
+hss_status hss_ulvfactsolve(hss_mat_t mat, int32_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx);
+
+/* Resident factorization (HBM) and solves that reuse it. */
+hss_status hss_ulv_factor(hss_mat_t mat, hss_prec prec, hss_fact_t* out);
+hss_status hss_fact_destroy(hss_fact_t fact);
+int64_t hss_fact_device_bytes(hss_fact_t fact);
+/* X = A \ B with host pointers.  opts may be NULL (no refinement for FP64, refinement to 1e-10 for FP32);
+ * report may be NULL. */
+hss_status hss_ulv_solve(hss_fact_t fact, int32_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx,
+                         const hss_refine_opts* opts, hss_solve_report* report);
+/* Same with device pointers, on the context stream.  Synchronizes only when refinement is on. */
+hss_status hss_ulv_solve_dev(hss_fact_t fact, int32_t nrhs, const double* dB, int64_t ldb, double* dX, int64_t ldx,
+                             const hss_refine_opts* opts, hss_solve_report* report);
+
+/* ---- batched solves over many (uneven) matrices: one launch per tree level ----
+ * Each request i solves fact[i] with nrhs[i] columns, device pointers. */
+hss_status hss_ulv_solve_batch_dev(int32_t count, const hss_fact_t* facts, const int32_t* nrhs,
+                                   const double* const* dB, const int64_t* ldb, double* const* dX,
+                                   const int64_t* ldx);
+/* Batched factorization of many matrices (one launch per level across all of them). */
+hss_status hss_ulv_factor_batch(int32_t count, const hss_mat_t* mats, hss_prec prec, hss_fact_t* out);
+
+/* ---- diagnostics ---- */
+const char* hss_last_error(void);
+/* Per-call device timing of the last matvec/solve on this thread (milliseconds, -1 if none). */
+double hss_last_device_ms(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HSS_HIP_H */

@@ -1,0 +1,96 @@
+"""Host-side data model: cluster trees, packing, dense expansion, compression."""
+import numpy as np
+import pytest
+
+from conftest import H
+
+
+def test_bisection_rule_c1():
+    """clustertree.jl:29-32: split after ceil(len/2) while len > leafsize."""
+    t = H.bisection_cluster(2001, 64)
+    leaves = []
+
+    def walk(c, d):
+        if c.isleaf():
+            leaves.append((c.size, d))
+        else:
+            assert c.left.size == (c.size + 1) // 2
+            walk(c.left, d + 1)
+            walk(c.right, d + 1)
+
+    walk(t, 0)
+    assert len(leaves) == 32
+    assert {s for s, _ in leaves} == {62, 63}
+    assert {d for _, d in leaves} == {5}
+    assert t.depth() == 6 and t.nleaves() == 32
+
+
+def test_bisection_uneven_depths():
+    t = H.bisection_cluster(129, 64)  # 65 -> 33,32 ; 64 stays a leaf
+    assert t.left.size == 65 and not t.left.isleaf()
+    assert t.right.size == 64 and t.right.isleaf()
+
+
+def test_pack_unpack_roundtrip():
+    rng = np.random.default_rng(0)
+    x = np.sort(rng.uniform(-1, 1, 300))
+    K = H.KernelMatrix(x, "cauchy", 1.0)
+    cl = H.bisection_cluster(300, 20)
+    h = H.compress(K, cl, cl, 1e-9, 1e-9)
+    nodes, blob = H.pack(h)
+    assert nodes.dtype.itemsize == 112
+    assert nodes["parent"][0] == -1
+    # breadth-first: depth non-decreasing, parents before children
+    assert np.all(np.diff(nodes["depth"]) >= 0)
+    assert np.all(nodes["parent"][1:] < np.arange(1, len(nodes)))
+    h2 = H.unpack(nodes, blob)
+    assert np.array_equal(H.full(h2), H.full(h))
+
+
+def test_compression_accuracy_reference_test():
+    """test/runtests.jl:36-38: ||A - full(hssA)|| / ||A|| <= 50 rtol (C1 kernel of runtests.jl:19)."""
+    x = np.arange(-1, 1.0005, 0.001)[:2001]
+    K = H.KernelMatrix(x, "cauchy_test")
+    A = K.dense()
+    cl = H.bisection_cluster(2001, 64)
+    h = H.compress(A, cl, cl, 1e-6, 1e-6)
+    assert np.linalg.norm(A - H.full(h)) / np.linalg.norm(A) <= 50 * 1e-6
+
+
+def test_lazy_kernel_equals_dense_compression():
+    rng = np.random.default_rng(1)
+    x = np.sort(rng.uniform(-1, 1, 256))
+    K = H.KernelMatrix(x, "log", 1.0)
+    cl = H.bisection_cluster(256, 32)
+    a = H.full(H.compress(K, cl, cl, 1e-9, 1e-9))
+    b = H.full(H.compress(K.dense(), cl, cl, 1e-9, 1e-9))
+    assert np.allclose(a, b, rtol=0, atol=1e-12)
+
+
+def test_full_matches_hssleaf():
+    """_full! (hssmatrix.jl:284-305) and _hssleaf (:338-346) expand to the same matrix."""
+    rng = np.random.default_rng(2)
+    x = np.sort(rng.uniform(-1, 1, 100))
+    K = H.KernelMatrix(x, "cauchy", 1.0)
+    cl = H.bisection_cluster(100, 10)
+    h = H.compress(K, cl, cl, 1e-10, 1e-10)
+    from hssmatrices_jl_amd.hssmatrix import _hssleaf
+    D, _, _ = _hssleaf(h)
+    assert np.allclose(D, H.full(h), rtol=0, atol=1e-13)
+
+
+def test_prune_leaves_preserves_matrix():
+    rng = np.random.default_rng(3)
+    x = np.sort(rng.uniform(-1, 1, 64))
+    K = H.KernelMatrix(x, "log", 1.0)
+    cl = H.bisection_cluster(64, 8)
+    h = H.compress(K, cl, cl, 1e-10, 1e-10)
+    F = H.full(h)
+    h.A11 = H.prune_leaves(h.A11)
+    h.sz1 = h.A11.shape
+    assert np.allclose(H.full(h), F, rtol=0, atol=1e-13)
+
+
+def test_mismatched_generators_raise():
+    with pytest.raises(ValueError):
+        H.HssMatrix(D=np.zeros((4, 4)), U=np.zeros((3, 1)), V=np.zeros((4, 1)))
