@@ -1,0 +1,375 @@
+#!/usr/bin/env python3
+"""Benchmark of the HSS ULV solve (and matvec) on one MI355X.
+
+Workload (the metric's configuration): n = 65536 points sorted uniform on
+[-1, 1] (numpy default_rng(0)), kernel log|x_i - x_j| with unit diagonal,
+bisection leaf size 128, direct compression at rtol = atol = 1e-9
+(512 leaves, depth 9), 64 right-hand sides N(0,1) (default_rng(1)).
+
+A "step" is one ULV solve of the 64-column block with a factorization resident
+in HBM (inputs and outputs already on the device):
+  * fp64  : Float64 factor + Float64 solve (backward stable, no refinement);
+  * mixed : Float32 factor + Float32 solve, Float64 HSS-matvec residual and
+            iterative refinement until max_j ||b_j - A x_j|| / ||b_j|| <= 1e-10.
+The headline ``value`` is the mixed-precision path (solves/s at <= 1e-10
+relative residual); the fp64 path, its ratio, p50/p99 latencies, the factor
+time, the matvec bandwidth, parity against the CPU oracle and the CPU
+baseline are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+Prints ONE JSON line on stdout (progress goes to stderr).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, GB/s
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector = matrix)
+FP32_MFMA_PEAK_TFLOPS = 157.3
+DATA = ROOT / "data" / "log_n65536_leaf128_tol1e-9.npz"
+N_POINTS, LEAF, TOL, NRHS = 65536, 128, 1e-9, 64
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_generators(pkg):
+    if DATA.exists():
+        d = np.load(DATA)
+        return d["nodes"], d["blob"]
+    log(f"[bench] {DATA.name} missing: compressing the n={N_POINTS} log kernel on the host (untimed)")
+    t = time.time()
+    x = np.sort(np.random.default_rng(0).uniform(-1, 1, N_POINTS))
+    K = pkg.KernelMatrix(x, "log", 1.0)
+    rcl = pkg.bisection_cluster(N_POINTS, LEAF)
+    h = pkg.compress(K, rcl, rcl, TOL, TOL)
+    nodes, blob = pkg.pack(h)
+    DATA.parent.mkdir(exist_ok=True)
+    np.savez(DATA, nodes=nodes, blob=blob, x=x)
+    log(f"[bench] compression took {time.time() - t:.1f} s")
+    return nodes, blob
+
+
+# ---------------------------------------------------------------------------
+# algorithmic byte / flop models (per launch), see DESIGN.md
+# ---------------------------------------------------------------------------
+def matvec_leaf_bytes(nodes, nrhs, beta_nonzero=False):
+    """k_mv_leaf: D, U, B_seg, F read once, C written once (8 B each)."""
+    b = 0
+    for r in nodes[nodes["left"] < 0]:
+        m, n, ku = int(r["m"]), int(r["n"]), int(r["ku"])
+        b += m * n + m * ku + (n + ku + m * (2 if beta_nonzero else 1)) * nrhs
+    return 8 * b
+
+
+def matvec_bytes(nodes, nrhs):
+    """Whole matvec: every generator once + B in + C out (algorithmic)."""
+    G = 0
+    for r in nodes:
+        if r["left"] < 0:
+            G += int(r["m"]) * int(r["n"]) + int(r["m"]) * int(r["ku"]) + int(r["n"]) * int(r["kv"])
+        else:
+            L, R = nodes[r["left"]], nodes[r["right"]]
+            G += int(L["ku"]) * int(R["kv"]) + int(R["ku"]) * int(L["kv"])
+        if r["parent"] >= 0:
+            p = nodes[r["parent"]]
+            G += int(r["ku"]) * int(p["ku"]) + int(r["kv"]) * int(p["kv"])
+    n, m = int(nodes["n"][0]), int(nodes["m"][0])
+    return 8 * G + 8 * nrhs * (n + m), 2 * nrhs * G
+
+
+def solve_kernel_model(nodes, fact, kernel, level, nrhs, wf, wv):
+    """Algorithmic bytes and flops of one launch of a solve kernel at one level.
+
+    k_ulv_sup (height ``level``): per node reads Y_ql (m k), T_ql (k^2), L1 (nk^2),
+      L1^-1 blocks, L2a (k nk), Vt (kv nk), P12/P21, the node's b (m x nrhs) and
+      children's u; writes zloc (nk), reduced b (k), u (kv) per column.
+    k_ulv_sdown (depth ``level``): reads Y_lq (n nk), T blocks, zloc and the
+      parent's z slice, writes z (n) per column.
+    Factor entries are counted once per launch (their minimum traffic).
+    """
+    NB = 16
+    byts = 0
+    flops = 0
+    heights = nodes["height"]
+    depths = nodes["depth"]
+    for i in range(1, len(nodes)):
+        f = fact.node_info(i)
+        m, n, k, nk, nblk = f["m"], f["n"], f["k"], f["nk"], f["nblk"]
+        kv = int(nodes["kv"][i])
+        if kernel == "k_ulv_sup" and heights[i] == level:
+            fe = 0 if f["pass"] else (m * k + k * k + nk * nk + nblk * NB * NB + k * nk + kv * nk)
+            if nodes["left"][i] >= 0:
+                L, R = int(nodes["left"][i]), int(nodes["right"][i])
+                fe += fact.node_info(L)["mr"] * int(nodes["kv"][R]) + fact.node_info(R)["mr"] * int(nodes["kv"][L])
+            vin = m * (8 if nodes["left"][i] < 0 else wv)
+            vout = wv * (nk + (f["mr"]) + kv)
+            byts += wf * fe + nrhs * (vin + vout)
+            flops += nrhs * (4 * m * k + 2 * k * k + nk * nk + 2 * k * nk + 2 * kv * nk)
+        elif kernel == "k_ulv_sdown" and depths[i] == level:
+            fe = 0 if f["pass"] else (n * nk + nblk * NB * NB)
+            vout = n * (8 if nodes["left"][i] < 0 else wv)
+            byts += wf * fe + nrhs * (wv * n + vout)
+            flops += nrhs * (4 * n * nk + 2 * nblk * NB * NB)
+    return byts, flops
+
+
+def profile_launches(hip, fn, reps):
+    hip.prof_reset()
+    hip.prof_enable(True)
+    for _ in range(reps):
+        fn()
+    recs = hip.prof_records()
+    hip.prof_enable(False)
+    hip.prof_reset()
+    return recs
+
+
+def summarize(recs, reps):
+    tot = {}
+    for r in recs:
+        key = (r["kernel"], r["level"])
+        t = tot.setdefault(key, [0.0, 0, r["nwork"]])
+        t[0] += r["ms"]
+        t[1] += 1
+    by_kernel = {}
+    for (k, lev), (ms, cnt, nw) in tot.items():
+        by_kernel.setdefault(k, 0.0)
+        by_kernel[k] += ms / reps
+    return tot, by_kernel
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nrhs", type=int, default=NRHS)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-reps", type=int, default=8, help="1-thread CPU oracle calls in the baseline sample")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              f"the sharding design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as G
+    pkg = G.load_package()
+    import importlib
+    hip = importlib.import_module("hssmatrices_jl_amd.hip")
+    import torch
+
+    nodes, blob = load_generators(pkg)
+    summ = pkg.tree_summary(nodes)
+    n = int(nodes["n"][0])
+    nrhs = args.nrhs
+    log(f"[bench] HSS n={n} leaves={summ['leaves']} depth={summ['max_depth']} ranks {summ['rank_min']}-"
+        f"{summ['rank_max']} G={summ['G']}")
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    ctx = hip.default_context()
+    A = hip.HipHssMatrix((nodes, blob), ctx)
+
+    t0 = time.perf_counter()
+    F64 = A.ulvfactor("fp64")
+    f64_wall = time.perf_counter() - t0
+    f64_ms = F64.factor_ms
+    t0 = time.perf_counter()
+    F32 = A.ulvfactor("fp32")
+    f32_wall = time.perf_counter() - t0
+    f32_ms = F32.factor_ms
+    log(f"[bench] factor fp64 {f64_ms:.2f} ms (wall {f64_wall*1e3:.1f}), fp32 {f32_ms:.2f} ms; "
+        f"factor bytes fp64 {F64.device_bytes()/1e6:.1f} MB")
+
+    rng = np.random.default_rng(1)
+    B_host = rng.standard_normal((n, nrhs))
+    Bt = torch.from_numpy(np.ascontiguousarray(B_host.T)).to(dev)  # (nrhs, n): column-major n x nrhs
+    Xt = torch.zeros_like(Bt)
+    X2t = torch.zeros_like(Bt)
+    Ct = torch.zeros_like(Bt)
+    ld = n
+
+    def solve64():
+        F64.solve_dev(nrhs, Bt.data_ptr(), ld, Xt.data_ptr(), ld)
+
+    def solve_mixed():
+        return F32.solve_dev(nrhs, Bt.data_ptr(), ld, X2t.data_ptr(), ld, tol=1e-10, max_iter=10)
+
+    def matvec():
+        A.matmul_dev(nrhs, 1.0, Bt.data_ptr(), ld, 0.0, Ct.data_ptr(), ld)
+
+    # ---- fp64: throughput (back to back) and latency (each step synchronized) ----
+    for _ in range(args.warmup):
+        solve64()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solve64()
+    torch.cuda.synchronize()
+    fp64_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    lat64 = []
+    for _ in range(args.steps):
+        s = time.perf_counter()
+        solve64()
+        torch.cuda.synchronize()
+        lat64.append((time.perf_counter() - s) * 1e3)
+    log(f"[bench] fp64 solve: {fp64_ms:.3f} ms/step, p50 {np.percentile(lat64, 50):.3f} ms")
+
+    # ---- mixed: every step refines to 1e-10 (synchronous) ----
+    reps = []
+    for _ in range(args.warmup):
+        solve_mixed()
+    torch.cuda.synchronize()
+    latm = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s = time.perf_counter()
+        reps.append(solve_mixed())
+        torch.cuda.synchronize()
+        latm.append((time.perf_counter() - s) * 1e3)
+    mixed_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    iters = [r["iterations"] for r in reps]
+    worst_res = max(r["rel_residual"] for r in reps)
+    log(f"[bench] mixed solve: {mixed_ms:.3f} ms/step, refinement steps {min(iters)}-{max(iters)}, "
+        f"worst residual {worst_res:.2e}")
+
+    # ---- matvec ----
+    for _ in range(args.warmup):
+        matvec()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        matvec()
+    torch.cuda.synchronize()
+    mv_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    mv_bytes, mv_flops = matvec_bytes(nodes, nrhs)
+    log(f"[bench] matvec: {mv_ms:.3f} ms, {mv_bytes / mv_ms / 1e6:.1f} GB/s algorithmic")
+
+    # ---- per-kernel timing (HIP events around each launch on the library stream) ----
+    prof_reps = 3
+    recs_solve = profile_launches(hip, solve64, prof_reps)
+    tot_s, byk_s = summarize(recs_solve, prof_reps)
+    recs_mv = profile_launches(hip, matvec, prof_reps)
+    tot_m, byk_m = summarize(recs_mv, prof_reps)
+    recs_mx = profile_launches(hip, solve_mixed, 1)
+    _, byk_x = summarize(recs_mx, 1)
+    log(f"[bench] fp64 solve per kernel (ms/solve): { {k: round(v, 4) for k, v in byk_s.items()} }")
+    log(f"[bench] matvec per kernel (ms): { {k: round(v, 4) for k, v in byk_m.items()} }")
+    log(f"[bench] mixed solve per kernel (ms/solve): { {k: round(v, 4) for k, v in byk_x.items()} }")
+    # dominant kernel of the fp64 solve and its busiest launch
+    dom = max(byk_s, key=byk_s.get)
+    cand = [(key, v) for key, v in tot_s.items() if key[0] == dom]
+    (dk, dlev), (dms, dcnt, dnw) = max(cand, key=lambda kv: kv[1][0])
+    avg_ms = dms / dcnt
+    dbytes, dflops = solve_kernel_model(nodes, F64, dk, dlev, nrhs, 8, 8)
+    achieved = dbytes / avg_ms / 1e6
+    log(f"[bench] dominant solve kernel {dk} level {dlev}: {avg_ms:.4f} ms/launch, "
+        f"{dbytes/1e6:.2f} MB -> {achieved:.1f} GB/s, {dflops/avg_ms/1e9:.2f} TFLOP/s")
+    mvleaf = [(key, v) for key, v in tot_m.items() if key[0] == "k_mv_leaf"]
+    mv_leaf_ms = mvleaf[0][1][0] / mvleaf[0][1][1]
+    mv_leaf_bytes = matvec_leaf_bytes(nodes, nrhs)
+
+    # ---- parity against the CPU oracle (untimed) and the CPU baseline ----
+    X64 = Xt.cpu().numpy().T
+    Xmx = X2t.cpu().numpy().T
+    parity = {}
+    cpu = None
+    from oracle import oracle as O
+    from threadpoolctl import threadpool_info, threadpool_limits
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        Xo = O.ulvfactsolve(nodes, blob, B_host)
+        t1 = time.perf_counter() - t0
+    parity["fp64_vs_oracle_rel"] = float(np.linalg.norm(X64 - Xo) / np.linalg.norm(Xo))
+    parity["mixed_vs_oracle_rel"] = float(np.linalg.norm(Xmx - Xo) / np.linalg.norm(Xo))
+    R = O.matmul(nodes, blob, X64) - B_host
+    parity["fp64_rel_residual"] = float(np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B_host, axis=0)))
+    R = O.matmul(nodes, blob, Xmx) - B_host
+    parity["mixed_rel_residual"] = float(np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B_host, axis=0)))
+    log(f"[bench] parity {parity}")
+    if not args.no_cpu:
+        times = [t1]
+        with threadpool_limits(1):
+            for _ in range(max(args.cpu_reps - 1, 0)):
+                t0 = time.perf_counter()
+                O.ulvfactsolve(nodes, blob, B_host)
+                times.append(time.perf_counter() - t0)
+        ncores = os.cpu_count() or 1
+        with threadpool_limits(ncores):
+            t0 = time.perf_counter()
+            O.ulvfactsolve(nodes, blob, B_host)
+            tall = time.perf_counter() - t0
+        blas = [i for i in threadpool_info() if i.get("internal_api") == "openblas"]
+        cpu = {
+            "value": round(1.0 / float(np.mean(times)), 4),
+            "unit": "solves/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": (f"{len(times)} calls of the oracle's ulvfactsolve (C restatement of ulvfactor.jl:10-107 "
+                       f"over OpenBLAS {blas[0].get('version') if blas else '?'} via scipy), n={n}, {nrhs} RHS, "
+                       f"1 BLAS thread; each call factors and solves as the reference's hssA\\B does"),
+            "all_threads": {"threads": ncores, "seconds_per_solve": round(tall, 3)},
+            "seconds_per_solve": round(float(np.mean(times)), 3),
+        }
+        log(f"[bench] cpu baseline {cpu}")
+
+    value = 1e3 / mixed_ms
+    line = {
+        "metric": "ULV solves/sec + p50/p99 latency, mixed-prec vs FP64 GPU at 1e-10 residual",
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 factor + f64 refinement",
+        "data": "synthetic: log|x_i-x_j| kernel on sorted uniform points, seeds 0/1",
+        "config": {
+            "workload": f"HSS n={n} log kernel, leaf {LEAF}, rtol {TOL:g}, {summ['leaves']} leaves, ranks "
+                        f"{summ['rank_min']}-{summ['rank_max']}, {nrhs} RHS, resident ULV factor, solve to "
+                        f"1e-10 relative residual",
+            "nrhs": nrhs,
+            "n": n,
+        },
+        "latency_ms": {"p50": round(float(np.percentile(latm, 50)), 4),
+                       "p99": round(float(np.percentile(latm, 99)), 4)},
+        "mixed": {"solves_per_s": round(1e3 / mixed_ms, 3), "refine_steps": [min(iters), max(iters)],
+                  "worst_rel_residual": worst_res, "factor_ms": round(f32_ms, 3)},
+        "fp64": {"solves_per_s": round(1e3 / fp64_ms, 3), "ms_per_step": round(fp64_ms, 4),
+                 "latency_ms": {"p50": round(float(np.percentile(lat64, 50)), 4),
+                                "p99": round(float(np.percentile(lat64, 99)), 4)},
+                 "factor_ms": round(f64_ms, 3), "factor_plus_solve_per_s": round(1e3 / (f64_ms + fp64_ms), 3)},
+        "mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
+        "matvec": {"ms": round(mv_ms, 4), "gbps_algorithmic": round(mv_bytes / mv_ms / 1e6, 1),
+                   "tflops": round(mv_flops / mv_ms / 1e9, 3),
+                   "leaf_kernel": {"ms": round(mv_leaf_ms, 4),
+                                   "gbps": round(mv_leaf_bytes / mv_leaf_ms / 1e6, 1)}},
+        "roofline": {"kernel": f"{dk} (level {dlev}, {dnw} nodes)", "bound": "hbm",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": int(dbytes), "avg_launch_ms": round(avg_ms, 5)},
+        "per_kernel_ms": {"fp64_solve": {k: round(v, 4) for k, v in byk_s.items()},
+                          "mixed_solve": {k: round(v, 4) for k, v in byk_x.items()},
+                          "matvec": {k: round(v, 4) for k, v in byk_m.items()}},
+        "parity": parity,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

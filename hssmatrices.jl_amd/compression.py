@@ -19,15 +19,35 @@ Two deliberate differences from the reference, both outside the GPU path:
   ``|R[k,k]| <= max(atol, rtol*|R[1,1]|)``.  For a wide block (more columns
   than twice the rows) the pivoted QR runs on ``R1^T`` where ``A^T = Q1 R1`` is
   an unpivoted QR: the column space and singular values are those of A, and
-  the cost drops from O(m*N*m) pivoting passes to one blocked QR.  Compression
+  the cost drops from O(m*N*m) pivoting passes to one blocked QR.  For very
+  wide blocks (N > 4m) the range is taken from a fixed-seed Gaussian sketch
+  Y = A*Omega with m + 32 columns (rank(A) <= m, so range(Y) = range(A)), and
+  the truncation is read off the pivoted QR of Y/sqrt(m+32).  Compression
   parity with the reference is therefore unpinned (its generators are not
   reproducible without LowRankApprox); every parity test of the GPU path feeds
   identical generators to the GPU and to the oracle instead.
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import scipy.linalg as sla
+
+try:
+    from threadpoolctl import ThreadpoolController as _TPC
+except Exception:  # pragma: no cover
+    _TPC = None
+_TP = [None]
+
+
+def _single_thread_blas():
+    if _TPC is None:
+        return None
+    if _TP[0] is None:
+        _TP[0] = _TPC()
+    return _TP[0].limit(limits=1, user_api="blas")
 
 try:  # torch's CPU LAPACK/BLAS (MKL, multi-threaded) is several times faster
     import torch as _torch  # than numpy's for the tall QR and the kernel blocks
@@ -58,51 +78,48 @@ class KernelMatrix:
         self.shape = (self.x.size, self.x.size)
 
     def block(self, r0: int, r1: int, c0: int, c1: int) -> np.ndarray:
-        if _torch is not None and (r1 - r0) * (c1 - c0) >= 65536:
-            return self._block_torch(r0, r1, c0, c1)
-        xr = self.x[r0:r1, None]
-        xc = self.x[None, c0:c1]
-        d = xr - xc
+        """A[r0:r1, c0:c1] as a Fortran-ordered array (evaluated in column chunks on host threads)."""
+        out = np.empty((r1 - r0, c1 - c0), order="F")
+        nc = c1 - c0
+        chunk = 2048
+        if (r1 - r0) * nc <= 4 * 65536:
+            self._fill(out, r0, r1, c0, c1)
+        else:
+            spans = [(a, min(a + chunk, nc)) for a in range(0, nc, chunk)]
+            list(_pool().map(lambda ab: self._fill(out[:, ab[0]:ab[1]], r0, r1, c0 + ab[0], c0 + ab[1]), spans))
+        return out
+
+    def _fill(self, out, r0, r1, c0, c1):
+        d = np.subtract.outer(self.x[r0:r1], self.x[c0:c1])
         with np.errstate(divide="ignore", invalid="ignore"):
             if self.kind == "cauchy":
-                out = 1.0 / d
+                np.divide(1.0, d, out=out)
             elif self.kind == "cauchy_test":
-                out = np.where(d > 0, 0.001 / d, 2.0)
+                out[...] = np.where(d > 0, 0.001 / d, 2.0)
             elif self.kind == "log":
-                out = np.log(np.abs(d))
+                np.abs(d, out=d)
+                np.log(d, out=out)
             else:
                 raise ValueError(f"unknown kernel {self.kind}")
         if self.kind != "cauchy_test":
-            # diagonal entries i == j inside this block
             lo, hi = max(r0, c0), min(r1, c1)
             if lo < hi:
                 idx = np.arange(lo, hi)
                 out[idx - r0, idx - c0] = self.diag
-        return np.asfortranarray(out)
-
-    def _block_torch(self, r0, r1, c0, c1):
-        x = _torch.from_numpy(self.x)
-        # evaluated transposed so that the result is column-major (Fortran) in numpy
-        d = x[r0:r1].unsqueeze(0) - x[c0:c1].unsqueeze(1)  # (cols x rows) = x_i - x_j
-        if self.kind == "cauchy":
-            out = 1.0 / d
-        elif self.kind == "cauchy_test":
-            out = _torch.where(d > 0, 0.001 / d, _torch.full_like(d, 2.0))
-        elif self.kind == "log":
-            out = _torch.log(_torch.abs(d))
-        else:
-            raise ValueError(f"unknown kernel {self.kind}")
-        out = out.numpy().T
-        if self.kind != "cauchy_test":
-            lo, hi = max(r0, c0), min(r1, c1)
-            if lo < hi:
-                idx = np.arange(lo, hi)
-                out[idx - r0, idx - c0] = self.diag
-        return out
 
     def dense(self) -> np.ndarray:
         n = self.shape[0]
         return self.block(0, n, 0, n)
+
+
+_POOL = None
+
+
+def _pool():
+    global _POOL
+    if _POOL is None:
+        _POOL = ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1))
+    return _POOL
 
 
 class _Dense:
@@ -126,7 +143,13 @@ def compress_block(A: np.ndarray, atol: float, rtol: float):
     m, N = A.shape
     if m == 0 or N == 0:
         return np.zeros((m, 0), order="F"), np.zeros((0, N), order="F")
-    if N > 2 * m:
+    if N > 4 * m and _torch is not None and m + 32 <= _SKETCH_COLS:
+        # wide block: Y = A Omega spans range(A) (rank(A) <= m < s almost surely);
+        # the pivoted QR of Y orders that range, and R = Q^T A below re-projects A
+        s_cols = m + 32
+        Om = _sketch(N, s_cols)
+        B = np.asfortranarray((_torch.from_numpy(A) @ Om).numpy()) / np.sqrt(s_cols)
+    elif N > 2 * m:
         if _torch is not None:
             # A is column-major, so A.T viewed row-major is (N x m) without a copy
             at = _torch.from_numpy(np.ascontiguousarray(A.T))
@@ -136,7 +159,12 @@ def compress_block(A: np.ndarray, atol: float, rtol: float):
         B = np.asfortranarray(r1.T)
     else:
         B = A
-    Q, R, _ = sla.qr(B, mode="economic", pivoting=True)
+    lim = _single_thread_blas()  # small problem: BLAS threading only adds overhead
+    try:
+        Q, R, _ = sla.qr(B, mode="economic", pivoting=True)
+    finally:
+        if lim is not None:
+            lim.restore_original_limits()
     d = np.abs(np.diag(R))
     if d.size == 0 or d[0] == 0.0:
         k = 0
@@ -154,6 +182,21 @@ def compress_block(A: np.ndarray, atol: float, rtol: float):
     else:
         Rk = np.asfortranarray(Qk.T @ A)
     return Qk, Rk
+
+
+_SKETCH = [None]
+_SKETCH_COLS = 512
+
+
+def _sketch(N: int, s: int):
+    """Fixed-seed Gaussian test matrix: the leading N x s block of one cached matrix."""
+    cur = _SKETCH[0]
+    if cur is None or cur.shape[0] < N:
+        rows = max(N, 65536 if cur is None else 2 * cur.shape[0])
+        g = _torch.Generator().manual_seed(1234567)
+        cur = _torch.randn(rows, _SKETCH_COLS, generator=g, dtype=_torch.float64)
+        _SKETCH[0] = cur
+    return cur[:N, :s]
 
 
 # --- lazily evaluated block rows / columns ----------------------------------

@@ -44,6 +44,47 @@ static hss_status fail(hss_status st, const std::string& msg) {
                         std::string(#expr) + ": " + hipGetErrorString(_e));                        \
     } while (0)
 
+// ---------------------------------------------------------------------------
+// optional per-launch timing (hss_prof_*): a HIP event pair around every
+// kernel launch on the library stream, resolved when the caller asks.
+// ---------------------------------------------------------------------------
+struct ProfRec {
+    const char* name;
+    int level;
+    int nwork;
+    hipEvent_t e0, e1;
+    float ms;
+};
+static bool g_prof = false;
+static std::vector<ProfRec> g_prof_recs;
+static std::vector<hipEvent_t> g_prof_pool;
+
+static hipEvent_t prof_event() {
+    if (!g_prof_pool.empty()) {
+        hipEvent_t e = g_prof_pool.back();
+        g_prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+struct ProfScope {
+    int idx = -1;
+    hipStream_t st;
+    ProfScope(const char* name, int level, int nwork, hipStream_t s) : st(s) {
+        if (!g_prof || nwork <= 0) return;
+        ProfRec r{name, level, nwork, prof_event(), prof_event(), -1.f};
+        (void)hipEventRecord(r.e0, st);
+        g_prof_recs.push_back(r);
+        idx = (int)g_prof_recs.size() - 1;
+    }
+    ~ProfScope() {
+        if (idx >= 0) (void)hipEventRecord(g_prof_recs[idx].e1, st);
+    }
+};
+
 template <typename T>
 static hipError_t dalloc(T** p, size_t count) {
     *p = nullptr;
@@ -398,14 +439,19 @@ static hss_status matmul_enqueue(hss_mat_s* M, int nrhs, double alpha, const dou
     if (M->nn > 1) {
         for (int h = 0; h < M->max_height; ++h) {
             const Range& r = M->by_height[h];
+            ProfScope ps("k_mv_up", h, r.count, st);
             HIP_TRY(launch_mv_up(di, M->d_work + r.off, r.count, nrhs, h == 0 ? M->lds_up : 0, st));
         }
         for (int d = 0; d <= M->max_depth; ++d) {
             const Range& r = M->br_depth[d];
+            ProfScope ps("k_mv_down", d, r.count, st);
             HIP_TRY(launch_mv_down(di, M->d_work + r.off, r.count, nrhs, st));
         }
     }
-    HIP_TRY(launch_mv_leaf(di, M->d_work + M->leaves.off, M->leaves.count, nrhs, M->lds_leaf, st));
+    {
+        ProfScope ps("k_mv_leaf", 0, M->leaves.count, st);
+        HIP_TRY(launch_mv_leaf(di, M->d_work + M->leaves.off, M->leaves.count, nrhs, M->lds_leaf, st));
+    }
     return HSS_OK;
 }
 
@@ -687,10 +733,18 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
         HIP_TRY(hipMemcpyAsync(d_inst, hinst.data(), sizeof(FInst<TF>) * cnt, hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(d_work, work.data(), sizeof(WorkItem) * work.size(), hipMemcpyHostToDevice, st));
         HIP_TRY(hipEventRecord(ctx->ev0, st));
-        for (int h = 0; h < maxh; ++h)
+        for (int h = 0; h < maxh; ++h) {
+            ProfScope ps("k_ulv_factor", h, lev[h].count, st);
             HIP_TRY((launch_ulv_factor<TF>(d_inst, d_work + lev[h].off, lev[h].count, lev_lds[h], st)));
-        HIP_TRY((launch_ulv_wy<TF>(d_inst, d_work, wy_count, st)));
-        HIP_TRY((launch_ulv_root<TF>(d_inst, d_work + roots.off, roots.count, st)));
+        }
+        {
+            ProfScope ps("k_ulv_wy", 0, wy_count, st);
+            HIP_TRY((launch_ulv_wy<TF>(d_inst, d_work, wy_count, st)));
+        }
+        {
+            ProfScope ps("k_ulv_root", 0, roots.count, st);
+            HIP_TRY((launch_ulv_root<TF>(d_inst, d_work + roots.off, roots.count, st)));
+        }
         HIP_TRY(hipEventRecord(ctx->ev1, st));
         HIP_TRY(hipStreamSynchronize(st));
         float ms = 0.f;
@@ -839,11 +893,18 @@ template <typename TF, typename TV>
 static hss_status launch_schedule(const SInst<TF, TV>* di, const WorkItem* dw, const std::vector<Range>& up,
                                   const Range& roots, const std::vector<Range>& down, int maxnrhs, size_t lsup,
                                   size_t lsroot, size_t lsdown, hipStream_t st) {
-    for (size_t h = 0; h < up.size(); ++h)
+    for (size_t h = 0; h < up.size(); ++h) {
+        ProfScope ps("k_ulv_sup", (int)h, up[h].count, st);
         HIP_TRY((launch_ulv_sup<TF, TV>(di, dw + up[h].off, up[h].count, maxnrhs, lsup, st)));
-    HIP_TRY((launch_ulv_sroot<TF, TV>(di, dw + roots.off, roots.count, maxnrhs, lsroot, st)));
-    for (size_t d = 1; d < down.size(); ++d)
+    }
+    {
+        ProfScope ps("k_ulv_sroot", 0, roots.count, st);
+        HIP_TRY((launch_ulv_sroot<TF, TV>(di, dw + roots.off, roots.count, maxnrhs, lsroot, st)));
+    }
+    for (size_t d = 1; d < down.size(); ++d) {
+        ProfScope ps("k_ulv_sdown", (int)d, down[d].count, st);
         HIP_TRY((launch_ulv_sdown<TF, TV>(di, dw + down[d].off, down[d].count, maxnrhs, lsdown, st)));
+    }
     return HSS_OK;
 }
 
@@ -1091,4 +1152,43 @@ extern "C" hss_status hss_ulv_solve_batch_dev(int32_t count, const hss_fact_t* f
 // diagnostics
 // ---------------------------------------------------------------------------
 extern "C" const char* hss_last_error(void) { return g_err.c_str(); }
+
+extern "C" void hss_prof_enable(int on) { g_prof = on != 0; }
+
+extern "C" void hss_prof_reset(void) {
+    for (auto& r : g_prof_recs) {
+        g_prof_pool.push_back(r.e0);
+        g_prof_pool.push_back(r.e1);
+    }
+    g_prof_recs.clear();
+}
+
+extern "C" int32_t hss_prof_count(void) {
+    // resolve pending timings (waits for the recorded events)
+    for (auto& r : g_prof_recs)
+        if (r.ms < 0.f) {
+            (void)hipEventSynchronize(r.e1);
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) r.ms = ms;
+        }
+    return (int32_t)g_prof_recs.size();
+}
+
+extern "C" hss_status hss_prof_get(int32_t i, const char** name, int32_t* level, int32_t* nwork, double* ms) {
+    if (i < 0 || i >= (int32_t)g_prof_recs.size()) return fail(HSS_ERR_ARG, "profile index out of range");
+    const ProfRec& r = g_prof_recs[i];
+    if (name) *name = r.name;
+    if (level) *level = r.level;
+    if (nwork) *nwork = r.nwork;
+    if (ms) *ms = r.ms;
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_fact_node_info(hss_fact_t F, int32_t node, int32_t* info) {
+    if (!F || node < 0 || node >= (int32_t)F->hfn.size() || !info) return fail(HSS_ERR_ARG, "bad node query");
+    const FNode& f = F->hfn[node];
+    info[0] = f.m; info[1] = f.n; info[2] = f.k; info[3] = f.nk;
+    info[4] = f.mr; info[5] = f.nr; info[6] = f.pass; info[7] = f.nblk;
+    return HSS_OK;
+}
 extern "C" double hss_last_device_ms(void) { return g_last_ms; }

@@ -46,6 +46,7 @@ EXPORTS = [
     "hss_ulvfactsolve", "hss_ulv_factor", "hss_fact_destroy", "hss_fact_device_bytes",
     "hss_ulv_solve", "hss_ulv_solve_dev", "hss_ulv_solve_batch_dev", "hss_ulv_factor_batch",
     "hss_last_error", "hss_last_device_ms",
+    "hss_prof_enable", "hss_prof_reset", "hss_prof_count", "hss_prof_get", "hss_fact_node_info",
 ]
 
 
@@ -104,6 +105,12 @@ def load_library(path: pathlib.Path | str | None = None):
         "hss_ulv_factor_batch": ([I32, V, ctypes.c_int, V], ctypes.c_int),
         "hss_last_error": ([], ctypes.c_char_p),
         "hss_last_device_ms": ([], D),
+        "hss_prof_enable": ([ctypes.c_int], None),
+        "hss_prof_reset": ([], None),
+        "hss_prof_count": ([], I32),
+        "hss_prof_get": ([I32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(I32), ctypes.POINTER(I32),
+                          ctypes.POINTER(D)], ctypes.c_int),
+        "hss_fact_node_info": ([V, I32, ctypes.POINTER(I32)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -130,6 +137,29 @@ def _check(status: int):
 
 def last_device_ms() -> float:
     return float(load_library().hss_last_device_ms())
+
+
+def prof_enable(on: bool = True):
+    """Bracket every library kernel launch with HIP events (``hss_prof_enable``)."""
+    load_library().hss_prof_enable(1 if on else 0)
+
+
+def prof_reset():
+    load_library().hss_prof_reset()
+
+
+def prof_records():
+    """[{kernel, level, nwork, ms}] of the launches recorded since the last reset (waits for them)."""
+    lib = load_library()
+    out = []
+    for i in range(lib.hss_prof_count()):
+        name = ctypes.c_char_p()
+        lev = ctypes.c_int32()
+        nw = ctypes.c_int32()
+        ms = ctypes.c_double()
+        _check(lib.hss_prof_get(i, ctypes.byref(name), ctypes.byref(lev), ctypes.byref(nw), ctypes.byref(ms)))
+        out.append({"kernel": name.value.decode(), "level": lev.value, "nwork": nw.value, "ms": ms.value})
+    return out
 
 
 class Context:
@@ -302,6 +332,13 @@ class HipUlvFactor:
 
     def device_bytes(self) -> int:
         return int(_lib.hss_fact_device_bytes(self._h))
+
+    def node_info(self, node: int) -> dict:
+        """Factor sizes of one node (``hss_fact_node_info``)."""
+        buf = (ctypes.c_int32 * 8)()
+        _check(_lib.hss_fact_node_info(self._h, node, buf))
+        keys = ("m", "n", "k", "nk", "mr", "nr", "pass", "nblk")
+        return dict(zip(keys, list(buf)))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -1,0 +1,175 @@
+# HssHip.jl -- Julia binding of libhss_hip.so (include/hss_hip.h) for HssMatrices.jl.
+#
+# The reference (bonevbs/HssMatrices.jl 0.1.6) has no FFI: its boundary is
+# dispatch on HssMatrix{T}.  This module is the binding a maintainer would add:
+# it packs an HssMatrix{Float64} level-major (breadth-first node table +
+# column-major generator blob, the layout hss_mat_create expects), keeps the
+# generators in HBM, and provides the hot-path methods on a wrapper type
+# (no type piracy on HssMatrix itself):
+#
+#   *(A::HipHss, B)            <- *(hssA, B)            src/matmul.jl:13,15
+#   mul!(C, A::HipHss, B, a, b) <- mul!(C, hssA, B, a, b) src/matmul.jl:18
+#   \(A::HipHss, B)            <- \(hssA, B::Matrix)     src/hssmatrix.jl:234
+#   ulvfactsolve(A::HipHss, b) <- ulvfactsolve           src/ulvfactor.jl:10
+#   ulvfactor(A::HipHss)       -> resident factor (no reference counterpart)
+#
+# Status codes map to the reference's exceptions: DimensionMismatch,
+# ArgumentError, SingularException.  Julia is not available in the build
+# container, so this file is not executed by the test suite; the same ABI is
+# exercised by the Python binding (hssmatrices.jl_amd/hip.py) and the tests.
+module HssHip
+
+using LinearAlgebra
+using HssMatrices
+import Base: *, \, size
+import LinearAlgebra: mul!
+
+const libhss = joinpath(@__DIR__, "..", "libhss_hip.so")
+
+# hss_node_desc (112 bytes), field order of include/hss_hip.h
+struct NodeDesc
+    parent::Int32; left::Int32; right::Int32; depth::Int32; height::Int32
+    m::Int32; n::Int32; ku::Int32; kv::Int32; pad::Int32
+    row_off::Int64; col_off::Int64
+    off_D::Int64; off_U::Int64; off_V::Int64
+    off_B12::Int64; off_B21::Int64
+    off_R::Int64; off_W::Int64
+end
+
+struct HssHipError <: Exception
+    status::Cint
+    msg::String
+end
+
+function check(st::Cint)
+    st == 0 && return nothing
+    msg = unsafe_string(ccall((:hss_last_error, libhss), Cstring, ()))
+    st == 1 && throw(DimensionMismatch(msg))
+    st == 2 && throw(ArgumentError(msg))
+    st == 3 && throw(SingularException(0))
+    throw(HssHipError(st, msg))
+end
+
+mutable struct Context
+    h::Ptr{Cvoid}
+end
+function Context(device::Integer=0)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hss_ctx_create, libhss), Cint, (Cint, Ptr{Ptr{Cvoid}}), device, r))
+    c = Context(r[])
+    finalizer(c -> ccall((:hss_ctx_destroy, libhss), Cint, (Ptr{Cvoid},), c.h), c)
+    return c
+end
+const DEFAULT_CTX = Ref{Union{Nothing,Context}}(nothing)
+default_context() = something(DEFAULT_CTX[], (DEFAULT_CTX[] = Context(0)))
+
+# Breadth-first packing; R1/W1 (R2/W2) are stored with the left (right) child.
+function pack(hssA::HssMatrix{Float64})
+    order = Any[(hssA, -1, nothing, nothing, 0, 0, 0)]
+    i = 1
+    while i <= length(order)
+        node, _, _, _, d, ro, co = order[i]
+        if !HssMatrices.isleaf(node)
+            m1, n1 = node.sz1
+            push!(order, (node.A11, i - 1, node.R1, node.W1, d + 1, ro, co))
+            push!(order, (node.A22, i - 1, node.R2, node.W2, d + 1, ro + m1, co + n1))
+        end
+        i += 1
+    end
+    blob = Float64[]
+    put(A) = (off = length(blob); append!(blob, vec(A)); Int64(off))
+    recs = Vector{Vector{Int64}}()
+    left = fill(Int32(-1), length(order)); right = fill(Int32(-1), length(order))
+    for (k, (node, parent, R, W, d, ro, co)) in enumerate(order)
+        if parent >= 0
+            left[parent + 1] < 0 ? (left[parent + 1] = k - 1) : (right[parent + 1] = k - 1)
+        end
+    end
+    nodes = NodeDesc[]
+    for (k, (node, parent, R, W, d, ro, co)) in enumerate(order)
+        m, n = size(node)
+        ku, kv = parent < 0 ? (0, 0) : HssMatrices.gensize(node)
+        oD = oU = oV = oB12 = oB21 = oR = oW = Int64(-1)
+        if HssMatrices.isleaf(node)
+            oD = put(node.D)
+            oU = put(parent < 0 ? zeros(m, 0) : node.U)
+            oV = put(parent < 0 ? zeros(n, 0) : node.V)
+        else
+            oB12 = put(node.B12); oB21 = put(node.B21)
+        end
+        if parent >= 0
+            oR = put(R); oW = put(W)
+        end
+        push!(nodes, NodeDesc(parent, left[k], right[k], d, 0, m, n, ku, kv, 0, ro, co,
+                              oD, oU, oV, oB12, oB21, oR, oW))
+    end
+    return nodes, blob   # heights are recomputed by the library
+end
+
+mutable struct HipHss
+    h::Ptr{Cvoid}
+    m::Int
+    n::Int
+    ctx::Context
+end
+
+function HipHss(hssA::HssMatrix{Float64}, ctx::Context=default_context())
+    nodes, blob = pack(hssA)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hss_mat_create, libhss), Cint,
+                (Ptr{Cvoid}, Ptr{NodeDesc}, Int32, Ptr{Float64}, Int64, Ptr{Ptr{Cvoid}}),
+                ctx.h, nodes, length(nodes), blob, length(blob), r))
+    A = HipHss(r[], size(hssA, 1), size(hssA, 2), ctx)
+    finalizer(A -> ccall((:hss_mat_destroy, libhss), Cint, (Ptr{Cvoid},), A.h), A)
+    return A
+end
+
+size(A::HipHss) = (A.m, A.n)
+size(A::HipHss, d::Integer) = size(A)[d]
+
+function mul!(C::StridedMatrix{Float64}, A::HipHss, B::StridedMatrix{Float64}, α::Number, β::Number)
+    size(B, 1) == A.n || throw(DimensionMismatch("First dimension of B does not match second dimension of A. Expected $(A.n), got $(size(B, 1))"))
+    size(C) == (A.m, size(B, 2)) || throw(DimensionMismatch("Dimensions of C don't match up with A and B."))
+    check(ccall((:hss_matmul, libhss), Cint,
+                (Ptr{Cvoid}, Int32, Float64, Ptr{Float64}, Int64, Float64, Ptr{Float64}, Int64),
+                A.h, size(B, 2), α, B, stride(B, 2), β, C, stride(C, 2)))
+    return C
+end
+*(A::HipHss, B::AbstractMatrix) = mul!(zeros(A.m, size(B, 2)), A, Matrix{Float64}(B), 1.0, 0.0)
+*(A::HipHss, x::AbstractVector) = reshape(A * reshape(x, length(x), 1), length(x))
+
+function ulvfactsolve(A::HipHss, b::Matrix{Float64})
+    A.m == A.n == size(b, 1) || throw(DimensionMismatch("matrix is $(size(A)), right-hand side has $(size(b, 1)) rows"))
+    X = zeros(A.n, size(b, 2))
+    check(ccall((:hss_ulvfactsolve, libhss), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64, Ptr{Float64}, Int64),
+                A.h, size(b, 2), b, stride(b, 2), X, stride(X, 2)))
+    return X
+end
+\(A::HipHss, B::Matrix{Float64}) = ulvfactsolve(A, B)
+# the vector method the reference lacks (its Vector RHS falls back to a dense solve)
+\(A::HipHss, x::Vector{Float64}) = vec(ulvfactsolve(A, reshape(x, length(x), 1)))
+
+mutable struct HipUlv
+    h::Ptr{Cvoid}
+    A::HipHss
+end
+function ulvfactor(A::HipHss; prec::Symbol=:fp64)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hss_ulv_factor, libhss), Cint, (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}),
+                A.h, prec === :fp32 ? 1 : 0, r))
+    F = HipUlv(r[], A)
+    finalizer(F -> ccall((:hss_fact_destroy, libhss), Cint, (Ptr{Cvoid},), F.h), F)
+    return F
+end
+function \(F::HipUlv, B::Matrix{Float64})
+    X = zeros(F.A.n, size(B, 2))
+    check(ccall((:hss_ulv_solve, libhss), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Cvoid}, Ptr{Cvoid}),
+                F.h, size(B, 2), B, stride(B, 2), X, stride(X, 2), C_NULL, C_NULL))
+    return X
+end
+
+export HipHss, ulvfactsolve, ulvfactor
+
+end # module

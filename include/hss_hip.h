@@ -147,6 +147,16 @@ hss_status hss_ulv_factor_batch(int32_t count, const hss_mat_t* mats, hss_prec p
 
 /* ---- diagnostics ---- */
 const char* hss_last_error(void);
+/* Per-launch timing of the library's kernels (no counterpart in the reference;
+ * for profiling and the roofline report): when enabled, every kernel launch is
+ * bracketed by HIP events on the context stream.  hss_prof_count() waits for
+ * the recorded launches and returns how many there are. */
+void hss_prof_enable(int on);
+void hss_prof_reset(void);
+int32_t hss_prof_count(void);
+hss_status hss_prof_get(int32_t i, const char** name, int32_t* level, int32_t* nwork, double* ms);
+/* Factor sizes of one node: {m, n, k, nk, mr, nr, pass, nblk} (8 ints). */
+hss_status hss_fact_node_info(hss_fact_t fact, int32_t node, int32_t* info);
 /* Per-call device timing of the last matvec/solve on this thread (milliseconds, -1 if none). */
 double hss_last_device_ms(void);
 

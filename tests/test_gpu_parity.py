@@ -1,0 +1,208 @@
+"""GPU parity: libhss_hip.so (through its C ABI) against the CPU oracle.
+
+Tolerances (Float64):
+  * matvec: relative difference <= 1e-14 (a sum of the same products in a
+    different order);
+  * ULV solve: relative difference <= 1e-12 on config C1 (the north star's
+    bound; cond(A) ~ 3.1e3) and <= 50 * cond(A) * eps on the golden fixtures
+    (two backward-stable solvers differ by at most ~cond * eps);
+  * mixed precision: refinement must reach the requested residual 1e-10 and
+    the result is compared with the Float64 oracle.
+"""
+import pathlib
+
+import numpy as np
+import pytest
+
+from conftest import H, cauchy_c1, random_hss
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.load(pathlib.Path(__file__).with_name("golden") / "hss_golden.npz")
+NAMES = [str(s) for s in GOLD["names"]]
+EPS = 2.220446049250313e-16
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def c1_dev(hip, c1):
+    return hip.HipHssMatrix((c1["nodes"], c1["blob"]))
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_matvec_golden(hip, oracle, name):
+    nodes, blob = GOLD[name + "/nodes"], GOLD[name + "/blob"]
+    A = hip.HipHssMatrix((nodes, blob))
+    X = GOLD[name + "/X"]
+    Y = A @ X
+    assert rel(Y, GOLD[name + "/Y"]) < 1e-14
+    assert rel(Y, oracle.matmul(nodes, blob, X)) < 1e-14
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_solve_golden(hip, oracle, name):
+    nodes, blob = GOLD[name + "/nodes"], GOLD[name + "/blob"]
+    A = hip.HipHssMatrix((nodes, blob))
+    B = GOLD[name + "/B"]
+    Z = A.solve(B)
+    cond = float(GOLD[name + "/cond"])
+    assert rel(Z, GOLD[name + "/Z"]) < 50 * cond * EPS
+    assert rel(Z, oracle.ulvfactsolve(nodes, blob, B)) < 50 * cond * EPS
+
+
+@pytest.mark.parametrize("nrhs", [1, 5, 31, 32, 33, 64, 65])
+def test_c1_matvec_parity(c1_dev, oracle, c1, nrhs):
+    rng = np.random.default_rng(nrhs)
+    X = rng.standard_normal((2001, nrhs))
+    assert rel(c1_dev @ X, oracle.matmul(c1["nodes"], c1["blob"], X)) < 1e-14
+
+
+def test_c1_matvec_vector_and_alpha_beta(hip, c1_dev, oracle, c1):
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(2001)
+    y = c1_dev @ x
+    assert y.shape == (2001,)
+    assert rel(y, oracle.matmul(c1["nodes"], c1["blob"], x)) < 1e-14
+    B = rng.standard_normal((2001, 3))
+    C0 = np.asfortranarray(rng.standard_normal((2001, 3)))
+    C = C0.copy(order="F")
+    hip.mul(C, c1_dev, B, -0.75, 1.5)
+    ref = oracle.matmul(c1["nodes"], c1["blob"], B, alpha=-0.75, beta=1.5, C=C0)
+    assert rel(C, ref) < 1e-14
+
+
+@pytest.mark.parametrize("nrhs", [1, 5, 33, 64])
+def test_c1_solve_parity(c1_dev, oracle, c1, nrhs):
+    """North star: the FP64 GPU path matches the reference's hssA\\x to 1e-12 relative."""
+    rng = np.random.default_rng(100 + nrhs)
+    B = rng.standard_normal((2001, nrhs))
+    Z = c1_dev.solve(B)
+    assert rel(Z, oracle.ulvfactsolve(c1["nodes"], c1["blob"], B)) <= 1e-12
+
+
+def test_c1_reference_tolerances(c1_dev, c1):
+    """test/runtests.jl:63-67 against the dense matrix, on the GPU path."""
+    rng = np.random.default_rng(11)
+    A = c1["A"]
+    x = rng.standard_normal((2001, 5))
+    assert rel(c1_dev @ x, A @ x) <= 50 * 1e-6
+    rhs = rng.standard_normal((2001, 5))
+    assert rel(c1_dev.solve(rhs), np.linalg.solve(A, rhs)) <= 50 * 1e-6
+
+
+def test_resident_factor_reuse(c1_dev, oracle, c1):
+    F = c1_dev.ulvfactor("fp64")
+    rng = np.random.default_rng(12)
+    for nrhs in (1, 40, 3):
+        B = rng.standard_normal((2001, nrhs))
+        assert rel(F.solve(B), oracle.ulvfactsolve(c1["nodes"], c1["blob"], B)) <= 1e-12
+
+
+def test_solve_is_deterministic(c1_dev):
+    rng = np.random.default_rng(13)
+    B = rng.standard_normal((2001, 8))
+    F = c1_dev.ulvfactor("fp64")
+    a = F.solve(B)
+    b = F.solve(B)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind,n,leaf", [("cauchy", 2001, 64), ("log", 4096, 128)])
+def test_mixed_precision_refinement(hip, oracle, kind, n, leaf):
+    rng = np.random.default_rng(14)
+    if kind == "cauchy":
+        _, h = cauchy_c1()
+    else:
+        _, h = random_hss(rng, n, leaf, 0, "log", 1e-9)
+    nodes, blob = H.pack(h)
+    A = hip.HipHssMatrix((nodes, blob))
+    F = A.ulvfactor("fp32")
+    B = rng.standard_normal((n, 7))
+    X, rep = F.solve(B, tol=1e-10, max_iter=20, report=True)
+    assert rep["rel_residual"] <= 1e-10
+    R = oracle.matmul(nodes, blob, X) - B
+    assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)) <= 1e-10
+    Xo = oracle.ulvfactsolve(nodes, blob, B)
+    # error vs the FP64 oracle is bounded by cond * residual
+    assert rel(X, Xo) < 1e-6
+
+
+def test_batched_factor_and_solve_match_single(hip, oracle):
+    """Uneven instances in one launch per level give the same answers as one at a time."""
+    import torch
+
+    rng = np.random.default_rng(15)
+    mats, hs = [], []
+    for n, leaf in ((700, 64), (1500, 64), (333, 32), (1024, 64)):
+        _, h = random_hss(rng, n, leaf, 0, "cauchy", 1e-8)
+        nodes, blob = H.pack(h)
+        hs.append((nodes, blob))
+        mats.append(hip.HipHssMatrix((nodes, blob)))
+    facts = hip.ulvfactor_batch(mats, "fp64")
+    dev = torch.device("cuda:0")
+    Bs, Xs, nrhs = [], [], []
+    for i, (nodes, blob) in enumerate(hs):
+        n = int(nodes["n"][0])
+        r = 1 + 7 * i
+        B = rng.standard_normal((n, r))
+        Bs.append(torch.from_numpy(np.ascontiguousarray(B.T)).to(dev))
+        Xs.append(torch.zeros((r, n), dtype=torch.float64, device=dev))
+        nrhs.append(r)
+    hip.solve_batch_dev(facts, nrhs, [b.data_ptr() for b in Bs], [int(b.shape[1]) for b in Bs],
+                        [x.data_ptr() for x in Xs], [int(x.shape[1]) for x in Xs])
+    torch.cuda.synchronize()
+    for i, (nodes, blob) in enumerate(hs):
+        B = Bs[i].cpu().numpy().T
+        Xb = Xs[i].cpu().numpy().T
+        Xs1 = mats[i].ulvfactor("fp64").solve(B)
+        assert np.array_equal(Xb, Xs1)
+        assert rel(Xb, oracle.ulvfactsolve(nodes, blob, B)) < 1e-11
+
+
+def test_errors_map_to_reference_exceptions(hip, c1_dev):
+    from hssmatrices_jl_amd.hssmatrix import DimensionMismatch, SingularException
+    with pytest.raises(DimensionMismatch):
+        c1_dev @ np.zeros((2000, 2))
+    with pytest.raises(DimensionMismatch):
+        c1_dev.solve(np.zeros((7, 1)))
+    from test_oracle import singular_root_hss
+    S = hip.HipHssMatrix(H.pack(singular_root_hss()))
+    with pytest.raises(SingularException):
+        S.solve(np.ones(4))
+    Z = hip.HipHssMatrix(H.pack(H.HssMatrix(D=np.zeros((6, 6)), rootnode=True)))
+    with pytest.raises(SingularException):
+        Z.solve(np.ones(6))
+    # zero right-hand sides is a no-op
+    assert (c1_dev @ np.zeros((2001, 0))).shape == (2001, 0)
+
+
+C2 = ROOT / "data" / "log_n65536_leaf128_tol1e-9.npz"
+
+
+@pytest.mark.skipif(not C2.exists(), reason="n=65536 generators not present (bench.py writes them)")
+def test_c2_full_size(hip, oracle):
+    """Config C2 at full size: parity with the oracle plus size-independent properties."""
+    d = np.load(C2)
+    nodes, blob = d["nodes"], d["blob"]
+    n = int(nodes["n"][0])
+    A = hip.HipHssMatrix((nodes, blob))
+    rng = np.random.default_rng(16)
+    X = rng.standard_normal((n, 8))
+    Y = A @ X
+    assert rel(Y, oracle.matmul(nodes, blob, X)) < 1e-13
+    # linearity
+    Y2 = A @ (3.0 * X[:, :2] - X[:, 2:4])
+    assert rel(Y2, 3.0 * Y[:, :2] - Y[:, 2:4]) < 1e-13
+    F = A.ulvfactor("fp64")
+    B = rng.standard_normal((n, 4))
+    Z = F.solve(B)
+    R = oracle.matmul(nodes, blob, Z) - B
+    assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)) < 1e-11
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(1):
+        Zo = oracle.ulvfactsolve(nodes, blob, B)
+    assert rel(Z, Zo) < 1e-9

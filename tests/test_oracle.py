@@ -61,7 +61,8 @@ def test_oracle_c1_closed_form(oracle, c1):
     assert rel(oracle.matmul(c1["nodes"], c1["blob"], x), F @ x) < 1e-14
     z = oracle.ulvfactsolve(c1["nodes"], c1["blob"], x)
     assert rel(z, np.linalg.solve(F, x)) < 1e-12  # cond(F) ~ 3.1e3
-    assert rel(F @ z, x) < 1e-13
+    # backward error of a stable solve: ||F z - x|| / (||F|| ||z||) ~ eps
+    assert np.linalg.norm(F @ z - x) / (np.linalg.norm(F, 2) * np.linalg.norm(z)) < 1e-14
 
 
 def test_oracle_lowrank_known_answer(oracle):
