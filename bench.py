@@ -264,6 +264,11 @@ def main():
     tot_m, byk_m = summarize(recs_mv, prof_reps)
     recs_mx = profile_launches(hip, solve_mixed, 1)
     _, byk_x = summarize(recs_mx, 1)
+    fact_recs = profile_launches(hip, lambda: A.ulvfactor("fp64").close(), 1)
+    _, byk_f = summarize(fact_recs, 1)
+    fact_levels = {f"{r['kernel']}@{r['level']}": round(r["ms"], 4) for r in fact_recs}
+    log(f"[bench] fp64 factor per kernel (ms): { {k: round(v, 4) for k, v in byk_f.items()} }")
+    log(f"[bench] fp64 factor per launch (ms): {fact_levels}")
     log(f"[bench] fp64 solve per kernel (ms/solve): { {k: round(v, 4) for k, v in byk_s.items()} }")
     log(f"[bench] matvec per kernel (ms): { {k: round(v, 4) for k, v in byk_m.items()} }")
     log(f"[bench] mixed solve per kernel (ms/solve): { {k: round(v, 4) for k, v in byk_x.items()} }")
@@ -362,7 +367,8 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes": int(dbytes), "avg_launch_ms": round(avg_ms, 5)},
-        "per_kernel_ms": {"fp64_solve": {k: round(v, 4) for k, v in byk_s.items()},
+        "per_kernel_ms": {"fp64_factor": {k: round(v, 4) for k, v in byk_f.items()},
+                          "fp64_solve": {k: round(v, 4) for k, v in byk_s.items()},
                           "mixed_solve": {k: round(v, 4) for k, v in byk_x.items()},
                           "matvec": {k: round(v, 4) for k, v in byk_m.items()}},
         "parity": parity,

@@ -38,7 +38,7 @@ struct FNode {
     int32_t nblk;      // ceil(nk / NB)
     int32_t pad;
     // factor storage (elements of the factor type), -1 = none
-    int64_t Yql, Tql, Ylq, taulq, Tlq, L1, L1inv, L2a, Vt, P12, P21, LU, piv;
+    int64_t Yql, Tql, Ylq, taulq, Tlq, L1, L1inv, L2a, Vt, P12, P21, LU, LUinv, piv;
     // factorization-time workspace (reduced D/U/V passed to the parent, merged D scratch)
     int64_t rwD, rwU, rwV, rwM;
     // solve workspace rows (times nrhs): reduced b (mr rows), u (kv rows), z (n rows)
@@ -89,6 +89,17 @@ struct SInst {
     TV* wu;
     TV* wz;
     int32_t nrhs, pad;
+};
+
+// Per-request vectors of a batched refinement step.
+struct BatchVec {
+    const double* b;  // right-hand side (n x nrhs, ldb)
+    double* x;        // solution (ldx)
+    double* r;        // residual (n x nrhs, ld n)
+    double* d;        // correction (n x nrhs, ld n)
+    int64_t ldb, ldx, n;
+    int32_t nrhs;     // 0 = inactive (converged)
+    int32_t pad;
 };
 
 }  // namespace hss

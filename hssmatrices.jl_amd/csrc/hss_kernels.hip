@@ -65,36 +65,81 @@ __device__ __forceinline__ void larfg(T alpha, T xnorm2, T& beta, T& tau, T& sca
     scale = T(1) / (alpha - beta);
 }
 
-// Y[i + j*ldy] = alpha * sum_l opA(i,l) X[l + j*ldx] (+ Y if acc), i<M, j<N.
-// A lives in global memory (column-major, lda), X and Y anywhere the calling
-// block can address.  All threads of the block participate (no barrier inside).
-template <bool TA, typename TA_t, typename T>
-__device__ void blk_gemm(int M, int N, int K, const TA_t* __restrict__ A, int64_t lda, const T* X, int64_t ldx,
-                         T* Y, int64_t ldy, T alpha, bool acc) {
-    const int total = M * N;
-    for (int o = threadIdx.x; o < total; o += blockDim.x) {
-        const int i = o % M, j = o / M;
-        T s0 = T(0), s1 = T(0);
-        const T* x = X + (int64_t)j * ldx;
-        int l = 0;
-        if (TA) {
-            const TA_t* a = A + (int64_t)i * lda;
-            for (; l + 1 < K; l += 2) {
-                s0 += (T)a[l] * x[l];
-                s1 += (T)a[l + 1] * x[l + 1];
+// ---------------------------------------------------------------------------
+// workgroup GEMM on small blocks
+// ---------------------------------------------------------------------------
+// Right-hand-side tiles live in LDS row-major: element (r, j) at X[r * NTP + j],
+// NTP = RHS_TILE, so the CB columns a thread needs are contiguous.
+constexpr int NTP = RHS_TILE;
+
+// out(i, j, v) for v = sum_l opA(i, l) X(l, j), i < M, j < N.
+//   op(A) = A (M x K) when !TA, A^T with A stored K x M when TA; A column-major
+//   in global memory with leading dimension lda.
+// Threads form a (TR x TC) grid, TC = ceil(N / CB) (column groups fastest), each
+// owning an RB x CB register tile; the TC lanes of one thread-row read the same
+// A element in the same instruction, so A streams from memory once per
+// workgroup.  Outputs with j >= N are computed from padding and never stored.
+template <int RB, int CB, bool TA, typename TAe, typename T, typename Store>
+__device__ __forceinline__ void wg_gemm(int M, int N, int K, const TAe* __restrict__ A, int64_t lda, const T* X,
+                                        Store store) {
+    const int TC = (N + CB - 1) / CB;
+    const int TR = blockDim.x / TC;
+    const int tc = threadIdx.x % TC, tr = threadIdx.x / TC;
+    if (tr >= TR || M <= 0 || N <= 0) return;
+    const int j0 = tc * CB;
+    for (int i0 = tr * RB; i0 < M; i0 += TR * RB) {
+        T acc[RB][CB];
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int c = 0; c < CB; ++c) acc[r][c] = T(0);
+#pragma unroll 4
+        for (int l = 0; l < K; ++l) {
+            T a[RB];
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                const int i = i0 + r;
+                a[r] = (i < M) ? (T)(TA ? A[l + (int64_t)i * lda] : A[i + (int64_t)l * lda]) : T(0);
             }
-            if (l < K) s0 += (T)a[l] * x[l];
-        } else {
-            const TA_t* a = A + i;
-            for (; l + 1 < K; l += 2) {
-                s0 += (T)a[(int64_t)l * lda] * x[l];
-                s1 += (T)a[(int64_t)(l + 1) * lda] * x[l + 1];
-            }
-            if (l < K) s0 += (T)a[(int64_t)l * lda] * x[l];
+            const T* xr = X + l * NTP + j0;
+            T x[CB];
+#pragma unroll
+            for (int c = 0; c < CB; ++c) x[c] = xr[c];
+#pragma unroll
+            for (int r = 0; r < RB; ++r)
+#pragma unroll
+                for (int c = 0; c < CB; ++c) acc[r][c] += a[r] * x[c];
         }
-        const T s = alpha * (s0 + s1);
-        T* y = Y + i + (int64_t)j * ldy;
-        *y = acc ? *y + s : s;
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int c = 0; c < CB; ++c)
+                if (i0 + r < M && j0 + c < N) store(i0 + r, j0 + c, acc[r][c]);
+    }
+}
+
+// Default tile shapes: 2 x 8 per thread (256 threads cover 128 x 32).
+template <bool TA, typename TAe, typename T, typename Store>
+__device__ __forceinline__ void gemm(int M, int N, int K, const TAe* __restrict__ A, int64_t lda, const T* X,
+                                     Store store) {
+    wg_gemm<2, 8, TA>(M, N, K, A, lda, X, store);
+}
+
+// global column-major (ld) -> LDS row-major rows [r0, r0 + rows)
+template <typename Tsrc, typename T>
+__device__ __forceinline__ void stage(T* dst, int r0, const Tsrc* src, int64_t ld, int rows, int nt) {
+    for (int e = threadIdx.x; e < rows * nt; e += blockDim.x) {
+        const int r = e % rows, j = e / rows;
+        dst[(r0 + r) * NTP + j] = (T)src[r + (int64_t)j * ld];
+    }
+}
+
+// LDS row-major -> global column-major
+template <typename T, typename Tdst>
+__device__ __forceinline__ void unstage(Tdst* dst, int64_t ld, const T* src, int r0, int rows, int nt) {
+    for (int e = threadIdx.x; e < rows * nt; e += blockDim.x) {
+        const int r = e % rows, j = e / rows;
+        dst[r + (int64_t)j * ld] = (Tdst)src[(r0 + r) * NTP + j];
     }
 }
 
@@ -113,23 +158,26 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_mv_up(const MvInst* __restric
     extern __shared__ double lds_mv[];
     double* Zt = I.Z + nd.zrow * I.nrhs + (int64_t)j0 * nd.kv;  // kv x nt, ld kv
     if (nd.is_leaf) {
-        // stage B_seg (n x nt) in LDS
-        double* xs = lds_mv;
-        const double* Bs = I.B + nd.col_off + (int64_t)j0 * I.ldb;
-        for (int e = threadIdx.x; e < nd.n * nt; e += blockDim.x) {
-            const int r = e % nd.n, j = e / nd.n;
-            xs[e] = Bs[r + (int64_t)j * I.ldb];
-        }
+        double* xs = lds_mv;  // n x NTP
+        stage(xs, 0, I.B + nd.col_off + (int64_t)j0 * I.ldb, I.ldb, nd.n, nt);
         __syncthreads();
-        blk_gemm<true>(nd.kv, nt, nd.n, I.blob + nd.V, nd.n, xs, nd.n, Zt, nd.kv, 1.0, false);
+        const int kv = nd.kv;
+        gemm<true>(kv, nt, nd.n, I.blob + nd.V, nd.n, xs,
+                   [&](int i, int j, double v) { Zt[i + (int64_t)j * kv] = v; });
     } else {
         const DevNode L = I.nodes[nd.left];
         const DevNode R = I.nodes[nd.right];
-        const double* ZL = I.Z + L.zrow * I.nrhs + (int64_t)j0 * L.kv;
-        const double* ZR = I.Z + R.zrow * I.nrhs + (int64_t)j0 * R.kv;
-        blk_gemm<true>(nd.kv, nt, L.kv, I.blob + L.W, L.kv, ZL, L.kv, Zt, nd.kv, 1.0, false);
+        double* zs = lds_mv;                // (kv_L + kv_R) x NTP
+        double* acc = lds_mv + (L.kv + R.kv) * NTP;  // kv x NTP
+        stage(zs, 0, I.Z + L.zrow * I.nrhs + (int64_t)j0 * L.kv, L.kv, L.kv, nt);
+        stage(zs, L.kv, I.Z + R.zrow * I.nrhs + (int64_t)j0 * R.kv, R.kv, R.kv, nt);
         __syncthreads();
-        blk_gemm<true>(nd.kv, nt, R.kv, I.blob + R.W, R.kv, ZR, R.kv, Zt, nd.kv, 1.0, true);
+        gemm<true>(nd.kv, nt, L.kv, I.blob + L.W, L.kv, zs,
+                   [&](int i, int j, double v) { acc[i * NTP + j] = v; });
+        __syncthreads();
+        const int kv = nd.kv;
+        gemm<true>(nd.kv, nt, R.kv, I.blob + R.W, R.kv, zs + L.kv * NTP,
+                   [&](int i, int j, double v) { Zt[i + (int64_t)j * kv] = acc[i * NTP + j] + v; });
     }
 }
 
@@ -144,20 +192,34 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_mv_down(const MvInst* __restr
     const DevNode p = I.nodes[wi.node];
     const DevNode L = I.nodes[p.left];
     const DevNode R = I.nodes[p.right];
-    const bool has_f = p.parent >= 0;
-    const double* ZL = I.Z + L.zrow * I.nrhs + (int64_t)j0 * L.kv;
-    const double* ZR = I.Z + R.zrow * I.nrhs + (int64_t)j0 * R.kv;
+    const bool has_f = p.parent >= 0 && p.ku > 0;
+    extern __shared__ double lds_mv[];
+    double* zl = lds_mv;                    // kv_L x NTP
+    double* zr = zl + L.kv * NTP;           // kv_R x NTP
+    double* fp = zr + R.kv * NTP;           // ku_p x NTP
+    double* al = fp + p.ku * NTP;           // ku_L x NTP
+    double* ar = al + L.ku * NTP;           // ku_R x NTP
+    stage(zl, 0, I.Z + L.zrow * I.nrhs + (int64_t)j0 * L.kv, L.kv, L.kv, nt);
+    stage(zr, 0, I.Z + R.zrow * I.nrhs + (int64_t)j0 * R.kv, R.kv, R.kv, nt);
+    if (has_f) stage(fp, 0, I.F + p.frow * I.nrhs + (int64_t)j0 * p.ku, p.ku, p.ku, nt);
+    __syncthreads();
     double* FL = I.F + L.frow * I.nrhs + (int64_t)j0 * L.ku;
     double* FR = I.F + R.frow * I.nrhs + (int64_t)j0 * R.ku;
-    // F1 = B12 * Z2 ; F2 = B21 * Z1
-    blk_gemm<false>(L.ku, nt, R.kv, I.blob + p.B12, L.ku, ZR, R.kv, FL, L.ku, 1.0, false);
-    blk_gemm<false>(R.ku, nt, L.kv, I.blob + p.B21, R.ku, ZL, L.kv, FR, R.ku, 1.0, false);
+    const int kul = L.ku, kur = R.ku;
     if (has_f) {
-        const double* Fp = I.F + p.frow * I.nrhs + (int64_t)j0 * p.ku;
+        // a = R_c F (stored), then F_c = B Z_sibling + a
+        gemm<false>(L.ku, nt, p.ku, I.blob + L.R, L.ku, fp, [&](int i, int j, double v) { al[i * NTP + j] = v; });
+        gemm<false>(R.ku, nt, p.ku, I.blob + R.R, R.ku, fp, [&](int i, int j, double v) { ar[i * NTP + j] = v; });
         __syncthreads();
-        // F1 += R1 * F ; F2 += R2 * F
-        blk_gemm<false>(L.ku, nt, p.ku, I.blob + L.R, L.ku, Fp, p.ku, FL, L.ku, 1.0, true);
-        blk_gemm<false>(R.ku, nt, p.ku, I.blob + R.R, R.ku, Fp, p.ku, FR, R.ku, 1.0, true);
+        gemm<false>(L.ku, nt, R.kv, I.blob + p.B12, L.ku, zr,
+                    [&](int i, int j, double v) { FL[i + (int64_t)j * kul] = v + al[i * NTP + j]; });
+        gemm<false>(R.ku, nt, L.kv, I.blob + p.B21, R.ku, zl,
+                    [&](int i, int j, double v) { FR[i + (int64_t)j * kur] = v + ar[i * NTP + j]; });
+    } else {
+        gemm<false>(L.ku, nt, R.kv, I.blob + p.B12, L.ku, zr,
+                    [&](int i, int j, double v) { FL[i + (int64_t)j * kul] = v; });
+        gemm<false>(R.ku, nt, L.kv, I.blob + p.B21, R.ku, zl,
+                    [&](int i, int j, double v) { FR[i + (int64_t)j * kur] = v; });
     }
 }
 
@@ -172,61 +234,32 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_mv_leaf(const MvInst* __restr
     const DevNode nd = I.nodes[wi.node];
     const bool has_f = nd.parent >= 0 && nd.ku > 0;
     extern __shared__ double lds_mv[];
-    double* xs = lds_mv;                         // n x nt
-    double* fs = lds_mv + nd.n * nt;             // ku x nt
-    const double* Bs = I.B + nd.col_off + (int64_t)j0 * I.ldb;
-    for (int e = threadIdx.x; e < nd.n * nt; e += blockDim.x) {
-        const int r = e % nd.n, j = e / nd.n;
-        xs[e] = Bs[r + (int64_t)j * I.ldb];
-    }
-    if (has_f) {
-        const double* Fn = I.F + nd.frow * I.nrhs + (int64_t)j0 * nd.ku;
-        for (int e = threadIdx.x; e < nd.ku * nt; e += blockDim.x) fs[e] = Fn[e];
-    }
+    double* xs = lds_mv;              // n x NTP
+    double* ds = xs + nd.n * NTP;     // m x NTP : D B_seg
+    stage(xs, 0, I.B + nd.col_off + (int64_t)j0 * I.ldb, I.ldb, nd.n, nt);
+    double* fs = ds + nd.m * NTP;     // ku x NTP
+    if (has_f) stage(fs, 0, I.F + nd.frow * I.nrhs + (int64_t)j0 * nd.ku, nd.ku, nd.ku, nt);
     __syncthreads();
-    const double* D = I.blob + nd.D;
-    const double* U = I.blob + (nd.U < 0 ? 0 : nd.U);
     double* Cs = I.C + nd.row_off + (int64_t)j0 * I.ldc;
+    const int64_t ldc = I.ldc;
     const double alpha = I.alpha, beta = I.beta;
-    const int M = nd.m, N = nd.n;
-    for (int o = threadIdx.x; o < M * nt; o += blockDim.x) {
-        const int i = o % M, j = o / M;
-        double s0 = 0.0, s1 = 0.0;
-        const double* x = xs + j * N;
-        int l = 0;
-        for (; l + 1 < N; l += 2) {
-            s0 += D[i + (int64_t)l * M] * x[l];
-            s1 += D[i + (int64_t)(l + 1) * M] * x[l + 1];
-        }
-        if (l < N) s0 += D[i + (int64_t)l * M] * x[l];
-        double s = s0 + s1;
-        if (has_f) {
-            const double* fj = fs + j * nd.ku;
-            double t = 0.0;
-            for (int q = 0; q < nd.ku; ++q) t += U[i + (int64_t)q * M] * fj[q];
-            s += t;
-        }
-        double* c = Cs + i + (int64_t)j * I.ldc;
-        *c = (beta == 0.0) ? alpha * s : alpha * s + beta * (*c);
+    auto fin = [&](int i, int j, double v) {
+        double* c = Cs + i + j * ldc;
+        *c = (beta == 0.0) ? alpha * v : alpha * v + beta * (*c);
+    };
+    if (has_f) {
+        gemm<false>(nd.m, nt, nd.n, I.blob + nd.D, nd.m, xs, [&](int i, int j, double v) { ds[i * NTP + j] = v; });
+        __syncthreads();
+        gemm<false>(nd.m, nt, nd.ku, I.blob + nd.U, nd.m, fs,
+                    [&](int i, int j, double v) { fin(i, j, ds[i * NTP + j] + v); });
+    } else {
+        gemm<false>(nd.m, nt, nd.n, I.blob + nd.D, nd.m, xs, fin);
     }
 }
 
 // ---------------------------------------------------------------------------
 // ULV factorization
 // ---------------------------------------------------------------------------
-// LDS layout of k_ulv_factor (elements of T):
-//   Us    m x ku    row generators, QL-factored in place
-//   vb    2 x n     current LQ reflector (double buffered)
-//   red   nwaves x n partial column sums of the QL application
-//   tot   n         column sums
-//   sc    8         scalars
-//   taus  max(k, nk)
-//   gram  k x k     (QL T-factor scratch)
-template <typename T>
-struct FactLds {
-    T *Us, *vb, *red, *tot, *sc, *taus, *gram;
-};
-
 template <typename T>
 __device__ __forceinline__ T gen(const double* blob, int64_t off, int64_t idx) {
     return (T)blob[off + idx];
@@ -295,6 +328,11 @@ __device__ void merge_products(const FInst<T>& I, const DevNode& nd, const FNode
     }
 }
 
+// Padded index of column c in the LQ reflector buffer: the CPT-column chunks of
+// the S threads of a row start in different LDS banks.
+template <int CPT>
+__device__ __forceinline__ int vpad(int c) { return c + (c / CPT) * 2; }
+
 template <typename T, int CPT>
 __global__ __launch_bounds__(FACT_THREADS) void k_ulv_factor(const FInst<T>* __restrict__ insts,
                                                              const WorkItem* __restrict__ work) {
@@ -309,34 +347,26 @@ __global__ __launch_bounds__(FACT_THREADS) void k_ulv_factor(const FInst<T>* __r
     const int c0 = s * CPT;
     const int R = m + kv;
     const bool row_on = r < R;
+    const int wave = tid / 64, lane = tid % 64;
     const int nwaves = blockDim.x / 64;
 
+    // LDS: Us (m x ku) | G (k x k) | Tq (k x k) | M (k rows of vstride) | vb (2 x vstride) | taus | sc
     extern __shared__ unsigned char lds_raw[];
-    T* lds = reinterpret_cast<T*>(lds_raw);
-    FactLds<T> L;
-    L.Us = lds;
-    L.vb = L.Us + (size_t)m * ku;
-    L.red = L.vb + 2 * n;
-    L.tot = L.red + (size_t)nwaves * n;
-    L.sc = L.tot + n;
-    L.taus = L.sc + 8;
-    L.gram = L.taus + (k > nk ? k : nk) + 1;
+    const int vstride = vpad<CPT>(n) + 2;
+    T* Us = reinterpret_cast<T*>(lds_raw);
+    T* G = Us + (size_t)m * ku;
+    T* Tq = G + (size_t)k * k;
+    T* Mq = Tq + (size_t)k * k;  // row j of M at Mq + j * vstride, column c at vpad(c)
+    T* vbuf = Mq + (size_t)k * vstride;
+    T* taus = vbuf + 2 * vstride;
+    T* sc = taus + (k > nk ? k : nk) + 1;
 
-    // ---- phase 0: assemble W = [D; V'] (rows in registers) and U (LDS) ----
-    T w[CPT];
-#pragma unroll
-    for (int q = 0; q < CPT; ++q) w[q] = T(0);
-
+    // ---- phase 0: U (LDS); the source of W = [D; V'] ----
+    //   leaf:   D and V from the generator blob (column-major)
+    //   branch: merged rows written row-major into rwM (ulvfactor.jl:74-79)
+    const T* Wm = nullptr;
     if (nd.is_leaf) {
-        if (row_on) {
-#pragma unroll
-            for (int q = 0; q < CPT; ++q) {
-                const int c = c0 + q;
-                if (c < n) w[q] = (r < m) ? gen<T>(I.blob, nd.D, r + (int64_t)c * m)
-                                          : gen<T>(I.blob, nd.V, c + (int64_t)(r - m) * n);
-            }
-        }
-        for (int e = tid; e < m * ku; e += blockDim.x) L.Us[e] = gen<T>(I.blob, nd.U, e);
+        for (int e = tid; e < m * ku; e += blockDim.x) Us[e] = gen<T>(I.blob, nd.U, e);
     } else {
         const ChildRed<T> a = child_red(I, nd.left);
         const ChildRed<T> b = child_red(I, nd.right);
@@ -347,8 +377,7 @@ __global__ __launch_bounds__(FACT_THREADS) void k_ulv_factor(const FInst<T>* __r
         __syncthreads();
         const T* P12 = I.fac + f.P12;
         const T* P21 = I.fac + f.P21;
-        // merged rows of W = [D; V'] into the row-major scratch rwM ((m + kv) x n)
-        T* Wm = I.rw + f.rwM;
+        T* Wmw = I.rw + f.rwM;
         for (int e = tid; e < R * n; e += blockDim.x) {
             const int rr = e / n, c = e % n;
             T val;
@@ -368,7 +397,7 @@ __global__ __launch_bounds__(FACT_THREADS) void k_ulv_factor(const FInst<T>* __r
                 }
                 val = sum;
             }
-            Wm[e] = val;
+            Wmw[e] = val;
         }
         // U = [U1 R1; U2 R2]  (ulvfactor.jl:78)
         for (int e = tid; e < m * ku; e += blockDim.x) {
@@ -382,43 +411,33 @@ __global__ __launch_bounds__(FACT_THREADS) void k_ulv_factor(const FInst<T>* __r
                 for (int l = 0; l < b.ku; ++l)
                     sum += b.U[r2 + (int64_t)l * b.mr] * gen<T>(I.blob, dr.R, l + (int64_t)j * b.ku);
             }
-            L.Us[e] = sum;
+            Us[e] = sum;
         }
+        Wm = Wmw;
         __threadfence_block();
-        __syncthreads();
-        if (row_on) {
-            const T* wr = Wm + (int64_t)r * n;
-#pragma unroll
-            for (int q = 0; q < CPT; ++q)
-                if (c0 + q < n) w[q] = wr[c0 + q];
-        }
     }
     __syncthreads();
+    // entry (rr, c) of W = [D; V']
+    auto Wsrc = [&](int rr, int c) -> T {
+        if (Wm) return Wm[(int64_t)rr * n + c];
+        return rr < m ? gen<T>(I.blob, nd.D, rr + (int64_t)c * m) : gen<T>(I.blob, nd.V, c + (int64_t)(rr - m) * n);
+    };
 
     if (f.pass) {
         // k >= m: hand D, U, V to the parent unchanged (ulvfactor.jl:32-36)
         T* Dr = I.rw + f.rwD;
         T* Vr = I.rw + f.rwV;
-        if (row_on) {
-#pragma unroll
-            for (int q = 0; q < CPT; ++q) {
-                const int c = c0 + q;
-                if (c < n) {
-                    if (r < m) Dr[r + (int64_t)c * m] = w[q];
-                    else Vr[c + (int64_t)(r - m) * n] = w[q];
-                }
-            }
-        }
+        for (int e = tid; e < m * n; e += blockDim.x) Dr[e] = Wsrc(e % m, e / m);
+        for (int e = tid; e < n * kv; e += blockDim.x) Vr[e] = Wsrc(m + e / n, e % n);
         T* Ur = I.rw + f.rwU;
-        for (int e = tid; e < m * ku; e += blockDim.x) Ur[e] = L.Us[e];
+        for (int e = tid; e < m * ku; e += blockDim.x) Ur[e] = Us[e];
         return;
     }
 
     // ---- phase 1: QL of U (m x k) in LDS, LAPACK dgeqlf semantics (ulvfactor.jl:39) ----
-    const int wave = tid / 64, lane = tid % 64;
     for (int jj = k - 1; jj >= 0; --jj) {
         const int p = m - k + jj;
-        T* col = L.Us + (int64_t)jj * m;
+        T* col = Us + (int64_t)jj * m;
         if (wave == 0) {
             T part = T(0);
             for (int i = lane; i < p; i += 64) part += col[i] * col[i];
@@ -426,21 +445,21 @@ __global__ __launch_bounds__(FACT_THREADS) void k_ulv_factor(const FInst<T>* __r
             if (lane == 0) {
                 T beta, tau, scale;
                 larfg(col[p], part, beta, tau, scale);
-                L.sc[0] = beta;
-                L.sc[1] = tau;
-                L.sc[2] = scale;
-                L.taus[jj] = tau;
+                sc[0] = beta;
+                sc[1] = tau;
+                sc[2] = scale;
+                taus[jj] = tau;
             }
         }
         __syncthreads();
-        const T tau = L.sc[1], scale = L.sc[2];
+        const T tau = sc[1], scale = sc[2];
         if (tau != T(0))
             for (int i = tid; i < p; i += blockDim.x) col[i] *= scale;
         __syncthreads();
         // apply H = I - tau v v' to U[0:p+1, 0:jj], v = [col[0:p]; 1]
         if (tau != T(0)) {
             for (int c = wave; c < jj; c += nwaves) {
-                T* uc = L.Us + (int64_t)c * m;
+                T* uc = Us + (int64_t)c * m;
                 T part = T(0);
                 for (int i = lane; i < p; i += 64) part += col[i] * uc[i];
                 part = wave_reduce(part) + uc[p];
@@ -450,88 +469,131 @@ __global__ __launch_bounds__(FACT_THREADS) void k_ulv_factor(const FInst<T>* __r
             }
         }
         __syncthreads();
-        if (tid == 0) col[p] = L.sc[0];
+        if (tid == 0) col[p] = sc[0];
         __syncthreads();
     }
-
-    // Y_ql (explicit, m x k), T_ql (k x k), reduced U = tril(U[m-k:m, :]) for the parent
+    // reflector j: y_j[i] = Us[i, j] (i < p_j), 1 (i = p_j), 0 below
+    auto yql = [&](int i, int j) -> T {
+        const int p = m - k + j;
+        return i < p ? Us[i + (int64_t)j * m] : (i == p ? T(1) : T(0));
+    };
     {
         T* Y = I.fac + f.Yql;
-        for (int e = tid; e < m * k; e += blockDim.x) {
-            const int i = e % m, j = e / m;
-            const int p = m - k + j;
-            Y[e] = (i < p) ? L.Us[e] : (i == p ? T(1) : T(0));
-        }
-        T* Ur = I.rw + f.rwU;  // k x ku (ku == k here)
+        for (int e = tid; e < m * k; e += blockDim.x) Y[e] = yql(e % m, e / m);
+        T* Ur = I.rw + f.rwU;  // reduced U = tril(U[m-k:m, :]), k x ku (ku == k here)
         for (int e = tid; e < k * ku; e += blockDim.x) {
             const int i = e % k, j = e / k;
-            Ur[e] = (i >= j) ? L.Us[(m - k + i) + (int64_t)j * m] : T(0);
+            Ur[e] = (i >= j) ? Us[(m - k + i) + (int64_t)j * m] : T(0);
         }
-        // Gram G = Y'Y (k x k) for the T factor (dlarft, forward columnwise)
+        // Gram G[a][b] = y_a . y_b for a < b (y_a has its unit above y_b's)
         for (int e = tid; e < k * k; e += blockDim.x) {
             const int a = e % k, bcol = e / k;
+            T sum = T(0);
             if (a < bcol) {
-                // y_a has its unit at p_a < p_b; y_b is zero below p_b
-                const int pa = m - k + a, pb = m - k + bcol;
-                const T* ya = L.Us + (int64_t)a * m;
-                const T* yb = L.Us + (int64_t)bcol * m;
-                T sum = yb[pa];  // ya[pa] = 1
+                const int pa = m - k + a;
+                const T* ya = Us + (int64_t)a * m;
+                const T* yb = Us + (int64_t)bcol * m;
+                sum = yb[pa];
                 for (int i = 0; i < pa; ++i) sum += ya[i] * yb[i];
-                (void)pb;
-                L.gram[e] = sum;
+            }
+            G[e] = sum;
+        }
+    }
+    __syncthreads();
+    // padding columns of M (n <= c < S * CPT) must read as zero in phase 3
+    for (int e = tid; e < k * vstride; e += blockDim.x) Mq[e] = T(0);
+    // T factor of H(1)...H(k) = I - Y T Y' (dlarft, forward columnwise); row a of T
+    // depends only on row a itself, so thread a builds it alone
+    for (int a = tid; a < k; a += blockDim.x) {
+        for (int q = 0; q < k; ++q) Tq[a + (int64_t)q * k] = T(0);
+        Tq[a + (int64_t)a * k] = taus[a];
+        for (int i = a + 1; i < k; ++i) {
+            T sum = T(0);
+            for (int bb = a; bb < i; ++bb) sum += Tq[a + (int64_t)bb * k] * G[bb + (int64_t)i * k];
+            Tq[a + (int64_t)i * k] = -taus[i] * sum;
+        }
+    }
+    __syncthreads();
+    {
+        T* Tg = I.fac + f.Tql;
+        for (int e = tid; e < k * k; e += blockDim.x) Tg[e] = Tq[e];
+    }
+
+    // ---- phase 2: M = T (Y' D), so that Q_ql' D = D - Y M  (ulvfactor.jl:41) ----
+    // thread (column c, row group g): rows g, g+16, ... ; reduction over the 16 lanes of g
+    {
+        constexpr int RG = 16, MAXR = 16;
+        const int g = tid % RG;
+        const int cols_per_pass = blockDim.x / RG;
+        for (int cb = tid / RG; cb < ((n + cols_per_pass - 1) / cols_per_pass) * cols_per_pass; cb += cols_per_pass) {
+            const int c = cb;
+            T d[MAXR];
+#pragma unroll
+            for (int q = 0; q < MAXR; ++q) {
+                const int i = g + RG * q;
+                d[q] = (c < n && i < m) ? Wsrc(i, c) : T(0);
+            }
+            for (int j0 = 0; j0 < k; j0 += 8) {
+                T acc[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) acc[t] = T(0);
+#pragma unroll
+                for (int q = 0; q < MAXR; ++q) {
+                    const int i = g + RG * q;
+                    if (i < m) {
+#pragma unroll
+                        for (int t = 0; t < 8; ++t)
+                            if (j0 + t < k) acc[t] += yql(i, j0 + t) * d[q];
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    T v = acc[t];
+                    v += shfl_xor(v, 1);
+                    v += shfl_xor(v, 2);
+                    v += shfl_xor(v, 4);
+                    v += shfl_xor(v, 8);
+                    if (g == 0 && c < n && j0 + t < k) Mq[(j0 + t) * vstride + vpad<CPT>(c)] = v;
+                }
             }
         }
-        __syncthreads();
-        if (tid == 0) {
-            T* Tq = I.fac + f.Tql;
-            for (int e = 0; e < k * k; ++e) Tq[e] = T(0);
-            for (int i = 0; i < k; ++i) {
-                const T tau = L.taus[i];
-                Tq[i + (int64_t)i * k] = tau;
-                // T[0:i, i] = -tau T[0:i,0:i] G[0:i, i]
-                for (int a = 0; a < i; ++a) {
-                    T sum = T(0);
-                    for (int bb = a; bb < i; ++bb) sum += Tq[a + (int64_t)bb * k] * L.gram[bb + (int64_t)i * k];
-                    Tq[a + (int64_t)i * k] = -tau * sum;
+    }
+    __syncthreads();
+    // M <- T M in place (T upper triangular: ascending j reads only rows >= j)
+    for (int c = tid; c < n; c += blockDim.x) {
+        T* mc = Mq + vpad<CPT>(c);
+        for (int j = 0; j < k; ++j) {
+            T sum = T(0);
+            for (int l = j; l < k; ++l) sum += Tq[j + (int64_t)l * k] * mc[l * vstride];
+            mc[j * vstride] = sum;
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 3: rows of W in registers: D rows updated by -Y M, V' rows as is ----
+    T w[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) w[q] = T(0);
+    if (row_on) {
+#pragma unroll
+        for (int q = 0; q < CPT; ++q)
+            if (c0 + q < n) w[q] = Wsrc(r, c0 + q);
+        if (r < m) {
+            for (int j = 0; j < k; ++j) {
+                const T y = yql(r, j);
+                if (y != T(0)) {
+                    const T* mj = Mq + j * vstride + vpad<CPT>(c0);
+#pragma unroll
+                    for (int q = 0; q < CPT; ++q) w[q] -= y * mj[q];
                 }
             }
         }
     }
 
-    // ---- phase 2: D <- Q_ql' D = H(1)...H(k) D  (ulvfactor.jl:41) ----
-    for (int jj = k - 1; jj >= 0; --jj) {
-        const int p = m - k + jj;
-        const T tau = L.taus[jj];
-        if (tau == T(0)) continue;  // uniform across the block
-        T vr = T(0);
-        if (r < m) vr = (r < p) ? L.Us[r + (int64_t)jj * m] : (r == p ? T(1) : T(0));
-        // column sums of vr * D[r, c] over the rows of this wave (lanes with the same s)
-#pragma unroll
-        for (int q = 0; q < CPT; ++q) {
-            T pq = vr * w[q];
-            for (int o = S; o < 64; o <<= 1) pq += shfl_xor(pq, o);
-            if (lane < S && c0 + q < n) L.red[(int64_t)wave * n + c0 + q] = pq;
-        }
-        __syncthreads();
-        for (int c = tid; c < n; c += blockDim.x) {
-            T sum = T(0);
-            for (int wv = 0; wv < nwaves; ++wv) sum += L.red[(int64_t)wv * n + c];
-            L.tot[c] = sum;
-        }
-        __syncthreads();
-        if (vr != T(0)) {
-            const T t = tau * vr;
-#pragma unroll
-            for (int q = 0; q < CPT; ++q)
-                if (c0 + q < n) w[q] -= t * L.tot[c0 + q];
-        }
-        __syncthreads();
-    }
-
-    // ---- phase 3: LQ of the first nk rows of [D; V'], reflectors applied to all
+    // ---- phase 4: LQ of the first nk rows of [D; V'], reflectors applied to all
     //      rows below (L2 = D_bot Q', V <- Q V; ulvfactor.jl:44,47,50) ----
     for (int i = 0; i < nk; ++i) {
-        T* vb = L.vb + (i & 1) * n;
+        T* vb = vbuf + (i & 1) * vstride;
         if (r == i) {
             T part = T(0), alpha_p = T(0);
 #pragma unroll
@@ -544,43 +606,41 @@ __global__ __launch_bounds__(FACT_THREADS) void k_ulv_factor(const FInst<T>* __r
             const T alpha = row_reduce(alpha_p, S);
             T beta, tau, scale;
             larfg(alpha, part, beta, tau, scale);
+            // v = [0 .. 0, 1, x * scale]; row i keeps L (c < i), beta (c = i) and v (c > i).
+            // tau = 0 only when x = 0, and then x * scale = 0 = x as well.
+            T* vs = vb + vpad<CPT>(c0);
 #pragma unroll
             for (int q = 0; q < CPT; ++q) {
                 const int c = c0 + q;
-                if (c < n) {
-                    T v;
-                    if (c < i) v = T(0);
-                    else if (c == i) { v = T(1); w[q] = beta; }
-                    else { v = (tau != T(0)) ? w[q] * scale : T(0); if (tau != T(0)) w[q] = v; }
-                    vb[c] = v;
-                }
+                const T v = (c == i) ? T(1) : ((c > i) ? w[q] * scale : T(0));
+                w[q] = (c == i) ? beta : ((c > i) ? v : w[q]);
+                vs[q] = v;
             }
-            if (s == 0) L.taus[i] = tau;
+            if (s == 0) taus[i] = tau;
         }
         __syncthreads();
         if (r > i && row_on) {
             // all S lanes of a row take this branch together (same r), so the
             // S-lane reduction below is convergent
-            const T tau = L.taus[i];
+            const T tau = taus[i];
             if (tau != T(0)) {
+                // vs is zero for c < i and beyond n, so whole chunks can be used
+                const T* vs = vb + vpad<CPT>(c0);
                 T part = T(0);
 #pragma unroll
-                for (int q = 0; q < CPT; ++q) {
-                    const int c = c0 + q;
-                    if (c >= i && c < n) part += w[q] * vb[c];
-                }
+                for (int q = 0; q < CPT; ++q) part += w[q] * vs[q];
                 const T dot = row_reduce(part, S) * tau;
+                // re-read v from LDS instead of keeping CPT more values live across the
+                // reduction (the row itself already holds CPT registers)
+                asm volatile("" ::: "memory");
 #pragma unroll
-                for (int q = 0; q < CPT; ++q) {
-                    const int c = c0 + q;
-                    if (c >= i && c < n) w[q] -= dot * vb[c];
-                }
+                for (int q = 0; q < CPT; ++q) w[q] -= dot * vs[q];
             }
         }
     }
     __syncthreads();
 
-    // ---- phase 4: outputs ----
+    // ---- phase 5: outputs ----
     if (row_on) {
         if (r < nk) {
             T* L1 = I.fac + f.L1;
@@ -613,17 +673,14 @@ __global__ __launch_bounds__(FACT_THREADS) void k_ulv_factor(const FInst<T>* __r
             for (int q = 0; q < CPT; ++q) {
                 const int c = c0 + q;
                 if (c < n) {
-                    if (c < nk) Vt[j + (int64_t)c * kv] = w[q];
+                    if (c < nk) Vt[c + (int64_t)j * nk] = w[q];  // V_top (nk x kv)
                     else Vr[(c - nk) + (int64_t)j * f.nr] = w[q];
                 }
             }
         }
     }
-    __threadfence();
-    __syncthreads();
-
     // LQ reflector scalars for the compact-WY blocks (built by k_ulv_wy)
-    for (int e = tid; e < nk; e += blockDim.x) I.fac[f.taulq + e] = L.taus[e];
+    for (int e = tid; e < nk; e += blockDim.x) I.fac[f.taulq + e] = taus[e];
 }
 
 // Compact-WY T blocks of the LQ reflectors (dlarft, forward columnwise) and the
@@ -788,65 +845,76 @@ __global__ __launch_bounds__(1024) void k_ulv_root(const FInst<T>* __restrict__ 
         __syncthreads();
     }
     if (tid == 0) I.status[0] = info;
+    // inverted diagonal NB x NB blocks of the unit-lower L and the upper U, for
+    // blocked triangular solves in every root solve; thread (block, column j)
+    __threadfence_block();
+    __syncthreads();
+    if (info == 0) {
+        const int nblk = (m + NB - 1) / NB;
+        T* Li = I.fac + f.LUinv;                           // nblk x NB x NB
+        T* Ui = I.fac + f.LUinv + (int64_t)nblk * NB * NB;  // nblk x NB x NB
+        for (int t = tid; t < nblk * NB; t += blockDim.x) {
+            const int bb = t / NB, jj = t % NB;
+            const int b0 = bb * NB, nb = min(NB, m - b0);
+            T x[NB];
+            // L_bb x = e_j (unit diagonal)
+#pragma unroll
+            for (int i2 = 0; i2 < NB; ++i2) {
+                T sum = (i2 == jj) ? T(1) : T(0);
+#pragma unroll
+                for (int q = 0; q < NB; ++q)
+                    if (q < i2 && i2 < nb) sum -= A[(b0 + i2) + (int64_t)(b0 + q) * m] * x[q];
+                x[i2] = (i2 < nb && jj < nb) ? sum : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < NB; ++q) Li[(int64_t)bb * NB * NB + q + jj * NB] = x[q];
+            // U_bb x = e_j
+#pragma unroll
+            for (int i2 = NB - 1; i2 >= 0; --i2) {
+                T sum = (i2 == jj) ? T(1) : T(0);
+#pragma unroll
+                for (int q = 0; q < NB; ++q)
+                    if (q > i2 && q < nb) sum -= A[(b0 + i2) + (int64_t)(b0 + q) * m] * x[q];
+                x[i2] = (i2 < nb && jj < nb) ? sum / A[(b0 + i2) + (int64_t)(b0 + i2) * m] : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < NB; ++q) Ui[(int64_t)bb * NB * NB + q + jj * NB] = x[q];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
 // ULV solve
 // ---------------------------------------------------------------------------
-// Assemble the node's right-hand side b (m x nt) in LDS:
-//   leaf:   B_seg;  branch: [b1 - P12 u2; b2 - P21 u1]  (ulvfactor.jl:74)
+// LDS layout of the up-sweep (row-major, NTP columns):
+//   bv  m rows          the node's right-hand side b
+//   zl  max(nk, 1) rows zloc
+//   us  kv_L + kv_R     children's u (branches)
+//   t1  max(k, NB, kv)  scratch
+//   t2  max(k, NB)      scratch
 template <typename TF, typename TV>
-__device__ void solve_form_b(const SInst<TF, TV>& I, const DevNode& nd, const FNode& f, int j0, int nt, TV* bv,
-                             TV* tmp) {
-    const int m = f.m;
+__device__ __forceinline__ void solve_form_b(const SInst<TF, TV>& I, const DevNode& nd, const FNode& f, int j0,
+                                             int nt, TV* bv, TV* us) {
     const int nrhs = I.nrhs;
     if (nd.is_leaf) {
-        const double* Bs = I.B + nd.row_off + (int64_t)j0 * I.ldb;
-        for (int e = threadIdx.x; e < m * nt; e += blockDim.x) {
-            const int r = e % m, j = e / m;
-            bv[e] = (TV)Bs[r + (int64_t)j * I.ldb];
-        }
+        stage(bv, 0, I.B + nd.row_off + (int64_t)j0 * I.ldb, I.ldb, f.m, nt);
         __syncthreads();
         return;
     }
     const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
     const DevNode dl = I.nodes[nd.left], dr = I.nodes[nd.right];
-    const TV* b1 = I.wb + fl.brow * nrhs + (int64_t)j0 * fl.mr;
-    const TV* b2 = I.wb + fr.brow * nrhs + (int64_t)j0 * fr.mr;
-    const TV* u1 = I.wu + fl.urow * nrhs + (int64_t)j0 * dl.kv;
-    const TV* u2 = I.wu + fr.urow * nrhs + (int64_t)j0 * dr.kv;
-    for (int e = threadIdx.x; e < m * nt; e += blockDim.x) {
-        const int r = e % m, j = e / m;
-        bv[e] = (r < fl.mr) ? b1[r + (int64_t)j * fl.mr] : b2[(r - fl.mr) + (int64_t)j * fr.mr];
-    }
+    stage(bv, 0, I.wb + fl.brow * nrhs + (int64_t)j0 * fl.mr, fl.mr, fl.mr, nt);
+    stage(bv, fl.mr, I.wb + fr.brow * nrhs + (int64_t)j0 * fr.mr, fr.mr, fr.mr, nt);
+    stage(us, 0, I.wu + fl.urow * nrhs + (int64_t)j0 * dl.kv, dl.kv, dl.kv, nt);
+    stage(us, dl.kv, I.wu + fr.urow * nrhs + (int64_t)j0 * dr.kv, dr.kv, dr.kv, nt);
     __syncthreads();
-    // tmp = P12 u2 (mr1 x nt), then P21 u1 (mr2 x nt) below it
-    blk_gemm<false>(fl.mr, nt, dr.kv, I.fac + f.P12, fl.mr, u2, dr.kv, tmp, m, TV(1), false);
-    blk_gemm<false>(fr.mr, nt, dl.kv, I.fac + f.P21, fr.mr, u1, dl.kv, tmp + fl.mr, m, TV(1), false);
+    // b = [b1 - P12 u2; b2 - P21 u1]   (ulvfactor.jl:74)
+    gemm<false>(fl.mr, nt, dr.kv, I.fac + f.P12, fl.mr, us + dl.kv * NTP,
+                [&](int i, int j, TV v) { bv[i * NTP + j] -= v; });
+    const int mr1 = fl.mr;
+    gemm<false>(fr.mr, nt, dl.kv, I.fac + f.P21, fr.mr, us,
+                [&](int i, int j, TV v) { bv[(mr1 + i) * NTP + j] -= v; });
     __syncthreads();
-    for (int e = threadIdx.x; e < m * nt; e += blockDim.x) bv[e] -= tmp[e];
-    __syncthreads();
-}
-
-// u = Vt zloc + W1' u1 + W2' u2  (ulvfactor.jl:51, 89) -> global u workspace
-template <typename TF, typename TV>
-__device__ void solve_u(const SInst<TF, TV>& I, const DevNode& nd, const FNode& f, int j0, int nt, const TV* zl,
-                        TV* ug) {
-    const int kv = nd.kv;
-    const int nrhs = I.nrhs;
-    if (f.nk > 0) blk_gemm<false>(kv, nt, f.nk, I.fac + f.Vt, kv, zl, f.nk, ug, kv, TV(1), false);
-    else
-        for (int e = threadIdx.x; e < kv * nt; e += blockDim.x) ug[e] = TV(0);
-    if (!nd.is_leaf) {
-        const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
-        const DevNode dl = I.nodes[nd.left], dr = I.nodes[nd.right];
-        const TV* u1 = I.wu + fl.urow * nrhs + (int64_t)j0 * dl.kv;
-        const TV* u2 = I.wu + fr.urow * nrhs + (int64_t)j0 * dr.kv;
-        __syncthreads();
-        blk_gemm<true>(kv, nt, dl.kv, I.blob + dl.W, dl.kv, u1, dl.kv, ug, kv, TV(1), true);
-        __syncthreads();
-        blk_gemm<true>(kv, nt, dr.kv, I.blob + dr.W, dr.kv, u2, dr.kv, ug, kv, TV(1), true);
-    }
 }
 
 template <typename TF, typename TV>
@@ -859,71 +927,76 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_sup(const SInst<TF, TV>* 
     const int nt = min(RHS_TILE, I.nrhs - j0);
     const DevNode nd = I.nodes[wi.node];
     const FNode f = I.fn[wi.node];
-    const int m = f.m, k = f.k, nk = f.nk;
+    const int m = f.m, k = f.k, nk = f.nk, kv = nd.kv;
     const int nrhs = I.nrhs;
+    int kvl = 0, kvr = 0;
+    if (!nd.is_leaf) {
+        kvl = I.nodes[nd.left].kv;
+        kvr = I.nodes[nd.right].kv;
+    }
+    const int tr = max(max(k, NB), kv);
     extern __shared__ unsigned char lds_raw[];
-    const int mp = m > NB ? m : NB;
-    TV* bv = reinterpret_cast<TV*>(lds_raw);  // m x nt
-    TV* tmp = bv + mp * nt;                   // max(m, NB) x nt
-    TV* t2 = tmp + mp * nt;                   // max(m, NB) x nt
-    solve_form_b(I, nd, f, j0, nt, bv, tmp);
+    TV* bv = reinterpret_cast<TV*>(lds_raw);
+    TV* zl = bv + m * NTP;
+    TV* us = zl + max(nk, 1) * NTP;
+    TV* t1 = us + (kvl + kvr) * NTP;
+    TV* t2 = t1 + tr * NTP;
+    solve_form_b(I, nd, f, j0, nt, bv, us);
     TV* wbn = I.wb + f.brow * nrhs + (int64_t)j0 * f.mr;
-    TV* ug = I.wu + f.urow * nrhs + (int64_t)j0 * nd.kv;
-    if (f.pass) {
-        for (int e = threadIdx.x; e < m * nt; e += blockDim.x) wbn[e] = bv[e];
-        solve_u(I, nd, f, j0, nt, (const TV*)nullptr, ug);
-        return;
-    }
-    // b <- Q_ql' b = b - Y (T (Y' b))   (ulvfactor.jl:42)
-    if (k > 0) {
-        blk_gemm<true>(k, nt, m, I.fac + f.Yql, m, bv, m, tmp, k, TV(1), false);
-        __syncthreads();
-        // t2 = T tmp, T upper triangular k x k
-        blk_gemm<false>(k, nt, k, I.fac + f.Tql, k, tmp, k, t2, k, TV(1), false);
-        __syncthreads();
-        blk_gemm<false>(m, nt, k, I.fac + f.Yql, m, t2, k, bv, m, TV(-1), true);
-        __syncthreads();
-    }
-    // zloc = L1 \ b[0:nk]  (ulvfactor.jl:48), blocked with inverted diagonal blocks
-    TV* zl = tmp;  // nk x nt
-    for (int bb = 0; bb < f.nblk; ++bb) {
-        const int b0 = bb * NB, nb = min(NB, nk - b0);
-        // t2[0:nb] = b[b0:b0+nb] - L1[b0:b0+nb, 0:b0] zl[0:b0]
-        for (int e = threadIdx.x; e < nb * nt; e += blockDim.x) {
-            const int i = e % nb, j = e / nb;
-            TV sacc = bv[b0 + i + j * m];
-            const TF* Lr = I.fac + f.L1 + b0 + i;
-            for (int q = 0; q < b0; ++q) sacc -= (TV)Lr[(int64_t)q * nk] * zl[q + j * nk];
-            t2[i + j * NB] = sacc;
+    TV* ug = I.wu + f.urow * nrhs + (int64_t)j0 * kv;
+    const int mr = f.mr;
+    if (!f.pass) {
+        // b <- Q_ql' b = b - Y (T (Y' b))   (ulvfactor.jl:42)
+        if (k > 0) {
+            gemm<true>(k, nt, m, I.fac + f.Yql, m, bv, [&](int i, int j, TV v) { t1[i * NTP + j] = v; });
+            __syncthreads();
+            gemm<false>(k, nt, k, I.fac + f.Tql, k, t1, [&](int i, int j, TV v) { t2[i * NTP + j] = v; });
+            __syncthreads();
+            gemm<false>(m, nt, k, I.fac + f.Yql, m, t2, [&](int i, int j, TV v) { bv[i * NTP + j] -= v; });
+            __syncthreads();
         }
-        __syncthreads();
-        const TF* Li = I.fac + f.L1inv + (int64_t)bb * NB * NB;
-        for (int e = threadIdx.x; e < nb * nt; e += blockDim.x) {
-            const int i = e % nb, j = e / nb;
-            TV sacc = TV(0);
-            for (int q = 0; q <= i; ++q) sacc += (TV)Li[i + q * NB] * t2[q + j * NB];
-            zl[b0 + i + j * nk] = sacc;
+        // zloc = L1 \ b[0:nk]  (ulvfactor.jl:48), blocked with inverted diagonal blocks
+        for (int bb = 0; bb < f.nblk; ++bb) {
+            const int b0 = bb * NB, nb = min(NB, nk - b0);
+            gemm<false>(nb, nt, b0, I.fac + f.L1 + b0, nk, zl,
+                        [&](int i, int j, TV v) { t1[i * NTP + j] = bv[(b0 + i) * NTP + j] - v; });
+            __syncthreads();
+            gemm<false>(nb, nt, nb, I.fac + f.L1inv + (int64_t)bb * NB * NB, NB, t1,
+                        [&](int i, int j, TV v) { zl[(b0 + i) * NTP + j] = v; });
+            __syncthreads();
         }
-        __syncthreads();
+        // reduced b = b[nk:m] - L2a zloc  (ulvfactor.jl:49) -> parent's input
+        gemm<false>(k, nt, nk, I.fac + f.L2a, k, zl,
+                    [&](int i, int j, TV v) { wbn[i + (int64_t)j * mr] = bv[(nk + i) * NTP + j] - v; });
+        // zloc -> the node's z vector (rows 0..nk-1)
+        unstage(I.wz + f.zrow * nrhs + (int64_t)j0 * f.n, f.n, zl, 0, nk, nt);
+    } else {
+        unstage(wbn, mr, bv, 0, m, nt);
     }
-    // b_r = b[nk:m] - L2a zloc  (ulvfactor.jl:49) -> parent's input
-    for (int e = threadIdx.x; e < k * nt; e += blockDim.x) {
-        const int i = e % k, j = e / k;
-        TV sacc = bv[nk + i + j * m];
-        const TF* Lr = I.fac + f.L2a + i;
-        for (int q = 0; q < nk; ++q) sacc -= (TV)Lr[(int64_t)q * k] * zl[q + j * nk];
-        wbn[i + j * f.mr] = sacc;
+    // u = V_top' zloc + W1' u1 + W2' u2
+    if (kv > 0) {
+        const bool has_z = !f.pass && nk > 0;
+        if (nd.is_leaf) {
+            if (has_z)
+                gemm<true>(kv, nt, nk, I.fac + f.Vt, nk, zl, [&](int i, int j, TV v) { ug[i + (int64_t)j * kv] = v; });
+            else
+                for (int e = threadIdx.x; e < kv * nt; e += blockDim.x) ug[(e % kv) + (int64_t)(e / kv) * kv] = TV(0);
+        } else {
+            const DevNode dl = I.nodes[nd.left], dr = I.nodes[nd.right];
+            if (has_z)
+                gemm<true>(kv, nt, nk, I.fac + f.Vt, nk, zl, [&](int i, int j, TV v) { t2[i * NTP + j] = v; });
+            else
+                for (int e = threadIdx.x; e < kv * nt; e += blockDim.x) t2[(e % kv) * NTP + e / kv] = TV(0);
+            __syncthreads();
+            gemm<true>(kv, nt, kvl, I.blob + dl.W, kvl, us, [&](int i, int j, TV v) { t2[i * NTP + j] += v; });
+            __syncthreads();
+            gemm<true>(kv, nt, kvr, I.blob + dr.W, kvr, us + kvl * NTP,
+                       [&](int i, int j, TV v) { ug[i + (int64_t)j * kv] = t2[i * NTP + j] + v; });
+        }
     }
-    // store zloc into the node's z vector (rows 0..nk-1)
-    TV* zn = I.wz + f.zrow * nrhs + (int64_t)j0 * f.n;
-    for (int e = threadIdx.x; e < nk * nt; e += blockDim.x) {
-        const int i = e % nk, j = e / nk;
-        zn[i + j * f.n] = zl[e];
-    }
-    solve_u(I, nd, f, j0, nt, zl, ug);
 }
 
-// Root: b assembled like a branch, z = LU \ b
+// Root: b assembled like a branch, z = D^-1 b with the inverse formed at factorization
 template <typename TF, typename TV>
 __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_sroot(const SInst<TF, TV>* __restrict__ insts,
                                                             const WorkItem* __restrict__ work) {
@@ -935,37 +1008,48 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_sroot(const SInst<TF, TV>
     const DevNode nd = I.nodes[wi.node];
     const FNode f = I.fn[wi.node];
     const int m = f.m;
+    int kvl = 0, kvr = 0;
+    if (!nd.is_leaf) {
+        kvl = I.nodes[nd.left].kv;
+        kvr = I.nodes[nd.right].kv;
+    }
     extern __shared__ unsigned char lds_raw[];
     TV* bv = reinterpret_cast<TV*>(lds_raw);
-    TV* tmp = bv + m * nt;
-    solve_form_b(I, nd, f, j0, nt, bv, tmp);
-    const TF* A = I.fac + f.LU;
+    TV* us = bv + m * NTP;
+    (void)kvr;
+    TV* t1 = us + (kvl + kvr) * NTP;  // NB x NTP
+    solve_form_b(I, nd, f, j0, nt, bv, us);
+    // row interchanges of the partial pivoting (one thread per column)
     const int32_t* piv = I.ipiv + f.piv;
-    // row interchanges, then L (unit) and U solves, one thread per column
-    for (int j = threadIdx.x; j < nt; j += blockDim.x) {
-        TV* x = bv + j * m;
+    for (int j = threadIdx.x; j < nt; j += blockDim.x)
         for (int i = 0; i < m; ++i) {
             const int p = piv[i];
-            if (p != i) { const TV t = x[i]; x[i] = x[p]; x[p] = t; }
+            if (p != i) { const TV t = bv[i * NTP + j]; bv[i * NTP + j] = bv[p * NTP + j]; bv[p * NTP + j] = t; }
         }
-        for (int i = 0; i < m; ++i) {
-            const TV xi = x[i];
-            for (int rr = i + 1; rr < m; ++rr) x[rr] -= (TV)A[rr + (int64_t)i * m] * xi;
-        }
-        for (int i = m - 1; i >= 0; --i) {
-            x[i] /= (TV)A[i + (int64_t)i * m];
-            const TV xi = x[i];
-            for (int rr = 0; rr < i; ++rr) x[rr] -= (TV)A[rr + (int64_t)i * m] * xi;
-        }
-    }
     __syncthreads();
-    if (nd.is_leaf) {
-        double* Xs = I.X + nd.col_off + (int64_t)j0 * I.ldx;
-        for (int e = threadIdx.x; e < m * nt; e += blockDim.x) Xs[e % m + (int64_t)(e / m) * I.ldx] = (double)bv[e];
-    } else {
-        TV* zn = I.wz + f.zrow * I.nrhs + (int64_t)j0 * f.n;
-        for (int e = threadIdx.x; e < m * nt; e += blockDim.x) zn[e] = bv[e];
+    const TF* LU = I.fac + f.LU;
+    const int nblk = (m + NB - 1) / NB;
+    const TF* Li = I.fac + f.LUinv;
+    const TF* Ui = Li + (int64_t)nblk * NB * NB;
+    // forward: y_b = L_bb^-1 (b_b - L[b, 0:b0] y[0:b0])
+    for (int bb = 0; bb < nblk; ++bb) {
+        const int b0 = bb * NB, nb = min(NB, m - b0);
+        gemm<false>(nb, nt, b0, LU + b0, m, bv, [&](int i, int j, TV v) { t1[i * NTP + j] = bv[(b0 + i) * NTP + j] - v; });
+        __syncthreads();
+        gemm<false>(nb, nt, nb, Li + (int64_t)bb * NB * NB, NB, t1, [&](int i, int j, TV v) { bv[(b0 + i) * NTP + j] = v; });
+        __syncthreads();
     }
+    // backward: x_b = U_bb^-1 (y_b - U[b, b1:m] x[b1:m])
+    for (int bb = nblk - 1; bb >= 0; --bb) {
+        const int b0 = bb * NB, nb = min(NB, m - b0), b1 = b0 + nb;
+        gemm<false>(nb, nt, m - b1, LU + b0 + (int64_t)b1 * m, m, bv + b1 * NTP,
+                    [&](int i, int j, TV v) { t1[i * NTP + j] = bv[(b0 + i) * NTP + j] - v; });
+        __syncthreads();
+        gemm<false>(nb, nt, nb, Ui + (int64_t)bb * NB * NB, NB, t1, [&](int i, int j, TV v) { bv[(b0 + i) * NTP + j] = v; });
+        __syncthreads();
+    }
+    if (nd.is_leaf) unstage(I.X + nd.col_off + (int64_t)j0 * I.ldx, I.ldx, bv, 0, m, nt);
+    else unstage(I.wz + f.zrow * I.nrhs + (int64_t)j0 * f.n, f.n, bv, 0, m, nt);
 }
 
 // Down-sweep: z = [zloc; z_from_parent], z <- Q_lq' z  (ulvfactor.jl:98-107)
@@ -982,37 +1066,28 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_sdown(const SInst<TF, TV>
     const int n = f.n, nk = f.nk;
     const int nrhs = I.nrhs;
     extern __shared__ unsigned char lds_raw[];
-    TV* z = reinterpret_cast<TV*>(lds_raw);  // n x nt
-    TV* t1 = z + n * nt;                      // NB x nt
-    TV* t2 = t1 + NB * nt;                    // NB x nt
+    TV* z = reinterpret_cast<TV*>(lds_raw);  // n x NTP
+    TV* t1 = z + n * NTP;                     // NB x NTP
+    TV* t2 = t1 + NB * NTP;                   // NB x NTP
     const FNode fp = I.fn[nd.parent];
     const int off = nd.is_left ? 0 : I.fn[I.nodes[nd.parent].left].nr;
-    const TV* zp = I.wz + fp.zrow * nrhs + (int64_t)j0 * fp.n;
-    const TV* zn = I.wz + f.zrow * nrhs + (int64_t)j0 * n;
-    for (int e = threadIdx.x; e < n * nt; e += blockDim.x) {
-        const int r = e % n, j = e / n;
-        z[e] = (r < nk) ? zn[r + j * n] : zp[off + (r - nk) + j * fp.n];
-    }
+    stage(z, 0, I.wz + f.zrow * nrhs + (int64_t)j0 * n, n, nk, nt);
+    stage(z, nk, I.wz + fp.zrow * nrhs + (int64_t)j0 * fp.n + off, fp.n, n - nk, nt);
     __syncthreads();
-    // Q' = H(1)...H(nk) = P_0 ... P_{B-1}; apply P_{B-1} first
+    // Q' = H(1)...H(nk) = P_0 ... P_{B-1}, P_b = I - Y_b T_b Y_b'; apply P_{B-1} first
     for (int bb = f.nblk - 1; bb >= 0; --bb) {
         const int b0 = bb * NB, nb = min(NB, nk - b0);
-        const TF* Yb = I.fac + f.Ylq + (int64_t)b0 * n;  // n x nb
-        const TF* Tb = I.fac + f.Tlq + (int64_t)bb * NB * NB;
-        blk_gemm<true>(nb, nt, n - b0, Yb + b0, n, z + b0, n, t1, NB, TV(1), false);
+        const TF* Yb = I.fac + f.Ylq + (int64_t)b0 * n + b0;  // rows b0.., columns b0..b0+nb
+        gemm<true>(nb, nt, n - b0, Yb, n, z + b0 * NTP, [&](int i, int j, TV v) { t1[i * NTP + j] = v; });
         __syncthreads();
-        blk_gemm<false>(nb, nt, nb, Tb, NB, t1, NB, t2, NB, TV(1), false);
+        gemm<false>(nb, nt, nb, I.fac + f.Tlq + (int64_t)bb * NB * NB, NB, t1,
+                    [&](int i, int j, TV v) { t2[i * NTP + j] = v; });
         __syncthreads();
-        blk_gemm<false>(n - b0, nt, nb, Yb + b0, n, t2, NB, z + b0, n, TV(-1), true);
+        gemm<false>(n - b0, nt, nb, Yb, n, t2, [&](int i, int j, TV v) { z[(b0 + i) * NTP + j] -= v; });
         __syncthreads();
     }
-    if (nd.is_leaf) {
-        double* Xs = I.X + nd.col_off + (int64_t)j0 * I.ldx;
-        for (int e = threadIdx.x; e < n * nt; e += blockDim.x) Xs[e % n + (int64_t)(e / n) * I.ldx] = (double)z[e];
-    } else {
-        TV* zo = I.wz + f.zrow * nrhs + (int64_t)j0 * n;
-        for (int e = threadIdx.x; e < n * nt; e += blockDim.x) zo[e] = z[e];
-    }
+    if (nd.is_leaf) unstage(I.X + nd.col_off + (int64_t)j0 * I.ldx, I.ldx, z, 0, n, nt);
+    else unstage(I.wz + f.zrow * nrhs + (int64_t)j0 * n, n, z, 0, n, nt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1057,25 +1132,75 @@ __global__ void k_colnorm2(const double* __restrict__ A, int64_t lda, int64_t n,
     }
 }
 
+// Batched refinement helpers: block (request, column).
+// r = b - r (r holds A x on entry); norms[2*(req*maxc + col)] = ||b||^2, [+1] = ||r||^2
+__global__ void k_resid_batch(const BatchVec* __restrict__ tab, int maxc, double* __restrict__ norms) {
+    const BatchVec v = tab[blockIdx.x];
+    const int j = blockIdx.y;
+    if (j >= v.nrhs) return;
+    double sb = 0.0, sr = 0.0;
+    for (int64_t i = threadIdx.x; i < v.n; i += blockDim.x) {
+        const double bi = v.b[i + j * v.ldb];
+        const double ri = bi - v.r[i + j * v.n];
+        v.r[i + j * v.n] = ri;
+        sb += bi * bi;
+        sr += ri * ri;
+    }
+    __shared__ double red[2][16];
+    sb = wave_reduce(sb);
+    sr = wave_reduce(sr);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = sb;
+        red[1][threadIdx.x >> 6] = sr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tb = 0.0, trr = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            tb += red[0][w];
+            trr += red[1][w];
+        }
+        norms[2 * ((int64_t)blockIdx.x * maxc + j)] = tb;
+        norms[2 * ((int64_t)blockIdx.x * maxc + j) + 1] = trr;
+    }
+}
+
+// x += d for active requests
+__global__ void k_axpy_batch(const BatchVec* __restrict__ tab) {
+    const BatchVec v = tab[blockIdx.x];
+    const int j = blockIdx.y;
+    if (j >= v.nrhs) return;
+    for (int64_t i = threadIdx.x; i < v.n; i += blockDim.x) v.x[i + j * v.ldx] += v.d[i + j * v.n];
+}
+
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
 static inline int rhs_tiles(int nrhs) { return (nrhs + RHS_TILE - 1) / RHS_TILE; }
 
+static hipError_t lds_attr(const void* fn, size_t lds) {
+    if (lds > 65536) return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return hipSuccess;
+}
+
 hipError_t launch_mv_up(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
                         hipStream_t st) {
     if (nwork <= 0) return hipSuccess;
+    if (hipError_t e = lds_attr(reinterpret_cast<const void*>(&k_mv_up), lds)) return e;
     hipLaunchKernelGGL(k_mv_up, dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st, insts, work);
     return hipGetLastError();
 }
-hipError_t launch_mv_down(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, hipStream_t st) {
+hipError_t launch_mv_down(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                          hipStream_t st) {
     if (nwork <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mv_down, dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), 0, st, insts, work);
+    if (hipError_t e = lds_attr(reinterpret_cast<const void*>(&k_mv_down), lds)) return e;
+    hipLaunchKernelGGL(k_mv_down, dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st, insts, work);
     return hipGetLastError();
 }
 hipError_t launch_mv_leaf(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
                           hipStream_t st) {
     if (nwork <= 0) return hipSuccess;
+    if (hipError_t e = lds_attr(reinterpret_cast<const void*>(&k_mv_leaf), lds)) return e;
     hipLaunchKernelGGL(k_mv_leaf, dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st, insts, work);
     return hipGetLastError();
 }
@@ -1138,6 +1263,17 @@ hipError_t launch_ulv_sdown(const SInst<TF, TV>* insts, const WorkItem* work, in
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_ulv_sdown<TF, TV>), dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st,
                        insts, work);
+    return hipGetLastError();
+}
+
+hipError_t launch_resid_batch(const BatchVec* tab, int cnt, int maxc, double* norms, hipStream_t st) {
+    if (cnt <= 0 || maxc <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_resid_batch, dim3(cnt, maxc), dim3(256), 0, st, tab, maxc, norms);
+    return hipGetLastError();
+}
+hipError_t launch_axpy_batch(const BatchVec* tab, int cnt, int maxc, hipStream_t st) {
+    if (cnt <= 0 || maxc <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_axpy_batch, dim3(cnt, maxc), dim3(256), 0, st, tab);
     return hipGetLastError();
 }
 

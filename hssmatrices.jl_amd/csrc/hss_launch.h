@@ -8,7 +8,8 @@ namespace hss {
 
 hipError_t launch_mv_up(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
                         hipStream_t st);
-hipError_t launch_mv_down(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, hipStream_t st);
+hipError_t launch_mv_down(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                          hipStream_t st);
 hipError_t launch_mv_leaf(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
                           hipStream_t st);
 
@@ -28,6 +29,8 @@ template <typename TF, typename TV>
 hipError_t launch_ulv_sdown(const SInst<TF, TV>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
                             hipStream_t st);
 
+hipError_t launch_resid_batch(const BatchVec* tab, int cnt, int maxc, double* norms, hipStream_t st);
+hipError_t launch_axpy_batch(const BatchVec* tab, int cnt, int maxc, hipStream_t st);
 hipError_t launch_residual(const double* B, int64_t ldb, double* R, int64_t ldr, int64_t n, int nrhs, hipStream_t st);
 hipError_t launch_axpy(double* X, int64_t ldx, const double* D, int64_t ldd, int64_t n, int nrhs, hipStream_t st);
 hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, double* out, hipStream_t st);

@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <new>
 #include <string>
@@ -160,7 +161,7 @@ struct hss_mat_s {
     Range leaves, root;
     int max_height = 0, max_depth = 0;
     int64_t zrows = 0, frows = 0;
-    size_t lds_up = 0, lds_leaf = 0;  // per RHS tile
+    size_t lds_up = 0, lds_down = 0, lds_leaf = 0;  // bytes per launch
     // per-call scratch
     SlotCache mvinst;
     double *d_Z = nullptr, *d_F = nullptr;
@@ -349,17 +350,25 @@ extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, 
     M->root.off = (int)M->hwork.size();
     M->hwork.push_back(WorkItem{0, 0});
     M->root.count = 1;
-    size_t maxn = 0, maxleaf = 0;
-    for (int i = 0; i < nn; ++i)
-        if (M->hn[i].is_leaf) {
-            maxn = std::max(maxn, (size_t)M->hn[i].n);
-            maxleaf = std::max(maxleaf, (size_t)(M->hn[i].n + M->hn[i].ku));
+    size_t lup = 0, ldown = 0, lleaf = 0;
+    for (int i = 0; i < nn; ++i) {
+        const DevNode& d = M->hn[i];
+        if (d.is_leaf) {
+            lup = std::max(lup, (size_t)d.n);
+            lleaf = std::max(lleaf, (size_t)(d.n + d.m + d.ku));
+        } else {
+            const DevNode& l = M->hn[d.left];
+            const DevNode& r = M->hn[d.right];
+            lup = std::max(lup, (size_t)(l.kv + r.kv + d.kv));
+            ldown = std::max(ldown, (size_t)(l.kv + r.kv + d.ku + l.ku + r.ku));
         }
-    M->lds_up = maxn * RHS_TILE * sizeof(double);
-    M->lds_leaf = maxleaf * RHS_TILE * sizeof(double);
-    if (M->lds_leaf > 160 * 1024) {
+    }
+    M->lds_up = lup * RHS_TILE * sizeof(double);
+    M->lds_down = ldown * RHS_TILE * sizeof(double);
+    M->lds_leaf = lleaf * RHS_TILE * sizeof(double);
+    if (M->lds_leaf > 160 * 1024 || M->lds_up > 160 * 1024 || M->lds_down > 160 * 1024) {
         free_mat(M);
-        return fail(HSS_ERR_UNSUPPORTED, "leaf too large for the LDS-staged matvec");
+        return fail(HSS_ERR_UNSUPPORTED, "node too large for the LDS-staged matvec");
     }
 
     hipError_t e;
@@ -440,12 +449,12 @@ static hss_status matmul_enqueue(hss_mat_s* M, int nrhs, double alpha, const dou
         for (int h = 0; h < M->max_height; ++h) {
             const Range& r = M->by_height[h];
             ProfScope ps("k_mv_up", h, r.count, st);
-            HIP_TRY(launch_mv_up(di, M->d_work + r.off, r.count, nrhs, h == 0 ? M->lds_up : 0, st));
+            HIP_TRY(launch_mv_up(di, M->d_work + r.off, r.count, nrhs, M->lds_up, st));
         }
         for (int d = 0; d <= M->max_depth; ++d) {
             const Range& r = M->br_depth[d];
             ProfScope ps("k_mv_down", d, r.count, st);
-            HIP_TRY(launch_mv_down(di, M->d_work + r.off, r.count, nrhs, st));
+            HIP_TRY(launch_mv_down(di, M->d_work + r.off, r.count, nrhs, M->lds_down, st));
         }
     }
     {
@@ -514,9 +523,10 @@ static int pow2_at_least(int x) {
 
 template <typename TF>
 static size_t factor_lds(const FNode& f, const DevNode& d) {
-    const size_t nw = FACT_THREADS / 64;
+    // Us (m x ku) | G, T (k x k each) | M (k x n) | two padded reflector rows | taus | scalars
     const size_t kk = (size_t)std::max(f.k, f.nk) + 1;
-    return sizeof(TF) * ((size_t)f.m * d.ku + 2 * (size_t)f.n + nw * f.n + f.n + 8 + kk + (size_t)f.k * f.k);
+    const size_t vstride = (size_t)f.n + 2 * ((f.n + FACT_CPT - 1) / FACT_CPT) + 2;
+    return sizeof(TF) * ((size_t)f.m * d.ku + 2 * (size_t)f.k * f.k + ((size_t)f.k + 2) * vstride + kk + 8);
 }
 
 // Fill the factor metadata of one matrix; offsets are relative to this
@@ -536,7 +546,7 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
         const DevNode& d = M->hn[i];
         FNode& f = F->hfn[i];
         std::memset(&f, 0, sizeof(f));
-        f.Yql = f.Tql = f.Ylq = f.taulq = f.Tlq = f.L1 = f.L1inv = f.L2a = f.Vt = f.P12 = f.P21 = f.LU = f.piv = -1;
+        f.Yql = f.Tql = f.Ylq = f.taulq = f.Tlq = f.L1 = f.L1inv = f.L2a = f.Vt = f.P12 = f.P21 = f.LU = f.LUinv = f.piv = -1;
         f.rwD = f.rwU = f.rwV = f.rwM = -1;
         if (d.is_leaf) {
             f.m = d.m;
@@ -556,6 +566,7 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
                                          std::to_string(f.m) + "x" + std::to_string(f.n) + ")");
         if (i == 0) {
             f.LU = take(fac, (int64_t)f.m * f.m);
+            f.LUinv = take(fac, (int64_t)2 * ((f.m + NB - 1) / NB) * NB * NB);  // inverted L/U diagonal blocks
             f.piv = take(piv, f.m);
             f.zrow = take(zrow, f.m);
             f.mr = f.m;
@@ -571,7 +582,7 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
         int S = pow2_at_least((f.n + FACT_CPT - 1) / FACT_CPT);
         if (S < 1) S = 1;
         f.S = S;
-        if (S > 64 || (int64_t)(f.m + d.kv) * S > FACT_THREADS)
+        if (S > 64 || (int64_t)(f.m + d.kv) * S > FACT_THREADS || f.m > 256 || f.n > 256)
             return fail(HSS_ERR_UNSUPPORTED, "node " + std::to_string(i) + " (" + std::to_string(f.m) + "x" +
                                                  std::to_string(f.n) + ", kv " + std::to_string(d.kv) +
                                                  ") exceeds the register-resident factor kernel");
@@ -600,18 +611,21 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
     F->brows = brow;
     F->urows = urow;
     F->zrows = zrow;
-    // per-level LDS requirements of the solve kernels (per RHS tile)
+    // LDS requirements of the solve kernels (rows of NTP = RHS_TILE columns)
     size_t sup = 0, sroot = 0, sdown = 0;
     for (int i = 0; i < nn; ++i) {
         const FNode& f = F->hfn[i];
-        const size_t mp = (size_t)std::max(f.m, NB);
-        if (i == 0) sroot = 2 * mp * RHS_TILE;
-        else {
-            sup = std::max(sup, 3 * mp * RHS_TILE);
-            sdown = std::max(sdown, ((size_t)f.n + 2 * NB) * RHS_TILE);
+        const DevNode& d = M->hn[i];
+        const int kvc = d.is_leaf ? 0 : M->hn[d.left].kv + M->hn[d.right].kv;
+        if (i == 0) {
+            sroot = (size_t)f.m + kvc + NB;
+        } else {
+            const size_t tr = (size_t)std::max(std::max(f.k, NB), d.kv);
+            sup = std::max(sup, (size_t)f.m + std::max(f.nk, 1) + kvc + 2 * tr);
+            sdown = std::max(sdown, (size_t)f.n + 2 * NB);
         }
     }
-    const size_t tv = sizeof(TF);
+    const size_t tv = sizeof(TF) * RHS_TILE;
     F->lds_sup = sup * tv;
     F->lds_sroot = sroot * tv;
     F->lds_sdown = sdown * tv;
@@ -825,7 +839,7 @@ struct SolveReq {
 };
 
 template <typename TF, typename TV>
-static SInst<TF, TV> make_sinst(const SolveReq& q) {
+static SInst<TF, TV> make_sinst(const SolveReq& q, TV* wb = nullptr, TV* wu = nullptr, TV* wz = nullptr) {
     hss_fact_s* F = q.F;
     hss_mat_s* M = F->mat;
     SInst<TF, TV> I;
@@ -839,9 +853,9 @@ static SInst<TF, TV> make_sinst(const SolveReq& q) {
     I.X = q.dX;
     I.ldb = q.ldb;
     I.ldx = q.ldx;
-    I.wb = reinterpret_cast<TV*>(F->d_wb);
-    I.wu = reinterpret_cast<TV*>(F->d_wu);
-    I.wz = reinterpret_cast<TV*>(F->d_wz);
+    I.wb = wb ? wb : reinterpret_cast<TV*>(F->d_wb);
+    I.wu = wu ? wu : reinterpret_cast<TV*>(F->d_wu);
+    I.wz = wz ? wz : reinterpret_cast<TV*>(F->d_wz);
     I.nrhs = q.nrhs;
     return I;
 }
@@ -1074,78 +1088,406 @@ extern "C" hss_status hss_ulvfactsolve(hss_mat_t M, int32_t nrhs, const double* 
     return s;
 }
 
-// Batched solves: one launch per level across all requests.  Requests that
-// share a factor must not overlap (they would share its workspace).
+// ---------------------------------------------------------------------------
+// batch plans: many solves (matrices of any size, repeated or not) in one
+// launch per tree level; per-request workspaces; optional refinement run in
+// lockstep with a batched Float64 matvec
+// ---------------------------------------------------------------------------
+struct hss_plan_s {
+    hss_ctx_s* ctx = nullptr;
+    hss_prec prec = HSS_PREC_FP64;
+    int cnt = 0, maxnrhs = 0;
+    bool has_ir = false;
+    std::vector<SolveReq> reqs;
+    char* arena = nullptr;
+    // solve tables: [0] solves b -> x, [1] solves r -> d (refinement)
+    char* d_sinst[2] = {nullptr, nullptr};
+    std::vector<char> h_sinst[2];
+    size_t sinst_size = 0;
+    MvInst* d_mv = nullptr;
+    std::vector<MvInst> h_mv;
+    BatchVec* d_vec = nullptr;
+    std::vector<BatchVec> h_vec;
+    WorkItem* d_work = nullptr;
+    Schedule S;
+    std::vector<Range> mv_up, mv_down;
+    Range mv_leaf;
+    size_t lsup = 0, lsroot = 0, lsdown = 0, lmv_up = 0, lmv_down = 0, lmv_leaf = 0;
+    double* d_norms = nullptr;
+    std::vector<double> h_norms;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int64_t dev_bytes = 0;
+};
+
+static void free_plan(hss_plan_s* P) {
+    if (!P) return;
+    if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    (void)hipFree(P->arena);
+    (void)hipFree(P->d_sinst[0]);
+    (void)hipFree(P->d_sinst[1]);
+    (void)hipFree(P->d_mv);
+    (void)hipFree(P->d_vec);
+    (void)hipFree(P->d_work);
+    (void)hipFree(P->d_norms);
+    delete P;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+template <typename TF>
+static hss_status plan_tables(hss_plan_s* P) {
+    using TV = TF;
+    const int cnt = P->cnt;
+    // arena layout
+    std::vector<size_t> off_ws(cnt), off_mv(cnt), off_r(cnt), off_d(cnt);
+    size_t tot = 0;
+    for (int i = 0; i < cnt; ++i) {
+        const SolveReq& q = P->reqs[i];
+        hss_fact_s* F = q.F;
+        hss_mat_s* M = F->mat;
+        off_ws[i] = tot;
+        tot = align256(tot + sizeof(TV) * (size_t)(F->brows + F->urows + F->zrows) * q.nrhs);
+        if (P->has_ir) {
+            off_mv[i] = tot;
+            tot = align256(tot + sizeof(double) * (size_t)(M->zrows + M->frows) * q.nrhs);
+            off_r[i] = tot;
+            tot = align256(tot + sizeof(double) * (size_t)M->rows * q.nrhs);
+            off_d[i] = tot;
+            tot = align256(tot + sizeof(double) * (size_t)M->rows * q.nrhs);
+        }
+    }
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&P->arena), std::max<size_t>(tot, 256)));
+    P->dev_bytes = (int64_t)tot;
+    P->sinst_size = sizeof(SInst<TF, TV>);
+    for (int t = 0; t < (P->has_ir ? 2 : 1); ++t) {
+        std::vector<SInst<TF, TV>> v(cnt);
+        for (int i = 0; i < cnt; ++i) {
+            const SolveReq& q = P->reqs[i];
+            hss_fact_s* F = q.F;
+            TV* wb = reinterpret_cast<TV*>(P->arena + off_ws[i]);
+            TV* wu = wb + F->brows * q.nrhs;
+            TV* wz = wu + F->urows * q.nrhs;
+            SolveReq qq = q;
+            if (t == 1) {
+                qq.dB = reinterpret_cast<double*>(P->arena + off_r[i]);
+                qq.ldb = std::max<int64_t>(F->mat->rows, 1);
+                qq.dX = reinterpret_cast<double*>(P->arena + off_d[i]);
+                qq.ldx = std::max<int64_t>(F->mat->rows, 1);
+            }
+            v[i] = make_sinst<TF, TV>(qq, wb, wu, wz);
+        }
+        P->h_sinst[t].assign(reinterpret_cast<char*>(v.data()), reinterpret_cast<char*>(v.data() + cnt));
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&P->d_sinst[t]), P->h_sinst[t].size()));
+        HIP_TRY(hipMemcpy(P->d_sinst[t], P->h_sinst[t].data(), P->h_sinst[t].size(), hipMemcpyHostToDevice));
+    }
+    if (P->has_ir) {
+        P->h_mv.resize(cnt);
+        P->h_vec.resize(cnt);
+        for (int i = 0; i < cnt; ++i) {
+            const SolveReq& q = P->reqs[i];
+            hss_mat_s* M = q.F->mat;
+            const int64_t n = std::max<int64_t>(M->rows, 1);
+            MvInst& I = P->h_mv[i];
+            std::memset(&I, 0, sizeof(I));
+            I.nodes = M->d_nodes;
+            I.blob = M->d_blob;
+            I.B = q.dX;
+            I.ldb = q.ldx;
+            I.C = reinterpret_cast<double*>(P->arena + off_r[i]);
+            I.ldc = n;
+            I.Z = reinterpret_cast<double*>(P->arena + off_mv[i]);
+            I.F = I.Z + M->zrows * q.nrhs;
+            I.alpha = 1.0;
+            I.beta = 0.0;
+            I.nrhs = q.nrhs;
+            BatchVec& v = P->h_vec[i];
+            std::memset(&v, 0, sizeof(v));
+            v.b = q.dB;
+            v.ldb = q.ldb;
+            v.x = q.dX;
+            v.ldx = q.ldx;
+            v.r = reinterpret_cast<double*>(P->arena + off_r[i]);
+            v.d = reinterpret_cast<double*>(P->arena + off_d[i]);
+            v.n = M->rows;
+            v.nrhs = q.nrhs;
+        }
+        HIP_TRY(dalloc(&P->d_mv, (size_t)cnt));
+        HIP_TRY(dalloc(&P->d_vec, (size_t)cnt));
+        HIP_TRY(hipMemcpy(P->d_mv, P->h_mv.data(), sizeof(MvInst) * cnt, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P->d_vec, P->h_vec.data(), sizeof(BatchVec) * cnt, hipMemcpyHostToDevice));
+        HIP_TRY(dalloc(&P->d_norms, (size_t)2 * cnt * P->maxnrhs));
+        P->h_norms.assign((size_t)2 * cnt * P->maxnrhs, 0.0);
+    }
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_plan_create(int32_t count, const hss_fact_t* facts, const int32_t* nrhs,
+                                      const double* const* dB, const int64_t* ldb, double* const* dX,
+                                      const int64_t* ldx, int32_t with_refinement, hss_plan_t* out) {
+    if (count <= 0 || !facts || !nrhs || !dB || !ldb || !dX || !ldx || !out || !facts[0])
+        return fail(HSS_ERR_ARG, "bad plan arguments");
+    *out = nullptr;
+    hss_ctx_s* ctx = facts[0]->mat->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const hss_prec prec = facts[0]->prec;
+    auto* P = new (std::nothrow) hss_plan_s;
+    if (!P) return fail(HSS_ERR_OOM, "host allocation");
+    P->ctx = ctx;
+    P->prec = prec;
+    P->has_ir = with_refinement != 0;
+    for (int i = 0; i < count; ++i) {
+        hss_fact_s* F = facts[i];
+        if (!F || F->mat->ctx != ctx || F->prec != prec) {
+            free_plan(P);
+            return fail(HSS_ERR_ARG, "planned factors must share one context and one precision");
+        }
+        if (nrhs[i] <= 0 || ldb[i] < F->mat->rows || ldx[i] < F->mat->rows) {
+            free_plan(P);
+            return fail(HSS_ERR_DIM, "bad nrhs / leading dimension in request " + std::to_string(i));
+        }
+        P->reqs.push_back(SolveReq{F, nrhs[i], dB[i], ldb[i], dX[i], ldx[i]});
+        P->maxnrhs = std::max(P->maxnrhs, (int)nrhs[i]);
+        P->lsup = std::max(P->lsup, F->lds_sup);
+        P->lsroot = std::max(P->lsroot, F->lds_sroot);
+        P->lsdown = std::max(P->lsdown, F->lds_sdown);
+        P->lmv_up = std::max(P->lmv_up, F->mat->lds_up);
+        P->lmv_down = std::max(P->lmv_down, F->mat->lds_down);
+        P->lmv_leaf = std::max(P->lmv_leaf, F->mat->lds_leaf);
+    }
+    P->cnt = (int)P->reqs.size();
+    std::vector<hss_mat_s*> mats;
+    for (auto& q : P->reqs) mats.push_back(q.F->mat);
+    build_schedule(mats, P->S);
+    // matvec schedule appended after the solve schedule
+    std::vector<WorkItem> work = P->S.work;
+    if (P->has_ir) {
+        int maxh = 0, maxd = 0;
+        for (auto* M : mats) {
+            maxh = std::max(maxh, M->max_height);
+            maxd = std::max(maxd, M->max_depth);
+        }
+        P->mv_up.assign(maxh, Range{});
+        for (int h = 0; h < maxh; ++h) {
+            P->mv_up[h].off = (int)work.size();
+            for (int b = 0; b < P->cnt; ++b) {
+                hss_mat_s* M = mats[b];
+                if (h < M->max_height) {
+                    const Range& r = M->by_height[h];
+                    for (int t = 0; t < r.count; ++t) work.push_back(WorkItem{b, M->hwork[r.off + t].node});
+                }
+            }
+            P->mv_up[h].count = (int)work.size() - P->mv_up[h].off;
+        }
+        P->mv_down.assign(maxd + 1, Range{});
+        for (int d = 0; d <= maxd; ++d) {
+            P->mv_down[d].off = (int)work.size();
+            for (int b = 0; b < P->cnt; ++b) {
+                hss_mat_s* M = mats[b];
+                if (d <= M->max_depth) {
+                    const Range& r = M->br_depth[d];
+                    for (int t = 0; t < r.count; ++t) work.push_back(WorkItem{b, M->hwork[r.off + t].node});
+                }
+            }
+            P->mv_down[d].count = (int)work.size() - P->mv_down[d].off;
+        }
+        P->mv_leaf.off = (int)work.size();
+        for (int b = 0; b < P->cnt; ++b) {
+            hss_mat_s* M = mats[b];
+            const Range& r = M->leaves;
+            for (int t = 0; t < r.count; ++t) work.push_back(WorkItem{b, M->hwork[r.off + t].node});
+        }
+        P->mv_leaf.count = (int)work.size() - P->mv_leaf.off;
+    }
+    hipError_t e = dalloc(&P->d_work, work.size());
+    if (e == hipSuccess) e = hipMemcpy(P->d_work, work.data(), sizeof(WorkItem) * work.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_plan(P);
+        return fail(HSS_ERR_OOM, std::string("plan tables: ") + hipGetErrorString(e));
+    }
+    hss_status s = (prec == HSS_PREC_FP32) ? plan_tables<float>(P) : plan_tables<double>(P);
+    if (s != HSS_OK) {
+        free_plan(P);
+        return s;
+    }
+    *out = P;
+    return HSS_OK;
+}
+
+template <typename TF>
+static hss_status plan_solve(hss_plan_s* P, int t) {
+    using TV = TF;
+    return launch_schedule<TF, TV>(reinterpret_cast<const SInst<TF, TV>*>(P->d_sinst[t]), P->d_work, P->S.up,
+                                   P->S.roots, P->S.down, P->maxnrhs, P->lsup, P->lsroot, P->lsdown, P->ctx->stream);
+}
+
+static hss_status plan_matvec(hss_plan_s* P) {
+    hipStream_t st = P->ctx->stream;
+    for (size_t h = 0; h < P->mv_up.size(); ++h) {
+        ProfScope ps("k_mv_up", (int)h, P->mv_up[h].count, st);
+        HIP_TRY(launch_mv_up(P->d_mv, P->d_work + P->mv_up[h].off, P->mv_up[h].count, P->maxnrhs, P->lmv_up, st));
+    }
+    for (size_t d = 0; d < P->mv_down.size(); ++d) {
+        ProfScope ps("k_mv_down", (int)d, P->mv_down[d].count, st);
+        HIP_TRY(launch_mv_down(P->d_mv, P->d_work + P->mv_down[d].off, P->mv_down[d].count, P->maxnrhs, P->lmv_down,
+                               st));
+    }
+    ProfScope ps("k_mv_leaf", 0, P->mv_leaf.count, st);
+    HIP_TRY(launch_mv_leaf(P->d_mv, P->d_work + P->mv_leaf.off, P->mv_leaf.count, P->maxnrhs, P->lmv_leaf, st));
+    return HSS_OK;
+}
+
+// re-upload the tables whose nrhs changed (the stream is idle: callers synchronize first)
+using SInstF = SInst<float, float>;
+using SInstD = SInst<double, double>;
+
+static hss_status plan_set_active(hss_plan_s* P, const std::vector<char>& active) {
+    const size_t nrhs_off = (P->prec == HSS_PREC_FP32) ? offsetof(SInstF, nrhs) : offsetof(SInstD, nrhs);
+    for (int t = 0; t < 2; ++t) {
+        if (!P->d_sinst[t]) continue;
+        for (int i = 0; i < P->cnt; ++i) {
+            int32_t* nr = reinterpret_cast<int32_t*>(P->h_sinst[t].data() + i * P->sinst_size + nrhs_off);
+            *nr = active[i] ? P->reqs[i].nrhs : 0;
+        }
+        HIP_TRY(hipMemcpy(P->d_sinst[t], P->h_sinst[t].data(), P->h_sinst[t].size(), hipMemcpyHostToDevice));
+    }
+    for (int i = 0; i < P->cnt; ++i) {
+        P->h_mv[i].nrhs = active[i] ? P->reqs[i].nrhs : 0;
+        P->h_vec[i].nrhs = active[i] ? P->reqs[i].nrhs : 0;
+    }
+    HIP_TRY(hipMemcpy(P->d_mv, P->h_mv.data(), sizeof(MvInst) * P->cnt, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P->d_vec, P->h_vec.data(), sizeof(BatchVec) * P->cnt, hipMemcpyHostToDevice));
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_plan_execute(hss_plan_t P, const hss_refine_opts* opts, hss_solve_report* rep) {
+    if (!P) return fail(HSS_ERR_ARG, "null plan");
+    HIP_TRY(hipSetDevice(P->ctx->device));
+    hipStream_t st = P->ctx->stream;
+    const bool fp32 = P->prec == HSS_PREC_FP32;
+    if (rep) { rep->iterations = 0; rep->rel_residual = -1.0; rep->seconds = 0.0; }
+    HIP_TRY(hipEventRecord(P->ctx->ev0, st));
+    const bool refine = P->has_ir && opts && opts->mode == 1;
+    if (!refine && !g_prof && !fp32) {
+        // Float64 solve: replay one captured graph of all level launches
+        if (!P->gexec) {
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            hss_status s = plan_solve<double>(P, 0);
+            hipGraph_t g = nullptr;
+            hipError_t ee = hipStreamEndCapture(st, &g);
+            if (s != HSS_OK) { if (g) (void)hipGraphDestroy(g); return s; }
+            HIP_TRY(ee);
+            P->graph = g;
+            HIP_TRY(hipGraphInstantiate(&P->gexec, P->graph, nullptr, nullptr, 0));
+        }
+        HIP_TRY(hipGraphLaunch(P->gexec, st));
+        HIP_TRY(hipEventRecord(P->ctx->ev1, st));
+        return HSS_OK;
+    }
+    hss_status s = fp32 ? plan_solve<float>(P, 0) : plan_solve<double>(P, 0);
+    if (s != HSS_OK) return s;
+    if (!refine) {
+        HIP_TRY(hipEventRecord(P->ctx->ev1, st));
+        return HSS_OK;
+    }
+    std::vector<char> active(P->cnt, 1);
+    double worst = 0.0;
+    int it = 0;
+    for (;;) {
+        s = plan_matvec(P);  // r = A x
+        if (s != HSS_OK) return s;
+        HIP_TRY(launch_resid_batch(P->d_vec, P->cnt, P->maxnrhs, P->d_norms, st));
+        HIP_TRY(hipMemcpyAsync(P->h_norms.data(), P->d_norms, sizeof(double) * P->h_norms.size(),
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        worst = 0.0;
+        bool any = false;
+        for (int i = 0; i < P->cnt; ++i) {
+            if (!active[i]) continue;
+            double rel = 0.0;
+            for (int j = 0; j < P->reqs[i].nrhs; ++j) {
+                const double nb = std::sqrt(P->h_norms[2 * ((size_t)i * P->maxnrhs + j)]);
+                const double nr = std::sqrt(P->h_norms[2 * ((size_t)i * P->maxnrhs + j) + 1]);
+                rel = std::max(rel, nb > 0 ? nr / nb : nr);
+            }
+            worst = std::max(worst, rel);
+            if (rel <= opts->tol) active[i] = 0;
+            else any = true;
+        }
+        if (!any || it >= opts->max_iter) break;
+        s = plan_set_active(P, active);
+        if (s != HSS_OK) return s;
+        s = fp32 ? plan_solve<float>(P, 1) : plan_solve<double>(P, 1);  // d = A \ r
+        if (s != HSS_OK) return s;
+        HIP_TRY(launch_axpy_batch(P->d_vec, P->cnt, P->maxnrhs, st));  // x += d
+        ++it;
+    }
+    // restore all requests as active for the next execution
+    std::vector<char> all(P->cnt, 1);
+    s = plan_set_active(P, all);
+    if (s != HSS_OK) return s;
+    HIP_TRY(hipEventRecord(P->ctx->ev1, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, P->ctx->ev0, P->ctx->ev1) == hipSuccess) g_last_ms = ms;
+    if (rep) {
+        rep->iterations = it;
+        rep->rel_residual = worst;
+        rep->seconds = ms * 1e-3;
+    }
+    if (worst > opts->tol) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "batched refinement stopped at relative residual %.3e > tol %.3e after %d steps",
+                      worst, opts->tol, it);
+        return fail(HSS_ERR_NOCONV, buf);
+    }
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_plan_destroy(hss_plan_t P) {
+    if (!P) return HSS_OK;
+    (void)hipSetDevice(P->ctx->device);
+    (void)hipStreamSynchronize(P->ctx->stream);
+    free_plan(P);
+    return HSS_OK;
+}
+
+extern "C" int64_t hss_plan_device_bytes(hss_plan_t P) { return P ? P->dev_bytes : 0; }
+
+// Batched solves without refinement: a one-shot plan.
 extern "C" hss_status hss_ulv_solve_batch_dev(int32_t count, const hss_fact_t* facts, const int32_t* nrhs,
                                               const double* const* dB, const int64_t* ldb, double* const* dX,
                                               const int64_t* ldx) {
     if (count <= 0 || !facts || !nrhs || !dB || !ldb || !dX || !ldx || !facts[0])
         return fail(HSS_ERR_ARG, "bad batch arguments");
-    hss_ctx_s* ctx = facts[0]->mat->ctx;
-    HIP_TRY(hipSetDevice(ctx->device));
-    const hss_prec prec = facts[0]->prec;
-    std::vector<SolveReq> reqs;
+    std::vector<hss_fact_t> f;
+    std::vector<int32_t> nr;
+    std::vector<const double*> b;
+    std::vector<double*> x;
+    std::vector<int64_t> lb, lx;
     for (int i = 0; i < count; ++i) {
-        hss_fact_s* F = facts[i];
-        if (!F || F->mat->ctx != ctx || F->prec != prec)
-            return fail(HSS_ERR_ARG, "batched factors must share one context and one precision");
-        for (int j = 0; j < i; ++j)
-            if (facts[j] == F) return fail(HSS_ERR_ARG, "a factor may appear only once per batch");
-        if (nrhs[i] < 0 || ldb[i] < F->mat->rows || ldx[i] < F->mat->rows)
-            return fail(HSS_ERR_DIM, "bad nrhs / leading dimension in batch request " + std::to_string(i));
+        if (nrhs[i] < 0) return fail(HSS_ERR_ARG, "nrhs < 0");
         if (nrhs[i] == 0) continue;
-        reqs.push_back(SolveReq{F, nrhs[i], dB[i], ldb[i], dX[i], ldx[i]});
+        f.push_back(facts[i]);
+        nr.push_back(nrhs[i]);
+        b.push_back(dB[i]);
+        x.push_back(dX[i]);
+        lb.push_back(ldb[i]);
+        lx.push_back(ldx[i]);
     }
-    if (reqs.empty()) return HSS_OK;
-    hipStream_t st = ctx->stream;
-    int maxnrhs = 0;
-    size_t lsup = 0, lsroot = 0, lsdown = 0;
-    std::vector<hss_mat_s*> mats;
-    for (const SolveReq& q : reqs) {
-        hss_status s = (prec == HSS_PREC_FP32) ? ensure_solve_ws<float>(q.F, q.nrhs) : ensure_solve_ws<double>(q.F, q.nrhs);
-        if (s != HSS_OK) return s;
-        mats.push_back(q.F->mat);
-        maxnrhs = std::max(maxnrhs, q.nrhs);
-        lsup = std::max(lsup, q.F->lds_sup);
-        lsroot = std::max(lsroot, q.F->lds_sroot);
-        lsdown = std::max(lsdown, q.F->lds_sdown);
-    }
-    Schedule S;
-    build_schedule(mats, S);
-    // tables of the previous batch may still be read: wait, then upload synchronously
-    static thread_local char* tbl = nullptr;
-    static thread_local size_t tbl_cap = 0;
-    const size_t isz = (prec == HSS_PREC_FP32 ? sizeof(SInst<float, float>) : sizeof(SInst<double, double>)) * reqs.size();
-    const size_t wsz = sizeof(WorkItem) * S.work.size();
-    const size_t need = ((isz + 255) / 256) * 256 + wsz;
-    HIP_TRY(hipStreamSynchronize(st));
-    if (need > tbl_cap) {
-        (void)hipFree(tbl);
-        tbl = nullptr;
-        tbl_cap = 0;
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&tbl), need));
-        tbl_cap = need;
-    }
-    WorkItem* dw = reinterpret_cast<WorkItem*>(tbl + ((isz + 255) / 256) * 256);
-    HIP_TRY(hipMemcpy(dw, S.work.data(), wsz, hipMemcpyHostToDevice));
-    HIP_TRY(hipEventRecord(ctx->ev0, st));
-    if (prec == HSS_PREC_FP32) {
-        std::vector<SInst<float, float>> hi;
-        for (const SolveReq& q : reqs) hi.push_back(make_sinst<float, float>(q));
-        HIP_TRY(hipMemcpy(tbl, hi.data(), isz, hipMemcpyHostToDevice));
-        hss_status s = launch_schedule<float, float>(reinterpret_cast<const SInst<float, float>*>(tbl), dw, S.up,
-                                                     S.roots, S.down, maxnrhs, lsup, lsroot, lsdown, st);
-        if (s != HSS_OK) return s;
-    } else {
-        std::vector<SInst<double, double>> hi;
-        for (const SolveReq& q : reqs) hi.push_back(make_sinst<double, double>(q));
-        HIP_TRY(hipMemcpy(tbl, hi.data(), isz, hipMemcpyHostToDevice));
-        hss_status s = launch_schedule<double, double>(reinterpret_cast<const SInst<double, double>*>(tbl), dw, S.up,
-                                                       S.roots, S.down, maxnrhs, lsup, lsroot, lsdown, st);
-        if (s != HSS_OK) return s;
-    }
-    HIP_TRY(hipEventRecord(ctx->ev1, st));
-    return HSS_OK;
+    if (f.empty()) return HSS_OK;
+    hss_plan_t P = nullptr;
+    hss_status s = hss_plan_create((int32_t)f.size(), f.data(), nr.data(), b.data(), lb.data(), x.data(), lx.data(), 0, &P);
+    if (s != HSS_OK) return s;
+    hss_refine_opts none{0.0, 0, 0};
+    // without capture: a one-shot plan gains nothing from a graph
+    hipStream_t st = P->ctx->stream;
+    s = (P->prec == HSS_PREC_FP32) ? plan_solve<float>(P, 0) : plan_solve<double>(P, 0);
+    (void)none;
+    (void)hipStreamSynchronize(st);
+    hss_plan_destroy(P);
+    return s;
 }
 
 // ---------------------------------------------------------------------------

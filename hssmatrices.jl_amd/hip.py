@@ -47,6 +47,7 @@ EXPORTS = [
     "hss_ulv_solve", "hss_ulv_solve_dev", "hss_ulv_solve_batch_dev", "hss_ulv_factor_batch",
     "hss_last_error", "hss_last_device_ms",
     "hss_prof_enable", "hss_prof_reset", "hss_prof_count", "hss_prof_get", "hss_fact_node_info",
+    "hss_plan_create", "hss_plan_execute", "hss_plan_destroy", "hss_plan_device_bytes",
 ]
 
 
@@ -111,6 +112,10 @@ def load_library(path: pathlib.Path | str | None = None):
         "hss_prof_get": ([I32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(I32), ctypes.POINTER(I32),
                           ctypes.POINTER(D)], ctypes.c_int),
         "hss_fact_node_info": ([V, I32, ctypes.POINTER(I32)], ctypes.c_int),
+        "hss_plan_create": ([I32, V, V, V, V, V, V, I32, PV], ctypes.c_int),
+        "hss_plan_execute": ([V, ctypes.POINTER(RefineOpts), ctypes.POINTER(SolveReport)], ctypes.c_int),
+        "hss_plan_destroy": ([V], ctypes.c_int),
+        "hss_plan_device_bytes": ([V], I64),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -405,3 +410,49 @@ def solve_batch_dev(facts, nrhs, dB, ldb, dX, ldx):
     _check(lib.hss_ulv_solve_batch_dev(cnt, ctypes.cast(hs, ctypes.c_void_p), ctypes.cast(nr, ctypes.c_void_p),
                                        ctypes.cast(b, ctypes.c_void_p), ctypes.cast(lb, ctypes.c_void_p),
                                        ctypes.cast(x, ctypes.c_void_p), ctypes.cast(lx, ctypes.c_void_p)))
+
+
+class Plan:
+    """Batched solve plan (``hss_plan_create``): one launch per tree level across all requests.
+
+    ``facts`` may repeat and differ in size; ``dB``/``dX`` are device pointers
+    (column-major, leading dimensions ``ldb``/``ldx``) fixed for the plan's life.
+    """
+
+    def __init__(self, facts, nrhs, dB, ldb, dX, ldx, refinement: bool = False):
+        lib = load_library()
+        cnt = len(facts)
+        self.facts = list(facts)  # keep the factors alive
+        hs = (ctypes.c_void_p * cnt)(*[f.handle.value for f in facts])
+        nr = (ctypes.c_int32 * cnt)(*nrhs)
+        b = (ctypes.c_void_p * cnt)(*dB)
+        lb = (ctypes.c_int64 * cnt)(*ldb)
+        x = (ctypes.c_void_p * cnt)(*dX)
+        lx = (ctypes.c_int64 * cnt)(*ldx)
+        h = ctypes.c_void_p()
+        _check(lib.hss_plan_create(cnt, ctypes.cast(hs, ctypes.c_void_p), ctypes.cast(nr, ctypes.c_void_p),
+                                   ctypes.cast(b, ctypes.c_void_p), ctypes.cast(lb, ctypes.c_void_p),
+                                   ctypes.cast(x, ctypes.c_void_p), ctypes.cast(lx, ctypes.c_void_p),
+                                   1 if refinement else 0, ctypes.byref(h)))
+        self._h = h
+
+    def execute(self, tol: float | None = None, max_iter: int = 10):
+        """Enqueue the batched solve; with ``tol`` refine in lockstep (synchronous)."""
+        opts = RefineOpts(tol, max_iter, 1) if tol is not None else None
+        rep = SolveReport()
+        _check(_lib.hss_plan_execute(self._h, ctypes.byref(opts) if opts is not None else None, ctypes.byref(rep)))
+        return {"iterations": rep.iterations, "rel_residual": rep.rel_residual, "seconds": rep.seconds}
+
+    def device_bytes(self) -> int:
+        return int(_lib.hss_plan_device_bytes(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.hss_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

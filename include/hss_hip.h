@@ -142,6 +142,20 @@ hss_status hss_ulv_solve_dev(hss_fact_t fact, int32_t nrhs, const double* dB, in
 hss_status hss_ulv_solve_batch_dev(int32_t count, const hss_fact_t* facts, const int32_t* nrhs,
                                    const double* const* dB, const int64_t* ldb, double* const* dX,
                                    const int64_t* ldx);
+/* Batch plan: requests i = 0..count-1 solve facts[i] (facts may repeat, sizes may differ) with
+ * nrhs[i] columns from dB[i] into dX[i] (device pointers, fixed for the plan's lifetime).  Every
+ * execution issues one launch per tree level across all requests; without refinement the launch
+ * sequence is captured once in a HIP graph and replayed.  With with_refinement != 0 an execution
+ * may refine (opts->mode == 1) in lockstep: one batched Float64 HSS matvec per step, requests
+ * leave the batch as they reach opts->tol.  No counterpart in the reference (configs C4/C5). */
+typedef struct hss_plan_s* hss_plan_t;
+hss_status hss_plan_create(int32_t count, const hss_fact_t* facts, const int32_t* nrhs, const double* const* dB,
+                           const int64_t* ldb, double* const* dX, const int64_t* ldx, int32_t with_refinement,
+                           hss_plan_t* out);
+hss_status hss_plan_execute(hss_plan_t plan, const hss_refine_opts* opts, hss_solve_report* report);
+hss_status hss_plan_destroy(hss_plan_t plan);
+int64_t hss_plan_device_bytes(hss_plan_t plan);
+
 /* Batched factorization of many matrices (one launch per level across all of them). */
 hss_status hss_ulv_factor_batch(int32_t count, const hss_mat_t* mats, hss_prec prec, hss_fact_t* out);
 

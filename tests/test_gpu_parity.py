@@ -206,3 +206,77 @@ def test_c2_full_size(hip, oracle):
     with threadpool_limits(1):
         Zo = oracle.ulvfactsolve(nodes, blob, B)
     assert rel(Z, Zo) < 1e-9
+
+
+def _batch_inputs(hip, rng, specs, prec):
+    import torch
+    mats, packs = [], []
+    for n, leaf in specs:
+        _, h = random_hss(rng, n, leaf, 0, "cauchy", 1e-8)
+        packs.append(H.pack(h))
+        mats.append(hip.HipHssMatrix(packs[-1]))
+    facts = hip.ulvfactor_batch(mats, prec)
+    return mats, packs, facts
+
+
+def test_plan_repeated_factors_and_graph_replay(hip, oracle):
+    """A plan may name one factor several times; graph replays equal the eager solve."""
+    import torch
+    rng = np.random.default_rng(21)
+    mats, packs, facts = _batch_inputs(hip, rng, [(900, 64), (400, 32), (1300, 64)], "fp64")
+    dev = torch.device("cuda:0")
+    order = [0, 1, 0, 2, 1]  # factors 0 and 1 appear twice
+    Bs, Xs, nrhs = [], [], []
+    for t, i in enumerate(order):
+        n = int(packs[i][0]["n"][0])
+        r = 2 + 5 * t
+        Bs.append(torch.from_numpy(np.ascontiguousarray(rng.standard_normal((r, n)))).to(dev))
+        Xs.append(torch.zeros((r, n), dtype=torch.float64, device=dev))
+        nrhs.append(r)
+    plan = hip.Plan([facts[i] for i in order], nrhs, [b.data_ptr() for b in Bs], [int(b.shape[1]) for b in Bs],
+                    [x.data_ptr() for x in Xs], [int(x.shape[1]) for x in Xs])
+    plan.execute()
+    torch.cuda.synchronize()
+    first = [x.cpu().numpy().copy() for x in Xs]
+    for rep in range(3):  # replays of the captured graph, with host work between them
+        for x in Xs:
+            x.zero_()
+        torch.cuda.synchronize()
+        plan.execute()
+        torch.cuda.synchronize()
+        for x, f0 in zip(Xs, first):
+            assert np.array_equal(x.cpu().numpy(), f0)
+    for t, i in enumerate(order):
+        nodes, blob = packs[i]
+        B = Bs[t].cpu().numpy().T
+        X = first[t].T
+        assert np.array_equal(X, mats[i].ulvfactor("fp64").solve(B))
+        assert rel(X, oracle.ulvfactsolve(nodes, blob, B)) < 1e-11
+    plan.close()
+
+
+def test_plan_mixed_precision_lockstep_refinement(hip, oracle):
+    import torch
+    rng = np.random.default_rng(22)
+    mats, packs, facts = _batch_inputs(hip, rng, [(1024, 64), (700, 64), (2000, 64)], "fp32")
+    dev = torch.device("cuda:0")
+    Bs, Xs, nrhs = [], [], []
+    for i, (nodes, blob) in enumerate(packs):
+        n = int(nodes["n"][0])
+        r = 1 + 3 * i
+        Bs.append(torch.from_numpy(np.ascontiguousarray(rng.standard_normal((r, n)))).to(dev))
+        Xs.append(torch.zeros((r, n), dtype=torch.float64, device=dev))
+        nrhs.append(r)
+    plan = hip.Plan(facts, nrhs, [b.data_ptr() for b in Bs], [int(b.shape[1]) for b in Bs],
+                    [x.data_ptr() for x in Xs], [int(x.shape[1]) for x in Xs], refinement=True)
+    rep = plan.execute(tol=1e-10, max_iter=20)
+    assert rep["rel_residual"] <= 1e-10
+    for i, (nodes, blob) in enumerate(packs):
+        B = Bs[i].cpu().numpy().T
+        X = Xs[i].cpu().numpy().T
+        R = oracle.matmul(nodes, blob, X) - B
+        assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)) <= 1e-10
+    # a second execution starts again from the right-hand sides
+    rep2 = plan.execute(tol=1e-10, max_iter=20)
+    assert rep2["rel_residual"] <= 1e-10
+    plan.close()
