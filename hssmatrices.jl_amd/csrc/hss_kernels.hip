@@ -352,7 +352,8 @@ __global__ __launch_bounds__(FACT_THREADS) void k_ulv_factor(const FInst<T>* __r
 
     // LDS: Us (m x ku) | G (k x k) | Tq (k x k) | M (k rows of vstride) | vb (2 x vstride) | taus | sc
     extern __shared__ unsigned char lds_raw[];
-    const int vstride = vpad<CPT>(n) + 2;
+    // every row chunk of the S threads is CPT wide (columns >= n are zero padding)
+    const int vstride = S * (CPT + 2) + 2;
     T* Us = reinterpret_cast<T*>(lds_raw);
     T* G = Us + (size_t)m * ku;
     T* Tq = G + (size_t)k * k;

@@ -137,6 +137,18 @@ struct SlotCache {
     }
 };
 
+struct PlanKey {
+    int nrhs;
+    const double* dB;
+    int64_t ldb;
+    double* dX;
+    int64_t ldx;
+    bool operator==(const PlanKey& o) const {
+        return nrhs == o.nrhs && dB == o.dB && ldb == o.ldb && dX == o.dX && ldx == o.ldx;
+    }
+};
+static void destroy_plan_cache(std::vector<std::pair<PlanKey, hss_plan_t>>& cache);
+
 struct hss_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -195,6 +207,8 @@ struct hss_fact_s {
     double *d_io_b = nullptr, *d_io_x = nullptr;
     size_t io_cap = 0;
     int64_t dev_bytes = 0;
+    // captured single-request plans keyed by their buffers (repeated solves replay a graph)
+    std::vector<std::pair<PlanKey, hss_plan_t>> plan_cache;
 };
 
 // ---------------------------------------------------------------------------
@@ -525,7 +539,7 @@ template <typename TF>
 static size_t factor_lds(const FNode& f, const DevNode& d) {
     // Us (m x ku) | G, T (k x k each) | M (k x n) | two padded reflector rows | taus | scalars
     const size_t kk = (size_t)std::max(f.k, f.nk) + 1;
-    const size_t vstride = (size_t)f.n + 2 * ((f.n + FACT_CPT - 1) / FACT_CPT) + 2;
+    const size_t vstride = (size_t)f.S * (FACT_CPT + 2) + 2;  // S chunks of CPT columns, padded
     return sizeof(TF) * ((size_t)f.m * d.ku + 2 * (size_t)f.k * f.k + ((size_t)f.k + 2) * vstride + kk + 8);
 }
 
@@ -636,6 +650,7 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
 
 static void free_fact(hss_fact_s* F) {
     if (!F) return;
+    destroy_plan_cache(F->plan_cache);
     (void)hipFree(F->d_fn);
     (void)hipFree(F->d_fac);
     (void)hipFree(F->d_piv);
@@ -998,6 +1013,26 @@ static hss_status solve_dev_impl(hss_fact_s* F, int nrhs, const double* dB, int6
         return fp32 ? solve_one<float, float>(F, nrhs, b, lb, x, lx, st)
                     : solve_one<double, double>(F, nrhs, b, lb, x, lx, st);
     };
+    if (o.mode != 1 && !g_prof && !fp32) {
+        // repeated unrefined solves on the same buffers replay a captured graph
+        const PlanKey key{nrhs, dB, ldb, dX, ldx};
+        hss_plan_t P = nullptr;
+        for (auto& e : F->plan_cache)
+            if (e.first == key) P = e.second;
+        if (!P) {
+            if (F->plan_cache.size() >= 4) {
+                std::vector<std::pair<PlanKey, hss_plan_t>> old(F->plan_cache.begin(), F->plan_cache.begin() + 1);
+                destroy_plan_cache(old);
+                F->plan_cache.erase(F->plan_cache.begin());
+            }
+            hss_fact_t f1 = F;
+            int32_t nr = nrhs;
+            hss_status cs = hss_plan_create(1, &f1, &nr, &dB, &ldb, &dX, &ldx, 0, &P);
+            if (cs != HSS_OK) return cs;
+            F->plan_cache.push_back({key, P});
+        }
+        return hss_plan_execute(P, nullptr, nullptr);
+    }
     HIP_TRY(hipEventRecord(M->ctx->ev0, st));
     hss_status s = solve(dB, ldb, dX, ldx);
     if (s != HSS_OK) return s;
@@ -1455,6 +1490,11 @@ extern "C" hss_status hss_plan_destroy(hss_plan_t P) {
 
 extern "C" int64_t hss_plan_device_bytes(hss_plan_t P) { return P ? P->dev_bytes : 0; }
 
+static void destroy_plan_cache(std::vector<std::pair<PlanKey, hss_plan_t>>& cache) {
+    for (auto& e : cache) hss_plan_destroy(e.second);
+    cache.clear();
+}
+
 // Batched solves without refinement: a one-shot plan.
 extern "C" hss_status hss_ulv_solve_batch_dev(int32_t count, const hss_fact_t* facts, const int32_t* nrhs,
                                               const double* const* dB, const int64_t* ldb, double* const* dX,
@@ -1488,6 +1528,366 @@ extern "C" hss_status hss_ulv_solve_batch_dev(int32_t count, const hss_fact_t* f
     (void)hipStreamSynchronize(st);
     hss_plan_destroy(P);
     return s;
+}
+
+// ---------------------------------------------------------------------------
+// serving: resident factorizations admitted by an HBM budget, and continuous
+// batching of arriving solves.  Each round batches, for every request in
+// flight, either its first solve (b -> x) or its next refinement correction
+// (r -> d, x += d) into one launch per tree level, followed by one batched
+// Float64 matvec + residual for all of them; requests leave as soon as they
+// reach the tolerance and queued requests take their slots in the next round.
+// ---------------------------------------------------------------------------
+struct ServeReq {
+    int64_t ticket = 0;
+    int mid = 0;
+    int nrhs = 0;
+    const double* dB = nullptr;
+    int64_t ldb = 0;
+    double* dX = nullptr;
+    int64_t ldx = 0;
+    double* r = nullptr;  // n x nrhs residual / right-hand side of the correction
+    double* d = nullptr;  // n x nrhs correction
+    size_t rbytes = 0;
+    int steps = 0;        // solves done so far (1 = first solve)
+    double rel = -1.0;
+    int status = 0;       // 0 pending/in flight, 1 converged, 2 stopped at max_iter
+    std::chrono::steady_clock::time_point t_submit, t_done;
+};
+
+struct hss_server_s {
+    hss_ctx_s* ctx = nullptr;
+    hss_prec prec = HSS_PREC_FP32;
+    hss_refine_opts opts{1e-10, 10, 1};
+    int64_t budget = 0, used = 0;
+    int max_inflight = 64;
+    std::vector<hss_fact_s*> facts;  // by matrix id (nullptr after eviction)
+    std::vector<int64_t> fact_bytes;
+    std::vector<ServeReq*> pending, inflight;
+    std::vector<ServeReq*> finished;  // kept until polled
+    int64_t next_ticket = 1;
+    // per-round device scratch (grown on demand)
+    char* d_round = nullptr;
+    size_t round_cap = 0;
+    char* d_tab = nullptr;
+    size_t tab_cap = 0;
+    double* d_norms = nullptr;
+    size_t norms_cap = 0;
+    int64_t rounds = 0, solved = 0;
+};
+
+static hss_status grow(char** p, size_t* cap, size_t need) {
+    if (need <= *cap) return HSS_OK;
+    (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    size_t c = std::max(need, (size_t)1 << 20);
+    c = std::max(c, need + need / 2);
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(p), c));
+    *cap = c;
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_server_create(hss_ctx_t ctx, int64_t hbm_budget_bytes, hss_prec prec,
+                                        const hss_refine_opts* opts, int32_t max_inflight, hss_server_t* out) {
+    if (!ctx || !out || hbm_budget_bytes <= 0 || max_inflight <= 0) return fail(HSS_ERR_ARG, "bad server arguments");
+    auto* S = new (std::nothrow) hss_server_s;
+    if (!S) return fail(HSS_ERR_OOM, "host allocation");
+    S->ctx = ctx;
+    S->prec = prec;
+    if (opts) S->opts = *opts;
+    S->budget = hbm_budget_bytes;
+    S->max_inflight = max_inflight;
+    *out = S;
+    return HSS_OK;
+}
+
+// Factor `mat` (server precision) and keep it resident, if the budget allows.
+extern "C" hss_status hss_server_admit(hss_server_t S, hss_mat_t mat, int32_t* id) {
+    if (!S || !mat || !id) return fail(HSS_ERR_ARG, "null argument");
+    if (mat->ctx != S->ctx) return fail(HSS_ERR_ARG, "matrix belongs to another context");
+    // admission estimate before factoring: the factor is about as large as the generators
+    const int64_t estimate = mat->dev_bytes * (S->prec == HSS_PREC_FP32 ? 1 : 2);
+    if (S->used + estimate > S->budget)
+        return fail(HSS_ERR_BUDGET, "admission refused: " + std::to_string(S->used + estimate) + " bytes > budget " +
+                                        std::to_string(S->budget));
+    hss_fact_t F = nullptr;
+    hss_status st = hss_ulv_factor(mat, S->prec, &F);
+    if (st != HSS_OK) return st;
+    const int64_t bytes = F->dev_bytes;
+    if (S->used + bytes > S->budget) {
+        hss_fact_destroy(F);
+        return fail(HSS_ERR_BUDGET, "admission refused after factorization: over the HBM budget");
+    }
+    S->used += bytes;
+    S->facts.push_back(F);
+    S->fact_bytes.push_back(bytes);
+    *id = (int32_t)S->facts.size() - 1;
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_server_evict(hss_server_t S, int32_t id) {
+    if (!S || id < 0 || id >= (int32_t)S->facts.size() || !S->facts[id]) return fail(HSS_ERR_ARG, "bad matrix id");
+    for (auto* q : S->pending)
+        if (q->mid == id) return fail(HSS_ERR_ARG, "matrix has queued requests");
+    for (auto* q : S->inflight)
+        if (q->mid == id) return fail(HSS_ERR_ARG, "matrix has requests in flight");
+    hss_fact_destroy(S->facts[id]);
+    S->facts[id] = nullptr;
+    S->used -= S->fact_bytes[id];
+    return HSS_OK;
+}
+
+extern "C" int64_t hss_server_used_bytes(hss_server_t S) { return S ? S->used : 0; }
+
+extern "C" hss_status hss_server_submit(hss_server_t S, int32_t id, int32_t nrhs, const double* dB, int64_t ldb,
+                                        double* dX, int64_t ldx, int64_t* ticket) {
+    if (!S || !ticket || id < 0 || id >= (int32_t)S->facts.size() || !S->facts[id])
+        return fail(HSS_ERR_ARG, "bad submit arguments");
+    hss_mat_s* M = S->facts[id]->mat;
+    if (nrhs <= 0 || ldb < M->rows || ldx < M->rows) return fail(HSS_ERR_DIM, "bad nrhs / leading dimension");
+    auto* q = new (std::nothrow) ServeReq;
+    if (!q) return fail(HSS_ERR_OOM, "host allocation");
+    q->ticket = S->next_ticket++;
+    q->mid = id;
+    q->nrhs = nrhs;
+    q->dB = dB;
+    q->ldb = ldb;
+    q->dX = dX;
+    q->ldx = ldx;
+    q->t_submit = std::chrono::steady_clock::now();
+    S->pending.push_back(q);
+    *ticket = q->ticket;
+    return HSS_OK;
+}
+
+template <typename TF>
+static hss_status server_round(hss_server_s* S) {
+    using TV = TF;
+    hipStream_t st = S->ctx->stream;
+    // 1. admit queued requests into free slots (their r/d vectors come out of the budget)
+    while (!S->pending.empty() && (int)S->inflight.size() < S->max_inflight) {
+        ServeReq* q = S->pending.front();
+        hss_mat_s* M = S->facts[q->mid]->mat;
+        const size_t bytes = sizeof(double) * 2 * (size_t)M->rows * q->nrhs;
+        if (S->used + (int64_t)bytes > S->budget && !S->inflight.empty()) break;  // wait for memory to free up
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&q->r), bytes));
+        q->d = q->r + (size_t)M->rows * q->nrhs;
+        q->rbytes = bytes;
+        S->used += (int64_t)bytes;
+        S->pending.erase(S->pending.begin());
+        S->inflight.push_back(q);
+    }
+    const int cnt = (int)S->inflight.size();
+    if (cnt == 0) return HSS_OK;
+    // 2. tables: solve (first solve b->x, or correction r->d), axpy, matvec x->r, residual
+    size_t ws = 0;
+    int maxnrhs = 0;
+    size_t lsup = 0, lsroot = 0, lsdown = 0, lup = 0, ldown = 0, lleaf = 0;
+    std::vector<size_t> off_s(cnt), off_m(cnt);
+    std::vector<hss_mat_s*> mats(cnt);
+    for (int i = 0; i < cnt; ++i) {
+        ServeReq* q = S->inflight[i];
+        hss_fact_s* F = S->facts[q->mid];
+        hss_mat_s* M = F->mat;
+        mats[i] = M;
+        off_s[i] = ws;
+        ws = align256(ws + sizeof(TV) * (size_t)(F->brows + F->urows + F->zrows) * q->nrhs);
+        off_m[i] = ws;
+        ws = align256(ws + sizeof(double) * (size_t)(M->zrows + M->frows) * q->nrhs);
+        maxnrhs = std::max(maxnrhs, q->nrhs);
+        lsup = std::max(lsup, F->lds_sup);
+        lsroot = std::max(lsroot, F->lds_sroot);
+        lsdown = std::max(lsdown, F->lds_sdown);
+        lup = std::max(lup, M->lds_up);
+        ldown = std::max(ldown, M->lds_down);
+        lleaf = std::max(lleaf, M->lds_leaf);
+    }
+    hss_status s = grow(&S->d_round, &S->round_cap, ws);
+    if (s != HSS_OK) return s;
+    std::vector<SInst<TF, TV>> si(cnt);
+    std::vector<MvInst> mv(cnt);
+    std::vector<BatchVec> bv(cnt);
+    for (int i = 0; i < cnt; ++i) {
+        ServeReq* q = S->inflight[i];
+        hss_fact_s* F = S->facts[q->mid];
+        hss_mat_s* M = F->mat;
+        const int64_t n = std::max<int64_t>(M->rows, 1);
+        TV* wb = reinterpret_cast<TV*>(S->d_round + off_s[i]);
+        TV* wu = wb + F->brows * q->nrhs;
+        TV* wz = wu + F->urows * q->nrhs;
+        SolveReq sr = (q->steps == 0) ? SolveReq{F, q->nrhs, q->dB, q->ldb, q->dX, q->ldx}
+                                      : SolveReq{F, q->nrhs, q->r, n, q->d, n};
+        si[i] = make_sinst<TF, TV>(sr, wb, wu, wz);
+        MvInst& I = mv[i];
+        std::memset(&I, 0, sizeof(I));
+        I.nodes = M->d_nodes;
+        I.blob = M->d_blob;
+        I.B = q->dX;
+        I.ldb = q->ldx;
+        I.C = q->r;
+        I.ldc = n;
+        I.Z = reinterpret_cast<double*>(S->d_round + off_m[i]);
+        I.F = I.Z + M->zrows * q->nrhs;
+        I.alpha = 1.0;
+        I.beta = 0.0;
+        I.nrhs = q->nrhs;
+        BatchVec& v = bv[i];
+        std::memset(&v, 0, sizeof(v));
+        v.b = q->dB;
+        v.ldb = q->ldb;
+        v.x = q->dX;
+        v.ldx = q->ldx;
+        v.r = q->r;
+        v.d = q->d;
+        v.n = M->rows;
+        v.nrhs = (q->steps == 0) ? 0 : q->nrhs;  // axpy only for corrections
+    }
+    Schedule sch;
+    build_schedule(mats, sch);
+    std::vector<WorkItem> work = sch.work;
+    int maxh = 0, maxd = 0;
+    for (auto* M : mats) {
+        maxh = std::max(maxh, M->max_height);
+        maxd = std::max(maxd, M->max_depth);
+    }
+    std::vector<Range> mup(maxh), mdown(maxd + 1);
+    for (int h = 0; h < maxh; ++h) {
+        mup[h].off = (int)work.size();
+        for (int b = 0; b < cnt; ++b)
+            if (h < mats[b]->max_height) {
+                const Range& r = mats[b]->by_height[h];
+                for (int t = 0; t < r.count; ++t) work.push_back(WorkItem{b, mats[b]->hwork[r.off + t].node});
+            }
+        mup[h].count = (int)work.size() - mup[h].off;
+    }
+    for (int d = 0; d <= maxd; ++d) {
+        mdown[d].off = (int)work.size();
+        for (int b = 0; b < cnt; ++b)
+            if (d <= mats[b]->max_depth) {
+                const Range& r = mats[b]->br_depth[d];
+                for (int t = 0; t < r.count; ++t) work.push_back(WorkItem{b, mats[b]->hwork[r.off + t].node});
+            }
+        mdown[d].count = (int)work.size() - mdown[d].off;
+    }
+    Range mleaf{(int)work.size(), 0};
+    for (int b = 0; b < cnt; ++b) {
+        const Range& r = mats[b]->leaves;
+        for (int t = 0; t < r.count; ++t) work.push_back(WorkItem{b, mats[b]->hwork[r.off + t].node});
+    }
+    mleaf.count = (int)work.size() - mleaf.off;
+    // one table upload per round (the stream is idle here)
+    const size_t o_si = 0, o_mv = align256(sizeof(SInst<TF, TV>) * cnt), o_bv = o_mv + align256(sizeof(MvInst) * cnt),
+                 o_w = o_bv + align256(sizeof(BatchVec) * cnt), tab = o_w + sizeof(WorkItem) * work.size();
+    s = grow(&S->d_tab, &S->tab_cap, tab);
+    if (s != HSS_OK) return s;
+    std::vector<char> h(tab);
+    std::memcpy(h.data() + o_si, si.data(), sizeof(SInst<TF, TV>) * cnt);
+    std::memcpy(h.data() + o_mv, mv.data(), sizeof(MvInst) * cnt);
+    std::memcpy(h.data() + o_bv, bv.data(), sizeof(BatchVec) * cnt);
+    std::memcpy(h.data() + o_w, work.data(), sizeof(WorkItem) * work.size());
+    HIP_TRY(hipMemcpy(S->d_tab, h.data(), tab, hipMemcpyHostToDevice));
+    const size_t nb = sizeof(double) * 2 * (size_t)cnt * maxnrhs;
+    s = grow(reinterpret_cast<char**>(&S->d_norms), &S->norms_cap, nb);
+    if (s != HSS_OK) return s;
+    auto* dsi = reinterpret_cast<const SInst<TF, TV>*>(S->d_tab + o_si);
+    auto* dmv = reinterpret_cast<const MvInst*>(S->d_tab + o_mv);
+    auto* dbv = reinterpret_cast<BatchVec*>(S->d_tab + o_bv);
+    auto* dw = reinterpret_cast<const WorkItem*>(S->d_tab + o_w);
+    // 3. one batched solve, the corrections, one batched matvec + residual
+    s = launch_schedule<TF, TV>(dsi, dw, sch.up, sch.roots, sch.down, maxnrhs, lsup, lsroot, lsdown, st);
+    if (s != HSS_OK) return s;
+    HIP_TRY(launch_axpy_batch(dbv, cnt, maxnrhs, st));
+    for (int hh = 0; hh < maxh; ++hh) HIP_TRY(launch_mv_up(dmv, dw + mup[hh].off, mup[hh].count, maxnrhs, lup, st));
+    for (int d = 0; d <= maxd; ++d) HIP_TRY(launch_mv_down(dmv, dw + mdown[d].off, mdown[d].count, maxnrhs, ldown, st));
+    HIP_TRY(launch_mv_leaf(dmv, dw + mleaf.off, mleaf.count, maxnrhs, lleaf, st));
+    // the residual pass needs every request active: rewrite nrhs of the BatchVec rows
+    for (int i = 0; i < cnt; ++i) bv[i].nrhs = S->inflight[i]->nrhs;
+    HIP_TRY(hipMemcpyAsync(S->d_tab + o_bv, bv.data(), sizeof(BatchVec) * cnt, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));  // bv must stay valid until the copy is done
+    HIP_TRY(launch_resid_batch(dbv, cnt, maxnrhs, S->d_norms, st));
+    std::vector<double> norms(2 * (size_t)cnt * maxnrhs);
+    HIP_TRY(hipMemcpyAsync(norms.data(), S->d_norms, nb, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // 4. retire converged requests
+    const auto now = std::chrono::steady_clock::now();
+    std::vector<ServeReq*> keep;
+    for (int i = 0; i < cnt; ++i) {
+        ServeReq* q = S->inflight[i];
+        q->steps += 1;
+        double rel = 0.0;
+        for (int j = 0; j < q->nrhs; ++j) {
+            const double b2 = norms[2 * ((size_t)i * maxnrhs + j)], r2 = norms[2 * ((size_t)i * maxnrhs + j) + 1];
+            rel = std::max(rel, b2 > 0 ? std::sqrt(r2 / b2) : std::sqrt(r2));
+        }
+        q->rel = rel;
+        const bool conv = rel <= S->opts.tol || S->opts.mode == 0;
+        if (conv || q->steps > S->opts.max_iter) {
+            q->status = conv ? 1 : 2;
+            q->t_done = now;
+            (void)hipFree(q->r);
+            S->used -= (int64_t)q->rbytes;
+            q->r = q->d = nullptr;
+            S->finished.push_back(q);
+            S->solved += 1;
+        } else {
+            keep.push_back(q);
+        }
+    }
+    S->inflight.swap(keep);
+    S->rounds += 1;
+    return HSS_OK;
+}
+
+// One scheduling round; returns the number of requests still queued or in flight.
+extern "C" hss_status hss_server_step(hss_server_t S, int32_t* outstanding) {
+    if (!S) return fail(HSS_ERR_ARG, "null server");
+    HIP_TRY(hipSetDevice(S->ctx->device));
+    hss_status s = (S->prec == HSS_PREC_FP32) ? server_round<float>(S) : server_round<double>(S);
+    if (outstanding) *outstanding = (int32_t)(S->pending.size() + S->inflight.size());
+    return s;
+}
+
+// status: 0 = not finished, 1 = converged, 2 = stopped at max_iter.  A finished
+// ticket is reported once and then forgotten.
+extern "C" hss_status hss_server_poll(hss_server_t S, int64_t ticket, int32_t* status, double* latency_s,
+                                      hss_solve_report* rep) {
+    if (!S || !status) return fail(HSS_ERR_ARG, "null argument");
+    *status = 0;
+    for (size_t i = 0; i < S->finished.size(); ++i) {
+        ServeReq* q = S->finished[i];
+        if (q->ticket != ticket) continue;
+        *status = q->status;
+        if (latency_s) *latency_s = std::chrono::duration<double>(q->t_done - q->t_submit).count();
+        if (rep) {
+            rep->iterations = q->steps - 1;
+            rep->rel_residual = q->rel;
+            rep->seconds = 0.0;
+        }
+        S->finished.erase(S->finished.begin() + i);
+        delete q;
+        return HSS_OK;
+    }
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_server_destroy(hss_server_t S) {
+    if (!S) return HSS_OK;
+    (void)hipSetDevice(S->ctx->device);
+    (void)hipStreamSynchronize(S->ctx->stream);
+    for (auto* q : S->pending) delete q;
+    for (auto* q : S->inflight) {
+        (void)hipFree(q->r);
+        delete q;
+    }
+    for (auto* q : S->finished) delete q;
+    for (auto* F : S->facts)
+        if (F) hss_fact_destroy(F);
+    (void)hipFree(S->d_round);
+    (void)hipFree(S->d_tab);
+    (void)hipFree(S->d_norms);
+    delete S;
+    return HSS_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -48,6 +48,8 @@ EXPORTS = [
     "hss_last_error", "hss_last_device_ms",
     "hss_prof_enable", "hss_prof_reset", "hss_prof_count", "hss_prof_get", "hss_fact_node_info",
     "hss_plan_create", "hss_plan_execute", "hss_plan_destroy", "hss_plan_device_bytes",
+    "hss_server_create", "hss_server_admit", "hss_server_evict", "hss_server_used_bytes", "hss_server_submit",
+    "hss_server_step", "hss_server_poll", "hss_server_destroy",
 ]
 
 
@@ -116,6 +118,15 @@ def load_library(path: pathlib.Path | str | None = None):
         "hss_plan_execute": ([V, ctypes.POINTER(RefineOpts), ctypes.POINTER(SolveReport)], ctypes.c_int),
         "hss_plan_destroy": ([V], ctypes.c_int),
         "hss_plan_device_bytes": ([V], I64),
+        "hss_server_create": ([V, I64, ctypes.c_int, ctypes.POINTER(RefineOpts), I32, PV], ctypes.c_int),
+        "hss_server_admit": ([V, V, ctypes.POINTER(I32)], ctypes.c_int),
+        "hss_server_evict": ([V, I32], ctypes.c_int),
+        "hss_server_used_bytes": ([V], I64),
+        "hss_server_submit": ([V, I32, I32, V, I64, V, I64, ctypes.POINTER(I64)], ctypes.c_int),
+        "hss_server_step": ([V, ctypes.POINTER(I32)], ctypes.c_int),
+        "hss_server_poll": ([V, I64, ctypes.POINTER(I32), ctypes.POINTER(D), ctypes.POINTER(SolveReport)],
+                            ctypes.c_int),
+        "hss_server_destroy": ([V], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -449,6 +460,69 @@ class Plan:
     def close(self):
         if getattr(self, "_h", None):
             _lib.hss_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving engine (``hss_server_*``): resident factors under an HBM budget and
+    continuous batching of solves (first solves and refinement steps of all
+    requests in flight share one launch per tree level)."""
+
+    def __init__(self, budget_bytes: int, prec: str = "fp32", tol: float = 1e-10, max_iter: int = 10,
+                 max_inflight: int = 64, ctx: Context | None = None):
+        lib = load_library()
+        self.ctx = ctx or default_context()
+        p = PREC_FP32 if prec in ("fp32", "mixed") else PREC_FP64
+        opts = RefineOpts(tol, max_iter, 1)
+        h = ctypes.c_void_p()
+        _check(lib.hss_server_create(self.ctx.handle, int(budget_bytes), p, ctypes.byref(opts), max_inflight,
+                                     ctypes.byref(h)))
+        self._h = h
+        self.mats = {}
+
+    def admit(self, mat: HipHssMatrix) -> int:
+        mid = ctypes.c_int32()
+        _check(_lib.hss_server_admit(self._h, mat.handle, ctypes.byref(mid)))
+        self.mats[mid.value] = mat
+        return mid.value
+
+    def evict(self, mid: int):
+        _check(_lib.hss_server_evict(self._h, mid))
+        self.mats.pop(mid, None)
+
+    def used_bytes(self) -> int:
+        return int(_lib.hss_server_used_bytes(self._h))
+
+    def submit(self, mid: int, nrhs: int, dB: int, ldb: int, dX: int, ldx: int) -> int:
+        t = ctypes.c_int64()
+        _check(_lib.hss_server_submit(self._h, mid, nrhs, dB, ldb, dX, ldx, ctypes.byref(t)))
+        return t.value
+
+    def step(self) -> int:
+        out = ctypes.c_int32()
+        _check(_lib.hss_server_step(self._h, ctypes.byref(out)))
+        return out.value
+
+    def poll(self, ticket: int):
+        """None while running; else {status, latency_s, iterations, rel_residual}."""
+        st = ctypes.c_int32()
+        lat = ctypes.c_double()
+        rep = SolveReport()
+        _check(_lib.hss_server_poll(self._h, ticket, ctypes.byref(st), ctypes.byref(lat), ctypes.byref(rep)))
+        if st.value == 0:
+            return None
+        return {"status": st.value, "latency_s": lat.value, "iterations": rep.iterations,
+                "rel_residual": rep.rel_residual}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.hss_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

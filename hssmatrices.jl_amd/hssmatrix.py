@@ -481,3 +481,46 @@ def tree_summary(nodes) -> dict:
         "rank_max": int(ks.max()) if ks.size else 0,
         "G": int(G),
     }
+
+
+def random_hss(n: int, leafsize: int, rank_range=(10, 40), rng=None, diag: float = 4.0,
+               coupling: float = 1.0) -> HssMatrix:
+    """Synthetic HSS matrix with the shapes of a compressed kernel matrix.
+
+    Bisection tree (``bisection_cluster``), node ranks drawn uniformly from
+    ``rank_range`` (capped by what the children can carry), orthonormal nested
+    bases (leaf U/V and stacked translators [R1; R2], [W1; W2] with orthonormal
+    columns), couplings N(0, coupling^2 / k), and leaf blocks
+    N(0, 1/m) + ``diag`` * I.  Used for batch and serving benchmarks, where
+    compressing hundreds of kernel matrices on the host would dominate.
+    """
+    rng = np.random.default_rng(rng)
+    lo, hi = rank_range
+
+    def orth(r, c):
+        if r == 0 or c == 0:
+            return np.zeros((r, c), order="F")
+        q, _ = np.linalg.qr(rng.standard_normal((r, c)))
+        return np.asfortranarray(q)
+
+    def build(cl: ClusterTree, root: bool):
+        if cl.isleaf():
+            m = cl.size
+            D = rng.standard_normal((m, m)) / np.sqrt(m) + diag * np.eye(m)
+            if root:
+                return HssMatrix(D=D, rootnode=True), 0
+            k = int(min(rng.integers(lo, hi + 1), m))
+            return HssMatrix(D=D, U=orth(m, k), V=orth(m, k)), k
+        a, ka = build(cl.left, False)
+        b, kb = build(cl.right, False)
+        B12 = rng.standard_normal((ka, kb)) * coupling / np.sqrt(max(ka, 1))
+        B21 = rng.standard_normal((kb, ka)) * coupling / np.sqrt(max(kb, 1))
+        if root:
+            return HssMatrix(A11=a, A22=b, B12=B12, B21=B21, rootnode=True), 0
+        k = int(min(rng.integers(lo, hi + 1), ka + kb))
+        R = orth(ka + kb, k)
+        W = orth(ka + kb, k)
+        return HssMatrix(A11=a, A22=b, B12=B12, B21=B21, R1=R[:ka], W1=W[:ka], R2=R[ka:], W2=W[ka:]), k
+
+    h, _ = build(bisection_cluster(n, leafsize), True)
+    return h

@@ -156,6 +156,27 @@ hss_status hss_plan_execute(hss_plan_t plan, const hss_refine_opts* opts, hss_so
 hss_status hss_plan_destroy(hss_plan_t plan);
 int64_t hss_plan_device_bytes(hss_plan_t plan);
 
+/* ---- serving (no reference counterpart; config C5) ----
+ * Factorizations stay resident in HBM, admitted while their bytes fit hbm_budget_bytes
+ * (HSS_ERR_BUDGET otherwise).  Submitted solves queue; every hss_server_step runs one round:
+ * queued requests take free slots (max_inflight, memory), then all requests in flight advance
+ * together -- their first solve or their next refinement correction in one launch per tree
+ * level, and one batched Float64 matvec + residual -- and requests at opts->tol retire.
+ * Device pointers dB/dX must stay valid until the ticket is reported finished by
+ * hss_server_poll.  Not thread-safe: one caller drives a server. */
+typedef struct hss_server_s* hss_server_t;
+hss_status hss_server_create(hss_ctx_t ctx, int64_t hbm_budget_bytes, hss_prec prec, const hss_refine_opts* opts,
+                             int32_t max_inflight, hss_server_t* out);
+hss_status hss_server_admit(hss_server_t server, hss_mat_t mat, int32_t* matrix_id);
+hss_status hss_server_evict(hss_server_t server, int32_t matrix_id);
+int64_t hss_server_used_bytes(hss_server_t server);
+hss_status hss_server_submit(hss_server_t server, int32_t matrix_id, int32_t nrhs, const double* dB, int64_t ldb,
+                             double* dX, int64_t ldx, int64_t* ticket);
+hss_status hss_server_step(hss_server_t server, int32_t* outstanding);
+hss_status hss_server_poll(hss_server_t server, int64_t ticket, int32_t* status, double* latency_s,
+                           hss_solve_report* report);
+hss_status hss_server_destroy(hss_server_t server);
+
 /* Batched factorization of many matrices (one launch per level across all of them). */
 hss_status hss_ulv_factor_batch(int32_t count, const hss_mat_t* mats, hss_prec prec, hss_fact_t* out);
 

@@ -280,3 +280,59 @@ def test_plan_mixed_precision_lockstep_refinement(hip, oracle):
     rep2 = plan.execute(tol=1e-10, max_iter=20)
     assert rep2["rel_residual"] <= 1e-10
     plan.close()
+
+
+def test_server_alone_equals_mixed(hip, oracle):
+    """Continuous batching: each result is the same alone and in a mix with other requests."""
+    import torch
+    rng = np.random.default_rng(23)
+    packs, mats = [], []
+    for n, leaf in ((1024, 64), (1500, 64), (700, 32)):
+        _, h = random_hss(rng, n, leaf, 0, "cauchy", 1e-8)
+        packs.append(H.pack(h))
+        mats.append(hip.HipHssMatrix(packs[-1]))
+    srv = hip.Server(budget_bytes=1 << 30, prec="fp32", tol=1e-10, max_iter=20, max_inflight=8)
+    ids = [srv.admit(m) for m in mats]
+    dev = torch.device("cuda:0")
+
+    def make(i, r):
+        n = int(packs[i][0]["n"][0])
+        B = torch.from_numpy(np.ascontiguousarray(rng.standard_normal((r, n)))).to(dev)
+        return B, torch.zeros_like(B)
+
+    # alone
+    B0, X0 = make(0, 3)
+    t = srv.submit(ids[0], 3, B0.data_ptr(), B0.shape[1], X0.data_ptr(), X0.shape[1])
+    while srv.step() > 0:
+        pass
+    res = srv.poll(t)
+    assert res is not None and res["status"] == 1 and res["rel_residual"] <= 1e-10
+    alone = X0.cpu().numpy().copy()
+    # the same request in a mix, arriving between rounds of other requests
+    X0.zero_()
+    others = [make(i % 3, 1 + i) for i in range(6)]
+    tickets = []
+    for k, (B, X) in enumerate(others[:3]):
+        tickets.append(srv.submit(ids[k % 3], B.shape[0], B.data_ptr(), B.shape[1], X.data_ptr(), X.shape[1]))
+    srv.step()
+    t0 = srv.submit(ids[0], 3, B0.data_ptr(), B0.shape[1], X0.data_ptr(), X0.shape[1])
+    for k, (B, X) in enumerate(others[3:]):
+        tickets.append(srv.submit(ids[k % 3], B.shape[0], B.data_ptr(), B.shape[1], X.data_ptr(), X.shape[1]))
+    while srv.step() > 0:
+        pass
+    r0 = srv.poll(t0)
+    assert r0["status"] == 1
+    mixed = X0.cpu().numpy()
+    assert rel(mixed, alone) <= 1e-12
+    for k, ((B, X), tk) in enumerate(zip(others, tickets)):
+        r = srv.poll(tk)
+        assert r["status"] == 1
+        nodes, blob = packs[k % 3]
+        Bn, Xn = B.cpu().numpy().T, X.cpu().numpy().T
+        R = oracle.matmul(nodes, blob, Xn) - Bn
+        assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(Bn, axis=0)) <= 1e-10
+    # admission is refused beyond the budget
+    small = hip.Server(budget_bytes=1024, prec="fp32")
+    with pytest.raises(hip.HssError):
+        small.admit(mats[0])
+    srv.close()
