@@ -41,6 +41,10 @@ FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector = matrix)
 FP32_MFMA_PEAK_TFLOPS = 157.3
 DATA = ROOT / "data" / "log_n65536_leaf128_tol1e-9.npz"
 N_POINTS, LEAF, TOL, NRHS = 65536, 128, 1e-9, 64
+# diagonal shift of the synthetic batched/serving instances: leaf blocks N(0, 1/m) + 2 I with
+# orthonormal nested bases give condition numbers of about 20-40 (a shift of 1 puts the
+# circular-law spectrum of the leaf blocks against 0 and some instances become singular)
+SYNTH_DIAG = 2.0
 
 
 def log(*a):
@@ -150,7 +154,7 @@ def batched(pkg, hip, torch, steps=10, warmup=2, count=256, seed=3, log=print):
     t = time.time()
     mats, packs = [], []
     for i, n in enumerate(ns):
-        h = pkg.random_hss(int(n), 64, (10, 40), rng, diag=1.0)
+        h = pkg.random_hss(int(n), 64, (10, 40), rng, diag=SYNTH_DIAG)
         packs.append(pkg.pack(h))
         mats.append(hip.HipHssMatrix(packs[-1]))
     log(f"[bench] batch: {count} synthetic instances generated in {time.time() - t:.1f} s "
@@ -170,17 +174,26 @@ def batched(pkg, hip, torch, steps=10, warmup=2, count=256, seed=3, log=print):
         plan = hip.Plan(facts, [int(r) for r in rs], [b.data_ptr() for b in Bs], [int(b.shape[1]) for b in Bs],
                         [x.data_ptr() for x in Xs], [int(x.shape[1]) for x in Xs], refinement=(prec == "fp32"))
         tol = 1e-10 if prec == "fp32" else None
-        for _ in range(warmup):
-            plan.execute(tol=tol)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        rep = None
-        for _ in range(steps):
-            rep = plan.execute(tol=tol)
-        torch.cuda.synchronize()
+        try:
+            for _ in range(warmup):
+                plan.execute(tol=tol)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rep = None
+            for _ in range(steps):
+                rep = plan.execute(tol=tol)
+            torch.cuda.synchronize()
+        except hip.NotConverged as e:
+            log(f"[bench] batch {prec}: {e}")
+            out[prec] = {"error": str(e)}
+            plan.close()
+            for f in facts:
+                f.close()
+            continue
         ms = (time.perf_counter() - t0) * 1e3 / steps
         out[prec] = {"batch_ms": round(ms, 3), "solves_per_s": round(count * 1e3 / ms, 1),
-                     "factor_batch_ms": round(fms, 2)}
+                     "factor_batch_ms": round(fms, 2),
+                     "worst_rel_residual_fp64_check": _batch_residual(mats, Bs, Xs, rs, torch)}
         if rep is not None and prec == "fp32":
             out[prec]["refine_steps"] = rep["iterations"]
             out[prec]["worst_rel_residual"] = rep["rel_residual"]
@@ -203,9 +216,22 @@ def batched(pkg, hip, torch, steps=10, warmup=2, count=256, seed=3, log=print):
         plan.close()
         for f in facts:
             f.close()
-    out["mixed_batched_vs_plain_fp64"] = round(out["fp32"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"], 2)
-    out["fp64_batched_vs_plain_fp64"] = round(out["fp64"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"], 2)
+    for prec, key in (("fp32", "mixed_batched_vs_plain_fp64"), ("fp64", "fp64_batched_vs_plain_fp64")):
+        if "solves_per_s" in out[prec]:
+            out[key] = round(out[prec]["solves_per_s"] / out["plain_fp64"]["solves_per_s"], 2)
     return out
+
+
+def _batch_residual(mats, Bs, Xs, rs, torch):
+    """Largest relative residual max_j ||B_j - A X_j|| / ||B_j|| over every instance of the batch,
+    computed with the library's FP64 matvec (untimed check of the timed results)."""
+    worst = 0.0
+    for A, B, X, r in zip(mats, Bs, Xs, rs):
+        R = B.clone()
+        A.matmul_dev(int(r), 1.0, X.data_ptr(), X.shape[1], -1.0, R.data_ptr(), R.shape[1])
+        rel = (torch.linalg.vector_norm(R, dim=1) / torch.linalg.vector_norm(B, dim=1)).max().item()
+        worst = max(worst, rel)
+    return worst
 
 
 def serving(pkg, hip, torch, count=32, requests=400, rate=2000.0, seed=4, log=print, budget=200 * 2 ** 30):
@@ -216,7 +242,7 @@ def serving(pkg, hip, torch, count=32, requests=400, rate=2000.0, seed=4, log=pr
     dev = torch.device("cuda:0")
     ns = np.round(np.exp(rng.uniform(np.log(4096), np.log(65536), count))).astype(int)
     t = time.time()
-    mats = [hip.HipHssMatrix(pkg.pack(pkg.random_hss(int(n), 128, (10, 40), rng, diag=1.0))) for n in ns]
+    mats = [hip.HipHssMatrix(pkg.pack(pkg.random_hss(int(n), 128, (10, 40), rng, diag=SYNTH_DIAG))) for n in ns]
     srv = hip.Server(budget_bytes=budget, prec="fp32", tol=1e-10, max_iter=20, max_inflight=64)
     ids = [srv.admit(m) for m in mats]
     log(f"[bench] serve: {count} factors resident ({srv.used_bytes() / 2**20:.1f} MiB) in {time.time() - t:.1f} s")
@@ -228,6 +254,7 @@ def serving(pkg, hip, torch, count=32, requests=400, rate=2000.0, seed=4, log=pr
     tickets = {}
     lat = []
     iters = []
+    worst, failed = 0.0, 0
     t0 = time.perf_counter()
     nxt = 0
     done = 0
@@ -245,6 +272,8 @@ def serving(pkg, hip, torch, count=32, requests=400, rate=2000.0, seed=4, log=pr
             if r is not None:
                 lat.append(r["latency_s"] * 1e3)
                 iters.append(r["iterations"])
+                worst = max(worst, r["rel_residual"])
+                failed += int(r["status"] != 1)
                 del tickets[tk]
                 done += 1
     wall = time.perf_counter() - t0
@@ -253,9 +282,27 @@ def serving(pkg, hip, torch, count=32, requests=400, rate=2000.0, seed=4, log=pr
            "arrival_rate_per_s": rate, "solves_per_s": round(requests / wall, 1),
            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                           "p99": round(float(np.percentile(lat, 99)), 3)},
-           "refine_steps": [int(min(iters)), int(max(iters))], "rhs_mean": round(float(rhs.mean()), 2)}
+           "refine_steps": [int(min(iters)), int(max(iters))], "rhs_mean": round(float(rhs.mean()), 2),
+           "worst_rel_residual": worst, "not_converged": failed}
     log(f"[bench] serve: {out}")
     return out
+
+
+PROFILE_COUNTERS = ROOT / "profiles" / "hss_c2_n65536_hbm_counters.md"
+
+
+def profiled_traffic(kernel: str, type_args: str, grid_x: int, grid_y: int):
+    """HBM bytes of one launch as measured by the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    runs (copy-calibrated, last column of the table), or None when that launch shape was not
+    profiled."""
+    if not PROFILE_COUNTERS.exists():
+        return None
+    want = f"`hss::{kernel}{type_args}`"
+    for line in PROFILE_COUNTERS.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == want and cells[1] == str(grid_x) and cells[2] == str(grid_y):
+            return int(float(cells[5]) * 1e6)
+    return None
 
 
 def profile_launches(hip, fn, reps):
@@ -433,6 +480,7 @@ def main():
     achieved = dbytes / avg_ms / 1e6
     log(f"[bench] dominant solve kernel {dk} level {dlev}: {avg_ms:.4f} ms/launch, "
         f"{dbytes/1e6:.2f} MB -> {achieved:.1f} GB/s, {dflops/avg_ms/1e9:.2f} TFLOP/s")
+    traffic = profiled_traffic(dk, "<double, double>", int(dnw) * 256, (nrhs + 31) // 32)
     mvleaf = [(key, v) for key, v in tot_m.items() if key[0] == "k_mv_leaf"]
     mv_leaf_ms = mvleaf[0][1][0] / mvleaf[0][1][1]
     mv_leaf_bytes = matvec_leaf_bytes(nodes, nrhs)
@@ -525,7 +573,7 @@ def main():
                                    "gbps": round(mv_leaf_bytes / mv_leaf_ms / 1e6, 1)}},
         "roofline": {"kernel": f"{dk} (level {dlev}, {dnw} nodes)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": int(dbytes), "avg_launch_ms": round(avg_ms, 5)},
         "per_kernel_ms": {"fp64_factor": {k: round(v, 4) for k, v in byk_f.items()},
                           "fp64_solve": {k: round(v, 4) for k, v in byk_s.items()},
