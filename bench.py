@@ -99,36 +99,33 @@ def matvec_bytes(nodes, nrhs):
 def solve_kernel_model(nodes, fact, kernel, level, nrhs, wf, wv):
     """Algorithmic bytes and flops of one launch of a solve kernel at one level.
 
-    k_ulv_sup (height ``level``): per node reads Y_ql (m k), T_ql (k^2), L1 (nk^2),
-      L1^-1 blocks, L2a (k nk), Vt (kv nk), P12/P21, the node's b (m x nrhs) and
-      children's u; writes zloc (nk), reduced b (k), u (kv) per column.
-    k_ulv_sdown (depth ``level``): reads Y_lq (n nk), T blocks, zloc and the
-      parent's z slice, writes z (n) per column.
-    Factor entries are counted once per launch (their minimum traffic).
+    k_ulv_gup (height ``level``): per node zloc = Gz x (Gz nk x Kx), then
+      [reduced b; u] = Gr [x; zloc] (Gr R x (Kx + nk), R = My - nk): reads Gz and Gr once
+      and the input x (Kx rows: the leaf's rows of B in Float64, or the children's y
+      blocks), writes y (My rows) per column.
+    k_ulv_gdown (depth ``level``): per node z = Qt [zloc; z_parent] with Qt n x n
+      (identity for a pass-through node): reads Qt once and n input rows, writes n rows
+      (the solution X in Float64 at a leaf) per column.
+    wf / wv = bytes per factor / workspace element.
     """
-    NB = 16
     byts = 0
     flops = 0
     heights = nodes["height"]
     depths = nodes["depth"]
     for i in range(1, len(nodes)):
-        f = fact.node_info(i)
-        m, n, k, nk, nblk = f["m"], f["n"], f["k"], f["nk"], f["nblk"]
-        kv = int(nodes["kv"][i])
-        if kernel == "k_ulv_sup" and heights[i] == level:
-            fe = 0 if f["pass"] else (m * k + k * k + nk * nk + nblk * NB * NB + k * nk + kv * nk)
-            if nodes["left"][i] >= 0:
-                L, R = int(nodes["left"][i]), int(nodes["right"][i])
-                fe += fact.node_info(L)["mr"] * int(nodes["kv"][R]) + fact.node_info(R)["mr"] * int(nodes["kv"][L])
-            vin = m * (8 if nodes["left"][i] < 0 else wv)
-            vout = wv * (nk + (f["mr"]) + kv)
-            byts += wf * fe + nrhs * (vin + vout)
-            flops += nrhs * (4 * m * k + 2 * k * k + nk * nk + 2 * k * nk + 2 * kv * nk)
-        elif kernel == "k_ulv_sdown" and depths[i] == level:
-            fe = 0 if f["pass"] else (n * nk + nblk * NB * NB)
-            vout = n * (8 if nodes["left"][i] < 0 else wv)
-            byts += wf * fe + nrhs * (wv * n + vout)
-            flops += nrhs * (4 * n * nk + 2 * nblk * NB * NB)
+        leaf = nodes["left"][i] < 0
+        if kernel == "k_ulv_gup" and heights[i] == level:
+            f = fact.node_info(i)
+            My, Kx, nk = f["My"], f["Kx"], f["nk"]
+            g = nk * Kx + (My - nk) * (Kx + nk)
+            byts += wf * g + nrhs * (Kx * (8 if leaf else wv) + My * wv)
+            flops += 2 * g * nrhs
+        elif kernel == "k_ulv_gdown" and depths[i] == level:
+            f = fact.node_info(i)
+            n = f["n"]
+            q = 0 if f["pass"] else n * n
+            byts += wf * q + nrhs * (n * wv + n * (8 if leaf else wv))
+            flops += 2 * q * nrhs
     return byts, flops
 
 
@@ -289,6 +286,21 @@ def serving(pkg, hip, torch, count=32, requests=400, rate=2000.0, seed=4, log=pr
 
 
 PROFILE_COUNTERS = ROOT / "profiles" / "hss_c2_n65536_hbm_counters.md"
+
+
+def sweep_grid_threads(nodes, fact, kernel, level, nwork, nrhs):
+    """Launch size (threads) of a solve sweep launch, as the library computes it, padded to a
+    multiple of 8 workgroups of 256 threads.  k_ulv_gup: one workgroup per node and 32
+    columns.  k_ulv_gdown: tiles of 64 x 64 outputs (128 x 32 for at most 32 right-hand
+    sides)."""
+    if kernel == "k_ulv_gup":
+        total = nwork * (-(-nrhs // 32))
+    else:
+        tm_rows, tn_cols = (64, 64) if nrhs > 32 else (128, 32)
+        sel = [i for i in range(1, len(nodes)) if nodes["depth"][i] == level]
+        max_m = max(fact.node_info(i)["n"] for i in sel)
+        total = nwork * (-(-max_m // tm_rows)) * (-(-nrhs // tn_cols))
+    return 8 * (-(-total // 8)) * 256
 
 
 def profiled_traffic(kernel: str, type_args: str, grid_x: int, grid_y: int):
@@ -480,7 +492,8 @@ def main():
     achieved = dbytes / avg_ms / 1e6
     log(f"[bench] dominant solve kernel {dk} level {dlev}: {avg_ms:.4f} ms/launch, "
         f"{dbytes/1e6:.2f} MB -> {achieved:.1f} GB/s, {dflops/avg_ms/1e9:.2f} TFLOP/s")
-    traffic = profiled_traffic(dk, "<double, double>", int(dnw) * 256, (nrhs + 31) // 32)
+    targs = "<double>" if dk == "k_ulv_gup" else ("<double, 2, 2>" if nrhs > 32 else "<double, 4, 1>")
+    traffic = profiled_traffic(dk, targs, sweep_grid_threads(nodes, F64, dk, dlev, int(dnw), nrhs), 1)
     mvleaf = [(key, v) for key, v in tot_m.items() if key[0] == "k_mv_leaf"]
     mv_leaf_ms = mvleaf[0][1][0] / mvleaf[0][1][1]
     mv_leaf_bytes = matvec_leaf_bytes(nodes, nrhs)

@@ -37,12 +37,28 @@ struct FNode {
     int32_t S;         // factor threads per matrix row
     int32_t nblk;      // ceil(nk / NB)
     int32_t pad;
-    // factor storage (elements of the factor type), -1 = none
+    // Solve operators of a non-root node (resident).  The node's input x is the leaf's
+    // rows of B, or for a branch [b_1; u_1; b_2; u_2] (both children's reduced b and u,
+    // in that order); its output y = [zloc; reduced b; u] (My = nk + mr + kv rows):
+    //   Gz (nk x Kx):           zloc = Gz x
+    //   Gr (R x (Kx + nk)):     [reduced b; u] = Gr [x; zloc], R = mr + kv; the last nk
+    //                           columns are [-L2a; V_top'], applied to the zloc just
+    //                           computed (keeps the reduced b consistent with zloc)
+    //   Qt (n x n):             z = Qt [zloc; z from the parent] (Q_lq' explicitly;
+    //                           -1 = identity)
+    // Kx = m (leaf) or mr_1 + kv_1 + mr_2 + kv_2.
+    int32_t My, Kx;
+    int64_t Gz, Gr, Qt;
+    // factor storage (elements of the factor type), -1 = none.  Everything except G, Qt
+    // and the root's LU, LUinv, P12, P21 is factorization-time data, released once G and
+    // Qt are formed.
     int64_t Yql, Tql, Ylq, taulq, Tlq, L1, L1inv, L2a, Vt, P12, P21, LU, LUinv, piv;
     // factorization-time workspace (reduced D/U/V passed to the parent, merged D scratch)
     int64_t rwD, rwU, rwV, rwM;
-    // solve workspace rows (times nrhs): reduced b (mr rows), u (kv rows), z (n rows)
-    int64_t brow, urow, zrow;
+    // solve workspace rows (times nrhs; column-major blocks of nrhs columns):
+    //   yrow: the node's y = [zloc; reduced b; u] (My rows, non-root nodes)
+    //   zrow: the node's z (n rows, branches and the root)
+    int64_t yrow, zrow;
 };
 
 struct WorkItem {
@@ -74,20 +90,19 @@ struct FInst {
     int32_t* status;  // first zero pivot (1-based) of the root LU, 0 = none
 };
 
-// Per-instance arguments of a solve.
-template <typename TF, typename TV>
+// Per-instance arguments of a solve.  TF = element type of the factor and of the
+// solve workspaces; B and X are Float64.
+template <typename TF>
 struct SInst {
     const DevNode* nodes;
     const FNode* fn;
     const TF* fac;
     const int32_t* ipiv;
-    const double* blob;
     const double* B;
     double* X;
     int64_t ldb, ldx;
-    TV* wb;
-    TV* wu;
-    TV* wz;
+    TF* wy;  // y blocks (FNode::yrow)
+    TF* wz;  // z blocks (FNode::zrow)
     int32_t nrhs, pad;
 };
 

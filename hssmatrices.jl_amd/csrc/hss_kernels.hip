@@ -885,127 +885,170 @@ __global__ __launch_bounds__(1024) void k_ulv_root(const FInst<T>* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// ULV solve
+// ULV solve operators, formed once per factorization
 // ---------------------------------------------------------------------------
-// LDS layout of the up-sweep (row-major, NTP columns):
-//   bv  m rows          the node's right-hand side b
-//   zl  max(nk, 1) rows zloc
-//   us  kv_L + kv_R     children's u (branches)
-//   t1  max(k, NB, kv)  scratch
-//   t2  max(k, NB)      scratch
-template <typename TF, typename TV>
-__device__ __forceinline__ void solve_form_b(const SInst<TF, TV>& I, const DevNode& nd, const FNode& f, int j0,
-                                             int nt, TV* bv, TV* us) {
-    const int nrhs = I.nrhs;
-    if (nd.is_leaf) {
-        stage(bv, 0, I.B + nd.row_off + (int64_t)j0 * I.ldb, I.ldb, f.m, nt);
-        __syncthreads();
-        return;
-    }
-    const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
-    const DevNode dl = I.nodes[nd.left], dr = I.nodes[nd.right];
-    stage(bv, 0, I.wb + fl.brow * nrhs + (int64_t)j0 * fl.mr, fl.mr, fl.mr, nt);
-    stage(bv, fl.mr, I.wb + fr.brow * nrhs + (int64_t)j0 * fr.mr, fr.mr, fr.mr, nt);
-    stage(us, 0, I.wu + fl.urow * nrhs + (int64_t)j0 * dl.kv, dl.kv, dl.kv, nt);
-    stage(us, dl.kv, I.wu + fr.urow * nrhs + (int64_t)j0 * dr.kv, dr.kv, dr.kv, nt);
-    __syncthreads();
-    // b = [b1 - P12 u2; b2 - P21 u1]   (ulvfactor.jl:74)
-    gemm<false>(fl.mr, nt, dr.kv, I.fac + f.P12, fl.mr, us + dl.kv * NTP,
-                [&](int i, int j, TV v) { bv[i * NTP + j] -= v; });
-    const int mr1 = fl.mr;
-    gemm<false>(fr.mr, nt, dl.kv, I.fac + f.P21, fr.mr, us,
-                [&](int i, int j, TV v) { bv[(mr1 + i) * NTP + j] -= v; });
-    __syncthreads();
-}
-
-template <typename TF, typename TV>
-__global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_sup(const SInst<TF, TV>* __restrict__ insts,
-                                                          const WorkItem* __restrict__ work) {
+// The reduction of a node (ulvfactor.jl:42,48-51,74,89: Q_ql' b, the forward
+// substitution with L1, the reduced b and u, the children's merges) is linear in
+// its input x = the leaf's rows of B, or [b_1; u_1; b_2; u_2] for a branch.
+// k_ulv_gform runs its steps on columns of the identity:
+//   Gz (nk x Kx):   column c = zloc for x = e_c
+//   Gr (R x (Kx + nk)), R = mr + kv: column c < Kx = [(Q_ql' b)[nk:m]; W_1'u_1 + W_2'u_2]
+//                   for x = e_c (the reduced b and u before their zloc terms); the last
+//                   nk columns are [-L2a; V_top'], so [reduced b; u] = Gr [x; zloc].
+// Grid columns run over Kx + nk; the last nk are copies of factor blocks.
+//
+// LDS (row-major, NTP columns): bv m rows | zl max(nk,1) | us kv_1 + kv_2 | t1 tr | t2 tr,
+// tr = max(k, NB, kv)
+template <typename TF>
+__global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_gform(const FInst<TF>* __restrict__ insts,
+                                                            const WorkItem* __restrict__ work) {
     const WorkItem wi = work[blockIdx.x];
-    const SInst<TF, TV>& I = insts[wi.inst];
-    const int j0 = blockIdx.y * RHS_TILE;
-    if (j0 >= I.nrhs) return;
-    const int nt = min(RHS_TILE, I.nrhs - j0);
+    const FInst<TF>& I = insts[wi.inst];
     const DevNode nd = I.nodes[wi.node];
     const FNode f = I.fn[wi.node];
-    const int m = f.m, k = f.k, nk = f.nk, kv = nd.kv;
-    const int nrhs = I.nrhs;
-    int kvl = 0, kvr = 0;
+    const int j0 = blockIdx.y * RHS_TILE;
+    const int Kx = f.Kx, nk = f.nk, mr = f.mr, kv = nd.kv, R = mr + kv;
+    if (j0 >= Kx + nk) return;
+    TF* Gr = I.fac + f.Gr;  // ld R
+    // columns [Kx, Kx + nk) of Gr: [-L2a; V_top']
+    for (int c = max(j0, Kx); c < min(j0 + RHS_TILE, Kx + nk); ++c) {
+        const int l = c - Kx;
+        for (int i = threadIdx.x; i < R; i += blockDim.x)
+            Gr[i + (int64_t)c * R] = i < mr ? -I.fac[f.L2a + i + (int64_t)l * f.k] : I.fac[f.Vt + l + (int64_t)(i - mr) * nk];
+    }
+    if (j0 >= Kx) return;
+    const int nt = min(RHS_TILE, Kx - j0);
+    const int m = f.m, k = f.k;
+    int kvl = 0, kvr = 0, mrl = 0;
     if (!nd.is_leaf) {
         kvl = I.nodes[nd.left].kv;
         kvr = I.nodes[nd.right].kv;
+        mrl = I.fn[nd.left].mr;
     }
     const int tr = max(max(k, NB), kv);
     extern __shared__ unsigned char lds_raw[];
-    TV* bv = reinterpret_cast<TV*>(lds_raw);
-    TV* zl = bv + m * NTP;
-    TV* us = zl + max(nk, 1) * NTP;
-    TV* t1 = us + (kvl + kvr) * NTP;
-    TV* t2 = t1 + tr * NTP;
-    solve_form_b(I, nd, f, j0, nt, bv, us);
-    TV* wbn = I.wb + f.brow * nrhs + (int64_t)j0 * f.mr;
-    TV* ug = I.wu + f.urow * nrhs + (int64_t)j0 * kv;
-    const int mr = f.mr;
+    TF* bv = reinterpret_cast<TF*>(lds_raw);
+    TF* zl = bv + m * NTP;
+    TF* us = zl + max(nk, 1) * NTP;
+    TF* t1 = us + (kvl + kvr) * NTP;
+    TF* t2 = t1 + tr * NTP;
+    // identity columns j0.. scattered over b (bv) and the children's u (us):
+    // input index of bv row r is r (b_1) or r + kv_1 (b_2); of us row r, mr_1 + r (u_1) or m + r (u_2)
+    for (int e = threadIdx.x; e < m * NTP; e += blockDim.x) {
+        const int r = e / NTP, j = e % NTP;
+        const int c = (nd.is_leaf || r < mrl) ? r : r + kvl;
+        bv[e] = (j < nt && c == j0 + j) ? TF(1) : TF(0);
+    }
+    for (int e = threadIdx.x; e < (kvl + kvr) * NTP; e += blockDim.x) {
+        const int r = e / NTP, j = e % NTP;
+        const int c = r < kvl ? mrl + r : m + r;
+        us[e] = (j < nt && c == j0 + j) ? TF(1) : TF(0);
+    }
+    __syncthreads();
+    if (!nd.is_leaf) {
+        // b = [b_1 - P12 u_2; b_2 - P21 u_1]   (ulvfactor.jl:74)
+        const int mrr = m - mrl;
+        gemm<false>(mrl, nt, kvr, I.fac + f.P12, mrl, us + kvl * NTP,
+                    [&](int i, int j, TF v) { bv[i * NTP + j] -= v; });
+        gemm<false>(mrr, nt, kvl, I.fac + f.P21, mrr, us,
+                    [&](int i, int j, TF v) { bv[(mrl + i) * NTP + j] -= v; });
+        __syncthreads();
+    }
+    TF* Grc = Gr + (int64_t)j0 * R;  // column j0 of Gr
     if (!f.pass) {
         // b <- Q_ql' b = b - Y (T (Y' b))   (ulvfactor.jl:42)
         if (k > 0) {
-            gemm<true>(k, nt, m, I.fac + f.Yql, m, bv, [&](int i, int j, TV v) { t1[i * NTP + j] = v; });
+            gemm<true>(k, nt, m, I.fac + f.Yql, m, bv, [&](int i, int j, TF v) { t1[i * NTP + j] = v; });
             __syncthreads();
-            gemm<false>(k, nt, k, I.fac + f.Tql, k, t1, [&](int i, int j, TV v) { t2[i * NTP + j] = v; });
+            gemm<false>(k, nt, k, I.fac + f.Tql, k, t1, [&](int i, int j, TF v) { t2[i * NTP + j] = v; });
             __syncthreads();
-            gemm<false>(m, nt, k, I.fac + f.Yql, m, t2, [&](int i, int j, TV v) { bv[i * NTP + j] -= v; });
+            gemm<false>(m, nt, k, I.fac + f.Yql, m, t2, [&](int i, int j, TF v) { bv[i * NTP + j] -= v; });
             __syncthreads();
         }
         // zloc = L1 \ b[0:nk]  (ulvfactor.jl:48), blocked with inverted diagonal blocks
         for (int bb = 0; bb < f.nblk; ++bb) {
             const int b0 = bb * NB, nb = min(NB, nk - b0);
             gemm<false>(nb, nt, b0, I.fac + f.L1 + b0, nk, zl,
-                        [&](int i, int j, TV v) { t1[i * NTP + j] = bv[(b0 + i) * NTP + j] - v; });
+                        [&](int i, int j, TF v) { t1[i * NTP + j] = bv[(b0 + i) * NTP + j] - v; });
             __syncthreads();
             gemm<false>(nb, nt, nb, I.fac + f.L1inv + (int64_t)bb * NB * NB, NB, t1,
-                        [&](int i, int j, TV v) { zl[(b0 + i) * NTP + j] = v; });
+                        [&](int i, int j, TF v) { zl[(b0 + i) * NTP + j] = v; });
             __syncthreads();
         }
-        // reduced b = b[nk:m] - L2a zloc  (ulvfactor.jl:49) -> parent's input
-        gemm<false>(k, nt, nk, I.fac + f.L2a, k, zl,
-                    [&](int i, int j, TV v) { wbn[i + (int64_t)j * mr] = bv[(nk + i) * NTP + j] - v; });
-        // zloc -> the node's z vector (rows 0..nk-1)
-        unstage(I.wz + f.zrow * nrhs + (int64_t)j0 * f.n, f.n, zl, 0, nk, nt);
+        unstage(I.fac + f.Gz + (int64_t)j0 * nk, nk, zl, 0, nk, nt);
+        unstage(Grc, R, bv, nk, mr, nt);  // (Q_ql' b)[nk:m]
     } else {
-        unstage(wbn, mr, bv, 0, m, nt);
+        unstage(Grc, R, bv, 0, m, nt);  // nothing eliminated: reduced b = b
     }
-    // u = V_top' zloc + W1' u1 + W2' u2
+    // u part: W_1' u_1 + W_2' u_2 (ulvfactor.jl:89); a leaf has none
     if (kv > 0) {
-        const bool has_z = !f.pass && nk > 0;
+        TF* Gu = Grc + mr;
         if (nd.is_leaf) {
-            if (has_z)
-                gemm<true>(kv, nt, nk, I.fac + f.Vt, nk, zl, [&](int i, int j, TV v) { ug[i + (int64_t)j * kv] = v; });
-            else
-                for (int e = threadIdx.x; e < kv * nt; e += blockDim.x) ug[(e % kv) + (int64_t)(e / kv) * kv] = TV(0);
+            for (int e = threadIdx.x; e < kv * nt; e += blockDim.x) Gu[(e % kv) + (int64_t)(e / kv) * R] = TF(0);
         } else {
             const DevNode dl = I.nodes[nd.left], dr = I.nodes[nd.right];
-            if (has_z)
-                gemm<true>(kv, nt, nk, I.fac + f.Vt, nk, zl, [&](int i, int j, TV v) { t2[i * NTP + j] = v; });
-            else
-                for (int e = threadIdx.x; e < kv * nt; e += blockDim.x) t2[(e % kv) * NTP + e / kv] = TV(0);
-            __syncthreads();
-            gemm<true>(kv, nt, kvl, I.blob + dl.W, kvl, us, [&](int i, int j, TV v) { t2[i * NTP + j] += v; });
+            gemm<true>(kv, nt, kvl, I.blob + dl.W, kvl, us, [&](int i, int j, TF v) { t2[i * NTP + j] = v; });
             __syncthreads();
             gemm<true>(kv, nt, kvr, I.blob + dr.W, kvr, us + kvl * NTP,
-                       [&](int i, int j, TV v) { ug[i + (int64_t)j * kv] = t2[i * NTP + j] + v; });
+                       [&](int i, int j, TF v) { Gu[i + (int64_t)j * R] = t2[i * NTP + j] + v; });
         }
     }
 }
 
-// Root: b assembled like a branch, z = D^-1 b with the inverse formed at factorization
-template <typename TF, typename TV>
-__global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_sroot(const SInst<TF, TV>* __restrict__ insts,
+// Qt = Q_lq' explicitly (ulvfactor.jl:102): column c is H(1)...H(nk) e_c, applied
+// as blocks of NB reflectors z - Y_b T_b Y_b' z, the last block first.
+// LDS: z n x NTP | t1, t2 NB x NTP
+template <typename TF>
+__global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_qform(const FInst<TF>* __restrict__ insts,
                                                             const WorkItem* __restrict__ work) {
     const WorkItem wi = work[blockIdx.x];
-    const SInst<TF, TV>& I = insts[wi.inst];
+    const FInst<TF>& I = insts[wi.inst];
+    const FNode f = I.fn[wi.node];
+    if (f.Qt < 0) return;
+    const int n = f.n;
+    const int j0 = blockIdx.y * RHS_TILE;
+    if (j0 >= n) return;
+    const int nt = min(RHS_TILE, n - j0);
+    const int nk = f.nk;
+    extern __shared__ unsigned char lds_raw[];
+    TF* z = reinterpret_cast<TF*>(lds_raw);
+    TF* t1 = z + n * NTP;
+    TF* t2 = t1 + NB * NTP;
+    for (int e = threadIdx.x; e < n * NTP; e += blockDim.x) {
+        const int r = e / NTP, j = e % NTP;
+        z[e] = (j < nt && r == j0 + j) ? TF(1) : TF(0);
+    }
+    __syncthreads();
+    for (int bb = f.nblk - 1; bb >= 0; --bb) {
+        const int b0 = bb * NB, nb = min(NB, nk - b0);
+        const TF* Yb = I.fac + f.Ylq + (int64_t)b0 * n + b0;  // rows b0.., columns b0..b0+nb
+        gemm<true>(nb, nt, n - b0, Yb, n, z + b0 * NTP, [&](int i, int j, TF v) { t1[i * NTP + j] = v; });
+        __syncthreads();
+        gemm<false>(nb, nt, nb, I.fac + f.Tlq + (int64_t)bb * NB * NB, NB, t1,
+                    [&](int i, int j, TF v) { t2[i * NTP + j] = v; });
+        __syncthreads();
+        gemm<false>(n - b0, nt, nb, Yb, n, t2, [&](int i, int j, TF v) { z[(b0 + i) * NTP + j] -= v; });
+        __syncthreads();
+    }
+    unstage(I.fac + f.Qt + (int64_t)j0 * n, n, z, 0, n, nt);
+}
+
+// ---------------------------------------------------------------------------
+// ULV solve
+// ---------------------------------------------------------------------------
+// Root (ulvfactor.jl:74,83): b = [b_1 - P12 u_2; b_2 - P21 u_1] from the children's
+// y blocks, then the LU solve with partial pivoting (blocked triangular solves with
+// the inverted diagonal blocks of L and U).
+// LDS (row-major, NTP columns): bv m rows | us kv_1 + kv_2 | t1 NB
+template <typename TF>
+__global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_groot(const SInst<TF>* __restrict__ insts,
+                                                            const WorkItem* __restrict__ work) {
+    const WorkItem wi = work[blockIdx.x];
+    const SInst<TF>& I = insts[wi.inst];
     const int j0 = blockIdx.y * RHS_TILE;
     if (j0 >= I.nrhs) return;
     const int nt = min(RHS_TILE, I.nrhs - j0);
+    const int nrhs = I.nrhs;
     const DevNode nd = I.nodes[wi.node];
     const FNode f = I.fn[wi.node];
     const int m = f.m;
@@ -1015,17 +1058,34 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_sroot(const SInst<TF, TV>
         kvr = I.nodes[nd.right].kv;
     }
     extern __shared__ unsigned char lds_raw[];
-    TV* bv = reinterpret_cast<TV*>(lds_raw);
-    TV* us = bv + m * NTP;
-    (void)kvr;
-    TV* t1 = us + (kvl + kvr) * NTP;  // NB x NTP
-    solve_form_b(I, nd, f, j0, nt, bv, us);
+    TF* bv = reinterpret_cast<TF*>(lds_raw);
+    TF* us = bv + m * NTP;
+    TF* t1 = us + (kvl + kvr) * NTP;
+    if (nd.is_leaf) {
+        stage(bv, 0, I.B + nd.row_off + (int64_t)j0 * I.ldb, I.ldb, m, nt);
+        __syncthreads();
+    } else {
+        const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
+        const TF* yl = I.wy + fl.yrow * nrhs + (int64_t)j0 * fl.My;
+        const TF* yr = I.wy + fr.yrow * nrhs + (int64_t)j0 * fr.My;
+        stage(bv, 0, yl + fl.nk, fl.My, fl.mr, nt);
+        stage(bv, fl.mr, yr + fr.nk, fr.My, fr.mr, nt);
+        stage(us, 0, yl + fl.nk + fl.mr, fl.My, kvl, nt);
+        stage(us, kvl, yr + fr.nk + fr.mr, fr.My, kvr, nt);
+        __syncthreads();
+        gemm<false>(fl.mr, nt, kvr, I.fac + f.P12, fl.mr, us + kvl * NTP,
+                    [&](int i, int j, TF v) { bv[i * NTP + j] -= v; });
+        const int mr1 = fl.mr;
+        gemm<false>(fr.mr, nt, kvl, I.fac + f.P21, fr.mr, us,
+                    [&](int i, int j, TF v) { bv[(mr1 + i) * NTP + j] -= v; });
+        __syncthreads();
+    }
     // row interchanges of the partial pivoting (one thread per column)
     const int32_t* piv = I.ipiv + f.piv;
     for (int j = threadIdx.x; j < nt; j += blockDim.x)
         for (int i = 0; i < m; ++i) {
             const int p = piv[i];
-            if (p != i) { const TV t = bv[i * NTP + j]; bv[i * NTP + j] = bv[p * NTP + j]; bv[p * NTP + j] = t; }
+            if (p != i) { const TF t = bv[i * NTP + j]; bv[i * NTP + j] = bv[p * NTP + j]; bv[p * NTP + j] = t; }
         }
     __syncthreads();
     const TF* LU = I.fac + f.LU;
@@ -1035,62 +1095,280 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_sroot(const SInst<TF, TV>
     // forward: y_b = L_bb^-1 (b_b - L[b, 0:b0] y[0:b0])
     for (int bb = 0; bb < nblk; ++bb) {
         const int b0 = bb * NB, nb = min(NB, m - b0);
-        gemm<false>(nb, nt, b0, LU + b0, m, bv, [&](int i, int j, TV v) { t1[i * NTP + j] = bv[(b0 + i) * NTP + j] - v; });
+        gemm<false>(nb, nt, b0, LU + b0, m, bv, [&](int i, int j, TF v) { t1[i * NTP + j] = bv[(b0 + i) * NTP + j] - v; });
         __syncthreads();
-        gemm<false>(nb, nt, nb, Li + (int64_t)bb * NB * NB, NB, t1, [&](int i, int j, TV v) { bv[(b0 + i) * NTP + j] = v; });
+        gemm<false>(nb, nt, nb, Li + (int64_t)bb * NB * NB, NB, t1, [&](int i, int j, TF v) { bv[(b0 + i) * NTP + j] = v; });
         __syncthreads();
     }
     // backward: x_b = U_bb^-1 (y_b - U[b, b1:m] x[b1:m])
     for (int bb = nblk - 1; bb >= 0; --bb) {
         const int b0 = bb * NB, nb = min(NB, m - b0), b1 = b0 + nb;
         gemm<false>(nb, nt, m - b1, LU + b0 + (int64_t)b1 * m, m, bv + b1 * NTP,
-                    [&](int i, int j, TV v) { t1[i * NTP + j] = bv[(b0 + i) * NTP + j] - v; });
+                    [&](int i, int j, TF v) { t1[i * NTP + j] = bv[(b0 + i) * NTP + j] - v; });
         __syncthreads();
-        gemm<false>(nb, nt, nb, Ui + (int64_t)bb * NB * NB, NB, t1, [&](int i, int j, TV v) { bv[(b0 + i) * NTP + j] = v; });
+        gemm<false>(nb, nt, nb, Ui + (int64_t)bb * NB * NB, NB, t1, [&](int i, int j, TF v) { bv[(b0 + i) * NTP + j] = v; });
         __syncthreads();
     }
     if (nd.is_leaf) unstage(I.X + nd.col_off + (int64_t)j0 * I.ldx, I.ldx, bv, 0, m, nt);
-    else unstage(I.wz + f.zrow * I.nrhs + (int64_t)j0 * f.n, f.n, bv, 0, m, nt);
+    else unstage(I.wz + f.zrow * nrhs + (int64_t)j0 * f.n, f.n, bv, 0, m, nt);
 }
 
-// Down-sweep: z = [zloc; z_from_parent], z <- Q_lq' z  (ulvfactor.jl:98-107)
-template <typename TF, typename TV>
-__global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_sdown(const SInst<TF, TV>* __restrict__ insts,
-                                                            const WorkItem* __restrict__ work) {
-    const WorkItem wi = work[blockIdx.x];
-    const SInst<TF, TV>& I = insts[wi.inst];
-    const int j0 = blockIdx.y * RHS_TILE;
-    if (j0 >= I.nrhs) return;
-    const int nt = min(RHS_TILE, I.nrhs - j0);
-    const DevNode nd = I.nodes[wi.node];
-    const FNode f = I.fn[wi.node];
-    const int n = f.n, nk = f.nk;
-    const int nrhs = I.nrhs;
-    extern __shared__ unsigned char lds_raw[];
-    TV* z = reinterpret_cast<TV*>(lds_raw);  // n x NTP
-    TV* t1 = z + n * NTP;                     // NB x NTP
-    TV* t2 = t1 + NB * NTP;                   // NB x NTP
-    const FNode fp = I.fn[nd.parent];
-    const int off = nd.is_left ? 0 : I.fn[I.nodes[nd.parent].left].nr;
-    stage(z, 0, I.wz + f.zrow * nrhs + (int64_t)j0 * n, n, nk, nt);
-    stage(z, nk, I.wz + fp.zrow * nrhs + (int64_t)j0 * fp.n + off, fp.n, n - nk, nt);
+// ---- MFMA tile GEMM of the solve sweeps ----
+// y(0:M, 0:N) = A(0:M, 0:K) x(0:K, 0:N) over one workgroup tile of TM x TN outputs.
+// A is column-major (lda = M) in the factor type, or the identity when null.  x is
+// two stacked column-major segments (rows [0, r0) from seg 0, the rest from seg 1),
+// each Float64 (the caller's B) or the factor type (workspace blocks); y likewise.
+// The 4 waves form a WM x WN grid; each wave owns 2 x 2 blocks of 16 x 16 outputs
+// computed with v_mfma_*_16x16x4 (lane l supplies A(l%16, l/16) and x(l/16, l%16)
+// and receives 4 outputs of column l%16: rows 4r + l/16 in register r for the
+// Float64 instruction, rows 4(l/16) + r for the Float32 one).  K advances in chunks
+// of GK, staged through double-buffered LDS with one barrier per chunk.
+template <typename TF>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * r + lk; }
+};
+template <>
+struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
+
+// x operand: rows [0, r0) from segment 0, [r0, r1) from segment 1 (global,
+// column-major, Float64 or the factor type), rows [r1, K) from an LDS block zs
+// (row-major, pitch zp, columns local to the workgroup tile).
+struct SegOp {
+    const void* p0;
+    const void* p1;
+    int64_t ld0, ld1;
+    int r0, r1, dbl0, dbl1;
+};
+
+constexpr int GK = 16;  // K chunk of the MFMA tile GEMM
+
+template <typename TF>
+__device__ __forceinline__ TF seg_load(const SegOp& x, const TF* zs, int zp, int r, int64_t j, int jl) {
+    if (r < x.r0)
+        return x.dbl0 ? (TF)static_cast<const double*>(x.p0)[r + j * x.ld0] : static_cast<const TF*>(x.p0)[r + j * x.ld0];
+    if (r < x.r1) {
+        r -= x.r0;
+        return x.dbl1 ? (TF)static_cast<const double*>(x.p1)[r + j * x.ld1] : static_cast<const TF*>(x.p1)[r + j * x.ld1];
+    }
+    return zs[(r - x.r1) * zp + jl];
+}
+
+// LDS staging of one tile GEMM: two buffers of A (GK x (TM + 1)) and x (GK x (TN + 1)).
+template <typename TF, int WM, int WN>
+struct TileLds {
+    static constexpr int TM = 32 * WM, TN = 32 * WN, AP = TM + 1, XP = TN + 1;
+    TF As[2][GK * AP];
+    TF Xs[2][GK * XP];
+};
+
+// y(i0:i0+TM, j0:j0+TN) = A(., 0:K) x(0:K, .), outputs passed to store(i, j, v) for
+// i < M, j < N.
+template <typename TF, int WM, int WN, typename Store>
+__device__ __forceinline__ void tile_gemm(TileLds<TF, WM, WN>& L, int M, int N, int K, const TF* __restrict__ A,
+                                          int64_t lda, const SegOp& x, const TF* zs, int zp, int i0, int j0,
+                                          Store store) {
+    constexpr int TM = 32 * WM, TN = 32 * WN;
+    constexpr int AP = TM + 1, XP = TN + 1;  // padded LDS rows
+    constexpr int NA = TM * GK / 256, NX = GK * TN / 256;
+    auto& As = L.As;
+    auto& Xs = L.Xs;
+    using V4 = typename Mfma<TF>::v4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    TF ra[NA], rx[NX];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            const int e = tid + q * 256;
+            const int i = e % TM, kk = e / TM;
+            const int gi = i0 + i, gk = k0 + kk;
+            TF v = TF(0);
+            if (gi < M && gk < K) v = A ? A[gi + (int64_t)gk * lda] : (gi == gk ? TF(1) : TF(0));
+            ra[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            const int e = tid + q * 256;
+            const int kk = e % GK, j = e / GK;
+            const int gk = k0 + kk, gj = j0 + j;
+            rx[q] = (gk < K && gj < N) ? seg_load<TF>(x, zs, zp, gk, gj, j) : TF(0);
+        }
+    };
+    auto put = [&](int b) {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            const int e = tid + q * 256;
+            As[b][(e / TM) * AP + e % TM] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            const int e = tid + q * 256;
+            Xs[b][(e % GK) * XP + e / GK] = rx[q];
+        }
+    };
+    V4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = V4{TF(0), TF(0), TF(0), TF(0)};
+    const int nch = (K + GK - 1) / GK;
+    load(0);
+    put(0);
     __syncthreads();
-    // Q' = H(1)...H(nk) = P_0 ... P_{B-1}, P_b = I - Y_b T_b Y_b'; apply P_{B-1} first
-    for (int bb = f.nblk - 1; bb >= 0; --bb) {
-        const int b0 = bb * NB, nb = min(NB, nk - b0);
-        const TF* Yb = I.fac + f.Ylq + (int64_t)b0 * n + b0;  // rows b0.., columns b0..b0+nb
-        gemm<true>(nb, nt, n - b0, Yb, n, z + b0 * NTP, [&](int i, int j, TV v) { t1[i * NTP + j] = v; });
-        __syncthreads();
-        gemm<false>(nb, nt, nb, I.fac + f.Tlq + (int64_t)bb * NB * NB, NB, t1,
-                    [&](int i, int j, TV v) { t2[i * NTP + j] = v; });
-        __syncthreads();
-        gemm<false>(n - b0, nt, nb, Yb, n, t2, [&](int i, int j, TV v) { z[(b0 + i) * NTP + j] -= v; });
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int c = 0; c < nch; ++c) {
+        const int b = c & 1;
+        if (c + 1 < nch) load((c + 1) * GK);
+#pragma unroll
+        for (int kk = 0; kk < GK; kk += 4) {
+            TF av[2], xv[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                av[t] = As[b][(kk + lk) * AP + wm * 32 + t * 16 + lr];
+                xv[t] = Xs[b][(kk + lk) * XP + wn * 32 + t * 16 + lr];
+            }
+#pragma unroll
+            for (int ra_ = 0; ra_ < 2; ++ra_)
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb) acc[ra_][cb] = Mfma<TF>::mac(av[ra_], xv[cb], acc[ra_][cb]);
+        }
+        if (c + 1 < nch) put(b ^ 1);
         __syncthreads();
     }
-    if (nd.is_leaf) unstage(I.X + nd.col_off + (int64_t)j0 * I.ldx, I.ldx, z, 0, n, nt);
-    else unstage(I.wz + f.zrow * nrhs + (int64_t)j0 * n, n, z, 0, n, nt);
+#pragma unroll
+    for (int ra_ = 0; ra_ < 2; ++ra_)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+            const int gj = j0 + wn * 32 + cb * 16 + lr;
+            if (gj >= N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = i0 + wm * 32 + ra_ * 16 + Mfma<TF>::row(lk, r);
+                if (gi < M) store(gi, gj, acc[ra_][cb][r]);
+            }
+        }
 }
 
+// Flat tile index -> (work item, row tile, column tile).  Launches are padded to a
+// multiple of 8 workgroups; consecutive workgroups go to different XCDs, so the
+// tiles of one node are gathered onto one XCD (and its L2).
+__device__ __forceinline__ bool tile_of(int total, int tm, int tn, int& item, int& ti, int& tj) {
+    const int per = gridDim.x / 8;
+    const int logical = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (logical >= total) return false;
+    item = logical / (tm * tn);
+    const int rem = logical % (tm * tn);
+    ti = rem % tm;
+    tj = rem / tm;
+    return true;
+}
+
+// Up-sweep of one height (ulvfactor.jl:42,48-51,74,89 for every node of the level),
+// one workgroup per node and TN-column tile, in two phases:
+//   zloc = Gz x                   (stored to y and kept in LDS)
+//   [reduced b; u] = Gr [x; zloc]
+// x = the leaf's rows of B, or the children's reduced b and u (rows nk.. of their y).
+// Dynamic LDS: zloc, (max nk of the level) x (TN + 1).
+template <typename TF>
+__global__ __launch_bounds__(256) void k_ulv_gup(const SInst<TF>* __restrict__ insts,
+                                                 const WorkItem* __restrict__ work, int nwork, int tn) {
+    constexpr int WM = 4, WN = 1, TM = 32 * WM, TN = 32 * WN, ZP = TN + 1;
+    int item, ti, tj;
+    if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
+    const WorkItem wi = work[item];
+    const SInst<TF>& I = insts[wi.inst];
+    const FNode f = I.fn[wi.node];
+    const int nrhs = I.nrhs;
+    const int j0 = tj * TN;
+    if (j0 >= nrhs) return;
+    const DevNode nd = I.nodes[wi.node];
+    extern __shared__ unsigned char lds_raw[];
+    TF* zs = reinterpret_cast<TF*>(lds_raw);
+    SegOp x;
+    if (nd.is_leaf) {
+        x.p0 = I.B + nd.row_off;
+        x.ld0 = I.ldb;
+        x.dbl0 = 1;
+        x.r0 = f.Kx;
+        x.p1 = nullptr;
+        x.ld1 = 0;
+        x.dbl1 = 0;
+    } else {
+        const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
+        x.p0 = I.wy + fl.yrow * nrhs + fl.nk;
+        x.ld0 = fl.My;
+        x.dbl0 = 0;
+        x.r0 = fl.My - fl.nk;
+        x.p1 = I.wy + fr.yrow * nrhs + fr.nk;
+        x.ld1 = fr.My;
+        x.dbl1 = 0;
+    }
+    x.r1 = f.Kx;
+    TF* y = I.wy + f.yrow * nrhs;
+    const int My = f.My, nk = f.nk, Kx = f.Kx, R = My - nk;
+    __shared__ TileLds<TF, WM, WN> L;
+    // phase 0 rows [0, nk) of y from Gz, phase 1 rows [nk, My) from Gr
+    for (int ph = 0; ph < 2; ++ph) {
+        const int M = ph ? R : nk, K = ph ? Kx + nk : Kx;
+        const TF* A = I.fac + (ph ? f.Gr : f.Gz);
+        TF* yo = y + (ph ? nk : 0);
+        for (int p0 = 0; p0 < M; p0 += TM)
+            tile_gemm<TF, WM, WN>(L, M, nrhs, K, A, M, x, zs, ZP, p0, j0, [&](int i, int j, TF v) {
+                yo[i + (int64_t)j * My] = v;
+                if (!ph) zs[i * ZP + (j - j0)] = v;
+            });
+        __syncthreads();
+    }
+}
+
+// Down-sweep of one depth: z = Qt [zloc; z from the parent] (ulvfactor.jl:98-107);
+// leaves write the solution X.
+template <typename TF, int WM, int WN>
+__global__ __launch_bounds__(256) void k_ulv_gdown(const SInst<TF>* __restrict__ insts,
+                                                   const WorkItem* __restrict__ work, int nwork, int tm, int tn) {
+    int item, ti, tj;
+    if (!tile_of(nwork * tm * tn, tm, tn, item, ti, tj)) return;
+    const WorkItem wi = work[item];
+    const SInst<TF>& I = insts[wi.inst];
+    const FNode f = I.fn[wi.node];
+    const int nrhs = I.nrhs, n = f.n;
+    const int i0 = ti * 32 * WM, j0 = tj * 32 * WN;
+    if (i0 >= n || j0 >= nrhs) return;
+    const DevNode nd = I.nodes[wi.node];
+    const FNode fp = I.fn[nd.parent];
+    const int off = nd.is_left ? 0 : I.fn[I.nodes[nd.parent].left].nr;
+    SegOp x;
+    x.p0 = I.wy + f.yrow * nrhs;
+    x.ld0 = f.My;
+    x.dbl0 = 0;
+    x.r0 = f.nk;
+    x.p1 = I.wz + fp.zrow * nrhs + off;
+    x.ld1 = fp.n;
+    x.dbl1 = 0;
+    x.r1 = n;
+    const TF* Q = f.Qt >= 0 ? I.fac + f.Qt : nullptr;
+    __shared__ TileLds<TF, WM, WN> L;
+    const bool leaf = nd.is_leaf;
+    double* X = I.X + nd.col_off;
+    const int64_t ldx = I.ldx;
+    TF* z = I.wz + f.zrow * nrhs;
+    tile_gemm<TF, WM, WN>(L, n, nrhs, n, Q, n, x, nullptr, 0, i0, j0, [&](int i, int j, TF v) {
+        if (leaf) X[i + (int64_t)j * ldx] = (double)v;
+        else z[i + (int64_t)j * n] = v;
+    });
+}
 // ---------------------------------------------------------------------------
 // vector helpers for refinement
 // ---------------------------------------------------------------------------
@@ -1233,37 +1511,79 @@ hipError_t launch_ulv_root(const FInst<T>* insts, const WorkItem* work, int nwor
     hipLaunchKernelGGL((k_ulv_root<T>), dim3(nwork), dim3(1024), 0, st, insts, work);
     return hipGetLastError();
 }
-template <typename TF, typename TV>
-hipError_t launch_ulv_sup(const SInst<TF, TV>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                          hipStream_t st) {
-    if (nwork <= 0) return hipSuccess;
-    const void* fn = reinterpret_cast<const void*>(&k_ulv_sup<TF, TV>);
-    hipError_t e = set_lds_attr<TV>(fn, lds);
+template <typename T>
+hipError_t launch_ulv_gform(const FInst<T>* insts, const WorkItem* work, int nwork, int max_kx, size_t lds,
+                            hipStream_t st) {
+    if (nwork <= 0 || max_kx <= 0) return hipSuccess;
+    const void* fn = reinterpret_cast<const void*>(&k_ulv_gform<T>);
+    hipError_t e = set_lds_attr<T>(fn, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_sup<TF, TV>), dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st, insts,
+    hipLaunchKernelGGL((k_ulv_gform<T>), dim3(nwork, rhs_tiles(max_kx)), dim3(SWEEP_THREADS), lds, st, insts, work);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_ulv_qform(const FInst<T>* insts, const WorkItem* work, int nwork, int max_n, size_t lds,
+                            hipStream_t st) {
+    if (nwork <= 0 || max_n <= 0) return hipSuccess;
+    const void* fn = reinterpret_cast<const void*>(&k_ulv_qform<T>);
+    hipError_t e = set_lds_attr<T>(fn, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_ulv_qform<T>), dim3(nwork, rhs_tiles(max_n)), dim3(SWEEP_THREADS), lds, st, insts, work);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_ulv_groot(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                            hipStream_t st) {
+    if (nwork <= 0) return hipSuccess;
+    const void* fn = reinterpret_cast<const void*>(&k_ulv_groot<T>);
+    hipError_t e = set_lds_attr<T>(fn, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_ulv_groot<T>), dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st, insts,
                        work);
     return hipGetLastError();
 }
-template <typename TF, typename TV>
-hipError_t launch_ulv_sroot(const SInst<TF, TV>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                            hipStream_t st) {
-    if (nwork <= 0) return hipSuccess;
-    const void* fn = reinterpret_cast<const void*>(&k_ulv_sroot<TF, TV>);
-    hipError_t e = set_lds_attr<TV>(fn, lds);
+
+// Tile shapes of the MFMA sweeps: 64 x 64 outputs per workgroup for more than 32
+// right-hand sides, 128 x 32 otherwise.
+template <typename T>
+static hipError_t sweep_grid(int nwork, int max_m, int max_nrhs, int tm_rows, int tn_cols, dim3& grid, int& tm,
+                             int& tn) {
+    tm = (max_m + tm_rows - 1) / tm_rows;
+    tn = (max_nrhs + tn_cols - 1) / tn_cols;
+    const int64_t total = (int64_t)nwork * tm * tn;
+    if (total > (int64_t)1 << 30) return hipErrorInvalidValue;
+    grid = dim3((unsigned)(8 * ((total + 7) / 8)));
+    return hipSuccess;
+}
+
+template <typename T>
+hipError_t launch_ulv_gup(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nk, int max_nrhs,
+                          hipStream_t st) {
+    if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
+    const int tn = (max_nrhs + 31) / 32;
+    const int64_t total = (int64_t)nwork * tn;
+    if (total > (int64_t)1 << 30) return hipErrorInvalidValue;
+    const size_t lds = sizeof(T) * (size_t)(max_nk > 0 ? max_nk : 1) * 33;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gup<T>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_sroot<TF, TV>), dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st,
-                       insts, work);
+    hipLaunchKernelGGL((k_ulv_gup<T>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), lds, st, insts, work, nwork,
+                       tn);
     return hipGetLastError();
 }
-template <typename TF, typename TV>
-hipError_t launch_ulv_sdown(const SInst<TF, TV>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+template <typename T>
+hipError_t launch_ulv_gdown(const SInst<T>* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
                             hipStream_t st) {
-    if (nwork <= 0) return hipSuccess;
-    const void* fn = reinterpret_cast<const void*>(&k_ulv_sdown<TF, TV>);
-    hipError_t e = set_lds_attr<TV>(fn, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_sdown<TF, TV>), dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st,
-                       insts, work);
+    if (nwork <= 0 || max_m <= 0 || max_nrhs <= 0) return hipSuccess;
+    dim3 grid;
+    int tm, tn;
+    if (max_nrhs > 32) {
+        if (hipError_t e = sweep_grid<T>(nwork, max_m, max_nrhs, 64, 64, grid, tm, tn)) return e;
+        hipLaunchKernelGGL((k_ulv_gdown<T, 2, 2>), grid, dim3(256), 0, st, insts, work, nwork, tm, tn);
+    } else {
+        if (hipError_t e = sweep_grid<T>(nwork, max_m, max_nrhs, 128, 32, grid, tm, tn)) return e;
+        hipLaunchKernelGGL((k_ulv_gdown<T, 4, 1>), grid, dim3(256), 0, st, insts, work, nwork, tm, tn);
+    }
     return hipGetLastError();
 }
 
@@ -1307,17 +1627,14 @@ template hipError_t launch_ulv_wy<double>(const FInst<double>*, const WorkItem*,
 template hipError_t launch_ulv_wy<float>(const FInst<float>*, const WorkItem*, int, hipStream_t);
 template hipError_t launch_ulv_root<double>(const FInst<double>*, const WorkItem*, int, hipStream_t);
 template hipError_t launch_ulv_root<float>(const FInst<float>*, const WorkItem*, int, hipStream_t);
-template hipError_t launch_ulv_sup<double, double>(const SInst<double, double>*, const WorkItem*, int, int, size_t,
-                                                   hipStream_t);
-template hipError_t launch_ulv_sup<float, float>(const SInst<float, float>*, const WorkItem*, int, int, size_t,
-                                                 hipStream_t);
-template hipError_t launch_ulv_sroot<double, double>(const SInst<double, double>*, const WorkItem*, int, int, size_t,
-                                                     hipStream_t);
-template hipError_t launch_ulv_sroot<float, float>(const SInst<float, float>*, const WorkItem*, int, int, size_t,
-                                                   hipStream_t);
-template hipError_t launch_ulv_sdown<double, double>(const SInst<double, double>*, const WorkItem*, int, int, size_t,
-                                                     hipStream_t);
-template hipError_t launch_ulv_sdown<float, float>(const SInst<float, float>*, const WorkItem*, int, int, size_t,
-                                                   hipStream_t);
+#define HSS_INST_SOLVE(T)                                                                                      \
+    template hipError_t launch_ulv_gform<T>(const FInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
+    template hipError_t launch_ulv_qform<T>(const FInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
+    template hipError_t launch_ulv_gup<T>(const SInst<T>*, const WorkItem*, int, int, int, hipStream_t);            \
+    template hipError_t launch_ulv_groot<T>(const SInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
+    template hipError_t launch_ulv_gdown<T>(const SInst<T>*, const WorkItem*, int, int, int, hipStream_t);
+HSS_INST_SOLVE(double)
+HSS_INST_SOLVE(float)
+#undef HSS_INST_SOLVE
 
 }  // namespace hss

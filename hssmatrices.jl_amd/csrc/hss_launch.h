@@ -19,14 +19,22 @@ template <typename T>
 hipError_t launch_ulv_wy(const FInst<T>* insts, const WorkItem* work, int nwork, hipStream_t st);
 template <typename T>
 hipError_t launch_ulv_root(const FInst<T>* insts, const WorkItem* work, int nwork, hipStream_t st);
-template <typename TF, typename TV>
-hipError_t launch_ulv_sup(const SInst<TF, TV>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                          hipStream_t st);
-template <typename TF, typename TV>
-hipError_t launch_ulv_sroot(const SInst<TF, TV>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+// solve operators, once per factorization (every non-root node in one launch)
+template <typename T>
+hipError_t launch_ulv_gform(const FInst<T>* insts, const WorkItem* work, int nwork, int max_kx, size_t lds,
                             hipStream_t st);
-template <typename TF, typename TV>
-hipError_t launch_ulv_sdown(const SInst<TF, TV>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+template <typename T>
+hipError_t launch_ulv_qform(const FInst<T>* insts, const WorkItem* work, int nwork, int max_n, size_t lds,
+                            hipStream_t st);
+// solve sweeps: max_nk = largest nk (up), max_m = largest n (down) of the level's nodes
+template <typename T>
+hipError_t launch_ulv_gup(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nk, int max_nrhs,
+                          hipStream_t st);
+template <typename T>
+hipError_t launch_ulv_groot(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                            hipStream_t st);
+template <typename T>
+hipError_t launch_ulv_gdown(const SInst<T>* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
                             hipStream_t st);
 
 hipError_t launch_resid_batch(const BatchVec* tab, int cnt, int maxc, double* norms, hipStream_t st);

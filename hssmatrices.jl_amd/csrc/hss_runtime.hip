@@ -137,6 +137,15 @@ struct SlotCache {
     }
 };
 
+// Level schedule of a set of solve requests: up-sweep by height, roots, down-sweep by
+// depth; up_m = the largest nk, down_m = the largest n of a level's nodes.
+struct Schedule {
+    std::vector<WorkItem> work;
+    std::vector<Range> up, down;
+    std::vector<int> up_m, down_m;
+    Range roots;
+};
+
 struct PlanKey {
     int nrhs;
     const double* dB;
@@ -148,14 +157,26 @@ struct PlanKey {
     }
 };
 static void destroy_plan_cache(std::vector<std::pair<PlanKey, hss_plan_t>>& cache);
+struct hss_fact_s;
+// hold = the plan takes references on its factors and context (every plan but those a
+// factor caches for itself)
+static hss_status plan_create_impl(int32_t count, hss_fact_s* const* facts, const int32_t* nrhs,
+                                   const double* const* dB, const int64_t* ldb, double* const* dX, const int64_t* ldx,
+                                   int32_t with_refinement, bool hold, hss_plan_t* out);
 
+// Handles hold references on what they use (a matrix on its context, a factor on
+// its matrix, a plan on its factors and context, a server on its context), so
+// destroying them in any order -- as garbage-collected bindings do -- is safe: a
+// handle is released when it is destroyed and no other handle uses it.
 struct hss_ctx_s {
+    int refs = 1;
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 struct hss_mat_s {
+    int refs = 1;
     hss_ctx_s* ctx = nullptr;
     int nn = 0;
     int64_t rows = 0, cols = 0;
@@ -184,6 +205,7 @@ struct hss_mat_s {
 };
 
 struct hss_fact_s {
+    int refs = 1;
     hss_mat_s* mat = nullptr;
     hss_prec prec = HSS_PREC_FP64;
     std::vector<FNode> hfn;
@@ -192,15 +214,17 @@ struct hss_fact_s {
     int64_t fac_len = 0;
     int32_t* d_piv = nullptr;
     int32_t* d_status = nullptr;
-    int64_t brows = 0, urows = 0, zrows = 0;
-    size_t lds_sup = 0, lds_sroot = 0, lds_sdown = 0;  // per RHS tile
-    void *d_wb = nullptr, *d_wu = nullptr, *d_wz = nullptr;
+    int64_t res_len = 0;               // resident part of the factor (G, Qt, root LU)
+    int64_t yrows = 0, zrows = 0;      // solve workspace rows (times nrhs)
+    size_t lds_groot = 0;              // LDS of the root solve kernel
+    size_t lds_gform = 0, lds_qform = 0;
+    int max_kx = 0, max_nq = 0;        // widest Gr / Qt (columns formed per node)
+    void *d_wy = nullptr, *d_wz = nullptr;
     size_t ws_cap = 0;  // nrhs capacity
     SlotCache sinst;
     // single-request solve schedule (instance 0)
     WorkItem* d_swork = nullptr;
-    std::vector<Range> s_up, s_down;
-    Range s_root;
+    Schedule sched;
     // refinement scratch (Float64): residual and correction
     double *d_r = nullptr, *d_d = nullptr, *d_nrm = nullptr;
     size_t rf_cap = 0;
@@ -230,14 +254,17 @@ extern "C" hss_status hss_ctx_create(int device, hss_ctx_t* out) {
     return HSS_OK;
 }
 
-extern "C" hss_status hss_ctx_destroy(hss_ctx_t c) {
-    if (!c) return HSS_OK;
+static void ctx_release(hss_ctx_s* c) {
+    if (!c || --c->refs > 0) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     (void)hipEventDestroy(c->ev0);
     (void)hipEventDestroy(c->ev1);
     (void)hipStreamDestroy(c->stream);
     delete c;
+}
+extern "C" hss_status hss_ctx_destroy(hss_ctx_t c) {
+    ctx_release(c);
     return HSS_OK;
 }
 
@@ -398,15 +425,21 @@ extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, 
     HIP_TRY(hipMemcpyAsync(M->d_work, M->hwork.data(), sizeof(WorkItem) * M->hwork.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
     M->dev_bytes = (int64_t)(sizeof(double) * blob_len + sizeof(DevNode) * nn + sizeof(WorkItem) * M->hwork.size());
+    ctx->refs++;
     *out = M;
     return HSS_OK;
 }
 
-extern "C" hss_status hss_mat_destroy(hss_mat_t M) {
-    if (!M) return HSS_OK;
-    (void)hipSetDevice(M->ctx->device);
-    (void)hipStreamSynchronize(M->ctx->stream);
+static void mat_release(hss_mat_s* M) {
+    if (!M || --M->refs > 0) return;
+    hss_ctx_s* c = M->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     free_mat(M);
+    ctx_release(c);
+}
+extern "C" hss_status hss_mat_destroy(hss_mat_t M) {
+    mat_release(M);
     return HSS_OK;
 }
 
@@ -544,24 +577,27 @@ static size_t factor_lds(const FNode& f, const DevNode& d) {
 }
 
 // Fill the factor metadata of one matrix; offsets are relative to this
-// matrix's own factor / workspace buffers.
+// matrix's own factor / workspace buffers.  The factor buffer holds the resident
+// solve operators first ([0, res_len): every G and Qt, the root's LU data and
+// couplings) and the factorization-time data after them.
 template <typename TF>
 static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
     const int nn = M->nn;
     F->hfn.assign(nn, FNode{});
-    int64_t fac = 0, rw = 0, piv = 0, brow = 0, urow = 0, zrow = 0;
     auto take = [](int64_t& cur, int64_t count) {
         int64_t off = cur;
         cur += std::max<int64_t>(count, 0);
         return off;
     };
-    // post-order = reverse breadth-first is enough: children have larger indices
+    // sizes, in post-order (reverse breadth-first: children have larger indices)
     for (int i = nn - 1; i >= 0; --i) {
         const DevNode& d = M->hn[i];
         FNode& f = F->hfn[i];
         std::memset(&f, 0, sizeof(f));
+        f.Gz = f.Gr = f.Qt = -1;
         f.Yql = f.Tql = f.Ylq = f.taulq = f.Tlq = f.L1 = f.L1inv = f.L2a = f.Vt = f.P12 = f.P21 = f.LU = f.LUinv = f.piv = -1;
         f.rwD = f.rwU = f.rwV = f.rwM = -1;
+        f.yrow = f.zrow = -1;
         if (d.is_leaf) {
             f.m = d.m;
             f.n = d.n;
@@ -570,19 +606,11 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
             const FNode& b = F->hfn[d.right];
             f.m = a.mr + b.mr;
             f.n = a.nr + b.nr;
-            const DevNode& dl = M->hn[d.left];
-            const DevNode& dr = M->hn[d.right];
-            f.P12 = take(fac, (int64_t)a.mr * dr.kv);
-            f.P21 = take(fac, (int64_t)b.mr * dl.kv);
         }
         if (f.m != f.n)
             return fail(HSS_ERR_DIM, "ULV needs square diagonal blocks (node " + std::to_string(i) + ": " +
                                          std::to_string(f.m) + "x" + std::to_string(f.n) + ")");
         if (i == 0) {
-            f.LU = take(fac, (int64_t)f.m * f.m);
-            f.LUinv = take(fac, (int64_t)2 * ((f.m + NB - 1) / NB) * NB * NB);  // inverted L/U diagonal blocks
-            f.piv = take(piv, f.m);
-            f.zrow = take(zrow, f.m);
             f.mr = f.m;
             f.nr = f.n;
             continue;
@@ -593,6 +621,14 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
         f.mr = f.pass ? f.m : f.k;
         f.nr = f.n - f.nk;
         f.nblk = (f.nk + NB - 1) / NB;
+        f.My = f.nk + f.mr + d.kv;
+        if (d.is_leaf) {
+            f.Kx = f.m;
+        } else {
+            const FNode& a = F->hfn[d.left];
+            const FNode& b = F->hfn[d.right];
+            f.Kx = (a.My - a.nk) + (b.My - b.nk);
+        }
         int S = pow2_at_least((f.n + FACT_CPT - 1) / FACT_CPT);
         if (S < 1) S = 1;
         f.S = S;
@@ -602,6 +638,34 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
                                                  ") exceeds the register-resident factor kernel");
         if (factor_lds<TF>(f, d) > 160 * 1024)
             return fail(HSS_ERR_UNSUPPORTED, "node " + std::to_string(i) + " exceeds the LDS of the factor kernel");
+    }
+    // resident solve operators
+    int64_t fac = 0;
+    for (int i = 1; i < nn; ++i) {
+        FNode& f = F->hfn[i];
+        if (f.nk > 0) f.Gz = take(fac, (int64_t)f.nk * f.Kx);
+        f.Gr = take(fac, (int64_t)(f.My - f.nk) * (f.Kx + f.nk));
+        if (!f.pass) f.Qt = take(fac, (int64_t)f.n * f.n);
+    }
+    FNode& fr = F->hfn[0];
+    const DevNode& dr = M->hn[0];
+    fr.LU = take(fac, (int64_t)fr.m * fr.m);
+    fr.LUinv = take(fac, (int64_t)2 * ((fr.m + NB - 1) / NB) * NB * NB);  // inverted L/U diagonal blocks
+    if (!dr.is_leaf) {
+        fr.P12 = take(fac, (int64_t)F->hfn[dr.left].mr * M->hn[dr.right].kv);
+        fr.P21 = take(fac, (int64_t)F->hfn[dr.right].mr * M->hn[dr.left].kv);
+    }
+    fr.piv = 0;
+    F->res_len = fac;
+    // factorization-time data
+    int64_t rw = 0, yrows = 0, zrows = 0;
+    for (int i = nn - 1; i >= 1; --i) {
+        const DevNode& d = M->hn[i];
+        FNode& f = F->hfn[i];
+        if (!d.is_leaf) {
+            f.P12 = take(fac, (int64_t)F->hfn[d.left].mr * M->hn[d.right].kv);
+            f.P21 = take(fac, (int64_t)F->hfn[d.right].mr * M->hn[d.left].kv);
+        }
         if (!f.pass) {
             f.Yql = take(fac, (int64_t)f.m * f.k);
             f.Tql = take(fac, (int64_t)f.k * f.k);
@@ -617,34 +681,41 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
         f.rwD = take(rw, (int64_t)f.mr * f.nr);
         f.rwU = take(rw, (int64_t)f.mr * d.ku);
         f.rwV = take(rw, (int64_t)f.nr * d.kv);
-        f.brow = take(brow, f.mr);
-        f.urow = take(urow, d.kv);
-        f.zrow = take(zrow, f.n);
     }
     F->fac_len = fac;
-    F->brows = brow;
-    F->urows = urow;
-    F->zrows = zrow;
-    // LDS requirements of the solve kernels (rows of NTP = RHS_TILE columns)
-    size_t sup = 0, sroot = 0, sdown = 0;
+    // solve workspace: y blocks of non-root nodes, z blocks of branches and the root
+    for (int i = 0; i < nn; ++i) {
+        FNode& f = F->hfn[i];
+        if (i > 0) f.yrow = take(yrows, f.My);
+        if (!M->hn[i].is_leaf || i == 0) f.zrow = take(zrows, f.n);
+    }
+    F->yrows = yrows;
+    F->zrows = zrows;
+    // LDS (rows of NTP = RHS_TILE columns) and widths of the operator-forming kernels
+    size_t gform = 0, qform = 0, groot = 0;
+    int max_kx = 0, max_nq = 0;
     for (int i = 0; i < nn; ++i) {
         const FNode& f = F->hfn[i];
         const DevNode& d = M->hn[i];
         const int kvc = d.is_leaf ? 0 : M->hn[d.left].kv + M->hn[d.right].kv;
         if (i == 0) {
-            sroot = (size_t)f.m + kvc + NB;
+            groot = (size_t)f.m + kvc + NB;
         } else {
             const size_t tr = (size_t)std::max(std::max(f.k, NB), d.kv);
-            sup = std::max(sup, (size_t)f.m + std::max(f.nk, 1) + kvc + 2 * tr);
-            sdown = std::max(sdown, (size_t)f.n + 2 * NB);
+            gform = std::max(gform, (size_t)f.m + std::max(f.nk, 1) + kvc + 2 * tr);
+            qform = std::max(qform, (size_t)f.n + 2 * NB);
+            max_kx = std::max(max_kx, f.Kx + f.nk);
+            if (f.Qt >= 0) max_nq = std::max(max_nq, f.n);
         }
     }
     const size_t tv = sizeof(TF) * RHS_TILE;
-    F->lds_sup = sup * tv;
-    F->lds_sroot = sroot * tv;
-    F->lds_sdown = sdown * tv;
-    (void)rw;
-    (void)piv;
+    F->lds_gform = gform * tv;
+    F->lds_qform = qform * tv;
+    F->lds_groot = groot * tv;
+    F->max_kx = max_kx;
+    F->max_nq = max_nq;
+    if (F->lds_gform > 160 * 1024 || F->lds_groot > 160 * 1024)
+        return fail(HSS_ERR_UNSUPPORTED, "a node exceeds the LDS of the solve-operator kernels");
     return HSS_OK;
 }
 
@@ -655,8 +726,7 @@ static void free_fact(hss_fact_s* F) {
     (void)hipFree(F->d_fac);
     (void)hipFree(F->d_piv);
     (void)hipFree(F->d_status);
-    (void)hipFree(F->d_wb);
-    (void)hipFree(F->d_wu);
+    (void)hipFree(F->d_wy);
     (void)hipFree(F->d_wz);
     (void)hipFree(F->d_swork);
     (void)hipFree(F->d_r);
@@ -720,6 +790,14 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
     TF* d_rw = nullptr;
     FInst<TF>* d_inst = nullptr;
     WorkItem* d_work = nullptr;
+    size_t lds_gform = 0, lds_qform = 0;
+    int max_kx = 0, max_nq = 0;
+    for (auto* F : out) {
+        lds_gform = std::max(lds_gform, F->lds_gform);
+        lds_qform = std::max(lds_qform, F->lds_qform);
+        max_kx = std::max(max_kx, F->max_kx);
+        max_nq = std::max(max_nq, F->max_nq);
+    }
     // work lists across the batch
     std::vector<WorkItem> work;
     std::vector<Range> lev(maxh);
@@ -774,6 +852,15 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
             ProfScope ps("k_ulv_root", 0, roots.count, st);
             HIP_TRY((launch_ulv_root<TF>(d_inst, d_work + roots.off, roots.count, st)));
         }
+        // solve operators G and Qt of every non-root node (one launch each)
+        {
+            ProfScope ps("k_ulv_gform", 0, wy_count, st);
+            HIP_TRY((launch_ulv_gform<TF>(d_inst, d_work, wy_count, max_kx, lds_gform, st)));
+        }
+        {
+            ProfScope ps("k_ulv_qform", 0, wy_count, st);
+            HIP_TRY((launch_ulv_qform<TF>(d_inst, d_work, wy_count, max_nq, lds_qform, st)));
+        }
         HIP_TRY(hipEventRecord(ctx->ev1, st));
         HIP_TRY(hipStreamSynchronize(st));
         float ms = 0.f;
@@ -794,6 +881,22 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
                                               "): exactly zero pivot in the root block of matrix " + std::to_string(b));
         }
     }
+    // keep only the resident solve operators (the stream is idle here)
+    for (int b = 0; b < cnt; ++b) {
+        hss_fact_s* F = out[b];
+        TF* res = nullptr;
+        hipError_t e2 = dalloc(&res, (size_t)std::max<int64_t>(F->res_len, 1));
+        if (e2 != hipSuccess) {
+            cleanup();
+            return fail(HSS_ERR_OOM, std::string("factor allocation: ") + hipGetErrorString(e2));
+        }
+        HIP_TRY(hipMemcpy(res, F->d_fac, sizeof(TF) * (size_t)F->res_len, hipMemcpyDeviceToDevice));
+        (void)hipFree(F->d_fac);
+        F->d_fac = res;
+        F->fac_len = F->res_len;
+        F->dev_bytes = (int64_t)(sizeof(TF) * F->fac_len + sizeof(FNode) * F->hfn.size() + 4 * (F->hfn[0].m + 1));
+    }
+    for (auto* F : out) F->mat->refs++;
     return HSS_OK;
 }
 
@@ -816,11 +919,16 @@ extern "C" hss_status hss_ulv_factor(hss_mat_t M, hss_prec prec, hss_fact_t* out
     return hss_ulv_factor_batch(1, &M, prec, out);
 }
 
-extern "C" hss_status hss_fact_destroy(hss_fact_t F) {
-    if (!F) return HSS_OK;
-    (void)hipSetDevice(F->mat->ctx->device);
-    (void)hipStreamSynchronize(F->mat->ctx->stream);
+static void fact_release(hss_fact_s* F) {
+    if (!F || --F->refs > 0) return;
+    hss_mat_s* M = F->mat;
+    (void)hipSetDevice(M->ctx->device);
+    (void)hipStreamSynchronize(M->ctx->stream);
     free_fact(F);
+    mat_release(M);
+}
+extern "C" hss_status hss_fact_destroy(hss_fact_t F) {
+    fact_release(F);
     return HSS_OK;
 }
 
@@ -829,17 +937,15 @@ extern "C" int64_t hss_fact_device_bytes(hss_fact_t F) { return F ? F->dev_bytes
 // ---------------------------------------------------------------------------
 // ULV solve
 // ---------------------------------------------------------------------------
-template <typename TV>
+template <typename TF>
 static hss_status ensure_solve_ws(hss_fact_s* F, int nrhs) {
     if ((size_t)nrhs <= F->ws_cap) return HSS_OK;
-    (void)hipFree(F->d_wb);
-    (void)hipFree(F->d_wu);
+    (void)hipFree(F->d_wy);
     (void)hipFree(F->d_wz);
-    F->d_wb = F->d_wu = F->d_wz = nullptr;
+    F->d_wy = F->d_wz = nullptr;
     F->ws_cap = 0;
-    HIP_TRY(dalloc(reinterpret_cast<TV**>(&F->d_wb), (size_t)std::max<int64_t>(F->brows, 1) * nrhs));
-    HIP_TRY(dalloc(reinterpret_cast<TV**>(&F->d_wu), (size_t)std::max<int64_t>(F->urows, 1) * nrhs));
-    HIP_TRY(dalloc(reinterpret_cast<TV**>(&F->d_wz), (size_t)std::max<int64_t>(F->zrows, 1) * nrhs));
+    HIP_TRY(dalloc(reinterpret_cast<TF**>(&F->d_wy), (size_t)std::max<int64_t>(F->yrows, 1) * nrhs));
+    HIP_TRY(dalloc(reinterpret_cast<TF**>(&F->d_wz), (size_t)std::max<int64_t>(F->zrows, 1) * nrhs));
     F->ws_cap = nrhs;
     return HSS_OK;
 }
@@ -853,52 +959,57 @@ struct SolveReq {
     int64_t ldx;
 };
 
-template <typename TF, typename TV>
-static SInst<TF, TV> make_sinst(const SolveReq& q, TV* wb = nullptr, TV* wu = nullptr, TV* wz = nullptr) {
+// Workspace elements of one solve of `nrhs` columns (y blocks, then z blocks).
+static size_t solve_ws_elems(const hss_fact_s* F, int nrhs) { return (size_t)(F->yrows + F->zrows) * nrhs; }
+
+template <typename TF>
+static SInst<TF> make_sinst(const SolveReq& q, TF* ws = nullptr) {
     hss_fact_s* F = q.F;
     hss_mat_s* M = F->mat;
-    SInst<TF, TV> I;
+    SInst<TF> I;
     std::memset(&I, 0, sizeof(I));
     I.nodes = M->d_nodes;
     I.fn = F->d_fn;
     I.fac = reinterpret_cast<const TF*>(F->d_fac);
     I.ipiv = F->d_piv;
-    I.blob = M->d_blob;
     I.B = q.dB;
     I.X = q.dX;
     I.ldb = q.ldb;
     I.ldx = q.ldx;
-    I.wb = wb ? wb : reinterpret_cast<TV*>(F->d_wb);
-    I.wu = wu ? wu : reinterpret_cast<TV*>(F->d_wu);
-    I.wz = wz ? wz : reinterpret_cast<TV*>(F->d_wz);
+    if (ws) {
+        I.wy = ws;
+        I.wz = ws + F->yrows * q.nrhs;
+    } else {
+        I.wy = reinterpret_cast<TF*>(F->d_wy);
+        I.wz = reinterpret_cast<TF*>(F->d_wz);
+    }
     I.nrhs = q.nrhs;
     return I;
 }
 
-// Level schedule of a set of requests: up-sweep by height, roots, down-sweep by depth.
-struct Schedule {
-    std::vector<WorkItem> work;
-    std::vector<Range> up, down;
-    Range roots;
-};
-
-static void build_schedule(const std::vector<hss_mat_s*>& mats, Schedule& S) {
-    const int cnt = (int)mats.size();
+static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
+    const int cnt = (int)facts.size();
     int maxh = 0, maxd = 0;
-    for (auto* M : mats) {
-        maxh = std::max(maxh, M->max_height);
-        maxd = std::max(maxd, M->max_depth);
+    for (auto* F : facts) {
+        maxh = std::max(maxh, F->mat->max_height);
+        maxd = std::max(maxd, F->mat->max_depth);
     }
     S.work.clear();
     S.up.assign(maxh, Range{});
+    S.up_m.assign(maxh, 0);
     S.down.assign(maxd + 1, Range{});
+    S.down_m.assign(maxd + 1, 0);
     for (int h = 0; h < maxh; ++h) {
         S.up[h].off = (int)S.work.size();
         for (int b = 0; b < cnt; ++b) {
-            hss_mat_s* M = mats[b];
+            hss_mat_s* M = facts[b]->mat;
             if (h < M->max_height) {
                 const Range& r = M->by_height[h];
-                for (int t = 0; t < r.count; ++t) S.work.push_back(WorkItem{b, M->hwork[r.off + t].node});
+                for (int t = 0; t < r.count; ++t) {
+                    const int node = M->hwork[r.off + t].node;
+                    S.work.push_back(WorkItem{b, node});
+                    S.up_m[h] = std::max(S.up_m[h], facts[b]->hfn[node].nk);
+                }
             }
         }
         S.up[h].count = (int)S.work.size() - S.up[h].off;
@@ -908,59 +1019,58 @@ static void build_schedule(const std::vector<hss_mat_s*>& mats, Schedule& S) {
     for (int d = 1; d <= maxd; ++d) {
         S.down[d].off = (int)S.work.size();
         for (int b = 0; b < cnt; ++b) {
-            hss_mat_s* M = mats[b];
+            hss_mat_s* M = facts[b]->mat;
             if (d <= M->max_depth) {
                 const Range& r = M->by_depth[d];
-                for (int t = 0; t < r.count; ++t) S.work.push_back(WorkItem{b, M->hwork[r.off + t].node});
+                for (int t = 0; t < r.count; ++t) {
+                    const int node = M->hwork[r.off + t].node;
+                    S.work.push_back(WorkItem{b, node});
+                    S.down_m[d] = std::max(S.down_m[d], facts[b]->hfn[node].n);
+                }
             }
         }
         S.down[d].count = (int)S.work.size() - S.down[d].off;
     }
 }
 
-template <typename TF, typename TV>
-static hss_status launch_schedule(const SInst<TF, TV>* di, const WorkItem* dw, const std::vector<Range>& up,
-                                  const Range& roots, const std::vector<Range>& down, int maxnrhs, size_t lsup,
-                                  size_t lsroot, size_t lsdown, hipStream_t st) {
-    for (size_t h = 0; h < up.size(); ++h) {
-        ProfScope ps("k_ulv_sup", (int)h, up[h].count, st);
-        HIP_TRY((launch_ulv_sup<TF, TV>(di, dw + up[h].off, up[h].count, maxnrhs, lsup, st)));
+template <typename TF>
+static hss_status launch_schedule(const SInst<TF>* di, const WorkItem* dw, const Schedule& S, int maxnrhs,
+                                  size_t lgroot, hipStream_t st) {
+    for (size_t h = 0; h < S.up.size(); ++h) {
+        ProfScope ps("k_ulv_gup", (int)h, S.up[h].count, st);
+        HIP_TRY((launch_ulv_gup<TF>(di, dw + S.up[h].off, S.up[h].count, S.up_m[h], maxnrhs, st)));
     }
     {
-        ProfScope ps("k_ulv_sroot", 0, roots.count, st);
-        HIP_TRY((launch_ulv_sroot<TF, TV>(di, dw + roots.off, roots.count, maxnrhs, lsroot, st)));
+        ProfScope ps("k_ulv_groot", 0, S.roots.count, st);
+        HIP_TRY((launch_ulv_groot<TF>(di, dw + S.roots.off, S.roots.count, maxnrhs, lgroot, st)));
     }
-    for (size_t d = 1; d < down.size(); ++d) {
-        ProfScope ps("k_ulv_sdown", (int)d, down[d].count, st);
-        HIP_TRY((launch_ulv_sdown<TF, TV>(di, dw + down[d].off, down[d].count, maxnrhs, lsdown, st)));
+    for (size_t d = 1; d < S.down.size(); ++d) {
+        ProfScope ps("k_ulv_gdown", (int)d, S.down[d].count, st);
+        HIP_TRY((launch_ulv_gdown<TF>(di, dw + S.down[d].off, S.down[d].count, S.down_m[d], maxnrhs, st)));
     }
     return HSS_OK;
 }
 
 static hss_status ensure_schedule(hss_fact_s* F) {
     if (F->d_swork) return HSS_OK;
-    Schedule S;
-    build_schedule(std::vector<hss_mat_s*>{F->mat}, S);
-    HIP_TRY(dalloc(&F->d_swork, S.work.size()));
-    HIP_TRY(hipMemcpy(F->d_swork, S.work.data(), sizeof(WorkItem) * S.work.size(), hipMemcpyHostToDevice));
-    F->s_up = S.up;
-    F->s_down = S.down;
-    F->s_root = S.roots;
+    build_schedule(std::vector<hss_fact_s*>{F}, F->sched);
+    HIP_TRY(dalloc(&F->d_swork, F->sched.work.size()));
+    HIP_TRY(hipMemcpy(F->d_swork, F->sched.work.data(), sizeof(WorkItem) * F->sched.work.size(),
+                      hipMemcpyHostToDevice));
     return HSS_OK;
 }
 
-template <typename TF, typename TV>
+template <typename TF>
 static hss_status solve_one(hss_fact_s* F, int nrhs, const double* dB, int64_t ldb, double* dX, int64_t ldx,
                             hipStream_t st) {
-    hss_status s = ensure_solve_ws<TV>(F, nrhs);
+    hss_status s = ensure_solve_ws<TF>(F, nrhs);
     if (s != HSS_OK) return s;
     s = ensure_schedule(F);
     if (s != HSS_OK) return s;
-    SInst<TF, TV> I = make_sinst<TF, TV>(SolveReq{F, nrhs, dB, ldb, dX, ldx});
+    SInst<TF> I = make_sinst<TF>(SolveReq{F, nrhs, dB, ldb, dX, ldx});
     const void* dv = nullptr;
     HIP_TRY(F->sinst.get(&I, sizeof(I), st, &dv));
-    return launch_schedule<TF, TV>(reinterpret_cast<const SInst<TF, TV>*>(dv), F->d_swork, F->s_up, F->s_root,
-                                   F->s_down, nrhs, F->lds_sup, F->lds_sroot, F->lds_sdown, st);
+    return launch_schedule<TF>(reinterpret_cast<const SInst<TF>*>(dv), F->d_swork, F->sched, nrhs, F->lds_groot, st);
 }
 
 static hss_status ensure_refine(hss_fact_s* F, int nrhs) {
@@ -1010,8 +1120,7 @@ static hss_status solve_dev_impl(hss_fact_s* F, int nrhs, const double* dB, int6
     if (opts) o = *opts;
     else { o.tol = fp32 ? 1e-10 : 0.0; o.max_iter = fp32 ? 10 : 0; o.mode = fp32 ? 1 : 0; }
     auto solve = [&](const double* b, int64_t lb, double* x, int64_t lx) -> hss_status {
-        return fp32 ? solve_one<float, float>(F, nrhs, b, lb, x, lx, st)
-                    : solve_one<double, double>(F, nrhs, b, lb, x, lx, st);
+        return fp32 ? solve_one<float>(F, nrhs, b, lb, x, lx, st) : solve_one<double>(F, nrhs, b, lb, x, lx, st);
     };
     if (o.mode != 1 && !g_prof && !fp32) {
         // repeated unrefined solves on the same buffers replay a captured graph
@@ -1027,7 +1136,7 @@ static hss_status solve_dev_impl(hss_fact_s* F, int nrhs, const double* dB, int6
             }
             hss_fact_t f1 = F;
             int32_t nr = nrhs;
-            hss_status cs = hss_plan_create(1, &f1, &nr, &dB, &ldb, &dX, &ldx, 0, &P);
+            hss_status cs = plan_create_impl(1, &f1, &nr, &dB, &ldb, &dX, &ldx, 0, false, &P);
             if (cs != HSS_OK) return cs;
             F->plan_cache.push_back({key, P});
         }
@@ -1130,6 +1239,7 @@ extern "C" hss_status hss_ulvfactsolve(hss_mat_t M, int32_t nrhs, const double* 
 // ---------------------------------------------------------------------------
 struct hss_plan_s {
     hss_ctx_s* ctx = nullptr;
+    bool holds_refs = false;  // false for the plans a factor caches for itself
     hss_prec prec = HSS_PREC_FP64;
     int cnt = 0, maxnrhs = 0;
     bool has_ir = false;
@@ -1147,7 +1257,7 @@ struct hss_plan_s {
     Schedule S;
     std::vector<Range> mv_up, mv_down;
     Range mv_leaf;
-    size_t lsup = 0, lsroot = 0, lsdown = 0, lmv_up = 0, lmv_down = 0, lmv_leaf = 0;
+    size_t lgroot = 0, lmv_up = 0, lmv_down = 0, lmv_leaf = 0;
     double* d_norms = nullptr;
     std::vector<double> h_norms;
     hipGraph_t graph = nullptr;
@@ -1183,7 +1293,7 @@ static hss_status plan_tables(hss_plan_s* P) {
         hss_fact_s* F = q.F;
         hss_mat_s* M = F->mat;
         off_ws[i] = tot;
-        tot = align256(tot + sizeof(TV) * (size_t)(F->brows + F->urows + F->zrows) * q.nrhs);
+        tot = align256(tot + sizeof(TV) * solve_ws_elems(F, q.nrhs));
         if (P->has_ir) {
             off_mv[i] = tot;
             tot = align256(tot + sizeof(double) * (size_t)(M->zrows + M->frows) * q.nrhs);
@@ -1195,15 +1305,13 @@ static hss_status plan_tables(hss_plan_s* P) {
     }
     HIP_TRY(hipMalloc(reinterpret_cast<void**>(&P->arena), std::max<size_t>(tot, 256)));
     P->dev_bytes = (int64_t)tot;
-    P->sinst_size = sizeof(SInst<TF, TV>);
+    P->sinst_size = sizeof(SInst<TF>);
     for (int t = 0; t < (P->has_ir ? 2 : 1); ++t) {
-        std::vector<SInst<TF, TV>> v(cnt);
+        std::vector<SInst<TF>> v(cnt);
         for (int i = 0; i < cnt; ++i) {
             const SolveReq& q = P->reqs[i];
             hss_fact_s* F = q.F;
-            TV* wb = reinterpret_cast<TV*>(P->arena + off_ws[i]);
-            TV* wu = wb + F->brows * q.nrhs;
-            TV* wz = wu + F->urows * q.nrhs;
+            TV* ws = reinterpret_cast<TV*>(P->arena + off_ws[i]);
             SolveReq qq = q;
             if (t == 1) {
                 qq.dB = reinterpret_cast<double*>(P->arena + off_r[i]);
@@ -1211,7 +1319,7 @@ static hss_status plan_tables(hss_plan_s* P) {
                 qq.dX = reinterpret_cast<double*>(P->arena + off_d[i]);
                 qq.ldx = std::max<int64_t>(F->mat->rows, 1);
             }
-            v[i] = make_sinst<TF, TV>(qq, wb, wu, wz);
+            v[i] = make_sinst<TF>(qq, ws);
         }
         P->h_sinst[t].assign(reinterpret_cast<char*>(v.data()), reinterpret_cast<char*>(v.data() + cnt));
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&P->d_sinst[t]), P->h_sinst[t].size()));
@@ -1258,9 +1366,9 @@ static hss_status plan_tables(hss_plan_s* P) {
     return HSS_OK;
 }
 
-extern "C" hss_status hss_plan_create(int32_t count, const hss_fact_t* facts, const int32_t* nrhs,
-                                      const double* const* dB, const int64_t* ldb, double* const* dX,
-                                      const int64_t* ldx, int32_t with_refinement, hss_plan_t* out) {
+static hss_status plan_create_impl(int32_t count, hss_fact_s* const* facts, const int32_t* nrhs,
+                                   const double* const* dB, const int64_t* ldb, double* const* dX, const int64_t* ldx,
+                                   int32_t with_refinement, bool hold, hss_plan_t* out) {
     if (count <= 0 || !facts || !nrhs || !dB || !ldb || !dX || !ldx || !out || !facts[0])
         return fail(HSS_ERR_ARG, "bad plan arguments");
     *out = nullptr;
@@ -1284,17 +1392,19 @@ extern "C" hss_status hss_plan_create(int32_t count, const hss_fact_t* facts, co
         }
         P->reqs.push_back(SolveReq{F, nrhs[i], dB[i], ldb[i], dX[i], ldx[i]});
         P->maxnrhs = std::max(P->maxnrhs, (int)nrhs[i]);
-        P->lsup = std::max(P->lsup, F->lds_sup);
-        P->lsroot = std::max(P->lsroot, F->lds_sroot);
-        P->lsdown = std::max(P->lsdown, F->lds_sdown);
+        P->lgroot = std::max(P->lgroot, F->lds_groot);
         P->lmv_up = std::max(P->lmv_up, F->mat->lds_up);
         P->lmv_down = std::max(P->lmv_down, F->mat->lds_down);
         P->lmv_leaf = std::max(P->lmv_leaf, F->mat->lds_leaf);
     }
     P->cnt = (int)P->reqs.size();
     std::vector<hss_mat_s*> mats;
-    for (auto& q : P->reqs) mats.push_back(q.F->mat);
-    build_schedule(mats, P->S);
+    std::vector<hss_fact_s*> pf;
+    for (auto& q : P->reqs) {
+        mats.push_back(q.F->mat);
+        pf.push_back(q.F);
+    }
+    build_schedule(pf, P->S);
     // matvec schedule appended after the solve schedule
     std::vector<WorkItem> work = P->S.work;
     if (P->has_ir) {
@@ -1346,15 +1456,25 @@ extern "C" hss_status hss_plan_create(int32_t count, const hss_fact_t* facts, co
         free_plan(P);
         return s;
     }
+    if (hold) {
+        for (auto& q : P->reqs) q.F->refs++;
+        ctx->refs++;
+        P->holds_refs = true;
+    }
     *out = P;
     return HSS_OK;
 }
 
+extern "C" hss_status hss_plan_create(int32_t count, const hss_fact_t* facts, const int32_t* nrhs,
+                                      const double* const* dB, const int64_t* ldb, double* const* dX,
+                                      const int64_t* ldx, int32_t with_refinement, hss_plan_t* out) {
+    return plan_create_impl(count, facts, nrhs, dB, ldb, dX, ldx, with_refinement, true, out);
+}
+
 template <typename TF>
 static hss_status plan_solve(hss_plan_s* P, int t) {
-    using TV = TF;
-    return launch_schedule<TF, TV>(reinterpret_cast<const SInst<TF, TV>*>(P->d_sinst[t]), P->d_work, P->S.up,
-                                   P->S.roots, P->S.down, P->maxnrhs, P->lsup, P->lsroot, P->lsdown, P->ctx->stream);
+    return launch_schedule<TF>(reinterpret_cast<const SInst<TF>*>(P->d_sinst[t]), P->d_work, P->S, P->maxnrhs,
+                               P->lgroot, P->ctx->stream);
 }
 
 static hss_status plan_matvec(hss_plan_s* P) {
@@ -1374,8 +1494,8 @@ static hss_status plan_matvec(hss_plan_s* P) {
 }
 
 // re-upload the tables whose nrhs changed (the stream is idle: callers synchronize first)
-using SInstF = SInst<float, float>;
-using SInstD = SInst<double, double>;
+using SInstF = SInst<float>;
+using SInstD = SInst<double>;
 
 static hss_status plan_set_active(hss_plan_s* P, const std::vector<char>& active) {
     const size_t nrhs_off = (P->prec == HSS_PREC_FP32) ? offsetof(SInstF, nrhs) : offsetof(SInstD, nrhs);
@@ -1482,9 +1602,16 @@ extern "C" hss_status hss_plan_execute(hss_plan_t P, const hss_refine_opts* opts
 
 extern "C" hss_status hss_plan_destroy(hss_plan_t P) {
     if (!P) return HSS_OK;
-    (void)hipSetDevice(P->ctx->device);
-    (void)hipStreamSynchronize(P->ctx->stream);
+    hss_ctx_s* c = P->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    std::vector<hss_fact_s*> held;
+    if (P->holds_refs)
+        for (auto& q : P->reqs) held.push_back(q.F);
+    const bool hold = P->holds_refs;
     free_plan(P);
+    for (auto* F : held) fact_release(F);
+    if (hold) ctx_release(c);
     return HSS_OK;
 }
 
@@ -1594,6 +1721,7 @@ extern "C" hss_status hss_server_create(hss_ctx_t ctx, int64_t hbm_budget_bytes,
     auto* S = new (std::nothrow) hss_server_s;
     if (!S) return fail(HSS_ERR_OOM, "host allocation");
     S->ctx = ctx;
+    ctx->refs++;
     S->prec = prec;
     if (opts) S->opts = *opts;
     S->budget = hbm_budget_bytes;
@@ -1683,7 +1811,7 @@ static hss_status server_round(hss_server_s* S) {
     // 2. tables: solve (first solve b->x, or correction r->d), axpy, matvec x->r, residual
     size_t ws = 0;
     int maxnrhs = 0;
-    size_t lsup = 0, lsroot = 0, lsdown = 0, lup = 0, ldown = 0, lleaf = 0;
+    size_t lgroot = 0, lup = 0, ldown = 0, lleaf = 0;
     std::vector<size_t> off_s(cnt), off_m(cnt);
     std::vector<hss_mat_s*> mats(cnt);
     for (int i = 0; i < cnt; ++i) {
@@ -1692,20 +1820,18 @@ static hss_status server_round(hss_server_s* S) {
         hss_mat_s* M = F->mat;
         mats[i] = M;
         off_s[i] = ws;
-        ws = align256(ws + sizeof(TV) * (size_t)(F->brows + F->urows + F->zrows) * q->nrhs);
+        ws = align256(ws + sizeof(TV) * solve_ws_elems(F, q->nrhs));
         off_m[i] = ws;
         ws = align256(ws + sizeof(double) * (size_t)(M->zrows + M->frows) * q->nrhs);
         maxnrhs = std::max(maxnrhs, q->nrhs);
-        lsup = std::max(lsup, F->lds_sup);
-        lsroot = std::max(lsroot, F->lds_sroot);
-        lsdown = std::max(lsdown, F->lds_sdown);
+        lgroot = std::max(lgroot, F->lds_groot);
         lup = std::max(lup, M->lds_up);
         ldown = std::max(ldown, M->lds_down);
         lleaf = std::max(lleaf, M->lds_leaf);
     }
     hss_status s = grow(&S->d_round, &S->round_cap, ws);
     if (s != HSS_OK) return s;
-    std::vector<SInst<TF, TV>> si(cnt);
+    std::vector<SInst<TF>> si(cnt);
     std::vector<MvInst> mv(cnt);
     std::vector<BatchVec> bv(cnt);
     for (int i = 0; i < cnt; ++i) {
@@ -1713,12 +1839,10 @@ static hss_status server_round(hss_server_s* S) {
         hss_fact_s* F = S->facts[q->mid];
         hss_mat_s* M = F->mat;
         const int64_t n = std::max<int64_t>(M->rows, 1);
-        TV* wb = reinterpret_cast<TV*>(S->d_round + off_s[i]);
-        TV* wu = wb + F->brows * q->nrhs;
-        TV* wz = wu + F->urows * q->nrhs;
+        TV* ws = reinterpret_cast<TV*>(S->d_round + off_s[i]);
         SolveReq sr = (q->steps == 0) ? SolveReq{F, q->nrhs, q->dB, q->ldb, q->dX, q->ldx}
                                       : SolveReq{F, q->nrhs, q->r, n, q->d, n};
-        si[i] = make_sinst<TF, TV>(sr, wb, wu, wz);
+        si[i] = make_sinst<TF>(sr, ws);
         MvInst& I = mv[i];
         std::memset(&I, 0, sizeof(I));
         I.nodes = M->d_nodes;
@@ -1744,7 +1868,9 @@ static hss_status server_round(hss_server_s* S) {
         v.nrhs = (q->steps == 0) ? 0 : q->nrhs;  // axpy only for corrections
     }
     Schedule sch;
-    build_schedule(mats, sch);
+    std::vector<hss_fact_s*> rf(cnt);
+    for (int i = 0; i < cnt; ++i) rf[i] = S->facts[S->inflight[i]->mid];
+    build_schedule(rf, sch);
     std::vector<WorkItem> work = sch.work;
     int maxh = 0, maxd = 0;
     for (auto* M : mats) {
@@ -1777,12 +1903,12 @@ static hss_status server_round(hss_server_s* S) {
     }
     mleaf.count = (int)work.size() - mleaf.off;
     // one table upload per round (the stream is idle here)
-    const size_t o_si = 0, o_mv = align256(sizeof(SInst<TF, TV>) * cnt), o_bv = o_mv + align256(sizeof(MvInst) * cnt),
+    const size_t o_si = 0, o_mv = align256(sizeof(SInst<TF>) * cnt), o_bv = o_mv + align256(sizeof(MvInst) * cnt),
                  o_w = o_bv + align256(sizeof(BatchVec) * cnt), tab = o_w + sizeof(WorkItem) * work.size();
     s = grow(&S->d_tab, &S->tab_cap, tab);
     if (s != HSS_OK) return s;
     std::vector<char> h(tab);
-    std::memcpy(h.data() + o_si, si.data(), sizeof(SInst<TF, TV>) * cnt);
+    std::memcpy(h.data() + o_si, si.data(), sizeof(SInst<TF>) * cnt);
     std::memcpy(h.data() + o_mv, mv.data(), sizeof(MvInst) * cnt);
     std::memcpy(h.data() + o_bv, bv.data(), sizeof(BatchVec) * cnt);
     std::memcpy(h.data() + o_w, work.data(), sizeof(WorkItem) * work.size());
@@ -1790,12 +1916,12 @@ static hss_status server_round(hss_server_s* S) {
     const size_t nb = sizeof(double) * 2 * (size_t)cnt * maxnrhs;
     s = grow(reinterpret_cast<char**>(&S->d_norms), &S->norms_cap, nb);
     if (s != HSS_OK) return s;
-    auto* dsi = reinterpret_cast<const SInst<TF, TV>*>(S->d_tab + o_si);
+    auto* dsi = reinterpret_cast<const SInst<TF>*>(S->d_tab + o_si);
     auto* dmv = reinterpret_cast<const MvInst*>(S->d_tab + o_mv);
     auto* dbv = reinterpret_cast<BatchVec*>(S->d_tab + o_bv);
     auto* dw = reinterpret_cast<const WorkItem*>(S->d_tab + o_w);
     // 3. one batched solve, the corrections, one batched matvec + residual
-    s = launch_schedule<TF, TV>(dsi, dw, sch.up, sch.roots, sch.down, maxnrhs, lsup, lsroot, lsdown, st);
+    s = launch_schedule<TF>(dsi, dw, sch, maxnrhs, lgroot, st);
     if (s != HSS_OK) return s;
     HIP_TRY(launch_axpy_batch(dbv, cnt, maxnrhs, st));
     for (int hh = 0; hh < maxh; ++hh) HIP_TRY(launch_mv_up(dmv, dw + mup[hh].off, mup[hh].count, maxnrhs, lup, st));
@@ -1886,7 +2012,9 @@ extern "C" hss_status hss_server_destroy(hss_server_t S) {
     (void)hipFree(S->d_round);
     (void)hipFree(S->d_tab);
     (void)hipFree(S->d_norms);
+    hss_ctx_s* c = S->ctx;
     delete S;
+    ctx_release(c);
     return HSS_OK;
 }
 
@@ -1931,6 +2059,7 @@ extern "C" hss_status hss_fact_node_info(hss_fact_t F, int32_t node, int32_t* in
     const FNode& f = F->hfn[node];
     info[0] = f.m; info[1] = f.n; info[2] = f.k; info[3] = f.nk;
     info[4] = f.mr; info[5] = f.nr; info[6] = f.pass; info[7] = f.nblk;
+    info[8] = f.My; info[9] = f.Kx;
     return HSS_OK;
 }
 extern "C" double hss_last_device_ms(void) { return g_last_ms; }

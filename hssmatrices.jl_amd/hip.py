@@ -351,9 +351,9 @@ class HipUlvFactor:
 
     def node_info(self, node: int) -> dict:
         """Factor sizes of one node (``hss_fact_node_info``)."""
-        buf = (ctypes.c_int32 * 8)()
+        buf = (ctypes.c_int32 * 10)()
         _check(_lib.hss_fact_node_info(self._h, node, buf))
-        keys = ("m", "n", "k", "nk", "mr", "nr", "pass", "nblk")
+        keys = ("m", "n", "k", "nk", "mr", "nr", "pass", "nblk", "My", "Kx")
         return dict(zip(keys, list(buf)))
 
     def close(self):

@@ -97,6 +97,10 @@ typedef struct hss_solve_report {
     double seconds;      /* device time of the call */
 } hss_solve_report;
 
+/* Handles may be destroyed in any order: a context, matrix or factor that is destroyed
+ * while a matrix, factor, plan or server still uses it is released when the last of those
+ * is destroyed. */
+
 /* ---- context: owns the HIP stream used by every call on its handles ---- */
 hss_status hss_ctx_create(int device, hss_ctx_t* out);
 hss_status hss_ctx_destroy(hss_ctx_t ctx);
@@ -190,7 +194,8 @@ void hss_prof_enable(int on);
 void hss_prof_reset(void);
 int32_t hss_prof_count(void);
 hss_status hss_prof_get(int32_t i, const char** name, int32_t* level, int32_t* nwork, double* ms);
-/* Factor sizes of one node: {m, n, k, nk, mr, nr, pass, nblk} (8 ints). */
+/* Factor sizes of one node: {m, n, k, nk, mr, nr, pass, nblk, My, Kx} (10 ints); My x Kx is the
+ * node's up-sweep operator, n x n its down-sweep operator. */
 hss_status hss_fact_node_info(hss_fact_t fact, int32_t node, int32_t* info);
 /* Per-call device timing of the last matvec/solve on this thread (milliseconds, -1 if none). */
 double hss_last_device_ms(void);

@@ -163,6 +163,31 @@ def test_batched_factor_and_solve_match_single(hip, oracle):
         assert rel(Xb, oracle.ulvfactsolve(nodes, blob, B)) < 1e-11
 
 
+def test_handles_destroyed_in_any_order(hip, oracle, c1):
+    """Destroying a context, matrix or factor before the handles that use it keeps it alive
+    until they are destroyed too (garbage-collected bindings finalize in any order)."""
+    import torch
+
+    ctx = hip.Context(0)
+    A = hip.HipHssMatrix((c1["nodes"], c1["blob"]), ctx)
+    F = A.ulvfactor("fp64")
+    rng = np.random.default_rng(21)
+    B = rng.standard_normal((2001, 3))
+    Bt = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda:0")
+    Xt = torch.zeros_like(Bt)
+    plan = hip.Plan([F], [3], [Bt.data_ptr()], [2001], [Xt.data_ptr()], [2001])
+    ctx.close()
+    A.close()
+    X1 = F.solve(B)  # the factor still holds its matrix and context
+    F.close()
+    plan.execute()  # the plan still holds the factor
+    torch.cuda.synchronize()
+    plan.close()
+    ref = oracle.ulvfactsolve(c1["nodes"], c1["blob"], B)
+    assert rel(X1, ref) < 1e-12
+    assert rel(Xt.cpu().numpy().T, ref) < 1e-12
+
+
 def test_errors_map_to_reference_exceptions(hip, c1_dev):
     from hssmatrices_jl_amd.hssmatrix import DimensionMismatch, SingularException
     with pytest.raises(DimensionMismatch):
