@@ -112,15 +112,17 @@ def solve_kernel_model(nodes, fact, kernel, level, nrhs, wf, wv):
     flops = 0
     heights = nodes["height"]
     depths = nodes["depth"]
+    up = kernel.startswith("k_ulv_gup")
+    down = kernel.startswith("k_ulv_gdown")
     for i in range(1, len(nodes)):
         leaf = nodes["left"][i] < 0
-        if kernel == "k_ulv_gup" and heights[i] == level:
+        if up and heights[i] == level:
             f = fact.node_info(i)
             My, Kx, nk = f["My"], f["Kx"], f["nk"]
             g = nk * Kx + (My - nk) * (Kx + nk)
             byts += wf * g + nrhs * (Kx * (8 if leaf else wv) + My * wv)
             flops += 2 * g * nrhs
-        elif kernel == "k_ulv_gdown" and depths[i] == level:
+        elif down and depths[i] == level:
             f = fact.node_info(i)
             n = f["n"]
             q = 0 if f["pass"] else n * n
@@ -290,9 +292,13 @@ PROFILE_COUNTERS = ROOT / "profiles" / "hss_c2_n65536_hbm_counters.md"
 
 def sweep_grid_threads(nodes, fact, kernel, level, nwork, nrhs):
     """Launch size (threads) of a solve sweep launch, as the library computes it, padded to a
-    multiple of 8 workgroups of 256 threads.  k_ulv_gup: one workgroup per node and 32
-    columns.  k_ulv_gdown: tiles of 64 x 64 outputs (128 x 32 for at most 32 right-hand
+    multiple of 8 workgroups.  k_ulv_gup_s / k_ulv_gdown_s: one 512-thread workgroup per node
+    and 32 columns; k_ulv_gup: one 256-thread workgroup per node and 32 columns;
+    k_ulv_gdown: 256-thread tiles of 64 x 64 outputs (128 x 32 for at most 32 right-hand
     sides)."""
+    if kernel.endswith("_s"):
+        total = nwork * (-(-nrhs // 32))
+        return 8 * (-(-total // 8)) * 512
     if kernel == "k_ulv_gup":
         total = nwork * (-(-nrhs // 32))
     else:
@@ -483,16 +489,15 @@ def main():
     log(f"[bench] fp64 solve per kernel (ms/solve): { {k: round(v, 4) for k, v in byk_s.items()} }")
     log(f"[bench] matvec per kernel (ms): { {k: round(v, 4) for k, v in byk_m.items()} }")
     log(f"[bench] mixed solve per kernel (ms/solve): { {k: round(v, 4) for k, v in byk_x.items()} }")
-    # dominant kernel of the fp64 solve and its busiest launch
-    dom = max(byk_s, key=byk_s.get)
-    cand = [(key, v) for key, v in tot_s.items() if key[0] == dom]
+    # the busiest launch of the fp64 solve sweeps (largest time per solve)
+    cand = [(key, v) for key, v in tot_s.items() if key[0].startswith(("k_ulv_gup", "k_ulv_gdown"))]
     (dk, dlev), (dms, dcnt, dnw) = max(cand, key=lambda kv: kv[1][0])
     avg_ms = dms / dcnt
     dbytes, dflops = solve_kernel_model(nodes, F64, dk, dlev, nrhs, 8, 8)
     achieved = dbytes / avg_ms / 1e6
     log(f"[bench] dominant solve kernel {dk} level {dlev}: {avg_ms:.4f} ms/launch, "
         f"{dbytes/1e6:.2f} MB -> {achieved:.1f} GB/s, {dflops/avg_ms/1e9:.2f} TFLOP/s")
-    targs = "<double>" if dk == "k_ulv_gup" else ("<double, 2, 2>" if nrhs > 32 else "<double, 4, 1>")
+    targs = "<double>" if dk != "k_ulv_gdown" else ("<double, 2, 2>" if nrhs > 32 else "<double, 4, 1>")
     traffic = profiled_traffic(dk, targs, sweep_grid_threads(nodes, F64, dk, dlev, int(dnw), nrhs), 1)
     mvleaf = [(key, v) for key, v in tot_m.items() if key[0] == "k_mv_leaf"]
     mv_leaf_ms = mvleaf[0][1][0] / mvleaf[0][1][1]

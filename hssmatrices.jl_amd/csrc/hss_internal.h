@@ -26,6 +26,11 @@ struct DevNode {
     int64_t row_off, col_off;
     int64_t D, U, V, B12, B21, R, W;  // element offsets into the generator blob (-1 = none)
     int64_t zrow, frow;         // matvec workspace rows: Z (kv rows), F (ku rows)
+    // matvec operators (column-major, element offsets into the operator buffer, -1 = none):
+    //   opZ kv x (n | kv_1 + kv_2): V' at a leaf, [W_1' W_2'] at a branch
+    //   opF ku x (kv_sibling + ku_parent): [B_sibling R] (B12 for a left child, B21 for a right one)
+    //   opC m x (n + ku) at a leaf: [D U]
+    int64_t opZ, opF, opC;
 };
 
 // Factorization metadata of one node (host-computed before the factor runs).
@@ -55,6 +60,12 @@ struct FNode {
     int64_t Yql, Tql, Ylq, taulq, Tlq, L1, L1inv, L2a, Vt, P12, P21, LU, LUinv, piv;
     // factorization-time workspace (reduced D/U/V passed to the parent, merged D scratch)
     int64_t rwD, rwU, rwV, rwM;
+    // root: elements of the contiguous block LU | LUinv | P12 | P21
+    int64_t rlen;
+    // root (branch): the merged block D (m x m), its inverse Dinv (formed from the LU
+    // factors) and the merge Mb (m x Kx, b = Mb [b_1; u_1; b_2; u_2]); solves use
+    // z0 = Dinv b, z = z0 + Dinv (b - D z0)
+    int64_t Dr, Dinv, Mb;
     // solve workspace rows (times nrhs; column-major blocks of nrhs columns):
     //   yrow: the node's y = [zloc; reduced b; u] (My rows, non-root nodes)
     //   zrow: the node's z (n rows, branches and the root)
@@ -68,7 +79,7 @@ struct WorkItem {
 // Per-instance arguments of one matvec launch sequence.
 struct MvInst {
     const DevNode* nodes;
-    const double* blob;
+    const double* op;  // matvec operators (DevNode::opZ/opF/opC)
     const double* B;
     double* C;
     int64_t ldb, ldc;

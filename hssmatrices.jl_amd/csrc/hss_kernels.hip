@@ -126,12 +126,42 @@ __device__ __forceinline__ void gemm(int M, int N, int K, const TAe* __restrict_
 }
 
 // global column-major (ld) -> LDS row-major rows [r0, r0 + rows)
+// for e in [0, n): st(e, ld(e)), with U loads per thread issued before their stores
+// (a load and a store through generic pointers may alias, so a plain loop would wait
+// for every load in turn)
+template <int U, typename T, typename Ld, typename St>
+__device__ __forceinline__ void batched(int n, Ld ld, St st) {
+    const int bd = blockDim.x;
+    for (int e0 = threadIdx.x; e0 < n; e0 += U * bd) {
+        T r[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int e = e0 + q * bd;
+            r[q] = e < n ? ld(e) : T{};
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int e = e0 + q * bd;
+            if (e < n) st(e, r[q]);
+        }
+    }
+}
+
 template <typename Tsrc, typename T>
 __device__ __forceinline__ void stage(T* dst, int r0, const Tsrc* src, int64_t ld, int rows, int nt) {
-    for (int e = threadIdx.x; e < rows * nt; e += blockDim.x) {
-        const int r = e % rows, j = e / rows;
-        dst[(r0 + r) * NTP + j] = (T)src[r + (int64_t)j * ld];
-    }
+    batched<8, T>(
+        rows * nt, [&](int e) { return (T)src[(e % rows) + (int64_t)(e / rows) * ld]; },
+        [&](int e, T v) { dst[(r0 + e % rows) * NTP + e / rows] = v; });
+}
+
+// dst[0, n) = src[0, n): 16-byte vector loads, 16 in flight per thread; src and dst
+// 16-byte aligned; may read up to 15 bytes past n.
+template <typename TF>
+__device__ __forceinline__ void lds_copy(TF* dst, const TF* __restrict__ src, int64_t n) {
+    const float4* s = reinterpret_cast<const float4*>(src);
+    float4* d = reinterpret_cast<float4*>(dst);
+    const int nv = (int)((n * (int64_t)sizeof(TF) + 15) / 16);
+    batched<16, float4>(nv, [&](int v) { return s[v]; }, [&](int v, float4 x) { d[v] = x; });
 }
 
 // LDS row-major -> global column-major
@@ -146,117 +176,6 @@ __device__ __forceinline__ void unstage(Tdst* dst, int64_t ld, const T* src, int
 // ---------------------------------------------------------------------------
 // matvec
 // ---------------------------------------------------------------------------
-// Up-sweep of one level.  Leaf: Z = V' B_seg.  Branch: Z = W1' Z1 + W2' Z2.
-__global__ __launch_bounds__(SWEEP_THREADS) void k_mv_up(const MvInst* __restrict__ insts,
-                                                        const WorkItem* __restrict__ work) {
-    const WorkItem wi = work[blockIdx.x];
-    const MvInst& I = insts[wi.inst];
-    const int j0 = blockIdx.y * RHS_TILE;
-    if (j0 >= I.nrhs) return;
-    const int nt = min(RHS_TILE, I.nrhs - j0);
-    const DevNode nd = I.nodes[wi.node];
-    extern __shared__ double lds_mv[];
-    double* Zt = I.Z + nd.zrow * I.nrhs + (int64_t)j0 * nd.kv;  // kv x nt, ld kv
-    if (nd.is_leaf) {
-        double* xs = lds_mv;  // n x NTP
-        stage(xs, 0, I.B + nd.col_off + (int64_t)j0 * I.ldb, I.ldb, nd.n, nt);
-        __syncthreads();
-        const int kv = nd.kv;
-        gemm<true>(kv, nt, nd.n, I.blob + nd.V, nd.n, xs,
-                   [&](int i, int j, double v) { Zt[i + (int64_t)j * kv] = v; });
-    } else {
-        const DevNode L = I.nodes[nd.left];
-        const DevNode R = I.nodes[nd.right];
-        double* zs = lds_mv;                // (kv_L + kv_R) x NTP
-        double* acc = lds_mv + (L.kv + R.kv) * NTP;  // kv x NTP
-        stage(zs, 0, I.Z + L.zrow * I.nrhs + (int64_t)j0 * L.kv, L.kv, L.kv, nt);
-        stage(zs, L.kv, I.Z + R.zrow * I.nrhs + (int64_t)j0 * R.kv, R.kv, R.kv, nt);
-        __syncthreads();
-        gemm<true>(nd.kv, nt, L.kv, I.blob + L.W, L.kv, zs,
-                   [&](int i, int j, double v) { acc[i * NTP + j] = v; });
-        __syncthreads();
-        const int kv = nd.kv;
-        gemm<true>(nd.kv, nt, R.kv, I.blob + R.W, R.kv, zs + L.kv * NTP,
-                   [&](int i, int j, double v) { Zt[i + (int64_t)j * kv] = acc[i * NTP + j] + v; });
-    }
-}
-
-// Down-sweep of one level: for a branch node p, F of both children.
-__global__ __launch_bounds__(SWEEP_THREADS) void k_mv_down(const MvInst* __restrict__ insts,
-                                                          const WorkItem* __restrict__ work) {
-    const WorkItem wi = work[blockIdx.x];
-    const MvInst& I = insts[wi.inst];
-    const int j0 = blockIdx.y * RHS_TILE;
-    if (j0 >= I.nrhs) return;
-    const int nt = min(RHS_TILE, I.nrhs - j0);
-    const DevNode p = I.nodes[wi.node];
-    const DevNode L = I.nodes[p.left];
-    const DevNode R = I.nodes[p.right];
-    const bool has_f = p.parent >= 0 && p.ku > 0;
-    extern __shared__ double lds_mv[];
-    double* zl = lds_mv;                    // kv_L x NTP
-    double* zr = zl + L.kv * NTP;           // kv_R x NTP
-    double* fp = zr + R.kv * NTP;           // ku_p x NTP
-    double* al = fp + p.ku * NTP;           // ku_L x NTP
-    double* ar = al + L.ku * NTP;           // ku_R x NTP
-    stage(zl, 0, I.Z + L.zrow * I.nrhs + (int64_t)j0 * L.kv, L.kv, L.kv, nt);
-    stage(zr, 0, I.Z + R.zrow * I.nrhs + (int64_t)j0 * R.kv, R.kv, R.kv, nt);
-    if (has_f) stage(fp, 0, I.F + p.frow * I.nrhs + (int64_t)j0 * p.ku, p.ku, p.ku, nt);
-    __syncthreads();
-    double* FL = I.F + L.frow * I.nrhs + (int64_t)j0 * L.ku;
-    double* FR = I.F + R.frow * I.nrhs + (int64_t)j0 * R.ku;
-    const int kul = L.ku, kur = R.ku;
-    if (has_f) {
-        // a = R_c F (stored), then F_c = B Z_sibling + a
-        gemm<false>(L.ku, nt, p.ku, I.blob + L.R, L.ku, fp, [&](int i, int j, double v) { al[i * NTP + j] = v; });
-        gemm<false>(R.ku, nt, p.ku, I.blob + R.R, R.ku, fp, [&](int i, int j, double v) { ar[i * NTP + j] = v; });
-        __syncthreads();
-        gemm<false>(L.ku, nt, R.kv, I.blob + p.B12, L.ku, zr,
-                    [&](int i, int j, double v) { FL[i + (int64_t)j * kul] = v + al[i * NTP + j]; });
-        gemm<false>(R.ku, nt, L.kv, I.blob + p.B21, R.ku, zl,
-                    [&](int i, int j, double v) { FR[i + (int64_t)j * kur] = v + ar[i * NTP + j]; });
-    } else {
-        gemm<false>(L.ku, nt, R.kv, I.blob + p.B12, L.ku, zr,
-                    [&](int i, int j, double v) { FL[i + (int64_t)j * kul] = v; });
-        gemm<false>(R.ku, nt, L.kv, I.blob + p.B21, R.ku, zl,
-                    [&](int i, int j, double v) { FR[i + (int64_t)j * kur] = v; });
-    }
-}
-
-// Leaves: C_seg = alpha (D B_seg + U F) + beta C_seg.
-__global__ __launch_bounds__(SWEEP_THREADS) void k_mv_leaf(const MvInst* __restrict__ insts,
-                                                          const WorkItem* __restrict__ work) {
-    const WorkItem wi = work[blockIdx.x];
-    const MvInst& I = insts[wi.inst];
-    const int j0 = blockIdx.y * RHS_TILE;
-    if (j0 >= I.nrhs) return;
-    const int nt = min(RHS_TILE, I.nrhs - j0);
-    const DevNode nd = I.nodes[wi.node];
-    const bool has_f = nd.parent >= 0 && nd.ku > 0;
-    extern __shared__ double lds_mv[];
-    double* xs = lds_mv;              // n x NTP
-    double* ds = xs + nd.n * NTP;     // m x NTP : D B_seg
-    stage(xs, 0, I.B + nd.col_off + (int64_t)j0 * I.ldb, I.ldb, nd.n, nt);
-    double* fs = ds + nd.m * NTP;     // ku x NTP
-    if (has_f) stage(fs, 0, I.F + nd.frow * I.nrhs + (int64_t)j0 * nd.ku, nd.ku, nd.ku, nt);
-    __syncthreads();
-    double* Cs = I.C + nd.row_off + (int64_t)j0 * I.ldc;
-    const int64_t ldc = I.ldc;
-    const double alpha = I.alpha, beta = I.beta;
-    auto fin = [&](int i, int j, double v) {
-        double* c = Cs + i + j * ldc;
-        *c = (beta == 0.0) ? alpha * v : alpha * v + beta * (*c);
-    };
-    if (has_f) {
-        gemm<false>(nd.m, nt, nd.n, I.blob + nd.D, nd.m, xs, [&](int i, int j, double v) { ds[i * NTP + j] = v; });
-        __syncthreads();
-        gemm<false>(nd.m, nt, nd.ku, I.blob + nd.U, nd.m, fs,
-                    [&](int i, int j, double v) { fin(i, j, ds[i * NTP + j] + v); });
-    } else {
-        gemm<false>(nd.m, nt, nd.n, I.blob + nd.D, nd.m, xs, fin);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // ULV factorization
 // ---------------------------------------------------------------------------
@@ -797,6 +716,8 @@ __global__ __launch_bounds__(1024) void k_ulv_root(const FInst<T>* __restrict__ 
     }
     __threadfence_block();
     __syncthreads();
+    if (f.Dr >= 0)  // the merged block itself, for the refinement step of every root solve
+        for (int e = tid; e < m * m; e += blockDim.x) I.fac[f.Dr + e] = A[e];
     __shared__ int s_piv;
     __shared__ T s_val;
     const int wave = tid / 64, lane = tid % 64;
@@ -1036,61 +957,20 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_qform(const FInst<TF>* __
 // ---------------------------------------------------------------------------
 // ULV solve
 // ---------------------------------------------------------------------------
-// Root (ulvfactor.jl:74,83): b = [b_1 - P12 u_2; b_2 - P21 u_1] from the children's
-// y blocks, then the LU solve with partial pivoting (blocked triangular solves with
-// the inverted diagonal blocks of L and U).
-// LDS (row-major, NTP columns): bv m rows | us kv_1 + kv_2 | t1 NB
+// bv (m rows, NTP columns in LDS) <- A^-1 bv with the LU factors of A (partial
+// pivoting, unit-lower L and U in LU, m x m) and the inverted NB x NB diagonal blocks of
+// L then U in LUinv: row interchanges, then blocked forward and backward substitution.
+// t1: NB x NTP scratch.
 template <typename TF>
-__global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_groot(const SInst<TF>* __restrict__ insts,
-                                                            const WorkItem* __restrict__ work) {
-    const WorkItem wi = work[blockIdx.x];
-    const SInst<TF>& I = insts[wi.inst];
-    const int j0 = blockIdx.y * RHS_TILE;
-    if (j0 >= I.nrhs) return;
-    const int nt = min(RHS_TILE, I.nrhs - j0);
-    const int nrhs = I.nrhs;
-    const DevNode nd = I.nodes[wi.node];
-    const FNode f = I.fn[wi.node];
-    const int m = f.m;
-    int kvl = 0, kvr = 0;
-    if (!nd.is_leaf) {
-        kvl = I.nodes[nd.left].kv;
-        kvr = I.nodes[nd.right].kv;
-    }
-    extern __shared__ unsigned char lds_raw[];
-    TF* bv = reinterpret_cast<TF*>(lds_raw);
-    TF* us = bv + m * NTP;
-    TF* t1 = us + (kvl + kvr) * NTP;
-    if (nd.is_leaf) {
-        stage(bv, 0, I.B + nd.row_off + (int64_t)j0 * I.ldb, I.ldb, m, nt);
-        __syncthreads();
-    } else {
-        const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
-        const TF* yl = I.wy + fl.yrow * nrhs + (int64_t)j0 * fl.My;
-        const TF* yr = I.wy + fr.yrow * nrhs + (int64_t)j0 * fr.My;
-        stage(bv, 0, yl + fl.nk, fl.My, fl.mr, nt);
-        stage(bv, fl.mr, yr + fr.nk, fr.My, fr.mr, nt);
-        stage(us, 0, yl + fl.nk + fl.mr, fl.My, kvl, nt);
-        stage(us, kvl, yr + fr.nk + fr.mr, fr.My, kvr, nt);
-        __syncthreads();
-        gemm<false>(fl.mr, nt, kvr, I.fac + f.P12, fl.mr, us + kvl * NTP,
-                    [&](int i, int j, TF v) { bv[i * NTP + j] -= v; });
-        const int mr1 = fl.mr;
-        gemm<false>(fr.mr, nt, kvl, I.fac + f.P21, fr.mr, us,
-                    [&](int i, int j, TF v) { bv[(mr1 + i) * NTP + j] -= v; });
-        __syncthreads();
-    }
-    // row interchanges of the partial pivoting (one thread per column)
-    const int32_t* piv = I.ipiv + f.piv;
+__device__ __forceinline__ void lu_solve(TF* bv, TF* t1, int m, int nt, const TF* LU, const TF* LUinv, const int32_t* piv) {
     for (int j = threadIdx.x; j < nt; j += blockDim.x)
         for (int i = 0; i < m; ++i) {
             const int p = piv[i];
             if (p != i) { const TF t = bv[i * NTP + j]; bv[i * NTP + j] = bv[p * NTP + j]; bv[p * NTP + j] = t; }
         }
     __syncthreads();
-    const TF* LU = I.fac + f.LU;
     const int nblk = (m + NB - 1) / NB;
-    const TF* Li = I.fac + f.LUinv;
+    const TF* Li = LUinv;
     const TF* Ui = Li + (int64_t)nblk * NB * NB;
     // forward: y_b = L_bb^-1 (b_b - L[b, 0:b0] y[0:b0])
     for (int bb = 0; bb < nblk; ++bb) {
@@ -1109,8 +989,106 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_groot(const SInst<TF>* __
         gemm<false>(nb, nt, nb, Ui + (int64_t)bb * NB * NB, NB, t1, [&](int i, int j, TF v) { bv[(b0 + i) * NTP + j] = v; });
         __syncthreads();
     }
+}
+
+// Root (ulvfactor.jl:74,83): b = [b_1 - P12 u_2; b_2 - P21 u_1] from the children's
+// y blocks, then the LU solve with partial pivoting (blocked triangular solves with
+// the inverted diagonal blocks of L and U).
+// LDS (row-major, NTP columns): bv m rows | us kv_1 + kv_2 | t1 NB | with resident != 0,
+// the root factor block LU | LUinv | P12 | P21 (staged whole, so the solve's many
+// small dependent steps read LDS rather than memory)
+template <typename TF>
+__global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_groot(const SInst<TF>* __restrict__ insts,
+                                                            const WorkItem* __restrict__ work, int resident) {
+    const WorkItem wi = work[blockIdx.x];
+    const SInst<TF>& I = insts[wi.inst];
+    const int j0 = blockIdx.y * RHS_TILE;
+    if (j0 >= I.nrhs) return;
+    const int nt = min(RHS_TILE, I.nrhs - j0);
+    const int nrhs = I.nrhs;
+    const DevNode nd = I.nodes[wi.node];
+    const FNode f = I.fn[wi.node];
+    const int m = f.m;
+    int kvl = 0, kvr = 0;
+    if (!nd.is_leaf) {
+        kvl = I.nodes[nd.left].kv;
+        kvr = I.nodes[nd.right].kv;
+    }
+    extern __shared__ unsigned char lds_raw[];
+    TF* bv = reinterpret_cast<TF*>(lds_raw);
+    TF* us = bv + m * NTP;
+    TF* t1 = us + (kvl + kvr) * NTP;
+    // the root factor block, in LDS or in memory
+    const TF* fb = I.fac + f.LU;
+    if (resident) {
+        TF* Ls = t1 + NB * NTP;
+        Ls += (16 - ((Ls - reinterpret_cast<TF*>(lds_raw)) & 15)) & 15;  // 16-element aligned
+        lds_copy(Ls, fb, f.rlen);
+        fb = Ls;
+    }
+    const TF* P12 = fb + (f.P12 - f.LU);
+    const TF* P21 = fb + (f.P21 - f.LU);
+    if (nd.is_leaf) {
+        stage(bv, 0, I.B + nd.row_off + (int64_t)j0 * I.ldb, I.ldb, m, nt);
+        __syncthreads();
+    } else {
+        const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
+        const TF* yl = I.wy + fl.yrow * nrhs + (int64_t)j0 * fl.My;
+        const TF* yr = I.wy + fr.yrow * nrhs + (int64_t)j0 * fr.My;
+        stage(bv, 0, yl + fl.nk, fl.My, fl.mr, nt);
+        stage(bv, fl.mr, yr + fr.nk, fr.My, fr.mr, nt);
+        stage(us, 0, yl + fl.nk + fl.mr, fl.My, kvl, nt);
+        stage(us, kvl, yr + fr.nk + fr.mr, fr.My, kvr, nt);
+        __syncthreads();
+        gemm<false>(fl.mr, nt, kvr, P12, fl.mr, us + kvl * NTP, [&](int i, int j, TF v) { bv[i * NTP + j] -= v; });
+        const int mr1 = fl.mr;
+        gemm<false>(fr.mr, nt, kvl, P21, fr.mr, us, [&](int i, int j, TF v) { bv[(mr1 + i) * NTP + j] -= v; });
+        __syncthreads();
+    }
+    lu_solve(bv, t1, m, nt, fb, fb + (f.LUinv - f.LU), I.ipiv + f.piv);
     if (nd.is_leaf) unstage(I.X + nd.col_off + (int64_t)j0 * I.ldx, I.ldx, bv, 0, m, nt);
     else unstage(I.wz + f.zrow * nrhs + (int64_t)j0 * f.n, f.n, bv, 0, m, nt);
+}
+
+
+// Root operators for the inverse-and-refine solve (factor time, after k_ulv_root):
+// columns [j0, j0 + NTP) of Dinv (LU solves of identity columns) and of Mb.
+// LDS: bv m x NTP | t1 NB x NTP
+template <typename TF>
+__global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_rinv(const FInst<TF>* __restrict__ insts,
+                                                           const WorkItem* __restrict__ work) {
+    const WorkItem wi = work[blockIdx.x];
+    const FInst<TF>& I = insts[wi.inst];
+    const DevNode nd = I.nodes[wi.node];
+    const FNode f = I.fn[wi.node];
+    if (f.Dinv < 0) return;
+    const int m = f.m, Kx = f.Kx;
+    const int j0 = blockIdx.y * RHS_TILE;
+    // Mb columns: input order [b_1 (mr1); u_1 (kv1); b_2 (mr2); u_2 (kv2)]
+    const int mr1 = I.fn[nd.left].mr, mr2 = I.fn[nd.right].mr;
+    const int kv1 = I.nodes[nd.left].kv;
+    for (int e = threadIdx.x; e < m * RHS_TILE; e += blockDim.x) {
+        const int i = e % m, c = j0 + e / m;
+        if (c >= Kx) continue;
+        TF v = TF(0);
+        if (c < mr1) v = (i == c) ? TF(1) : TF(0);
+        else if (c < mr1 + kv1) v = i >= mr1 ? -I.fac[f.P21 + (i - mr1) + (int64_t)(c - mr1) * mr2] : TF(0);
+        else if (c < mr1 + kv1 + mr2) v = (i - mr1 == c - mr1 - kv1) ? TF(1) : TF(0);
+        else v = i < mr1 ? -I.fac[f.P12 + i + (int64_t)(c - mr1 - kv1 - mr2) * mr1] : TF(0);
+        I.fac[f.Mb + i + (int64_t)c * m] = v;
+    }
+    if (j0 >= m) return;
+    const int nt = min(RHS_TILE, m - j0);
+    extern __shared__ unsigned char lds_raw[];
+    TF* bv = reinterpret_cast<TF*>(lds_raw);
+    TF* t1 = bv + m * NTP;
+    for (int e = threadIdx.x; e < m * NTP; e += blockDim.x) {
+        const int r = e / NTP, j = e % NTP;
+        bv[e] = (j < nt && r == j0 + j) ? TF(1) : TF(0);
+    }
+    __syncthreads();
+    lu_solve(bv, t1, m, nt, I.fac + f.LU, I.fac + f.LUinv, I.ipiv + f.piv);
+    unstage(I.fac + f.Dinv + (int64_t)j0 * m, m, bv, 0, m, nt);
 }
 
 // ---- MFMA tile GEMM of the solve sweeps ----
@@ -1369,6 +1347,365 @@ __global__ __launch_bounds__(256) void k_ulv_gdown(const SInst<TF>* __restrict__
         else z[i + (int64_t)j * n] = v;
     });
 }
+// ---- solve sweeps for nodes whose operators fit in LDS ----
+// Upper tree levels have small operators and few nodes: a streamed K loop would pay
+// one memory latency per chunk.  These variants stage the node's whole operator and
+// input into LDS with batched 16-byte loads (one or two latencies), then run every
+// MFMA from LDS.  Column tile TN = 32; the 4 waves share the 16 x 16 output blocks.
+constexpr int XS = 33;  // LDS pitch of the staged input (32 columns + 1)
+
+// out(i, j) = sum_{k < K} A(i, k) xs(k, j), A column-major in LDS (lda = M); the
+// (16-row, 16-column) blocks of an M x 32 tile are dealt to the waves, each block
+// accumulated in two independent MFMA chains (even and odd k-steps).
+template <typename TF, typename Store>
+__device__ __forceinline__ void lds_gemm(int M, int K, const TF* As, const TF* xs, int ncols, Store store) {
+    using V4 = typename Mfma<TF>::v4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int nrb = (M + 15) / 16, ncb = (ncols + 15) / 16;
+    for (int t = wave; t < nrb * ncb; t += nw) {
+        const int rb = t % nrb, cb = t / nrb;
+        const int i = rb * 16 + lr, jc = cb * 16 + lr;
+        auto av = [&](int kk) { return (i < M && kk < K) ? As[i + kk * M] : TF(0); };
+        auto xv = [&](int kk) { return kk < K ? xs[kk * XS + jc] : TF(0); };
+        V4 acc0 = V4{TF(0), TF(0), TF(0), TF(0)}, acc1 = acc0;
+        int k = 0;
+        for (; k + 4 < K; k += 8) {
+            const TF a0 = av(k + lk), b0 = xv(k + lk), a1 = av(k + 4 + lk), b1 = xv(k + 4 + lk);
+            acc0 = Mfma<TF>::mac(a0, b0, acc0);
+            acc1 = Mfma<TF>::mac(a1, b1, acc1);
+        }
+        if (k < K) acc0 = Mfma<TF>::mac(av(k + lk), xv(k + lk), acc0);
+        if (jc < ncols) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int ii = rb * 16 + Mfma<TF>::row(lk, r);
+                if (ii < M) store(ii, jc, acc0[r] + acc1[r]);
+            }
+        }
+    }
+}
+
+// The same product with A read from memory (column-major, lda), for operands used
+// once per workgroup: each A element feeds exactly one wave, so the waves load their
+// own A fragments straight into registers, U k-steps ahead of the MFMAs.
+template <typename TF, typename Store>
+__device__ __forceinline__ void glb_gemm(int M, int K, const TF* __restrict__ A, int64_t lda, const TF* xs, int ncols,
+                                         Store store) {
+    using V4 = typename Mfma<TF>::v4;
+    constexpr int U = 8;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int nrb = (M + 15) / 16, ncb = (ncols + 15) / 16;
+    const int S = (K + 3) / 4;  // k-steps
+    for (int t = wave; t < nrb * ncb; t += nw) {
+        const int rb = t % nrb, cb = t / nrb;
+        const int i = rb * 16 + lr, jc = cb * 16 + lr;
+        const TF* Ai = A + (i < M ? i : 0);
+        auto ld = [&](int st) {
+            const int kk = 4 * st + lk;
+            return (i < M && kk < K) ? Ai[(int64_t)kk * lda] : TF(0);
+        };
+        V4 acc0 = V4{TF(0), TF(0), TF(0), TF(0)}, acc1 = acc0;
+        TF a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) a[u] = ld(u);
+        for (int s0 = 0; s0 < S; s0 += U) {
+            TF an[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) an[u] = (s0 + U + u < S) ? ld(s0 + U + u) : TF(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int kk = 4 * (s0 + u) + lk;
+                const TF b = kk < K ? xs[kk * XS + jc] : TF(0);
+                if (u & 1) acc1 = Mfma<TF>::mac(a[u], b, acc1);
+                else acc0 = Mfma<TF>::mac(a[u], b, acc0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) a[u] = an[u];
+        }
+        if (jc < ncols) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int ii = rb * 16 + Mfma<TF>::row(lk, r);
+                if (ii < M) store(ii, jc, acc0[r] + acc1[r]);
+            }
+        }
+    }
+}
+
+// Up-sweep, LDS-resident: LDS = [Gz Gr] (nk Kx + R (Kx + nk), rounded to 16) | xs (Kx + nk) x XS
+template <typename TF>
+__global__ __launch_bounds__(512) void k_ulv_gup_s(const SInst<TF>* __restrict__ insts,
+                                                   const WorkItem* __restrict__ work, int nwork, int tn) {
+    int item, ti, tj;
+    if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
+    const WorkItem wi = work[item];
+    const SInst<TF>& I = insts[wi.inst];
+    const FNode f = I.fn[wi.node];
+    const int nrhs = I.nrhs;
+    const int j0 = tj * 32;
+    if (j0 >= nrhs) return;
+    const int nc = min(32, nrhs - j0);
+    const DevNode nd = I.nodes[wi.node];
+    const int nk = f.nk, Kx = f.Kx, My = f.My, R = My - nk;
+    const int64_t na = (int64_t)nk * Kx + (int64_t)R * (Kx + nk);
+    extern __shared__ unsigned char lds_raw[];
+    TF* As = reinterpret_cast<TF*>(lds_raw);
+    TF* xs = As + ((na + 15) & ~int64_t(15));
+    lds_copy(As, I.fac + (nk > 0 ? f.Gz : f.Gr), na);
+    // x rows [0, Kx): the leaf's rows of B, or the children's reduced b and u
+    auto put = [&](int e, TF v) { xs[(e % Kx) * XS + e / Kx] = v; };
+    if (nd.is_leaf) {
+        const double* B = I.B + nd.row_off + (int64_t)j0 * I.ldb;
+        const int64_t ldb = I.ldb;
+        batched<16, TF>(
+            Kx * 32, [&](int e) { return e / Kx < nc ? (TF)B[(e % Kx) + (int64_t)(e / Kx) * ldb] : TF(0); }, put);
+    } else {
+        const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
+        const int r0 = fl.My - fl.nk;
+        const TF* y0 = I.wy + fl.yrow * nrhs + fl.nk + (int64_t)j0 * fl.My;
+        const TF* y1 = I.wy + fr.yrow * nrhs + fr.nk + (int64_t)j0 * fr.My;
+        const int l0 = fl.My, l1 = fr.My;
+        batched<16, TF>(
+            Kx * 32,
+            [&](int e) {
+                const int k = e % Kx, j = e / Kx;
+                if (j >= nc) return TF(0);
+                return k < r0 ? y0[k + (int64_t)j * l0] : y1[(k - r0) + (int64_t)j * l1];
+            },
+            put);
+    }
+    __syncthreads();
+    TF* y = I.wy + f.yrow * nrhs + (int64_t)j0 * My;
+    // zloc = Gz x  -> y rows [0, nk) and xs rows [Kx, Kx + nk)
+    auto st0 = [&](int i, int j, TF v) {
+        y[i + (int64_t)j * My] = v;
+        xs[(Kx + i) * XS + j] = v;
+    };
+    lds_gemm(nk, Kx, As, xs, nc, st0);
+    __syncthreads();
+    // [reduced b; u] = Gr [x; zloc]  -> y rows [nk, My)
+    auto st1 = [&](int i, int j, TF v) { y[nk + i + (int64_t)j * My] = v; };
+    lds_gemm(R, Kx + nk, As + (int64_t)nk * Kx, xs, nc, st1);
+}
+
+// Down-sweep, LDS-resident: LDS = Qt (n x n, rounded to 16; empty for a pass-through
+// node) | xs n x XS
+template <typename TF>
+__global__ __launch_bounds__(512) void k_ulv_gdown_s(const SInst<TF>* __restrict__ insts,
+                                                     const WorkItem* __restrict__ work, int nwork, int tn) {
+    int item, ti, tj;
+    if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
+    const WorkItem wi = work[item];
+    const SInst<TF>& I = insts[wi.inst];
+    const FNode f = I.fn[wi.node];
+    const int nrhs = I.nrhs, n = f.n, nk = f.nk;
+    const int j0 = tj * 32;
+    if (j0 >= nrhs) return;
+    const int nc = min(32, nrhs - j0);
+    const DevNode nd = I.nodes[wi.node];
+    const FNode fp = I.fn[nd.parent];
+    const int off = nd.is_left ? 0 : I.fn[I.nodes[nd.parent].left].nr;
+    const int64_t na = f.Qt >= 0 ? (int64_t)n * n : 0;
+    extern __shared__ unsigned char lds_raw[];
+    TF* As = reinterpret_cast<TF*>(lds_raw);
+    TF* xs = As + ((na + 15) & ~int64_t(15));
+    if (na > 0) lds_copy(As, I.fac + f.Qt, na);
+    // x = [zloc (rows 0..nk-1 of the node's y); z from the parent]
+    const TF* y0 = I.wy + f.yrow * nrhs + (int64_t)j0 * f.My;
+    const TF* z1 = I.wz + fp.zrow * nrhs + off + (int64_t)j0 * fp.n;
+    const int l0 = f.My, l1 = fp.n;
+    batched<16, TF>(
+        n * 32,
+        [&](int e) {
+            const int k = e % n, j = e / n;
+            if (j >= nc) return TF(0);
+            return k < nk ? y0[k + (int64_t)j * l0] : z1[(k - nk) + (int64_t)j * l1];
+        },
+        [&](int e, TF v) { xs[(e % n) * XS + e / n] = v; });
+    __syncthreads();
+    const bool leaf = nd.is_leaf;
+    double* X = I.X + nd.col_off + (int64_t)j0 * I.ldx;
+    const int64_t ldx = I.ldx;
+    TF* z = I.wz + f.zrow * nrhs + (int64_t)j0 * n;
+    auto store = [&](int i, int j, TF v) {
+        if (leaf) X[i + (int64_t)j * ldx] = (double)v;
+        else z[i + (int64_t)j * n] = v;
+    };
+    if (na > 0) {
+        lds_gemm(n, n, As, xs, nc, store);
+    } else {
+        for (int e = threadIdx.x; e < n * nc; e += blockDim.x) store(e % n, e / n, xs[(e % n) * XS + e / n]);
+    }
+}
+
+// Root solve by inverse and one refinement step (ulvfactor.jl:74,83): with the merge
+// b = Mb x (x = the children's reduced b and u), z0 = Dinv b, r = b - D z0,
+// z = z0 + Dinv r.  One refinement step restores the accuracy of the LU solve.
+// LDS: Dinv (m x m, rounded to 16) | xs max(Kx, 2m) x XS | bm m x XS
+template <typename TF>
+__global__ __launch_bounds__(512) void k_ulv_groot2(const SInst<TF>* __restrict__ insts,
+                                                    const WorkItem* __restrict__ work, int tn) {
+    const int item = blockIdx.x / tn, tj = blockIdx.x % tn;
+    const WorkItem wi = work[item];
+    const SInst<TF>& I = insts[wi.inst];
+    const int nrhs = I.nrhs;
+    const int j0 = tj * 32;
+    if (j0 >= nrhs) return;
+    const int nc = min(32, nrhs - j0);
+    const DevNode nd = I.nodes[wi.node];
+    const FNode f = I.fn[wi.node];
+    const int m = f.m, Kx = f.Kx;
+    extern __shared__ unsigned char lds_raw[];
+    TF* Di = reinterpret_cast<TF*>(lds_raw);
+    TF* xs = Di + (((int64_t)m * m + 15) & ~int64_t(15));
+    TF* bm = xs + max(Kx, 2 * m) * XS;
+    lds_copy(Di, I.fac + f.Dinv, (int64_t)m * m);
+    const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
+    const int r0 = fl.My - fl.nk;
+    const TF* y0 = I.wy + fl.yrow * nrhs + fl.nk + (int64_t)j0 * fl.My;
+    const TF* y1 = I.wy + fr.yrow * nrhs + fr.nk + (int64_t)j0 * fr.My;
+    const int l0 = fl.My, l1 = fr.My;
+    batched<16, TF>(
+        Kx * 32,
+        [&](int e) {
+            const int k = e % Kx, j = e / Kx;
+            if (j >= nc) return TF(0);
+            return k < r0 ? y0[k + (int64_t)j * l0] : y1[(k - r0) + (int64_t)j * l1];
+        },
+        [&](int e, TF v) { xs[(e % Kx) * XS + e / Kx] = v; });
+    __syncthreads();
+    glb_gemm(m, Kx, I.fac + f.Mb, m, xs, nc, [&](int i, int j, TF v) { bm[i * XS + j] = v; });
+    __syncthreads();
+    lds_gemm(m, m, Di, bm, nc, [&](int i, int j, TF v) { xs[i * XS + j] = v; });  // z0
+    __syncthreads();
+    glb_gemm(m, m, I.fac + f.Dr, m, xs, nc,
+             [&](int i, int j, TF v) { xs[(m + i) * XS + j] = bm[i * XS + j] - v; });  // r
+    __syncthreads();
+    TF* z = I.wz + f.zrow * nrhs + (int64_t)j0 * f.n;
+    lds_gemm(m, m, Di, xs + m * XS, nc, [&](int i, int j, TF v) { z[i + (int64_t)j * f.n] = xs[i * XS + j] + v; });
+}
+
+// ---------------------------------------------------------------------------
+// matvec (reference src/matmul.jl:18-62) over per-node operators
+// ---------------------------------------------------------------------------
+// Each sweep step of a node is one product y = A [x_0; x_1] (A = the node's operator):
+//   up   (per height):  Z = V' B_seg (leaf) | [W_1' W_2'] [Z_1; Z_2]       (_matmatup, matmul.jl:32-42)
+//   down (per depth):   F = [B_sibling R] [Z_sibling; F_parent]             (_matmatdown!, matmul.jl:51-57)
+//   leaf:               C_seg = alpha [D U] [B_seg; F] + beta C_seg         (matmul.jl:46-47)
+struct MvOp {
+    const double* A;
+    int M, K;
+    SegOp x;
+    double* y;
+    int64_t ldy;
+};
+
+template <int KIND>
+__device__ __forceinline__ void mv_op(const MvInst& I, int node, MvOp& o) {
+    const DevNode nd = I.nodes[node];
+    const int nrhs = I.nrhs;
+    o.x.dbl0 = o.x.dbl1 = 1;
+    o.x.p1 = nullptr;
+    o.x.ld1 = 1;
+    if (KIND == 0) {
+        o.A = I.op + nd.opZ;
+        o.M = nd.kv;
+        if (nd.is_leaf) {
+            o.K = nd.n;
+            o.x.p0 = I.B + nd.col_off;
+            o.x.ld0 = I.ldb;
+            o.x.r0 = nd.n;
+        } else {
+            const DevNode L = I.nodes[nd.left], R = I.nodes[nd.right];
+            o.K = L.kv + R.kv;
+            o.x.p0 = I.Z + L.zrow * nrhs;
+            o.x.ld0 = L.kv > 0 ? L.kv : 1;
+            o.x.r0 = L.kv;
+            o.x.p1 = I.Z + R.zrow * nrhs;
+            o.x.ld1 = R.kv > 0 ? R.kv : 1;
+        }
+        o.y = I.Z + nd.zrow * nrhs;
+        o.ldy = nd.kv;
+    } else if (KIND == 1) {
+        const DevNode p = I.nodes[nd.parent];
+        const DevNode sb = I.nodes[nd.is_left ? p.right : p.left];
+        o.A = I.op + nd.opF;
+        o.M = nd.ku;
+        o.K = sb.kv + p.ku;
+        o.x.p0 = I.Z + sb.zrow * nrhs;
+        o.x.ld0 = sb.kv > 0 ? sb.kv : 1;
+        o.x.r0 = sb.kv;
+        o.x.p1 = I.F + p.frow * nrhs;
+        o.x.ld1 = p.ku > 0 ? p.ku : 1;
+        o.y = I.F + nd.frow * nrhs;
+        o.ldy = nd.ku;
+    } else {
+        o.A = I.op + nd.opC;
+        o.M = nd.m;
+        o.K = nd.n + nd.ku;
+        o.x.p0 = I.B + nd.col_off;
+        o.x.ld0 = I.ldb;
+        o.x.r0 = nd.n;
+        o.x.p1 = I.F + nd.frow * nrhs;
+        o.x.ld1 = nd.ku > 0 ? nd.ku : 1;
+        o.y = I.C + nd.row_off;
+        o.ldy = I.ldc;
+    }
+    o.x.r1 = o.K;
+}
+
+template <int KIND>
+__device__ __forceinline__ void mv_store(const MvInst& I, const MvOp& o, int i, int j, double v) {
+    double* c = o.y + i + (int64_t)j * o.ldy;
+    if (KIND == 2) *c = (I.beta == 0.0) ? I.alpha * v : I.alpha * v + I.beta * (*c);
+    else *c = v;
+}
+
+// LDS-resident: LDS = A (M x K, rounded to 16) | xs K x XS
+template <int KIND>
+__global__ __launch_bounds__(512) void k_mv_s(const MvInst* __restrict__ insts, const WorkItem* __restrict__ work,
+                                              int nwork, int tn) {
+    int item, ti, tj;
+    if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
+    const WorkItem wi = work[item];
+    const MvInst& I = insts[wi.inst];
+    const int j0 = tj * 32;
+    if (j0 >= I.nrhs) return;
+    const int nc = min(32, I.nrhs - j0);
+    MvOp o;
+    mv_op<KIND>(I, wi.node, o);
+    const int M = o.M, K = o.K;
+    const int64_t na = (int64_t)M * K;
+    extern __shared__ unsigned char lds_raw[];
+    double* As = reinterpret_cast<double*>(lds_raw);
+    double* xs = As + ((na + 15) & ~int64_t(15));
+    lds_copy(As, o.A, na);
+    batched<16, double>(
+        K * 32, [&](int e) { return e / K < nc ? seg_load<double>(o.x, nullptr, 0, e % K, j0 + e / K, 0) : 0.0; },
+        [&](int e, double v) { xs[(e % K) * XS + e / K] = v; });
+    __syncthreads();
+    lds_gemm(M, K, As, xs, nc, [&](int i, int j, double v) { mv_store<KIND>(I, o, i, j0 + j, v); });
+}
+
+// Streamed through the chunked MFMA tile GEMM (nodes whose operators do not fit in LDS).
+template <int KIND, int WM, int WN>
+__global__ __launch_bounds__(256) void k_mv_t(const MvInst* __restrict__ insts, const WorkItem* __restrict__ work,
+                                              int nwork, int tm, int tn) {
+    int item, ti, tj;
+    if (!tile_of(nwork * tm * tn, tm, tn, item, ti, tj)) return;
+    const WorkItem wi = work[item];
+    const MvInst& I = insts[wi.inst];
+    const int i0 = ti * 32 * WM, j0 = tj * 32 * WN;
+    MvOp o;
+    mv_op<KIND>(I, wi.node, o);
+    if (i0 >= o.M || j0 >= I.nrhs) return;
+    __shared__ TileLds<double, WM, WN> L;
+    tile_gemm<double, WM, WN>(L, o.M, I.nrhs, o.K, o.A, o.M, o.x, nullptr, 0, i0, j0,
+                              [&](int i, int j, double v) { mv_store<KIND>(I, o, i, j, v); });
+}
+
 // ---------------------------------------------------------------------------
 // vector helpers for refinement
 // ---------------------------------------------------------------------------
@@ -1462,28 +1799,6 @@ static hipError_t lds_attr(const void* fn, size_t lds) {
     return hipSuccess;
 }
 
-hipError_t launch_mv_up(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                        hipStream_t st) {
-    if (nwork <= 0) return hipSuccess;
-    if (hipError_t e = lds_attr(reinterpret_cast<const void*>(&k_mv_up), lds)) return e;
-    hipLaunchKernelGGL(k_mv_up, dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st, insts, work);
-    return hipGetLastError();
-}
-hipError_t launch_mv_down(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                          hipStream_t st) {
-    if (nwork <= 0) return hipSuccess;
-    if (hipError_t e = lds_attr(reinterpret_cast<const void*>(&k_mv_down), lds)) return e;
-    hipLaunchKernelGGL(k_mv_down, dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st, insts, work);
-    return hipGetLastError();
-}
-hipError_t launch_mv_leaf(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                          hipStream_t st) {
-    if (nwork <= 0) return hipSuccess;
-    if (hipError_t e = lds_attr(reinterpret_cast<const void*>(&k_mv_leaf), lds)) return e;
-    hipLaunchKernelGGL(k_mv_leaf, dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st, insts, work);
-    return hipGetLastError();
-}
-
 template <typename T>
 static hipError_t set_lds_attr(const void* fn, size_t lds) {
     if (lds > 65536) return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1532,14 +1847,37 @@ hipError_t launch_ulv_qform(const FInst<T>* insts, const WorkItem* work, int nwo
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_ulv_groot(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                            hipStream_t st) {
-    if (nwork <= 0) return hipSuccess;
-    const void* fn = reinterpret_cast<const void*>(&k_ulv_groot<T>);
+hipError_t launch_ulv_rinv(const FInst<T>* insts, const WorkItem* work, int nwork, int max_cols, size_t lds,
+                           hipStream_t st) {
+    if (nwork <= 0 || max_cols <= 0) return hipSuccess;
+    const void* fn = reinterpret_cast<const void*>(&k_ulv_rinv<T>);
     hipError_t e = set_lds_attr<T>(fn, lds);
     if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_ulv_rinv<T>), dim3(nwork, rhs_tiles(max_cols)), dim3(SWEEP_THREADS), lds, st, insts, work);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_ulv_groot2(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                             hipStream_t st) {
+    if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const int tn = (max_nrhs + 31) / 32;
+    const void* fn = reinterpret_cast<const void*>(&k_ulv_groot2<T>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_ulv_groot2<T>), dim3(nwork * tn), dim3(512), lds, st, insts, work, tn);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_ulv_groot(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                            int resident, hipStream_t st) {
+    if (nwork <= 0) return hipSuccess;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const void* fn = reinterpret_cast<const void*>(&k_ulv_groot<T>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_ulv_groot<T>), dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st, insts,
-                       work);
+                       work, resident);
     return hipGetLastError();
 }
 
@@ -1569,6 +1907,34 @@ hipError_t launch_ulv_gup(const SInst<T>* insts, const WorkItem* work, int nwork
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_ulv_gup<T>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), lds, st, insts, work, nwork,
                        tn);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                            hipStream_t st) {
+    if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
+    const int tn = (max_nrhs + 31) / 32;
+    const int64_t total = (int64_t)nwork * tn;
+    if (total > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gup_s<T>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_ulv_gup_s<T>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts, work,
+                       nwork, tn);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                              hipStream_t st) {
+    if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
+    const int tn = (max_nrhs + 31) / 32;
+    const int64_t total = (int64_t)nwork * tn;
+    if (total > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gdown_s<T>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_ulv_gdown_s<T>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts, work,
+                       nwork, tn);
     return hipGetLastError();
 }
 template <typename T>
@@ -1620,6 +1986,41 @@ hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, do
     return hipGetLastError();
 }
 
+template <int KIND>
+static hipError_t launch_mv_kind(const MvInst* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
+                                 size_t lds, hipStream_t st) {
+    if (nwork <= 0 || max_nrhs <= 0 || max_m <= 0) return hipSuccess;
+    if (lds > 0) {
+        if (lds > 160 * 1024) return hipErrorInvalidValue;
+        const int tn = (max_nrhs + 31) / 32;
+        const int64_t total = (int64_t)nwork * tn;
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mv_s<KIND>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_mv_s<KIND>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts, work,
+                           nwork, tn);
+        return hipGetLastError();
+    }
+    dim3 grid;
+    int tm, tn;
+    if (max_nrhs > 32) {
+        if (hipError_t e = sweep_grid<double>(nwork, max_m, max_nrhs, 64, 64, grid, tm, tn)) return e;
+        hipLaunchKernelGGL((k_mv_t<KIND, 2, 2>), grid, dim3(256), 0, st, insts, work, nwork, tm, tn);
+    } else {
+        if (hipError_t e = sweep_grid<double>(nwork, max_m, max_nrhs, 128, 32, grid, tm, tn)) return e;
+        hipLaunchKernelGGL((k_mv_t<KIND, 4, 1>), grid, dim3(256), 0, st, insts, work, nwork, tm, tn);
+    }
+    return hipGetLastError();
+}
+hipError_t launch_mv(int kind, const MvInst* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
+                     size_t lds, hipStream_t st) {
+    switch (kind) {
+        case 0: return launch_mv_kind<0>(insts, work, nwork, max_m, max_nrhs, lds, st);
+        case 1: return launch_mv_kind<1>(insts, work, nwork, max_m, max_nrhs, lds, st);
+        default: return launch_mv_kind<2>(insts, work, nwork, max_m, max_nrhs, lds, st);
+    }
+}
+
 // explicit instantiations
 template hipError_t launch_ulv_factor<double>(const FInst<double>*, const WorkItem*, int, size_t, hipStream_t);
 template hipError_t launch_ulv_factor<float>(const FInst<float>*, const WorkItem*, int, size_t, hipStream_t);
@@ -1631,8 +2032,12 @@ template hipError_t launch_ulv_root<float>(const FInst<float>*, const WorkItem*,
     template hipError_t launch_ulv_gform<T>(const FInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
     template hipError_t launch_ulv_qform<T>(const FInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
     template hipError_t launch_ulv_gup<T>(const SInst<T>*, const WorkItem*, int, int, int, hipStream_t);            \
-    template hipError_t launch_ulv_groot<T>(const SInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
-    template hipError_t launch_ulv_gdown<T>(const SInst<T>*, const WorkItem*, int, int, int, hipStream_t);
+    template hipError_t launch_ulv_groot<T>(const SInst<T>*, const WorkItem*, int, int, size_t, int, hipStream_t);   \
+    template hipError_t launch_ulv_rinv<T>(const FInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
+    template hipError_t launch_ulv_groot2<T>(const SInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
+    template hipError_t launch_ulv_gdown<T>(const SInst<T>*, const WorkItem*, int, int, int, hipStream_t);      \
+    template hipError_t launch_ulv_gup_s<T>(const SInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);      \
+    template hipError_t launch_ulv_gdown_s<T>(const SInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);
 HSS_INST_SOLVE(double)
 HSS_INST_SOLVE(float)
 #undef HSS_INST_SOLVE

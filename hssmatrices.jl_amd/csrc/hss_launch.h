@@ -6,12 +6,10 @@
 
 namespace hss {
 
-hipError_t launch_mv_up(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                        hipStream_t st);
-hipError_t launch_mv_down(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                          hipStream_t st);
-hipError_t launch_mv_leaf(const MvInst* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                          hipStream_t st);
+// matvec sweep step over nodes (kind 0 = up: Z, 1 = down: F, 2 = leaves: C); lds > 0
+// selects the LDS-resident kernel with that many bytes, 0 the chunked one
+hipError_t launch_mv(int kind, const MvInst* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
+                     size_t lds, hipStream_t st);
 
 template <typename T>
 hipError_t launch_ulv_factor(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds, hipStream_t st);
@@ -30,12 +28,28 @@ hipError_t launch_ulv_qform(const FInst<T>* insts, const WorkItem* work, int nwo
 template <typename T>
 hipError_t launch_ulv_gup(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nk, int max_nrhs,
                           hipStream_t st);
+// root operators of the inverse-and-refine solve (factor time; max_cols = max(m, Kx))
+template <typename T>
+hipError_t launch_ulv_rinv(const FInst<T>* insts, const WorkItem* work, int nwork, int max_cols, size_t lds,
+                           hipStream_t st);
+// root solve by inverse and one refinement step (branch roots whose Dinv fits in LDS)
+template <typename T>
+hipError_t launch_ulv_groot2(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                             hipStream_t st);
+// resident != 0: the root factor is staged in LDS (lds includes it)
 template <typename T>
 hipError_t launch_ulv_groot(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                            hipStream_t st);
+                            int resident, hipStream_t st);
 template <typename T>
 hipError_t launch_ulv_gdown(const SInst<T>* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
                             hipStream_t st);
+// LDS-resident variants for levels whose operators fit (lds = bytes per workgroup)
+template <typename T>
+hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                            hipStream_t st);
+template <typename T>
+hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+                              hipStream_t st);
 
 hipError_t launch_resid_batch(const BatchVec* tab, int cnt, int maxc, double* norms, hipStream_t st);
 hipError_t launch_axpy_batch(const BatchVec* tab, int cnt, int maxc, hipStream_t st);

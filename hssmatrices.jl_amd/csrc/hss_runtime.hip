@@ -138,12 +138,27 @@ struct SlotCache {
 };
 
 // Level schedule of a set of solve requests: up-sweep by height, roots, down-sweep by
-// depth; up_m = the largest nk, down_m = the largest n of a level's nodes.
+// depth; up_m = the largest nk, down_m = the largest n of a level's nodes; up_lds /
+// down_lds = LDS bytes of the level's largest node with operator and input staged.
 struct Schedule {
     std::vector<WorkItem> work;
     std::vector<Range> up, down;
     std::vector<int> up_m, down_m;
+    std::vector<size_t> up_lds, down_lds;
     Range roots;
+};
+// Largest LDS of a workgroup (bytes); levels above it use the streamed kernels.
+constexpr size_t kLdsResidentMax = 160 * 1024;
+
+// Matvec schedule: up-sweep steps by height, down-sweep steps by depth (1..), leaves;
+// per step the largest operator height and the LDS of the resident kernel (0 = chunked).
+struct MvSched {
+    std::vector<Range> up, down;
+    Range leaf;
+    std::vector<int> up_m, down_m;
+    int leaf_m = 0;
+    std::vector<size_t> up_lds, down_lds;
+    size_t leaf_lds = 0;
 };
 
 struct PlanKey {
@@ -189,12 +204,14 @@ struct hss_mat_s {
     std::vector<WorkItem> hwork;
     WorkItem* d_work = nullptr;
     std::vector<Range> by_height;  // non-root nodes of height h
-    std::vector<Range> br_depth;   // branch nodes of depth d
     std::vector<Range> by_depth;   // nodes of depth d (d >= 1)
     Range leaves, root;
     int max_height = 0, max_depth = 0;
     int64_t zrows = 0, frows = 0;
-    size_t lds_up = 0, lds_down = 0, lds_leaf = 0;  // bytes per launch
+    double* d_op = nullptr;  // matvec operators (DevNode::opZ/opF/opC)
+    int64_t op_len = 0;
+    MvSched mvs;             // matvec schedule of this matrix (instance 0)
+    WorkItem* d_mvwork = nullptr;
     // per-call scratch
     SlotCache mvinst;
     double *d_Z = nullptr, *d_F = nullptr;
@@ -217,6 +234,8 @@ struct hss_fact_s {
     int64_t res_len = 0;               // resident part of the factor (G, Qt, root LU)
     int64_t yrows = 0, zrows = 0;      // solve workspace rows (times nrhs)
     size_t lds_groot = 0;              // LDS of the root solve kernel
+    size_t lds_groot_res = 0;          // the same with the root's factor staged in LDS
+    size_t lds_groot2 = 0;             // the inverse-and-refine root kernel (0: not available)
     size_t lds_gform = 0, lds_qform = 0;
     int max_kx = 0, max_nq = 0;        // widest Gr / Qt (columns formed per node)
     void *d_wy = nullptr, *d_wz = nullptr;
@@ -317,6 +336,8 @@ static hss_status validate(const hss_node_desc* nd, int nn, int64_t blob_len) {
 static void free_mat(hss_mat_s* M) {
     if (!M) return;
     (void)hipFree(M->d_blob);
+    (void)hipFree(M->d_op);
+    (void)hipFree(M->d_mvwork);
     (void)hipFree(M->d_nodes);
     (void)hipFree(M->d_work);
     (void)hipFree(M->d_Z);
@@ -324,6 +345,132 @@ static void free_mat(hss_mat_s* M) {
     (void)hipFree(M->d_io_b);
     (void)hipFree(M->d_io_c);
     delete M;
+}
+
+// Operator shape of node i in a matvec step (kind 0 up, 1 down, 2 leaf): rows M, columns K.
+static void mv_shape(const hss_mat_s* M, int i, int kind, int& rows, int& cols) {
+    const DevNode& d = M->hn[i];
+    if (kind == 0) {
+        rows = d.kv;
+        cols = d.is_leaf ? d.n : M->hn[d.left].kv + M->hn[d.right].kv;
+    } else if (kind == 1) {
+        const DevNode& p = M->hn[d.parent];
+        const DevNode& sb = M->hn[d.is_left ? p.right : p.left];
+        rows = d.ku;
+        cols = sb.kv + p.ku;
+    } else {
+        rows = d.m;
+        cols = d.n + d.ku;
+    }
+}
+
+// Appends the matvec work lists of `mats` (instance b = position in mats) to `work`.
+static void build_mv_schedule(const std::vector<hss_mat_s*>& mats, std::vector<WorkItem>& work, MvSched& S) {
+    int maxh = 0, maxd = 0;
+    for (auto* M : mats) {
+        maxh = std::max(maxh, M->max_height);
+        maxd = std::max(maxd, M->max_depth);
+    }
+    auto lds_of = [](int rows, int cols) {
+        const size_t b = sizeof(double) * (size_t)(((int64_t)rows * cols + 15) / 16 * 16 + (int64_t)cols * 33);
+        return b;
+    };
+    auto level = [&](int kind, auto sel, Range& r, int& mmax, size_t& lds) {
+        r.off = (int)work.size();
+        bool fits = true;
+        size_t need = 0;
+        for (int b = 0; b < (int)mats.size(); ++b) {
+            hss_mat_s* M = mats[b];
+            for (int i = 0; i < M->nn; ++i) {
+                if (!sel(M, i)) continue;
+                work.push_back(WorkItem{b, i});
+                int rows, cols;
+                mv_shape(M, i, kind, rows, cols);
+                mmax = std::max(mmax, rows);
+                const size_t l = lds_of(rows, cols);
+                need = std::max(need, l);
+                fits = fits && l <= kLdsResidentMax;
+            }
+        }
+        r.count = (int)work.size() - r.off;
+        lds = fits ? need : 0;
+    };
+    S.up.assign(maxh, Range{});
+    S.up_m.assign(maxh, 0);
+    S.up_lds.assign(maxh, 0);
+    for (int h = 0; h < maxh; ++h)
+        level(0, [&](const hss_mat_s* M, int i) { return i != 0 && M->hn[i].height == h; }, S.up[h], S.up_m[h],
+              S.up_lds[h]);
+    S.down.assign(maxd + 1, Range{});
+    S.down_m.assign(maxd + 1, 0);
+    S.down_lds.assign(maxd + 1, 0);
+    for (int d = 1; d <= maxd; ++d)
+        level(1, [&](const hss_mat_s* M, int i) { return i != 0 && M->hn[i].depth == d; }, S.down[d], S.down_m[d],
+              S.down_lds[d]);
+    S.leaf_m = 0;
+    level(2, [&](const hss_mat_s* M, int i) { return M->hn[i].is_leaf != 0; }, S.leaf, S.leaf_m, S.leaf_lds);
+}
+
+static hss_status launch_mv_schedule(const MvInst* di, const WorkItem* dw, const MvSched& S, int maxnrhs,
+                                     hipStream_t st) {
+    for (size_t h = 0; h < S.up.size(); ++h) {
+        ProfScope ps(S.up_lds[h] ? "k_mv_up_s" : "k_mv_up", (int)h, S.up[h].count, st);
+        HIP_TRY(launch_mv(0, di, dw + S.up[h].off, S.up[h].count, S.up_m[h], maxnrhs, S.up_lds[h], st));
+    }
+    for (size_t d = 1; d < S.down.size(); ++d) {
+        ProfScope ps(S.down_lds[d] ? "k_mv_down_s" : "k_mv_down", (int)d, S.down[d].count, st);
+        HIP_TRY(launch_mv(1, di, dw + S.down[d].off, S.down[d].count, S.down_m[d], maxnrhs, S.down_lds[d], st));
+    }
+    ProfScope ps(S.leaf_lds ? "k_mv_leaf_s" : "k_mv_leaf", 0, S.leaf.count, st);
+    HIP_TRY(launch_mv(2, di, dw + S.leaf.off, S.leaf.count, S.leaf_m, maxnrhs, S.leaf_lds, st));
+    return HSS_OK;
+}
+
+// Matvec operators of every node, column-major, each starting on a 16-element boundary.
+static void build_mv_operators(hss_mat_s* M, const double* blob, std::vector<double>& op) {
+    auto a16 = [&]() { op.resize((op.size() + 15) & ~size_t(15), 0.0); };
+    for (int i = 0; i < M->nn; ++i) {
+        DevNode& d = M->hn[i];
+        d.opZ = d.opF = d.opC = -1;
+        if (i != 0) {
+            a16();
+            d.opZ = (int64_t)op.size();
+            const int kv = d.kv;
+            if (d.is_leaf) {
+                op.resize(op.size() + (size_t)kv * d.n);
+                for (int c = 0; c < d.n; ++c)
+                    for (int r = 0; r < kv; ++r) op[d.opZ + r + (int64_t)c * kv] = blob[d.V + c + (int64_t)r * d.n];
+            } else {
+                const DevNode& L = M->hn[d.left];
+                const DevNode& R = M->hn[d.right];
+                op.resize(op.size() + (size_t)kv * (L.kv + R.kv));
+                for (int c = 0; c < L.kv; ++c)
+                    for (int r = 0; r < kv; ++r) op[d.opZ + r + (int64_t)c * kv] = blob[L.W + c + (int64_t)r * L.kv];
+                for (int c = 0; c < R.kv; ++c)
+                    for (int r = 0; r < kv; ++r)
+                        op[d.opZ + r + (int64_t)(L.kv + c) * kv] = blob[R.W + c + (int64_t)r * R.kv];
+            }
+            a16();
+            d.opF = (int64_t)op.size();
+            const DevNode& p = M->hn[d.parent];
+            const DevNode& sb = M->hn[d.is_left ? p.right : p.left];
+            const int64_t boff = d.is_left ? p.B12 : p.B21;
+            const int ku = d.ku;
+            op.resize(op.size() + (size_t)ku * (sb.kv + p.ku));
+            for (int c = 0; c < sb.kv; ++c)
+                for (int r = 0; r < ku; ++r) op[d.opF + r + (int64_t)c * ku] = blob[boff + r + (int64_t)c * ku];
+            for (int c = 0; c < p.ku; ++c)
+                for (int r = 0; r < ku; ++r) op[d.opF + r + (int64_t)(sb.kv + c) * ku] = blob[d.R + r + (int64_t)c * ku];
+        }
+        if (d.is_leaf) {
+            a16();
+            d.opC = (int64_t)op.size();
+            op.insert(op.end(), blob + d.D, blob + d.D + (int64_t)d.m * d.n);
+            if (d.ku > 0) op.insert(op.end(), blob + d.U, blob + d.U + (int64_t)d.m * d.ku);
+        }
+    }
+    a16();
+    op.resize(op.size() + 16, 0.0);  // whole-vector reads past the last operator stay in bounds
 }
 
 extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, int32_t nn, const double* blob,
@@ -381,8 +528,6 @@ extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, 
         }
     };
     add_range(M->by_height, [&](int i) { return i != 0; }, M->max_height, [&](int i) { return M->hn[i].height; });
-    add_range(M->br_depth, [&](int i) { return !M->hn[i].is_leaf; }, M->max_depth + 1,
-              [&](int i) { return M->hn[i].depth; });
     add_range(M->by_depth, [&](int i) { return i != 0; }, M->max_depth + 1, [&](int i) { return M->hn[i].depth; });
     M->leaves.off = (int)M->hwork.size();
     for (int i = 0; i < nn; ++i)
@@ -391,29 +536,16 @@ extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, 
     M->root.off = (int)M->hwork.size();
     M->hwork.push_back(WorkItem{0, 0});
     M->root.count = 1;
-    size_t lup = 0, ldown = 0, lleaf = 0;
-    for (int i = 0; i < nn; ++i) {
-        const DevNode& d = M->hn[i];
-        if (d.is_leaf) {
-            lup = std::max(lup, (size_t)d.n);
-            lleaf = std::max(lleaf, (size_t)(d.n + d.m + d.ku));
-        } else {
-            const DevNode& l = M->hn[d.left];
-            const DevNode& r = M->hn[d.right];
-            lup = std::max(lup, (size_t)(l.kv + r.kv + d.kv));
-            ldown = std::max(ldown, (size_t)(l.kv + r.kv + d.ku + l.ku + r.ku));
-        }
-    }
-    M->lds_up = lup * RHS_TILE * sizeof(double);
-    M->lds_down = ldown * RHS_TILE * sizeof(double);
-    M->lds_leaf = lleaf * RHS_TILE * sizeof(double);
-    if (M->lds_leaf > 160 * 1024 || M->lds_up > 160 * 1024 || M->lds_down > 160 * 1024) {
-        free_mat(M);
-        return fail(HSS_ERR_UNSUPPORTED, "node too large for the LDS-staged matvec");
-    }
+    std::vector<double> op;
+    build_mv_operators(M, blob, op);
+    M->op_len = (int64_t)op.size();
+    std::vector<WorkItem> mvwork;
+    build_mv_schedule(std::vector<hss_mat_s*>{M}, mvwork, M->mvs);
 
     hipError_t e;
     if ((e = dalloc(&M->d_blob, (size_t)blob_len)) != hipSuccess ||
+        (e = dalloc(&M->d_op, (size_t)M->op_len)) != hipSuccess ||
+        (e = dalloc(&M->d_mvwork, mvwork.size())) != hipSuccess ||
         (e = dalloc(&M->d_nodes, (size_t)nn)) != hipSuccess ||
         (e = dalloc(&M->d_work, M->hwork.size())) != hipSuccess) {
         free_mat(M);
@@ -421,10 +553,13 @@ extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, 
     }
     hipStream_t st = ctx->stream;
     if (blob_len > 0) HIP_TRY(hipMemcpyAsync(M->d_blob, blob, sizeof(double) * blob_len, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(M->d_op, op.data(), sizeof(double) * op.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(M->d_mvwork, mvwork.data(), sizeof(WorkItem) * mvwork.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(M->d_nodes, M->hn.data(), sizeof(DevNode) * nn, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(M->d_work, M->hwork.data(), sizeof(WorkItem) * M->hwork.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
-    M->dev_bytes = (int64_t)(sizeof(double) * blob_len + sizeof(DevNode) * nn + sizeof(WorkItem) * M->hwork.size());
+    M->dev_bytes = (int64_t)(sizeof(double) * (blob_len + M->op_len) + sizeof(DevNode) * nn +
+                             sizeof(WorkItem) * (M->hwork.size() + mvwork.size()));
     ctx->refs++;
     *out = M;
     return HSS_OK;
@@ -479,7 +614,7 @@ static hss_status matmul_enqueue(hss_mat_s* M, int nrhs, double alpha, const dou
     MvInst I;
     std::memset(&I, 0, sizeof(I));
     I.nodes = M->d_nodes;
-    I.blob = M->d_blob;
+    I.op = M->d_op;
     I.B = dB;
     I.C = dC;
     I.ldb = ldb;
@@ -491,24 +626,7 @@ static hss_status matmul_enqueue(hss_mat_s* M, int nrhs, double alpha, const dou
     I.nrhs = nrhs;
     const void* dv = nullptr;
     HIP_TRY(M->mvinst.get(&I, sizeof(I), st, &dv));
-    const MvInst* di = reinterpret_cast<const MvInst*>(dv);
-    if (M->nn > 1) {
-        for (int h = 0; h < M->max_height; ++h) {
-            const Range& r = M->by_height[h];
-            ProfScope ps("k_mv_up", h, r.count, st);
-            HIP_TRY(launch_mv_up(di, M->d_work + r.off, r.count, nrhs, M->lds_up, st));
-        }
-        for (int d = 0; d <= M->max_depth; ++d) {
-            const Range& r = M->br_depth[d];
-            ProfScope ps("k_mv_down", d, r.count, st);
-            HIP_TRY(launch_mv_down(di, M->d_work + r.off, r.count, nrhs, M->lds_down, st));
-        }
-    }
-    {
-        ProfScope ps("k_mv_leaf", 0, M->leaves.count, st);
-        HIP_TRY(launch_mv_leaf(di, M->d_work + M->leaves.off, M->leaves.count, nrhs, M->lds_leaf, st));
-    }
-    return HSS_OK;
+    return launch_mv_schedule(reinterpret_cast<const MvInst*>(dv), M->d_mvwork, M->mvs, nrhs, st);
 }
 
 extern "C" hss_status hss_matmul_dev(hss_mat_t M, int32_t nrhs, double alpha, const double* dB, int64_t ldb,
@@ -598,6 +716,7 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
         f.Yql = f.Tql = f.Ylq = f.taulq = f.Tlq = f.L1 = f.L1inv = f.L2a = f.Vt = f.P12 = f.P21 = f.LU = f.LUinv = f.piv = -1;
         f.rwD = f.rwU = f.rwV = f.rwM = -1;
         f.yrow = f.zrow = -1;
+        f.Dr = f.Dinv = f.Mb = -1;
         if (d.is_leaf) {
             f.m = d.m;
             f.n = d.n;
@@ -613,6 +732,11 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
         if (i == 0) {
             f.mr = f.m;
             f.nr = f.n;
+            if (!d.is_leaf) {
+                const FNode& a = F->hfn[d.left];
+                const FNode& b = F->hfn[d.right];
+                f.Kx = (a.My - a.nk) + (b.My - b.nk);
+            }
             continue;
         }
         f.k = std::min(d.ku, f.m);
@@ -639,14 +763,19 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
         if (factor_lds<TF>(f, d) > 160 * 1024)
             return fail(HSS_ERR_UNSUPPORTED, "node " + std::to_string(i) + " exceeds the LDS of the factor kernel");
     }
-    // resident solve operators
+    // resident solve operators; each node's [Gz Gr] and Qt start on a 16-element boundary
+    // (whole-operator vector loads)
     int64_t fac = 0;
+    auto align16 = [](int64_t& cur) { cur = (cur + 15) & ~int64_t(15); };
     for (int i = 1; i < nn; ++i) {
         FNode& f = F->hfn[i];
+        align16(fac);
         if (f.nk > 0) f.Gz = take(fac, (int64_t)f.nk * f.Kx);
         f.Gr = take(fac, (int64_t)(f.My - f.nk) * (f.Kx + f.nk));
+        align16(fac);
         if (!f.pass) f.Qt = take(fac, (int64_t)f.n * f.n);
     }
+    align16(fac);
     FNode& fr = F->hfn[0];
     const DevNode& dr = M->hn[0];
     fr.LU = take(fac, (int64_t)fr.m * fr.m);
@@ -656,6 +785,13 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
         fr.P21 = take(fac, (int64_t)F->hfn[dr.right].mr * M->hn[dr.left].kv);
     }
     fr.piv = 0;
+    fr.rlen = fac - fr.LU;
+    if (!dr.is_leaf) {
+        align16(fac);
+        fr.Dinv = take(fac, (int64_t)fr.m * fr.m);
+        fr.Dr = take(fac, (int64_t)fr.m * fr.m);
+        fr.Mb = take(fac, (int64_t)fr.m * fr.Kx);
+    }
     F->res_len = fac;
     // factorization-time data
     int64_t rw = 0, yrows = 0, zrows = 0;
@@ -712,6 +848,10 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
     F->lds_gform = gform * tv;
     F->lds_qform = qform * tv;
     F->lds_groot = groot * tv;
+    F->lds_groot_res = groot * tv + sizeof(TF) * (size_t)((fr.rlen + 15) & ~int64_t(15));
+    if (fr.Dinv >= 0)
+        F->lds_groot2 = sizeof(TF) * (size_t)(((int64_t)fr.m * fr.m + 15) / 16 * 16 +
+                                              (int64_t)std::max(fr.Kx, 2 * fr.m) * 33 + (int64_t)fr.m * 33);
     F->max_kx = max_kx;
     F->max_nq = max_nq;
     if (F->lds_gform > 160 * 1024 || F->lds_groot > 160 * 1024)
@@ -790,13 +930,18 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
     TF* d_rw = nullptr;
     FInst<TF>* d_inst = nullptr;
     WorkItem* d_work = nullptr;
-    size_t lds_gform = 0, lds_qform = 0;
-    int max_kx = 0, max_nq = 0;
+    size_t lds_gform = 0, lds_qform = 0, lds_rinv = 0;
+    int max_kx = 0, max_nq = 0, max_rcols = 0;
     for (auto* F : out) {
         lds_gform = std::max(lds_gform, F->lds_gform);
         lds_qform = std::max(lds_qform, F->lds_qform);
         max_kx = std::max(max_kx, F->max_kx);
         max_nq = std::max(max_nq, F->max_nq);
+        const FNode& r = F->hfn[0];
+        if (r.Dinv >= 0) {
+            max_rcols = std::max(max_rcols, std::max(r.m, r.Kx));
+            lds_rinv = std::max(lds_rinv, sizeof(TF) * (size_t)(r.m + NB) * RHS_TILE);
+        }
     }
     // work lists across the batch
     std::vector<WorkItem> work;
@@ -852,6 +997,10 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
             ProfScope ps("k_ulv_root", 0, roots.count, st);
             HIP_TRY((launch_ulv_root<TF>(d_inst, d_work + roots.off, roots.count, st)));
         }
+        {
+            ProfScope ps("k_ulv_rinv", 0, roots.count, st);
+            HIP_TRY((launch_ulv_rinv<TF>(d_inst, d_work + roots.off, roots.count, max_rcols, lds_rinv, st)));
+        }
         // solve operators G and Qt of every non-root node (one launch each)
         {
             ProfScope ps("k_ulv_gform", 0, wy_count, st);
@@ -885,7 +1034,7 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
     for (int b = 0; b < cnt; ++b) {
         hss_fact_s* F = out[b];
         TF* res = nullptr;
-        hipError_t e2 = dalloc(&res, (size_t)std::max<int64_t>(F->res_len, 1));
+        hipError_t e2 = dalloc(&res, (size_t)F->res_len + 16);
         if (e2 != hipSuccess) {
             cleanup();
             return fail(HSS_ERR_OOM, std::string("factor allocation: ") + hipGetErrorString(e2));
@@ -997,8 +1146,11 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
     S.work.clear();
     S.up.assign(maxh, Range{});
     S.up_m.assign(maxh, 0);
+    S.up_lds.assign(maxh, 0);
     S.down.assign(maxd + 1, Range{});
     S.down_m.assign(maxd + 1, 0);
+    S.down_lds.assign(maxd + 1, 0);
+    auto r16 = [](int64_t x) { return (x + 15) & ~int64_t(15); };
     for (int h = 0; h < maxh; ++h) {
         S.up[h].off = (int)S.work.size();
         for (int b = 0; b < cnt; ++b) {
@@ -1008,7 +1160,11 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
                 for (int t = 0; t < r.count; ++t) {
                     const int node = M->hwork[r.off + t].node;
                     S.work.push_back(WorkItem{b, node});
-                    S.up_m[h] = std::max(S.up_m[h], facts[b]->hfn[node].nk);
+                    const FNode& f = facts[b]->hfn[node];
+                    const size_t w = facts[b]->prec == HSS_PREC_FP32 ? 4 : 8;
+                    const int64_t na = (int64_t)f.nk * f.Kx + (int64_t)(f.My - f.nk) * (f.Kx + f.nk);
+                    S.up_m[h] = std::max(S.up_m[h], f.nk);
+                    S.up_lds[h] = std::max(S.up_lds[h], w * (size_t)(r16(na) + (int64_t)(f.Kx + f.nk) * 33));
                 }
             }
         }
@@ -1025,7 +1181,11 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
                 for (int t = 0; t < r.count; ++t) {
                     const int node = M->hwork[r.off + t].node;
                     S.work.push_back(WorkItem{b, node});
-                    S.down_m[d] = std::max(S.down_m[d], facts[b]->hfn[node].n);
+                    const FNode& f = facts[b]->hfn[node];
+                    const size_t w = facts[b]->prec == HSS_PREC_FP32 ? 4 : 8;
+                    const int64_t na = f.Qt >= 0 ? (int64_t)f.n * f.n : 0;
+                    S.down_m[d] = std::max(S.down_m[d], f.n);
+                    S.down_lds[d] = std::max(S.down_lds[d], w * (size_t)(r16(na) + (int64_t)f.n * 33));
                 }
             }
         }
@@ -1033,20 +1193,40 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
     }
 }
 
+// lgroot_res = LDS of the root kernel with the root factor staged (used when it fits)
+// lgroot2 = LDS of the inverse-and-refine root kernel (0 when some root lacks it)
 template <typename TF>
 static hss_status launch_schedule(const SInst<TF>* di, const WorkItem* dw, const Schedule& S, int maxnrhs,
-                                  size_t lgroot, hipStream_t st) {
+                                  size_t lgroot, size_t lgroot_res, size_t lgroot2, hipStream_t st) {
+    // operators staged whole in LDS when they fit, else streamed through the chunked kernels
     for (size_t h = 0; h < S.up.size(); ++h) {
-        ProfScope ps("k_ulv_gup", (int)h, S.up[h].count, st);
-        HIP_TRY((launch_ulv_gup<TF>(di, dw + S.up[h].off, S.up[h].count, S.up_m[h], maxnrhs, st)));
+        if (S.up_lds[h] <= kLdsResidentMax) {
+            ProfScope ps("k_ulv_gup_s", (int)h, S.up[h].count, st);
+            HIP_TRY((launch_ulv_gup_s<TF>(di, dw + S.up[h].off, S.up[h].count, maxnrhs, S.up_lds[h], st)));
+        } else {
+            ProfScope ps("k_ulv_gup", (int)h, S.up[h].count, st);
+            HIP_TRY((launch_ulv_gup<TF>(di, dw + S.up[h].off, S.up[h].count, S.up_m[h], maxnrhs, st)));
+        }
     }
     {
-        ProfScope ps("k_ulv_groot", 0, S.roots.count, st);
-        HIP_TRY((launch_ulv_groot<TF>(di, dw + S.roots.off, S.roots.count, maxnrhs, lgroot, st)));
+        if (lgroot2 > 0 && lgroot2 <= kLdsResidentMax) {
+            ProfScope ps("k_ulv_groot2", 0, S.roots.count, st);
+            HIP_TRY((launch_ulv_groot2<TF>(di, dw + S.roots.off, S.roots.count, maxnrhs, lgroot2, st)));
+        } else {
+            ProfScope ps("k_ulv_groot", 0, S.roots.count, st);
+            const bool res = lgroot_res <= kLdsResidentMax;
+            HIP_TRY((launch_ulv_groot<TF>(di, dw + S.roots.off, S.roots.count, maxnrhs, res ? lgroot_res : lgroot,
+                                          res ? 1 : 0, st)));
+        }
     }
     for (size_t d = 1; d < S.down.size(); ++d) {
-        ProfScope ps("k_ulv_gdown", (int)d, S.down[d].count, st);
-        HIP_TRY((launch_ulv_gdown<TF>(di, dw + S.down[d].off, S.down[d].count, S.down_m[d], maxnrhs, st)));
+        if (S.down_lds[d] <= kLdsResidentMax) {
+            ProfScope ps("k_ulv_gdown_s", (int)d, S.down[d].count, st);
+            HIP_TRY((launch_ulv_gdown_s<TF>(di, dw + S.down[d].off, S.down[d].count, maxnrhs, S.down_lds[d], st)));
+        } else {
+            ProfScope ps("k_ulv_gdown", (int)d, S.down[d].count, st);
+            HIP_TRY((launch_ulv_gdown<TF>(di, dw + S.down[d].off, S.down[d].count, S.down_m[d], maxnrhs, st)));
+        }
     }
     return HSS_OK;
 }
@@ -1070,7 +1250,8 @@ static hss_status solve_one(hss_fact_s* F, int nrhs, const double* dB, int64_t l
     SInst<TF> I = make_sinst<TF>(SolveReq{F, nrhs, dB, ldb, dX, ldx});
     const void* dv = nullptr;
     HIP_TRY(F->sinst.get(&I, sizeof(I), st, &dv));
-    return launch_schedule<TF>(reinterpret_cast<const SInst<TF>*>(dv), F->d_swork, F->sched, nrhs, F->lds_groot, st);
+    return launch_schedule<TF>(reinterpret_cast<const SInst<TF>*>(dv), F->d_swork, F->sched, nrhs, F->lds_groot,
+                               F->lds_groot_res, F->lds_groot2, st);
 }
 
 static hss_status ensure_refine(hss_fact_s* F, int nrhs) {
@@ -1255,9 +1436,9 @@ struct hss_plan_s {
     std::vector<BatchVec> h_vec;
     WorkItem* d_work = nullptr;
     Schedule S;
-    std::vector<Range> mv_up, mv_down;
-    Range mv_leaf;
-    size_t lgroot = 0, lmv_up = 0, lmv_down = 0, lmv_leaf = 0;
+    MvSched mvs;
+    size_t lgroot = 0, lgroot_res = 0, lgroot2 = 0;
+    bool all_groot2 = true;
     double* d_norms = nullptr;
     std::vector<double> h_norms;
     hipGraph_t graph = nullptr;
@@ -1335,7 +1516,7 @@ static hss_status plan_tables(hss_plan_s* P) {
             MvInst& I = P->h_mv[i];
             std::memset(&I, 0, sizeof(I));
             I.nodes = M->d_nodes;
-            I.blob = M->d_blob;
+            I.op = M->d_op;
             I.B = q.dX;
             I.ldb = q.ldx;
             I.C = reinterpret_cast<double*>(P->arena + off_r[i]);
@@ -1393,9 +1574,9 @@ static hss_status plan_create_impl(int32_t count, hss_fact_s* const* facts, cons
         P->reqs.push_back(SolveReq{F, nrhs[i], dB[i], ldb[i], dX[i], ldx[i]});
         P->maxnrhs = std::max(P->maxnrhs, (int)nrhs[i]);
         P->lgroot = std::max(P->lgroot, F->lds_groot);
-        P->lmv_up = std::max(P->lmv_up, F->mat->lds_up);
-        P->lmv_down = std::max(P->lmv_down, F->mat->lds_down);
-        P->lmv_leaf = std::max(P->lmv_leaf, F->mat->lds_leaf);
+        P->lgroot_res = std::max(P->lgroot_res, F->lds_groot_res);
+        P->lgroot2 = std::max(P->lgroot2, F->lds_groot2);
+        P->all_groot2 = P->all_groot2 && F->lds_groot2 > 0;
     }
     P->cnt = (int)P->reqs.size();
     std::vector<hss_mat_s*> mats;
@@ -1407,44 +1588,7 @@ static hss_status plan_create_impl(int32_t count, hss_fact_s* const* facts, cons
     build_schedule(pf, P->S);
     // matvec schedule appended after the solve schedule
     std::vector<WorkItem> work = P->S.work;
-    if (P->has_ir) {
-        int maxh = 0, maxd = 0;
-        for (auto* M : mats) {
-            maxh = std::max(maxh, M->max_height);
-            maxd = std::max(maxd, M->max_depth);
-        }
-        P->mv_up.assign(maxh, Range{});
-        for (int h = 0; h < maxh; ++h) {
-            P->mv_up[h].off = (int)work.size();
-            for (int b = 0; b < P->cnt; ++b) {
-                hss_mat_s* M = mats[b];
-                if (h < M->max_height) {
-                    const Range& r = M->by_height[h];
-                    for (int t = 0; t < r.count; ++t) work.push_back(WorkItem{b, M->hwork[r.off + t].node});
-                }
-            }
-            P->mv_up[h].count = (int)work.size() - P->mv_up[h].off;
-        }
-        P->mv_down.assign(maxd + 1, Range{});
-        for (int d = 0; d <= maxd; ++d) {
-            P->mv_down[d].off = (int)work.size();
-            for (int b = 0; b < P->cnt; ++b) {
-                hss_mat_s* M = mats[b];
-                if (d <= M->max_depth) {
-                    const Range& r = M->br_depth[d];
-                    for (int t = 0; t < r.count; ++t) work.push_back(WorkItem{b, M->hwork[r.off + t].node});
-                }
-            }
-            P->mv_down[d].count = (int)work.size() - P->mv_down[d].off;
-        }
-        P->mv_leaf.off = (int)work.size();
-        for (int b = 0; b < P->cnt; ++b) {
-            hss_mat_s* M = mats[b];
-            const Range& r = M->leaves;
-            for (int t = 0; t < r.count; ++t) work.push_back(WorkItem{b, M->hwork[r.off + t].node});
-        }
-        P->mv_leaf.count = (int)work.size() - P->mv_leaf.off;
-    }
+    if (P->has_ir) build_mv_schedule(mats, work, P->mvs);
     hipError_t e = dalloc(&P->d_work, work.size());
     if (e == hipSuccess) e = hipMemcpy(P->d_work, work.data(), sizeof(WorkItem) * work.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -1474,23 +1618,11 @@ extern "C" hss_status hss_plan_create(int32_t count, const hss_fact_t* facts, co
 template <typename TF>
 static hss_status plan_solve(hss_plan_s* P, int t) {
     return launch_schedule<TF>(reinterpret_cast<const SInst<TF>*>(P->d_sinst[t]), P->d_work, P->S, P->maxnrhs,
-                               P->lgroot, P->ctx->stream);
+                               P->lgroot, P->lgroot_res, P->all_groot2 ? P->lgroot2 : 0, P->ctx->stream);
 }
 
 static hss_status plan_matvec(hss_plan_s* P) {
-    hipStream_t st = P->ctx->stream;
-    for (size_t h = 0; h < P->mv_up.size(); ++h) {
-        ProfScope ps("k_mv_up", (int)h, P->mv_up[h].count, st);
-        HIP_TRY(launch_mv_up(P->d_mv, P->d_work + P->mv_up[h].off, P->mv_up[h].count, P->maxnrhs, P->lmv_up, st));
-    }
-    for (size_t d = 0; d < P->mv_down.size(); ++d) {
-        ProfScope ps("k_mv_down", (int)d, P->mv_down[d].count, st);
-        HIP_TRY(launch_mv_down(P->d_mv, P->d_work + P->mv_down[d].off, P->mv_down[d].count, P->maxnrhs, P->lmv_down,
-                               st));
-    }
-    ProfScope ps("k_mv_leaf", 0, P->mv_leaf.count, st);
-    HIP_TRY(launch_mv_leaf(P->d_mv, P->d_work + P->mv_leaf.off, P->mv_leaf.count, P->maxnrhs, P->lmv_leaf, st));
-    return HSS_OK;
+    return launch_mv_schedule(P->d_mv, P->d_work, P->mvs, P->maxnrhs, P->ctx->stream);
 }
 
 // re-upload the tables whose nrhs changed (the stream is idle: callers synchronize first)
@@ -1811,7 +1943,8 @@ static hss_status server_round(hss_server_s* S) {
     // 2. tables: solve (first solve b->x, or correction r->d), axpy, matvec x->r, residual
     size_t ws = 0;
     int maxnrhs = 0;
-    size_t lgroot = 0, lup = 0, ldown = 0, lleaf = 0;
+    size_t lgroot = 0, lgroot_res = 0, lgroot2 = 0;
+    bool all_groot2 = true;
     std::vector<size_t> off_s(cnt), off_m(cnt);
     std::vector<hss_mat_s*> mats(cnt);
     for (int i = 0; i < cnt; ++i) {
@@ -1825,9 +1958,9 @@ static hss_status server_round(hss_server_s* S) {
         ws = align256(ws + sizeof(double) * (size_t)(M->zrows + M->frows) * q->nrhs);
         maxnrhs = std::max(maxnrhs, q->nrhs);
         lgroot = std::max(lgroot, F->lds_groot);
-        lup = std::max(lup, M->lds_up);
-        ldown = std::max(ldown, M->lds_down);
-        lleaf = std::max(lleaf, M->lds_leaf);
+        lgroot_res = std::max(lgroot_res, F->lds_groot_res);
+        lgroot2 = std::max(lgroot2, F->lds_groot2);
+        all_groot2 = all_groot2 && F->lds_groot2 > 0;
     }
     hss_status s = grow(&S->d_round, &S->round_cap, ws);
     if (s != HSS_OK) return s;
@@ -1846,7 +1979,7 @@ static hss_status server_round(hss_server_s* S) {
         MvInst& I = mv[i];
         std::memset(&I, 0, sizeof(I));
         I.nodes = M->d_nodes;
-        I.blob = M->d_blob;
+        I.op = M->d_op;
         I.B = q->dX;
         I.ldb = q->ldx;
         I.C = q->r;
@@ -1872,36 +2005,8 @@ static hss_status server_round(hss_server_s* S) {
     for (int i = 0; i < cnt; ++i) rf[i] = S->facts[S->inflight[i]->mid];
     build_schedule(rf, sch);
     std::vector<WorkItem> work = sch.work;
-    int maxh = 0, maxd = 0;
-    for (auto* M : mats) {
-        maxh = std::max(maxh, M->max_height);
-        maxd = std::max(maxd, M->max_depth);
-    }
-    std::vector<Range> mup(maxh), mdown(maxd + 1);
-    for (int h = 0; h < maxh; ++h) {
-        mup[h].off = (int)work.size();
-        for (int b = 0; b < cnt; ++b)
-            if (h < mats[b]->max_height) {
-                const Range& r = mats[b]->by_height[h];
-                for (int t = 0; t < r.count; ++t) work.push_back(WorkItem{b, mats[b]->hwork[r.off + t].node});
-            }
-        mup[h].count = (int)work.size() - mup[h].off;
-    }
-    for (int d = 0; d <= maxd; ++d) {
-        mdown[d].off = (int)work.size();
-        for (int b = 0; b < cnt; ++b)
-            if (d <= mats[b]->max_depth) {
-                const Range& r = mats[b]->br_depth[d];
-                for (int t = 0; t < r.count; ++t) work.push_back(WorkItem{b, mats[b]->hwork[r.off + t].node});
-            }
-        mdown[d].count = (int)work.size() - mdown[d].off;
-    }
-    Range mleaf{(int)work.size(), 0};
-    for (int b = 0; b < cnt; ++b) {
-        const Range& r = mats[b]->leaves;
-        for (int t = 0; t < r.count; ++t) work.push_back(WorkItem{b, mats[b]->hwork[r.off + t].node});
-    }
-    mleaf.count = (int)work.size() - mleaf.off;
+    MvSched mvs;
+    build_mv_schedule(mats, work, mvs);
     // one table upload per round (the stream is idle here)
     const size_t o_si = 0, o_mv = align256(sizeof(SInst<TF>) * cnt), o_bv = o_mv + align256(sizeof(MvInst) * cnt),
                  o_w = o_bv + align256(sizeof(BatchVec) * cnt), tab = o_w + sizeof(WorkItem) * work.size();
@@ -1921,12 +2026,11 @@ static hss_status server_round(hss_server_s* S) {
     auto* dbv = reinterpret_cast<BatchVec*>(S->d_tab + o_bv);
     auto* dw = reinterpret_cast<const WorkItem*>(S->d_tab + o_w);
     // 3. one batched solve, the corrections, one batched matvec + residual
-    s = launch_schedule<TF>(dsi, dw, sch, maxnrhs, lgroot, st);
+    s = launch_schedule<TF>(dsi, dw, sch, maxnrhs, lgroot, lgroot_res, all_groot2 ? lgroot2 : 0, st);
     if (s != HSS_OK) return s;
     HIP_TRY(launch_axpy_batch(dbv, cnt, maxnrhs, st));
-    for (int hh = 0; hh < maxh; ++hh) HIP_TRY(launch_mv_up(dmv, dw + mup[hh].off, mup[hh].count, maxnrhs, lup, st));
-    for (int d = 0; d <= maxd; ++d) HIP_TRY(launch_mv_down(dmv, dw + mdown[d].off, mdown[d].count, maxnrhs, ldown, st));
-    HIP_TRY(launch_mv_leaf(dmv, dw + mleaf.off, mleaf.count, maxnrhs, lleaf, st));
+    s = launch_mv_schedule(dmv, dw, mvs, maxnrhs, st);
+    if (s != HSS_OK) return s;
     // the residual pass needs every request active: rewrite nrhs of the BatchVec rows
     for (int i = 0; i < cnt; ++i) bv[i].nrhs = S->inflight[i]->nrhs;
     HIP_TRY(hipMemcpyAsync(S->d_tab + o_bv, bv.data(), sizeof(BatchVec) * cnt, hipMemcpyHostToDevice, st));
