@@ -138,8 +138,12 @@ def solve_kernel_model(nodes, fact, kernel, level, nrhs, wf, wv):
 # same kernels are pinned by the GPU parity tests on compressed kernel matrices.
 # ---------------------------------------------------------------------------
 def _dev_rhs(torch, rng, n, r, dev):
+    """B (r x n torch tensor = column-major n x r) and a zero X; synchronized, because torch
+    fills on its own stream and the library works on its context stream."""
     B = torch.from_numpy(np.ascontiguousarray(rng.standard_normal((r, n)))).to(dev)
-    return B, torch.zeros_like(B)
+    X = torch.zeros_like(B)
+    torch.cuda.synchronize()
+    return B, X
 
 
 def batched(pkg, hip, torch, steps=10, warmup=2, count=256, seed=3, log=print):
@@ -227,7 +231,9 @@ def _batch_residual(mats, Bs, Xs, rs, torch):
     worst = 0.0
     for A, B, X, r in zip(mats, Bs, Xs, rs):
         R = B.clone()
+        torch.cuda.synchronize()  # torch's stream and the library's stream are distinct
         A.matmul_dev(int(r), 1.0, X.data_ptr(), X.shape[1], -1.0, R.data_ptr(), R.shape[1])
+        torch.cuda.synchronize()
         rel = (torch.linalg.vector_norm(R, dim=1) / torch.linalg.vector_norm(B, dim=1)).max().item()
         worst = max(worst, rel)
     return worst
@@ -401,6 +407,7 @@ def main():
     Xt = torch.zeros_like(Bt)
     X2t = torch.zeros_like(Bt)
     Ct = torch.zeros_like(Bt)
+    torch.cuda.synchronize()  # torch's fills run on its own stream, the library on its context stream
     ld = n
 
     def solve64():

@@ -164,6 +164,41 @@ __device__ __forceinline__ void lds_copy(TF* dst, const TF* __restrict__ src, in
     batched<16, float4>(nv, [&](int v) { return s[v]; }, [&](int v, float4 x) { d[v] = x; });
 }
 
+// One round of loads for an LDS-resident step: the operator (dst[0, na) = src[0, na), as
+// lds_copy) and the input gather (st(e, ld(e)) for e < nx), all issued before any LDS store.
+template <typename TF, typename Ld, typename St>
+__device__ __forceinline__ void stage2(TF* dst, const TF* __restrict__ src, int64_t na, int nx, Ld ld, St st) {
+    constexpr int VA = 16, VX = 8;
+    const float4* s = reinterpret_cast<const float4*>(src);
+    float4* d = reinterpret_cast<float4*>(dst);
+    const int nv = (int)((na * (int64_t)sizeof(TF) + 15) / 16);
+    const int bd = blockDim.x;
+    for (int v0 = threadIdx.x, e0 = threadIdx.x; v0 < nv || e0 < nx; v0 += VA * bd, e0 += VX * bd) {
+        float4 ra[VA];
+        TF rx[VX];
+#pragma unroll
+        for (int q = 0; q < VA; ++q) {
+            const int v = v0 + q * bd;
+            ra[q] = v < nv ? s[v] : float4{};
+        }
+#pragma unroll
+        for (int q = 0; q < VX; ++q) {
+            const int e = e0 + q * bd;
+            rx[q] = e < nx ? ld(e) : TF{};
+        }
+#pragma unroll
+        for (int q = 0; q < VA; ++q) {
+            const int v = v0 + q * bd;
+            if (v < nv) d[v] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < VX; ++q) {
+            const int e = e0 + q * bd;
+            if (e < nx) st(e, rx[q]);
+        }
+    }
+}
+
 // LDS row-major -> global column-major
 template <typename T, typename Tdst>
 __device__ __forceinline__ void unstage(Tdst* dst, int64_t ld, const T* src, int r0, int rows, int nt) {
@@ -1434,41 +1469,44 @@ __device__ __forceinline__ void glb_gemm(int M, int K, const TF* __restrict__ A,
     }
 }
 
-// Up-sweep, LDS-resident: LDS = [Gz Gr] (nk Kx + R (Kx + nk), rounded to 16) | xs (Kx + nk) x XS
+// Up-sweep, LDS-resident: LDS = [Gz Gr] (nk Kx + R (Kx + nk), rounded to 16) | xs (Kx + nk) x XS.
+// tw = columns per workgroup (16 or 32).
 template <typename TF>
 __global__ __launch_bounds__(512) void k_ulv_gup_s(const SInst<TF>* __restrict__ insts,
-                                                   const WorkItem* __restrict__ work, int nwork, int tn) {
+                                                   const WorkItem* __restrict__ work, int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
     const WorkItem wi = work[item];
     const SInst<TF>& I = insts[wi.inst];
     const FNode f = I.fn[wi.node];
     const int nrhs = I.nrhs;
-    const int j0 = tj * 32;
+    const int j0 = tj * tw;
     if (j0 >= nrhs) return;
-    const int nc = min(32, nrhs - j0);
+    const int nc = min(tw, nrhs - j0);
     const DevNode nd = I.nodes[wi.node];
     const int nk = f.nk, Kx = f.Kx, My = f.My, R = My - nk;
     const int64_t na = (int64_t)nk * Kx + (int64_t)R * (Kx + nk);
     extern __shared__ unsigned char lds_raw[];
     TF* As = reinterpret_cast<TF*>(lds_raw);
     TF* xs = As + ((na + 15) & ~int64_t(15));
-    lds_copy(As, I.fac + (nk > 0 ? f.Gz : f.Gr), na);
-    // x rows [0, Kx): the leaf's rows of B, or the children's reduced b and u
+    // operator and x rows [0, Kx) (the leaf's rows of B, or the children's reduced b and u)
+    // in one round of loads; columns past nc are zero
+    const TF* Ag = I.fac + (nk > 0 ? f.Gz : f.Gr);
     auto put = [&](int e, TF v) { xs[(e % Kx) * XS + e / Kx] = v; };
     if (nd.is_leaf) {
         const double* B = I.B + nd.row_off + (int64_t)j0 * I.ldb;
         const int64_t ldb = I.ldb;
-        batched<16, TF>(
-            Kx * 32, [&](int e) { return e / Kx < nc ? (TF)B[(e % Kx) + (int64_t)(e / Kx) * ldb] : TF(0); }, put);
+        stage2(
+            As, Ag, na, Kx * tw, [&](int e) { return e / Kx < nc ? (TF)B[(e % Kx) + (int64_t)(e / Kx) * ldb] : TF(0); },
+            put);
     } else {
         const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
         const int r0 = fl.My - fl.nk;
         const TF* y0 = I.wy + fl.yrow * nrhs + fl.nk + (int64_t)j0 * fl.My;
         const TF* y1 = I.wy + fr.yrow * nrhs + fr.nk + (int64_t)j0 * fr.My;
         const int l0 = fl.My, l1 = fr.My;
-        batched<16, TF>(
-            Kx * 32,
+        stage2(
+            As, Ag, na, Kx * tw,
             [&](int e) {
                 const int k = e % Kx, j = e / Kx;
                 if (j >= nc) return TF(0);
@@ -1494,16 +1532,16 @@ __global__ __launch_bounds__(512) void k_ulv_gup_s(const SInst<TF>* __restrict__
 // node) | xs n x XS
 template <typename TF>
 __global__ __launch_bounds__(512) void k_ulv_gdown_s(const SInst<TF>* __restrict__ insts,
-                                                     const WorkItem* __restrict__ work, int nwork, int tn) {
+                                                     const WorkItem* __restrict__ work, int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
     const WorkItem wi = work[item];
     const SInst<TF>& I = insts[wi.inst];
     const FNode f = I.fn[wi.node];
     const int nrhs = I.nrhs, n = f.n, nk = f.nk;
-    const int j0 = tj * 32;
+    const int j0 = tj * tw;
     if (j0 >= nrhs) return;
-    const int nc = min(32, nrhs - j0);
+    const int nc = min(tw, nrhs - j0);
     const DevNode nd = I.nodes[wi.node];
     const FNode fp = I.fn[nd.parent];
     const int off = nd.is_left ? 0 : I.fn[I.nodes[nd.parent].left].nr;
@@ -1511,13 +1549,12 @@ __global__ __launch_bounds__(512) void k_ulv_gdown_s(const SInst<TF>* __restrict
     extern __shared__ unsigned char lds_raw[];
     TF* As = reinterpret_cast<TF*>(lds_raw);
     TF* xs = As + ((na + 15) & ~int64_t(15));
-    if (na > 0) lds_copy(As, I.fac + f.Qt, na);
-    // x = [zloc (rows 0..nk-1 of the node's y); z from the parent]
+    // Qt and x = [zloc (rows 0..nk-1 of the node's y); z from the parent] in one round
     const TF* y0 = I.wy + f.yrow * nrhs + (int64_t)j0 * f.My;
     const TF* z1 = I.wz + fp.zrow * nrhs + off + (int64_t)j0 * fp.n;
     const int l0 = f.My, l1 = fp.n;
-    batched<16, TF>(
-        n * 32,
+    stage2(
+        As, na > 0 ? I.fac + f.Qt : I.fac, na, n * tw,
         [&](int e) {
             const int k = e % n, j = e / n;
             if (j >= nc) return TF(0);
@@ -1666,14 +1703,14 @@ __device__ __forceinline__ void mv_store(const MvInst& I, const MvOp& o, int i, 
 // LDS-resident: LDS = A (M x K, rounded to 16) | xs K x XS
 template <int KIND>
 __global__ __launch_bounds__(512) void k_mv_s(const MvInst* __restrict__ insts, const WorkItem* __restrict__ work,
-                                              int nwork, int tn) {
+                                              int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
     const WorkItem wi = work[item];
     const MvInst& I = insts[wi.inst];
-    const int j0 = tj * 32;
+    const int j0 = tj * tw;
     if (j0 >= I.nrhs) return;
-    const int nc = min(32, I.nrhs - j0);
+    const int nc = min(tw, I.nrhs - j0);
     MvOp o;
     mv_op<KIND>(I, wi.node, o);
     const int M = o.M, K = o.K;
@@ -1681,9 +1718,9 @@ __global__ __launch_bounds__(512) void k_mv_s(const MvInst* __restrict__ insts, 
     extern __shared__ unsigned char lds_raw[];
     double* As = reinterpret_cast<double*>(lds_raw);
     double* xs = As + ((na + 15) & ~int64_t(15));
-    lds_copy(As, o.A, na);
-    batched<16, double>(
-        K * 32, [&](int e) { return e / K < nc ? seg_load<double>(o.x, nullptr, 0, e % K, j0 + e / K, 0) : 0.0; },
+    stage2(
+        As, o.A, na, K * tw,
+        [&](int e) { return e / K < nc ? seg_load<double>(o.x, nullptr, 0, e % K, j0 + e / K, 0) : 0.0; },
         [&](int e, double v) { xs[(e % K) * XS + e / K] = v; });
     __syncthreads();
     lds_gemm(M, K, As, xs, nc, [&](int i, int j, double v) { mv_store<KIND>(I, o, i, j0 + j, v); });
@@ -1909,32 +1946,40 @@ hipError_t launch_ulv_gup(const SInst<T>* insts, const WorkItem* work, int nwork
                        tn);
     return hipGetLastError();
 }
+// Column width of the LDS-resident steps: 16 when a level has too few 32-column tiles to
+// occupy the CUs (halves each workgroup's MFMA chain), else 32.
+static inline int resident_tw(int nwork, int max_nrhs) {
+    return (max_nrhs > 16 && (int64_t)nwork * ((max_nrhs + 31) / 32) < 128) ? 16 : 32;
+}
+
 template <typename T>
 hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
                             hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
-    const int tn = (max_nrhs + 31) / 32;
+    const int tw = resident_tw(nwork, max_nrhs);
+    const int tn = (max_nrhs + tw - 1) / tw;
     const int64_t total = (int64_t)nwork * tn;
     if (total > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gup_s<T>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_ulv_gup_s<T>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts, work,
-                       nwork, tn);
+                       nwork, tn, tw);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
                               hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
-    const int tn = (max_nrhs + 31) / 32;
+    const int tw = resident_tw(nwork, max_nrhs);
+    const int tn = (max_nrhs + tw - 1) / tw;
     const int64_t total = (int64_t)nwork * tn;
     if (total > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gdown_s<T>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_ulv_gdown_s<T>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts, work,
-                       nwork, tn);
+                       nwork, tn, tw);
     return hipGetLastError();
 }
 template <typename T>
@@ -1992,13 +2037,14 @@ static hipError_t launch_mv_kind(const MvInst* insts, const WorkItem* work, int 
     if (nwork <= 0 || max_nrhs <= 0 || max_m <= 0) return hipSuccess;
     if (lds > 0) {
         if (lds > 160 * 1024) return hipErrorInvalidValue;
-        const int tn = (max_nrhs + 31) / 32;
+        const int tw = resident_tw(nwork, max_nrhs);
+        const int tn = (max_nrhs + tw - 1) / tw;
         const int64_t total = (int64_t)nwork * tn;
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mv_s<KIND>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((k_mv_s<KIND>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts, work,
-                           nwork, tn);
+                           nwork, tn, tw);
         return hipGetLastError();
     }
     dim3 grid;

@@ -149,6 +149,12 @@ struct Schedule {
 };
 // Largest LDS of a workgroup (bytes); levels above it use the streamed kernels.
 constexpr size_t kLdsResidentMax = 160 * 1024;
+// A level runs LDS-resident when its staging fits and either leaves room for two
+// workgroups per CU or the level has at most one workgroup per CU; a wide level of
+// large operators keeps more bytes in flight through the chunked kernels.
+static bool use_resident(size_t lds, int64_t workgroups) {
+    return lds > 0 && lds <= kLdsResidentMax && (lds <= 80 * 1024 || workgroups <= 256);
+}
 
 // Matvec schedule: up-sweep steps by height, down-sweep steps by depth (1..), leaves;
 // per step the largest operator height and the LDS of the resident kernel (0 = chunked).
@@ -413,16 +419,21 @@ static void build_mv_schedule(const std::vector<hss_mat_s*>& mats, std::vector<W
 
 static hss_status launch_mv_schedule(const MvInst* di, const WorkItem* dw, const MvSched& S, int maxnrhs,
                                      hipStream_t st) {
+    const int64_t ntiles = (maxnrhs + 31) / 32;
+    auto pick = [&](size_t lds, int count) { return use_resident(lds, count * ntiles) ? lds : (size_t)0; };
     for (size_t h = 0; h < S.up.size(); ++h) {
-        ProfScope ps(S.up_lds[h] ? "k_mv_up_s" : "k_mv_up", (int)h, S.up[h].count, st);
-        HIP_TRY(launch_mv(0, di, dw + S.up[h].off, S.up[h].count, S.up_m[h], maxnrhs, S.up_lds[h], st));
+        const size_t l = pick(S.up_lds[h], S.up[h].count);
+        ProfScope ps(l ? "k_mv_up_s" : "k_mv_up", (int)h, S.up[h].count, st);
+        HIP_TRY(launch_mv(0, di, dw + S.up[h].off, S.up[h].count, S.up_m[h], maxnrhs, l, st));
     }
     for (size_t d = 1; d < S.down.size(); ++d) {
-        ProfScope ps(S.down_lds[d] ? "k_mv_down_s" : "k_mv_down", (int)d, S.down[d].count, st);
-        HIP_TRY(launch_mv(1, di, dw + S.down[d].off, S.down[d].count, S.down_m[d], maxnrhs, S.down_lds[d], st));
+        const size_t l = pick(S.down_lds[d], S.down[d].count);
+        ProfScope ps(l ? "k_mv_down_s" : "k_mv_down", (int)d, S.down[d].count, st);
+        HIP_TRY(launch_mv(1, di, dw + S.down[d].off, S.down[d].count, S.down_m[d], maxnrhs, l, st));
     }
-    ProfScope ps(S.leaf_lds ? "k_mv_leaf_s" : "k_mv_leaf", 0, S.leaf.count, st);
-    HIP_TRY(launch_mv(2, di, dw + S.leaf.off, S.leaf.count, S.leaf_m, maxnrhs, S.leaf_lds, st));
+    const size_t l = pick(S.leaf_lds, S.leaf.count);
+    ProfScope ps(l ? "k_mv_leaf_s" : "k_mv_leaf", 0, S.leaf.count, st);
+    HIP_TRY(launch_mv(2, di, dw + S.leaf.off, S.leaf.count, S.leaf_m, maxnrhs, l, st));
     return HSS_OK;
 }
 
@@ -1198,9 +1209,10 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
 template <typename TF>
 static hss_status launch_schedule(const SInst<TF>* di, const WorkItem* dw, const Schedule& S, int maxnrhs,
                                   size_t lgroot, size_t lgroot_res, size_t lgroot2, hipStream_t st) {
-    // operators staged whole in LDS when they fit, else streamed through the chunked kernels
+    // operators staged whole in LDS (use_resident), else streamed through the chunked kernels
+    const int64_t ntiles = (maxnrhs + 31) / 32;
     for (size_t h = 0; h < S.up.size(); ++h) {
-        if (S.up_lds[h] <= kLdsResidentMax) {
+        if (use_resident(S.up_lds[h], S.up[h].count * ntiles)) {
             ProfScope ps("k_ulv_gup_s", (int)h, S.up[h].count, st);
             HIP_TRY((launch_ulv_gup_s<TF>(di, dw + S.up[h].off, S.up[h].count, maxnrhs, S.up_lds[h], st)));
         } else {
@@ -1220,7 +1232,7 @@ static hss_status launch_schedule(const SInst<TF>* di, const WorkItem* dw, const
         }
     }
     for (size_t d = 1; d < S.down.size(); ++d) {
-        if (S.down_lds[d] <= kLdsResidentMax) {
+        if (use_resident(S.down_lds[d], S.down[d].count * ntiles)) {
             ProfScope ps("k_ulv_gdown_s", (int)d, S.down[d].count, st);
             HIP_TRY((launch_ulv_gdown_s<TF>(di, dw + S.down[d].off, S.down[d].count, maxnrhs, S.down_lds[d], st)));
         } else {

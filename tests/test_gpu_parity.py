@@ -152,6 +152,7 @@ def test_batched_factor_and_solve_match_single(hip, oracle):
         Bs.append(torch.from_numpy(np.ascontiguousarray(B.T)).to(dev))
         Xs.append(torch.zeros((r, n), dtype=torch.float64, device=dev))
         nrhs.append(r)
+    torch.cuda.synchronize()  # torch fills on its own stream; the library uses its context stream
     hip.solve_batch_dev(facts, nrhs, [b.data_ptr() for b in Bs], [int(b.shape[1]) for b in Bs],
                         [x.data_ptr() for x in Xs], [int(x.shape[1]) for x in Xs])
     torch.cuda.synchronize()
@@ -175,6 +176,7 @@ def test_handles_destroyed_in_any_order(hip, oracle, c1):
     B = rng.standard_normal((2001, 3))
     Bt = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda:0")
     Xt = torch.zeros_like(Bt)
+    torch.cuda.synchronize()
     plan = hip.Plan([F], [3], [Bt.data_ptr()], [2001], [Xt.data_ptr()], [2001])
     ctx.close()
     A.close()
@@ -258,6 +260,7 @@ def test_plan_repeated_factors_and_graph_replay(hip, oracle):
         Bs.append(torch.from_numpy(np.ascontiguousarray(rng.standard_normal((r, n)))).to(dev))
         Xs.append(torch.zeros((r, n), dtype=torch.float64, device=dev))
         nrhs.append(r)
+    torch.cuda.synchronize()
     plan = hip.Plan([facts[i] for i in order], nrhs, [b.data_ptr() for b in Bs], [int(b.shape[1]) for b in Bs],
                     [x.data_ptr() for x in Xs], [int(x.shape[1]) for x in Xs])
     plan.execute()
@@ -292,6 +295,7 @@ def test_plan_mixed_precision_lockstep_refinement(hip, oracle):
         Bs.append(torch.from_numpy(np.ascontiguousarray(rng.standard_normal((r, n)))).to(dev))
         Xs.append(torch.zeros((r, n), dtype=torch.float64, device=dev))
         nrhs.append(r)
+    torch.cuda.synchronize()
     plan = hip.Plan(facts, nrhs, [b.data_ptr() for b in Bs], [int(b.shape[1]) for b in Bs],
                     [x.data_ptr() for x in Xs], [int(x.shape[1]) for x in Xs], refinement=True)
     rep = plan.execute(tol=1e-10, max_iter=20)
@@ -323,7 +327,9 @@ def test_server_alone_equals_mixed(hip, oracle):
     def make(i, r):
         n = int(packs[i][0]["n"][0])
         B = torch.from_numpy(np.ascontiguousarray(rng.standard_normal((r, n)))).to(dev)
-        return B, torch.zeros_like(B)
+        X = torch.zeros_like(B)
+        torch.cuda.synchronize()
+        return B, X
 
     # alone
     B0, X0 = make(0, 3)
@@ -335,6 +341,7 @@ def test_server_alone_equals_mixed(hip, oracle):
     alone = X0.cpu().numpy().copy()
     # the same request in a mix, arriving between rounds of other requests
     X0.zero_()
+    torch.cuda.synchronize()
     others = [make(i % 3, 1 + i) for i in range(6)]
     tickets = []
     for k, (B, X) in enumerate(others[:3]):
