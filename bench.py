@@ -194,9 +194,13 @@ def batched(pkg, hip, torch, steps=10, warmup=2, count=256, seed=3, log=print):
                 f.close()
             continue
         ms = (time.perf_counter() - t0) * 1e3 / steps
+        recs = profile_launches(hip, lambda: plan.execute(tol=tol), 1)
+        _, byk = summarize(recs, 1)
         out[prec] = {"batch_ms": round(ms, 3), "solves_per_s": round(count * 1e3 / ms, 1),
                      "factor_batch_ms": round(fms, 2),
-                     "worst_rel_residual_fp64_check": _batch_residual(mats, Bs, Xs, rs, torch)}
+                     "worst_rel_residual_fp64_check": _batch_residual(mats, Bs, Xs, rs, torch),
+                     "per_kernel_ms": {k: round(v, 4) for k, v in byk.items()}}
+        log(f"[bench] batch {prec} per kernel (ms per batch): {out[prec]['per_kernel_ms']}")
         if rep is not None and prec == "fp32":
             out[prec]["refine_steps"] = rep["iterations"]
             out[prec]["worst_rel_residual"] = rep["rel_residual"]

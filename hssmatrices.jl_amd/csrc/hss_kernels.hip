@@ -168,7 +168,7 @@ __device__ __forceinline__ void lds_copy(TF* dst, const TF* __restrict__ src, in
 // lds_copy) and the input gather (st(e, ld(e)) for e < nx), all issued before any LDS store.
 template <typename TF, typename Ld, typename St>
 __device__ __forceinline__ void stage2(TF* dst, const TF* __restrict__ src, int64_t na, int nx, Ld ld, St st) {
-    constexpr int VA = 16, VX = 8;
+    constexpr int VA = 8, VX = 8;
     const float4* s = reinterpret_cast<const float4*>(src);
     float4* d = reinterpret_cast<float4*>(dst);
     const int nv = (int)((na * (int64_t)sizeof(TF) + 15) / 16);
@@ -1393,7 +1393,8 @@ constexpr int XS = 33;  // LDS pitch of the staged input (32 columns + 1)
 // (16-row, 16-column) blocks of an M x 32 tile are dealt to the waves, each block
 // accumulated in two independent MFMA chains (even and odd k-steps).
 template <typename TF, typename Store>
-__device__ __forceinline__ void lds_gemm(int M, int K, const TF* As, const TF* xs, int ncols, Store store) {
+__device__ __forceinline__ void lds_gemm(int M, int K, const TF* As, const TF* xs, int ncols, Store store,
+                                         int xp = XS) {
     using V4 = typename Mfma<TF>::v4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int lr = lane & 15, lk = lane >> 4;
@@ -1402,7 +1403,7 @@ __device__ __forceinline__ void lds_gemm(int M, int K, const TF* As, const TF* x
         const int rb = t % nrb, cb = t / nrb;
         const int i = rb * 16 + lr, jc = cb * 16 + lr;
         auto av = [&](int kk) { return (i < M && kk < K) ? As[i + kk * M] : TF(0); };
-        auto xv = [&](int kk) { return kk < K ? xs[kk * XS + jc] : TF(0); };
+        auto xv = [&](int kk) { return kk < K ? xs[kk * xp + jc] : TF(0); };
         V4 acc0 = V4{TF(0), TF(0), TF(0), TF(0)}, acc1 = acc0;
         int k = 0;
         for (; k + 4 < K; k += 8) {
@@ -1472,7 +1473,7 @@ __device__ __forceinline__ void glb_gemm(int M, int K, const TF* __restrict__ A,
 // Up-sweep, LDS-resident: LDS = [Gz Gr] (nk Kx + R (Kx + nk), rounded to 16) | xs (Kx + nk) x XS.
 // tw = columns per workgroup (16 or 32).
 template <typename TF>
-__global__ __launch_bounds__(512) void k_ulv_gup_s(const SInst<TF>* __restrict__ insts,
+__global__ __launch_bounds__(512, 2) void k_ulv_gup_s(const SInst<TF>* __restrict__ insts,
                                                    const WorkItem* __restrict__ work, int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
@@ -1483,6 +1484,7 @@ __global__ __launch_bounds__(512) void k_ulv_gup_s(const SInst<TF>* __restrict__
     const int j0 = tj * tw;
     if (j0 >= nrhs) return;
     const int nc = min(tw, nrhs - j0);
+    const int xp = tw + 1;  // LDS pitch of xs
     const DevNode nd = I.nodes[wi.node];
     const int nk = f.nk, Kx = f.Kx, My = f.My, R = My - nk;
     const int64_t na = (int64_t)nk * Kx + (int64_t)R * (Kx + nk);
@@ -1492,7 +1494,7 @@ __global__ __launch_bounds__(512) void k_ulv_gup_s(const SInst<TF>* __restrict__
     // operator and x rows [0, Kx) (the leaf's rows of B, or the children's reduced b and u)
     // in one round of loads; columns past nc are zero
     const TF* Ag = I.fac + (nk > 0 ? f.Gz : f.Gr);
-    auto put = [&](int e, TF v) { xs[(e % Kx) * XS + e / Kx] = v; };
+    auto put = [&](int e, TF v) { xs[(e % Kx) * xp + e / Kx] = v; };
     if (nd.is_leaf) {
         const double* B = I.B + nd.row_off + (int64_t)j0 * I.ldb;
         const int64_t ldb = I.ldb;
@@ -1519,19 +1521,19 @@ __global__ __launch_bounds__(512) void k_ulv_gup_s(const SInst<TF>* __restrict__
     // zloc = Gz x  -> y rows [0, nk) and xs rows [Kx, Kx + nk)
     auto st0 = [&](int i, int j, TF v) {
         y[i + (int64_t)j * My] = v;
-        xs[(Kx + i) * XS + j] = v;
+        xs[(Kx + i) * xp + j] = v;
     };
-    lds_gemm(nk, Kx, As, xs, nc, st0);
+    lds_gemm(nk, Kx, As, xs, nc, st0, xp);
     __syncthreads();
     // [reduced b; u] = Gr [x; zloc]  -> y rows [nk, My)
     auto st1 = [&](int i, int j, TF v) { y[nk + i + (int64_t)j * My] = v; };
-    lds_gemm(R, Kx + nk, As + (int64_t)nk * Kx, xs, nc, st1);
+    lds_gemm(R, Kx + nk, As + (int64_t)nk * Kx, xs, nc, st1, xp);
 }
 
 // Down-sweep, LDS-resident: LDS = Qt (n x n, rounded to 16; empty for a pass-through
 // node) | xs n x XS
 template <typename TF>
-__global__ __launch_bounds__(512) void k_ulv_gdown_s(const SInst<TF>* __restrict__ insts,
+__global__ __launch_bounds__(512, 2) void k_ulv_gdown_s(const SInst<TF>* __restrict__ insts,
                                                      const WorkItem* __restrict__ work, int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
@@ -1542,6 +1544,7 @@ __global__ __launch_bounds__(512) void k_ulv_gdown_s(const SInst<TF>* __restrict
     const int j0 = tj * tw;
     if (j0 >= nrhs) return;
     const int nc = min(tw, nrhs - j0);
+    const int xp = tw + 1;  // LDS pitch of xs
     const DevNode nd = I.nodes[wi.node];
     const FNode fp = I.fn[nd.parent];
     const int off = nd.is_left ? 0 : I.fn[I.nodes[nd.parent].left].nr;
@@ -1560,7 +1563,7 @@ __global__ __launch_bounds__(512) void k_ulv_gdown_s(const SInst<TF>* __restrict
             if (j >= nc) return TF(0);
             return k < nk ? y0[k + (int64_t)j * l0] : z1[(k - nk) + (int64_t)j * l1];
         },
-        [&](int e, TF v) { xs[(e % n) * XS + e / n] = v; });
+        [&](int e, TF v) { xs[(e % n) * xp + e / n] = v; });
     __syncthreads();
     const bool leaf = nd.is_leaf;
     double* X = I.X + nd.col_off + (int64_t)j0 * I.ldx;
@@ -1571,9 +1574,9 @@ __global__ __launch_bounds__(512) void k_ulv_gdown_s(const SInst<TF>* __restrict
         else z[i + (int64_t)j * n] = v;
     };
     if (na > 0) {
-        lds_gemm(n, n, As, xs, nc, store);
+        lds_gemm(n, n, As, xs, nc, store, xp);
     } else {
-        for (int e = threadIdx.x; e < n * nc; e += blockDim.x) store(e % n, e / n, xs[(e % n) * XS + e / n]);
+        for (int e = threadIdx.x; e < n * nc; e += blockDim.x) store(e % n, e / n, xs[(e % n) * xp + e / n]);
     }
 }
 
@@ -1702,7 +1705,7 @@ __device__ __forceinline__ void mv_store(const MvInst& I, const MvOp& o, int i, 
 
 // LDS-resident: LDS = A (M x K, rounded to 16) | xs K x XS
 template <int KIND>
-__global__ __launch_bounds__(512) void k_mv_s(const MvInst* __restrict__ insts, const WorkItem* __restrict__ work,
+__global__ __launch_bounds__(512, 2) void k_mv_s(const MvInst* __restrict__ insts, const WorkItem* __restrict__ work,
                                               int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
@@ -1711,6 +1714,7 @@ __global__ __launch_bounds__(512) void k_mv_s(const MvInst* __restrict__ insts, 
     const int j0 = tj * tw;
     if (j0 >= I.nrhs) return;
     const int nc = min(tw, I.nrhs - j0);
+    const int xp = tw + 1;  // LDS pitch of xs
     MvOp o;
     mv_op<KIND>(I, wi.node, o);
     const int M = o.M, K = o.K;
@@ -1721,9 +1725,9 @@ __global__ __launch_bounds__(512) void k_mv_s(const MvInst* __restrict__ insts, 
     stage2(
         As, o.A, na, K * tw,
         [&](int e) { return e / K < nc ? seg_load<double>(o.x, nullptr, 0, e % K, j0 + e / K, 0) : 0.0; },
-        [&](int e, double v) { xs[(e % K) * XS + e / K] = v; });
+        [&](int e, double v) { xs[(e % K) * xp + e / K] = v; });
     __syncthreads();
-    lds_gemm(M, K, As, xs, nc, [&](int i, int j, double v) { mv_store<KIND>(I, o, i, j0 + j, v); });
+    lds_gemm(M, K, As, xs, nc, [&](int i, int j, double v) { mv_store<KIND>(I, o, i, j0 + j, v); }, xp);
 }
 
 // Streamed through the chunked MFMA tile GEMM (nodes whose operators do not fit in LDS).
@@ -1946,17 +1950,11 @@ hipError_t launch_ulv_gup(const SInst<T>* insts, const WorkItem* work, int nwork
                        tn);
     return hipGetLastError();
 }
-// Column width of the LDS-resident steps: 16 when a level has too few 32-column tiles to
-// occupy the CUs (halves each workgroup's MFMA chain), else 32.
-static inline int resident_tw(int nwork, int max_nrhs) {
-    return (max_nrhs > 16 && (int64_t)nwork * ((max_nrhs + 31) / 32) < 128) ? 16 : 32;
-}
-
 template <typename T>
-hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                            hipStream_t st) {
+hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw,
+                            size_t lds, hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
-    const int tw = resident_tw(nwork, max_nrhs);
+    if (tw != 16 && tw != 32) return hipErrorInvalidValue;
     const int tn = (max_nrhs + tw - 1) / tw;
     const int64_t total = (int64_t)nwork * tn;
     if (total > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
@@ -1968,10 +1966,10 @@ hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwo
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                              hipStream_t st) {
+hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw,
+                              size_t lds, hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
-    const int tw = resident_tw(nwork, max_nrhs);
+    if (tw != 16 && tw != 32) return hipErrorInvalidValue;
     const int tn = (max_nrhs + tw - 1) / tw;
     const int64_t total = (int64_t)nwork * tn;
     if (total > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
@@ -2033,11 +2031,10 @@ hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, do
 
 template <int KIND>
 static hipError_t launch_mv_kind(const MvInst* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
-                                 size_t lds, hipStream_t st) {
+                                 int tw, size_t lds, hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0 || max_m <= 0) return hipSuccess;
     if (lds > 0) {
-        if (lds > 160 * 1024) return hipErrorInvalidValue;
-        const int tw = resident_tw(nwork, max_nrhs);
+        if (lds > 160 * 1024 || (tw != 16 && tw != 32)) return hipErrorInvalidValue;
         const int tn = (max_nrhs + tw - 1) / tw;
         const int64_t total = (int64_t)nwork * tn;
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mv_s<KIND>),
@@ -2058,12 +2055,12 @@ static hipError_t launch_mv_kind(const MvInst* insts, const WorkItem* work, int 
     }
     return hipGetLastError();
 }
-hipError_t launch_mv(int kind, const MvInst* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
+hipError_t launch_mv(int kind, const MvInst* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs, int tw,
                      size_t lds, hipStream_t st) {
     switch (kind) {
-        case 0: return launch_mv_kind<0>(insts, work, nwork, max_m, max_nrhs, lds, st);
-        case 1: return launch_mv_kind<1>(insts, work, nwork, max_m, max_nrhs, lds, st);
-        default: return launch_mv_kind<2>(insts, work, nwork, max_m, max_nrhs, lds, st);
+        case 0: return launch_mv_kind<0>(insts, work, nwork, max_m, max_nrhs, tw, lds, st);
+        case 1: return launch_mv_kind<1>(insts, work, nwork, max_m, max_nrhs, tw, lds, st);
+        default: return launch_mv_kind<2>(insts, work, nwork, max_m, max_nrhs, tw, lds, st);
     }
 }
 
@@ -2082,8 +2079,8 @@ template hipError_t launch_ulv_root<float>(const FInst<float>*, const WorkItem*,
     template hipError_t launch_ulv_rinv<T>(const FInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
     template hipError_t launch_ulv_groot2<T>(const SInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
     template hipError_t launch_ulv_gdown<T>(const SInst<T>*, const WorkItem*, int, int, int, hipStream_t);      \
-    template hipError_t launch_ulv_gup_s<T>(const SInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);      \
-    template hipError_t launch_ulv_gdown_s<T>(const SInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);
+    template hipError_t launch_ulv_gup_s<T>(const SInst<T>*, const WorkItem*, int, int, int, size_t, hipStream_t); \
+    template hipError_t launch_ulv_gdown_s<T>(const SInst<T>*, const WorkItem*, int, int, int, size_t, hipStream_t);
 HSS_INST_SOLVE(double)
 HSS_INST_SOLVE(float)
 #undef HSS_INST_SOLVE

@@ -6,9 +6,17 @@
 
 namespace hss {
 
+// Column width of the LDS-resident steps (16 or 32): 16 when a level has too few 32-column
+// tiles to occupy the CUs (halves each workgroup's MFMA chain) or at most 16 columns.
+inline int resident_tw(int nwork, int max_nrhs) {
+    if (max_nrhs <= 16) return 16;
+    return ((int64_t)nwork * ((max_nrhs + 31) / 32) < 128) ? 16 : 32;
+}
+
 // matvec sweep step over nodes (kind 0 = up: Z, 1 = down: F, 2 = leaves: C); lds > 0
-// selects the LDS-resident kernel with that many bytes, 0 the chunked one
-hipError_t launch_mv(int kind, const MvInst* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
+// selects the LDS-resident kernel with that many bytes and tw columns per workgroup,
+// 0 the chunked one
+hipError_t launch_mv(int kind, const MvInst* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs, int tw,
                      size_t lds, hipStream_t st);
 
 template <typename T>
@@ -43,13 +51,14 @@ hipError_t launch_ulv_groot(const SInst<T>* insts, const WorkItem* work, int nwo
 template <typename T>
 hipError_t launch_ulv_gdown(const SInst<T>* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
                             hipStream_t st);
-// LDS-resident variants for levels whose operators fit (lds = bytes per workgroup)
+// LDS-resident variants for levels whose operators fit (tw = 16 or 32 columns per
+// workgroup, lds = bytes per workgroup)
 template <typename T>
-hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw, size_t lds,
                             hipStream_t st);
 template <typename T>
-hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
-                              hipStream_t st);
+hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw,
+                              size_t lds, hipStream_t st);
 
 hipError_t launch_resid_batch(const BatchVec* tab, int cnt, int maxc, double* norms, hipStream_t st);
 hipError_t launch_axpy_batch(const BatchVec* tab, int cnt, int maxc, hipStream_t st);

@@ -138,13 +138,15 @@ struct SlotCache {
 };
 
 // Level schedule of a set of solve requests: up-sweep by height, roots, down-sweep by
-// depth; up_m = the largest nk, down_m = the largest n of a level's nodes; up_lds /
-// down_lds = LDS bytes of the level's largest node with operator and input staged.
+// depth; up_m = the largest nk, down_m = the largest n of a level's nodes; for the
+// LDS-resident kernels, up_na / down_na = the largest operator (elements, rounded to 16)
+// and up_k / down_k = the most input rows of a level's nodes; w = bytes per element.
 struct Schedule {
     std::vector<WorkItem> work;
     std::vector<Range> up, down;
     std::vector<int> up_m, down_m;
-    std::vector<size_t> up_lds, down_lds;
+    std::vector<int64_t> up_na, down_na, up_k, down_k;
+    size_t w = 8;
     Range roots;
 };
 // Largest LDS of a workgroup (bytes); levels above it use the streamed kernels.
@@ -155,16 +157,29 @@ constexpr size_t kLdsResidentMax = 160 * 1024;
 static bool use_resident(size_t lds, int64_t workgroups) {
     return lds > 0 && lds <= kLdsResidentMax && (lds <= 80 * 1024 || workgroups <= 256);
 }
+// LDS-resident choice for a level whose largest node has an operator of na elements (w
+// bytes each, rounded to 16) and k input rows: columns per workgroup (tw) and LDS bytes.
+static bool pick_resident(size_t w, int64_t na, int64_t k, int count, int maxnrhs, int& tw, size_t& lds) {
+    for (int t : {resident_tw(count, maxnrhs), 16}) {
+        lds = w * (size_t)(na + k * (t + 1));
+        if (use_resident(lds, (int64_t)count * ((maxnrhs + t - 1) / t))) {
+            tw = t;
+            return true;
+        }
+    }
+    return false;
+}
 
 // Matvec schedule: up-sweep steps by height, down-sweep steps by depth (1..), leaves;
-// per step the largest operator height and the LDS of the resident kernel (0 = chunked).
+// per step the largest operator height (m), operator size (na, elements rounded to 16)
+// and input rows (k).
 struct MvSched {
     std::vector<Range> up, down;
     Range leaf;
     std::vector<int> up_m, down_m;
     int leaf_m = 0;
-    std::vector<size_t> up_lds, down_lds;
-    size_t leaf_lds = 0;
+    std::vector<int64_t> up_na, down_na, up_k, down_k;
+    int64_t leaf_na = 0, leaf_k = 0;
 };
 
 struct PlanKey {
@@ -377,14 +392,8 @@ static void build_mv_schedule(const std::vector<hss_mat_s*>& mats, std::vector<W
         maxh = std::max(maxh, M->max_height);
         maxd = std::max(maxd, M->max_depth);
     }
-    auto lds_of = [](int rows, int cols) {
-        const size_t b = sizeof(double) * (size_t)(((int64_t)rows * cols + 15) / 16 * 16 + (int64_t)cols * 33);
-        return b;
-    };
-    auto level = [&](int kind, auto sel, Range& r, int& mmax, size_t& lds) {
+    auto level = [&](int kind, auto sel, Range& r, int& mmax, int64_t& na, int64_t& kk) {
         r.off = (int)work.size();
-        bool fits = true;
-        size_t need = 0;
         for (int b = 0; b < (int)mats.size(); ++b) {
             hss_mat_s* M = mats[b];
             for (int i = 0; i < M->nn; ++i) {
@@ -393,47 +402,51 @@ static void build_mv_schedule(const std::vector<hss_mat_s*>& mats, std::vector<W
                 int rows, cols;
                 mv_shape(M, i, kind, rows, cols);
                 mmax = std::max(mmax, rows);
-                const size_t l = lds_of(rows, cols);
-                need = std::max(need, l);
-                fits = fits && l <= kLdsResidentMax;
+                na = std::max(na, ((int64_t)rows * cols + 15) / 16 * 16);
+                kk = std::max(kk, (int64_t)cols);
             }
         }
         r.count = (int)work.size() - r.off;
-        lds = fits ? need : 0;
     };
     S.up.assign(maxh, Range{});
     S.up_m.assign(maxh, 0);
-    S.up_lds.assign(maxh, 0);
+    S.up_na.assign(maxh, 0);
+    S.up_k.assign(maxh, 0);
     for (int h = 0; h < maxh; ++h)
         level(0, [&](const hss_mat_s* M, int i) { return i != 0 && M->hn[i].height == h; }, S.up[h], S.up_m[h],
-              S.up_lds[h]);
+              S.up_na[h], S.up_k[h]);
     S.down.assign(maxd + 1, Range{});
     S.down_m.assign(maxd + 1, 0);
-    S.down_lds.assign(maxd + 1, 0);
+    S.down_na.assign(maxd + 1, 0);
+    S.down_k.assign(maxd + 1, 0);
     for (int d = 1; d <= maxd; ++d)
         level(1, [&](const hss_mat_s* M, int i) { return i != 0 && M->hn[i].depth == d; }, S.down[d], S.down_m[d],
-              S.down_lds[d]);
+              S.down_na[d], S.down_k[d]);
     S.leaf_m = 0;
-    level(2, [&](const hss_mat_s* M, int i) { return M->hn[i].is_leaf != 0; }, S.leaf, S.leaf_m, S.leaf_lds);
+    S.leaf_na = S.leaf_k = 0;
+    level(2, [&](const hss_mat_s* M, int i) { return M->hn[i].is_leaf != 0; }, S.leaf, S.leaf_m, S.leaf_na, S.leaf_k);
 }
 
 static hss_status launch_mv_schedule(const MvInst* di, const WorkItem* dw, const MvSched& S, int maxnrhs,
                                      hipStream_t st) {
-    const int64_t ntiles = (maxnrhs + 31) / 32;
-    auto pick = [&](size_t lds, int count) { return use_resident(lds, count * ntiles) ? lds : (size_t)0; };
+    int tw = 32;
+    size_t lds = 0;
+    auto pick = [&](int64_t na, int64_t k, int count) {
+        return pick_resident(sizeof(double), na, k, count, maxnrhs, tw, lds) ? lds : (size_t)0;
+    };
     for (size_t h = 0; h < S.up.size(); ++h) {
-        const size_t l = pick(S.up_lds[h], S.up[h].count);
+        const size_t l = pick(S.up_na[h], S.up_k[h], S.up[h].count);
         ProfScope ps(l ? "k_mv_up_s" : "k_mv_up", (int)h, S.up[h].count, st);
-        HIP_TRY(launch_mv(0, di, dw + S.up[h].off, S.up[h].count, S.up_m[h], maxnrhs, l, st));
+        HIP_TRY(launch_mv(0, di, dw + S.up[h].off, S.up[h].count, S.up_m[h], maxnrhs, tw, l, st));
     }
     for (size_t d = 1; d < S.down.size(); ++d) {
-        const size_t l = pick(S.down_lds[d], S.down[d].count);
+        const size_t l = pick(S.down_na[d], S.down_k[d], S.down[d].count);
         ProfScope ps(l ? "k_mv_down_s" : "k_mv_down", (int)d, S.down[d].count, st);
-        HIP_TRY(launch_mv(1, di, dw + S.down[d].off, S.down[d].count, S.down_m[d], maxnrhs, l, st));
+        HIP_TRY(launch_mv(1, di, dw + S.down[d].off, S.down[d].count, S.down_m[d], maxnrhs, tw, l, st));
     }
-    const size_t l = pick(S.leaf_lds, S.leaf.count);
+    const size_t l = pick(S.leaf_na, S.leaf_k, S.leaf.count);
     ProfScope ps(l ? "k_mv_leaf_s" : "k_mv_leaf", 0, S.leaf.count, st);
-    HIP_TRY(launch_mv(2, di, dw + S.leaf.off, S.leaf.count, S.leaf_m, maxnrhs, l, st));
+    HIP_TRY(launch_mv(2, di, dw + S.leaf.off, S.leaf.count, S.leaf_m, maxnrhs, tw, l, st));
     return HSS_OK;
 }
 
@@ -1157,10 +1170,13 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
     S.work.clear();
     S.up.assign(maxh, Range{});
     S.up_m.assign(maxh, 0);
-    S.up_lds.assign(maxh, 0);
+    S.up_na.assign(maxh, 0);
+    S.up_k.assign(maxh, 0);
     S.down.assign(maxd + 1, Range{});
     S.down_m.assign(maxd + 1, 0);
-    S.down_lds.assign(maxd + 1, 0);
+    S.down_na.assign(maxd + 1, 0);
+    S.down_k.assign(maxd + 1, 0);
+    S.w = (!facts.empty() && facts[0]->prec == HSS_PREC_FP32) ? 4 : 8;
     auto r16 = [](int64_t x) { return (x + 15) & ~int64_t(15); };
     for (int h = 0; h < maxh; ++h) {
         S.up[h].off = (int)S.work.size();
@@ -1172,10 +1188,10 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
                     const int node = M->hwork[r.off + t].node;
                     S.work.push_back(WorkItem{b, node});
                     const FNode& f = facts[b]->hfn[node];
-                    const size_t w = facts[b]->prec == HSS_PREC_FP32 ? 4 : 8;
                     const int64_t na = (int64_t)f.nk * f.Kx + (int64_t)(f.My - f.nk) * (f.Kx + f.nk);
                     S.up_m[h] = std::max(S.up_m[h], f.nk);
-                    S.up_lds[h] = std::max(S.up_lds[h], w * (size_t)(r16(na) + (int64_t)(f.Kx + f.nk) * 33));
+                    S.up_na[h] = std::max(S.up_na[h], r16(na));
+                    S.up_k[h] = std::max(S.up_k[h], (int64_t)(f.Kx + f.nk));
                 }
             }
         }
@@ -1193,10 +1209,10 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
                     const int node = M->hwork[r.off + t].node;
                     S.work.push_back(WorkItem{b, node});
                     const FNode& f = facts[b]->hfn[node];
-                    const size_t w = facts[b]->prec == HSS_PREC_FP32 ? 4 : 8;
                     const int64_t na = f.Qt >= 0 ? (int64_t)f.n * f.n : 0;
                     S.down_m[d] = std::max(S.down_m[d], f.n);
-                    S.down_lds[d] = std::max(S.down_lds[d], w * (size_t)(r16(na) + (int64_t)f.n * 33));
+                    S.down_na[d] = std::max(S.down_na[d], r16(na));
+                    S.down_k[d] = std::max(S.down_k[d], (int64_t)f.n);
                 }
             }
         }
@@ -1209,12 +1225,17 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
 template <typename TF>
 static hss_status launch_schedule(const SInst<TF>* di, const WorkItem* dw, const Schedule& S, int maxnrhs,
                                   size_t lgroot, size_t lgroot_res, size_t lgroot2, hipStream_t st) {
-    // operators staged whole in LDS (use_resident), else streamed through the chunked kernels
-    const int64_t ntiles = (maxnrhs + 31) / 32;
+    // operators staged whole in LDS (use_resident, with 32 or 16 columns per workgroup),
+    // else streamed through the chunked kernels
+    auto resident = [&](int64_t na, int64_t k, int count, int& tw, size_t& lds) {
+        return pick_resident(S.w, na, k, count, maxnrhs, tw, lds);
+    };
+    int tw = 32;
+    size_t lds = 0;
     for (size_t h = 0; h < S.up.size(); ++h) {
-        if (use_resident(S.up_lds[h], S.up[h].count * ntiles)) {
+        if (resident(S.up_na[h], S.up_k[h], S.up[h].count, tw, lds)) {
             ProfScope ps("k_ulv_gup_s", (int)h, S.up[h].count, st);
-            HIP_TRY((launch_ulv_gup_s<TF>(di, dw + S.up[h].off, S.up[h].count, maxnrhs, S.up_lds[h], st)));
+            HIP_TRY((launch_ulv_gup_s<TF>(di, dw + S.up[h].off, S.up[h].count, maxnrhs, tw, lds, st)));
         } else {
             ProfScope ps("k_ulv_gup", (int)h, S.up[h].count, st);
             HIP_TRY((launch_ulv_gup<TF>(di, dw + S.up[h].off, S.up[h].count, S.up_m[h], maxnrhs, st)));
@@ -1232,9 +1253,9 @@ static hss_status launch_schedule(const SInst<TF>* di, const WorkItem* dw, const
         }
     }
     for (size_t d = 1; d < S.down.size(); ++d) {
-        if (use_resident(S.down_lds[d], S.down[d].count * ntiles)) {
+        if (resident(S.down_na[d], S.down_k[d], S.down[d].count, tw, lds)) {
             ProfScope ps("k_ulv_gdown_s", (int)d, S.down[d].count, st);
-            HIP_TRY((launch_ulv_gdown_s<TF>(di, dw + S.down[d].off, S.down[d].count, maxnrhs, S.down_lds[d], st)));
+            HIP_TRY((launch_ulv_gdown_s<TF>(di, dw + S.down[d].off, S.down[d].count, maxnrhs, tw, lds, st)));
         } else {
             ProfScope ps("k_ulv_gdown", (int)d, S.down[d].count, st);
             HIP_TRY((launch_ulv_gdown<TF>(di, dw + S.down[d].off, S.down[d].count, S.down_m[d], maxnrhs, st)));
