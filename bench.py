@@ -195,7 +195,10 @@ def batched(pkg, hip, torch, steps=10, warmup=2, count=256, seed=3, log=print):
             continue
         ms = (time.perf_counter() - t0) * 1e3 / steps
         recs = profile_launches(hip, lambda: plan.execute(tol=tol), 1)
-        _, byk = summarize(recs, 1)
+        tot, byk = summarize(recs, 1)
+        levels = sorted(((k, lev, v[0], v[2]) for (k, lev), v in tot.items()), key=lambda t: -t[2])[:12]
+        log(f"[bench] batch {prec} busiest launches (kernel, level, ms, nodes): "
+            f"{[(k, lev, round(ms, 3), nw) for k, lev, ms, nw in levels]}")
         out[prec] = {"batch_ms": round(ms, 3), "solves_per_s": round(count * 1e3 / ms, 1),
                      "factor_batch_ms": round(fms, 2),
                      "worst_rel_residual_fp64_check": _batch_residual(mats, Bs, Xs, rs, torch),

@@ -1427,9 +1427,9 @@ __device__ __forceinline__ void lds_gemm(int M, int K, const TF* As, const TF* x
 // own A fragments straight into registers, U k-steps ahead of the MFMAs.
 template <typename TF, typename Store>
 __device__ __forceinline__ void glb_gemm(int M, int K, const TF* __restrict__ A, int64_t lda, const TF* xs, int ncols,
-                                         Store store) {
+                                         Store store, int xp = XS) {
     using V4 = typename Mfma<TF>::v4;
-    constexpr int U = 8;
+    constexpr int U = 6;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const int nrb = (M + 15) / 16, ncb = (ncols + 15) / 16;
@@ -1453,7 +1453,7 @@ __device__ __forceinline__ void glb_gemm(int M, int K, const TF* __restrict__ A,
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int kk = 4 * (s0 + u) + lk;
-                const TF b = kk < K ? xs[kk * XS + jc] : TF(0);
+                const TF b = kk < K ? xs[kk * xp + jc] : TF(0);
                 if (u & 1) acc1 = Mfma<TF>::mac(a[u], b, acc1);
                 else acc0 = Mfma<TF>::mac(a[u], b, acc0);
             }
@@ -1470,10 +1470,12 @@ __device__ __forceinline__ void glb_gemm(int M, int K, const TF* __restrict__ A,
     }
 }
 
-// Up-sweep, LDS-resident: LDS = [Gz Gr] (nk Kx + R (Kx + nk), rounded to 16) | xs (Kx + nk) x XS.
-// tw = columns per workgroup (16 or 32).
-template <typename TF>
-__global__ __launch_bounds__(512, 2) void k_ulv_gup_s(const SInst<TF>* __restrict__ insts,
+// Up-sweep with the input staged in LDS; tw = columns per workgroup (16 or 32), xs pitch
+// tw + 1.  AR: the operator [Gz Gr] is staged too (LDS = [Gz Gr] (nk Kx + R (Kx + nk),
+// rounded to 16) | xs (Kx + nk) rows); otherwise the waves stream it from memory
+// (LDS = xs), for operators too large to stage whose rows fill few 128-row tiles.
+template <typename TF, bool AR>
+__global__ __launch_bounds__(512, 4) void k_ulv_gup_s(const SInst<TF>* __restrict__ insts,
                                                    const WorkItem* __restrict__ work, int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
@@ -1490,7 +1492,8 @@ __global__ __launch_bounds__(512, 2) void k_ulv_gup_s(const SInst<TF>* __restric
     const int64_t na = (int64_t)nk * Kx + (int64_t)R * (Kx + nk);
     extern __shared__ unsigned char lds_raw[];
     TF* As = reinterpret_cast<TF*>(lds_raw);
-    TF* xs = As + ((na + 15) & ~int64_t(15));
+    TF* xs = AR ? As + ((na + 15) & ~int64_t(15)) : As;
+    const int64_t nst = AR ? na : 0;  // operator elements staged
     // operator and x rows [0, Kx) (the leaf's rows of B, or the children's reduced b and u)
     // in one round of loads; columns past nc are zero
     const TF* Ag = I.fac + (nk > 0 ? f.Gz : f.Gr);
@@ -1499,7 +1502,7 @@ __global__ __launch_bounds__(512, 2) void k_ulv_gup_s(const SInst<TF>* __restric
         const double* B = I.B + nd.row_off + (int64_t)j0 * I.ldb;
         const int64_t ldb = I.ldb;
         stage2(
-            As, Ag, na, Kx * tw, [&](int e) { return e / Kx < nc ? (TF)B[(e % Kx) + (int64_t)(e / Kx) * ldb] : TF(0); },
+            As, Ag, nst, Kx * tw, [&](int e) { return e / Kx < nc ? (TF)B[(e % Kx) + (int64_t)(e / Kx) * ldb] : TF(0); },
             put);
     } else {
         const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
@@ -1508,7 +1511,7 @@ __global__ __launch_bounds__(512, 2) void k_ulv_gup_s(const SInst<TF>* __restric
         const TF* y1 = I.wy + fr.yrow * nrhs + fr.nk + (int64_t)j0 * fr.My;
         const int l0 = fl.My, l1 = fr.My;
         stage2(
-            As, Ag, na, Kx * tw,
+            As, Ag, nst, Kx * tw,
             [&](int e) {
                 const int k = e % Kx, j = e / Kx;
                 if (j >= nc) return TF(0);
@@ -1523,17 +1526,19 @@ __global__ __launch_bounds__(512, 2) void k_ulv_gup_s(const SInst<TF>* __restric
         y[i + (int64_t)j * My] = v;
         xs[(Kx + i) * xp + j] = v;
     };
-    lds_gemm(nk, Kx, As, xs, nc, st0, xp);
+    if (AR) lds_gemm(nk, Kx, As, xs, nc, st0, xp);
+    else glb_gemm(nk, Kx, Ag, nk, xs, nc, st0, xp);
     __syncthreads();
     // [reduced b; u] = Gr [x; zloc]  -> y rows [nk, My)
     auto st1 = [&](int i, int j, TF v) { y[nk + i + (int64_t)j * My] = v; };
-    lds_gemm(R, Kx + nk, As + (int64_t)nk * Kx, xs, nc, st1, xp);
+    if (AR) lds_gemm(R, Kx + nk, As + (int64_t)nk * Kx, xs, nc, st1, xp);
+    else glb_gemm(R, Kx + nk, Ag + (int64_t)nk * Kx, R, xs, nc, st1, xp);
 }
 
-// Down-sweep, LDS-resident: LDS = Qt (n x n, rounded to 16; empty for a pass-through
-// node) | xs n x XS
-template <typename TF>
-__global__ __launch_bounds__(512, 2) void k_ulv_gdown_s(const SInst<TF>* __restrict__ insts,
+// Down-sweep with the input staged in LDS.  AR: Qt is staged too (LDS = Qt (n x n,
+// rounded to 16; empty for a pass-through node) | xs n rows); otherwise streamed.
+template <typename TF, bool AR>
+__global__ __launch_bounds__(512, 4) void k_ulv_gdown_s(const SInst<TF>* __restrict__ insts,
                                                      const WorkItem* __restrict__ work, int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
@@ -1551,13 +1556,13 @@ __global__ __launch_bounds__(512, 2) void k_ulv_gdown_s(const SInst<TF>* __restr
     const int64_t na = f.Qt >= 0 ? (int64_t)n * n : 0;
     extern __shared__ unsigned char lds_raw[];
     TF* As = reinterpret_cast<TF*>(lds_raw);
-    TF* xs = As + ((na + 15) & ~int64_t(15));
+    TF* xs = AR ? As + ((na + 15) & ~int64_t(15)) : As;
     // Qt and x = [zloc (rows 0..nk-1 of the node's y); z from the parent] in one round
     const TF* y0 = I.wy + f.yrow * nrhs + (int64_t)j0 * f.My;
     const TF* z1 = I.wz + fp.zrow * nrhs + off + (int64_t)j0 * fp.n;
     const int l0 = f.My, l1 = fp.n;
     stage2(
-        As, na > 0 ? I.fac + f.Qt : I.fac, na, n * tw,
+        As, na > 0 ? I.fac + f.Qt : I.fac, AR ? na : 0, n * tw,
         [&](int e) {
             const int k = e % n, j = e / n;
             if (j >= nc) return TF(0);
@@ -1574,7 +1579,8 @@ __global__ __launch_bounds__(512, 2) void k_ulv_gdown_s(const SInst<TF>* __restr
         else z[i + (int64_t)j * n] = v;
     };
     if (na > 0) {
-        lds_gemm(n, n, As, xs, nc, store, xp);
+        if (AR) lds_gemm(n, n, As, xs, nc, store, xp);
+        else glb_gemm(n, n, I.fac + f.Qt, n, xs, nc, store, xp);
     } else {
         for (int e = threadIdx.x; e < n * nc; e += blockDim.x) store(e % n, e / n, xs[(e % n) * xp + e / n]);
     }
@@ -1705,7 +1711,7 @@ __device__ __forceinline__ void mv_store(const MvInst& I, const MvOp& o, int i, 
 
 // LDS-resident: LDS = A (M x K, rounded to 16) | xs K x XS
 template <int KIND>
-__global__ __launch_bounds__(512, 2) void k_mv_s(const MvInst* __restrict__ insts, const WorkItem* __restrict__ work,
+__global__ __launch_bounds__(512, 4) void k_mv_s(const MvInst* __restrict__ insts, const WorkItem* __restrict__ work,
                                               int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
@@ -1950,35 +1956,47 @@ hipError_t launch_ulv_gup(const SInst<T>* insts, const WorkItem* work, int nwork
                        tn);
     return hipGetLastError();
 }
-template <typename T>
-hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw,
-                            size_t lds, hipStream_t st) {
-    if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
-    if (tw != 16 && tw != 32) return hipErrorInvalidValue;
-    const int tn = (max_nrhs + tw - 1) / tw;
+template <typename T, bool AR>
+static hipError_t launch_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int tn, int tw, size_t lds,
+                               hipStream_t st) {
     const int64_t total = (int64_t)nwork * tn;
-    if (total > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gup_s<T>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gup_s<T, AR>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_gup_s<T>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts, work,
+    hipLaunchKernelGGL((k_ulv_gup_s<T, AR>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts, work,
                        nwork, tn, tw);
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw,
-                              size_t lds, hipStream_t st) {
+hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw,
+                            size_t lds, int resident, hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
     if (tw != 16 && tw != 32) return hipErrorInvalidValue;
     const int tn = (max_nrhs + tw - 1) / tw;
+    if ((int64_t)nwork * tn > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
+    return resident ? launch_gup_s<T, true>(insts, work, nwork, tn, tw, lds, st)
+                    : launch_gup_s<T, false>(insts, work, nwork, tn, tw, lds, st);
+}
+template <typename T, bool AR>
+static hipError_t launch_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int tn, int tw, size_t lds,
+                                 hipStream_t st) {
     const int64_t total = (int64_t)nwork * tn;
-    if (total > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gdown_s<T>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gdown_s<T, AR>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_gdown_s<T>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts, work,
-                       nwork, tn, tw);
+    hipLaunchKernelGGL((k_ulv_gdown_s<T, AR>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts,
+                       work, nwork, tn, tw);
     return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw,
+                              size_t lds, int resident, hipStream_t st) {
+    if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
+    if (tw != 16 && tw != 32) return hipErrorInvalidValue;
+    const int tn = (max_nrhs + tw - 1) / tw;
+    if ((int64_t)nwork * tn > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
+    return resident ? launch_gdown_s<T, true>(insts, work, nwork, tn, tw, lds, st)
+                    : launch_gdown_s<T, false>(insts, work, nwork, tn, tw, lds, st);
 }
 template <typename T>
 hipError_t launch_ulv_gdown(const SInst<T>* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
@@ -2079,8 +2097,9 @@ template hipError_t launch_ulv_root<float>(const FInst<float>*, const WorkItem*,
     template hipError_t launch_ulv_rinv<T>(const FInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
     template hipError_t launch_ulv_groot2<T>(const SInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
     template hipError_t launch_ulv_gdown<T>(const SInst<T>*, const WorkItem*, int, int, int, hipStream_t);      \
-    template hipError_t launch_ulv_gup_s<T>(const SInst<T>*, const WorkItem*, int, int, int, size_t, hipStream_t); \
-    template hipError_t launch_ulv_gdown_s<T>(const SInst<T>*, const WorkItem*, int, int, int, size_t, hipStream_t);
+    template hipError_t launch_ulv_gup_s<T>(const SInst<T>*, const WorkItem*, int, int, int, size_t, int, hipStream_t); \
+    template hipError_t launch_ulv_gdown_s<T>(const SInst<T>*, const WorkItem*, int, int, int, size_t, int,            \
+                                              hipStream_t);
 HSS_INST_SOLVE(double)
 HSS_INST_SOLVE(float)
 #undef HSS_INST_SOLVE

@@ -51,14 +51,14 @@ hipError_t launch_ulv_groot(const SInst<T>* insts, const WorkItem* work, int nwo
 template <typename T>
 hipError_t launch_ulv_gdown(const SInst<T>* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
                             hipStream_t st);
-// LDS-resident variants for levels whose operators fit (tw = 16 or 32 columns per
-// workgroup, lds = bytes per workgroup)
+// Variants with the input staged in LDS (tw = 16 or 32 columns per workgroup, lds = bytes
+// per workgroup); resident != 0: the node operators are staged too, else streamed
 template <typename T>
 hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw, size_t lds,
-                            hipStream_t st);
+                            int resident, hipStream_t st);
 template <typename T>
 hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw,
-                              size_t lds, hipStream_t st);
+                              size_t lds, int resident, hipStream_t st);
 
 hipError_t launch_resid_batch(const BatchVec* tab, int cnt, int maxc, double* norms, hipStream_t st);
 hipError_t launch_axpy_batch(const BatchVec* tab, int cnt, int maxc, hipStream_t st);

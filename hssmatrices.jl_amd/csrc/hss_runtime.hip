@@ -144,25 +144,37 @@ struct SlotCache {
 struct Schedule {
     std::vector<WorkItem> work;
     std::vector<Range> up, down;
-    std::vector<int> up_m, down_m;
+    std::vector<int> up_m, down_m, up_rows;
     std::vector<int64_t> up_na, down_na, up_k, down_k;
     size_t w = 8;
     Range roots;
 };
 // Largest LDS of a workgroup (bytes); levels above it use the streamed kernels.
 constexpr size_t kLdsResidentMax = 160 * 1024;
-// A level runs LDS-resident when its staging fits and either leaves room for two
-// workgroups per CU or the level has at most one workgroup per CU; a wide level of
-// large operators keeps more bytes in flight through the chunked kernels.
-static bool use_resident(size_t lds, int64_t workgroups) {
-    return lds > 0 && lds <= kLdsResidentMax && (lds <= 80 * 1024 || workgroups <= 256);
-}
-// LDS-resident choice for a level whose largest node has an operator of na elements (w
-// bytes each, rounded to 16) and k input rows: columns per workgroup (tw) and LDS bytes.
+// A level runs LDS-resident (operator and input staged) when the staging fits and either
+// the operators are small (at most 72 KiB) or the level has at most one workgroup per CU:
+// a wide level of large operators keeps more bytes in flight streaming them.  tw =
+// columns per workgroup, lds = bytes per workgroup; na = the level's largest operator
+// (elements of w bytes, rounded to 16), k = its most input rows.
 static bool pick_resident(size_t w, int64_t na, int64_t k, int count, int maxnrhs, int& tw, size_t& lds) {
     for (int t : {resident_tw(count, maxnrhs), 16}) {
         lds = w * (size_t)(na + k * (t + 1));
-        if (use_resident(lds, (int64_t)count * ((maxnrhs + t - 1) / t))) {
+        const int64_t wgs = (int64_t)count * ((maxnrhs + t - 1) / t);
+        if (lds <= kLdsResidentMax && (w * (size_t)na <= 72 * 1024 || wgs <= 256)) {
+            tw = t;
+            return true;
+        }
+    }
+    return false;
+}
+// Otherwise operators whose rows fill few 128-row tiles (rows < 96) are streamed by the
+// waves with only the input staged (at most 80 KiB, two workgroups per CU); taller ones go
+// through the chunked tile kernels.
+static bool pick_streamed(size_t w, int64_t k, int rows, int count, int maxnrhs, int& tw, size_t& lds) {
+    if (rows >= 96) return false;
+    for (int t : {resident_tw(count, maxnrhs), 16}) {
+        lds = w * (size_t)(k * (t + 1));
+        if (lds <= 80 * 1024) {
             tw = t;
             return true;
         }
@@ -1172,6 +1184,7 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
     S.up_m.assign(maxh, 0);
     S.up_na.assign(maxh, 0);
     S.up_k.assign(maxh, 0);
+    S.up_rows.assign(maxh, 0);
     S.down.assign(maxd + 1, Range{});
     S.down_m.assign(maxd + 1, 0);
     S.down_na.assign(maxd + 1, 0);
@@ -1192,6 +1205,7 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
                     S.up_m[h] = std::max(S.up_m[h], f.nk);
                     S.up_na[h] = std::max(S.up_na[h], r16(na));
                     S.up_k[h] = std::max(S.up_k[h], (int64_t)(f.Kx + f.nk));
+                    S.up_rows[h] = std::max(S.up_rows[h], std::max(f.nk, f.My - f.nk));
                 }
             }
         }
@@ -1227,15 +1241,16 @@ static hss_status launch_schedule(const SInst<TF>* di, const WorkItem* dw, const
                                   size_t lgroot, size_t lgroot_res, size_t lgroot2, hipStream_t st) {
     // operators staged whole in LDS (use_resident, with 32 or 16 columns per workgroup),
     // else streamed through the chunked kernels
-    auto resident = [&](int64_t na, int64_t k, int count, int& tw, size_t& lds) {
-        return pick_resident(S.w, na, k, count, maxnrhs, tw, lds);
-    };
     int tw = 32;
     size_t lds = 0;
     for (size_t h = 0; h < S.up.size(); ++h) {
-        if (resident(S.up_na[h], S.up_k[h], S.up[h].count, tw, lds)) {
-            ProfScope ps("k_ulv_gup_s", (int)h, S.up[h].count, st);
-            HIP_TRY((launch_ulv_gup_s<TF>(di, dw + S.up[h].off, S.up[h].count, maxnrhs, tw, lds, st)));
+        const int cnt = S.up[h].count;
+        if (pick_resident(S.w, S.up_na[h], S.up_k[h], cnt, maxnrhs, tw, lds)) {
+            ProfScope ps("k_ulv_gup_s", (int)h, cnt, st);
+            HIP_TRY((launch_ulv_gup_s<TF>(di, dw + S.up[h].off, cnt, maxnrhs, tw, lds, 1, st)));
+        } else if (pick_streamed(S.w, S.up_k[h], S.up_rows[h], cnt, maxnrhs, tw, lds)) {
+            ProfScope ps("k_ulv_gup_x", (int)h, cnt, st);
+            HIP_TRY((launch_ulv_gup_s<TF>(di, dw + S.up[h].off, cnt, maxnrhs, tw, lds, 0, st)));
         } else {
             ProfScope ps("k_ulv_gup", (int)h, S.up[h].count, st);
             HIP_TRY((launch_ulv_gup<TF>(di, dw + S.up[h].off, S.up[h].count, S.up_m[h], maxnrhs, st)));
@@ -1253,9 +1268,13 @@ static hss_status launch_schedule(const SInst<TF>* di, const WorkItem* dw, const
         }
     }
     for (size_t d = 1; d < S.down.size(); ++d) {
-        if (resident(S.down_na[d], S.down_k[d], S.down[d].count, tw, lds)) {
-            ProfScope ps("k_ulv_gdown_s", (int)d, S.down[d].count, st);
-            HIP_TRY((launch_ulv_gdown_s<TF>(di, dw + S.down[d].off, S.down[d].count, maxnrhs, tw, lds, st)));
+        const int cnt = S.down[d].count;
+        if (pick_resident(S.w, S.down_na[d], S.down_k[d], cnt, maxnrhs, tw, lds)) {
+            ProfScope ps("k_ulv_gdown_s", (int)d, cnt, st);
+            HIP_TRY((launch_ulv_gdown_s<TF>(di, dw + S.down[d].off, cnt, maxnrhs, tw, lds, 1, st)));
+        } else if (pick_streamed(S.w, S.down_k[d], S.down_m[d], cnt, maxnrhs, tw, lds)) {
+            ProfScope ps("k_ulv_gdown_x", (int)d, cnt, st);
+            HIP_TRY((launch_ulv_gdown_s<TF>(di, dw + S.down[d].off, cnt, maxnrhs, tw, lds, 0, st)));
         } else {
             ProfScope ps("k_ulv_gdown", (int)d, S.down[d].count, st);
             HIP_TRY((launch_ulv_gdown<TF>(di, dw + S.down[d].off, S.down[d].count, S.down_m[d], maxnrhs, st)));
@@ -1477,6 +1496,7 @@ struct hss_plan_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int64_t dev_bytes = 0;
+    std::vector<char> uploaded_active;  // request activity the device tables currently hold
 };
 
 static void free_plan(hss_plan_s* P) {
@@ -1612,6 +1632,7 @@ static hss_status plan_create_impl(int32_t count, hss_fact_s* const* facts, cons
         P->all_groot2 = P->all_groot2 && F->lds_groot2 > 0;
     }
     P->cnt = (int)P->reqs.size();
+    P->uploaded_active.assign(P->cnt, 1);  // tables are built with every request active
     std::vector<hss_mat_s*> mats;
     std::vector<hss_fact_s*> pf;
     for (auto& q : P->reqs) {
@@ -1663,6 +1684,8 @@ using SInstF = SInst<float>;
 using SInstD = SInst<double>;
 
 static hss_status plan_set_active(hss_plan_s* P, const std::vector<char>& active) {
+    if (active == P->uploaded_active) return HSS_OK;  // the device tables already say so
+    P->uploaded_active = active;
     const size_t nrhs_off = (P->prec == HSS_PREC_FP32) ? offsetof(SInstF, nrhs) : offsetof(SInstD, nrhs);
     for (int t = 0; t < 2; ++t) {
         if (!P->d_sinst[t]) continue;
