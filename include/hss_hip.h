@@ -194,8 +194,9 @@ void hss_prof_enable(int on);
 void hss_prof_reset(void);
 int32_t hss_prof_count(void);
 hss_status hss_prof_get(int32_t i, const char** name, int32_t* level, int32_t* nwork, double* ms);
-/* Factor sizes of one node: {m, n, k, nk, mr, nr, pass, nblk, My, Kx} (10 ints); My x Kx is the
- * node's up-sweep operator, n x n its down-sweep operator. */
+/* Factor sizes of one node: {m, n, k, nk, mr, nr, pass, nblk, My, Kx} (10 ints).  The node's
+ * up-sweep reads Kx input rows and writes My = nk + mr + kv output rows through two operators,
+ * nk x Kx and (My - nk) x (Kx + nk); its down-sweep operator is n x n. */
 hss_status hss_fact_node_info(hss_fact_t fact, int32_t node, int32_t* info);
 /* Per-call device timing of the last matvec/solve on this thread (milliseconds, -1 if none). */
 double hss_last_device_ms(void);
