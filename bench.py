@@ -303,23 +303,29 @@ def serving(pkg, hip, torch, count=32, requests=400, rate=2000.0, seed=4, log=pr
 PROFILE_COUNTERS = ROOT / "profiles" / "hss_c2_n65536_hbm_counters.md"
 
 
-def sweep_grid_threads(nodes, fact, kernel, level, nwork, nrhs):
-    """Launch size (threads) of a solve sweep launch, as the library computes it, padded to a
-    multiple of 8 workgroups.  k_ulv_gup_s / k_ulv_gdown_s: one 512-thread workgroup per node
-    and 32 columns; k_ulv_gup: one 256-thread workgroup per node and 32 columns;
-    k_ulv_gdown: 256-thread tiles of 64 x 64 outputs (128 x 32 for at most 32 right-hand
-    sides)."""
-    if kernel.endswith("_s"):
-        total = nwork * (-(-nrhs // 32))
-        return 8 * (-(-total // 8)) * 512
+def sweep_launch(nodes, fact, kernel, level, nwork, nrhs):
+    """rocprofv3 name, template arguments and launch size (threads) of one FP64 solve sweep
+    launch recorded as ``kernel``, as the library launches it (padded to a multiple of 8
+    workgroups).
+    k_ulv_gup_s / k_ulv_gdown_s (operator staged in LDS) and k_ulv_gup_x / k_ulv_gdown_x
+    (operator streamed; the same kernels with the second template argument false): one
+    512-thread workgroup per node and tw-column tile, tw = 16 for narrow levels (fewer
+    than 128 tiles of 32 columns) else 32; k_ulv_gup: one 256-thread workgroup per node
+    and 32 columns; k_ulv_gdown: 256-thread tiles of 64 x 64 outputs (128 x 32 for at
+    most 32 right-hand sides)."""
+    if kernel.endswith(("_s", "_x")):
+        tw = 16 if nrhs <= 16 or nwork * (-(-nrhs // 32)) < 128 else 32
+        total = nwork * (-(-nrhs // tw))
+        name = kernel[:-2] + "_s"
+        return name, "<double, true>" if kernel.endswith("_s") else "<double, false>", 8 * (-(-total // 8)) * 512
     if kernel == "k_ulv_gup":
         total = nwork * (-(-nrhs // 32))
-    else:
-        tm_rows, tn_cols = (64, 64) if nrhs > 32 else (128, 32)
-        sel = [i for i in range(1, len(nodes)) if nodes["depth"][i] == level]
-        max_m = max(fact.node_info(i)["n"] for i in sel)
-        total = nwork * (-(-max_m // tm_rows)) * (-(-nrhs // tn_cols))
-    return 8 * (-(-total // 8)) * 256
+        return kernel, "<double>", 8 * (-(-total // 8)) * 256
+    tm_rows, tn_cols = (64, 64) if nrhs > 32 else (128, 32)
+    sel = [i for i in range(1, len(nodes)) if nodes["depth"][i] == level]
+    max_m = max(fact.node_info(i)["n"] for i in sel)
+    total = nwork * (-(-max_m // tm_rows)) * (-(-nrhs // tn_cols))
+    return kernel, "<double, 2, 2>" if nrhs > 32 else "<double, 4, 1>", 8 * (-(-total // 8)) * 256
 
 
 def profiled_traffic(kernel: str, type_args: str, grid_x: int, grid_y: int):
@@ -511,8 +517,8 @@ def main():
     achieved = dbytes / avg_ms / 1e6
     log(f"[bench] dominant solve kernel {dk} level {dlev}: {avg_ms:.4f} ms/launch, "
         f"{dbytes/1e6:.2f} MB -> {achieved:.1f} GB/s, {dflops/avg_ms/1e9:.2f} TFLOP/s")
-    targs = "<double>" if dk != "k_ulv_gdown" else ("<double, 2, 2>" if nrhs > 32 else "<double, 4, 1>")
-    traffic = profiled_traffic(dk, targs, sweep_grid_threads(nodes, F64, dk, dlev, int(dnw), nrhs), 1)
+    pname, targs, pgrid = sweep_launch(nodes, F64, dk, dlev, int(dnw), nrhs)
+    traffic = profiled_traffic(pname, targs, pgrid, 1)
     mvleaf = [(key, v) for key, v in tot_m.items() if key[0] == "k_mv_leaf"]
     mv_leaf_ms = mvleaf[0][1][0] / mvleaf[0][1][1]
     mv_leaf_bytes = matvec_leaf_bytes(nodes, nrhs)

@@ -1,20 +1,25 @@
 // hss_kernels.hip -- gfx950 kernels of libhss_hip.so.
 //
-// Matvec (reference src/matmul.jl:18-62) as level-synchronous sweeps:
-//   k_mv_up      Z_leaf = V' B_seg ; Z = W1' Z1 + W2' Z2        (_matmatup, matmul.jl:32-42)
-//   k_mv_down    F1 = B12 Z2 + R1 F ; F2 = B21 Z1 + R2 F       (_matmatdown!, matmul.jl:51-57)
-//   k_mv_leaf    C_seg = alpha (D B_seg + U F) + beta C_seg      (matmul.jl:46-47)
+// Matvec (reference src/matmul.jl:18-62) as level-synchronous sweeps over per-node
+// operators, one MFMA product per node and step:
+//   k_mv_s<0> / k_mv_t<0>   Z = V' B_seg (leaf) | [W1' W2'] [Z1; Z2]     (_matmatup, matmul.jl:32-42)
+//   k_mv_s<1> / k_mv_t<1>   F = [B_sibling R] [Z_sibling; F_parent]      (_matmatdown!, matmul.jl:51-57)
+//   k_mv_s<2> / k_mv_t<2>   C_seg = alpha [D U] [B_seg; F] + beta C_seg  (matmul.jl:46-47)
 //
 // ULV (reference src/ulvfactor.jl:10-107), split into a resident factor and a solve:
 //   k_ulv_factor  per node: merge of the children (ulvfactor.jl:74-79), QL of U,
 //                 Q'D, LQ of the top rows with the reflectors applied to the
-//                 bottom rows and to V' (ulvfactor.jl:39-54); stores the
-//                 reflectors in compact-WY blocks for the solve
+//                 bottom rows and to V' (ulvfactor.jl:39-54)
+//   k_ulv_wy      compact-WY blocks of the LQ reflectors
 //   k_ulv_root    merge + LU with partial pivoting of the root block (ulvfactor.jl:82-86)
-//   k_ulv_sup     solve up-sweep: Q_ql' b, blocked forward substitution, b and u
-//                 updates (ulvfactor.jl:42,48-51,74,89)
-//   k_ulv_sroot   root LU solve (ulvfactor.jl:83)
-//   k_ulv_sdown   solve down-sweep: z <- Q_lq' z (ulvfactor.jl:98-107)
+//   k_ulv_rinv    the root block's inverse and merge operator
+//   k_ulv_gform / k_ulv_qform   explicit per-node solve operators (Gz, Gr, Qt), formed
+//                 by running the node's reduction on identity columns
+//   k_ulv_gup*    solve up-sweep: zloc = Gz x, [b; u] = Gr [x; zloc] (ulvfactor.jl:42,48-51,74,89)
+//   k_ulv_groot*  root solve (ulvfactor.jl:83)
+//   k_ulv_gdown*  solve down-sweep: z = Qt [zloc; z_parent] (ulvfactor.jl:98-107)
+// Sweep kernels come in three forms: _s with operator and input staged in LDS,
+// _s<.., false> with the operator streamed by the waves, and chunked tile kernels.
 //
 // Every kernel takes a list of (instance, node) work items, so one launch can
 // cover the same tree level of many HSS matrices.
@@ -30,6 +35,24 @@ namespace hss {
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
+// Device pointers read from argument tables are generic to the compiler, which then
+// emits flat loads and stores.  Flat operations also count against the LDS wait
+// counter, so a wait for an LDS read would wait for every outstanding global
+// prefetch.  Cast to the global address space, they become global_load / global_store.
+// (The host compilation pass only parses the device code; there the cast is a no-op.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+#else
+template <typename T>
+using gptr = T*;
+#endif
+template <typename T>
+__device__ __forceinline__ gptr<T> gbl(T* p) {
+    return (gptr<T>)p;
+}
+// 16 bytes moved as one unit by the staging copies
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 template <typename T>
 __device__ __forceinline__ T shfl_xor(T v, int mask) {
     return __shfl_xor(v, mask, 64);
@@ -158,10 +181,10 @@ __device__ __forceinline__ void stage(T* dst, int r0, const Tsrc* src, int64_t l
 // 16-byte aligned; may read up to 15 bytes past n.
 template <typename TF>
 __device__ __forceinline__ void lds_copy(TF* dst, const TF* __restrict__ src, int64_t n) {
-    const float4* s = reinterpret_cast<const float4*>(src);
-    float4* d = reinterpret_cast<float4*>(dst);
+    const gptr<const u32x4> s = gbl(reinterpret_cast<const u32x4*>(src));
+    u32x4* d = reinterpret_cast<u32x4*>(dst);
     const int nv = (int)((n * (int64_t)sizeof(TF) + 15) / 16);
-    batched<16, float4>(nv, [&](int v) { return s[v]; }, [&](int v, float4 x) { d[v] = x; });
+    batched<16, u32x4>(nv, [&](int v) { return s[v]; }, [&](int v, u32x4 x) { d[v] = x; });
 }
 
 // One round of loads for an LDS-resident step: the operator (dst[0, na) = src[0, na), as
@@ -169,17 +192,17 @@ __device__ __forceinline__ void lds_copy(TF* dst, const TF* __restrict__ src, in
 template <typename TF, typename Ld, typename St>
 __device__ __forceinline__ void stage2(TF* dst, const TF* __restrict__ src, int64_t na, int nx, Ld ld, St st) {
     constexpr int VA = 8, VX = 8;
-    const float4* s = reinterpret_cast<const float4*>(src);
-    float4* d = reinterpret_cast<float4*>(dst);
+    const gptr<const u32x4> s = gbl(reinterpret_cast<const u32x4*>(src));
+    u32x4* d = reinterpret_cast<u32x4*>(dst);
     const int nv = (int)((na * (int64_t)sizeof(TF) + 15) / 16);
     const int bd = blockDim.x;
     for (int v0 = threadIdx.x, e0 = threadIdx.x; v0 < nv || e0 < nx; v0 += VA * bd, e0 += VX * bd) {
-        float4 ra[VA];
+        u32x4 ra[VA];
         TF rx[VX];
 #pragma unroll
         for (int q = 0; q < VA; ++q) {
             const int v = v0 + q * bd;
-            ra[q] = v < nv ? s[v] : float4{};
+            ra[q] = v < nv ? s[v] : u32x4{0, 0, 0, 0};
         }
 #pragma unroll
         for (int q = 0; q < VX; ++q) {
@@ -1170,10 +1193,12 @@ constexpr int GK = 16;  // K chunk of the MFMA tile GEMM
 template <typename TF>
 __device__ __forceinline__ TF seg_load(const SegOp& x, const TF* zs, int zp, int r, int64_t j, int jl) {
     if (r < x.r0)
-        return x.dbl0 ? (TF)static_cast<const double*>(x.p0)[r + j * x.ld0] : static_cast<const TF*>(x.p0)[r + j * x.ld0];
+        return x.dbl0 ? (TF)gbl(static_cast<const double*>(x.p0))[r + j * x.ld0]
+                      : gbl(static_cast<const TF*>(x.p0))[r + j * x.ld0];
     if (r < x.r1) {
         r -= x.r0;
-        return x.dbl1 ? (TF)static_cast<const double*>(x.p1)[r + j * x.ld1] : static_cast<const TF*>(x.p1)[r + j * x.ld1];
+        return x.dbl1 ? (TF)gbl(static_cast<const double*>(x.p1))[r + j * x.ld1]
+                      : gbl(static_cast<const TF*>(x.p1))[r + j * x.ld1];
     }
     return zs[(r - x.r1) * zp + jl];
 }
@@ -1208,7 +1233,7 @@ __device__ __forceinline__ void tile_gemm(TileLds<TF, WM, WN>& L, int M, int N, 
             const int i = e % TM, kk = e / TM;
             const int gi = i0 + i, gk = k0 + kk;
             TF v = TF(0);
-            if (gi < M && gk < K) v = A ? A[gi + (int64_t)gk * lda] : (gi == gk ? TF(1) : TF(0));
+            if (gi < M && gk < K) v = A ? gbl(A)[gi + (int64_t)gk * lda] : (gi == gk ? TF(1) : TF(0));
             ra[q] = v;
         }
 #pragma unroll
@@ -1302,11 +1327,11 @@ __global__ __launch_bounds__(256) void k_ulv_gup(const SInst<TF>* __restrict__ i
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
     const WorkItem wi = work[item];
     const SInst<TF>& I = insts[wi.inst];
-    const FNode f = I.fn[wi.node];
+    const FNode f = gbl(I.fn)[wi.node];
     const int nrhs = I.nrhs;
     const int j0 = tj * TN;
     if (j0 >= nrhs) return;
-    const DevNode nd = I.nodes[wi.node];
+    const DevNode nd = gbl(I.nodes)[wi.node];
     extern __shared__ unsigned char lds_raw[];
     TF* zs = reinterpret_cast<TF*>(lds_raw);
     SegOp x;
@@ -1319,7 +1344,7 @@ __global__ __launch_bounds__(256) void k_ulv_gup(const SInst<TF>* __restrict__ i
         x.ld1 = 0;
         x.dbl1 = 0;
     } else {
-        const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
+        const FNode fl = gbl(I.fn)[nd.left], fr = gbl(I.fn)[nd.right];
         x.p0 = I.wy + fl.yrow * nrhs + fl.nk;
         x.ld0 = fl.My;
         x.dbl0 = 0;
@@ -1329,14 +1354,14 @@ __global__ __launch_bounds__(256) void k_ulv_gup(const SInst<TF>* __restrict__ i
         x.dbl1 = 0;
     }
     x.r1 = f.Kx;
-    TF* y = I.wy + f.yrow * nrhs;
+    const gptr<TF> y = gbl(I.wy + f.yrow * nrhs);
     const int My = f.My, nk = f.nk, Kx = f.Kx, R = My - nk;
     __shared__ TileLds<TF, WM, WN> L;
     // phase 0 rows [0, nk) of y from Gz, phase 1 rows [nk, My) from Gr
     for (int ph = 0; ph < 2; ++ph) {
         const int M = ph ? R : nk, K = ph ? Kx + nk : Kx;
         const TF* A = I.fac + (ph ? f.Gr : f.Gz);
-        TF* yo = y + (ph ? nk : 0);
+        const gptr<TF> yo = y + (ph ? nk : 0);
         for (int p0 = 0; p0 < M; p0 += TM)
             tile_gemm<TF, WM, WN>(L, M, nrhs, K, A, M, x, zs, ZP, p0, j0, [&](int i, int j, TF v) {
                 yo[i + (int64_t)j * My] = v;
@@ -1355,13 +1380,13 @@ __global__ __launch_bounds__(256) void k_ulv_gdown(const SInst<TF>* __restrict__
     if (!tile_of(nwork * tm * tn, tm, tn, item, ti, tj)) return;
     const WorkItem wi = work[item];
     const SInst<TF>& I = insts[wi.inst];
-    const FNode f = I.fn[wi.node];
+    const FNode f = gbl(I.fn)[wi.node];
     const int nrhs = I.nrhs, n = f.n;
     const int i0 = ti * 32 * WM, j0 = tj * 32 * WN;
     if (i0 >= n || j0 >= nrhs) return;
-    const DevNode nd = I.nodes[wi.node];
-    const FNode fp = I.fn[nd.parent];
-    const int off = nd.is_left ? 0 : I.fn[I.nodes[nd.parent].left].nr;
+    const DevNode nd = gbl(I.nodes)[wi.node];
+    const FNode fp = gbl(I.fn)[nd.parent];
+    const int off = nd.is_left ? 0 : gbl(I.fn)[gbl(I.nodes)[nd.parent].left].nr;
     SegOp x;
     x.p0 = I.wy + f.yrow * nrhs;
     x.ld0 = f.My;
@@ -1374,9 +1399,9 @@ __global__ __launch_bounds__(256) void k_ulv_gdown(const SInst<TF>* __restrict__
     const TF* Q = f.Qt >= 0 ? I.fac + f.Qt : nullptr;
     __shared__ TileLds<TF, WM, WN> L;
     const bool leaf = nd.is_leaf;
-    double* X = I.X + nd.col_off;
+    const gptr<double> X = gbl(I.X + nd.col_off);
     const int64_t ldx = I.ldx;
-    TF* z = I.wz + f.zrow * nrhs;
+    const gptr<TF> z = gbl(I.wz + f.zrow * nrhs);
     tile_gemm<TF, WM, WN>(L, n, nrhs, n, Q, n, x, nullptr, 0, i0, j0, [&](int i, int j, TF v) {
         if (leaf) X[i + (int64_t)j * ldx] = (double)v;
         else z[i + (int64_t)j * n] = v;
@@ -1423,51 +1448,116 @@ __device__ __forceinline__ void lds_gemm(int M, int K, const TF* As, const TF* x
 }
 
 // The same product with A read from memory (column-major, lda), for operands used
-// once per workgroup: each A element feeds exactly one wave, so the waves load their
-// own A fragments straight into registers, U k-steps ahead of the MFMAs.
-template <typename TF, typename Store>
-__device__ __forceinline__ void glb_gemm(int M, int K, const TF* __restrict__ A, int64_t lda, const TF* xs, int ncols,
-                                         Store store, int xp = XS) {
+// once per workgroup.  Each wave owns 16-row strips of the output across all (one or
+// two) 16-column blocks, so every A element is loaded by exactly one wave, straight
+// into registers U k-steps ahead of the MFMAs, and feeds one MFMA per column block.
+// Rows past M read row M - 1 (in bounds) and are never stored.
+template <bool two, typename TF, typename Store>
+__device__ __forceinline__ void glb_gemm_cb(int M, int K, const TF* __restrict__ A, int64_t lda, const TF* xs,
+                                            int ncols, Store store, int xp) {
     using V4 = typename Mfma<TF>::v4;
-    constexpr int U = 6;
+    constexpr int U = 8;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int lr = lane & 15, lk = lane >> 4;
-    const int nrb = (M + 15) / 16, ncb = (ncols + 15) / 16;
+    const int nrb = (M + 15) / 16;
     const int S = (K + 3) / 4;  // k-steps
-    for (int t = wave; t < nrb * ncb; t += nw) {
-        const int rb = t % nrb, cb = t / nrb;
-        const int i = rb * 16 + lr, jc = cb * 16 + lr;
-        const TF* Ai = A + (i < M ? i : 0);
+    if (K <= 0) {
+        for (int e = threadIdx.x; e < M * ncols; e += blockDim.x) store(e % M, e / M, TF(0));
+        return;
+    }
+    for (int rb = wave; rb < nrb; rb += nw) {
+        const int i = rb * 16 + lr;
+        const gptr<const TF> Ai = gbl(A) + (i < M ? i : M - 1);
+        // Prefetch loads are unconditional (steps past S and columns past K are clamped
+        // in bounds), so no wait is forced until a value is used; k past K is zeroed at
+        // the MFMA.
         auto ld = [&](int st) {
-            const int kk = 4 * st + lk;
-            return (i < M && kk < K) ? Ai[(int64_t)kk * lda] : TF(0);
+            const int kk = 4 * (st < S ? st : S - 1) + lk;
+            return Ai[(int64_t)(kk < K ? kk : K - 1) * lda];
         };
         V4 acc0 = V4{TF(0), TF(0), TF(0), TF(0)}, acc1 = acc0;
         TF a[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) a[u] = ld(u);
-        for (int s0 = 0; s0 < S; s0 += U) {
-            TF an[U];
+        int s0 = 0;
+        // blocks of U k-steps that lie entirely inside K (no clamping or zeroing); the
+        // two register sets alternate, so no prefetched value is copied (a copy would
+        // wait for its load)
+        auto blk = [&](TF(&cur)[U], TF(&nxt)[U], int b0s) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) an[u] = (s0 + U + u < S) ? ld(s0 + U + u) : TF(0);
+            for (int u = 0; u < U; ++u) nxt[u] = ld(b0s + U + u);
+            const TF* xb = xs + (4 * b0s + lk) * xp + lr;
+            TF b0[U], b1[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int kk = 4 * (s0 + u) + lk;
-                const TF b = kk < K ? xs[kk * xp + jc] : TF(0);
-                if (u & 1) acc1 = Mfma<TF>::mac(a[u], b, acc1);
-                else acc0 = Mfma<TF>::mac(a[u], b, acc0);
+                b0[u] = xb[4 * u * xp];
+                if (two) b1[u] = xb[4 * u * xp + 16];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (two) {
+                    acc0 = Mfma<TF>::mac(cur[u], b0[u], acc0);
+                    acc1 = Mfma<TF>::mac(cur[u], b1[u], acc1);
+                } else if (u & 1) {
+                    acc1 = Mfma<TF>::mac(cur[u], b0[u], acc1);
+                } else {
+                    acc0 = Mfma<TF>::mac(cur[u], b0[u], acc0);
+                }
+            }
+        };
+        const int Sf = K / 4;
+        TF an[U];
+        for (; s0 + 2 * U <= Sf; s0 += 2 * U) {
+            blk(a, an, s0);
+            blk(an, a, s0 + U);
+        }
+        if (s0 + U <= Sf) {
+            blk(a, an, s0);
+            s0 += U;
+#pragma unroll
+            for (int u = 0; u < U; ++u) a[u] = an[u];
+        }
+        for (; s0 < S; s0 += U) {
+            TF an[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) an[u] = ld(s0 + U + u);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (s0 + u < S) {
+                    const int kk = 4 * (s0 + u) + lk;
+                    const TF av = kk < K ? a[u] : TF(0);
+                    const TF* xk = xs + (kk < K ? kk : K - 1) * xp + lr;
+                    if (two) {
+                        acc0 = Mfma<TF>::mac(av, xk[0], acc0);
+                        acc1 = Mfma<TF>::mac(av, xk[16], acc1);
+                    } else if (u & 1) {
+                        acc1 = Mfma<TF>::mac(av, xk[0], acc1);
+                    } else {
+                        acc0 = Mfma<TF>::mac(av, xk[0], acc0);
+                    }
+                }
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) a[u] = an[u];
         }
-        if (jc < ncols) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int ii = rb * 16 + Mfma<TF>::row(lk, r);
-                if (ii < M) store(ii, jc, acc0[r] + acc1[r]);
+        for (int r = 0; r < 4; ++r) {
+            const int ii = rb * 16 + Mfma<TF>::row(lk, r);
+            if (ii >= M) continue;
+            if (two) {
+                store(ii, lr, acc0[r]);
+                if (16 + lr < ncols) store(ii, 16 + lr, acc1[r]);
+            } else if (lr < ncols) {
+                store(ii, lr, acc0[r] + acc1[r]);
             }
         }
     }
+}
+template <typename TF, typename Store>
+__device__ __forceinline__ void glb_gemm(int M, int K, const TF* __restrict__ A, int64_t lda, const TF* xs, int ncols,
+                                         Store store, int xp = XS) {
+    if (ncols > 16) glb_gemm_cb<true>(M, K, A, lda, xs, ncols, store, xp);
+    else glb_gemm_cb<false>(M, K, A, lda, xs, ncols, store, xp);
 }
 
 // Up-sweep with the input staged in LDS; tw = columns per workgroup (16 or 32), xs pitch
@@ -1481,13 +1571,13 @@ __global__ __launch_bounds__(512, 4) void k_ulv_gup_s(const SInst<TF>* __restric
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
     const WorkItem wi = work[item];
     const SInst<TF>& I = insts[wi.inst];
-    const FNode f = I.fn[wi.node];
+    const FNode f = gbl(I.fn)[wi.node];
     const int nrhs = I.nrhs;
     const int j0 = tj * tw;
     if (j0 >= nrhs) return;
     const int nc = min(tw, nrhs - j0);
     const int xp = tw + 1;  // LDS pitch of xs
-    const DevNode nd = I.nodes[wi.node];
+    const DevNode nd = gbl(I.nodes)[wi.node];
     const int nk = f.nk, Kx = f.Kx, My = f.My, R = My - nk;
     const int64_t na = (int64_t)nk * Kx + (int64_t)R * (Kx + nk);
     extern __shared__ unsigned char lds_raw[];
@@ -1499,16 +1589,16 @@ __global__ __launch_bounds__(512, 4) void k_ulv_gup_s(const SInst<TF>* __restric
     const TF* Ag = I.fac + (nk > 0 ? f.Gz : f.Gr);
     auto put = [&](int e, TF v) { xs[(e % Kx) * xp + e / Kx] = v; };
     if (nd.is_leaf) {
-        const double* B = I.B + nd.row_off + (int64_t)j0 * I.ldb;
+        const gptr<const double> B = gbl(I.B + nd.row_off + (int64_t)j0 * I.ldb);
         const int64_t ldb = I.ldb;
         stage2(
             As, Ag, nst, Kx * tw, [&](int e) { return e / Kx < nc ? (TF)B[(e % Kx) + (int64_t)(e / Kx) * ldb] : TF(0); },
             put);
     } else {
-        const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
+        const FNode fl = gbl(I.fn)[nd.left], fr = gbl(I.fn)[nd.right];
         const int r0 = fl.My - fl.nk;
-        const TF* y0 = I.wy + fl.yrow * nrhs + fl.nk + (int64_t)j0 * fl.My;
-        const TF* y1 = I.wy + fr.yrow * nrhs + fr.nk + (int64_t)j0 * fr.My;
+        const gptr<const TF> y0 = gbl(I.wy + fl.yrow * nrhs + fl.nk + (int64_t)j0 * fl.My);
+        const gptr<const TF> y1 = gbl(I.wy + fr.yrow * nrhs + fr.nk + (int64_t)j0 * fr.My);
         const int l0 = fl.My, l1 = fr.My;
         stage2(
             As, Ag, nst, Kx * tw,
@@ -1520,7 +1610,7 @@ __global__ __launch_bounds__(512, 4) void k_ulv_gup_s(const SInst<TF>* __restric
             put);
     }
     __syncthreads();
-    TF* y = I.wy + f.yrow * nrhs + (int64_t)j0 * My;
+    const gptr<TF> y = gbl(I.wy + f.yrow * nrhs + (int64_t)j0 * My);
     // zloc = Gz x  -> y rows [0, nk) and xs rows [Kx, Kx + nk)
     auto st0 = [&](int i, int j, TF v) {
         y[i + (int64_t)j * My] = v;
@@ -1544,22 +1634,22 @@ __global__ __launch_bounds__(512, 4) void k_ulv_gdown_s(const SInst<TF>* __restr
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
     const WorkItem wi = work[item];
     const SInst<TF>& I = insts[wi.inst];
-    const FNode f = I.fn[wi.node];
+    const FNode f = gbl(I.fn)[wi.node];
     const int nrhs = I.nrhs, n = f.n, nk = f.nk;
     const int j0 = tj * tw;
     if (j0 >= nrhs) return;
     const int nc = min(tw, nrhs - j0);
     const int xp = tw + 1;  // LDS pitch of xs
-    const DevNode nd = I.nodes[wi.node];
-    const FNode fp = I.fn[nd.parent];
-    const int off = nd.is_left ? 0 : I.fn[I.nodes[nd.parent].left].nr;
+    const DevNode nd = gbl(I.nodes)[wi.node];
+    const FNode fp = gbl(I.fn)[nd.parent];
+    const int off = nd.is_left ? 0 : gbl(I.fn)[gbl(I.nodes)[nd.parent].left].nr;
     const int64_t na = f.Qt >= 0 ? (int64_t)n * n : 0;
     extern __shared__ unsigned char lds_raw[];
     TF* As = reinterpret_cast<TF*>(lds_raw);
     TF* xs = AR ? As + ((na + 15) & ~int64_t(15)) : As;
     // Qt and x = [zloc (rows 0..nk-1 of the node's y); z from the parent] in one round
-    const TF* y0 = I.wy + f.yrow * nrhs + (int64_t)j0 * f.My;
-    const TF* z1 = I.wz + fp.zrow * nrhs + off + (int64_t)j0 * fp.n;
+    const gptr<const TF> y0 = gbl(I.wy + f.yrow * nrhs + (int64_t)j0 * f.My);
+    const gptr<const TF> z1 = gbl(I.wz + fp.zrow * nrhs + off + (int64_t)j0 * fp.n);
     const int l0 = f.My, l1 = fp.n;
     stage2(
         As, na > 0 ? I.fac + f.Qt : I.fac, AR ? na : 0, n * tw,
@@ -1571,9 +1661,9 @@ __global__ __launch_bounds__(512, 4) void k_ulv_gdown_s(const SInst<TF>* __restr
         [&](int e, TF v) { xs[(e % n) * xp + e / n] = v; });
     __syncthreads();
     const bool leaf = nd.is_leaf;
-    double* X = I.X + nd.col_off + (int64_t)j0 * I.ldx;
+    const gptr<double> X = gbl(I.X + nd.col_off + (int64_t)j0 * I.ldx);
     const int64_t ldx = I.ldx;
-    TF* z = I.wz + f.zrow * nrhs + (int64_t)j0 * n;
+    const gptr<TF> z = gbl(I.wz + f.zrow * nrhs + (int64_t)j0 * n);
     auto store = [&](int i, int j, TF v) {
         if (leaf) X[i + (int64_t)j * ldx] = (double)v;
         else z[i + (int64_t)j * n] = v;
@@ -1600,18 +1690,18 @@ __global__ __launch_bounds__(512) void k_ulv_groot2(const SInst<TF>* __restrict_
     const int j0 = tj * 32;
     if (j0 >= nrhs) return;
     const int nc = min(32, nrhs - j0);
-    const DevNode nd = I.nodes[wi.node];
-    const FNode f = I.fn[wi.node];
+    const DevNode nd = gbl(I.nodes)[wi.node];
+    const FNode f = gbl(I.fn)[wi.node];
     const int m = f.m, Kx = f.Kx;
     extern __shared__ unsigned char lds_raw[];
     TF* Di = reinterpret_cast<TF*>(lds_raw);
     TF* xs = Di + (((int64_t)m * m + 15) & ~int64_t(15));
     TF* bm = xs + max(Kx, 2 * m) * XS;
     lds_copy(Di, I.fac + f.Dinv, (int64_t)m * m);
-    const FNode fl = I.fn[nd.left], fr = I.fn[nd.right];
+    const FNode fl = gbl(I.fn)[nd.left], fr = gbl(I.fn)[nd.right];
     const int r0 = fl.My - fl.nk;
-    const TF* y0 = I.wy + fl.yrow * nrhs + fl.nk + (int64_t)j0 * fl.My;
-    const TF* y1 = I.wy + fr.yrow * nrhs + fr.nk + (int64_t)j0 * fr.My;
+    const gptr<const TF> y0 = gbl(I.wy + fl.yrow * nrhs + fl.nk + (int64_t)j0 * fl.My);
+    const gptr<const TF> y1 = gbl(I.wy + fr.yrow * nrhs + fr.nk + (int64_t)j0 * fr.My);
     const int l0 = fl.My, l1 = fr.My;
     batched<16, TF>(
         Kx * 32,
@@ -1629,7 +1719,7 @@ __global__ __launch_bounds__(512) void k_ulv_groot2(const SInst<TF>* __restrict_
     glb_gemm(m, m, I.fac + f.Dr, m, xs, nc,
              [&](int i, int j, TF v) { xs[(m + i) * XS + j] = bm[i * XS + j] - v; });  // r
     __syncthreads();
-    TF* z = I.wz + f.zrow * nrhs + (int64_t)j0 * f.n;
+    const gptr<TF> z = gbl(I.wz + f.zrow * nrhs + (int64_t)j0 * f.n);
     lds_gemm(m, m, Di, xs + m * XS, nc, [&](int i, int j, TF v) { z[i + (int64_t)j * f.n] = xs[i * XS + j] + v; });
 }
 
@@ -1650,7 +1740,7 @@ struct MvOp {
 
 template <int KIND>
 __device__ __forceinline__ void mv_op(const MvInst& I, int node, MvOp& o) {
-    const DevNode nd = I.nodes[node];
+    const DevNode nd = gbl(I.nodes)[node];
     const int nrhs = I.nrhs;
     o.x.dbl0 = o.x.dbl1 = 1;
     o.x.p1 = nullptr;
@@ -1664,7 +1754,7 @@ __device__ __forceinline__ void mv_op(const MvInst& I, int node, MvOp& o) {
             o.x.ld0 = I.ldb;
             o.x.r0 = nd.n;
         } else {
-            const DevNode L = I.nodes[nd.left], R = I.nodes[nd.right];
+            const DevNode L = gbl(I.nodes)[nd.left], R = gbl(I.nodes)[nd.right];
             o.K = L.kv + R.kv;
             o.x.p0 = I.Z + L.zrow * nrhs;
             o.x.ld0 = L.kv > 0 ? L.kv : 1;
@@ -1675,8 +1765,8 @@ __device__ __forceinline__ void mv_op(const MvInst& I, int node, MvOp& o) {
         o.y = I.Z + nd.zrow * nrhs;
         o.ldy = nd.kv;
     } else if (KIND == 1) {
-        const DevNode p = I.nodes[nd.parent];
-        const DevNode sb = I.nodes[nd.is_left ? p.right : p.left];
+        const DevNode p = gbl(I.nodes)[nd.parent];
+        const DevNode sb = gbl(I.nodes)[nd.is_left ? p.right : p.left];
         o.A = I.op + nd.opF;
         o.M = nd.ku;
         o.K = sb.kv + p.ku;
@@ -1704,7 +1794,7 @@ __device__ __forceinline__ void mv_op(const MvInst& I, int node, MvOp& o) {
 
 template <int KIND>
 __device__ __forceinline__ void mv_store(const MvInst& I, const MvOp& o, int i, int j, double v) {
-    double* c = o.y + i + (int64_t)j * o.ldy;
+    const gptr<double> c = gbl(o.y + i + (int64_t)j * o.ldy);
     if (KIND == 2) *c = (I.beta == 0.0) ? I.alpha * v : I.alpha * v + I.beta * (*c);
     else *c = v;
 }

@@ -167,11 +167,10 @@ static bool pick_resident(size_t w, int64_t na, int64_t k, int count, int maxnrh
     }
     return false;
 }
-// Otherwise operators whose rows fill few 128-row tiles (rows < 96) are streamed by the
-// waves with only the input staged (at most 80 KiB, two workgroups per CU); taller ones go
-// through the chunked tile kernels.
+// Otherwise the operators are streamed by the waves with only the input staged (at most
+// 80 KiB, two workgroups per CU); larger inputs go through the chunked tile kernels.
 static bool pick_streamed(size_t w, int64_t k, int rows, int count, int maxnrhs, int& tw, size_t& lds) {
-    if (rows >= 96) return false;
+    (void)rows;
     for (int t : {resident_tw(count, maxnrhs), 16}) {
         lds = w * (size_t)(k * (t + 1));
         if (lds <= 80 * 1024) {
