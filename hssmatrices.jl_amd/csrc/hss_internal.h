@@ -76,6 +76,25 @@ struct WorkItem {
     int32_t inst, node;
 };
 
+// One work item of a solve sweep with the node's sizes and row offsets resolved on the
+// host, so a workgroup reads this record and its instance before loading data (instead
+// of walking node, child and parent tables).  Row bases (y, x0, x1) are in units of
+// nrhs elements (the workspace blocks are nrhs columns wide).
+//   up (k_ulv_gup*):  K = Kx; A = Gz (Gr when nk == 0); y = own yrow;
+//                     leaf: x0 = row_off into B; branch: x = rows [add0, add0 + r0) of the
+//                     first child's y (x0 = its yrow, pitch l0), then rows from add1 of the
+//                     second child's y (x1 = its yrow, pitch l1)
+//   down (k_ulv_gdown*): K = n; A = Qt (-1 = identity); x0 = own yrow (zloc, pitch My);
+//                     x1 = parent's zrow, rows from add1, pitch l1 = the parent's n;
+//                     y = own zrow, or col_off into X at a leaf
+//   root: inst and node only
+struct SweepRec {
+    int32_t inst, node, leaf, nk;
+    int32_t K, My, r0, l0;
+    int32_t l1, add0, add1, pad;
+    int64_t A, y, x0, x1;
+};
+
 // Per-instance arguments of one matvec launch sequence.
 struct MvInst {
     const DevNode* nodes;

@@ -1057,8 +1057,8 @@ __device__ __forceinline__ void lu_solve(TF* bv, TF* t1, int m, int nt, const TF
 // small dependent steps read LDS rather than memory)
 template <typename TF>
 __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_groot(const SInst<TF>* __restrict__ insts,
-                                                            const WorkItem* __restrict__ work, int resident) {
-    const WorkItem wi = work[blockIdx.x];
+                                                            const SweepRec* __restrict__ work, int resident) {
+    const SweepRec wi = work[blockIdx.x];
     const SInst<TF>& I = insts[wi.inst];
     const int j0 = blockIdx.y * RHS_TILE;
     if (j0 >= I.nrhs) return;
@@ -1321,46 +1321,43 @@ __device__ __forceinline__ bool tile_of(int total, int tm, int tn, int& item, in
 // Dynamic LDS: zloc, (max nk of the level) x (TN + 1).
 template <typename TF>
 __global__ __launch_bounds__(256) void k_ulv_gup(const SInst<TF>* __restrict__ insts,
-                                                 const WorkItem* __restrict__ work, int nwork, int tn) {
+                                                 const SweepRec* __restrict__ work, int nwork, int tn) {
     constexpr int WM = 4, WN = 1, TM = 32 * WM, TN = 32 * WN, ZP = TN + 1;
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
-    const WorkItem wi = work[item];
-    const SInst<TF>& I = insts[wi.inst];
-    const FNode f = gbl(I.fn)[wi.node];
+    const SweepRec rc = work[item];
+    const SInst<TF>& I = insts[rc.inst];
     const int nrhs = I.nrhs;
     const int j0 = tj * TN;
     if (j0 >= nrhs) return;
-    const DevNode nd = gbl(I.nodes)[wi.node];
     extern __shared__ unsigned char lds_raw[];
     TF* zs = reinterpret_cast<TF*>(lds_raw);
+    const int My = rc.My, nk = rc.nk, Kx = rc.K, R = My - nk;
     SegOp x;
-    if (nd.is_leaf) {
-        x.p0 = I.B + nd.row_off;
+    if (rc.leaf) {
+        x.p0 = I.B + rc.x0;
         x.ld0 = I.ldb;
         x.dbl0 = 1;
-        x.r0 = f.Kx;
+        x.r0 = Kx;
         x.p1 = nullptr;
         x.ld1 = 0;
         x.dbl1 = 0;
     } else {
-        const FNode fl = gbl(I.fn)[nd.left], fr = gbl(I.fn)[nd.right];
-        x.p0 = I.wy + fl.yrow * nrhs + fl.nk;
-        x.ld0 = fl.My;
+        x.p0 = I.wy + rc.x0 * nrhs + rc.add0;
+        x.ld0 = rc.l0;
         x.dbl0 = 0;
-        x.r0 = fl.My - fl.nk;
-        x.p1 = I.wy + fr.yrow * nrhs + fr.nk;
-        x.ld1 = fr.My;
+        x.r0 = rc.r0;
+        x.p1 = I.wy + rc.x1 * nrhs + rc.add1;
+        x.ld1 = rc.l1;
         x.dbl1 = 0;
     }
-    x.r1 = f.Kx;
-    const gptr<TF> y = gbl(I.wy + f.yrow * nrhs);
-    const int My = f.My, nk = f.nk, Kx = f.Kx, R = My - nk;
+    x.r1 = Kx;
+    const gptr<TF> y = gbl(I.wy + rc.y * nrhs);
     __shared__ TileLds<TF, WM, WN> L;
-    // phase 0 rows [0, nk) of y from Gz, phase 1 rows [nk, My) from Gr
+    // phase 0 rows [0, nk) of y from Gz, phase 1 rows [nk, My) from Gr (stored after Gz)
     for (int ph = 0; ph < 2; ++ph) {
         const int M = ph ? R : nk, K = ph ? Kx + nk : Kx;
-        const TF* A = I.fac + (ph ? f.Gr : f.Gz);
+        const TF* A = I.fac + rc.A + (ph ? (int64_t)nk * Kx : 0);
         const gptr<TF> yo = y + (ph ? nk : 0);
         for (int p0 = 0; p0 < M; p0 += TM)
             tile_gemm<TF, WM, WN>(L, M, nrhs, K, A, M, x, zs, ZP, p0, j0, [&](int i, int j, TF v) {
@@ -1375,33 +1372,29 @@ __global__ __launch_bounds__(256) void k_ulv_gup(const SInst<TF>* __restrict__ i
 // leaves write the solution X.
 template <typename TF, int WM, int WN>
 __global__ __launch_bounds__(256) void k_ulv_gdown(const SInst<TF>* __restrict__ insts,
-                                                   const WorkItem* __restrict__ work, int nwork, int tm, int tn) {
+                                                   const SweepRec* __restrict__ work, int nwork, int tm, int tn) {
     int item, ti, tj;
     if (!tile_of(nwork * tm * tn, tm, tn, item, ti, tj)) return;
-    const WorkItem wi = work[item];
-    const SInst<TF>& I = insts[wi.inst];
-    const FNode f = gbl(I.fn)[wi.node];
-    const int nrhs = I.nrhs, n = f.n;
+    const SweepRec rc = work[item];
+    const SInst<TF>& I = insts[rc.inst];
+    const int nrhs = I.nrhs, n = rc.K;
     const int i0 = ti * 32 * WM, j0 = tj * 32 * WN;
     if (i0 >= n || j0 >= nrhs) return;
-    const DevNode nd = gbl(I.nodes)[wi.node];
-    const FNode fp = gbl(I.fn)[nd.parent];
-    const int off = nd.is_left ? 0 : gbl(I.fn)[gbl(I.nodes)[nd.parent].left].nr;
     SegOp x;
-    x.p0 = I.wy + f.yrow * nrhs;
-    x.ld0 = f.My;
+    x.p0 = I.wy + rc.x0 * nrhs;
+    x.ld0 = rc.My;
     x.dbl0 = 0;
-    x.r0 = f.nk;
-    x.p1 = I.wz + fp.zrow * nrhs + off;
-    x.ld1 = fp.n;
+    x.r0 = rc.nk;
+    x.p1 = I.wz + rc.x1 * nrhs + rc.add1;
+    x.ld1 = rc.l1;
     x.dbl1 = 0;
     x.r1 = n;
-    const TF* Q = f.Qt >= 0 ? I.fac + f.Qt : nullptr;
+    const TF* Q = rc.A >= 0 ? I.fac + rc.A : nullptr;
     __shared__ TileLds<TF, WM, WN> L;
-    const bool leaf = nd.is_leaf;
-    const gptr<double> X = gbl(I.X + nd.col_off);
+    const bool leaf = rc.leaf;
+    const gptr<double> X = gbl(I.X + rc.y);
     const int64_t ldx = I.ldx;
-    const gptr<TF> z = gbl(I.wz + f.zrow * nrhs);
+    const gptr<TF> z = gbl(I.wz + rc.y * nrhs);
     tile_gemm<TF, WM, WN>(L, n, nrhs, n, Q, n, x, nullptr, 0, i0, j0, [&](int i, int j, TF v) {
         if (leaf) X[i + (int64_t)j * ldx] = (double)v;
         else z[i + (int64_t)j * n] = v;
@@ -1566,19 +1559,17 @@ __device__ __forceinline__ void glb_gemm(int M, int K, const TF* __restrict__ A,
 // (LDS = xs), for operators too large to stage whose rows fill few 128-row tiles.
 template <typename TF, bool AR>
 __global__ __launch_bounds__(512, 4) void k_ulv_gup_s(const SInst<TF>* __restrict__ insts,
-                                                   const WorkItem* __restrict__ work, int nwork, int tn, int tw) {
+                                                   const SweepRec* __restrict__ work, int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
-    const WorkItem wi = work[item];
-    const SInst<TF>& I = insts[wi.inst];
-    const FNode f = gbl(I.fn)[wi.node];
+    const SweepRec rc = work[item];
+    const SInst<TF>& I = insts[rc.inst];
     const int nrhs = I.nrhs;
     const int j0 = tj * tw;
     if (j0 >= nrhs) return;
     const int nc = min(tw, nrhs - j0);
     const int xp = tw + 1;  // LDS pitch of xs
-    const DevNode nd = gbl(I.nodes)[wi.node];
-    const int nk = f.nk, Kx = f.Kx, My = f.My, R = My - nk;
+    const int nk = rc.nk, Kx = rc.K, My = rc.My, R = My - nk;
     const int64_t na = (int64_t)nk * Kx + (int64_t)R * (Kx + nk);
     extern __shared__ unsigned char lds_raw[];
     TF* As = reinterpret_cast<TF*>(lds_raw);
@@ -1586,20 +1577,18 @@ __global__ __launch_bounds__(512, 4) void k_ulv_gup_s(const SInst<TF>* __restric
     const int64_t nst = AR ? na : 0;  // operator elements staged
     // operator and x rows [0, Kx) (the leaf's rows of B, or the children's reduced b and u)
     // in one round of loads; columns past nc are zero
-    const TF* Ag = I.fac + (nk > 0 ? f.Gz : f.Gr);
+    const TF* Ag = I.fac + rc.A;  // Gz, then Gr
     auto put = [&](int e, TF v) { xs[(e % Kx) * xp + e / Kx] = v; };
-    if (nd.is_leaf) {
-        const gptr<const double> B = gbl(I.B + nd.row_off + (int64_t)j0 * I.ldb);
+    if (rc.leaf) {
+        const gptr<const double> B = gbl(I.B + rc.x0 + (int64_t)j0 * I.ldb);
         const int64_t ldb = I.ldb;
         stage2(
             As, Ag, nst, Kx * tw, [&](int e) { return e / Kx < nc ? (TF)B[(e % Kx) + (int64_t)(e / Kx) * ldb] : TF(0); },
             put);
     } else {
-        const FNode fl = gbl(I.fn)[nd.left], fr = gbl(I.fn)[nd.right];
-        const int r0 = fl.My - fl.nk;
-        const gptr<const TF> y0 = gbl(I.wy + fl.yrow * nrhs + fl.nk + (int64_t)j0 * fl.My);
-        const gptr<const TF> y1 = gbl(I.wy + fr.yrow * nrhs + fr.nk + (int64_t)j0 * fr.My);
-        const int l0 = fl.My, l1 = fr.My;
+        const int r0 = rc.r0, l0 = rc.l0, l1 = rc.l1;
+        const gptr<const TF> y0 = gbl(I.wy + rc.x0 * nrhs + rc.add0 + (int64_t)j0 * l0);
+        const gptr<const TF> y1 = gbl(I.wy + rc.x1 * nrhs + rc.add1 + (int64_t)j0 * l1);
         stage2(
             As, Ag, nst, Kx * tw,
             [&](int e) {
@@ -1610,7 +1599,7 @@ __global__ __launch_bounds__(512, 4) void k_ulv_gup_s(const SInst<TF>* __restric
             put);
     }
     __syncthreads();
-    const gptr<TF> y = gbl(I.wy + f.yrow * nrhs + (int64_t)j0 * My);
+    const gptr<TF> y = gbl(I.wy + rc.y * nrhs + (int64_t)j0 * My);
     // zloc = Gz x  -> y rows [0, nk) and xs rows [Kx, Kx + nk)
     auto st0 = [&](int i, int j, TF v) {
         y[i + (int64_t)j * My] = v;
@@ -1629,30 +1618,26 @@ __global__ __launch_bounds__(512, 4) void k_ulv_gup_s(const SInst<TF>* __restric
 // rounded to 16; empty for a pass-through node) | xs n rows); otherwise streamed.
 template <typename TF, bool AR>
 __global__ __launch_bounds__(512, 4) void k_ulv_gdown_s(const SInst<TF>* __restrict__ insts,
-                                                     const WorkItem* __restrict__ work, int nwork, int tn, int tw) {
+                                                     const SweepRec* __restrict__ work, int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
-    const WorkItem wi = work[item];
-    const SInst<TF>& I = insts[wi.inst];
-    const FNode f = gbl(I.fn)[wi.node];
-    const int nrhs = I.nrhs, n = f.n, nk = f.nk;
+    const SweepRec rc = work[item];
+    const SInst<TF>& I = insts[rc.inst];
+    const int nrhs = I.nrhs, n = rc.K, nk = rc.nk;
     const int j0 = tj * tw;
     if (j0 >= nrhs) return;
     const int nc = min(tw, nrhs - j0);
     const int xp = tw + 1;  // LDS pitch of xs
-    const DevNode nd = gbl(I.nodes)[wi.node];
-    const FNode fp = gbl(I.fn)[nd.parent];
-    const int off = nd.is_left ? 0 : gbl(I.fn)[gbl(I.nodes)[nd.parent].left].nr;
-    const int64_t na = f.Qt >= 0 ? (int64_t)n * n : 0;
+    const int64_t na = rc.A >= 0 ? (int64_t)n * n : 0;
     extern __shared__ unsigned char lds_raw[];
     TF* As = reinterpret_cast<TF*>(lds_raw);
     TF* xs = AR ? As + ((na + 15) & ~int64_t(15)) : As;
     // Qt and x = [zloc (rows 0..nk-1 of the node's y); z from the parent] in one round
-    const gptr<const TF> y0 = gbl(I.wy + f.yrow * nrhs + (int64_t)j0 * f.My);
-    const gptr<const TF> z1 = gbl(I.wz + fp.zrow * nrhs + off + (int64_t)j0 * fp.n);
-    const int l0 = f.My, l1 = fp.n;
+    const int l0 = rc.My, l1 = rc.l1;
+    const gptr<const TF> y0 = gbl(I.wy + rc.x0 * nrhs + (int64_t)j0 * l0);
+    const gptr<const TF> z1 = gbl(I.wz + rc.x1 * nrhs + rc.add1 + (int64_t)j0 * l1);
     stage2(
-        As, na > 0 ? I.fac + f.Qt : I.fac, AR ? na : 0, n * tw,
+        As, na > 0 ? I.fac + rc.A : I.fac, AR ? na : 0, n * tw,
         [&](int e) {
             const int k = e % n, j = e / n;
             if (j >= nc) return TF(0);
@@ -1660,17 +1645,17 @@ __global__ __launch_bounds__(512, 4) void k_ulv_gdown_s(const SInst<TF>* __restr
         },
         [&](int e, TF v) { xs[(e % n) * xp + e / n] = v; });
     __syncthreads();
-    const bool leaf = nd.is_leaf;
-    const gptr<double> X = gbl(I.X + nd.col_off + (int64_t)j0 * I.ldx);
+    const bool leaf = rc.leaf;
+    const gptr<double> X = gbl(I.X + rc.y + (int64_t)j0 * I.ldx);
     const int64_t ldx = I.ldx;
-    const gptr<TF> z = gbl(I.wz + f.zrow * nrhs + (int64_t)j0 * n);
+    const gptr<TF> z = gbl(I.wz + rc.y * nrhs + (int64_t)j0 * n);
     auto store = [&](int i, int j, TF v) {
         if (leaf) X[i + (int64_t)j * ldx] = (double)v;
         else z[i + (int64_t)j * n] = v;
     };
     if (na > 0) {
         if (AR) lds_gemm(n, n, As, xs, nc, store, xp);
-        else glb_gemm(n, n, I.fac + f.Qt, n, xs, nc, store, xp);
+        else glb_gemm(n, n, I.fac + rc.A, n, xs, nc, store, xp);
     } else {
         for (int e = threadIdx.x; e < n * nc; e += blockDim.x) store(e % n, e / n, xs[(e % n) * xp + e / n]);
     }
@@ -1682,9 +1667,9 @@ __global__ __launch_bounds__(512, 4) void k_ulv_gdown_s(const SInst<TF>* __restr
 // LDS: Dinv (m x m, rounded to 16) | xs max(Kx, 2m) x XS | bm m x XS
 template <typename TF>
 __global__ __launch_bounds__(512) void k_ulv_groot2(const SInst<TF>* __restrict__ insts,
-                                                    const WorkItem* __restrict__ work, int tn) {
+                                                    const SweepRec* __restrict__ work, int tn) {
     const int item = blockIdx.x / tn, tj = blockIdx.x % tn;
-    const WorkItem wi = work[item];
+    const SweepRec wi = work[item];
     const SInst<TF>& I = insts[wi.inst];
     const int nrhs = I.nrhs;
     const int j0 = tj * 32;
@@ -1994,7 +1979,7 @@ hipError_t launch_ulv_rinv(const FInst<T>* insts, const WorkItem* work, int nwor
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_ulv_groot2(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+hipError_t launch_ulv_groot2(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, size_t lds,
                              hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
@@ -2006,7 +1991,7 @@ hipError_t launch_ulv_groot2(const SInst<T>* insts, const WorkItem* work, int nw
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_ulv_groot(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+hipError_t launch_ulv_groot(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, size_t lds,
                             int resident, hipStream_t st) {
     if (nwork <= 0) return hipSuccess;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
@@ -2032,7 +2017,7 @@ static hipError_t sweep_grid(int nwork, int max_m, int max_nrhs, int tm_rows, in
 }
 
 template <typename T>
-hipError_t launch_ulv_gup(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nk, int max_nrhs,
+hipError_t launch_ulv_gup(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nk, int max_nrhs,
                           hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
     const int tn = (max_nrhs + 31) / 32;
@@ -2047,7 +2032,7 @@ hipError_t launch_ulv_gup(const SInst<T>* insts, const WorkItem* work, int nwork
     return hipGetLastError();
 }
 template <typename T, bool AR>
-static hipError_t launch_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int tn, int tw, size_t lds,
+static hipError_t launch_gup_s(const SInst<T>* insts, const SweepRec* work, int nwork, int tn, int tw, size_t lds,
                                hipStream_t st) {
     const int64_t total = (int64_t)nwork * tn;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gup_s<T, AR>),
@@ -2058,7 +2043,7 @@ static hipError_t launch_gup_s(const SInst<T>* insts, const WorkItem* work, int 
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw,
+hipError_t launch_ulv_gup_s(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, int tw,
                             size_t lds, int resident, hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
     if (tw != 16 && tw != 32) return hipErrorInvalidValue;
@@ -2068,7 +2053,7 @@ hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwo
                     : launch_gup_s<T, false>(insts, work, nwork, tn, tw, lds, st);
 }
 template <typename T, bool AR>
-static hipError_t launch_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int tn, int tw, size_t lds,
+static hipError_t launch_gdown_s(const SInst<T>* insts, const SweepRec* work, int nwork, int tn, int tw, size_t lds,
                                  hipStream_t st) {
     const int64_t total = (int64_t)nwork * tn;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gdown_s<T, AR>),
@@ -2079,7 +2064,7 @@ static hipError_t launch_gdown_s(const SInst<T>* insts, const WorkItem* work, in
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw,
+hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, int tw,
                               size_t lds, int resident, hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
     if (tw != 16 && tw != 32) return hipErrorInvalidValue;
@@ -2089,7 +2074,7 @@ hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const WorkItem* work, int n
                     : launch_gdown_s<T, false>(insts, work, nwork, tn, tw, lds, st);
 }
 template <typename T>
-hipError_t launch_ulv_gdown(const SInst<T>* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
+hipError_t launch_ulv_gdown(const SInst<T>* insts, const SweepRec* work, int nwork, int max_m, int max_nrhs,
                             hipStream_t st) {
     if (nwork <= 0 || max_m <= 0 || max_nrhs <= 0) return hipSuccess;
     dim3 grid;
@@ -2182,13 +2167,13 @@ template hipError_t launch_ulv_root<float>(const FInst<float>*, const WorkItem*,
 #define HSS_INST_SOLVE(T)                                                                                      \
     template hipError_t launch_ulv_gform<T>(const FInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
     template hipError_t launch_ulv_qform<T>(const FInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
-    template hipError_t launch_ulv_gup<T>(const SInst<T>*, const WorkItem*, int, int, int, hipStream_t);            \
-    template hipError_t launch_ulv_groot<T>(const SInst<T>*, const WorkItem*, int, int, size_t, int, hipStream_t);   \
+    template hipError_t launch_ulv_gup<T>(const SInst<T>*, const SweepRec*, int, int, int, hipStream_t);            \
+    template hipError_t launch_ulv_groot<T>(const SInst<T>*, const SweepRec*, int, int, size_t, int, hipStream_t);   \
     template hipError_t launch_ulv_rinv<T>(const FInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
-    template hipError_t launch_ulv_groot2<T>(const SInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
-    template hipError_t launch_ulv_gdown<T>(const SInst<T>*, const WorkItem*, int, int, int, hipStream_t);      \
-    template hipError_t launch_ulv_gup_s<T>(const SInst<T>*, const WorkItem*, int, int, int, size_t, int, hipStream_t); \
-    template hipError_t launch_ulv_gdown_s<T>(const SInst<T>*, const WorkItem*, int, int, int, size_t, int,            \
+    template hipError_t launch_ulv_groot2<T>(const SInst<T>*, const SweepRec*, int, int, size_t, hipStream_t);       \
+    template hipError_t launch_ulv_gdown<T>(const SInst<T>*, const SweepRec*, int, int, int, hipStream_t);      \
+    template hipError_t launch_ulv_gup_s<T>(const SInst<T>*, const SweepRec*, int, int, int, size_t, int, hipStream_t); \
+    template hipError_t launch_ulv_gdown_s<T>(const SInst<T>*, const SweepRec*, int, int, int, size_t, int,            \
                                               hipStream_t);
 HSS_INST_SOLVE(double)
 HSS_INST_SOLVE(float)

@@ -34,7 +34,7 @@ hipError_t launch_ulv_qform(const FInst<T>* insts, const WorkItem* work, int nwo
                             hipStream_t st);
 // solve sweeps: max_nk = largest nk (up), max_m = largest n (down) of the level's nodes
 template <typename T>
-hipError_t launch_ulv_gup(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nk, int max_nrhs,
+hipError_t launch_ulv_gup(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nk, int max_nrhs,
                           hipStream_t st);
 // root operators of the inverse-and-refine solve (factor time; max_cols = max(m, Kx))
 template <typename T>
@@ -42,22 +42,22 @@ hipError_t launch_ulv_rinv(const FInst<T>* insts, const WorkItem* work, int nwor
                            hipStream_t st);
 // root solve by inverse and one refinement step (branch roots whose Dinv fits in LDS)
 template <typename T>
-hipError_t launch_ulv_groot2(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+hipError_t launch_ulv_groot2(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, size_t lds,
                              hipStream_t st);
 // resident != 0: the root factor is staged in LDS (lds includes it)
 template <typename T>
-hipError_t launch_ulv_groot(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, size_t lds,
+hipError_t launch_ulv_groot(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, size_t lds,
                             int resident, hipStream_t st);
 template <typename T>
-hipError_t launch_ulv_gdown(const SInst<T>* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
+hipError_t launch_ulv_gdown(const SInst<T>* insts, const SweepRec* work, int nwork, int max_m, int max_nrhs,
                             hipStream_t st);
 // Variants with the input staged in LDS (tw = 16 or 32 columns per workgroup, lds = bytes
 // per workgroup); resident != 0: the node operators are staged too, else streamed
 template <typename T>
-hipError_t launch_ulv_gup_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw, size_t lds,
+hipError_t launch_ulv_gup_s(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, int tw, size_t lds,
                             int resident, hipStream_t st);
 template <typename T>
-hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const WorkItem* work, int nwork, int max_nrhs, int tw,
+hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, int tw,
                               size_t lds, int resident, hipStream_t st);
 
 hipError_t launch_resid_batch(const BatchVec* tab, int cnt, int maxc, double* norms, hipStream_t st);

@@ -143,6 +143,7 @@ struct SlotCache {
 // and up_k / down_k = the most input rows of a level's nodes; w = bytes per element.
 struct Schedule {
     std::vector<WorkItem> work;
+    std::vector<SweepRec> rec;  // the solve sweeps' records, same order as work
     std::vector<Range> up, down;
     std::vector<int> up_m, down_m, up_rows;
     std::vector<int64_t> up_na, down_na, up_k, down_k;
@@ -274,7 +275,7 @@ struct hss_fact_s {
     size_t ws_cap = 0;  // nrhs capacity
     SlotCache sinst;
     // single-request solve schedule (instance 0)
-    WorkItem* d_swork = nullptr;
+    SweepRec* d_swork = nullptr;
     Schedule sched;
     // refinement scratch (Float64): residual and correction
     double *d_r = nullptr, *d_d = nullptr, *d_nrm = nullptr;
@@ -1179,6 +1180,7 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
         maxd = std::max(maxd, F->mat->max_depth);
     }
     S.work.clear();
+    S.rec.clear();
     S.up.assign(maxh, Range{});
     S.up_m.assign(maxh, 0);
     S.up_na.assign(maxh, 0);
@@ -1199,7 +1201,31 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
                 for (int t = 0; t < r.count; ++t) {
                     const int node = M->hwork[r.off + t].node;
                     S.work.push_back(WorkItem{b, node});
-                    const FNode& f = facts[b]->hfn[node];
+                    const std::vector<FNode>& fn = facts[b]->hfn;
+                    const FNode& f = fn[node];
+                    const DevNode& nd = M->hn[node];
+                    SweepRec q{};
+                    q.inst = b;
+                    q.node = node;
+                    q.leaf = nd.is_leaf;
+                    q.nk = f.nk;
+                    q.K = f.Kx;
+                    q.My = f.My;
+                    q.A = f.nk > 0 ? f.Gz : f.Gr;  // Gr follows Gz
+                    q.y = f.yrow;
+                    if (nd.is_leaf) {
+                        q.x0 = nd.row_off;
+                    } else {
+                        const FNode &fl = fn[nd.left], &fr = fn[nd.right];
+                        q.x0 = fl.yrow;
+                        q.add0 = fl.nk;
+                        q.l0 = fl.My;
+                        q.r0 = fl.My - fl.nk;
+                        q.x1 = fr.yrow;
+                        q.add1 = fr.nk;
+                        q.l1 = fr.My;
+                    }
+                    S.rec.push_back(q);
                     const int64_t na = (int64_t)f.nk * f.Kx + (int64_t)(f.My - f.nk) * (f.Kx + f.nk);
                     S.up_m[h] = std::max(S.up_m[h], f.nk);
                     S.up_na[h] = std::max(S.up_na[h], r16(na));
@@ -1211,7 +1237,12 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
         S.up[h].count = (int)S.work.size() - S.up[h].off;
     }
     S.roots = Range{(int)S.work.size(), cnt};
-    for (int b = 0; b < cnt; ++b) S.work.push_back(WorkItem{b, 0});
+    for (int b = 0; b < cnt; ++b) {
+        S.work.push_back(WorkItem{b, 0});
+        SweepRec q{};
+        q.inst = b;
+        S.rec.push_back(q);
+    }
     for (int d = 1; d <= maxd; ++d) {
         S.down[d].off = (int)S.work.size();
         for (int b = 0; b < cnt; ++b) {
@@ -1221,7 +1252,24 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
                 for (int t = 0; t < r.count; ++t) {
                     const int node = M->hwork[r.off + t].node;
                     S.work.push_back(WorkItem{b, node});
-                    const FNode& f = facts[b]->hfn[node];
+                    const std::vector<FNode>& fn = facts[b]->hfn;
+                    const FNode& f = fn[node];
+                    const DevNode& nd = M->hn[node];
+                    const FNode& fp = fn[nd.parent];
+                    SweepRec q{};
+                    q.inst = b;
+                    q.node = node;
+                    q.leaf = nd.is_leaf;
+                    q.nk = f.nk;
+                    q.K = f.n;
+                    q.My = f.My;
+                    q.A = f.Qt;
+                    q.x0 = f.yrow;
+                    q.x1 = fp.zrow;
+                    q.add1 = nd.is_left ? 0 : fn[M->hn[nd.parent].left].nr;
+                    q.l1 = fp.n;
+                    q.y = nd.is_leaf ? nd.col_off : f.zrow;
+                    S.rec.push_back(q);
                     const int64_t na = f.Qt >= 0 ? (int64_t)f.n * f.n : 0;
                     S.down_m[d] = std::max(S.down_m[d], f.n);
                     S.down_na[d] = std::max(S.down_na[d], r16(na));
@@ -1236,7 +1284,7 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
 // lgroot_res = LDS of the root kernel with the root factor staged (used when it fits)
 // lgroot2 = LDS of the inverse-and-refine root kernel (0 when some root lacks it)
 template <typename TF>
-static hss_status launch_schedule(const SInst<TF>* di, const WorkItem* dw, const Schedule& S, int maxnrhs,
+static hss_status launch_schedule(const SInst<TF>* di, const SweepRec* dw, const Schedule& S, int maxnrhs,
                                   size_t lgroot, size_t lgroot_res, size_t lgroot2, hipStream_t st) {
     // operators staged whole in LDS (use_resident, with 32 or 16 columns per workgroup),
     // else streamed through the chunked kernels
@@ -1285,8 +1333,8 @@ static hss_status launch_schedule(const SInst<TF>* di, const WorkItem* dw, const
 static hss_status ensure_schedule(hss_fact_s* F) {
     if (F->d_swork) return HSS_OK;
     build_schedule(std::vector<hss_fact_s*>{F}, F->sched);
-    HIP_TRY(dalloc(&F->d_swork, F->sched.work.size()));
-    HIP_TRY(hipMemcpy(F->d_swork, F->sched.work.data(), sizeof(WorkItem) * F->sched.work.size(),
+    HIP_TRY(dalloc(&F->d_swork, F->sched.rec.size()));
+    HIP_TRY(hipMemcpy(F->d_swork, F->sched.rec.data(), sizeof(SweepRec) * F->sched.rec.size(),
                       hipMemcpyHostToDevice));
     return HSS_OK;
 }
@@ -1485,7 +1533,8 @@ struct hss_plan_s {
     std::vector<MvInst> h_mv;
     BatchVec* d_vec = nullptr;
     std::vector<BatchVec> h_vec;
-    WorkItem* d_work = nullptr;
+    WorkItem* d_work = nullptr;  // matvec work items (after the solve schedule's)
+    SweepRec* d_rec = nullptr;   // solve sweep records
     Schedule S;
     MvSched mvs;
     size_t lgroot = 0, lgroot_res = 0, lgroot2 = 0;
@@ -1508,6 +1557,7 @@ static void free_plan(hss_plan_s* P) {
     (void)hipFree(P->d_mv);
     (void)hipFree(P->d_vec);
     (void)hipFree(P->d_work);
+    (void)hipFree(P->d_rec);
     (void)hipFree(P->d_norms);
     delete P;
 }
@@ -1644,6 +1694,9 @@ static hss_status plan_create_impl(int32_t count, hss_fact_s* const* facts, cons
     if (P->has_ir) build_mv_schedule(mats, work, P->mvs);
     hipError_t e = dalloc(&P->d_work, work.size());
     if (e == hipSuccess) e = hipMemcpy(P->d_work, work.data(), sizeof(WorkItem) * work.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = dalloc(&P->d_rec, P->S.rec.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(P->d_rec, P->S.rec.data(), sizeof(SweepRec) * P->S.rec.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         free_plan(P);
         return fail(HSS_ERR_OOM, std::string("plan tables: ") + hipGetErrorString(e));
@@ -1670,7 +1723,7 @@ extern "C" hss_status hss_plan_create(int32_t count, const hss_fact_t* facts, co
 
 template <typename TF>
 static hss_status plan_solve(hss_plan_s* P, int t) {
-    return launch_schedule<TF>(reinterpret_cast<const SInst<TF>*>(P->d_sinst[t]), P->d_work, P->S, P->maxnrhs,
+    return launch_schedule<TF>(reinterpret_cast<const SInst<TF>*>(P->d_sinst[t]), P->d_rec, P->S, P->maxnrhs,
                                P->lgroot, P->lgroot_res, P->all_groot2 ? P->lgroot2 : 0, P->ctx->stream);
 }
 
@@ -2064,7 +2117,8 @@ static hss_status server_round(hss_server_s* S) {
     build_mv_schedule(mats, work, mvs);
     // one table upload per round (the stream is idle here)
     const size_t o_si = 0, o_mv = align256(sizeof(SInst<TF>) * cnt), o_bv = o_mv + align256(sizeof(MvInst) * cnt),
-                 o_w = o_bv + align256(sizeof(BatchVec) * cnt), tab = o_w + sizeof(WorkItem) * work.size();
+                 o_w = o_bv + align256(sizeof(BatchVec) * cnt), o_r = o_w + align256(sizeof(WorkItem) * work.size()),
+                 tab = o_r + sizeof(SweepRec) * sch.rec.size();
     s = grow(&S->d_tab, &S->tab_cap, tab);
     if (s != HSS_OK) return s;
     std::vector<char> h(tab);
@@ -2072,6 +2126,7 @@ static hss_status server_round(hss_server_s* S) {
     std::memcpy(h.data() + o_mv, mv.data(), sizeof(MvInst) * cnt);
     std::memcpy(h.data() + o_bv, bv.data(), sizeof(BatchVec) * cnt);
     std::memcpy(h.data() + o_w, work.data(), sizeof(WorkItem) * work.size());
+    std::memcpy(h.data() + o_r, sch.rec.data(), sizeof(SweepRec) * sch.rec.size());
     HIP_TRY(hipMemcpy(S->d_tab, h.data(), tab, hipMemcpyHostToDevice));
     const size_t nb = sizeof(double) * 2 * (size_t)cnt * maxnrhs;
     s = grow(reinterpret_cast<char**>(&S->d_norms), &S->norms_cap, nb);
@@ -2080,8 +2135,9 @@ static hss_status server_round(hss_server_s* S) {
     auto* dmv = reinterpret_cast<const MvInst*>(S->d_tab + o_mv);
     auto* dbv = reinterpret_cast<BatchVec*>(S->d_tab + o_bv);
     auto* dw = reinterpret_cast<const WorkItem*>(S->d_tab + o_w);
+    auto* dr = reinterpret_cast<const SweepRec*>(S->d_tab + o_r);
     // 3. one batched solve, the corrections, one batched matvec + residual
-    s = launch_schedule<TF>(dsi, dw, sch, maxnrhs, lgroot, lgroot_res, all_groot2 ? lgroot2 : 0, st);
+    s = launch_schedule<TF>(dsi, dr, sch, maxnrhs, lgroot, lgroot_res, all_groot2 ? lgroot2 : 0, st);
     if (s != HSS_OK) return s;
     HIP_TRY(launch_axpy_batch(dbv, cnt, maxnrhs, st));
     s = launch_mv_schedule(dmv, dw, mvs, maxnrhs, st);
