@@ -51,8 +51,6 @@ template <typename T>
 __device__ __forceinline__ gptr<T> gbl(T* p) {
     return (gptr<T>)p;
 }
-// 16 bytes moved as one unit by the staging copies
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 template <typename T>
 __device__ __forceinline__ T shfl_xor(T v, int mask) {
     return __shfl_xor(v, mask, 64);
@@ -177,49 +175,48 @@ __device__ __forceinline__ void stage(T* dst, int r0, const Tsrc* src, int64_t l
         [&](int e, T v) { dst[(r0 + e % rows) * NTP + e / rows] = v; });
 }
 
-// dst[0, n) = src[0, n): 16-byte vector loads, 16 in flight per thread; src and dst
-// 16-byte aligned; may read up to 15 bytes past n.
+// dst[0, n) = src[0, n) with direct global -> LDS loads (global_load_lds_dwordx4: lane l
+// of a wave writes 16 bytes at the wave's LDS base + 16 l), so the whole copy is in
+// flight at once without passing through registers.  src and dst 16-byte aligned; may
+// read up to 15 bytes past n.  The data is in LDS after s_waitcnt vmcnt(0) and, for
+// other waves, a barrier.
 template <typename TF>
-__device__ __forceinline__ void lds_copy(TF* dst, const TF* __restrict__ src, int64_t n) {
-    const gptr<const u32x4> s = gbl(reinterpret_cast<const u32x4*>(src));
-    u32x4* d = reinterpret_cast<u32x4*>(dst);
+__device__ __forceinline__ void lds_dma(TF* dst, const TF* __restrict__ src, int64_t n) {
     const int nv = (int)((n * (int64_t)sizeof(TF) + 15) / 16);
-    batched<16, u32x4>(nv, [&](int v) { return s[v]; }, [&](int v, u32x4 x) { d[v] = x; });
-}
-
-// One round of loads for an LDS-resident step: the operator (dst[0, na) = src[0, na), as
-// lds_copy) and the input gather (st(e, ld(e)) for e < nx), all issued before any LDS store.
-template <typename TF, typename Ld, typename St>
-__device__ __forceinline__ void stage2(TF* dst, const TF* __restrict__ src, int64_t na, int nx, Ld ld, St st) {
-    constexpr int VA = 8, VX = 8;
-    const gptr<const u32x4> s = gbl(reinterpret_cast<const u32x4*>(src));
-    u32x4* d = reinterpret_cast<u32x4*>(dst);
-    const int nv = (int)((na * (int64_t)sizeof(TF) + 15) / 16);
-    const int bd = blockDim.x;
-    for (int v0 = threadIdx.x, e0 = threadIdx.x; v0 < nv || e0 < nx; v0 += VA * bd, e0 += VX * bd) {
-        u32x4 ra[VA];
-        TF rx[VX];
-#pragma unroll
-        for (int q = 0; q < VA; ++q) {
-            const int v = v0 + q * bd;
-            ra[q] = v < nv ? s[v] : u32x4{0, 0, 0, 0};
-        }
-#pragma unroll
-        for (int q = 0; q < VX; ++q) {
-            const int e = e0 + q * bd;
-            rx[q] = e < nx ? ld(e) : TF{};
-        }
-#pragma unroll
-        for (int q = 0; q < VA; ++q) {
-            const int v = v0 + q * bd;
-            if (v < nv) d[v] = ra[q];
-        }
-#pragma unroll
-        for (int q = 0; q < VX; ++q) {
-            const int e = e0 + q * bd;
-            if (e < nx) st(e, rx[q]);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const char* s = reinterpret_cast<const char*>(src);
+    char* d = reinterpret_cast<char*>(dst);
+    for (int c0 = wave * 64; c0 < nv; c0 += nw * 64) {
+        if (c0 + lane < nv) {
+#if defined(__HIP_DEVICE_COMPILE__)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(s + (int64_t)(c0 + lane) * 16),
+                                             (__attribute__((address_space(3))) void*)(d + (int64_t)c0 * 16), 16, 0, 0);
+#endif
         }
     }
+}
+// wait for this wave's outstanding memory operations (direct LDS loads included)
+__device__ __forceinline__ void wait_all() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_s_waitcnt(0);
+#endif
+}
+
+// dst[0, n) = src[0, n) (lds_dma and the wait; the caller's barrier publishes it)
+template <typename TF>
+__device__ __forceinline__ void lds_copy(TF* dst, const TF* __restrict__ src, int64_t n) {
+    lds_dma(dst, src, n);
+    wait_all();
+}
+
+// One round of loads for an LDS-resident step: the operator (dst[0, na) = src[0, na) by
+// direct LDS loads) and the input gather (st(e, ld(e)) for e < nx, through registers,
+// eight loads per thread issued before their stores); the caller's barrier publishes both.
+template <typename TF, typename Ld, typename St>
+__device__ __forceinline__ void stage2(TF* dst, const TF* __restrict__ src, int64_t na, int nx, Ld ld, St st) {
+    lds_dma(dst, src, na);
+    batched<8, TF>(nx, ld, st);
+    wait_all();
 }
 
 // LDS row-major -> global column-major
@@ -1664,8 +1661,10 @@ __global__ __launch_bounds__(512, 4) void k_ulv_gdown_s(const SInst<TF>* __restr
 // Root solve by inverse and one refinement step (ulvfactor.jl:74,83): with the merge
 // b = Mb x (x = the children's reduced b and u), z0 = Dinv b, r = b - D z0,
 // z = z0 + Dinv r.  One refinement step restores the accuracy of the LU solve.
-// LDS: Dinv (m x m, rounded to 16) | xs max(Kx, 2m) x XS | bm m x XS
-template <typename TF>
+// ST (all staged): LDS = [Dinv D Mb] (2 m^2 + m Kx, rounded to 16; contiguous in the
+// factor) | xs max(Kx, 2m) x XS | bm m x XS, everything loaded in one round;
+// otherwise LDS = Dinv (m x m, rounded to 16) | xs | bm and D, Mb are read from memory.
+template <typename TF, bool ST>
 __global__ __launch_bounds__(512) void k_ulv_groot2(const SInst<TF>* __restrict__ insts,
                                                     const SweepRec* __restrict__ work, int tn) {
     const int item = blockIdx.x / tn, tj = blockIdx.x % tn;
@@ -1678,11 +1677,13 @@ __global__ __launch_bounds__(512) void k_ulv_groot2(const SInst<TF>* __restrict_
     const DevNode nd = gbl(I.nodes)[wi.node];
     const FNode f = gbl(I.fn)[wi.node];
     const int m = f.m, Kx = f.Kx;
+    const int64_t mm = (int64_t)m * m;
     extern __shared__ unsigned char lds_raw[];
     TF* Di = reinterpret_cast<TF*>(lds_raw);
-    TF* xs = Di + (((int64_t)m * m + 15) & ~int64_t(15));
+    const int64_t nops = ST ? 2 * mm + (int64_t)m * Kx : mm;
+    TF* xs = Di + ((nops + 15) & ~int64_t(15));
     TF* bm = xs + max(Kx, 2 * m) * XS;
-    lds_copy(Di, I.fac + f.Dinv, (int64_t)m * m);
+    lds_dma(Di, I.fac + f.Dinv, nops);  // Dinv, then (ST) D and Mb, which follow it
     const FNode fl = gbl(I.fn)[nd.left], fr = gbl(I.fn)[nd.right];
     const int r0 = fl.My - fl.nk;
     const gptr<const TF> y0 = gbl(I.wy + fl.yrow * nrhs + fl.nk + (int64_t)j0 * fl.My);
@@ -1696,13 +1697,17 @@ __global__ __launch_bounds__(512) void k_ulv_groot2(const SInst<TF>* __restrict_
             return k < r0 ? y0[k + (int64_t)j * l0] : y1[(k - r0) + (int64_t)j * l1];
         },
         [&](int e, TF v) { xs[(e % Kx) * XS + e / Kx] = v; });
+    wait_all();
     __syncthreads();
-    glb_gemm(m, Kx, I.fac + f.Mb, m, xs, nc, [&](int i, int j, TF v) { bm[i * XS + j] = v; });
+    auto st_b = [&](int i, int j, TF v) { bm[i * XS + j] = v; };
+    if (ST) lds_gemm(m, Kx, Di + 2 * mm, xs, nc, st_b);
+    else glb_gemm(m, Kx, I.fac + f.Mb, m, xs, nc, st_b);
     __syncthreads();
     lds_gemm(m, m, Di, bm, nc, [&](int i, int j, TF v) { xs[i * XS + j] = v; });  // z0
     __syncthreads();
-    glb_gemm(m, m, I.fac + f.Dr, m, xs, nc,
-             [&](int i, int j, TF v) { xs[(m + i) * XS + j] = bm[i * XS + j] - v; });  // r
+    auto st_r = [&](int i, int j, TF v) { xs[(m + i) * XS + j] = bm[i * XS + j] - v; };  // r
+    if (ST) lds_gemm(m, m, Di + mm, xs, nc, st_r);
+    else glb_gemm(m, m, I.fac + f.Dr, m, xs, nc, st_r);
     __syncthreads();
     const gptr<TF> z = gbl(I.wz + f.zrow * nrhs + (int64_t)j0 * f.n);
     lds_gemm(m, m, Di, xs + m * XS, nc, [&](int i, int j, TF v) { z[i + (int64_t)j * f.n] = xs[i * XS + j] + v; });
@@ -1980,14 +1985,16 @@ hipError_t launch_ulv_rinv(const FInst<T>* insts, const WorkItem* work, int nwor
 }
 template <typename T>
 hipError_t launch_ulv_groot2(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, size_t lds,
-                             hipStream_t st) {
+                             int staged, hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const int tn = (max_nrhs + 31) / 32;
-    const void* fn = reinterpret_cast<const void*>(&k_ulv_groot2<T>);
+    const void* fn = staged ? reinterpret_cast<const void*>(&k_ulv_groot2<T, true>)
+                            : reinterpret_cast<const void*>(&k_ulv_groot2<T, false>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_groot2<T>), dim3(nwork * tn), dim3(512), lds, st, insts, work, tn);
+    if (staged) hipLaunchKernelGGL((k_ulv_groot2<T, true>), dim3(nwork * tn), dim3(512), lds, st, insts, work, tn);
+    else hipLaunchKernelGGL((k_ulv_groot2<T, false>), dim3(nwork * tn), dim3(512), lds, st, insts, work, tn);
     return hipGetLastError();
 }
 template <typename T>
@@ -2170,7 +2177,7 @@ template hipError_t launch_ulv_root<float>(const FInst<float>*, const WorkItem*,
     template hipError_t launch_ulv_gup<T>(const SInst<T>*, const SweepRec*, int, int, int, hipStream_t);            \
     template hipError_t launch_ulv_groot<T>(const SInst<T>*, const SweepRec*, int, int, size_t, int, hipStream_t);   \
     template hipError_t launch_ulv_rinv<T>(const FInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
-    template hipError_t launch_ulv_groot2<T>(const SInst<T>*, const SweepRec*, int, int, size_t, hipStream_t);       \
+    template hipError_t launch_ulv_groot2<T>(const SInst<T>*, const SweepRec*, int, int, size_t, int, hipStream_t);  \
     template hipError_t launch_ulv_gdown<T>(const SInst<T>*, const SweepRec*, int, int, int, hipStream_t);      \
     template hipError_t launch_ulv_gup_s<T>(const SInst<T>*, const SweepRec*, int, int, int, size_t, int, hipStream_t); \
     template hipError_t launch_ulv_gdown_s<T>(const SInst<T>*, const SweepRec*, int, int, int, size_t, int,            \

@@ -43,7 +43,7 @@ hipError_t launch_ulv_rinv(const FInst<T>* insts, const WorkItem* work, int nwor
 // root solve by inverse and one refinement step (branch roots whose Dinv fits in LDS)
 template <typename T>
 hipError_t launch_ulv_groot2(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, size_t lds,
-                             hipStream_t st);
+                             int staged, hipStream_t st);
 // resident != 0: the root factor is staged in LDS (lds includes it)
 template <typename T>
 hipError_t launch_ulv_groot(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, size_t lds,

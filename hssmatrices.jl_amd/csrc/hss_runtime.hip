@@ -269,6 +269,7 @@ struct hss_fact_s {
     size_t lds_groot = 0;              // LDS of the root solve kernel
     size_t lds_groot_res = 0;          // the same with the root's factor staged in LDS
     size_t lds_groot2 = 0;             // the inverse-and-refine root kernel (0: not available)
+    size_t lds_groot2s = 0;            // the same with D and Mb staged too
     size_t lds_gform = 0, lds_qform = 0;
     int max_kx = 0, max_nq = 0;        // widest Gr / Qt (columns formed per node)
     void *d_wy = nullptr, *d_wz = nullptr;
@@ -885,9 +886,11 @@ static hss_status plan_factor(hss_mat_s* M, hss_fact_s* F) {
     F->lds_qform = qform * tv;
     F->lds_groot = groot * tv;
     F->lds_groot_res = groot * tv + sizeof(TF) * (size_t)((fr.rlen + 15) & ~int64_t(15));
-    if (fr.Dinv >= 0)
-        F->lds_groot2 = sizeof(TF) * (size_t)(((int64_t)fr.m * fr.m + 15) / 16 * 16 +
-                                              (int64_t)std::max(fr.Kx, 2 * fr.m) * 33 + (int64_t)fr.m * 33);
+    if (fr.Dinv >= 0) {
+        const int64_t io = (int64_t)std::max(fr.Kx, 2 * fr.m) * 33 + (int64_t)fr.m * 33;
+        F->lds_groot2 = sizeof(TF) * (size_t)(((int64_t)fr.m * fr.m + 15) / 16 * 16 + io);
+        F->lds_groot2s = sizeof(TF) * (size_t)((2 * (int64_t)fr.m * fr.m + (int64_t)fr.m * fr.Kx + 15) / 16 * 16 + io);
+    }
     F->max_kx = max_kx;
     F->max_nq = max_nq;
     if (F->lds_gform > 160 * 1024 || F->lds_groot > 160 * 1024)
@@ -1282,10 +1285,11 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
 }
 
 // lgroot_res = LDS of the root kernel with the root factor staged (used when it fits)
-// lgroot2 = LDS of the inverse-and-refine root kernel (0 when some root lacks it)
+// lgroot2 = LDS of the inverse-and-refine root kernel (0 when some root lacks it),
+// lgroot2s = the same with every root operator staged
 template <typename TF>
 static hss_status launch_schedule(const SInst<TF>* di, const SweepRec* dw, const Schedule& S, int maxnrhs,
-                                  size_t lgroot, size_t lgroot_res, size_t lgroot2, hipStream_t st) {
+                                  size_t lgroot, size_t lgroot_res, size_t lgroot2, size_t lgroot2s, hipStream_t st) {
     // operators staged whole in LDS (use_resident, with 32 or 16 columns per workgroup),
     // else streamed through the chunked kernels
     int tw = 32;
@@ -1304,9 +1308,12 @@ static hss_status launch_schedule(const SInst<TF>* di, const SweepRec* dw, const
         }
     }
     {
-        if (lgroot2 > 0 && lgroot2 <= kLdsResidentMax) {
+        if (lgroot2 > 0 && lgroot2s > 0 && lgroot2s <= kLdsResidentMax) {
             ProfScope ps("k_ulv_groot2", 0, S.roots.count, st);
-            HIP_TRY((launch_ulv_groot2<TF>(di, dw + S.roots.off, S.roots.count, maxnrhs, lgroot2, st)));
+            HIP_TRY((launch_ulv_groot2<TF>(di, dw + S.roots.off, S.roots.count, maxnrhs, lgroot2s, 1, st)));
+        } else if (lgroot2 > 0 && lgroot2 <= kLdsResidentMax) {
+            ProfScope ps("k_ulv_groot2", 0, S.roots.count, st);
+            HIP_TRY((launch_ulv_groot2<TF>(di, dw + S.roots.off, S.roots.count, maxnrhs, lgroot2, 0, st)));
         } else {
             ProfScope ps("k_ulv_groot", 0, S.roots.count, st);
             const bool res = lgroot_res <= kLdsResidentMax;
@@ -1350,7 +1357,7 @@ static hss_status solve_one(hss_fact_s* F, int nrhs, const double* dB, int64_t l
     const void* dv = nullptr;
     HIP_TRY(F->sinst.get(&I, sizeof(I), st, &dv));
     return launch_schedule<TF>(reinterpret_cast<const SInst<TF>*>(dv), F->d_swork, F->sched, nrhs, F->lds_groot,
-                               F->lds_groot_res, F->lds_groot2, st);
+                               F->lds_groot_res, F->lds_groot2, F->lds_groot2s, st);
 }
 
 static hss_status ensure_refine(hss_fact_s* F, int nrhs) {
@@ -1537,7 +1544,7 @@ struct hss_plan_s {
     SweepRec* d_rec = nullptr;   // solve sweep records
     Schedule S;
     MvSched mvs;
-    size_t lgroot = 0, lgroot_res = 0, lgroot2 = 0;
+    size_t lgroot = 0, lgroot_res = 0, lgroot2 = 0, lgroot2s = 0;
     bool all_groot2 = true;
     double* d_norms = nullptr;
     std::vector<double> h_norms;
@@ -1678,6 +1685,7 @@ static hss_status plan_create_impl(int32_t count, hss_fact_s* const* facts, cons
         P->lgroot = std::max(P->lgroot, F->lds_groot);
         P->lgroot_res = std::max(P->lgroot_res, F->lds_groot_res);
         P->lgroot2 = std::max(P->lgroot2, F->lds_groot2);
+        P->lgroot2s = std::max(P->lgroot2s, F->lds_groot2s);
         P->all_groot2 = P->all_groot2 && F->lds_groot2 > 0;
     }
     P->cnt = (int)P->reqs.size();
@@ -1724,7 +1732,7 @@ extern "C" hss_status hss_plan_create(int32_t count, const hss_fact_t* facts, co
 template <typename TF>
 static hss_status plan_solve(hss_plan_s* P, int t) {
     return launch_schedule<TF>(reinterpret_cast<const SInst<TF>*>(P->d_sinst[t]), P->d_rec, P->S, P->maxnrhs,
-                               P->lgroot, P->lgroot_res, P->all_groot2 ? P->lgroot2 : 0, P->ctx->stream);
+                               P->lgroot, P->lgroot_res, P->all_groot2 ? P->lgroot2 : 0, P->lgroot2s, P->ctx->stream);
 }
 
 static hss_status plan_matvec(hss_plan_s* P) {
@@ -2051,7 +2059,7 @@ static hss_status server_round(hss_server_s* S) {
     // 2. tables: solve (first solve b->x, or correction r->d), axpy, matvec x->r, residual
     size_t ws = 0;
     int maxnrhs = 0;
-    size_t lgroot = 0, lgroot_res = 0, lgroot2 = 0;
+    size_t lgroot = 0, lgroot_res = 0, lgroot2 = 0, lgroot2s = 0;
     bool all_groot2 = true;
     std::vector<size_t> off_s(cnt), off_m(cnt);
     std::vector<hss_mat_s*> mats(cnt);
@@ -2068,6 +2076,7 @@ static hss_status server_round(hss_server_s* S) {
         lgroot = std::max(lgroot, F->lds_groot);
         lgroot_res = std::max(lgroot_res, F->lds_groot_res);
         lgroot2 = std::max(lgroot2, F->lds_groot2);
+        lgroot2s = std::max(lgroot2s, F->lds_groot2s);
         all_groot2 = all_groot2 && F->lds_groot2 > 0;
     }
     hss_status s = grow(&S->d_round, &S->round_cap, ws);
@@ -2137,7 +2146,7 @@ static hss_status server_round(hss_server_s* S) {
     auto* dw = reinterpret_cast<const WorkItem*>(S->d_tab + o_w);
     auto* dr = reinterpret_cast<const SweepRec*>(S->d_tab + o_r);
     // 3. one batched solve, the corrections, one batched matvec + residual
-    s = launch_schedule<TF>(dsi, dr, sch, maxnrhs, lgroot, lgroot_res, all_groot2 ? lgroot2 : 0, st);
+    s = launch_schedule<TF>(dsi, dr, sch, maxnrhs, lgroot, lgroot_res, all_groot2 ? lgroot2 : 0, lgroot2s, st);
     if (s != HSS_OK) return s;
     HIP_TRY(launch_axpy_batch(dbv, cnt, maxnrhs, st));
     s = launch_mv_schedule(dmv, dw, mvs, maxnrhs, st);
