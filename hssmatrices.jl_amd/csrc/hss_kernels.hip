@@ -1437,110 +1437,152 @@ __device__ __forceinline__ void lds_gemm(int M, int K, const TF* As, const TF* x
     }
 }
 
-// The same product with A read from memory (column-major, lda), for operands used
-// once per workgroup.  Each wave owns 16-row strips of the output across all (one or
-// two) 16-column blocks, so every A element is loaded by exactly one wave, straight
-// into registers U k-steps ahead of the MFMAs, and feeds one MFMA per column block.
-// Rows past M read row M - 1 (in bounds) and are never stored.
-template <bool two, typename TF, typename Store>
-__device__ __forceinline__ void glb_gemm_cb(int M, int K, const TF* __restrict__ A, int64_t lda, const TF* xs,
-                                            int ncols, Store store, int xp) {
-    using V4 = typename Mfma<TF>::v4;
-    constexpr int U = 8;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
-    const int nrb = (M + 15) / 16;
-    const int S = (K + 3) / 4;  // k-steps
-    if (K <= 0) {
-        for (int e = threadIdx.x; e < M * ncols; e += blockDim.x) store(e % M, e / M, TF(0));
-        return;
-    }
-    for (int rb = wave; rb < nrb; rb += nw) {
-        const int i = rb * 16 + lr;
-        const gptr<const TF> Ai = gbl(A) + (i < M ? i : M - 1);
-        // Prefetch loads are unconditional (steps past S and columns past K are clamped
-        // in bounds), so no wait is forced until a value is used; k past K is zeroed at
-        // the MFMA.
-        auto ld = [&](int st) {
-            const int kk = 4 * (st < S ? st : S - 1) + lk;
-            return Ai[(int64_t)(kk < K ? kk : K - 1) * lda];
-        };
-        V4 acc0 = V4{TF(0), TF(0), TF(0), TF(0)}, acc1 = acc0;
-        TF a[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) a[u] = ld(u);
-        int s0 = 0;
-        // blocks of U k-steps that lie entirely inside K (no clamping or zeroing); the
-        // two register sets alternate, so no prefetched value is copied (a copy would
-        // wait for its load)
-        auto blk = [&](TF(&cur)[U], TF(&nxt)[U], int b0s) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) nxt[u] = ld(b0s + U + u);
-            const TF* xb = xs + (4 * b0s + lk) * xp + lr;
-            TF b0[U], b1[U];
+// Strip products: acc += A(16-row strip rb, 0:K) xs(0:K, 0:16 or 0:32), A column-major
+// (lda) in LDS (AL) or in memory, xs row-major in LDS (pitch xp).  A strip covers all
+// (one or two) 16-column blocks, so every A element is read by one wave and feeds one
+// MFMA per column block; with one block the k-steps alternate between acc0 and acc1.
+// A in memory is loaded straight into registers U k-steps ahead of the MFMAs; the loads
+// are unconditional (columns past K are clamped in bounds and k past K is zeroed at the
+// MFMA), so no wait is forced before a value is used.  Rows past M read row M - 1 and
+// are never stored.
+template <bool AL, bool two, typename TF>
+__device__ __forceinline__ void strip_acc(int M, int K, const TF* __restrict__ A, int64_t lda, const TF* xs, int xp,
+                                          int rb, typename Mfma<TF>::v4& acc0, typename Mfma<TF>::v4& acc1) {
+    if (K <= 0) return;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    const int i = rb * 16 + lr;
+    const int ic = i < M ? i : M - 1;
+    const int S = (K + 3) / 4, Sf = K / 4;  // k-steps; steps [0, Sf) lie entirely inside K
+    auto mac = [&](int u, TF av, const TF* xk) {
+        if (two) {
+            acc0 = Mfma<TF>::mac(av, xk[0], acc0);
+            acc1 = Mfma<TF>::mac(av, xk[16], acc1);
+        } else if (u & 1) {
+            acc1 = Mfma<TF>::mac(av, xk[0], acc1);
+        } else {
+            acc0 = Mfma<TF>::mac(av, xk[0], acc0);
+        }
+    };
+    // a step that may pass K: clamped x row, A value zeroed past K
+    auto edge = [&](int u, int st, TF aval) {
+        const int kk = 4 * st + lk;
+        mac(u, kk < K ? aval : TF(0), xs + (kk < K ? kk : K - 1) * xp + lr);
+    };
+    if (AL) {
+        constexpr int U = 4;
+        const TF* Ai = A + ic;
+        int st = 0;
+        for (; st + U <= Sf; st += U) {
+            TF av[U], b0[U], b1[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                b0[u] = xb[4 * u * xp];
-                if (two) b1[u] = xb[4 * u * xp + 16];
+                const int kk = 4 * (st + u) + lk;
+                av[u] = Ai[(int64_t)kk * lda];
+                b0[u] = xs[kk * xp + lr];
+                if (two) b1[u] = xs[kk * xp + 16 + lr];
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (two) {
-                    acc0 = Mfma<TF>::mac(cur[u], b0[u], acc0);
-                    acc1 = Mfma<TF>::mac(cur[u], b1[u], acc1);
+                    acc0 = Mfma<TF>::mac(av[u], b0[u], acc0);
+                    acc1 = Mfma<TF>::mac(av[u], b1[u], acc1);
                 } else if (u & 1) {
-                    acc1 = Mfma<TF>::mac(cur[u], b0[u], acc1);
+                    acc1 = Mfma<TF>::mac(av[u], b0[u], acc1);
                 } else {
-                    acc0 = Mfma<TF>::mac(cur[u], b0[u], acc0);
+                    acc0 = Mfma<TF>::mac(av[u], b0[u], acc0);
                 }
             }
-        };
-        const int Sf = K / 4;
-        TF an[U];
-        for (; s0 + 2 * U <= Sf; s0 += 2 * U) {
-            blk(a, an, s0);
-            blk(an, a, s0 + U);
         }
-        if (s0 + U <= Sf) {
-            blk(a, an, s0);
-            s0 += U;
-#pragma unroll
-            for (int u = 0; u < U; ++u) a[u] = an[u];
+        for (int u = 0; st < S; ++st, ++u) {
+            const int kk = 4 * st + lk;
+            edge(u, st, Ai[(int64_t)(kk < K ? kk : K - 1) * lda]);
         }
-        for (; s0 < S; s0 += U) {
-            TF an[U];
+        return;
+    }
+    constexpr int U = 8;
+    const gptr<const TF> Ai = gbl(A) + ic;
+    auto ld = [&](int st) {
+        const int kk = 4 * (st < S ? st : S - 1) + lk;
+        return Ai[(int64_t)(kk < K ? kk : K - 1) * lda];
+    };
+    TF a[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) an[u] = ld(s0 + U + u);
+    for (int u = 0; u < U; ++u) a[u] = ld(u);
+    int s0 = 0;
+    // blocks of U full steps; the two register sets alternate, so no prefetched value is
+    // copied (a copy would wait for its load)
+    auto blk = [&](TF(&cur)[U], TF(&nxt)[U], int b0s) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (s0 + u < S) {
-                    const int kk = 4 * (s0 + u) + lk;
-                    const TF av = kk < K ? a[u] : TF(0);
-                    const TF* xk = xs + (kk < K ? kk : K - 1) * xp + lr;
-                    if (two) {
-                        acc0 = Mfma<TF>::mac(av, xk[0], acc0);
-                        acc1 = Mfma<TF>::mac(av, xk[16], acc1);
-                    } else if (u & 1) {
-                        acc1 = Mfma<TF>::mac(av, xk[0], acc1);
-                    } else {
-                        acc0 = Mfma<TF>::mac(av, xk[0], acc0);
-                    }
-                }
-            }
+        for (int u = 0; u < U; ++u) nxt[u] = ld(b0s + U + u);
+        const TF* xb = xs + (4 * b0s + lk) * xp + lr;
+        TF b0[U], b1[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) a[u] = an[u];
+        for (int u = 0; u < U; ++u) {
+            b0[u] = xb[4 * u * xp];
+            if (two) b1[u] = xb[4 * u * xp + 16];
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int ii = rb * 16 + Mfma<TF>::row(lk, r);
-            if (ii >= M) continue;
+        for (int u = 0; u < U; ++u) {
             if (two) {
-                store(ii, lr, acc0[r]);
-                if (16 + lr < ncols) store(ii, 16 + lr, acc1[r]);
-            } else if (lr < ncols) {
-                store(ii, lr, acc0[r] + acc1[r]);
+                acc0 = Mfma<TF>::mac(cur[u], b0[u], acc0);
+                acc1 = Mfma<TF>::mac(cur[u], b1[u], acc1);
+            } else if (u & 1) {
+                acc1 = Mfma<TF>::mac(cur[u], b0[u], acc1);
+            } else {
+                acc0 = Mfma<TF>::mac(cur[u], b0[u], acc0);
             }
         }
+    };
+    TF an[U];
+    for (; s0 + 2 * U <= Sf; s0 += 2 * U) {
+        blk(a, an, s0);
+        blk(an, a, s0 + U);
+    }
+    if (s0 + U <= Sf) {
+        blk(a, an, s0);
+        s0 += U;
+#pragma unroll
+        for (int u = 0; u < U; ++u) a[u] = an[u];
+    }
+    for (; s0 < S; s0 += U) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) an[u] = ld(s0 + U + u);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (s0 + u < S) edge(u, s0 + u, a[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) a[u] = an[u];
+    }
+}
+
+// store(i, j, v) for the rows i < M of strip rb held by lane ln
+template <bool two, typename TF, typename Store>
+__device__ __forceinline__ void strip_put(int M, int ncols, int rb, int ln, const typename Mfma<TF>::v4& a0,
+                                          const typename Mfma<TF>::v4& a1, Store store) {
+    const int lr = ln & 15, lk = ln >> 4;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int ii = rb * 16 + Mfma<TF>::row(lk, r);
+        if (ii >= M) continue;
+        if (two) {
+            store(ii, lr, a0[r]);
+            if (16 + lr < ncols) store(ii, 16 + lr, a1[r]);
+        } else if (lr < ncols) {
+            store(ii, lr, a0[r] + a1[r]);
+        }
+    }
+}
+
+// out = A xs (M x ncols, ncols <= 32) with A in memory, one strip per wave at a time
+template <bool two, typename TF, typename Store>
+__device__ __forceinline__ void glb_gemm_cb(int M, int K, const TF* __restrict__ A, int64_t lda, const TF* xs,
+                                            int ncols, Store store, int xp) {
+    using V4 = typename Mfma<TF>::v4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int rb = wave; rb < (M + 15) / 16; rb += nw) {
+        V4 a0 = V4{TF(0), TF(0), TF(0), TF(0)}, a1 = a0;
+        strip_acc<false, two>(M, K, A, lda, xs, xp, rb, a0, a1);
+        strip_put<two, TF>(M, ncols, rb, lane, a0, a1, store);
     }
 }
 template <typename TF, typename Store>
@@ -1548,6 +1590,55 @@ __device__ __forceinline__ void glb_gemm(int M, int K, const TF* __restrict__ A,
                                          Store store, int xp = XS) {
     if (ncols > 16) glb_gemm_cb<true>(M, K, A, lda, xs, ncols, store, xp);
     else glb_gemm_cb<false>(M, K, A, lda, xs, ncols, store, xp);
+}
+
+// The two products of a node's up-sweep step with the input x in xs rows [0, Kx):
+//   zloc = Gz x           (nk x Kx, lda nk)  -> st0, which also writes xs rows [Kx, Kx + nk)
+//   [b; u] = Gr [x; zloc] (R x (Kx + nk), lda R) -> st1
+// The strips of Gr start on the x part (its first Kx columns) while other waves compute
+// zloc, hold their sums across the barrier that publishes zloc, then add the zloc part.
+// Waves [0, n2) own the Gr strips, the Gz strips go to the waves after them.  (With more
+// Gr strips than waves the two products run one after the other.)  All threads of the
+// workgroup call it; it contains a barrier.
+template <bool AL, bool two, typename TF, typename S0, typename S1>
+__device__ __forceinline__ void up_gemms_cb(int nk, int Kx, int R, const TF* Gz, const TF* Gr, const TF* xs, int xp,
+                                            int ncols, S0 st0, S1 st1) {
+    using V4 = typename Mfma<TF>::v4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int n1 = (nk + 15) / 16, n2 = (R + 15) / 16;
+    const V4 z4 = V4{TF(0), TF(0), TF(0), TF(0)};
+    if (n2 > nw) {
+        for (int rb = wave; rb < n1; rb += nw) {
+            V4 a0 = z4, a1 = z4;
+            strip_acc<AL, two>(nk, Kx, Gz, nk, xs, xp, rb, a0, a1);
+            strip_put<two, TF>(nk, ncols, rb, lane, a0, a1, st0);
+        }
+        __syncthreads();
+        for (int rb = wave; rb < n2; rb += nw) {
+            V4 a0 = z4, a1 = z4;
+            strip_acc<AL, two>(R, Kx + nk, Gr, R, xs, xp, rb, a0, a1);
+            strip_put<two, TF>(R, ncols, rb, lane, a0, a1, st1);
+        }
+        return;
+    }
+    for (int rb = (wave - n2 + nw) % nw; rb < n1; rb += nw) {
+        V4 a0 = z4, a1 = z4;
+        strip_acc<AL, two>(nk, Kx, Gz, nk, xs, xp, rb, a0, a1);
+        strip_put<two, TF>(nk, ncols, rb, lane, a0, a1, st0);
+    }
+    V4 b0 = z4, b1 = z4;
+    if (wave < n2) strip_acc<AL, two>(R, Kx, Gr, R, xs, xp, wave, b0, b1);
+    __syncthreads();  // zloc is in xs rows [Kx, Kx + nk)
+    if (wave < n2) {
+        strip_acc<AL, two>(R, nk, Gr + (int64_t)Kx * R, R, xs + (int64_t)Kx * xp, xp, wave, b0, b1);
+        strip_put<two, TF>(R, ncols, wave, lane, b0, b1, st1);
+    }
+}
+template <bool AL, typename TF, typename S0, typename S1>
+__device__ __forceinline__ void up_gemms(int nk, int Kx, int R, const TF* Gz, const TF* Gr, const TF* xs, int xp,
+                                         int ncols, S0 st0, S1 st1) {
+    if (ncols > 16) up_gemms_cb<AL, true>(nk, Kx, R, Gz, Gr, xs, xp, ncols, st0, st1);
+    else up_gemms_cb<AL, false>(nk, Kx, R, Gz, Gr, xs, xp, ncols, st0, st1);
 }
 
 // Up-sweep with the input staged in LDS; tw = columns per workgroup (16 or 32), xs pitch
@@ -1602,13 +1693,10 @@ __global__ __launch_bounds__(512, 4) void k_ulv_gup_s(const SInst<TF>* __restric
         y[i + (int64_t)j * My] = v;
         xs[(Kx + i) * xp + j] = v;
     };
-    if (AR) lds_gemm(nk, Kx, As, xs, nc, st0, xp);
-    else glb_gemm(nk, Kx, Ag, nk, xs, nc, st0, xp);
-    __syncthreads();
     // [reduced b; u] = Gr [x; zloc]  -> y rows [nk, My)
     auto st1 = [&](int i, int j, TF v) { y[nk + i + (int64_t)j * My] = v; };
-    if (AR) lds_gemm(R, Kx + nk, As + (int64_t)nk * Kx, xs, nc, st1, xp);
-    else glb_gemm(R, Kx + nk, Ag + (int64_t)nk * Kx, R, xs, nc, st1, xp);
+    const TF* Gz = AR ? As : Ag;
+    up_gemms<AR>(nk, Kx, R, Gz, Gz + (int64_t)nk * Kx, xs, xp, nc, st0, st1);
 }
 
 // Down-sweep with the input staged in LDS.  AR: Qt is staged too (LDS = Qt (n x n,
