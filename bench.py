@@ -72,7 +72,7 @@ def load_generators(pkg):
 # algorithmic byte / flop models (per launch), see DESIGN.md
 # ---------------------------------------------------------------------------
 def matvec_leaf_bytes(nodes, nrhs, beta_nonzero=False):
-    """k_mv_leaf: D, U, B_seg, F read once, C written once (8 B each)."""
+    """k_mv_leaf*: D, U, B_seg, F read once, C written once (8 B each)."""
     b = 0
     for r in nodes[nodes["left"] < 0]:
         m, n, ku = int(r["m"]), int(r["n"]), int(r["ku"])
@@ -519,7 +519,7 @@ def main():
         f"{dbytes/1e6:.2f} MB -> {achieved:.1f} GB/s, {dflops/avg_ms/1e9:.2f} TFLOP/s")
     pname, targs, pgrid = sweep_launch(nodes, F64, dk, dlev, int(dnw), nrhs)
     traffic = profiled_traffic(pname, targs, pgrid, 1)
-    mvleaf = [(key, v) for key, v in tot_m.items() if key[0] == "k_mv_leaf"]
+    mvleaf = [(key, v) for key, v in tot_m.items() if key[0].startswith("k_mv_leaf")]
     mv_leaf_ms = mvleaf[0][1][0] / mvleaf[0][1][1]
     mv_leaf_bytes = matvec_leaf_bytes(nodes, nrhs)
 

@@ -95,6 +95,20 @@ struct SweepRec {
     int64_t A, y, x0, x1;
 };
 
+// One work item of a matvec sweep with the node's operator and segment offsets resolved
+// on the host.  The step is y = A [x_0; x_1] with A (M x K, column-major, lda M) at
+// element offset A of the operator buffer and rows [0, r0) of x from segment 0:
+//   kind 0 (up):   Z_node = opZ [B rows (leaf) | Z_left; Z_right]
+//   kind 1 (down): F_node = opF [Z_sibling; F_parent]
+//   kind 2 (leaf): C rows = alpha opC [B rows; F_node] + beta C rows
+// B / C offsets (x0 with from_b, y of kind 2) are row offsets (pitch: the call's ldb /
+// ldc); Z / F offsets are row bases in units of nrhs elements, with pitches ld0, ld1, ldy.
+struct MvRec {
+    int32_t inst, M, K, r0;
+    int32_t ld0, ld1, ldy, from_b;
+    int64_t A, x0, x1, y;
+};
+
 // Per-instance arguments of one matvec launch sequence.
 struct MvInst {
     const DevNode* nodes;

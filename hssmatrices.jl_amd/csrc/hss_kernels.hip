@@ -1816,58 +1816,32 @@ struct MvOp {
     int64_t ldy;
 };
 
+// segment pointers of one matvec record (Z / F bases scaled by nrhs)
 template <int KIND>
-__device__ __forceinline__ void mv_op(const MvInst& I, int node, MvOp& o) {
-    const DevNode nd = gbl(I.nodes)[node];
+__device__ __forceinline__ void mv_op(const MvInst& I, const MvRec& rc, MvOp& o) {
     const int nrhs = I.nrhs;
+    o.A = I.op + rc.A;
+    o.M = rc.M;
+    o.K = rc.K;
     o.x.dbl0 = o.x.dbl1 = 1;
-    o.x.p1 = nullptr;
-    o.x.ld1 = 1;
-    if (KIND == 0) {
-        o.A = I.op + nd.opZ;
-        o.M = nd.kv;
-        if (nd.is_leaf) {
-            o.K = nd.n;
-            o.x.p0 = I.B + nd.col_off;
-            o.x.ld0 = I.ldb;
-            o.x.r0 = nd.n;
-        } else {
-            const DevNode L = gbl(I.nodes)[nd.left], R = gbl(I.nodes)[nd.right];
-            o.K = L.kv + R.kv;
-            o.x.p0 = I.Z + L.zrow * nrhs;
-            o.x.ld0 = L.kv > 0 ? L.kv : 1;
-            o.x.r0 = L.kv;
-            o.x.p1 = I.Z + R.zrow * nrhs;
-            o.x.ld1 = R.kv > 0 ? R.kv : 1;
-        }
-        o.y = I.Z + nd.zrow * nrhs;
-        o.ldy = nd.kv;
-    } else if (KIND == 1) {
-        const DevNode p = gbl(I.nodes)[nd.parent];
-        const DevNode sb = gbl(I.nodes)[nd.is_left ? p.right : p.left];
-        o.A = I.op + nd.opF;
-        o.M = nd.ku;
-        o.K = sb.kv + p.ku;
-        o.x.p0 = I.Z + sb.zrow * nrhs;
-        o.x.ld0 = sb.kv > 0 ? sb.kv : 1;
-        o.x.r0 = sb.kv;
-        o.x.p1 = I.F + p.frow * nrhs;
-        o.x.ld1 = p.ku > 0 ? p.ku : 1;
-        o.y = I.F + nd.frow * nrhs;
-        o.ldy = nd.ku;
-    } else {
-        o.A = I.op + nd.opC;
-        o.M = nd.m;
-        o.K = nd.n + nd.ku;
-        o.x.p0 = I.B + nd.col_off;
+    o.x.r0 = rc.r0;
+    o.x.r1 = rc.K;
+    if (rc.from_b) {
+        o.x.p0 = I.B + rc.x0;
         o.x.ld0 = I.ldb;
-        o.x.r0 = nd.n;
-        o.x.p1 = I.F + nd.frow * nrhs;
-        o.x.ld1 = nd.ku > 0 ? nd.ku : 1;
-        o.y = I.C + nd.row_off;
-        o.ldy = I.ldc;
+    } else {
+        o.x.p0 = I.Z + rc.x0 * nrhs;
+        o.x.ld0 = rc.ld0;
     }
-    o.x.r1 = o.K;
+    o.x.p1 = (KIND == 0 ? I.Z : I.F) + rc.x1 * nrhs;
+    o.x.ld1 = rc.ld1;
+    if (KIND == 2) {
+        o.y = I.C + rc.y;
+        o.ldy = I.ldc;
+    } else {
+        o.y = (KIND == 0 ? I.Z : I.F) + rc.y * nrhs;
+        o.ldy = rc.ldy;
+    }
 }
 
 template <int KIND>
@@ -1877,44 +1851,67 @@ __device__ __forceinline__ void mv_store(const MvInst& I, const MvOp& o, int i, 
     else *c = v;
 }
 
-// LDS-resident: LDS = A (M x K, rounded to 16) | xs K x XS
-template <int KIND>
-__global__ __launch_bounds__(512, 4) void k_mv_s(const MvInst* __restrict__ insts, const WorkItem* __restrict__ work,
+// One matvec step per (node, tw-column tile) with the input staged in LDS (xs, K rows,
+// pitch tw + 1).  AR: the operator is staged too (LDS = A (M x K, rounded to 16) | xs);
+// otherwise the waves stream it from memory (LDS = xs).  Strip products as in the solve.
+template <int KIND, bool AR>
+__global__ __launch_bounds__(512, 4) void k_mv_s(const MvInst* __restrict__ insts, const MvRec* __restrict__ work,
                                               int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
-    const WorkItem wi = work[item];
-    const MvInst& I = insts[wi.inst];
+    const MvRec rc = work[item];
+    const MvInst& I = insts[rc.inst];
     const int j0 = tj * tw;
     if (j0 >= I.nrhs) return;
     const int nc = min(tw, I.nrhs - j0);
     const int xp = tw + 1;  // LDS pitch of xs
     MvOp o;
-    mv_op<KIND>(I, wi.node, o);
+    mv_op<KIND>(I, rc, o);
     const int M = o.M, K = o.K;
     const int64_t na = (int64_t)M * K;
     extern __shared__ unsigned char lds_raw[];
     double* As = reinterpret_cast<double*>(lds_raw);
-    double* xs = As + ((na + 15) & ~int64_t(15));
+    double* xs = AR ? As + ((na + 15) & ~int64_t(15)) : As;
+    const int r0 = o.x.r0;
+    const gptr<const double> x0 = gbl(static_cast<const double*>(o.x.p0) + (int64_t)j0 * o.x.ld0);
+    const gptr<const double> x1 = gbl(static_cast<const double*>(o.x.p1) + (int64_t)j0 * o.x.ld1);
+    const int64_t l0 = o.x.ld0, l1 = o.x.ld1;
     stage2(
-        As, o.A, na, K * tw,
-        [&](int e) { return e / K < nc ? seg_load<double>(o.x, nullptr, 0, e % K, j0 + e / K, 0) : 0.0; },
+        As, o.A, AR ? na : 0, K * tw,
+        [&](int e) {
+            const int k = e % K, j = e / K;
+            if (j >= nc) return 0.0;
+            return k < r0 ? x0[k + j * l0] : x1[(k - r0) + j * l1];
+        },
         [&](int e, double v) { xs[(e % K) * xp + e / K] = v; });
     __syncthreads();
-    lds_gemm(M, K, As, xs, nc, [&](int i, int j, double v) { mv_store<KIND>(I, o, i, j0 + j, v); }, xp);
+    auto store = [&](int i, int j, double v) { mv_store<KIND>(I, o, i, j0 + j, v); };
+    using V4 = typename Mfma<double>::v4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const double* Ap = AR ? As : o.A;
+    for (int rb = wave; rb < (M + 15) / 16; rb += nw) {
+        V4 a0 = V4{0.0, 0.0, 0.0, 0.0}, a1 = a0;
+        if (nc > 16) {
+            strip_acc<AR, true>(M, K, Ap, M, xs, xp, rb, a0, a1);
+            strip_put<true, double>(M, nc, rb, lane, a0, a1, store);
+        } else {
+            strip_acc<AR, false>(M, K, Ap, M, xs, xp, rb, a0, a1);
+            strip_put<false, double>(M, nc, rb, lane, a0, a1, store);
+        }
+    }
 }
 
-// Streamed through the chunked MFMA tile GEMM (nodes whose operators do not fit in LDS).
+// Streamed through the chunked MFMA tile GEMM (nodes whose inputs do not fit in LDS).
 template <int KIND, int WM, int WN>
-__global__ __launch_bounds__(256) void k_mv_t(const MvInst* __restrict__ insts, const WorkItem* __restrict__ work,
+__global__ __launch_bounds__(256) void k_mv_t(const MvInst* __restrict__ insts, const MvRec* __restrict__ work,
                                               int nwork, int tm, int tn) {
     int item, ti, tj;
     if (!tile_of(nwork * tm * tn, tm, tn, item, ti, tj)) return;
-    const WorkItem wi = work[item];
-    const MvInst& I = insts[wi.inst];
+    const MvRec rc = work[item];
+    const MvInst& I = insts[rc.inst];
     const int i0 = ti * 32 * WM, j0 = tj * 32 * WN;
     MvOp o;
-    mv_op<KIND>(I, wi.node, o);
+    mv_op<KIND>(I, rc, o);
     if (i0 >= o.M || j0 >= I.nrhs) return;
     __shared__ TileLds<double, WM, WN> L;
     tile_gemm<double, WM, WN>(L, o.M, I.nrhs, o.K, o.A, o.M, o.x, nullptr, 0, i0, j0,
@@ -2217,20 +2214,27 @@ hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, do
     return hipGetLastError();
 }
 
+template <int KIND, bool AR>
+static hipError_t launch_mv_s(const MvInst* insts, const MvRec* work, int nwork, int tn, int tw, size_t lds,
+                              hipStream_t st) {
+    const int64_t total = (int64_t)nwork * tn;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mv_s<KIND, AR>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_mv_s<KIND, AR>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts, work,
+                       nwork, tn, tw);
+    return hipGetLastError();
+}
 template <int KIND>
-static hipError_t launch_mv_kind(const MvInst* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs,
-                                 int tw, size_t lds, hipStream_t st) {
+static hipError_t launch_mv_kind(const MvInst* insts, const MvRec* work, int nwork, int max_m, int max_nrhs, int tw,
+                                 size_t lds, int resident, hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0 || max_m <= 0) return hipSuccess;
     if (lds > 0) {
         if (lds > 160 * 1024 || (tw != 16 && tw != 32)) return hipErrorInvalidValue;
         const int tn = (max_nrhs + tw - 1) / tw;
-        const int64_t total = (int64_t)nwork * tn;
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mv_s<KIND>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_mv_s<KIND>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts, work,
-                           nwork, tn, tw);
-        return hipGetLastError();
+        if ((int64_t)nwork * tn > (int64_t)1 << 30) return hipErrorInvalidValue;
+        return resident ? launch_mv_s<KIND, true>(insts, work, nwork, tn, tw, lds, st)
+                        : launch_mv_s<KIND, false>(insts, work, nwork, tn, tw, lds, st);
     }
     dim3 grid;
     int tm, tn;
@@ -2243,12 +2247,12 @@ static hipError_t launch_mv_kind(const MvInst* insts, const WorkItem* work, int 
     }
     return hipGetLastError();
 }
-hipError_t launch_mv(int kind, const MvInst* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs, int tw,
-                     size_t lds, hipStream_t st) {
+hipError_t launch_mv(int kind, const MvInst* insts, const MvRec* work, int nwork, int max_m, int max_nrhs, int tw,
+                     size_t lds, int resident, hipStream_t st) {
     switch (kind) {
-        case 0: return launch_mv_kind<0>(insts, work, nwork, max_m, max_nrhs, tw, lds, st);
-        case 1: return launch_mv_kind<1>(insts, work, nwork, max_m, max_nrhs, tw, lds, st);
-        default: return launch_mv_kind<2>(insts, work, nwork, max_m, max_nrhs, tw, lds, st);
+        case 0: return launch_mv_kind<0>(insts, work, nwork, max_m, max_nrhs, tw, lds, resident, st);
+        case 1: return launch_mv_kind<1>(insts, work, nwork, max_m, max_nrhs, tw, lds, resident, st);
+        default: return launch_mv_kind<2>(insts, work, nwork, max_m, max_nrhs, tw, lds, resident, st);
     }
 }
 

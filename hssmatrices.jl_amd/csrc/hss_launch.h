@@ -14,10 +14,10 @@ inline int resident_tw(int nwork, int max_nrhs) {
 }
 
 // matvec sweep step over nodes (kind 0 = up: Z, 1 = down: F, 2 = leaves: C); lds > 0
-// selects the LDS-resident kernel with that many bytes and tw columns per workgroup,
-// 0 the chunked one
-hipError_t launch_mv(int kind, const MvInst* insts, const WorkItem* work, int nwork, int max_m, int max_nrhs, int tw,
-                     size_t lds, hipStream_t st);
+// selects the LDS kernel with that many bytes and tw columns per workgroup, with the
+// operator staged (resident != 0) or streamed by the waves; lds == 0 the chunked one
+hipError_t launch_mv(int kind, const MvInst* insts, const MvRec* work, int nwork, int max_m, int max_nrhs, int tw,
+                     size_t lds, int resident, hipStream_t st);
 
 template <typename T>
 hipError_t launch_ulv_factor(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds, hipStream_t st);

@@ -244,7 +244,7 @@ struct hss_mat_s {
     double* d_op = nullptr;  // matvec operators (DevNode::opZ/opF/opC)
     int64_t op_len = 0;
     MvSched mvs;             // matvec schedule of this matrix (instance 0)
-    WorkItem* d_mvwork = nullptr;
+    MvRec* d_mvwork = nullptr;
     // per-call scratch
     SlotCache mvinst;
     double *d_Z = nullptr, *d_F = nullptr;
@@ -398,8 +398,57 @@ static void mv_shape(const hss_mat_s* M, int i, int kind, int& rows, int& cols) 
     }
 }
 
+// The matvec record of node i (kind 0 up, 1 down, 2 leaf) of instance b.
+static MvRec mv_record(const hss_mat_s* M, int b, int i, int kind) {
+    const DevNode& d = M->hn[i];
+    MvRec q{};
+    q.inst = b;
+    int rows, cols;
+    mv_shape(M, i, kind, rows, cols);
+    q.M = rows;
+    q.K = cols;
+    if (kind == 0) {
+        q.A = d.opZ;
+        q.y = d.zrow;
+        q.ldy = std::max(d.kv, 1);
+        if (d.is_leaf) {
+            q.from_b = 1;
+            q.x0 = d.col_off;
+            q.r0 = d.n;
+            q.ld1 = 1;
+        } else {
+            const DevNode &L = M->hn[d.left], &R = M->hn[d.right];
+            q.x0 = L.zrow;
+            q.ld0 = std::max(L.kv, 1);
+            q.r0 = L.kv;
+            q.x1 = R.zrow;
+            q.ld1 = std::max(R.kv, 1);
+        }
+    } else if (kind == 1) {
+        const DevNode& p = M->hn[d.parent];
+        const DevNode& sb = M->hn[d.is_left ? p.right : p.left];
+        q.A = d.opF;
+        q.x0 = sb.zrow;
+        q.ld0 = std::max(sb.kv, 1);
+        q.r0 = sb.kv;
+        q.x1 = p.frow;
+        q.ld1 = std::max(p.ku, 1);
+        q.y = d.frow;
+        q.ldy = std::max(d.ku, 1);
+    } else {
+        q.A = d.opC;
+        q.from_b = 1;
+        q.x0 = d.col_off;
+        q.r0 = d.n;
+        q.x1 = d.frow;
+        q.ld1 = std::max(d.ku, 1);
+        q.y = d.row_off;
+    }
+    return q;
+}
+
 // Appends the matvec work lists of `mats` (instance b = position in mats) to `work`.
-static void build_mv_schedule(const std::vector<hss_mat_s*>& mats, std::vector<WorkItem>& work, MvSched& S) {
+static void build_mv_schedule(const std::vector<hss_mat_s*>& mats, std::vector<MvRec>& work, MvSched& S) {
     int maxh = 0, maxd = 0;
     for (auto* M : mats) {
         maxh = std::max(maxh, M->max_height);
@@ -411,7 +460,7 @@ static void build_mv_schedule(const std::vector<hss_mat_s*>& mats, std::vector<W
             hss_mat_s* M = mats[b];
             for (int i = 0; i < M->nn; ++i) {
                 if (!sel(M, i)) continue;
-                work.push_back(WorkItem{b, i});
+                work.push_back(mv_record(M, b, i, kind));
                 int rows, cols;
                 mv_shape(M, i, kind, rows, cols);
                 mmax = std::max(mmax, rows);
@@ -440,27 +489,32 @@ static void build_mv_schedule(const std::vector<hss_mat_s*>& mats, std::vector<W
     level(2, [&](const hss_mat_s* M, int i) { return M->hn[i].is_leaf != 0; }, S.leaf, S.leaf_m, S.leaf_na, S.leaf_k);
 }
 
-static hss_status launch_mv_schedule(const MvInst* di, const WorkItem* dw, const MvSched& S, int maxnrhs,
+// Each step: operator and input staged in LDS (pick_resident), else the operator
+// streamed by the waves with the input staged (pick_streamed), else the chunked kernel.
+static hss_status launch_mv_schedule(const MvInst* di, const MvRec* dw, const MvSched& S, int maxnrhs,
                                      hipStream_t st) {
-    int tw = 32;
-    size_t lds = 0;
-    auto pick = [&](int64_t na, int64_t k, int count) {
-        return pick_resident(sizeof(double), na, k, count, maxnrhs, tw, lds) ? lds : (size_t)0;
+    static const char* names[3][3] = {{"k_mv_up_s", "k_mv_up_x", "k_mv_up"},
+                                      {"k_mv_down_s", "k_mv_down_x", "k_mv_down"},
+                                      {"k_mv_leaf_s", "k_mv_leaf_x", "k_mv_leaf"}};
+    auto step = [&](int kind, int level, const Range& r, int m, int64_t na, int64_t k) -> hss_status {
+        int tw = 32, form = 2;
+        size_t lds = 0;
+        if (pick_resident(sizeof(double), na, k, r.count, maxnrhs, tw, lds)) form = 0;
+        else if (pick_streamed(sizeof(double), k, m, r.count, maxnrhs, tw, lds)) form = 1;
+        else lds = 0;
+        ProfScope ps(names[kind][form], level, r.count, st);
+        HIP_TRY(launch_mv(kind, di, dw + r.off, r.count, m, maxnrhs, tw, lds, form == 0 ? 1 : 0, st));
+        return HSS_OK;
     };
     for (size_t h = 0; h < S.up.size(); ++h) {
-        const size_t l = pick(S.up_na[h], S.up_k[h], S.up[h].count);
-        ProfScope ps(l ? "k_mv_up_s" : "k_mv_up", (int)h, S.up[h].count, st);
-        HIP_TRY(launch_mv(0, di, dw + S.up[h].off, S.up[h].count, S.up_m[h], maxnrhs, tw, l, st));
+        hss_status s = step(0, (int)h, S.up[h], S.up_m[h], S.up_na[h], S.up_k[h]);
+        if (s != HSS_OK) return s;
     }
     for (size_t d = 1; d < S.down.size(); ++d) {
-        const size_t l = pick(S.down_na[d], S.down_k[d], S.down[d].count);
-        ProfScope ps(l ? "k_mv_down_s" : "k_mv_down", (int)d, S.down[d].count, st);
-        HIP_TRY(launch_mv(1, di, dw + S.down[d].off, S.down[d].count, S.down_m[d], maxnrhs, tw, l, st));
+        hss_status s = step(1, (int)d, S.down[d], S.down_m[d], S.down_na[d], S.down_k[d]);
+        if (s != HSS_OK) return s;
     }
-    const size_t l = pick(S.leaf_na, S.leaf_k, S.leaf.count);
-    ProfScope ps(l ? "k_mv_leaf_s" : "k_mv_leaf", 0, S.leaf.count, st);
-    HIP_TRY(launch_mv(2, di, dw + S.leaf.off, S.leaf.count, S.leaf_m, maxnrhs, tw, l, st));
-    return HSS_OK;
+    return step(2, 0, S.leaf, S.leaf_m, S.leaf_na, S.leaf_k);
 }
 
 // Matvec operators of every node, column-major, each starting on a 16-element boundary.
@@ -576,7 +630,7 @@ extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, 
     std::vector<double> op;
     build_mv_operators(M, blob, op);
     M->op_len = (int64_t)op.size();
-    std::vector<WorkItem> mvwork;
+    std::vector<MvRec> mvwork;
     build_mv_schedule(std::vector<hss_mat_s*>{M}, mvwork, M->mvs);
 
     hipError_t e;
@@ -591,12 +645,12 @@ extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, 
     hipStream_t st = ctx->stream;
     if (blob_len > 0) HIP_TRY(hipMemcpyAsync(M->d_blob, blob, sizeof(double) * blob_len, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(M->d_op, op.data(), sizeof(double) * op.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(M->d_mvwork, mvwork.data(), sizeof(WorkItem) * mvwork.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(M->d_mvwork, mvwork.data(), sizeof(MvRec) * mvwork.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(M->d_nodes, M->hn.data(), sizeof(DevNode) * nn, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(M->d_work, M->hwork.data(), sizeof(WorkItem) * M->hwork.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
     M->dev_bytes = (int64_t)(sizeof(double) * (blob_len + M->op_len) + sizeof(DevNode) * nn +
-                             sizeof(WorkItem) * (M->hwork.size() + mvwork.size()));
+                             sizeof(WorkItem) * M->hwork.size() + sizeof(MvRec) * mvwork.size());
     ctx->refs++;
     *out = M;
     return HSS_OK;
@@ -1540,7 +1594,7 @@ struct hss_plan_s {
     std::vector<MvInst> h_mv;
     BatchVec* d_vec = nullptr;
     std::vector<BatchVec> h_vec;
-    WorkItem* d_work = nullptr;  // matvec work items (after the solve schedule's)
+    MvRec* d_work = nullptr;     // matvec records
     SweepRec* d_rec = nullptr;   // solve sweep records
     Schedule S;
     MvSched mvs;
@@ -1697,11 +1751,14 @@ static hss_status plan_create_impl(int32_t count, hss_fact_s* const* facts, cons
         pf.push_back(q.F);
     }
     build_schedule(pf, P->S);
-    // matvec schedule appended after the solve schedule
-    std::vector<WorkItem> work = P->S.work;
+    // matvec records (refinement plans)
+    std::vector<MvRec> work;
     if (P->has_ir) build_mv_schedule(mats, work, P->mvs);
-    hipError_t e = dalloc(&P->d_work, work.size());
-    if (e == hipSuccess) e = hipMemcpy(P->d_work, work.data(), sizeof(WorkItem) * work.size(), hipMemcpyHostToDevice);
+    hipError_t e = hipSuccess;
+    if (!work.empty()) {
+        e = dalloc(&P->d_work, work.size());
+        if (e == hipSuccess) e = hipMemcpy(P->d_work, work.data(), sizeof(MvRec) * work.size(), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess) e = dalloc(&P->d_rec, P->S.rec.size());
     if (e == hipSuccess)
         e = hipMemcpy(P->d_rec, P->S.rec.data(), sizeof(SweepRec) * P->S.rec.size(), hipMemcpyHostToDevice);
@@ -2121,12 +2178,12 @@ static hss_status server_round(hss_server_s* S) {
     std::vector<hss_fact_s*> rf(cnt);
     for (int i = 0; i < cnt; ++i) rf[i] = S->facts[S->inflight[i]->mid];
     build_schedule(rf, sch);
-    std::vector<WorkItem> work = sch.work;
+    std::vector<MvRec> work;
     MvSched mvs;
     build_mv_schedule(mats, work, mvs);
     // one table upload per round (the stream is idle here)
     const size_t o_si = 0, o_mv = align256(sizeof(SInst<TF>) * cnt), o_bv = o_mv + align256(sizeof(MvInst) * cnt),
-                 o_w = o_bv + align256(sizeof(BatchVec) * cnt), o_r = o_w + align256(sizeof(WorkItem) * work.size()),
+                 o_w = o_bv + align256(sizeof(BatchVec) * cnt), o_r = o_w + align256(sizeof(MvRec) * work.size()),
                  tab = o_r + sizeof(SweepRec) * sch.rec.size();
     s = grow(&S->d_tab, &S->tab_cap, tab);
     if (s != HSS_OK) return s;
@@ -2134,7 +2191,7 @@ static hss_status server_round(hss_server_s* S) {
     std::memcpy(h.data() + o_si, si.data(), sizeof(SInst<TF>) * cnt);
     std::memcpy(h.data() + o_mv, mv.data(), sizeof(MvInst) * cnt);
     std::memcpy(h.data() + o_bv, bv.data(), sizeof(BatchVec) * cnt);
-    std::memcpy(h.data() + o_w, work.data(), sizeof(WorkItem) * work.size());
+    std::memcpy(h.data() + o_w, work.data(), sizeof(MvRec) * work.size());
     std::memcpy(h.data() + o_r, sch.rec.data(), sizeof(SweepRec) * sch.rec.size());
     HIP_TRY(hipMemcpy(S->d_tab, h.data(), tab, hipMemcpyHostToDevice));
     const size_t nb = sizeof(double) * 2 * (size_t)cnt * maxnrhs;
@@ -2143,7 +2200,7 @@ static hss_status server_round(hss_server_s* S) {
     auto* dsi = reinterpret_cast<const SInst<TF>*>(S->d_tab + o_si);
     auto* dmv = reinterpret_cast<const MvInst*>(S->d_tab + o_mv);
     auto* dbv = reinterpret_cast<BatchVec*>(S->d_tab + o_bv);
-    auto* dw = reinterpret_cast<const WorkItem*>(S->d_tab + o_w);
+    auto* dw = reinterpret_cast<const MvRec*>(S->d_tab + o_w);
     auto* dr = reinterpret_cast<const SweepRec*>(S->d_tab + o_r);
     // 3. one batched solve, the corrections, one batched matvec + residual
     s = launch_schedule<TF>(dsi, dr, sch, maxnrhs, lgroot, lgroot_res, all_groot2 ? lgroot2 : 0, lgroot2s, st);
