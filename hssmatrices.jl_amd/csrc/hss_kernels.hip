@@ -1645,8 +1645,8 @@ __device__ __forceinline__ void up_gemms(int nk, int Kx, int R, const TF* Gz, co
 // tw + 1.  AR: the operator [Gz Gr] is staged too (LDS = [Gz Gr] (nk Kx + R (Kx + nk),
 // rounded to 16) | xs (Kx + nk) rows); otherwise the waves stream it from memory
 // (LDS = xs), for operators too large to stage whose rows fill few 128-row tiles.
-template <typename TF, bool AR>
-__global__ __launch_bounds__(512, 4) void k_ulv_gup_s(const SInst<TF>* __restrict__ insts,
+template <typename TF, bool AR, int NT>
+__global__ __launch_bounds__(NT, 4) void k_ulv_gup_s(const SInst<TF>* __restrict__ insts,
                                                    const SweepRec* __restrict__ work, int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
@@ -1701,8 +1701,8 @@ __global__ __launch_bounds__(512, 4) void k_ulv_gup_s(const SInst<TF>* __restric
 
 // Down-sweep with the input staged in LDS.  AR: Qt is staged too (LDS = Qt (n x n,
 // rounded to 16; empty for a pass-through node) | xs n rows); otherwise streamed.
-template <typename TF, bool AR>
-__global__ __launch_bounds__(512, 4) void k_ulv_gdown_s(const SInst<TF>* __restrict__ insts,
+template <typename TF, bool AR, int NT>
+__global__ __launch_bounds__(NT, 4) void k_ulv_gdown_s(const SInst<TF>* __restrict__ insts,
                                                      const SweepRec* __restrict__ work, int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
@@ -1854,8 +1854,8 @@ __device__ __forceinline__ void mv_store(const MvInst& I, const MvOp& o, int i, 
 // One matvec step per (node, tw-column tile) with the input staged in LDS (xs, K rows,
 // pitch tw + 1).  AR: the operator is staged too (LDS = A (M x K, rounded to 16) | xs);
 // otherwise the waves stream it from memory (LDS = xs).  Strip products as in the solve.
-template <int KIND, bool AR>
-__global__ __launch_bounds__(512, 4) void k_mv_s(const MvInst* __restrict__ insts, const MvRec* __restrict__ work,
+template <int KIND, bool AR, int NT>
+__global__ __launch_bounds__(NT, 4) void k_mv_s(const MvInst* __restrict__ insts, const MvRec* __restrict__ work,
                                               int nwork, int tn, int tw) {
     int item, ti, tj;
     if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
@@ -2123,16 +2123,23 @@ hipError_t launch_ulv_gup(const SInst<T>* insts, const SweepRec* work, int nwork
                        tn);
     return hipGetLastError();
 }
-template <typename T, bool AR>
-static hipError_t launch_gup_s(const SInst<T>* insts, const SweepRec* work, int nwork, int tn, int tw, size_t lds,
-                               hipStream_t st) {
+// LDS sweep kernels run 256-thread workgroups on wide levels (wide_level), 512 otherwise.
+template <typename T, bool AR, int NT>
+static hipError_t launch_gup_s_nt(const SInst<T>* insts, const SweepRec* work, int nwork, int tn, int tw, size_t lds,
+                                  hipStream_t st) {
     const int64_t total = (int64_t)nwork * tn;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gup_s<T, AR>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gup_s<T, AR, NT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_gup_s<T, AR>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts, work,
-                       nwork, tn, tw);
+    hipLaunchKernelGGL((k_ulv_gup_s<T, AR, NT>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(NT), lds, st, insts,
+                       work, nwork, tn, tw);
     return hipGetLastError();
+}
+template <typename T, bool AR>
+static hipError_t launch_gup_s(const SInst<T>* insts, const SweepRec* work, int nwork, int tn, int tw, size_t lds,
+                               bool wide, hipStream_t st) {
+    return wide ? launch_gup_s_nt<T, AR, 256>(insts, work, nwork, tn, tw, lds, st)
+                : launch_gup_s_nt<T, AR, 512>(insts, work, nwork, tn, tw, lds, st);
 }
 template <typename T>
 hipError_t launch_ulv_gup_s(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, int tw,
@@ -2141,19 +2148,26 @@ hipError_t launch_ulv_gup_s(const SInst<T>* insts, const SweepRec* work, int nwo
     if (tw != 16 && tw != 32) return hipErrorInvalidValue;
     const int tn = (max_nrhs + tw - 1) / tw;
     if ((int64_t)nwork * tn > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
-    return resident ? launch_gup_s<T, true>(insts, work, nwork, tn, tw, lds, st)
-                    : launch_gup_s<T, false>(insts, work, nwork, tn, tw, lds, st);
+    const bool wide = wide_level(nwork, max_nrhs);
+    return resident ? launch_gup_s<T, true>(insts, work, nwork, tn, tw, lds, wide, st)
+                    : launch_gup_s<T, false>(insts, work, nwork, tn, tw, lds, wide, st);
+}
+template <typename T, bool AR, int NT>
+static hipError_t launch_gdown_s_nt(const SInst<T>* insts, const SweepRec* work, int nwork, int tn, int tw,
+                                    size_t lds, hipStream_t st) {
+    const int64_t total = (int64_t)nwork * tn;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gdown_s<T, AR, NT>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_ulv_gdown_s<T, AR, NT>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(NT), lds, st, insts,
+                       work, nwork, tn, tw);
+    return hipGetLastError();
 }
 template <typename T, bool AR>
 static hipError_t launch_gdown_s(const SInst<T>* insts, const SweepRec* work, int nwork, int tn, int tw, size_t lds,
-                                 hipStream_t st) {
-    const int64_t total = (int64_t)nwork * tn;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gdown_s<T, AR>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_gdown_s<T, AR>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts,
-                       work, nwork, tn, tw);
-    return hipGetLastError();
+                                 bool wide, hipStream_t st) {
+    return wide ? launch_gdown_s_nt<T, AR, 256>(insts, work, nwork, tn, tw, lds, st)
+                : launch_gdown_s_nt<T, AR, 512>(insts, work, nwork, tn, tw, lds, st);
 }
 template <typename T>
 hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, int tw,
@@ -2162,8 +2176,9 @@ hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const SweepRec* work, int n
     if (tw != 16 && tw != 32) return hipErrorInvalidValue;
     const int tn = (max_nrhs + tw - 1) / tw;
     if ((int64_t)nwork * tn > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
-    return resident ? launch_gdown_s<T, true>(insts, work, nwork, tn, tw, lds, st)
-                    : launch_gdown_s<T, false>(insts, work, nwork, tn, tw, lds, st);
+    const bool wide = wide_level(nwork, max_nrhs);
+    return resident ? launch_gdown_s<T, true>(insts, work, nwork, tn, tw, lds, wide, st)
+                    : launch_gdown_s<T, false>(insts, work, nwork, tn, tw, lds, wide, st);
 }
 template <typename T>
 hipError_t launch_ulv_gdown(const SInst<T>* insts, const SweepRec* work, int nwork, int max_m, int max_nrhs,
@@ -2214,16 +2229,22 @@ hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, do
     return hipGetLastError();
 }
 
-template <int KIND, bool AR>
-static hipError_t launch_mv_s(const MvInst* insts, const MvRec* work, int nwork, int tn, int tw, size_t lds,
-                              hipStream_t st) {
+template <int KIND, bool AR, int NT>
+static hipError_t launch_mv_s_nt(const MvInst* insts, const MvRec* work, int nwork, int tn, int tw, size_t lds,
+                                 hipStream_t st) {
     const int64_t total = (int64_t)nwork * tn;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mv_s<KIND, AR>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mv_s<KIND, AR, NT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_mv_s<KIND, AR>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(512), lds, st, insts, work,
-                       nwork, tn, tw);
+    hipLaunchKernelGGL((k_mv_s<KIND, AR, NT>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(NT), lds, st, insts,
+                       work, nwork, tn, tw);
     return hipGetLastError();
+}
+template <int KIND, bool AR>
+static hipError_t launch_mv_s(const MvInst* insts, const MvRec* work, int nwork, int tn, int tw, size_t lds,
+                              bool wide, hipStream_t st) {
+    return wide ? launch_mv_s_nt<KIND, AR, 256>(insts, work, nwork, tn, tw, lds, st)
+                : launch_mv_s_nt<KIND, AR, 512>(insts, work, nwork, tn, tw, lds, st);
 }
 template <int KIND>
 static hipError_t launch_mv_kind(const MvInst* insts, const MvRec* work, int nwork, int max_m, int max_nrhs, int tw,
@@ -2233,8 +2254,9 @@ static hipError_t launch_mv_kind(const MvInst* insts, const MvRec* work, int nwo
         if (lds > 160 * 1024 || (tw != 16 && tw != 32)) return hipErrorInvalidValue;
         const int tn = (max_nrhs + tw - 1) / tw;
         if ((int64_t)nwork * tn > (int64_t)1 << 30) return hipErrorInvalidValue;
-        return resident ? launch_mv_s<KIND, true>(insts, work, nwork, tn, tw, lds, st)
-                        : launch_mv_s<KIND, false>(insts, work, nwork, tn, tw, lds, st);
+        const bool wide = wide_level(nwork, max_nrhs);
+        return resident ? launch_mv_s<KIND, true>(insts, work, nwork, tn, tw, lds, wide, st)
+                        : launch_mv_s<KIND, false>(insts, work, nwork, tn, tw, lds, wide, st);
     }
     dim3 grid;
     int tm, tn;

@@ -8,6 +8,11 @@ namespace hss {
 
 // Column width of the LDS-resident steps (16 or 32): 16 when a level has too few 32-column
 // tiles to occupy the CUs (halves each workgroup's MFMA chain) or at most 16 columns.
+// A level is wide when it has at least 2048 (node, 32-column) tiles, eight per CU: its
+// LDS kernels then run 256-thread workgroups with the operators streamed (more
+// independent workgroups per CU hide each one's staging); narrower levels run 512
+// threads per workgroup (more waves per node).
+inline bool wide_level(int64_t nwork, int max_nrhs) { return nwork * ((max_nrhs + 31) / 32) >= 2048; }
 inline int resident_tw(int nwork, int max_nrhs) {
     if (max_nrhs <= 16) return 16;
     return ((int64_t)nwork * ((max_nrhs + 31) / 32) < 128) ? 16 : 32;

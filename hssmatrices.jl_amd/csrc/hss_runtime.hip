@@ -152,12 +152,14 @@ struct Schedule {
 };
 // Largest LDS of a workgroup (bytes); levels above it use the streamed kernels.
 constexpr size_t kLdsResidentMax = 160 * 1024;
-// A level runs LDS-resident (operator and input staged) when the staging fits and either
-// the operators are small (at most 72 KiB) or the level has at most one workgroup per CU:
-// a wide level of large operators keeps more bytes in flight streaming them.  tw =
+// A level that is not wide runs LDS-resident (operator and input staged) when the
+// staging fits and either the operators are small (at most 72 KiB) or the level has at
+// most one workgroup per CU: a level of many large operators keeps more bytes in flight
+// streaming them.  tw =
 // columns per workgroup, lds = bytes per workgroup; na = the level's largest operator
 // (elements of w bytes, rounded to 16), k = its most input rows.
 static bool pick_resident(size_t w, int64_t na, int64_t k, int count, int maxnrhs, int& tw, size_t& lds) {
+    if (wide_level(count, maxnrhs)) return false;  // wide levels stream their operators
     for (int t : {resident_tw(count, maxnrhs), 16}) {
         lds = w * (size_t)(na + k * (t + 1));
         const int64_t wgs = (int64_t)count * ((maxnrhs + t - 1) / t);

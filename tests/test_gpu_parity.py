@@ -311,6 +311,59 @@ def test_plan_mixed_precision_lockstep_refinement(hip, oracle):
     plan.close()
 
 
+def test_wide_batch_levels(hip, oracle):
+    """Levels of at least 2048 (node, 32-column) tiles run 256-thread workgroups with the
+    operators streamed; every instance matches its own single solve and the oracle, and an
+    FP32 lockstep plan over the same levels (wide matvec too) reaches its residual."""
+    import torch
+    rng = np.random.default_rng(31)
+    _, h = random_hss(rng, 4096, 32, 0, "cauchy", 1e-8)
+    nodes, blob = H.pack(h)
+    copies = 20  # 20 x 128 leaves
+    mats = [hip.HipHssMatrix((nodes, blob)) for _ in range(copies)]
+    n = int(nodes["n"][0])
+    dev = torch.device("cuda:0")
+    Bs, Xs, nrhs = [], [], []
+    for i in range(copies):
+        r = 1 + 7 * (i % 5)
+        Bs.append(torch.from_numpy(np.ascontiguousarray(rng.standard_normal((r, n)))).to(dev))
+        Xs.append(torch.zeros((r, n), dtype=torch.float64, device=dev))
+        nrhs.append(r)
+    torch.cuda.synchronize()
+    facts = hip.ulvfactor_batch(mats, "fp64")
+    hip.solve_batch_dev(facts, nrhs, [b.data_ptr() for b in Bs], [n] * copies, [x.data_ptr() for x in Xs],
+                        [n] * copies)
+    torch.cuda.synchronize()
+    single = mats[0].ulvfactor("fp64")
+    for i in range(copies):
+        B = Bs[i].cpu().numpy().T
+        Xb = Xs[i].cpu().numpy().T
+        assert rel(Xb, single.solve(B)) < 1e-13
+    def resid(X, B):
+        R = oracle.matmul(nodes, blob, X) - B
+        return np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0))
+
+    for i in (0, 3, copies - 1):
+        # as accurate as the reference: residual within 10x the oracle's own
+        B = Bs[i].cpu().numpy().T
+        X = Xs[i].cpu().numpy().T
+        Xo = oracle.ulvfactsolve(nodes, blob, B)
+        assert resid(X, B) <= 10 * resid(Xo, B) + 1e-14
+        assert rel(X, Xo) < 1e-10
+    f32 = hip.ulvfactor_batch(mats, "fp32")
+    for x in Xs:
+        x.zero_()
+    torch.cuda.synchronize()
+    plan = hip.Plan(f32, nrhs, [b.data_ptr() for b in Bs], [n] * copies, [x.data_ptr() for x in Xs], [n] * copies,
+                    refinement=True)
+    rep = plan.execute(tol=1e-10, max_iter=20)
+    torch.cuda.synchronize()
+    assert rep["rel_residual"] <= 1e-10
+    for i in (0, 7, copies - 1):
+        assert resid(Xs[i].cpu().numpy().T, Bs[i].cpu().numpy().T) <= 1e-10
+    plan.close()
+
+
 def test_server_alone_equals_mixed(hip, oracle):
     """Continuous batching: each result is the same alone and in a mix with other requests."""
     import torch
