@@ -51,6 +51,30 @@ template <typename T>
 __device__ __forceinline__ gptr<T> gbl(T* p) {
     return (gptr<T>)p;
 }
+#if defined(__HIP_DEVICE_COMPILE__)
+// Raw buffer descriptor over [p, p + bytes) (gfx9 format word 0x00020000) and loads through
+// it: the descriptor and a per-instruction offset live in scalar registers.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+#else
+struct HostRsrc {};
+__device__ inline HostRsrc buf_rsrc(const void*, int) { return {}; }
+template <typename T>
+__device__ inline T buf_load(HostRsrc, int, int) {
+    return T(0);
+}
+#endif
 template <typename T>
 __device__ __forceinline__ T shfl_xor(T v, int mask) {
     return __shfl_xor(v, mask, 64);
@@ -1500,10 +1524,18 @@ __device__ __forceinline__ void strip_acc(int M, int K, const TF* __restrict__ A
         return;
     }
     constexpr int U = 8;
-    const gptr<const TF> Ai = gbl(A) + ic;
-    auto ld = [&](int st) {
-        const int kk = 4 * (st < S ? st : S - 1) + lk;
-        return Ai[(int64_t)(kk < K ? kk : K - 1) * lda];
+    // A through buffer loads: the descriptor and the per-step offset live in scalar
+    // registers, the lane's row/column offset in one VGPR, so the prefetch costs two
+    // VGPRs per step.  Every access is inside A: the lane's column is clamped to K - 1
+    // (full steps never need it) and steps at or past Sf are loaded by the edge code.
+    const int kl = lk < K ? lk : K - 1;
+    const int voff = (ic + kl * (int)lda) * (int)sizeof(TF);
+    const int step = 4 * (int)lda * (int)sizeof(TF);
+    const auto rsrc = buf_rsrc(A, (int)((lda * (K - 1) + M) * (int64_t)sizeof(TF)));
+    auto ld = [&](int st) { return buf_load<TF>(rsrc, voff, (st < Sf ? st : (Sf > 0 ? Sf - 1 : 0)) * step); };
+    auto ld_edge = [&](int st) {
+        const int kk = 4 * st + lk;
+        return buf_load<TF>(rsrc, (ic + (kk < K ? kk : K - 1) * (int)lda) * (int)sizeof(TF), 0);
     };
     TF a[U];
 #pragma unroll
@@ -1515,21 +1547,26 @@ __device__ __forceinline__ void strip_acc(int M, int K, const TF* __restrict__ A
 #pragma unroll
         for (int u = 0; u < U; ++u) nxt[u] = ld(b0s + U + u);
         const TF* xb = xs + (4 * b0s + lk) * xp + lr;
-        TF b0[U], b1[U];
+        // x from LDS in groups of G steps (registers go to the A prefetch)
+        constexpr int G = 4;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            b0[u] = xb[4 * u * xp];
-            if (two) b1[u] = xb[4 * u * xp + 16];
-        }
+        for (int g = 0; g < U; g += G) {
+            TF b0[G], b1[G];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (two) {
-                acc0 = Mfma<TF>::mac(cur[u], b0[u], acc0);
-                acc1 = Mfma<TF>::mac(cur[u], b1[u], acc1);
-            } else if (u & 1) {
-                acc1 = Mfma<TF>::mac(cur[u], b0[u], acc1);
-            } else {
-                acc0 = Mfma<TF>::mac(cur[u], b0[u], acc0);
+            for (int u = 0; u < G; ++u) {
+                b0[u] = xb[4 * (g + u) * xp];
+                if (two) b1[u] = xb[4 * (g + u) * xp + 16];
+            }
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                if (two) {
+                    acc0 = Mfma<TF>::mac(cur[g + u], b0[u], acc0);
+                    acc1 = Mfma<TF>::mac(cur[g + u], b1[u], acc1);
+                } else if (u & 1) {
+                    acc1 = Mfma<TF>::mac(cur[g + u], b0[u], acc1);
+                } else {
+                    acc0 = Mfma<TF>::mac(cur[g + u], b0[u], acc0);
+                }
             }
         }
     };
@@ -1544,14 +1581,11 @@ __device__ __forceinline__ void strip_acc(int M, int K, const TF* __restrict__ A
 #pragma unroll
         for (int u = 0; u < U; ++u) a[u] = an[u];
     }
-    for (; s0 < S; s0 += U) {
+    // the remaining steps: full ones from the prefetch, the partial one loaded here
 #pragma unroll
-        for (int u = 0; u < U; ++u) an[u] = ld(s0 + U + u);
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (s0 + u < S) edge(u, s0 + u, a[u]);
-#pragma unroll
-        for (int u = 0; u < U; ++u) a[u] = an[u];
+    for (int u = 0; u < U; ++u) {
+        const int st = s0 + u;
+        if (st < S) edge(u, st, st < Sf ? a[u] : ld_edge(st));
     }
 }
 
@@ -1597,8 +1631,8 @@ __device__ __forceinline__ void glb_gemm(int M, int K, const TF* __restrict__ A,
 //   [b; u] = Gr [x; zloc] (R x (Kx + nk), lda R) -> st1
 // The strips of Gr start on the x part (its first Kx columns) while other waves compute
 // zloc, hold their sums across the barrier that publishes zloc, then add the zloc part.
-// Waves [0, n2) own the Gr strips, the Gz strips go to the waves after them.  (With more
-// Gr strips than waves the two products run one after the other.)  All threads of the
+// Waves [0, n2) own the Gr strips, the Gz strips go to the other waves.  (Unless every
+// wave owns a Gr strip; then the two products run one after the other.)  All threads of the
 // workgroup call it; it contains a barrier.
 template <bool AL, bool two, typename TF, typename S0, typename S1>
 __device__ __forceinline__ void up_gemms_cb(int nk, int Kx, int R, const TF* Gz, const TF* Gr, const TF* xs, int xp,
@@ -1607,7 +1641,7 @@ __device__ __forceinline__ void up_gemms_cb(int nk, int Kx, int R, const TF* Gz,
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int n1 = (nk + 15) / 16, n2 = (R + 15) / 16;
     const V4 z4 = V4{TF(0), TF(0), TF(0), TF(0)};
-    if (n2 > nw) {
+    if (n2 >= nw) {
         for (int rb = wave; rb < n1; rb += nw) {
             V4 a0 = z4, a1 = z4;
             strip_acc<AL, two>(nk, Kx, Gz, nk, xs, xp, rb, a0, a1);
@@ -1621,7 +1655,10 @@ __device__ __forceinline__ void up_gemms_cb(int nk, int Kx, int R, const TF* Gz,
         }
         return;
     }
-    for (int rb = (wave - n2 + nw) % nw; rb < n1; rb += nw) {
+    // Gz strips go to the waves that own no Gr strip (round robin over them), so the
+    // MFMA work of each SIMD's waves stays balanced
+    const int nf = nw - n2;
+    for (int rb = wave - n2; wave >= n2 && rb < n1; rb += nf) {
         V4 a0 = z4, a1 = z4;
         strip_acc<AL, two>(nk, Kx, Gz, nk, xs, xp, rb, a0, a1);
         strip_put<two, TF>(nk, ncols, rb, lane, a0, a1, st0);
