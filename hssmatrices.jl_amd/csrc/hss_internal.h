@@ -6,6 +6,17 @@
 
 namespace hss {
 
+// Device memory pointers in argument tables: in the device compilation of the kernels they
+// carry the global address space, so the compiler emits global (not flat) loads and stores
+// through them, also after they pass through inlined helpers taking plain pointers.  The
+// layout is the same as a plain pointer's; the host runtime (which only fills the tables)
+// sees plain pointers.
+#if defined(__HIP_DEVICE_COMPILE__) && defined(HSS_KERNELS_TU)
+#define HSS_GLOBAL __attribute__((address_space(1)))
+#else
+#define HSS_GLOBAL
+#endif
+
 // Reflector block size of the compact-WY representation used by the solve.
 constexpr int NB = 16;
 // Threads of the factor kernel (one workgroup per tree node).
@@ -125,13 +136,13 @@ struct MvInst {
 // Per-instance arguments of a factorization.
 template <typename TF>
 struct FInst {
-    const DevNode* nodes;
-    const FNode* fn;
-    const double* blob;
-    TF* fac;
-    TF* rw;
-    int32_t* ipiv;
-    int32_t* status;  // first zero pivot (1-based) of the root LU, 0 = none
+    const DevNode HSS_GLOBAL* nodes;
+    const FNode HSS_GLOBAL* fn;
+    const double HSS_GLOBAL* blob;
+    TF HSS_GLOBAL* fac;
+    TF HSS_GLOBAL* rw;
+    int32_t HSS_GLOBAL* ipiv;
+    int32_t HSS_GLOBAL* status;  // first zero pivot (1-based) of the root LU, 0 = none
 };
 
 // Per-instance arguments of a solve.  TF = element type of the factor and of the

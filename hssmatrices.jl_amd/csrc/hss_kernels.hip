@@ -27,6 +27,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#define HSS_KERNELS_TU
 #include "hss_internal.h"
 #include "hss_launch.h"
 
@@ -1166,7 +1167,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_rinv(const FInst<TF>* __r
         bv[e] = (j < nt && r == j0 + j) ? TF(1) : TF(0);
     }
     __syncthreads();
-    lu_solve(bv, t1, m, nt, I.fac + f.LU, I.fac + f.LUinv, I.ipiv + f.piv);
+    lu_solve<TF>(bv, t1, m, nt, I.fac + f.LU, I.fac + f.LUinv, I.ipiv + f.piv);
     unstage(I.fac + f.Dinv + (int64_t)j0 * m, m, bv, 0, m, nt);
 }
 
