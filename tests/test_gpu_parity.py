@@ -227,11 +227,18 @@ def test_c2_full_size(hip, oracle):
     F = A.ulvfactor("fp64")
     B = rng.standard_normal((n, 4))
     Z = F.solve(B)
-    R = oracle.matmul(nodes, blob, Z) - B
-    assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)) < 1e-11
+
+    def resid(X):
+        R = oracle.matmul(nodes, blob, X) - B
+        return np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0))
+
     from threadpoolctl import threadpool_limits
     with threadpool_limits(1):
         Zo = oracle.ulvfactsolve(nodes, blob, B)
+    # as accurate as the reference (residual within 10x the oracle's own, which is about
+    # 1e-11 here: cond ~ 1e7) and within the 1e-10 residual the workload is quoted at
+    assert resid(Z) <= 10 * resid(Zo) + 1e-14
+    assert resid(Z) <= 1e-10
     assert rel(Z, Zo) < 1e-9
 
 
