@@ -666,6 +666,63 @@ static void mat_release(hss_mat_s* M) {
     free_mat(M);
     ctx_release(c);
 }
+// Adjoint of an HSS matrix as a new matrix on the same context (reference
+// src/hssmatrix.jl:165-171): leaf D -> D', U <-> V; branch B12 -> B21', B21 -> B12';
+// every node's R <-> W; rows and columns exchange sizes and offsets.  The generators are
+// rebuilt on the host from the device copy and packed in the same node order.
+extern "C" hss_status hss_mat_adjoint(hss_mat_t A, hss_mat_t* out) {
+    if (!A || !out) return fail(HSS_ERR_ARG, "null argument");
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(A->ctx->device));
+    std::vector<double> src((size_t)std::max<int64_t>(A->blob_len, 0));
+    if (A->blob_len > 0)
+        HIP_TRY(hipMemcpy(src.data(), A->d_blob, sizeof(double) * (size_t)A->blob_len, hipMemcpyDeviceToHost));
+    const int nn = A->nn;
+    const std::vector<hss_node_desc>& d = A->desc;
+    std::vector<hss_node_desc> t(d);
+    std::vector<double> blob;
+    blob.reserve(src.size());
+    // copy of a rows x cols block (transposed when trans), or -1 for an absent generator
+    auto put = [&](int64_t off, int64_t rows, int64_t cols, bool trans) -> int64_t {
+        if (off < 0) return -1;
+        const int64_t at = (int64_t)blob.size();
+        if (!trans) {
+            blob.insert(blob.end(), src.begin() + off, src.begin() + off + rows * cols);
+        } else {  // (cols x rows) = block'
+            blob.resize(blob.size() + (size_t)(rows * cols));
+            for (int64_t j = 0; j < rows; ++j)
+                for (int64_t i = 0; i < cols; ++i) blob[at + i + j * cols] = src[off + j + i * rows];
+        }
+        return at;
+    };
+    for (int i = 0; i < nn; ++i) {
+        const hss_node_desc& x = d[i];
+        hss_node_desc& y = t[i];
+        y.m = x.n;
+        y.n = x.m;
+        y.ku = x.kv;
+        y.kv = x.ku;
+        y.row_off = x.col_off;
+        y.col_off = x.row_off;
+        y.off_D = y.off_U = y.off_V = y.off_B12 = y.off_B21 = y.off_R = y.off_W = -1;
+        if (x.left < 0) {
+            y.off_D = put(x.off_D, x.m, x.n, true);
+            y.off_U = put(x.off_V, x.n, x.kv, false);
+            y.off_V = put(x.off_U, x.m, x.ku, false);
+        } else {
+            const hss_node_desc &l = d[x.left], &r = d[x.right];
+            y.off_B12 = put(x.off_B21, r.ku, l.kv, true);
+            y.off_B21 = put(x.off_B12, l.ku, r.kv, true);
+        }
+        if (x.parent >= 0) {
+            const hss_node_desc& p = d[x.parent];
+            y.off_R = put(x.off_W, x.kv, p.kv, false);
+            y.off_W = put(x.off_R, x.ku, p.ku, false);
+        }
+    }
+    return hss_mat_create(A->ctx, t.data(), nn, blob.data(), (int64_t)blob.size(), out);
+}
+
 extern "C" hss_status hss_mat_destroy(hss_mat_t M) {
     mat_release(M);
     return HSS_OK;

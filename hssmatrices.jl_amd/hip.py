@@ -14,6 +14,9 @@ reference (bonevbs/HssMatrices.jl)     here
 ``ulvfactsolve(hssA, b)``              ``ulvfactsolve(hssA, b)``
   ``src/ulvfactor.jl:10``
 (no counterpart: refactors each call)  ``hssA.ulvfactor(prec)`` -> resident ``HipUlvFactor``
+``hssA'`` (``hssmatrix.jl:165-171``)   ``hssA.adjoint()`` / ``hssA.H``
+``A * hssB`` (``matmul.jl:14``)        ``A @ hssB``  (``HipHssMatrix.__rmatmul__``)
+``A / hssB`` (``hssmatrix.jl:236``)    ``A / hssB`` / ``hssB.rdiv(A)``
 =====================================  =============================================
 
 Errors follow the reference: ``DimensionMismatch`` (``matmul.jl:19-20``),
@@ -41,7 +44,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 # every symbol include/hss_hip.h declares
 EXPORTS = [
     "hss_ctx_create", "hss_ctx_destroy", "hss_ctx_stream",
-    "hss_mat_create", "hss_mat_destroy", "hss_mat_size", "hss_mat_device_bytes",
+    "hss_mat_create", "hss_mat_destroy", "hss_mat_adjoint", "hss_mat_size", "hss_mat_device_bytes",
     "hss_matmul", "hss_matmul_dev",
     "hss_ulvfactsolve", "hss_ulv_factor", "hss_fact_destroy", "hss_fact_device_bytes",
     "hss_ulv_solve", "hss_ulv_solve_dev", "hss_ulv_solve_batch_dev", "hss_ulv_factor_batch",
@@ -92,6 +95,7 @@ def load_library(path: pathlib.Path | str | None = None):
         "hss_ctx_stream": ([V], V),
         "hss_mat_create": ([V, V, I32, V, I64, PV], ctypes.c_int),
         "hss_mat_destroy": ([V], ctypes.c_int),
+        "hss_mat_adjoint": ([V, PV], ctypes.c_int),
         "hss_mat_size": ([V, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
         "hss_mat_device_bytes": ([V], I64),
         "hss_matmul": ([V, I32, D, V, I64, D, V, I64], ctypes.c_int),
@@ -226,12 +230,28 @@ def _as_f64_fortran(B):
     return np.asfortranarray(B), vec
 
 
+class _Adopt:
+    """A matrix handle created by the library, with its host node table."""
+
+    def __init__(self, handle, nodes):
+        self.handle = handle
+        self.nodes = nodes
+
+
 class HipHssMatrix:
     """Device-resident HSS matrix (``hss_mat_create``); the drop-in for ``HssMatrix{Float64}``."""
+
+    # numpy operands hand ``X @ A`` and ``X / A`` to __rmatmul__ / __rtruediv__
+    __array_ufunc__ = None
 
     def __init__(self, hss, ctx: Context | None = None):
         lib = load_library()
         self.ctx = ctx or default_context()
+        if isinstance(hss, _Adopt):  # a handle made by the library (hss_mat_adjoint)
+            self.nodes = hss.nodes
+            self._h = hss.handle
+            self.shape = (int(self.nodes["m"][0]), int(self.nodes["n"][0]))
+            return
         if isinstance(hss, HssMatrix):
             nodes, blob = pack(hss)
         else:
@@ -280,6 +300,48 @@ class HipHssMatrix:
     def matmul_dev(self, nrhs: int, alpha: float, dB: int, ldb: int, beta: float, dC: int, ldc: int):
         """``hss_matmul_dev`` on device pointers (e.g. ``torch.Tensor.data_ptr()``)."""
         _check(_lib.hss_matmul_dev(self._h, nrhs, alpha, dB, ldb, beta, dC, ldc))
+
+    # ---- adjoint (reference hssmatrix.jl:165-171) ----
+    def adjoint(self) -> "HipHssMatrix":
+        """``hssA'`` as a new device matrix (``hss_mat_adjoint``): D', U <-> V, B12 <- B21',
+        B21 <- B12', R <-> W."""
+        h = ctypes.c_void_p()
+        _check(_lib.hss_mat_adjoint(self._h, ctypes.byref(h)))
+        t = self.nodes.copy()
+        t["m"], t["n"] = self.nodes["n"], self.nodes["m"]
+        t["ku"], t["kv"] = self.nodes["kv"], self.nodes["ku"]
+        t["row_off"], t["col_off"] = self.nodes["col_off"], self.nodes["row_off"]
+        return HipHssMatrix(_Adopt(h, t), self.ctx)
+
+    @property
+    def H(self) -> "HipHssMatrix":
+        return self.adjoint()
+
+    def __rmatmul__(self, A):
+        """``A * hssB`` = ``(hssB' * A')'`` (reference ``matmul.jl:14``)."""
+        Af = np.asarray(A, dtype=np.float64)
+        if Af.ndim != 2 or Af.shape[1] != self.shape[0]:
+            raise DimensionMismatch(f"Second dimension of A does not match first dimension of B. "
+                                    f"Expected {self.shape[0]}, got {Af.shape[-1]}")
+        At = self.adjoint()
+        try:
+            return (At @ np.asfortranarray(Af.T)).T
+        finally:
+            At.close()
+
+    def rdiv(self, A):
+        """``A / hssB`` = ``ulvfactsolve(hssB', A')'`` (reference ``hssmatrix.jl:236``)."""
+        Af = np.asarray(A, dtype=np.float64)
+        if Af.ndim != 2 or Af.shape[1] != self.shape[1]:
+            raise DimensionMismatch(f"Second dimension of A does not match second dimension of B. "
+                                    f"Expected {self.shape[1]}, got {Af.shape[-1]}")
+        At = self.adjoint()
+        try:
+            return ulvfactsolve(At, np.asfortranarray(Af.T)).T
+        finally:
+            At.close()
+
+    __rtruediv__ = rdiv
 
     # ---- ULV ----
     def ulvfactor(self, prec: str = "fp64") -> "HipUlvFactor":

@@ -12,6 +12,9 @@
 #   \(A::HipHss, B)            <- \(hssA, B::Matrix)     src/hssmatrix.jl:234
 #   ulvfactsolve(A::HipHss, b) <- ulvfactsolve           src/ulvfactor.jl:10
 #   ulvfactor(A::HipHss)       -> resident factor (no reference counterpart)
+#   adjoint(A::HipHss), A'     <- adjoint(hssA)         src/hssmatrix.jl:165-171
+#   *(X, A::HipHss)            <- *(A, hssB)            src/matmul.jl:14
+#   /(X, A::HipHss)            <- /(A, hssB)            src/hssmatrix.jl:236
 #
 # Status codes map to the reference's exceptions: DimensionMismatch,
 # ArgumentError, SingularException.  Julia is not available in the build
@@ -21,7 +24,7 @@ module HssHip
 
 using LinearAlgebra
 using HssMatrices
-import Base: *, \, size
+import Base: *, \, /, adjoint, size
 import LinearAlgebra: mul!
 
 const libhss = joinpath(@__DIR__, "..", "libhss_hip.so")
@@ -149,6 +152,17 @@ end
 \(A::HipHss, B::Matrix{Float64}) = ulvfactsolve(A, B)
 # the vector method the reference lacks (its Vector RHS falls back to a dense solve)
 \(A::HipHss, x::Vector{Float64}) = vec(ulvfactsolve(A, reshape(x, length(x), 1)))
+
+# hssA' as a new device matrix: D', U <-> V, B12 <- B21', B21 <- B12', R <-> W
+function adjoint(A::HipHss)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hss_mat_adjoint, libhss), Cint, (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}), A.h, r))
+    At = HipHss(r[], A.n, A.m, A.ctx)
+    finalizer(At -> ccall((:hss_mat_destroy, libhss), Cint, (Ptr{Cvoid},), At.h), At)
+    return At
+end
+*(X::AbstractMatrix, A::HipHss) = copy((A' * copy(X'))')
+/(X::Matrix{Float64}, A::HipHss) = copy(ulvfactsolve(A', collect(X'))')
 
 mutable struct HipUlv
     h::Ptr{Cvoid}

@@ -112,6 +112,11 @@ void* hss_ctx_stream(hss_ctx_t ctx);
 hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, int32_t num_nodes, const double* blob,
                           int64_t blob_len, hss_mat_t* out);
 hss_status hss_mat_destroy(hss_mat_t mat);
+/* The adjoint A' as a new matrix on the same context (adjoint(hssA), src/hssmatrix.jl:165-171:
+ * leaf D', U <-> V; branch B12 <- B21', B21 <- B12'; R <-> W).  With it, hss_matmul gives
+ * A' * B and A * hssB = (hssB' * A')' (src/matmul.jl:14), and hss_ulvfactsolve on the adjoint
+ * gives A / hssB = (hssB' \ A')' (src/hssmatrix.jl:236). */
+hss_status hss_mat_adjoint(hss_mat_t mat, hss_mat_t* out);
 /* size(hssA) (src/hssmatrix.jl:94) */
 hss_status hss_mat_size(hss_mat_t mat, int64_t* rows, int64_t* cols);
 /* bytes of HBM held by the matrix (generators + tables) */

@@ -57,3 +57,18 @@ def c1():
     K, h = cauchy_c1()
     nodes, blob = H.pack(h)
     return {"K": K, "h": h, "nodes": nodes, "blob": blob, "A": K.dense(), "F": H.full(h)}
+
+
+def rect_hss(rng, ms, ns, ku, kv, root=True):
+    """A random rectangular HSS tree over leaves of ms[i] x ns[i] (len a power of two), with
+    generator ranks ku (rows) and kv (columns) on every non-root node (hssmatrix.jl:11-68)."""
+    g = lambda r, c: rng.standard_normal((r, c))
+    if len(ms) == 1:
+        return H.HssMatrix(D=g(ms[0], ns[0]), U=g(ms[0], ku), V=g(ns[0], kv), rootnode=root)
+    h = len(ms) // 2
+    a = rect_hss(rng, ms[:h], ns[:h], ku, kv, False)
+    b = rect_hss(rng, ms[h:], ns[h:], ku, kv, False)
+    if root:
+        return H.HssMatrix(A11=a, A22=b, B12=g(ku, kv), B21=g(ku, kv), rootnode=True)
+    return H.HssMatrix(A11=a, A22=b, B12=g(ku, kv), B21=g(ku, kv), R1=g(ku, ku), W1=g(kv, kv),
+                       R2=g(ku, ku), W2=g(kv, kv))

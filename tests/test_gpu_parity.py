@@ -14,7 +14,7 @@ import pathlib
 import numpy as np
 import pytest
 
-from conftest import H, cauchy_c1, random_hss
+from conftest import H, cauchy_c1, random_hss, rect_hss
 
 pytestmark = pytest.mark.gpu
 
@@ -207,6 +207,65 @@ def test_errors_map_to_reference_exceptions(hip, c1_dev):
     assert (c1_dev @ np.zeros((2001, 0))).shape == (2001, 0)
 
 
+@pytest.mark.parametrize("name", NAMES)
+def test_adjoint_golden(hip, oracle, name):
+    """hss_mat_adjoint (hssmatrix.jl:165-171): A' X, X A (matmul.jl:14) and X / A
+    (hssmatrix.jl:236) against the oracle on the host adjoint; A'' X against A X."""
+    nodes, blob = GOLD[name + "/nodes"], GOLD[name + "/blob"]
+    A = hip.HipHssMatrix((nodes, blob))
+    At = A.adjoint()
+    assert At.shape == (A.shape[1], A.shape[0])
+    nt, bt = H.pack(H.unpack(nodes, blob).adjoint())
+    X = GOLD[name + "/X"]
+    Yt = oracle.matmul(nt, bt, X)
+    assert rel(At @ X, Yt) < 1e-14
+    assert rel(X.T @ A, Yt.T) < 1e-14
+    assert rel(At.adjoint() @ X, A @ X) < 1e-15
+    B = GOLD[name + "/B"]
+    cond = float(GOLD[name + "/cond"])
+    Zo = oracle.ulvfactsolve(nt, bt, B)
+    assert rel(B.T / A, Zo.T) < 50 * cond * EPS
+    assert rel(A.rdiv(B.T), Zo.T) < 50 * cond * EPS
+    # (X / A) A = X
+    assert rel((B.T / A) @ A, B.T) < 50 * cond * EPS
+
+
+def test_adjoint_rectangular(hip, oracle):
+    """A rectangular tree (73 x 71, ku = 3, kv = 5): the adjoint exchanges rows and columns,
+    sizes and offsets; A' X and X A match the oracle and the dense transpose."""
+    rng = np.random.default_rng(31)
+    h = rect_hss(rng, [7, 12, 9, 15, 5, 11, 8, 6], [10, 6, 14, 8, 9, 4, 13, 7], 3, 5)
+    F = H.full(h)
+    nodes, blob = H.pack(h)
+    nt, bt = H.pack(h.adjoint())
+    A = hip.HipHssMatrix((nodes, blob))
+    At = A.adjoint()
+    assert A.shape == (73, 71) and At.shape == (71, 73)
+    X = rng.standard_normal((71, 5))
+    Y = rng.standard_normal((73, 6))
+    assert rel(A @ X, oracle.matmul(nodes, blob, X)) < 1e-14
+    assert rel(A @ X, F @ X) < 1e-13
+    assert rel(At @ Y, oracle.matmul(nt, bt, Y)) < 1e-14
+    assert rel(At @ Y, F.T @ Y) < 1e-13
+    assert rel(Y.T @ A, (F.T @ Y).T) < 1e-13
+    with pytest.raises(H.DimensionMismatch):
+        X.T @ A
+    with pytest.raises(H.DimensionMismatch):
+        Y.T / A
+
+
+def test_c1_left_multiply_and_rdiv(c1_dev, oracle, c1):
+    """Config C1 through the adjoint: X A and X / A against the oracle (1e-12 for the solve)."""
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal((7, 2001))
+    nt, bt = H.pack(c1["h"].adjoint())
+    assert rel(X @ c1_dev, oracle.matmul(nt, bt, X.T).T) < 1e-14
+    assert rel(X @ c1_dev, X @ c1["F"]) < 1e-13
+    Z = X / c1_dev
+    assert rel(Z, oracle.ulvfactsolve(nt, bt, X.T).T) < 1e-12
+    assert rel(Z @ c1["F"], X) < 1e-12
+
+
 C2 = ROOT / "data" / "log_n65536_leaf128_tol1e-9.npz"
 
 
@@ -240,6 +299,14 @@ def test_c2_full_size(hip, oracle):
     assert resid(Z) <= 10 * resid(Zo) + 1e-14
     assert resid(Z) <= 1e-10
     assert rel(Z, Zo) < 1e-9
+    # adjoint identity <W, A X> = <A' W, X>, and X / A through the adjoint solves X = Z A
+    At = A.adjoint()
+    W = rng.standard_normal((n, 8))
+    lhs, rhs = np.sum(W * Y), np.sum((At @ W) * X)
+    assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(W) * np.linalg.norm(Y)
+    Zr = B.T / A
+    Rr = (Zr @ A) - B.T
+    assert np.max(np.linalg.norm(Rr, axis=1) / np.linalg.norm(B, axis=0)) <= 1e-10
 
 
 def _batch_inputs(hip, rng, specs, prec):

@@ -16,7 +16,7 @@ against dense matrices only), so the oracle is pinned by
 import numpy as np
 import pytest
 
-from conftest import H, cauchy_c1
+from conftest import H, cauchy_c1, rect_hss
 
 GOLD = np.load(__import__("pathlib").Path(__file__).with_name("golden") / "hss_golden.npz")
 NAMES = [str(s) for s in GOLD["names"]]
@@ -110,6 +110,35 @@ def test_oracle_adjoint_identity(oracle):
     lhs = y @ oracle.matmul(nodes, blob, x)
     rhs = oracle.matmul(nt, bt, y) @ x
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_adjoint_closed_form(oracle, name):
+    """The host adjoint (hssmatrix.jl:165-171: D', U <-> V, B12 <- B21', B21 <- B12', R <-> W)
+    expands to the transpose (up to the rounding of the re-ordered products), and the oracle's
+    matvec on it matches the dense transpose."""
+    nodes, blob = GOLD[f"{name}/nodes"], GOLD[f"{name}/blob"]
+    h = H.unpack(nodes, blob)
+    F = H.full(h)
+    assert rel(H.full(h.adjoint()), F.T) < 1e-14
+    nt, bt = H.pack(h.adjoint())
+    X = np.random.default_rng(8).standard_normal((F.shape[0], 3))
+    assert rel(oracle.matmul(nt, bt, X), F.T @ X) < 1e-13
+
+
+def test_oracle_rectangular(oracle):
+    """Rectangular trees (m != n, ku != kv, hssmatrix.jl:11-68): the oracle's matvec of A and of
+    its adjoint against the dense expansion."""
+    rng = np.random.default_rng(3)
+    h = rect_hss(rng, [7, 12, 9, 15, 5, 11, 8, 6], [10, 6, 14, 8, 9, 4, 13, 7], 3, 5)
+    F = H.full(h)
+    assert F.shape == (73, 71)
+    nodes, blob = H.pack(h)
+    X = rng.standard_normal((71, 4))
+    assert rel(oracle.matmul(nodes, blob, X), F @ X) < 1e-14
+    nt, bt = H.pack(h.adjoint())
+    Y = rng.standard_normal((73, 4))
+    assert rel(oracle.matmul(nt, bt, Y), F.T @ Y) < 1e-14
 
 
 def singular_root_hss():
