@@ -97,9 +97,27 @@ def test_c1_reference_tolerances(c1_dev, c1):
 def test_resident_factor_reuse(c1_dev, oracle, c1):
     F = c1_dev.ulvfactor("fp64")
     rng = np.random.default_rng(12)
-    for nrhs in (1, 40, 3):
+    for nrhs in (1, 40, 3, 64, 17, 64):
         B = rng.standard_normal((2001, nrhs))
         assert rel(F.solve(B), oracle.ulvfactsolve(c1["nodes"], c1["blob"], B)) <= 1e-12
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_resident_factor_tiles_reuse(hip, oracle, name):
+    """Repeated solves of one resident factor, with the number of column tiles changing
+    between them (workspace regrown, per-level tile widths re-chosen), match the oracle and
+    repeat bit for bit."""
+    nodes, blob = GOLD[name + "/nodes"], GOLD[name + "/blob"]
+    A = hip.HipHssMatrix((nodes, blob))
+    F = A.ulvfactor("fp64")
+    cond = float(GOLD[name + "/cond"])
+    n = A.shape[0]
+    rng = np.random.default_rng(40)
+    for nrhs in (3, 33, 3, 70):
+        B = rng.standard_normal((n, nrhs))
+        Z = F.solve(B)
+        assert rel(Z, oracle.ulvfactsolve(nodes, blob, B)) < 50 * cond * EPS
+        assert np.array_equal(F.solve(B), Z)
 
 
 def test_solve_is_deterministic(c1_dev):
