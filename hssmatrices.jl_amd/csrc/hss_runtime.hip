@@ -723,6 +723,100 @@ extern "C" hss_status hss_mat_adjoint(hss_mat_t A, hss_mat_t* out) {
     return hss_mat_create(A->ctx, t.data(), nn, blob.data(), (int64_t)blob.size(), out);
 }
 
+// ---------------------------------------------------------------------------
+// generator interchange file: hss_gen_header | node table | Float64 blob
+// ---------------------------------------------------------------------------
+static_assert(sizeof(hss_gen_header) == 40, "hss_gen_header is 40 bytes");
+static_assert(sizeof(hss_node_desc) == 112, "hss_node_desc is 112 bytes");
+
+namespace {
+struct OwnedFile {
+    FILE* f = nullptr;
+    explicit OwnedFile(FILE* p) : f(p) {}
+    ~OwnedFile() {
+        if (f) std::fclose(f);
+    }
+};
+}  // namespace
+
+extern "C" hss_status hss_gen_write(const char* path, const hss_node_desc* nodes, int32_t nn, const double* blob,
+                                    int64_t blob_len) {
+    if (!path || !nodes || (blob_len > 0 && !blob)) return fail(HSS_ERR_ARG, "null argument");
+    if (blob_len < 0) return fail(HSS_ERR_ARG, "blob_len < 0");
+    hss_status s = validate(nodes, nn, blob_len);
+    if (s != HSS_OK) return s;
+    hss_gen_header h{};
+    std::memcpy(h.magic, "HSSG", 4);
+    h.version = 1;
+    h.elem_size = sizeof(double);
+    h.num_nodes = nn;
+    h.blob_len = blob_len;
+    OwnedFile F(std::fopen(path, "wb"));
+    if (!F.f) return fail(HSS_ERR_IO, std::string("cannot open ") + path + " for writing");
+    bool ok = std::fwrite(&h, sizeof h, 1, F.f) == 1 &&
+              std::fwrite(nodes, sizeof(hss_node_desc), (size_t)nn, F.f) == (size_t)nn &&
+              (blob_len == 0 || std::fwrite(blob, sizeof(double), (size_t)blob_len, F.f) == (size_t)blob_len);
+    ok = (std::fclose(F.f) == 0) && ok;
+    F.f = nullptr;
+    if (!ok) return fail(HSS_ERR_IO, std::string("writing ") + path + " failed");
+    return HSS_OK;
+}
+
+extern "C" hss_status hss_gen_read(const char* path, hss_node_desc* nodes, int32_t nodes_cap, int32_t* num_nodes,
+                                   double* blob, int64_t blob_cap, int64_t* blob_len) {
+    if (!path) return fail(HSS_ERR_ARG, "null path");
+    OwnedFile F(std::fopen(path, "rb"));
+    if (!F.f) return fail(HSS_ERR_IO, std::string("cannot open ") + path);
+    hss_gen_header h;
+    if (std::fread(&h, sizeof h, 1, F.f) != 1) return fail(HSS_ERR_ARG, "generator file shorter than its header");
+    if (std::memcmp(h.magic, "HSSG", 4) != 0) return fail(HSS_ERR_ARG, "not a generator file (magic)");
+    if (h.version != 1) return fail(HSS_ERR_ARG, "unsupported generator file version " + std::to_string(h.version));
+    if (h.elem_size != sizeof(double)) return fail(HSS_ERR_ARG, "generator file element size is not 8 (Float64)");
+    if (h.num_nodes <= 0 || h.blob_len < 0 || h.blob_len > ((int64_t)1 << 56))
+        return fail(HSS_ERR_ARG, "bad sizes in the generator file header");
+    // the sizes must account for the whole file, before anything is allocated from them
+    const int64_t want = (int64_t)sizeof h + (int64_t)h.num_nodes * (int64_t)sizeof(hss_node_desc) +
+                         h.blob_len * (int64_t)sizeof(double);
+    if (fseeko(F.f, 0, SEEK_END) != 0) return fail(HSS_ERR_IO, std::string("cannot seek in ") + path);
+    const int64_t have = (int64_t)ftello(F.f);
+    if (have != want)
+        return fail(HSS_ERR_ARG, "generator file is " + std::to_string(have) + " bytes, its header says " +
+                                     std::to_string(want));
+    if (num_nodes) *num_nodes = h.num_nodes;
+    if (blob_len) *blob_len = h.blob_len;
+    if (!nodes || (!blob && h.blob_len > 0)) return HSS_OK;
+    if (nodes_cap < h.num_nodes || blob_cap < h.blob_len)
+        return fail(HSS_ERR_DIM, "buffers smaller than the generator file's node table or blob");
+    if (fseeko(F.f, (off_t)sizeof h, SEEK_SET) != 0) return fail(HSS_ERR_IO, std::string("cannot seek in ") + path);
+    if (std::fread(nodes, sizeof(hss_node_desc), (size_t)h.num_nodes, F.f) != (size_t)h.num_nodes ||
+        (h.blob_len > 0 && std::fread(blob, sizeof(double), (size_t)h.blob_len, F.f) != (size_t)h.blob_len))
+        return fail(HSS_ERR_IO, std::string("reading ") + path + " failed");
+    return validate(nodes, h.num_nodes, h.blob_len);
+}
+
+extern "C" hss_status hss_mat_load(hss_ctx_t ctx, const char* path, hss_mat_t* out) {
+    if (!ctx || !out) return fail(HSS_ERR_ARG, "null argument");
+    *out = nullptr;
+    int32_t nn = 0;
+    int64_t bl = 0;
+    hss_status s = hss_gen_read(path, nullptr, 0, &nn, nullptr, 0, &bl);
+    if (s != HSS_OK) return s;
+    std::vector<hss_node_desc> nodes((size_t)nn);
+    std::vector<double> blob((size_t)std::max<int64_t>(bl, 1));
+    s = hss_gen_read(path, nodes.data(), nn, &nn, blob.data(), bl, &bl);
+    if (s != HSS_OK) return s;
+    return hss_mat_create(ctx, nodes.data(), nn, blob.data(), bl, out);
+}
+
+extern "C" hss_status hss_mat_save(hss_mat_t A, const char* path) {
+    if (!A || !path) return fail(HSS_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(A->ctx->device));
+    std::vector<double> blob((size_t)std::max<int64_t>(A->blob_len, 1));
+    if (A->blob_len > 0)
+        HIP_TRY(hipMemcpy(blob.data(), A->d_blob, sizeof(double) * (size_t)A->blob_len, hipMemcpyDeviceToHost));
+    return hss_gen_write(path, A->desc.data(), A->nn, blob.data(), A->blob_len);
+}
+
 extern "C" hss_status hss_mat_destroy(hss_mat_t M) {
     mat_release(M);
     return HSS_OK;

@@ -17,6 +17,8 @@ reference (bonevbs/HssMatrices.jl)     here
 ``hssA'`` (``hssmatrix.jl:165-171``)   ``hssA.adjoint()`` / ``hssA.H``
 ``A * hssB`` (``matmul.jl:14``)        ``A @ hssB``  (``HipHssMatrix.__rmatmul__``)
 ``A / hssB`` (``hssmatrix.jl:236``)    ``A / hssB`` / ``hssB.rdiv(A)``
+(in memory only, hssmatrix.jl:11-68)   ``HipHssMatrix.load(path)`` / ``.save(path)``,
+                                       ``gen_read`` / ``gen_write`` (generator files)
 =====================================  =============================================
 
 Errors follow the reference: ``DimensionMismatch`` (``matmul.jl:19-20``),
@@ -38,13 +40,14 @@ from .hssmatrix import NODE_DTYPE, DimensionMismatch, HssMatrix, SingularExcepti
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libhss_hip.so"
 
 HSS_OK, HSS_ERR_DIM, HSS_ERR_ARG, HSS_ERR_SINGULAR, HSS_ERR_NOCONV = 0, 1, 2, 3, 4
-HSS_ERR_OOM, HSS_ERR_BUDGET, HSS_ERR_HIP, HSS_ERR_UNSUPPORTED = 5, 6, 7, 8
+HSS_ERR_OOM, HSS_ERR_BUDGET, HSS_ERR_HIP, HSS_ERR_UNSUPPORTED, HSS_ERR_IO = 5, 6, 7, 8, 9
 PREC_FP64, PREC_FP32 = 0, 1
 
 # every symbol include/hss_hip.h declares
 EXPORTS = [
     "hss_ctx_create", "hss_ctx_destroy", "hss_ctx_stream",
     "hss_mat_create", "hss_mat_destroy", "hss_mat_adjoint", "hss_mat_size", "hss_mat_device_bytes",
+    "hss_gen_write", "hss_gen_read", "hss_mat_load", "hss_mat_save",
     "hss_matmul", "hss_matmul_dev",
     "hss_ulvfactsolve", "hss_ulv_factor", "hss_fact_destroy", "hss_fact_device_bytes",
     "hss_ulv_solve", "hss_ulv_solve_dev", "hss_ulv_solve_batch_dev", "hss_ulv_factor_batch",
@@ -98,6 +101,10 @@ def load_library(path: pathlib.Path | str | None = None):
         "hss_mat_adjoint": ([V, PV], ctypes.c_int),
         "hss_mat_size": ([V, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
         "hss_mat_device_bytes": ([V], I64),
+        "hss_gen_write": ([ctypes.c_char_p, V, I32, V, I64], ctypes.c_int),
+        "hss_gen_read": ([ctypes.c_char_p, V, I32, ctypes.POINTER(I32), V, I64, ctypes.POINTER(I64)], ctypes.c_int),
+        "hss_mat_load": ([V, ctypes.c_char_p, PV], ctypes.c_int),
+        "hss_mat_save": ([V, ctypes.c_char_p], ctypes.c_int),
         "hss_matmul": ([V, I32, D, V, I64, D, V, I64], ctypes.c_int),
         "hss_matmul_dev": ([V, I32, D, V, I64, D, V, I64], ctypes.c_int),
         "hss_ulvfactsolve": ([V, I32, V, I64, V, I64], ctypes.c_int),
@@ -152,6 +159,8 @@ def _check(status: int):
         raise SingularException(msg)
     if status == HSS_ERR_NOCONV:
         raise NotConverged(status, msg)
+    if status == HSS_ERR_IO:
+        raise OSError(msg)
     raise HssError(status, msg)
 
 
@@ -268,6 +277,21 @@ class HipHssMatrix:
     def handle(self):
         return self._h
 
+    # ---- generator files (hss_gen_header layout) ----
+    @classmethod
+    def load(cls, path, ctx: Context | None = None) -> "HipHssMatrix":
+        """A device matrix from a generator file (``hss_mat_load``)."""
+        lib = load_library()
+        ctx = ctx or default_context()
+        nodes, _ = gen_read(path)
+        h = ctypes.c_void_p()
+        _check(lib.hss_mat_load(ctx.handle, str(path).encode(), ctypes.byref(h)))
+        return cls(_Adopt(h, nodes), ctx)
+
+    def save(self, path) -> None:
+        """The matrix's generators, read back from HBM, to a generator file (``hss_mat_save``)."""
+        _check(_lib.hss_mat_save(self._h, str(path).encode()))
+
     def device_bytes(self) -> int:
         return int(_lib.hss_mat_device_bytes(self._h))
 
@@ -350,6 +374,28 @@ class HipHssMatrix:
     def solve(self, B):
         """``hssA \\ B`` (reference ``hssmatrix.jl:234`` -> ``ulvfactsolve``)."""
         return ulvfactsolve(self, B)
+
+
+def gen_write(path, nodes, blob) -> None:
+    """Write a node table and blob as a generator file with the library (``hss_gen_write``;
+    host only)."""
+    lib = load_library()
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    blob = np.ascontiguousarray(blob, dtype=np.float64)
+    _check(lib.hss_gen_write(str(path).encode(), nodes.ctypes.data, len(nodes), blob.ctypes.data, blob.size))
+
+
+def gen_read(path):
+    """(nodes, blob) of a generator file read by the library (``hss_gen_read``; host only)."""
+    lib = load_library()
+    nn, bl = ctypes.c_int32(), ctypes.c_int64()
+    p = str(path).encode()
+    _check(lib.hss_gen_read(p, None, 0, ctypes.byref(nn), None, 0, ctypes.byref(bl)))
+    nodes = np.zeros(nn.value, dtype=NODE_DTYPE)
+    blob = np.zeros(max(bl.value, 1), dtype=np.float64)
+    _check(lib.hss_gen_read(p, nodes.ctypes.data, nn.value, ctypes.byref(nn), blob.ctypes.data, bl.value,
+                            ctypes.byref(bl)))
+    return nodes, blob[:bl.value]
 
 
 def mul(C, hssA: HipHssMatrix, B, alpha: float = 1.0, beta: float = 0.0):

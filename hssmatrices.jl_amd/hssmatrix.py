@@ -33,7 +33,7 @@ import numpy as np
 __all__ = [
     "ClusterTree", "bisection_cluster", "HssMatrix", "full", "hssrank", "gensize",
     "prune_leaves", "lowrank2hss", "NODE_DTYPE", "pack", "unpack", "tree_summary",
-    "DimensionMismatch", "SingularException",
+    "DimensionMismatch", "SingularException", "GEN_HEADER_DTYPE", "save_generators", "load_generators",
 ]
 
 
@@ -455,6 +455,48 @@ def unpack(nodes, blob) -> HssMatrix:
         return HssMatrix(A11=A11, A22=A22, B12=B12, B21=B21, R1=R1, W1=W1, R2=R2, W2=W2, rootnode=root)
 
     return build(0)
+
+
+# ----------------------------------------------------------------------------
+# generator interchange file (the layout of include/hss_hip.h's hss_gen_header):
+# 40-byte header ("HSSG", version 1, element size 8, node count, blob length, two
+# reserved int64), the node table (NODE_DTYPE records), the Float64 blob; little-endian.
+# The reference keeps HssMatrix{T} in memory only (src/hssmatrix.jl:11-68).
+# ----------------------------------------------------------------------------
+GEN_HEADER_DTYPE = np.dtype([("magic", "S4"), ("version", "<u4"), ("elem_size", "<u4"),
+                             ("num_nodes", "<i4"), ("blob_len", "<i8"), ("reserved", "<i8", (2,))])
+
+
+def save_generators(path, nodes, blob) -> None:
+    """Write a node table and blob (``pack``'s output) as a generator file."""
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    blob = np.ascontiguousarray(blob, dtype="<f8")
+    hdr = np.zeros((), GEN_HEADER_DTYPE)
+    hdr["magic"], hdr["version"], hdr["elem_size"] = b"HSSG", 1, 8
+    hdr["num_nodes"], hdr["blob_len"] = len(nodes), blob.size
+    with open(path, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(nodes.tobytes())
+        f.write(blob.tobytes())
+
+
+def load_generators(path):
+    """(nodes, blob) of a generator file; ValueError for a malformed one."""
+    raw = np.fromfile(path, dtype=np.uint8)
+    if raw.size < GEN_HEADER_DTYPE.itemsize:
+        raise ValueError(f"{path}: shorter than a generator file header")
+    hdr = raw[:GEN_HEADER_DTYPE.itemsize].view(GEN_HEADER_DTYPE)[0]
+    if hdr["magic"] != b"HSSG" or hdr["version"] != 1 or hdr["elem_size"] != 8:
+        raise ValueError(f"{path}: not a version-1 Float64 generator file")
+    nn, bl = int(hdr["num_nodes"]), int(hdr["blob_len"])
+    if nn <= 0 or bl < 0:
+        raise ValueError(f"{path}: bad sizes in the header")
+    o1 = GEN_HEADER_DTYPE.itemsize + nn * NODE_DTYPE.itemsize
+    if raw.size != o1 + 8 * bl:
+        raise ValueError(f"{path}: {raw.size} bytes, the header says {o1 + 8 * bl}")
+    nodes = raw[GEN_HEADER_DTYPE.itemsize:o1].view(NODE_DTYPE).copy()
+    blob = raw[o1:].view("<f8").copy()
+    return nodes, blob
 
 
 def tree_summary(nodes) -> dict:

@@ -15,6 +15,9 @@
 #   adjoint(A::HipHss), A'     <- adjoint(hssA)         src/hssmatrix.jl:165-171
 #   *(X, A::HipHss)            <- *(A, hssB)            src/matmul.jl:14
 #   /(X, A::HipHss)            <- /(A, hssB)            src/hssmatrix.jl:236
+#   HssHip.save / HssHip.load  generator files (include/hss_hip.h, hss_gen_header):
+#                              HssMatrix{Float64} <-> file in Julia, HipHss <-> file
+#                              through hss_mat_save / hss_mat_load
 #
 # Status codes map to the reference's exceptions: DimensionMismatch,
 # ArgumentError, SingularException.  Julia is not available in the build
@@ -50,6 +53,7 @@ function check(st::Cint)
     st == 1 && throw(DimensionMismatch(msg))
     st == 2 && throw(ArgumentError(msg))
     st == 3 && throw(SingularException(0))
+    st == 9 && throw(Base.IOError(msg, 0))
     throw(HssHipError(st, msg))
 end
 
@@ -109,6 +113,51 @@ function pack(hssA::HssMatrix{Float64})
     return nodes, blob   # heights are recomputed by the library
 end
 
+# Rebuilds an HssMatrix{Float64} from a node table and blob (the inverse of pack).
+function unpack(nodes::Vector{NodeDesc}, blob::Vector{Float64})
+    mat(off, r, c) = off < 0 ? zeros(r, c) : reshape(blob[off+1:off+r*c], r, c)
+    function build(i)
+        x = nodes[i]
+        root = x.parent < 0
+        if x.left < 0
+            return HssMatrix(mat(x.off_D, x.m, x.n), mat(x.off_U, x.m, x.ku), mat(x.off_V, x.n, x.kv), root)
+        end
+        l, r = nodes[x.left + 1], nodes[x.right + 1]
+        A11, A22 = build(x.left + 1), build(x.right + 1)
+        return HssMatrix(A11, A22, mat(x.off_B12, l.ku, r.kv), mat(x.off_B21, r.ku, l.kv),
+                         mat(l.off_R, l.ku, x.ku), mat(l.off_W, l.kv, x.kv),
+                         mat(r.off_R, r.ku, x.ku), mat(r.off_W, r.kv, x.kv), root)
+    end
+    return build(1)
+end
+
+# Generator file: 40-byte header ("HSSG", version 1, element size 8, node count, blob
+# length, two reserved Int64), the node table, the Float64 blob; little-endian.
+const GEN_MAGIC = UInt8['H', 'S', 'S', 'G']
+function save(path::AbstractString, hssA::HssMatrix{Float64})
+    nodes, blob = pack(hssA)
+    open(path, "w") do io
+        write(io, GEN_MAGIC, UInt32(1), UInt32(8), Int32(length(nodes)), Int64(length(blob)), Int64(0), Int64(0))
+        write(io, nodes)
+        write(io, blob)
+    end
+    return path
+end
+function load(path::AbstractString)
+    open(path, "r") do io
+        read(io, 4) == GEN_MAGIC || throw(ArgumentError("$path: not a generator file"))
+        (read(io, UInt32) == 1 && read(io, UInt32) == 8) ||
+            throw(ArgumentError("$path: not a version-1 Float64 generator file"))
+        nn, bl = read(io, Int32), read(io, Int64)
+        read(io, Int64); read(io, Int64)
+        (nn > 0 && bl >= 0 && filesize(path) == 40 + 112 * Int64(nn) + 8 * bl) ||
+            throw(ArgumentError("$path: sizes in the header do not match the file"))
+        nodes = read!(io, Vector{NodeDesc}(undef, nn))
+        blob = read!(io, Vector{Float64}(undef, bl))
+        return unpack(nodes, blob)
+    end
+end
+
 mutable struct HipHss
     h::Ptr{Cvoid}
     m::Int
@@ -125,6 +174,21 @@ function HipHss(hssA::HssMatrix{Float64}, ctx::Context=default_context())
     A = HipHss(r[], size(hssA, 1), size(hssA, 2), ctx)
     finalizer(A -> ccall((:hss_mat_destroy, libhss), Cint, (Ptr{Cvoid},), A.h), A)
     return A
+end
+
+# a device matrix from a generator file, and back
+function HipHss(path::AbstractString, ctx::Context=default_context())
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hss_mat_load, libhss), Cint, (Ptr{Cvoid}, Cstring, Ptr{Ptr{Cvoid}}), ctx.h, path, r))
+    m, n = Ref{Int64}(0), Ref{Int64}(0)
+    check(ccall((:hss_mat_size, libhss), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}), r[], m, n))
+    A = HipHss(r[], m[], n[], ctx)
+    finalizer(A -> ccall((:hss_mat_destroy, libhss), Cint, (Ptr{Cvoid},), A.h), A)
+    return A
+end
+function save(path::AbstractString, A::HipHss)
+    check(ccall((:hss_mat_save, libhss), Cint, (Ptr{Cvoid}, Cstring), A.h, path))
+    return path
 end
 
 size(A::HipHss) = (A.m, A.n)

@@ -47,7 +47,8 @@ typedef enum hss_status {
     HSS_ERR_OOM = 5,      /* device allocation failed */
     HSS_ERR_BUDGET = 6,   /* admission refused: over the memory budget */
     HSS_ERR_HIP = 7,      /* HIP runtime error */
-    HSS_ERR_UNSUPPORTED = 8
+    HSS_ERR_UNSUPPORTED = 8,
+    HSS_ERR_IO = 9        /* a generator file could not be opened, read or written */
 } hss_status;
 
 /*
@@ -121,6 +122,37 @@ hss_status hss_mat_adjoint(hss_mat_t mat, hss_mat_t* out);
 hss_status hss_mat_size(hss_mat_t mat, int64_t* rows, int64_t* cols);
 /* bytes of HBM held by the matrix (generators + tables) */
 int64_t hss_mat_device_bytes(hss_mat_t mat);
+
+/* ---- generator interchange file ----
+ * The reference keeps HssMatrix{T} in memory only (src/hssmatrix.jl:11-68).  This file
+ * carries the same generators between processes and languages, e.g. written by the Julia
+ * binding from a matrix the reference compressed (src/compression.jl:47-63) and loaded
+ * here, or the reverse.  Layout, little-endian: an hss_gen_header, then num_nodes
+ * hss_node_desc records (the hss_mat_create table), then blob_len Float64 values (the
+ * generator blob). */
+typedef struct hss_gen_header {
+    char magic[4];        /* "HSSG" */
+    uint32_t version;     /* 1 */
+    uint32_t elem_size;   /* 8: Float64 generators */
+    int32_t num_nodes;
+    int64_t blob_len;     /* Float64 elements */
+    int64_t reserved[2];  /* 0 */
+} hss_gen_header; /* 40 bytes */
+/* Writes a table and blob that hss_mat_create accepts (else HSS_ERR_ARG / HSS_ERR_DIM as it
+ * would return).  Host only: needs no device. */
+hss_status hss_gen_write(const char* path, const hss_node_desc* nodes, int32_t num_nodes, const double* blob,
+                         int64_t blob_len);
+/* Reads a generator file.  *num_nodes and *blob_len are set whenever the header is valid;
+ * nodes and blob are filled when both are non-null (HSS_ERR_DIM if nodes_cap or blob_cap
+ * is too small).  A malformed file (magic, version, element size, sizes, truncation) is
+ * HSS_ERR_ARG; a table hss_mat_create would refuse returns the status it would return.
+ * A file that cannot be opened or read is HSS_ERR_IO.  Host only. */
+hss_status hss_gen_read(const char* path, hss_node_desc* nodes, int32_t nodes_cap, int32_t* num_nodes, double* blob,
+                        int64_t blob_cap, int64_t* blob_len);
+/* hss_mat_create from a generator file */
+hss_status hss_mat_load(hss_ctx_t ctx, const char* path, hss_mat_t* out);
+/* The matrix's generators (read back from HBM) and node table to a generator file */
+hss_status hss_mat_save(hss_mat_t mat, const char* path);
 
 /* ---- matvec: C = alpha*A*B + beta*C,  mul!(C, hssA, B, alpha, beta), src/matmul.jl:18-62 ----
  * Host pointers; B is cols(A) x nrhs (ldb), C is rows(A) x nrhs (ldc). */
