@@ -586,6 +586,104 @@ def main():
     mv_bytes, mv_flops = matvec_bytes(nodes, nrhs)
     log(f"[bench] matvec: {mv_ms:.3f} ms, {mv_bytes / mv_ms / 1e6:.1f} GB/s algorithmic")
 
+    # ---- adjoint (hssmatrix.jl:165-171): A' x and A' \ b, which X A (matmul.jl:14) and
+    # X / A (hssmatrix.jl:236) run, on the same kernels and buffers ----
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    At = A.adjoint()
+    adj_create_ms = (time.perf_counter() - t0) * 1e3
+
+    def matvec_t():
+        At.matmul_dev(nrhs, 1.0, Bt.data_ptr(), ld, 0.0, Ct.data_ptr(), ld)
+
+    for _ in range(args.warmup):
+        matvec_t()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        matvec_t()
+    torch.cuda.synchronize()
+    adj_mv_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    adj_mv_bytes, _ = matvec_bytes(At.nodes, nrhs)
+    Ft = At.ulvfactor("fp64")
+    Xa = torch.zeros_like(Bt)
+    torch.cuda.synchronize()
+
+    def solve_t():
+        Ft.solve_dev(nrhs, Bt.data_ptr(), ld, Xa.data_ptr(), ld)
+
+    for _ in range(args.warmup):
+        solve_t()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solve_t()
+    torch.cuda.synchronize()
+    adj_solve_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    Xa_h = Xa.cpu().numpy().T
+    Ra = (At @ Xa_h) - B_host
+    adj_res = float(np.max(np.linalg.norm(Ra, axis=0) / np.linalg.norm(B_host, axis=0)))
+    Ft.close()
+    At.close()
+    del Xa
+    log(f"[bench] adjoint: created in {adj_create_ms:.1f} ms; matvec {adj_mv_ms:.3f} ms "
+        f"({adj_mv_bytes / adj_mv_ms / 1e6:.1f} GB/s algorithmic); fp64 solve {adj_solve_ms:.3f} ms, "
+        f"residual {adj_res:.2e}")
+    # ---- generator file (hss_mat_save / hss_mat_load) of the same matrix ----
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        gpath = pathlib.Path(td) / "a.hssg"
+        t0 = time.perf_counter()
+        A.save(gpath)
+        gen_save_ms = (time.perf_counter() - t0) * 1e3
+        gen_bytes = gpath.stat().st_size
+        t0 = time.perf_counter()
+        Ag = hip.HipHssMatrix.load(gpath)
+        gen_load_ms = (time.perf_counter() - t0) * 1e3
+        Cg = torch.zeros_like(Bt)
+        torch.cuda.synchronize()
+        Ag.matmul_dev(nrhs, 1.0, Bt.data_ptr(), ld, 0.0, Cg.data_ptr(), ld)
+        matvec()
+        torch.cuda.synchronize()
+        gen_same = bool(torch.equal(Cg, Ct))
+        Ag.close()
+        del Cg
+    log(f"[bench] generator file: {gen_bytes / 1e6:.1f} MB, save {gen_save_ms:.1f} ms, load {gen_load_ms:.1f} ms, "
+        f"loaded matrix's matvec identical: {gen_same}")
+
+    # ---- HSS x HSS product (matmul.jl:65-118): C = A A, ranks doubled ----
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    C = A @ A
+    prod_ms = (time.perf_counter() - t0) * 1e3
+    hip.prof_enable(True)
+    hip.prof_reset()
+    C2 = A @ A
+    prod_dev_ms = sum(r["ms"] for r in hip.prof_records() if r["kernel"] == "k_gemm_tasks")
+    hip.prof_enable(False)
+    C2.close()
+    Cp = torch.zeros_like(Bt)
+    torch.cuda.synchronize()
+
+    def matvec_c():
+        C.matmul_dev(nrhs, 1.0, Bt.data_ptr(), ld, 0.0, Cp.data_ptr(), ld)
+
+    for _ in range(args.warmup):
+        matvec_c()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        matvec_c()
+    torch.cuda.synchronize()
+    prod_mv_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    Yc = Cp.cpu().numpy().T
+    Yaa = A @ (A @ B_host)
+    prod_rel = float(np.linalg.norm(Yc - Yaa) / np.linalg.norm(Yaa))
+    C.close()
+    del Cp
+    log(f"[bench] HSS product A A: {prod_ms:.1f} ms wall ({prod_dev_ms:.2f} ms of device tasks); its matvec "
+        f"{prod_mv_ms:.3f} ms; vs A (A X): {prod_rel:.1e}")
+
     # ---- per-kernel timing (HIP events around each launch on the library stream) ----
     prof_reps = 3
     recs_solve = profile_launches(hip, solve64, prof_reps)
@@ -711,6 +809,14 @@ def main():
                    "tflops": round(mv_flops / mv_ms / 1e9, 3),
                    "leaf_kernel": {"ms": round(mv_leaf_ms, 4),
                                    "gbps": round(mv_leaf_bytes / mv_leaf_ms / 1e6, 1)}},
+        "adjoint": {"create_ms": round(adj_create_ms, 2), "matvec_ms": round(adj_mv_ms, 4),
+                    "matvec_gbps_algorithmic": round(adj_mv_bytes / adj_mv_ms / 1e6, 1),
+                    "fp64_solve_ms": round(adj_solve_ms, 4), "fp64_solve_rel_residual": adj_res},
+        "hss_product": {"case": "A A (ranks doubled)", "wall_ms": round(prod_ms, 1),
+                        "device_task_ms": round(prod_dev_ms, 3), "product_matvec_ms": round(prod_mv_ms, 4),
+                        "rel_diff_vs_A_A_x": prod_rel},
+        "generator_file": {"bytes": int(gen_bytes), "save_ms": round(gen_save_ms, 1),
+                           "load_ms": round(gen_load_ms, 1), "loaded_matvec_identical": gen_same},
         "roofline": {"kernel": f"{dk} (level {dlev}, {dnw} nodes)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

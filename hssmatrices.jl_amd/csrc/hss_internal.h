@@ -161,6 +161,19 @@ struct SInst {
     int32_t nrhs, pad;
 };
 
+// One small product of an HSS x HSS product's generator build (src/matmul.jl:65-118):
+//   c = op(a) op(b)  (+ c when GT_ACC), op = transpose when GT_TA / GT_TB;
+//   GT_COPY: c = op(a) (b, k unused).
+// Column-major, absolute device pointers; m x n output.
+enum { GT_TA = 1, GT_TB = 2, GT_ACC = 4, GT_COPY = 8 };
+struct GemmTask {
+    int32_t m, n, k, flags;
+    int32_t lda, ldb, ldc, pad;
+    const double* a;
+    const double* b;
+    double* c;
+};
+
 // Per-request vectors of a batched refinement step.
 struct BatchVec {
     const double* b;  // right-hand side (n x nrhs, ldb)

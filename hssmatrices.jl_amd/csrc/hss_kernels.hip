@@ -2245,6 +2245,37 @@ hipError_t launch_axpy_batch(const BatchVec* tab, int cnt, int maxc, hipStream_t
     return hipGetLastError();
 }
 
+// The small products that build an HSS x HSS product's generators (src/matmul.jl:65-118),
+// one workgroup per task; consecutive threads take consecutive rows of c.
+__global__ __launch_bounds__(256) void k_gemm_tasks(const GemmTask* __restrict__ tasks) {
+    const GemmTask g = tasks[blockIdx.x];
+    const bool ta = g.flags & GT_TA, tb = g.flags & GT_TB, acc = g.flags & GT_ACC, cp = g.flags & GT_COPY;
+    const gptr<const double> a = gbl(g.a);
+    const gptr<const double> b = gbl(g.b);
+    const gptr<double> c = gbl(g.c);
+    for (int e = threadIdx.x; e < g.m * g.n; e += blockDim.x) {
+        const int i = e % g.m, j = e / g.m;
+        double s = 0.0;
+        if (cp) {
+            s = ta ? a[j + (int64_t)i * g.lda] : a[i + (int64_t)j * g.lda];
+        } else {
+            for (int l = 0; l < g.k; ++l) {
+                const double av = ta ? a[l + (int64_t)i * g.lda] : a[i + (int64_t)l * g.lda];
+                const double bv = tb ? b[j + (int64_t)l * g.ldb] : b[l + (int64_t)j * g.ldb];
+                s = fma(av, bv, s);
+            }
+        }
+        const int64_t o = i + (int64_t)j * g.ldc;
+        c[o] = acc ? c[o] + s : s;
+    }
+}
+
+hipError_t launch_gemm_tasks(const GemmTask* tasks, int count, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gemm_tasks, dim3((unsigned)count), dim3(256), 0, st, tasks);
+    return hipGetLastError();
+}
+
 hipError_t launch_residual(const double* B, int64_t ldb, double* R, int64_t ldr, int64_t n, int nrhs, hipStream_t st) {
     const int64_t total = n * nrhs;
     int blocks = (int)((total + 255) / 256);

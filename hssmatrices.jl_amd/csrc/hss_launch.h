@@ -65,6 +65,8 @@ template <typename T>
 hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, int tw,
                               size_t lds, int resident, hipStream_t st);
 
+// one workgroup per task (HSS x HSS product generators)
+hipError_t launch_gemm_tasks(const GemmTask* tasks, int count, hipStream_t st);
 hipError_t launch_resid_batch(const BatchVec* tab, int cnt, int maxc, double* norms, hipStream_t st);
 hipError_t launch_axpy_batch(const BatchVec* tab, int cnt, int maxc, hipStream_t st);
 hipError_t launch_residual(const double* B, int64_t ldb, double* R, int64_t ldr, int64_t n, int nrhs, hipStream_t st);

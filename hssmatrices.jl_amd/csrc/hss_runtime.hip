@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstddef>
 #include <cstring>
+#include <deque>
 #include <new>
 #include <string>
 #include <vector>
@@ -815,6 +816,265 @@ extern "C" hss_status hss_mat_save(hss_mat_t A, const char* path) {
     if (A->blob_len > 0)
         HIP_TRY(hipMemcpy(blob.data(), A->d_blob, sizeof(double) * (size_t)A->blob_len, hipMemcpyDeviceToHost));
     return hss_gen_write(path, A->desc.data(), A->nn, blob.data(), A->blob_len);
+}
+
+// ---------------------------------------------------------------------------
+// HSS x HSS product (src/matmul.jl:65-118)
+// ---------------------------------------------------------------------------
+// C = A B for A and B on one tree (A's column clusters are B's row clusters).  C's ranks
+// are the sums of A's and B's (ku_C = ku_A + ku_B, kv_C = kv_A + kv_B); like the
+// reference, nothing is recompressed.  For a branch i with children l, r:
+//   up:    Z_leaf = V_A' U_B,   Z_i = W_A,l' Z_l R_B,l + W_A,r' Z_r R_B,r
+//   down:  F_l = B_A12 Z_r B_B21 + R_A,l F_i W_B,l',  F_r = B_A21 Z_l B_B12 + R_A,r F_i W_B,r'
+//          (F_i = 0 at the root)
+//   C:     B12 = [B_A12, R_A,l F_i W_B,r'; 0, B_B12],  B21 = [B_A21, R_A,r F_i W_B,l'; 0, B_B21]
+//          R_l = [R_A,l, B_A12 Z_r R_B,r; 0, R_B,l],   W_l = [W_A,l, 0; B_B21' Z_r' W_A,r, W_B,l]
+//          R_r = [R_A,r, B_A21 Z_l R_B,l; 0, R_B,r],   W_r = [W_A,r, 0; B_B12' Z_l' W_A,l, W_B,r]
+//   leaf:  D = D_A D_B + U_A F V_B',  U = [U_A, D_A U_B],  V = [D_B' V_A, V_B]
+// The products run on the device as ordered phases of small GEMM tasks (k_gemm_tasks)
+// into a zeroed generator blob, which then becomes the matrix C.
+extern "C" hss_status hss_mat_mul(hss_mat_t A, hss_mat_t B, hss_mat_t* out) {
+    if (!A || !B || !out) return fail(HSS_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (A->ctx != B->ctx) return fail(HSS_ERR_ARG, "the matrices live on different contexts");
+    if (A->cols != B->rows)
+        return fail(HSS_ERR_DIM, "second dimension of A (" + std::to_string(A->cols) +
+                                     ") does not match first dimension of B (" + std::to_string(B->rows) + ")");
+    const std::vector<hss_node_desc>& a = A->desc;
+    const std::vector<hss_node_desc>& b = B->desc;
+    const int nn = A->nn;
+    if (B->nn != nn) return fail(HSS_ERR_DIM, "clusters don't match: the trees of A and B differ");
+    for (int i = 0; i < nn; ++i)
+        if (a[i].parent != b[i].parent || a[i].left != b[i].left || a[i].right != b[i].right || a[i].n != b[i].m)
+            return fail(HSS_ERR_DIM, "clusters don't match at node " + std::to_string(i));
+    // C's node table and generator layout
+    std::vector<hss_node_desc> c(nn);
+    int64_t len = 0;
+    auto take = [](int64_t& cur, int64_t cnt) {
+        const int64_t o = cur;
+        cur += cnt;
+        return o;
+    };
+    for (int i = 0; i < nn; ++i) {
+        c[i] = a[i];
+        c[i].n = b[i].n;
+        c[i].ku = a[i].ku + b[i].ku;
+        c[i].kv = a[i].kv + b[i].kv;
+        c[i].col_off = b[i].col_off;
+        c[i].off_D = c[i].off_U = c[i].off_V = c[i].off_B12 = c[i].off_B21 = c[i].off_R = c[i].off_W = -1;
+    }
+    for (int i = 0; i < nn; ++i) {
+        hss_node_desc& x = c[i];
+        if (x.left < 0) {
+            x.off_D = take(len, (int64_t)x.m * x.n);
+            x.off_U = take(len, (int64_t)x.m * x.ku);
+            x.off_V = take(len, (int64_t)x.n * x.kv);
+        } else {
+            x.off_B12 = take(len, (int64_t)c[x.left].ku * c[x.right].kv);
+            x.off_B21 = take(len, (int64_t)c[x.right].ku * c[x.left].kv);
+        }
+        if (x.parent >= 0) {
+            x.off_R = take(len, (int64_t)x.ku * c[x.parent].ku);
+            x.off_W = take(len, (int64_t)x.kv * c[x.parent].kv);
+        }
+    }
+    // device workspace: per non-root node Z (kv_A x ku_B), F (ku_A x kv_B), TZ = Z R_B
+    // (kv_A x ku_B of the parent), P (B_A12 Z_r or B_A21 Z_l: ku_A x ku_B of the sibling),
+    // Y (Z_r B_B21 or Z_l B_B12: kv_A of the sibling x kv_B), Q = R_A F_parent (ku_A x
+    // kv_B of the parent); per non-root leaf TL = U_A F (m x kv_B)
+    std::vector<int64_t> oZ(nn, 0), oF(nn, 0), oTZ(nn, 0), oP(nn, 0), oY(nn, 0), oQ(nn, 0), oTL(nn, 0);
+    int64_t wl = 0;
+    for (int i = 1; i < nn; ++i) {
+        const int p = a[i].parent, sib = (a[p].left == i) ? a[p].right : a[p].left;
+        oZ[i] = take(wl, (int64_t)a[i].kv * b[i].ku);
+        oF[i] = take(wl, (int64_t)a[i].ku * b[i].kv);
+        oTZ[i] = take(wl, (int64_t)a[i].kv * b[p].ku);
+        oP[i] = take(wl, (int64_t)a[i].ku * b[sib].ku);
+        oY[i] = take(wl, (int64_t)a[sib].kv * b[i].kv);
+        oQ[i] = take(wl, (int64_t)a[i].ku * b[p].kv);
+        if (a[i].left < 0) oTL[i] = take(wl, (int64_t)a[i].m * b[i].kv);
+    }
+    HIP_TRY(hipSetDevice(A->ctx->device));
+    hipStream_t st = A->ctx->stream;
+    double* dC = nullptr;
+    double* dW = nullptr;
+    GemmTask* dT = nullptr;
+    struct Free {
+        double** c;
+        double** w;
+        GemmTask** t;
+        ~Free() {
+            (void)hipFree(*c);
+            (void)hipFree(*w);
+            (void)hipFree(*t);
+        }
+    } guard{&dC, &dW, &dT};
+    HIP_TRY(dalloc(&dC, (size_t)std::max<int64_t>(len, 1)));
+    HIP_TRY(dalloc(&dW, (size_t)std::max<int64_t>(wl, 1)));
+    const double* pa = A->d_blob;
+    const double* pb = B->d_blob;
+    auto PA = [&](int64_t o) { return pa + o; };
+    auto PB = [&](int64_t o) { return pb + o; };
+    auto PC = [&](int64_t o) { return dC + o; };
+    auto PW = [&](int64_t o) { return dW + o; };
+    std::deque<std::vector<GemmTask>> ph;  // appending keeps references to earlier phases valid
+    auto phase = [&]() -> std::vector<GemmTask>& {
+        ph.emplace_back();
+        return ph.back();
+    };
+    // c = op(x) op(y) (+ c); zero-sized or k = 0 products leave the zeroed c alone
+    auto mm = [](std::vector<GemmTask>& v, int m, int n, int k, const double* x, int ldx, bool tx, const double* y,
+                 int ldy, bool ty, double* z, int ldz, bool acc) {
+        if (m <= 0 || n <= 0 || k <= 0) return;
+        GemmTask t{};
+        t.m = m; t.n = n; t.k = k;
+        t.flags = (tx ? GT_TA : 0) | (ty ? GT_TB : 0) | (acc ? GT_ACC : 0);
+        t.lda = std::max(ldx, 1); t.ldb = std::max(ldy, 1); t.ldc = std::max(ldz, 1);
+        t.a = x; t.b = y; t.c = z;
+        v.push_back(t);
+    };
+    auto cp = [](std::vector<GemmTask>& v, int m, int n, const double* x, int ldx, double* z, int ldz) {
+        if (m <= 0 || n <= 0) return;
+        GemmTask t{};
+        t.m = m; t.n = n; t.flags = GT_COPY;
+        t.lda = std::max(ldx, 1); t.ldb = 1; t.ldc = std::max(ldz, 1);
+        t.a = x; t.c = z;
+        v.push_back(t);
+    };
+    const std::vector<DevNode>& hn = A->hn;
+    // up: Z by height
+    for (int h = 0; h < A->max_height; ++h) {
+        auto& u1 = phase();
+        for (int i = 1; i < nn; ++i) {
+            if (hn[i].height != h) continue;
+            if (a[i].left < 0) {
+                mm(u1, a[i].kv, b[i].ku, a[i].n, PA(a[i].off_V), a[i].n, true, PB(b[i].off_U), b[i].m, false,
+                   PW(oZ[i]), a[i].kv, false);
+            } else {
+                for (int ch : {a[i].left, a[i].right})
+                    mm(u1, a[ch].kv, b[i].ku, b[ch].ku, PW(oZ[ch]), a[ch].kv, false, PB(b[ch].off_R), b[ch].ku, false,
+                       PW(oTZ[ch]), a[ch].kv, false);
+            }
+        }
+        if (h == 0) continue;
+        auto& u2 = phase();
+        auto& u3 = phase();
+        for (int i = 1; i < nn; ++i) {
+            if (hn[i].height != h || a[i].left < 0) continue;
+            const int l = a[i].left, r = a[i].right;
+            mm(u2, a[i].kv, b[i].ku, a[l].kv, PA(a[l].off_W), a[l].kv, true, PW(oTZ[l]), a[l].kv, false, PW(oZ[i]),
+               a[i].kv, false);
+            mm(u3, a[i].kv, b[i].ku, a[r].kv, PA(a[r].off_W), a[r].kv, true, PW(oTZ[r]), a[r].kv, false, PW(oZ[i]),
+               a[i].kv, true);
+        }
+    }
+    // down: F, and C's couplings and translators, by depth
+    for (int d = 0; d <= A->max_depth; ++d) {
+        auto& d1 = phase();
+        auto& d2 = phase();
+        auto& d3 = phase();
+        for (int i = 0; i < nn; ++i) {
+            if (hn[i].depth != d || a[i].left < 0) continue;
+            const int l = a[i].left, r = a[i].right;
+            const bool root = i == 0;
+            const int kuCl = c[l].ku, kuCr = c[r].ku;
+            // D1: P, Y, Q and the blocks copied from A and B
+            mm(d1, a[l].ku, b[r].ku, a[r].kv, PA(a[i].off_B12), a[l].ku, false, PW(oZ[r]), a[r].kv, false, PW(oP[l]),
+               a[l].ku, false);
+            mm(d1, a[r].ku, b[l].ku, a[l].kv, PA(a[i].off_B21), a[r].ku, false, PW(oZ[l]), a[l].kv, false, PW(oP[r]),
+               a[r].ku, false);
+            mm(d1, a[r].kv, b[l].kv, b[r].ku, PW(oZ[r]), a[r].kv, false, PB(b[i].off_B21), b[r].ku, false, PW(oY[l]),
+               a[r].kv, false);
+            mm(d1, a[l].kv, b[r].kv, b[l].ku, PW(oZ[l]), a[l].kv, false, PB(b[i].off_B12), b[l].ku, false, PW(oY[r]),
+               a[l].kv, false);
+            cp(d1, a[l].ku, a[r].kv, PA(a[i].off_B12), a[l].ku, PC(c[i].off_B12), kuCl);
+            cp(d1, b[l].ku, b[r].kv, PB(b[i].off_B12), b[l].ku, PC(c[i].off_B12) + a[l].ku + (int64_t)a[r].kv * kuCl,
+               kuCl);
+            cp(d1, a[r].ku, a[l].kv, PA(a[i].off_B21), a[r].ku, PC(c[i].off_B21), kuCr);
+            cp(d1, b[r].ku, b[l].kv, PB(b[i].off_B21), b[r].ku, PC(c[i].off_B21) + a[r].ku + (int64_t)a[l].kv * kuCr,
+               kuCr);
+            if (!root) {
+                for (int ch : {l, r}) {
+                    mm(d1, a[ch].ku, b[i].kv, a[i].ku, PA(a[ch].off_R), a[ch].ku, false, PW(oF[i]), a[i].ku, false,
+                       PW(oQ[ch]), a[ch].ku, false);
+                    cp(d1, a[ch].ku, a[i].ku, PA(a[ch].off_R), a[ch].ku, PC(c[ch].off_R), c[ch].ku);
+                    cp(d1, b[ch].ku, b[i].ku, PB(b[ch].off_R), b[ch].ku,
+                       PC(c[ch].off_R) + a[ch].ku + (int64_t)a[i].ku * c[ch].ku, c[ch].ku);
+                    cp(d1, a[ch].kv, a[i].kv, PA(a[ch].off_W), a[ch].kv, PC(c[ch].off_W), c[ch].kv);
+                    cp(d1, b[ch].kv, b[i].kv, PB(b[ch].off_W), b[ch].kv,
+                       PC(c[ch].off_W) + a[ch].kv + (int64_t)a[i].kv * c[ch].kv, c[ch].kv);
+                }
+            }
+            // D2: F = B_A Y, and the product blocks of C
+            mm(d2, a[l].ku, b[l].kv, a[r].kv, PA(a[i].off_B12), a[l].ku, false, PW(oY[l]), a[r].kv, false, PW(oF[l]),
+               a[l].ku, false);
+            mm(d2, a[r].ku, b[r].kv, a[l].kv, PA(a[i].off_B21), a[r].ku, false, PW(oY[r]), a[l].kv, false, PW(oF[r]),
+               a[r].ku, false);
+            if (!root) {
+                for (int ch : {l, r}) {
+                    const int sb = ch == l ? r : l;
+                    // R_ch[0:ku_A,ch, ku_A,i:] = P_ch R_B,sib;  W_ch[kv_A,ch:, 0:kv_A,i] = Y_ch' W_A,sib
+                    mm(d2, a[ch].ku, b[i].ku, b[sb].ku, PW(oP[ch]), a[ch].ku, false, PB(b[sb].off_R), b[sb].ku, false,
+                       PC(c[ch].off_R) + (int64_t)a[i].ku * c[ch].ku, c[ch].ku, false);
+                    mm(d2, b[ch].kv, a[i].kv, a[sb].kv, PW(oY[ch]), a[sb].kv, true, PA(a[sb].off_W), a[sb].kv, false,
+                       PC(c[ch].off_W) + a[ch].kv, c[ch].kv, false);
+                }
+                // B12[0:ku_A,l, kv_A,r:] = Q_l W_B,r';  B21[0:ku_A,r, kv_A,l:] = Q_r W_B,l'
+                mm(d2, a[l].ku, b[r].kv, b[i].kv, PW(oQ[l]), a[l].ku, false, PB(b[r].off_W), b[r].kv, true,
+                   PC(c[i].off_B12) + (int64_t)a[r].kv * kuCl, kuCl, false);
+                mm(d2, a[r].ku, b[l].kv, b[i].kv, PW(oQ[r]), a[r].ku, false, PB(b[l].off_W), b[l].kv, true,
+                   PC(c[i].off_B21) + (int64_t)a[l].kv * kuCr, kuCr, false);
+                // D3: F += Q W_B'
+                mm(d3, a[l].ku, b[l].kv, b[i].kv, PW(oQ[l]), a[l].ku, false, PB(b[l].off_W), b[l].kv, true, PW(oF[l]),
+                   a[l].ku, true);
+                mm(d3, a[r].ku, b[r].kv, b[i].kv, PW(oQ[r]), a[r].ku, false, PB(b[r].off_W), b[r].kv, true, PW(oF[r]),
+                   a[r].ku, true);
+            }
+        }
+    }
+    // leaves: D = D_A D_B + (U_A F) V_B', U = [U_A, D_A U_B], V = [D_B' V_A, V_B]
+    {
+        auto& l1 = phase();
+        auto& l2 = phase();
+        for (int i = 0; i < nn; ++i) {
+            if (a[i].left >= 0) continue;
+            const int m = a[i].m, n = b[i].n, k = a[i].n;
+            const bool root = i == 0;
+            mm(l1, m, n, k, PA(a[i].off_D), m, false, PB(b[i].off_D), b[i].m, false, PC(c[i].off_D), m, false);
+            cp(l1, m, a[i].ku, PA(a[i].off_U), m, PC(c[i].off_U), m);
+            mm(l1, m, b[i].ku, k, PA(a[i].off_D), m, false, PB(b[i].off_U), b[i].m, false,
+               PC(c[i].off_U) + (int64_t)a[i].ku * m, m, false);
+            mm(l1, n, a[i].kv, b[i].m, PB(b[i].off_D), b[i].m, true, PA(a[i].off_V), a[i].n, false, PC(c[i].off_V), n,
+               false);
+            cp(l1, n, b[i].kv, PB(b[i].off_V), n, PC(c[i].off_V) + (int64_t)a[i].kv * n, n);
+            if (!root) {
+                mm(l1, m, b[i].kv, a[i].ku, PA(a[i].off_U), m, false, PW(oF[i]), a[i].ku, false, PW(oTL[i]), m, false);
+                mm(l2, m, n, b[i].kv, PW(oTL[i]), m, false, PB(b[i].off_V), n, true, PC(c[i].off_D), m, true);
+            }
+        }
+    }
+    size_t ntask = 0;
+    for (auto& v : ph) ntask += v.size();
+    std::vector<GemmTask> all;
+    all.reserve(ntask);
+    for (auto& v : ph) all.insert(all.end(), v.begin(), v.end());
+    HIP_TRY(dalloc(&dT, std::max<size_t>(ntask, 1)));
+    HIP_TRY(hipMemsetAsync(dC, 0, sizeof(double) * (size_t)std::max<int64_t>(len, 1), st));
+    HIP_TRY(hipMemsetAsync(dW, 0, sizeof(double) * (size_t)std::max<int64_t>(wl, 1), st));
+    if (ntask) HIP_TRY(hipMemcpyAsync(dT, all.data(), sizeof(GemmTask) * ntask, hipMemcpyHostToDevice, st));
+    size_t at = 0;
+    for (auto& v : ph) {
+        if (!v.empty()) {
+            ProfScope ps("k_gemm_tasks", 0, (int)v.size(), st);
+            HIP_TRY(launch_gemm_tasks(dT + at, (int)v.size(), st));
+        }
+        at += v.size();
+    }
+    std::vector<double> blob((size_t)std::max<int64_t>(len, 1));
+    HIP_TRY(hipMemcpyAsync(blob.data(), dC, sizeof(double) * (size_t)std::max<int64_t>(len, 1), hipMemcpyDeviceToHost,
+                           st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return hss_mat_create(A->ctx, c.data(), nn, blob.data(), len, out);
 }
 
 extern "C" hss_status hss_mat_destroy(hss_mat_t M) {

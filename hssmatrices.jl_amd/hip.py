@@ -17,6 +17,7 @@ reference (bonevbs/HssMatrices.jl)     here
 ``hssA'`` (``hssmatrix.jl:165-171``)   ``hssA.adjoint()`` / ``hssA.H``
 ``A * hssB`` (``matmul.jl:14``)        ``A @ hssB``  (``HipHssMatrix.__rmatmul__``)
 ``A / hssB`` (``hssmatrix.jl:236``)    ``A / hssB`` / ``hssB.rdiv(A)``
+``hssA * hssB`` (``matmul.jl:65-118``) ``hssA @ hssB`` / ``hssA.hss_mul(hssB)``
 (in memory only, hssmatrix.jl:11-68)   ``HipHssMatrix.load(path)`` / ``.save(path)``,
                                        ``gen_read`` / ``gen_write`` (generator files)
 =====================================  =============================================
@@ -46,7 +47,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 # every symbol include/hss_hip.h declares
 EXPORTS = [
     "hss_ctx_create", "hss_ctx_destroy", "hss_ctx_stream",
-    "hss_mat_create", "hss_mat_destroy", "hss_mat_adjoint", "hss_mat_size", "hss_mat_device_bytes",
+    "hss_mat_create", "hss_mat_destroy", "hss_mat_adjoint", "hss_mat_mul", "hss_mat_size", "hss_mat_device_bytes",
     "hss_gen_write", "hss_gen_read", "hss_mat_load", "hss_mat_save",
     "hss_matmul", "hss_matmul_dev",
     "hss_ulvfactsolve", "hss_ulv_factor", "hss_fact_destroy", "hss_fact_device_bytes",
@@ -99,6 +100,7 @@ def load_library(path: pathlib.Path | str | None = None):
         "hss_mat_create": ([V, V, I32, V, I64, PV], ctypes.c_int),
         "hss_mat_destroy": ([V], ctypes.c_int),
         "hss_mat_adjoint": ([V, PV], ctypes.c_int),
+        "hss_mat_mul": ([V, V, PV], ctypes.c_int),
         "hss_mat_size": ([V, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
         "hss_mat_device_bytes": ([V], I64),
         "hss_gen_write": ([ctypes.c_char_p, V, I32, V, I64], ctypes.c_int),
@@ -311,7 +313,10 @@ class HipHssMatrix:
 
     # ---- matvec ----
     def __matmul__(self, B):
-        """``hssA * B`` (reference ``matmul.jl:13,15``)."""
+        """``hssA * B`` (reference ``matmul.jl:13,15``); with B a device HSS matrix, the HSS
+        product ``hssA * hssB`` (``matmul.jl:65-118``, ``hss_mat_mul``) as a new one."""
+        if isinstance(B, HipHssMatrix):
+            return self.hss_mul(B)
         Bf, vec = _as_f64_fortran(B)
         if Bf.shape[0] != self.shape[1]:
             raise DimensionMismatch(
@@ -320,6 +325,19 @@ class HipHssMatrix:
         C = np.zeros((self.shape[0], Bf.shape[1]), order="F")
         mul(C, self, Bf, 1.0, 0.0)
         return C[:, 0] if vec else C
+
+    def hss_mul(self, B: "HipHssMatrix") -> "HipHssMatrix":
+        """``hssA * hssB`` (``matmul.jl:65-118``): ranks add, nothing is recompressed."""
+        h = ctypes.c_void_p()
+        _check(_lib.hss_mat_mul(self._h, B._h, ctypes.byref(h)))
+        nodes = np.zeros(len(self.nodes), dtype=NODE_DTYPE)
+        # the product's node table, as hss_mat_mul lays it out (sizes and offsets of rows
+        # from A, of columns from B); only its shape fields are read on the host
+        nodes[:] = self.nodes
+        nodes["n"], nodes["col_off"] = B.nodes["n"], B.nodes["col_off"]
+        nodes["ku"] = self.nodes["ku"] + B.nodes["ku"]
+        nodes["kv"] = self.nodes["kv"] + B.nodes["kv"]
+        return HipHssMatrix(_Adopt(h, nodes), self.ctx)
 
     def matmul_dev(self, nrhs: int, alpha: float, dB: int, ldb: int, beta: float, dC: int, ldc: int):
         """``hss_matmul_dev`` on device pointers (e.g. ``torch.Tensor.data_ptr()``)."""

@@ -15,6 +15,7 @@
 #   adjoint(A::HipHss), A'     <- adjoint(hssA)         src/hssmatrix.jl:165-171
 #   *(X, A::HipHss)            <- *(A, hssB)            src/matmul.jl:14
 #   /(X, A::HipHss)            <- /(A, hssB)            src/hssmatrix.jl:236
+#   *(A::HipHss, B::HipHss)    <- *(hssA, hssB)         src/matmul.jl:65-118
 #   HssHip.save / HssHip.load  generator files (include/hss_hip.h, hss_gen_header):
 #                              HssMatrix{Float64} <-> file in Julia, HipHss <-> file
 #                              through hss_mat_save / hss_mat_load
@@ -226,6 +227,14 @@ function adjoint(A::HipHss)
     return At
 end
 *(X::AbstractMatrix, A::HipHss) = copy((A' * copy(X'))')
+# HSS x HSS product (ranks add; nothing is recompressed, as in the reference)
+function *(A::HipHss, B::HipHss)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hss_mat_mul, libhss), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Ptr{Cvoid}}), A.h, B.h, r))
+    C = HipHss(r[], A.m, B.n, A.ctx)
+    finalizer(C -> ccall((:hss_mat_destroy, libhss), Cint, (Ptr{Cvoid},), C.h), C)
+    return C
+end
 /(X::Matrix{Float64}, A::HipHss) = copy(ulvfactsolve(A', collect(X'))')
 
 mutable struct HipUlv

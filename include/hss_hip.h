@@ -118,6 +118,11 @@ hss_status hss_mat_destroy(hss_mat_t mat);
  * A' * B and A * hssB = (hssB' * A')' (src/matmul.jl:14), and hss_ulvfactsolve on the adjoint
  * gives A / hssB = (hssB' \ A')' (src/hssmatrix.jl:236). */
 hss_status hss_mat_adjoint(hss_mat_t mat, hss_mat_t* out);
+/* The HSS x HSS product A * B as a new matrix on the same context (*(hssA, hssB),
+ * src/matmul.jl:65-118).  A and B must share one tree (A's column clusters are B's row
+ * clusters; else HSS_ERR_DIM).  C's generator ranks are the sums of A's and B's, as in the
+ * reference, which does not recompress; the generators are formed on the device. */
+hss_status hss_mat_mul(hss_mat_t a, hss_mat_t b, hss_mat_t* out);
 /* size(hssA) (src/hssmatrix.jl:94) */
 hss_status hss_mat_size(hss_mat_t mat, int64_t* rows, int64_t* cols);
 /* bytes of HBM held by the matrix (generators + tables) */

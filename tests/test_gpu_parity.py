@@ -273,6 +273,65 @@ def test_generator_file_load_and_save(hip, oracle, tmp_path, name):
     assert rel(oracle.matmul(nt, bt, X), F.T @ X) < 1e-13
 
 
+def _tree_rel(t1, t2):
+    """Largest relative difference between the generators of two trees of one shape."""
+    worst = 0.0
+    gens = ("D", "U", "V") if t1.leafnode else ("B12", "B21", "R1", "W1", "R2", "W2")
+    for g in gens:
+        x, y = getattr(t1, g), getattr(t2, g)
+        assert x.shape == y.shape, (g, x.shape, y.shape)
+        if y.size:
+            worst = max(worst, float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-300)))
+    if not t1.leafnode:
+        worst = max(worst, _tree_rel(t1.A11, t2.A11), _tree_rel(t1.A22, t2.A22))
+    return worst
+
+
+def _device_tree(M, tmp_path):
+    p = tmp_path / "m.hssg"
+    M.save(p)
+    return H.unpack(*H.load_generators(p))
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_hss_product_golden(hip, tmp_path, name):
+    """hss_mat_mul (matmul.jl:65-118): A A and A A' formed on the device, generator by
+    generator against the oracle's restatement, against the dense product, and through the
+    product's own matvec against A (B X)."""
+    from oracle import hss_product as P
+    nodes, blob = GOLD[name + "/nodes"], GOLD[name + "/blob"]
+    h = H.unpack(nodes, blob)
+    F = H.full(h)
+    A = hip.HipHssMatrix((nodes, blob))
+    X = GOLD[name + "/X"]
+    for B, hb, Fb in ((A, h, F), (A.adjoint(), h.adjoint(), F.T)):
+        C = A @ B
+        assert C.shape == (F.shape[0], Fb.shape[1])
+        tc = _device_tree(C, tmp_path)
+        assert _tree_rel(tc, P.product(h, hb, H)) < 1e-12
+        assert rel(H.full(tc), F @ Fb) < 1e-12
+        assert rel(C @ X, A @ (B @ X)) < 1e-12
+
+
+def test_hss_product_rectangular_and_mismatch(hip, tmp_path):
+    from oracle import hss_product as P
+    rng = np.random.default_rng(9)
+    ns = [10, 6, 14, 8, 9, 4, 13, 7]
+    a = rect_hss(rng, [7, 12, 9, 15, 5, 11, 8, 6], ns, 3, 5)
+    b = rect_hss(rng, ns, [4, 9, 6, 11, 7, 5, 10, 8], 4, 2)
+    A, B = hip.HipHssMatrix(a), hip.HipHssMatrix(b)
+    C = A @ B
+    assert C.shape == (73, 64)
+    tc = _device_tree(C, tmp_path)
+    assert _tree_rel(tc, P.product(a, b, H)) < 1e-12
+    assert rel(H.full(tc), H.full(a) @ H.full(b)) < 1e-12
+    X = rng.standard_normal((64, 3))
+    assert rel(C @ X, A @ (B @ X)) < 1e-12
+    # clusters that do not match (A A: A's columns are 71, its rows 73)
+    with pytest.raises(H.DimensionMismatch):
+        A @ A
+
+
 def test_adjoint_rectangular(hip, oracle):
     """A rectangular tree (73 x 71, ku = 3, kv = 5): the adjoint exchanges rows and columns,
     sizes and offsets; A' X and X A match the oracle and the dense transpose."""
@@ -350,6 +409,9 @@ def test_c2_full_size(hip, oracle):
     Zr = B.T / A
     Rr = (Zr @ A) - B.T
     assert np.max(np.linalg.norm(Rr, axis=1) / np.linalg.norm(B, axis=0)) <= 1e-10
+    # the HSS product A A (ranks doubled) applies A twice
+    C = A @ A
+    assert rel(C @ X[:, :2], A @ Y[:, :2]) < 1e-12
 
 
 def _batch_inputs(hip, rng, specs, prec):

@@ -141,6 +141,35 @@ def test_oracle_rectangular(oracle):
     assert rel(oracle.matmul(nt, bt, Y), F.T @ Y) < 1e-14
 
 
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_hss_product_closed_form(name):
+    """The restated HSS x HSS product (matmul.jl:65-118) expands to the dense product, for
+    A A and A A' on every golden tree, with ranks that add (no recompression)."""
+    from oracle import hss_product as P
+    h = H.unpack(GOLD[f"{name}/nodes"], GOLD[f"{name}/blob"])
+    F = H.full(h)
+    for b, Fb in ((h, F), (h.adjoint(), F.T)):
+        c = P.product(h, b, H)
+        assert rel(H.full(c), F @ Fb) < 1e-13
+        if not h.leafnode:
+            assert H.gensize(c.A11) == tuple(x + y for x, y in zip(H.gensize(h.A11), H.gensize(b.A11)))
+
+
+def test_oracle_hss_product_rectangular():
+    from oracle import hss_product as P
+    rng = np.random.default_rng(9)
+    ns = [10, 6, 14, 8, 9, 4, 13, 7]
+    a = rect_hss(rng, [7, 12, 9, 15, 5, 11, 8, 6], ns, 3, 5)
+    b = rect_hss(rng, ns, [4, 9, 6, 11, 7, 5, 10, 8], 4, 2)
+    c = P.product(a, b, H)
+    assert rel(H.full(c), H.full(a) @ H.full(b)) < 1e-13
+    # a planted mistake (the cross term of W1 dropped) is caught
+    bad = P.product(a, b, H)
+    bad.A11.W1 = bad.A11.W1.copy()
+    bad.A11.W1[a.A11.W1.shape[0]:, : a.A11.W1.shape[1]] = 0.0
+    assert rel(H.full(bad), H.full(a) @ H.full(b)) > 1e-6
+
+
 def singular_root_hss():
     """Two pass-through leaves (ku = m, ulvfactor.jl:32-36) whose merged root block is zero:
     only the dense root solve (ulvfactor.jl:83) sees the singularity."""
