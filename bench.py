@@ -235,6 +235,25 @@ def batched(pkg, hip, torch, steps=10, warmup=2, count=256, seed=3, log=print):
             out[prec]["worst_rel_residual"] = rep["rel_residual"]
         log(f"[bench] batch {prec}: {ms:.3f} ms per batch -> {count * 1e3 / ms:.1f} solves/s "
             f"(factor {fms:.1f} ms)")
+        # the same plan after the autotuner chose each level's launch (hss_plan_tune)
+        t0 = time.perf_counter()
+        tune = plan.tune(reps=3)
+        tune_wall = (time.perf_counter() - t0) * 1e3
+        for _ in range(warmup):
+            plan.execute(tol=tol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan.execute(tol=tol)
+        torch.cuda.synchronize()
+        tms = (time.perf_counter() - t0) * 1e3 / steps
+        out[prec]["tuned"] = {"batch_ms": round(tms, 3), "solves_per_s": round(count * 1e3 / tms, 1),
+                              "tuning_wall_ms": round(tune_wall, 1),
+                              "one_solve_ms_before_after": [round(tune["ms_before"], 3), round(tune["ms_after"], 3)],
+                              "worst_rel_residual_fp64_check": _batch_residual(mats, Bs, Xs, rs, torch)}
+        out[prec]["best_solves_per_s"] = max(out[prec]["solves_per_s"], out[prec]["tuned"]["solves_per_s"])
+        log(f"[bench] batch {prec} tuned (in {tune_wall:.0f} ms): {tms:.3f} ms per batch -> "
+            f"{count * 1e3 / tms:.1f} solves/s")
         if prec == "fp64":
             # plain path: one resident-factor FP64 solve call per instance
             for _ in range(warmup):
@@ -254,7 +273,7 @@ def batched(pkg, hip, torch, steps=10, warmup=2, count=256, seed=3, log=print):
             f.close()
     for prec, key in (("fp32", "mixed_batched_vs_plain_fp64"), ("fp64", "fp64_batched_vs_plain_fp64")):
         if "solves_per_s" in out[prec]:
-            out[key] = round(out[prec]["solves_per_s"] / out["plain_fp64"]["solves_per_s"], 2)
+            out[key] = round(out[prec]["best_solves_per_s"] / out["plain_fp64"]["solves_per_s"], 2)
     return out
 
 

@@ -2181,12 +2181,12 @@ static hipError_t launch_gup_s(const SInst<T>* insts, const SweepRec* work, int 
 }
 template <typename T>
 hipError_t launch_ulv_gup_s(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, int tw,
-                            size_t lds, int resident, hipStream_t st) {
+                            size_t lds, int resident, int nt, hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
-    if (tw != 16 && tw != 32) return hipErrorInvalidValue;
+    if ((tw != 16 && tw != 32) || (nt != 0 && nt != 256 && nt != 512)) return hipErrorInvalidValue;
     const int tn = (max_nrhs + tw - 1) / tw;
     if ((int64_t)nwork * tn > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
-    const bool wide = wide_level(nwork, max_nrhs);
+    const bool wide = nt ? nt == 256 : wide_level(nwork, max_nrhs);
     return resident ? launch_gup_s<T, true>(insts, work, nwork, tn, tw, lds, wide, st)
                     : launch_gup_s<T, false>(insts, work, nwork, tn, tw, lds, wide, st);
 }
@@ -2209,12 +2209,12 @@ static hipError_t launch_gdown_s(const SInst<T>* insts, const SweepRec* work, in
 }
 template <typename T>
 hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, int tw,
-                              size_t lds, int resident, hipStream_t st) {
+                              size_t lds, int resident, int nt, hipStream_t st) {
     if (nwork <= 0 || max_nrhs <= 0) return hipSuccess;
-    if (tw != 16 && tw != 32) return hipErrorInvalidValue;
+    if ((tw != 16 && tw != 32) || (nt != 0 && nt != 256 && nt != 512)) return hipErrorInvalidValue;
     const int tn = (max_nrhs + tw - 1) / tw;
     if ((int64_t)nwork * tn > (int64_t)1 << 30 || lds > 160 * 1024) return hipErrorInvalidValue;
-    const bool wide = wide_level(nwork, max_nrhs);
+    const bool wide = nt ? nt == 256 : wide_level(nwork, max_nrhs);
     return resident ? launch_gdown_s<T, true>(insts, work, nwork, tn, tw, lds, wide, st)
                     : launch_gdown_s<T, false>(insts, work, nwork, tn, tw, lds, wide, st);
 }
@@ -2362,8 +2362,9 @@ template hipError_t launch_ulv_root<float>(const FInst<float>*, const WorkItem*,
     template hipError_t launch_ulv_rinv<T>(const FInst<T>*, const WorkItem*, int, int, size_t, hipStream_t);       \
     template hipError_t launch_ulv_groot2<T>(const SInst<T>*, const SweepRec*, int, int, size_t, int, hipStream_t);  \
     template hipError_t launch_ulv_gdown<T>(const SInst<T>*, const SweepRec*, int, int, int, hipStream_t);      \
-    template hipError_t launch_ulv_gup_s<T>(const SInst<T>*, const SweepRec*, int, int, int, size_t, int, hipStream_t); \
-    template hipError_t launch_ulv_gdown_s<T>(const SInst<T>*, const SweepRec*, int, int, int, size_t, int,            \
+    template hipError_t launch_ulv_gup_s<T>(const SInst<T>*, const SweepRec*, int, int, int, size_t, int, int,       \
+                                            hipStream_t);                                                              \
+    template hipError_t launch_ulv_gdown_s<T>(const SInst<T>*, const SweepRec*, int, int, int, size_t, int, int,       \
                                               hipStream_t);
 HSS_INST_SOLVE(double)
 HSS_INST_SOLVE(float)

@@ -57,13 +57,14 @@ template <typename T>
 hipError_t launch_ulv_gdown(const SInst<T>* insts, const SweepRec* work, int nwork, int max_m, int max_nrhs,
                             hipStream_t st);
 // Variants with the input staged in LDS (tw = 16 or 32 columns per workgroup, lds = bytes
-// per workgroup); resident != 0: the node operators are staged too, else streamed
+// per workgroup); resident != 0: the node operators are staged too, else streamed;
+// nt = threads per workgroup (256 or 512; 0: 256 on wide levels, else 512)
 template <typename T>
 hipError_t launch_ulv_gup_s(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, int tw, size_t lds,
-                            int resident, hipStream_t st);
+                            int resident, int nt, hipStream_t st);
 template <typename T>
 hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, int tw,
-                              size_t lds, int resident, hipStream_t st);
+                              size_t lds, int resident, int nt, hipStream_t st);
 
 // one workgroup per task (HSS x HSS product generators)
 hipError_t launch_gemm_tasks(const GemmTask* tasks, int count, hipStream_t st);

@@ -17,6 +17,8 @@
 #include <cstddef>
 #include <cstring>
 #include <deque>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -142,6 +144,13 @@ struct SlotCache {
 // depth; up_m = the largest nk, down_m = the largest n of a level's nodes; for the
 // LDS-resident kernels, up_na / down_na = the largest operator (elements, rounded to 16)
 // and up_k / down_k = the most input rows of a level's nodes; w = bytes per element.
+// Launch choice for one sweep level: form 0 = LDS-resident (operator and input staged),
+// 1 = operator streamed by the waves, 2 = chunked tile kernel; tw = columns per workgroup
+// (16 or 32); nt = threads per workgroup (256, 512; 0 = by the level's width).
+// form < 0: the built-in rules (pick_resident, pick_streamed).
+struct LevelCfg {
+    int form = -1, tw = 0, nt = 0;
+};
 struct Schedule {
     std::vector<WorkItem> work;
     std::vector<SweepRec> rec;  // the solve sweeps' records, same order as work
@@ -150,6 +159,8 @@ struct Schedule {
     std::vector<int64_t> up_na, down_na, up_k, down_k;
     size_t w = 8;
     Range roots;
+    // per-level choices made by the autotuner (hss_plan_tune); empty = built-in rules
+    std::vector<LevelCfg> up_cfg, down_cfg;
 };
 // Largest LDS of a workgroup (bytes); levels above it use the streamed kernels.
 constexpr size_t kLdsResidentMax = 160 * 1024;
@@ -1754,25 +1765,65 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
 // lgroot_res = LDS of the root kernel with the root factor staged (used when it fits)
 // lgroot2 = LDS of the inverse-and-refine root kernel (0 when some root lacks it),
 // lgroot2s = the same with every root operator staged
+// LDS bytes of a sweep step in form 0 (operator na elements and k input rows staged) or
+// form 1 (input only) with tw columns; false when they do not fit
+static bool level_lds(size_t w, int form, int64_t na, int64_t k, int tw, size_t& lds) {
+    lds = w * (size_t)((form == 0 ? na : 0) + k * (tw + 1));
+    return lds <= kLdsResidentMax;
+}
+// The built-in choice for a level
+static LevelCfg default_cfg(size_t w, int64_t na, int64_t k, int rows, int count, int maxnrhs) {
+    int tw = 32;
+    size_t lds = 0;
+    if (pick_resident(w, na, k, count, maxnrhs, tw, lds)) return LevelCfg{0, tw, 0};
+    if (pick_streamed(w, k, rows, count, maxnrhs, tw, lds)) return LevelCfg{1, tw, 0};
+    return LevelCfg{2, 0, 0};
+}
+
+// One up-sweep level (height h) with choice c (c.form < 0: the built-in rules)
+template <typename TF>
+static hss_status launch_up_level(const SInst<TF>* di, const SweepRec* dw, const Schedule& S, size_t h, int maxnrhs,
+                                  LevelCfg c, hipStream_t st) {
+    const int cnt = S.up[h].count;
+    if (c.form < 0) c = default_cfg(S.w, S.up_na[h], S.up_k[h], S.up_rows[h], cnt, maxnrhs);
+    if (c.form == 2) {
+        ProfScope ps("k_ulv_gup", (int)h, cnt, st);
+        HIP_TRY((launch_ulv_gup<TF>(di, dw + S.up[h].off, cnt, S.up_m[h], maxnrhs, st)));
+        return HSS_OK;
+    }
+    size_t lds = 0;
+    if (!level_lds(S.w, c.form, S.up_na[h], S.up_k[h], c.tw, lds))
+        return fail(HSS_ERR_ARG, "level choice does not fit in LDS");
+    ProfScope ps(c.form == 0 ? "k_ulv_gup_s" : "k_ulv_gup_x", (int)h, cnt, st);
+    HIP_TRY((launch_ulv_gup_s<TF>(di, dw + S.up[h].off, cnt, maxnrhs, c.tw, lds, c.form == 0 ? 1 : 0, c.nt, st)));
+    return HSS_OK;
+}
+// One down-sweep level (depth d) with choice c
+template <typename TF>
+static hss_status launch_down_level(const SInst<TF>* di, const SweepRec* dw, const Schedule& S, size_t d, int maxnrhs,
+                                    LevelCfg c, hipStream_t st) {
+    const int cnt = S.down[d].count;
+    if (c.form < 0) c = default_cfg(S.w, S.down_na[d], S.down_k[d], S.down_m[d], cnt, maxnrhs);
+    if (c.form == 2) {
+        ProfScope ps("k_ulv_gdown", (int)d, cnt, st);
+        HIP_TRY((launch_ulv_gdown<TF>(di, dw + S.down[d].off, cnt, S.down_m[d], maxnrhs, st)));
+        return HSS_OK;
+    }
+    size_t lds = 0;
+    if (!level_lds(S.w, c.form, S.down_na[d], S.down_k[d], c.tw, lds))
+        return fail(HSS_ERR_ARG, "level choice does not fit in LDS");
+    ProfScope ps(c.form == 0 ? "k_ulv_gdown_s" : "k_ulv_gdown_x", (int)d, cnt, st);
+    HIP_TRY((launch_ulv_gdown_s<TF>(di, dw + S.down[d].off, cnt, maxnrhs, c.tw, lds, c.form == 0 ? 1 : 0, c.nt,
+                                    st)));
+    return HSS_OK;
+}
+
 template <typename TF>
 static hss_status launch_schedule(const SInst<TF>* di, const SweepRec* dw, const Schedule& S, int maxnrhs,
                                   size_t lgroot, size_t lgroot_res, size_t lgroot2, size_t lgroot2s, hipStream_t st) {
-    // operators staged whole in LDS (use_resident, with 32 or 16 columns per workgroup),
-    // else streamed through the chunked kernels
-    int tw = 32;
-    size_t lds = 0;
     for (size_t h = 0; h < S.up.size(); ++h) {
-        const int cnt = S.up[h].count;
-        if (pick_resident(S.w, S.up_na[h], S.up_k[h], cnt, maxnrhs, tw, lds)) {
-            ProfScope ps("k_ulv_gup_s", (int)h, cnt, st);
-            HIP_TRY((launch_ulv_gup_s<TF>(di, dw + S.up[h].off, cnt, maxnrhs, tw, lds, 1, st)));
-        } else if (pick_streamed(S.w, S.up_k[h], S.up_rows[h], cnt, maxnrhs, tw, lds)) {
-            ProfScope ps("k_ulv_gup_x", (int)h, cnt, st);
-            HIP_TRY((launch_ulv_gup_s<TF>(di, dw + S.up[h].off, cnt, maxnrhs, tw, lds, 0, st)));
-        } else {
-            ProfScope ps("k_ulv_gup", (int)h, S.up[h].count, st);
-            HIP_TRY((launch_ulv_gup<TF>(di, dw + S.up[h].off, S.up[h].count, S.up_m[h], maxnrhs, st)));
-        }
+        hss_status s = launch_up_level<TF>(di, dw, S, h, maxnrhs, h < S.up_cfg.size() ? S.up_cfg[h] : LevelCfg{}, st);
+        if (s != HSS_OK) return s;
     }
     {
         if (lgroot2 > 0 && lgroot2s > 0 && lgroot2s <= kLdsResidentMax) {
@@ -1789,17 +1840,9 @@ static hss_status launch_schedule(const SInst<TF>* di, const SweepRec* dw, const
         }
     }
     for (size_t d = 1; d < S.down.size(); ++d) {
-        const int cnt = S.down[d].count;
-        if (pick_resident(S.w, S.down_na[d], S.down_k[d], cnt, maxnrhs, tw, lds)) {
-            ProfScope ps("k_ulv_gdown_s", (int)d, cnt, st);
-            HIP_TRY((launch_ulv_gdown_s<TF>(di, dw + S.down[d].off, cnt, maxnrhs, tw, lds, 1, st)));
-        } else if (pick_streamed(S.w, S.down_k[d], S.down_m[d], cnt, maxnrhs, tw, lds)) {
-            ProfScope ps("k_ulv_gdown_x", (int)d, cnt, st);
-            HIP_TRY((launch_ulv_gdown_s<TF>(di, dw + S.down[d].off, cnt, maxnrhs, tw, lds, 0, st)));
-        } else {
-            ProfScope ps("k_ulv_gdown", (int)d, S.down[d].count, st);
-            HIP_TRY((launch_ulv_gdown<TF>(di, dw + S.down[d].off, S.down[d].count, S.down_m[d], maxnrhs, st)));
-        }
+        hss_status s =
+            launch_down_level<TF>(di, dw, S, d, maxnrhs, d < S.down_cfg.size() ? S.down_cfg[d] : LevelCfg{}, st);
+        if (s != HSS_OK) return s;
     }
     return HSS_OK;
 }
@@ -2203,6 +2246,151 @@ template <typename TF>
 static hss_status plan_solve(hss_plan_s* P, int t) {
     return launch_schedule<TF>(reinterpret_cast<const SInst<TF>*>(P->d_sinst[t]), P->d_rec, P->S, P->maxnrhs,
                                P->lgroot, P->lgroot_res, P->all_groot2 ? P->lgroot2 : 0, P->lgroot2s, P->ctx->stream);
+}
+
+// ---- autotuner: per-level launch choices of a plan's solve sweeps, by timing ----
+// Candidates: LDS-resident or streamed at 16 or 32 columns and 256 or 512 threads (where
+// the staging fits), and the chunked kernel.  A level's inputs are the previous levels'
+// outputs, so one solve runs first; every sweep step is a pure function of its inputs,
+// so launching one level repeatedly leaves the results as they were.  Choices are kept
+// per level shape class in a process-wide cache (precision, direction, node count,
+// columns, operator and input sizes), so identical levels are timed once.
+namespace {
+struct TuneKey {
+    int64_t v[7];
+    bool operator<(const TuneKey& o) const { return std::lexicographical_compare(v, v + 7, o.v, o.v + 7); }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, LevelCfg> g_tune_cache;
+}  // namespace
+
+template <typename TF>
+static hss_status tune_plan(hss_plan_s* P, int reps) {
+    hipStream_t st = P->ctx->stream;
+    const SInst<TF>* di = reinterpret_cast<const SInst<TF>*>(P->d_sinst[0]);
+    Schedule& S = P->S;
+    const int nrhs = P->maxnrhs;
+    hss_status s = plan_solve<TF>(P, 0);  // every level's inputs in place
+    if (s != HSS_OK) return s;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    struct Ev {
+        hipEvent_t a, b;
+        ~Ev() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } ev{e0, e1};
+    auto candidates = [&](int64_t na, int64_t k) {
+        std::vector<LevelCfg> v;
+        for (int form : {0, 1})
+            for (int tw : {16, 32}) {
+                if (tw == 32 && nrhs <= 16) continue;
+                size_t lds = 0;
+                if (!level_lds(S.w, form, na, k, tw, lds)) continue;
+                for (int nt : {256, 512}) v.push_back(LevelCfg{form, tw, nt});
+            }
+        v.push_back(LevelCfg{2, 0, 0});
+        return v;
+    };
+    // ms per launch of one level under choice c; < 0 if it cannot launch
+    auto time_level = [&](auto&& launch, const LevelCfg& c) -> double {
+        if (launch(c) != HSS_OK) return -1.0;
+        if (hipEventRecord(e0, st) != hipSuccess) return -1.0;
+        for (int r = 0; r < reps; ++r)
+            if (launch(c) != HSS_OK) return -1.0;
+        float ms = 0.f;
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            return -1.0;
+        return ms / reps;
+    };
+    auto tune = [&](int dir, int count, int64_t na, int64_t k, int rows, auto&& launch) -> LevelCfg {
+        const TuneKey key{{(int64_t)S.w, dir, count, nrhs, na, k, rows}};
+        {
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            auto it = g_tune_cache.find(key);
+            if (it != g_tune_cache.end()) return it->second;
+        }
+        LevelCfg best = default_cfg(S.w, na, k, rows, count, nrhs);
+        double tbest = time_level(launch, best);
+        for (const LevelCfg& c : candidates(na, k)) {
+            const double t = time_level(launch, c);
+            if (t >= 0 && (tbest < 0 || t < tbest)) {
+                tbest = t;
+                best = c;
+            }
+        }
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune_cache[key] = best;
+        return best;
+    };
+    S.up_cfg.assign(S.up.size(), LevelCfg{});
+    S.down_cfg.assign(S.down.size(), LevelCfg{});
+    for (size_t h = 0; h < S.up.size(); ++h) {
+        if (S.up[h].count == 0) continue;
+        S.up_cfg[h] = tune(0, S.up[h].count, S.up_na[h], S.up_k[h], S.up_rows[h], [&](const LevelCfg& c) {
+            return launch_up_level<TF>(di, P->d_rec, S, h, nrhs, c, st);
+        });
+    }
+    for (size_t d = 1; d < S.down.size(); ++d) {
+        if (S.down[d].count == 0) continue;
+        S.down_cfg[d] = tune(1, S.down[d].count, S.down_na[d], S.down_k[d], S.down_m[d], [&](const LevelCfg& c) {
+            return launch_down_level<TF>(di, P->d_rec, S, d, nrhs, c, st);
+        });
+    }
+    // the results of the tuned launches were left as a solve leaves them; re-solve once so
+    // the outputs are the tuned schedule's own
+    return plan_solve<TF>(P, 0);
+}
+
+extern "C" hss_status hss_plan_tune(hss_plan_t P, int32_t reps, double* ms_before, double* ms_after) {
+    if (!P) return fail(HSS_ERR_ARG, "null plan");
+    if (reps < 1) return fail(HSS_ERR_ARG, "reps < 1");
+    HIP_TRY(hipSetDevice(P->ctx->device));
+    hipStream_t st = P->ctx->stream;
+    const bool fp32 = P->prec == HSS_PREC_FP32;
+    auto whole = [&](double* out) -> hss_status {
+        if (!out) return HSS_OK;
+        hss_status s = fp32 ? plan_solve<float>(P, 0) : plan_solve<double>(P, 0);  // warm
+        if (s != HSS_OK) return s;
+        HIP_TRY(hipEventRecord(P->ctx->ev0, st));
+        for (int r = 0; r < reps; ++r) {
+            s = fp32 ? plan_solve<float>(P, 0) : plan_solve<double>(P, 0);
+            if (s != HSS_OK) return s;
+        }
+        HIP_TRY(hipEventRecord(P->ctx->ev1, st));
+        HIP_TRY(hipEventSynchronize(P->ctx->ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, P->ctx->ev0, P->ctx->ev1));
+        *out = ms / reps;
+        return HSS_OK;
+    };
+    hss_status s = whole(ms_before);
+    if (s != HSS_OK) return s;
+    s = fp32 ? tune_plan<float>(P, reps) : tune_plan<double>(P, reps);
+    if (s != HSS_OK) return s;
+    // a captured graph holds the old launches: capture again on the next execute
+    if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    P->gexec = nullptr;
+    P->graph = nullptr;
+    return whole(ms_after);
+}
+
+extern "C" hss_status hss_plan_tune_choice(hss_plan_t P, int32_t dir, int32_t level, int32_t* form, int32_t* tw,
+                                           int32_t* nt) {
+    if (!P || !form || !tw || !nt) return fail(HSS_ERR_ARG, "null argument");
+    const std::vector<LevelCfg>& v = dir == 0 ? P->S.up_cfg : P->S.down_cfg;
+    if (dir != 0 && dir != 1) return fail(HSS_ERR_ARG, "dir must be 0 (up) or 1 (down)");
+    const int n = (int)(dir == 0 ? P->S.up.size() : P->S.down.size());
+    if (level < 0 || level >= n) return fail(HSS_ERR_ARG, "level out of range");
+    const LevelCfg c = level < (int)v.size() ? v[level] : LevelCfg{};
+    *form = c.form;
+    *tw = c.tw;
+    *nt = c.nt;
+    return HSS_OK;
 }
 
 static hss_status plan_matvec(hss_plan_s* P) {

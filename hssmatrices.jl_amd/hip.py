@@ -55,6 +55,7 @@ EXPORTS = [
     "hss_last_error", "hss_last_device_ms",
     "hss_prof_enable", "hss_prof_reset", "hss_prof_count", "hss_prof_get", "hss_fact_node_info",
     "hss_plan_create", "hss_plan_execute", "hss_plan_destroy", "hss_plan_device_bytes",
+    "hss_plan_tune", "hss_plan_tune_choice",
     "hss_server_create", "hss_server_admit", "hss_server_evict", "hss_server_used_bytes", "hss_server_submit",
     "hss_server_step", "hss_server_poll", "hss_server_destroy",
 ]
@@ -130,6 +131,9 @@ def load_library(path: pathlib.Path | str | None = None):
         "hss_plan_create": ([I32, V, V, V, V, V, V, I32, PV], ctypes.c_int),
         "hss_plan_execute": ([V, ctypes.POINTER(RefineOpts), ctypes.POINTER(SolveReport)], ctypes.c_int),
         "hss_plan_destroy": ([V], ctypes.c_int),
+        "hss_plan_tune": ([V, I32, ctypes.POINTER(D), ctypes.POINTER(D)], ctypes.c_int),
+        "hss_plan_tune_choice": ([V, I32, I32, ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(I32)],
+                                 ctypes.c_int),
         "hss_plan_device_bytes": ([V], I64),
         "hss_server_create": ([V, I64, ctypes.c_int, ctypes.POINTER(RefineOpts), I32, PV], ctypes.c_int),
         "hss_server_admit": ([V, V, ctypes.POINTER(I32)], ctypes.c_int),
@@ -572,6 +576,21 @@ class Plan:
                                    ctypes.cast(x, ctypes.c_void_p), ctypes.cast(lx, ctypes.c_void_p),
                                    1 if refinement else 0, ctypes.byref(h)))
         self._h = h
+
+    def tune(self, reps: int = 5, levels: int = 0):
+        """Autotune the per-level launch choices (``hss_plan_tune``): returns the plan's solve
+        time before and after (ms) and, for the first ``levels`` up/down levels, the choices
+        as (form, tw, nt) with form 0 resident, 1 streamed, 2 chunked."""
+        b, a = ctypes.c_double(), ctypes.c_double()
+        _check(_lib.hss_plan_tune(self._h, reps, ctypes.byref(b), ctypes.byref(a)))
+        out = {"ms_before": b.value, "ms_after": a.value, "up": [], "down": []}
+        for d, key in ((0, "up"), (1, "down")):
+            for lv in range(levels):
+                f, t, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+                if _lib.hss_plan_tune_choice(self._h, d, lv, ctypes.byref(f), ctypes.byref(t), ctypes.byref(n)):
+                    break
+                out[key].append((f.value, t.value, n.value))
+        return out
 
     def execute(self, tol: float | None = None, max_iter: int = 10):
         """Enqueue the batched solve; with ``tol`` refine in lockstep (synchronous)."""

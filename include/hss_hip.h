@@ -200,6 +200,16 @@ hss_status hss_plan_create(int32_t count, const hss_fact_t* facts, const int32_t
                            hss_plan_t* out);
 hss_status hss_plan_execute(hss_plan_t plan, const hss_refine_opts* opts, hss_solve_report* report);
 hss_status hss_plan_destroy(hss_plan_t plan);
+/* Autotuner (no reference counterpart; the reference has no device kernels to tune): times
+ * every valid launch choice of each up- and down-sweep level of the plan's solve (kernel form
+ * LDS-resident / streamed / chunked, 16 or 32 columns, 256 or 512 threads per workgroup) with
+ * reps launches each and keeps the fastest per level.  Choices are cached per level shape
+ * for the process.  Results stay those of the same solve up to rounding.  ms_before /
+ * ms_after (may be null): one solve of the plan before and after, in ms.  Synchronous. */
+hss_status hss_plan_tune(hss_plan_t plan, int32_t reps, double* ms_before, double* ms_after);
+/* The choice for level `level` of the up (dir 0, by height) or down (dir 1, by depth)
+ * sweep: form 0 resident, 1 streamed, 2 chunked, -1 built-in rules; tw; nt (0 = by width) */
+hss_status hss_plan_tune_choice(hss_plan_t plan, int32_t dir, int32_t level, int32_t* form, int32_t* tw, int32_t* nt);
 int64_t hss_plan_device_bytes(hss_plan_t plan);
 
 /* ---- serving (no reference counterpart; config C5) ----

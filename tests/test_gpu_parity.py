@@ -490,6 +490,75 @@ def test_plan_mixed_precision_lockstep_refinement(hip, oracle):
     plan.close()
 
 
+def _plan_inputs(rng, packs, rs):
+    import torch
+    dev = torch.device("cuda:0")
+    Bs, Xs = [], []
+    for (nodes, _), r in zip(packs, rs):
+        n = int(nodes["n"][0])
+        Bs.append(torch.from_numpy(np.ascontiguousarray(rng.standard_normal((r, n)))).to(dev))
+        Xs.append(torch.zeros((r, n), dtype=torch.float64, device=dev))
+    torch.cuda.synchronize()
+    return Bs, Xs
+
+
+def test_plan_autotune(hip, oracle):
+    """hss_plan_tune: every level gets a valid launch choice; the tuned plan's solves equal
+    the untuned plan's up to rounding and match the oracle; its graph replays repeat bit for
+    bit; an FP32 lockstep plan still refines to its tolerance after tuning."""
+    import torch
+    rng = np.random.default_rng(33)
+    specs = [(900, 64), (400, 32), (1300, 64), (700, 32)]
+    rs = [3, 17, 32, 1]
+    mats, packs, facts = _batch_inputs(hip, rng, specs, "fp64")
+    Bs, Xs = _plan_inputs(rng, packs, rs)
+    plan = hip.Plan(facts, rs, [b.data_ptr() for b in Bs], [int(b.shape[1]) for b in Bs],
+                    [x.data_ptr() for x in Xs], [int(x.shape[1]) for x in Xs])
+    plan.execute()
+    torch.cuda.synchronize()
+    before = [x.cpu().numpy().copy() for x in Xs]
+    rep = plan.tune(reps=2, levels=12)
+    assert rep["ms_before"] > 0 and rep["ms_after"] > 0
+    assert rep["up"] and rep["down"]
+    for f, tw, nt in rep["up"] + rep["down"]:
+        assert f in (-1, 0, 1, 2)
+        if f in (0, 1):
+            assert tw in (16, 32) and nt in (0, 256, 512)
+    for x in Xs:
+        x.zero_()
+    torch.cuda.synchronize()
+    plan.execute()
+    torch.cuda.synchronize()
+    after = [x.cpu().numpy().copy() for x in Xs]
+    for a, b0 in zip(after, before):
+        assert rel(a, b0) < 1e-12
+    for _ in range(2):  # replays of the graph captured with the tuned launches
+        for x in Xs:
+            x.zero_()
+        torch.cuda.synchronize()
+        plan.execute()
+        torch.cuda.synchronize()
+        for x, a in zip(Xs, after):
+            assert np.array_equal(x.cpu().numpy(), a)
+    for i, (nodes, blob) in enumerate(packs):
+        B = Bs[i].cpu().numpy().T
+        assert rel(after[i].T, oracle.ulvfactsolve(nodes, blob, B)) < 1e-11
+    plan.close()
+    # mixed precision: FP32 factors, lockstep refinement after tuning
+    facts32 = hip.ulvfactor_batch(mats, "fp32")
+    Bs, Xs = _plan_inputs(rng, packs, rs)
+    plan = hip.Plan(facts32, rs, [b.data_ptr() for b in Bs], [int(b.shape[1]) for b in Bs],
+                    [x.data_ptr() for x in Xs], [int(x.shape[1]) for x in Xs], refinement=True)
+    plan.tune(reps=2)
+    r = plan.execute(tol=1e-10, max_iter=20)
+    assert r["rel_residual"] <= 1e-10
+    for i, (nodes, blob) in enumerate(packs):
+        B = Bs[i].cpu().numpy().T
+        R = oracle.matmul(nodes, blob, Xs[i].cpu().numpy().T) - B
+        assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)) <= 1e-10
+    plan.close()
+
+
 def test_wide_batch_levels(hip, oracle):
     """Levels of at least 2048 (node, 32-column) tiles run 256-thread workgroups with the
     operators streamed; every instance matches its own single solve and the oracle, and an
