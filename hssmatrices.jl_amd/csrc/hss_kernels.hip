@@ -372,49 +372,141 @@ __global__ __launch_bounds__(FACT_THREADS) void k_ulv_factor(const FInst<T>* __r
         const ChildRed<T> b = child_red(I, nd.right);
         const DevNode dl = I.nodes[nd.left];
         const DevNode dr = I.nodes[nd.right];
-        merge_products(I, nd, f, a, b);
-        __threadfence_block();
-        __syncthreads();
-        const T* P12 = I.fac + f.P12;
-        const T* P21 = I.fac + f.P21;
         T* Wmw = I.rw + f.rwM;
-        for (int e = tid; e < R * n; e += blockDim.x) {
-            const int rr = e / n, c = e % n;
-            T val;
-            if (rr < m) {
-                val = merged_D(a, b, P12, P21, rr, c);
-            } else {
-                // V = [V1 W1; V2 W2]  (ulvfactor.jl:79), row j of V'
-                const int j = rr - m;
-                T sum = T(0);
-                if (c < a.nr) {
-                    for (int l = 0; l < a.kv; ++l)
-                        sum += a.V[c + (int64_t)l * a.nr] * gen<T>(I.blob, dl.W, l + (int64_t)j * a.kv);
-                } else {
-                    const int cc = c - a.nr;
-                    for (int l = 0; l < b.kv; ++l)
-                        sum += b.V[cc + (int64_t)l * b.nr] * gen<T>(I.blob, dr.W, l + (int64_t)j * b.kv);
+        // The merge's inner products run over the children's ranks, so they are staged:
+        // the LDS after Us (used by phases 1-5 only later) holds the rank-sized operands,
+        // loaded with many loads in flight, and the products read LDS.  This node's own
+        // LDS size (the formula of the launch's, which is the level's largest) bounds
+        // what fits; a merge that does not fit reads its operands from memory.
+        const int64_t cap = 2 * (int64_t)k * k + (int64_t)(k + 2) * vstride + (k > nk ? k : nk) + 1 + 8;
+        auto mx = [](int64_t x, int64_t y) { return x > y ? x : y; };
+        const int64_t nP12 = (int64_t)a.mr * b.kv, nP21 = (int64_t)b.mr * a.kv;
+        const int64_t nU1 = (int64_t)a.mr * a.ku, nU2 = (int64_t)b.mr * b.ku;
+        const int64_t nV1 = (int64_t)a.nr * a.kv, nV2 = (int64_t)b.nr * b.kv;
+        // the most any one step below holds at once
+        const int64_t need = mx(mx(nP12 + mx(nU1 + (int64_t)a.ku * b.kv, nV2), nP21 + mx(nU2 + (int64_t)b.ku * a.kv, nV1)),
+                                mx(mx(nV1 + (int64_t)a.kv * kv, nV2 + (int64_t)b.kv * kv),
+                                   mx(nU1 + (int64_t)a.ku * ku, nU2 + (int64_t)b.ku * ku)));
+        if (need <= cap) {
+            auto stage_to = [&](T* dst, int cnt, auto ld) { batched<8, T>(cnt, ld, [&](int e, T v) { dst[e] = v; }); };
+            // rows of the merged D (ulvfactor.jl:75): [D1, U1 B12 V2'] then [U2 B21 V1', D2].
+            // P = U1 B12 (or U2 B21) is kept in the factor too: the solve operators use it.
+            for (int half = 0; half < 2; ++half) {
+                const ChildRed<T>& x = half ? b : a;  // rows from this child
+                const ChildRed<T>& o = half ? a : b;  // the other child (columns of V')
+                const int64_t nP = (int64_t)x.mr * o.kv;
+                T* sP = G;
+                T* sU = sP + nP;
+                T* sB = sU + (int64_t)x.mr * x.ku;
+                stage_to(sU, x.mr * x.ku, [&](int e) { return x.U[e]; });
+                stage_to(sB, x.ku * o.kv, [&](int e) { return gen<T>(I.blob, half ? nd.B21 : nd.B12, e); });
+                __syncthreads();
+                T* Pg = I.fac + (half ? f.P21 : f.P12);
+                for (int e = tid; e < nP; e += blockDim.x) {
+                    const int rr = e % x.mr, c = e / x.mr;
+                    T sum = T(0);
+                    for (int l = 0; l < x.ku; ++l) sum += sU[rr + l * x.mr] * sB[l + c * x.ku];
+                    sP[e] = sum;
+                    Pg[e] = sum;
                 }
-                val = sum;
+                __syncthreads();
+                T* sV = sU;  // the other child's V, over U and B
+                stage_to(sV, o.nr * o.kv, [&](int e) { return o.V[e]; });
+                __syncthreads();
+                const int r0 = half ? a.mr : 0, own0 = half ? a.nr : 0;  // own columns start
+                for (int e = tid; e < x.mr * n; e += blockDim.x) {
+                    const int rr = e / n, c = e % n;
+                    T val = T(0);
+                    if (c >= own0 && c < own0 + x.nr) {
+                        val = x.D[rr + (int64_t)(c - own0) * x.mr];
+                    } else {
+                        const int cc = half ? c : c - a.nr;  // column of the other child's V'
+                        for (int l = 0; l < o.kv; ++l) val += sP[rr + l * x.mr] * sV[cc + l * o.nr];
+                    }
+                    Wmw[(int64_t)(r0 + rr) * n + c] = val;
+                }
+                __syncthreads();
             }
-            Wmw[e] = val;
-        }
-        // U = [U1 R1; U2 R2]  (ulvfactor.jl:78)
-        for (int e = tid; e < m * ku; e += blockDim.x) {
-            const int rr = e % m, j = e / m;
-            T sum = T(0);
-            if (rr < a.mr) {
-                for (int l = 0; l < a.ku; ++l)
-                    sum += a.U[rr + (int64_t)l * a.mr] * gen<T>(I.blob, dl.R, l + (int64_t)j * a.ku);
-            } else {
-                const int r2 = rr - a.mr;
-                for (int l = 0; l < b.ku; ++l)
-                    sum += b.U[r2 + (int64_t)l * b.mr] * gen<T>(I.blob, dr.R, l + (int64_t)j * b.ku);
+            // rows of V' (ulvfactor.jl:79): V = [V1 W1; V2 W2]; columns from each child in turn
+            for (int half = 0; half < 2; ++half) {
+                const ChildRed<T>& x = half ? b : a;
+                T* sV = G;
+                T* sW = sV + (int64_t)x.nr * x.kv;
+                stage_to(sV, x.nr * x.kv, [&](int e) { return x.V[e]; });
+                stage_to(sW, x.kv * kv, [&](int e) { return gen<T>(I.blob, half ? dr.W : dl.W, e); });
+                __syncthreads();
+                const int c0x = half ? a.nr : 0;
+                for (int e = tid; e < kv * x.nr; e += blockDim.x) {
+                    const int j = e / x.nr, c = e % x.nr;
+                    T val = T(0);
+                    for (int l = 0; l < x.kv; ++l) val += sV[c + l * x.nr] * sW[l + j * x.kv];
+                    Wmw[(int64_t)(m + j) * n + c0x + c] = val;
+                }
+                __syncthreads();
             }
-            Us[e] = sum;
+            // U = [U1 R1; U2 R2]  (ulvfactor.jl:78)
+            for (int half = 0; half < 2; ++half) {
+                const ChildRed<T>& x = half ? b : a;
+                T* sU = G;
+                T* sR = sU + (int64_t)x.mr * x.ku;
+                stage_to(sU, x.mr * x.ku, [&](int e) { return x.U[e]; });
+                stage_to(sR, x.ku * ku, [&](int e) { return gen<T>(I.blob, half ? dr.R : dl.R, e); });
+                __syncthreads();
+                const int r0 = half ? a.mr : 0;
+                for (int e = tid; e < x.mr * ku; e += blockDim.x) {
+                    const int rr = e % x.mr, j = e / x.mr;
+                    T sum = T(0);
+                    for (int l = 0; l < x.ku; ++l) sum += sU[rr + l * x.mr] * sR[l + j * x.ku];
+                    Us[(r0 + rr) + (int64_t)j * m] = sum;
+                }
+                __syncthreads();
+            }
+            Wm = Wmw;
+            __threadfence_block();
+        } else {
+            merge_products(I, nd, f, a, b);
+            __threadfence_block();
+            __syncthreads();
+            const T* P12 = I.fac + f.P12;
+            const T* P21 = I.fac + f.P21;
+            for (int e = tid; e < R * n; e += blockDim.x) {
+                const int rr = e / n, c = e % n;
+                T val;
+                if (rr < m) {
+                    val = merged_D(a, b, P12, P21, rr, c);
+                } else {
+                    // V = [V1 W1; V2 W2]  (ulvfactor.jl:79), row j of V'
+                    const int j = rr - m;
+                    T sum = T(0);
+                    if (c < a.nr) {
+                        for (int l = 0; l < a.kv; ++l)
+                            sum += a.V[c + (int64_t)l * a.nr] * gen<T>(I.blob, dl.W, l + (int64_t)j * a.kv);
+                    } else {
+                        const int cc = c - a.nr;
+                        for (int l = 0; l < b.kv; ++l)
+                            sum += b.V[cc + (int64_t)l * b.nr] * gen<T>(I.blob, dr.W, l + (int64_t)j * b.kv);
+                    }
+                    val = sum;
+                }
+                Wmw[e] = val;
+            }
+            // U = [U1 R1; U2 R2]  (ulvfactor.jl:78)
+            for (int e = tid; e < m * ku; e += blockDim.x) {
+                const int rr = e % m, j = e / m;
+                T sum = T(0);
+                if (rr < a.mr) {
+                    for (int l = 0; l < a.ku; ++l)
+                        sum += a.U[rr + (int64_t)l * a.mr] * gen<T>(I.blob, dl.R, l + (int64_t)j * a.ku);
+                } else {
+                    const int r2 = rr - a.mr;
+                    for (int l = 0; l < b.ku; ++l)
+                        sum += b.U[r2 + (int64_t)l * b.mr] * gen<T>(I.blob, dr.R, l + (int64_t)j * b.ku);
+                }
+                Us[e] = sum;
+            }
+            Wm = Wmw;
+            __threadfence_block();
         }
-        Wm = Wmw;
-        __threadfence_block();
     }
     __syncthreads();
     // entry (rr, c) of W = [D; V']
