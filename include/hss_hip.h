@@ -204,8 +204,10 @@ hss_status hss_plan_destroy(hss_plan_t plan);
  * every valid launch choice of each up- and down-sweep level of the plan's solve (kernel form
  * LDS-resident / streamed / chunked, 16 or 32 columns, 256 or 512 threads per workgroup) with
  * reps launches each and keeps the fastest per level.  Choices are cached per level shape
- * for the process.  Results stay those of the same solve up to rounding.  ms_before /
- * ms_after (may be null): one solve of the plan before and after, in ms.  Synchronous. */
+ * for the process.  Results stay those of the same solve up to rounding.  Tuning runs the
+ * plan's solve (b -> x) repeatedly, so the plan's solution buffers hold its result afterwards.
+ * ms_before / ms_after (may be null): one solve of the plan before and after, in ms.
+ * Synchronous. */
 hss_status hss_plan_tune(hss_plan_t plan, int32_t reps, double* ms_before, double* ms_after);
 /* The choice for level `level` of the up (dir 0, by height) or down (dir 1, by depth)
  * sweep: form 0 resident, 1 streamed, 2 chunked, -1 built-in rules; tw; nt (0 = by width) */
