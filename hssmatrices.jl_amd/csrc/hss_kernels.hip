@@ -81,16 +81,51 @@ __device__ __forceinline__ T shfl_xor(T v, int mask) {
     return __shfl_xor(v, mask, 64);
 }
 
-// sum over the S adjacent lanes of a row group (S a power of two <= 64)
+// v from another lane by a DPP lane move (no LDS round trip); 64-bit values move as two
+// 32-bit halves.  CTRL: 0xB1 = lane ^ 1, 0x4E = lane ^ 2 (quad permutes), 0x141 = mirror
+// within each half row of 8 lanes, 0x140 = mirror within each row of 16.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+#else
+    return v;
+#endif
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+#else
+    return v;
+#endif
+}
+
+// sum over the S adjacent lanes of a row group (S a power of two <= 64, groups aligned to
+// S, all lanes of a group active): the first two steps are quad permutes
 template <typename T>
 __device__ __forceinline__ T row_reduce(T v, int S) {
-    for (int o = 1; o < S; o <<= 1) v += shfl_xor(v, o);
+    if (S > 1) v += dpp_mov<0xB1>(v);
+    if (S > 2) v += dpp_mov<0x4E>(v);
+    for (int o = 4; o < S; o <<= 1) v += shfl_xor(v, o);
     return v;
 }
 
+// sum over the wave (all lanes active), as the butterfly lane ^ 1, 2, 4, 8, 16, 32: once
+// the values are equal within quads (then within half rows), the half-row mirror (then the
+// row mirror) pairs lanes exactly as ^ 4 (then ^ 8) does, so every lane ends with the same
+// sum, added in the same order
 template <typename T>
 __device__ __forceinline__ T wave_reduce(T v) {
-    for (int o = 1; o < 64; o <<= 1) v += shfl_xor(v, o);
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x141>(v);
+    v += dpp_mov<0x140>(v);
+    v += shfl_xor(v, 16);
+    v += shfl_xor(v, 32);
     return v;
 }
 
