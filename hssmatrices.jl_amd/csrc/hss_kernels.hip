@@ -367,8 +367,12 @@ __device__ void merge_products(const FInst<T>& I, const DevNode& nd, const FNode
 template <int CPT>
 __device__ __forceinline__ int vpad(int c) { return c + (c / CPT) * 2; }
 
-template <typename T, int CPT>
-__global__ __launch_bounds__(FACT_THREADS) void k_ulv_factor(const FInst<T>* __restrict__ insts,
+// NT = threads per workgroup (the launch bound).  FP64 levels whose nodes need at most 768
+// threads run 768: the register budget per thread grows from 128 to 168 VGPRs and the
+// 1024-thread build's spills go (the leaves' LQ 28 % faster).  FP32 fits 1024 threads
+// without spills and keeps them.
+template <typename T, int CPT, int NT>
+__global__ __launch_bounds__(NT) void k_ulv_factor(const FInst<T>* __restrict__ insts,
                                                              const WorkItem* __restrict__ work) {
     const WorkItem wi = work[blockIdx.x];
     const FInst<T>& I = insts[wi.inst];
@@ -2182,14 +2186,22 @@ static hipError_t set_lds_attr(const void* fn, size_t lds) {
     return hipSuccess;
 }
 
-template <typename T>
-hipError_t launch_ulv_factor(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds, hipStream_t st) {
-    if (nwork <= 0) return hipSuccess;
-    const void* fn = reinterpret_cast<const void*>(&k_ulv_factor<T, FACT_CPT>);
+template <typename T, int NT>
+static hipError_t launch_factor_nt(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds,
+                                   hipStream_t st) {
+    const void* fn = reinterpret_cast<const void*>(&k_ulv_factor<T, FACT_CPT, NT>);
     hipError_t e = set_lds_attr<T>(fn, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_factor<T, FACT_CPT>), dim3(nwork), dim3(FACT_THREADS), lds, st, insts, work);
+    hipLaunchKernelGGL((k_ulv_factor<T, FACT_CPT, NT>), dim3(nwork), dim3(NT), lds, st, insts, work);
     return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_ulv_factor(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds, int threads,
+                             hipStream_t st) {
+    if (nwork <= 0) return hipSuccess;
+    if (threads <= 0 || threads > FACT_THREADS) return hipErrorInvalidValue;
+    return (sizeof(T) == 8 && threads <= 768) ? launch_factor_nt<T, 768>(insts, work, nwork, lds, st)
+                                              : launch_factor_nt<T, FACT_THREADS>(insts, work, nwork, lds, st);
 }
 template <typename T>
 hipError_t launch_ulv_wy(const FInst<T>* insts, const WorkItem* work, int nwork, hipStream_t st) {
@@ -2475,8 +2487,8 @@ hipError_t launch_mv(int kind, const MvInst* insts, const MvRec* work, int nwork
 }
 
 // explicit instantiations
-template hipError_t launch_ulv_factor<double>(const FInst<double>*, const WorkItem*, int, size_t, hipStream_t);
-template hipError_t launch_ulv_factor<float>(const FInst<float>*, const WorkItem*, int, size_t, hipStream_t);
+template hipError_t launch_ulv_factor<double>(const FInst<double>*, const WorkItem*, int, size_t, int, hipStream_t);
+template hipError_t launch_ulv_factor<float>(const FInst<float>*, const WorkItem*, int, size_t, int, hipStream_t);
 template hipError_t launch_ulv_wy<double>(const FInst<double>*, const WorkItem*, int, hipStream_t);
 template hipError_t launch_ulv_wy<float>(const FInst<float>*, const WorkItem*, int, hipStream_t);
 template hipError_t launch_ulv_root<double>(const FInst<double>*, const WorkItem*, int, hipStream_t);

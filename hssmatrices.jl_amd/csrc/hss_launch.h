@@ -24,8 +24,10 @@ inline int resident_tw(int nwork, int max_nrhs) {
 hipError_t launch_mv(int kind, const MvInst* insts, const MvRec* work, int nwork, int max_m, int max_nrhs, int tw,
                      size_t lds, int resident, hipStream_t st);
 
+// threads = the most any node of the launch needs ((m + kv) * S, at most FACT_THREADS)
 template <typename T>
-hipError_t launch_ulv_factor(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds, hipStream_t st);
+hipError_t launch_ulv_factor(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds, int threads,
+                             hipStream_t st);
 template <typename T>
 hipError_t launch_ulv_wy(const FInst<T>* insts, const WorkItem* work, int nwork, hipStream_t st);
 template <typename T>

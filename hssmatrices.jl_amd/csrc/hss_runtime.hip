@@ -1464,6 +1464,7 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
     std::vector<WorkItem> work;
     std::vector<Range> lev(maxh);
     std::vector<size_t> lev_lds(maxh, 0);
+    std::vector<int> lev_thr(maxh, 1);
     for (int h = 0; h < maxh; ++h) {
         lev[h].off = (int)work.size();
         for (int b = 0; b < cnt; ++b) {
@@ -1471,7 +1472,9 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
             for (int i = 1; i < M->nn; ++i)
                 if (M->hn[i].height == h) {
                     work.push_back(WorkItem{b, i});
-                    lev_lds[h] = std::max(lev_lds[h], factor_lds<TF>(out[b]->hfn[i], M->hn[i]));
+                    const FNode& fi = out[b]->hfn[i];
+                    lev_lds[h] = std::max(lev_lds[h], factor_lds<TF>(fi, M->hn[i]));
+                    lev_thr[h] = std::max(lev_thr[h], (fi.m + M->hn[i].kv) * fi.S);
                 }
         }
         lev[h].count = (int)work.size() - lev[h].off;
@@ -1504,7 +1507,7 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
         HIP_TRY(hipEventRecord(ctx->ev0, st));
         for (int h = 0; h < maxh; ++h) {
             ProfScope ps("k_ulv_factor", h, lev[h].count, st);
-            HIP_TRY((launch_ulv_factor<TF>(d_inst, d_work + lev[h].off, lev[h].count, lev_lds[h], st)));
+            HIP_TRY((launch_ulv_factor<TF>(d_inst, d_work + lev[h].off, lev[h].count, lev_lds[h], lev_thr[h], st)));
         }
         {
             ProfScope ps("k_ulv_wy", 0, wy_count, st);
