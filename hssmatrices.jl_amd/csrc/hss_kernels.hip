@@ -199,11 +199,17 @@ __device__ __forceinline__ void wg_gemm(int M, int N, int K, const TAe* __restri
     }
 }
 
-// Default tile shapes: 2 x 8 per thread (256 threads cover 128 x 32).
+// Tile shape by the row count, so that short products (the 16-row reflector and
+// substitution blocks) still keep every thread busy: 1 x 2 per thread up to 16 rows, 1 x 4
+// up to 32, 1 x 8 up to 64, else 2 x 8 (256 threads cover 128 x 32).  Every output sums over
+// K in the same order whatever the shape.
 template <bool TA, typename TAe, typename T, typename Store>
 __device__ __forceinline__ void gemm(int M, int N, int K, const TAe* __restrict__ A, int64_t lda, const T* X,
                                      Store store) {
-    wg_gemm<2, 8, TA>(M, N, K, A, lda, X, store);
+    if (M <= 16) wg_gemm<1, 2, TA>(M, N, K, A, lda, X, store);
+    else if (M <= 32) wg_gemm<1, 4, TA>(M, N, K, A, lda, X, store);
+    else if (M <= 64) wg_gemm<1, 8, TA>(M, N, K, A, lda, X, store);
+    else wg_gemm<2, 8, TA>(M, N, K, A, lda, X, store);
 }
 
 // global column-major (ld) -> LDS row-major rows [r0, r0 + rows)
