@@ -531,56 +531,79 @@ static hss_status launch_mv_schedule(const MvInst* di, const MvRec* dw, const Mv
     return step(2, 0, S.leaf, S.leaf_m, S.leaf_na, S.leaf_k);
 }
 
-// Matvec operators of every node, column-major, each starting on a 16-element boundary.
-static void build_mv_operators(hss_mat_s* M, const double* blob, std::vector<double>& op) {
-    auto a16 = [&]() { op.resize((op.size() + 15) & ~size_t(15), 0.0); };
+// One piece of a matvec operator, copied from the generator blob:
+//   op[dst + r + c * rows] = blob[src + r + c * ld]   (blob[src + c + r * ld] when trans)
+// for r < rows, c < cols.
+struct OpPiece {
+    int64_t src, dst;
+    int rows, cols, ld;
+    bool trans;
+};
+
+// Layout of the matvec operators of every node, column-major, each starting on a
+// 16-element boundary (gaps zero): sets DevNode::opZ/opF/opC, appends the pieces that fill
+// it and returns its length.  Leaf opZ = V'; branch opZ = [W_l' W_r']; opF = [B_sibling R];
+// leaf opC = [D U].
+static int64_t plan_mv_operators(hss_mat_s* M, std::vector<OpPiece>& pc) {
+    int64_t len = 0;
+    auto a16 = [&]() { len = (len + 15) & ~int64_t(15); };
+    auto piece = [&](int64_t src, int rows, int cols, int ld, bool trans, int64_t dst) {
+        if (rows > 0 && cols > 0) pc.push_back(OpPiece{src, dst, rows, cols, ld, trans});
+    };
     for (int i = 0; i < M->nn; ++i) {
         DevNode& d = M->hn[i];
         d.opZ = d.opF = d.opC = -1;
         if (i != 0) {
             a16();
-            d.opZ = (int64_t)op.size();
+            d.opZ = len;
             const int kv = d.kv;
             if (d.is_leaf) {
-                op.resize(op.size() + (size_t)kv * d.n);
-                for (int c = 0; c < d.n; ++c)
-                    for (int r = 0; r < kv; ++r) op[d.opZ + r + (int64_t)c * kv] = blob[d.V + c + (int64_t)r * d.n];
+                piece(d.V, kv, d.n, d.n, true, len);
+                len += (int64_t)kv * d.n;
             } else {
                 const DevNode& L = M->hn[d.left];
                 const DevNode& R = M->hn[d.right];
-                op.resize(op.size() + (size_t)kv * (L.kv + R.kv));
-                for (int c = 0; c < L.kv; ++c)
-                    for (int r = 0; r < kv; ++r) op[d.opZ + r + (int64_t)c * kv] = blob[L.W + c + (int64_t)r * L.kv];
-                for (int c = 0; c < R.kv; ++c)
-                    for (int r = 0; r < kv; ++r)
-                        op[d.opZ + r + (int64_t)(L.kv + c) * kv] = blob[R.W + c + (int64_t)r * R.kv];
+                piece(L.W, kv, L.kv, L.kv, true, len);
+                piece(R.W, kv, R.kv, R.kv, true, len + (int64_t)L.kv * kv);
+                len += (int64_t)kv * (L.kv + R.kv);
             }
             a16();
-            d.opF = (int64_t)op.size();
+            d.opF = len;
             const DevNode& p = M->hn[d.parent];
             const DevNode& sb = M->hn[d.is_left ? p.right : p.left];
-            const int64_t boff = d.is_left ? p.B12 : p.B21;
             const int ku = d.ku;
-            op.resize(op.size() + (size_t)ku * (sb.kv + p.ku));
-            for (int c = 0; c < sb.kv; ++c)
-                for (int r = 0; r < ku; ++r) op[d.opF + r + (int64_t)c * ku] = blob[boff + r + (int64_t)c * ku];
-            for (int c = 0; c < p.ku; ++c)
-                for (int r = 0; r < ku; ++r) op[d.opF + r + (int64_t)(sb.kv + c) * ku] = blob[d.R + r + (int64_t)c * ku];
+            piece(d.is_left ? p.B12 : p.B21, ku, sb.kv, ku, false, len);
+            piece(d.R, ku, p.ku, ku, false, len + (int64_t)sb.kv * ku);
+            len += (int64_t)ku * (sb.kv + p.ku);
         }
         if (d.is_leaf) {
             a16();
-            d.opC = (int64_t)op.size();
-            op.insert(op.end(), blob + d.D, blob + d.D + (int64_t)d.m * d.n);
-            if (d.ku > 0) op.insert(op.end(), blob + d.U, blob + d.U + (int64_t)d.m * d.ku);
+            d.opC = len;
+            piece(d.D, d.m, d.n, d.m, false, len);
+            len += (int64_t)d.m * d.n;
+            piece(d.U, d.m, d.ku, d.m, false, len);
+            len += (int64_t)d.m * d.ku;
         }
     }
     a16();
-    op.resize(op.size() + 16, 0.0);  // whole-vector reads past the last operator stay in bounds
+    return len + 16;  // whole-vector reads past the last operator stay in bounds
 }
 
-extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, int32_t nn, const double* blob,
-                                     int64_t blob_len, hss_mat_t* out) {
-    if (!ctx || !nodes || !out || (blob_len > 0 && !blob)) return fail(HSS_ERR_ARG, "null argument");
+static void fill_mv_operators(const std::vector<OpPiece>& pc, const double* blob, double* op) {
+    for (const OpPiece& q : pc)
+        for (int c = 0; c < q.cols; ++c)
+            for (int r = 0; r < q.rows; ++r)
+                op[q.dst + r + (int64_t)c * q.rows] =
+                    q.trans ? blob[q.src + c + (int64_t)r * q.ld] : blob[q.src + r + (int64_t)c * q.ld];
+}
+
+// The matrix of a validated node table whose generators are either on the host (hblob) or
+// already on the device (*dblob, an allocation of dalloc on ctx's device).  On success the
+// matrix owns *dblob and *dblob is set to null; on failure the caller still owns it.  A
+// device blob's operators are gathered on the device (k_gemm_tasks copies), so the
+// generators never cross PCIe.
+static hss_status mat_create_impl(hss_ctx_t ctx, const hss_node_desc* nodes, int32_t nn, const double* hblob,
+                                  double** dblob, int64_t blob_len, hss_mat_t* out) {
     *out = nullptr;
     hss_status vs = validate(nodes, nn, blob_len);
     if (vs != HSS_OK) return vs;
@@ -641,14 +664,13 @@ extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, 
     M->root.off = (int)M->hwork.size();
     M->hwork.push_back(WorkItem{0, 0});
     M->root.count = 1;
-    std::vector<double> op;
-    build_mv_operators(M, blob, op);
-    M->op_len = (int64_t)op.size();
+    std::vector<OpPiece> pieces;
+    M->op_len = plan_mv_operators(M, pieces);
     std::vector<MvRec> mvwork;
     build_mv_schedule(std::vector<hss_mat_s*>{M}, mvwork, M->mvs);
 
     hipError_t e;
-    if ((e = dalloc(&M->d_blob, (size_t)blob_len)) != hipSuccess ||
+    if ((!dblob && (e = dalloc(&M->d_blob, (size_t)blob_len)) != hipSuccess) ||
         (e = dalloc(&M->d_op, (size_t)M->op_len)) != hipSuccess ||
         (e = dalloc(&M->d_mvwork, mvwork.size())) != hipSuccess ||
         (e = dalloc(&M->d_nodes, (size_t)nn)) != hipSuccess ||
@@ -656,18 +678,72 @@ extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, 
         free_mat(M);
         return fail(HSS_ERR_OOM, std::string("hss_mat_create: ") + hipGetErrorString(e));
     }
+    // from here on a failure frees M and, for a device blob, leaves *dblob to the caller
+    auto bail = [&](hipError_t err) {
+        (void)hipStreamSynchronize(ctx->stream);
+        if (dblob) M->d_blob = nullptr;
+        free_mat(M);
+        return fail(HSS_ERR_HIP, std::string("hss_mat_create: ") + hipGetErrorString(err));
+    };
     hipStream_t st = ctx->stream;
-    if (blob_len > 0) HIP_TRY(hipMemcpyAsync(M->d_blob, blob, sizeof(double) * blob_len, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(M->d_op, op.data(), sizeof(double) * op.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(M->d_mvwork, mvwork.data(), sizeof(MvRec) * mvwork.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(M->d_nodes, M->hn.data(), sizeof(DevNode) * nn, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(M->d_work, M->hwork.data(), sizeof(WorkItem) * M->hwork.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    GemmTask* dT = nullptr;
+    if (dblob) {
+        M->d_blob = *dblob;
+        std::vector<GemmTask> tasks;
+        tasks.reserve(pieces.size());
+        for (const OpPiece& q : pieces) {
+            GemmTask t{};
+            t.m = q.rows; t.n = q.cols; t.flags = GT_COPY | (q.trans ? GT_TA : 0);
+            t.lda = std::max(q.ld, 1); t.ldb = 1; t.ldc = q.rows;
+            t.a = M->d_blob + q.src; t.c = M->d_op + q.dst;
+            tasks.push_back(t);
+        }
+        if ((e = hipMemsetAsync(M->d_op, 0, sizeof(double) * (size_t)M->op_len, st)) != hipSuccess) return bail(e);
+        if (!tasks.empty()) {
+            if ((e = dalloc(&dT, tasks.size())) != hipSuccess ||
+                (e = hipMemcpyAsync(dT, tasks.data(), sizeof(GemmTask) * tasks.size(), hipMemcpyHostToDevice, st)) !=
+                    hipSuccess ||
+                (e = launch_gemm_tasks(dT, (int)tasks.size(), st)) != hipSuccess) {
+                (void)hipStreamSynchronize(st);
+                (void)hipFree(dT);
+                return bail(e);
+            }
+        }
+    } else {
+        std::vector<double> op((size_t)M->op_len, 0.0);
+        fill_mv_operators(pieces, hblob, op.data());
+        if ((blob_len > 0 &&
+             (e = hipMemcpyAsync(M->d_blob, hblob, sizeof(double) * blob_len, hipMemcpyHostToDevice, st)) !=
+                 hipSuccess) ||
+            (e = hipMemcpyAsync(M->d_op, op.data(), sizeof(double) * op.size(), hipMemcpyHostToDevice, st)) !=
+                hipSuccess ||
+            (e = hipStreamSynchronize(st)) != hipSuccess)
+            return bail(e);
+    }
+    if ((e = hipMemcpyAsync(M->d_mvwork, mvwork.data(), sizeof(MvRec) * mvwork.size(), hipMemcpyHostToDevice, st)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(M->d_nodes, M->hn.data(), sizeof(DevNode) * nn, hipMemcpyHostToDevice, st)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(M->d_work, M->hwork.data(), sizeof(WorkItem) * M->hwork.size(), hipMemcpyHostToDevice,
+                            st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(dT);
+        return bail(e);
+    }
+    (void)hipFree(dT);
+    if (dblob) *dblob = nullptr;
     M->dev_bytes = (int64_t)(sizeof(double) * (blob_len + M->op_len) + sizeof(DevNode) * nn +
                              sizeof(WorkItem) * M->hwork.size() + sizeof(MvRec) * mvwork.size());
     ctx->refs++;
     *out = M;
     return HSS_OK;
+}
+
+extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, int32_t nn, const double* blob,
+                                     int64_t blob_len, hss_mat_t* out) {
+    if (!ctx || !nodes || !out || (blob_len > 0 && !blob)) return fail(HSS_ERR_ARG, "null argument");
+    return mat_create_impl(ctx, nodes, nn, blob, nullptr, blob_len, out);
 }
 
 static void mat_release(hss_mat_s* M) {
@@ -681,29 +757,32 @@ static void mat_release(hss_mat_s* M) {
 // Adjoint of an HSS matrix as a new matrix on the same context (reference
 // src/hssmatrix.jl:165-171): leaf D -> D', U <-> V; branch B12 -> B21', B21 -> B12';
 // every node's R <-> W; rows and columns exchange sizes and offsets.  The generators are
-// rebuilt on the host from the device copy and packed in the same node order.
+// copied (and transposed) on the device into a new blob packed in the same node order.
 extern "C" hss_status hss_mat_adjoint(hss_mat_t A, hss_mat_t* out) {
     if (!A || !out) return fail(HSS_ERR_ARG, "null argument");
     *out = nullptr;
     HIP_TRY(hipSetDevice(A->ctx->device));
-    std::vector<double> src((size_t)std::max<int64_t>(A->blob_len, 0));
-    if (A->blob_len > 0)
-        HIP_TRY(hipMemcpy(src.data(), A->d_blob, sizeof(double) * (size_t)A->blob_len, hipMemcpyDeviceToHost));
     const int nn = A->nn;
     const std::vector<hss_node_desc>& d = A->desc;
     std::vector<hss_node_desc> t(d);
-    std::vector<double> blob;
-    blob.reserve(src.size());
-    // copy of a rows x cols block (transposed when trans), or -1 for an absent generator
+    std::vector<GemmTask> tasks;
+    std::vector<int64_t> dst;  // each task's offset in the new blob
+    int64_t len = 0;
+    // a rows x cols block of A's blob, copied (or transposed into cols x rows) to the end
+    // of the new blob; -1 for an absent generator
     auto put = [&](int64_t off, int64_t rows, int64_t cols, bool trans) -> int64_t {
         if (off < 0) return -1;
-        const int64_t at = (int64_t)blob.size();
-        if (!trans) {
-            blob.insert(blob.end(), src.begin() + off, src.begin() + off + rows * cols);
-        } else {  // (cols x rows) = block'
-            blob.resize(blob.size() + (size_t)(rows * cols));
-            for (int64_t j = 0; j < rows; ++j)
-                for (int64_t i = 0; i < cols; ++i) blob[at + i + j * cols] = src[off + j + i * rows];
+        const int64_t at = len;
+        len += rows * cols;
+        if (rows > 0 && cols > 0) {
+            GemmTask g{};
+            g.m = (int)(trans ? cols : rows);
+            g.n = (int)(trans ? rows : cols);
+            g.flags = GT_COPY | (trans ? GT_TA : 0);
+            g.lda = (int)rows; g.ldb = 1; g.ldc = g.m;
+            g.a = A->d_blob + off;
+            tasks.push_back(g);
+            dst.push_back(at);
         }
         return at;
     };
@@ -732,7 +811,28 @@ extern "C" hss_status hss_mat_adjoint(hss_mat_t A, hss_mat_t* out) {
             y.off_W = put(x.off_R, x.ku, p.ku, false);
         }
     }
-    return hss_mat_create(A->ctx, t.data(), nn, blob.data(), (int64_t)blob.size(), out);
+    hipStream_t st = A->ctx->stream;
+    double* dB = nullptr;
+    GemmTask* dT = nullptr;
+    struct Free {
+        double** b;
+        GemmTask** t;
+        ~Free() {
+            (void)hipFree(*b);
+            (void)hipFree(*t);
+        }
+    } guard{&dB, &dT};
+    HIP_TRY(dalloc(&dB, (size_t)std::max<int64_t>(len, 1)));
+    for (size_t k = 0; k < tasks.size(); ++k) tasks[k].c = dB + dst[k];
+    HIP_TRY(hipMemsetAsync(dB, 0, sizeof(double) * (size_t)std::max<int64_t>(len, 1), st));
+    if (!tasks.empty()) {
+        HIP_TRY(dalloc(&dT, tasks.size()));
+        HIP_TRY(hipMemcpyAsync(dT, tasks.data(), sizeof(GemmTask) * tasks.size(), hipMemcpyHostToDevice, st));
+        HIP_TRY(launch_gemm_tasks(dT, (int)tasks.size(), st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    // the new matrix takes dB over (the guard then frees null)
+    return mat_create_impl(A->ctx, t.data(), nn, nullptr, &dB, len, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -843,7 +943,7 @@ extern "C" hss_status hss_mat_save(hss_mat_t A, const char* path) {
 //          R_r = [R_A,r, B_A21 Z_l R_B,l; 0, R_B,r],   W_r = [W_A,r, 0; B_B12' Z_l' W_A,l, W_B,r]
 //   leaf:  D = D_A D_B + U_A F V_B',  U = [U_A, D_A U_B],  V = [D_B' V_A, V_B]
 // The products run on the device as ordered phases of small GEMM tasks (k_gemm_tasks)
-// into a zeroed generator blob, which then becomes the matrix C.
+// into a zeroed generator blob, which then becomes the matrix C without leaving the device.
 extern "C" hss_status hss_mat_mul(hss_mat_t A, hss_mat_t B, hss_mat_t* out) {
     if (!A || !B || !out) return fail(HSS_ERR_ARG, "null argument");
     *out = nullptr;
@@ -1081,11 +1181,9 @@ extern "C" hss_status hss_mat_mul(hss_mat_t A, hss_mat_t B, hss_mat_t* out) {
         }
         at += v.size();
     }
-    std::vector<double> blob((size_t)std::max<int64_t>(len, 1));
-    HIP_TRY(hipMemcpyAsync(blob.data(), dC, sizeof(double) * (size_t)std::max<int64_t>(len, 1), hipMemcpyDeviceToHost,
-                           st));
     HIP_TRY(hipStreamSynchronize(st));
-    return hss_mat_create(A->ctx, c.data(), nn, blob.data(), len, out);
+    // C's generators stay on the device: the matrix takes dC over (the guard then frees null)
+    return mat_create_impl(A->ctx, c.data(), nn, nullptr, &dC, len, out);
 }
 
 extern "C" hss_status hss_mat_destroy(hss_mat_t M) {
