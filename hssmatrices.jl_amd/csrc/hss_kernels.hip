@@ -2391,33 +2391,70 @@ hipError_t launch_axpy_batch(const BatchVec* tab, int cnt, int maxc, hipStream_t
 }
 
 // The small products that build an HSS x HSS product's generators (src/matmul.jl:65-118),
-// one workgroup per task; consecutive threads take consecutive rows of c.
+// and the copies that gather a matrix's matvec operators.  Task blockIdx.x runs on
+// gridDim.y workgroups.  A product gives each thread 4 x 4 outputs (consecutive threads take
+// consecutive 4-row groups); every output is one fma chain over l = 0..k-1 in order, so the
+// result does not depend on the tiling.  Loads past the task's rows or columns are clamped
+// to its last row or column and their sums are not stored.
+constexpr int GT_SPLIT = 4;
 __global__ __launch_bounds__(256) void k_gemm_tasks(const GemmTask* __restrict__ tasks) {
     const GemmTask g = tasks[blockIdx.x];
     const bool ta = g.flags & GT_TA, tb = g.flags & GT_TB, acc = g.flags & GT_ACC, cp = g.flags & GT_COPY;
     const gptr<const double> a = gbl(g.a);
     const gptr<const double> b = gbl(g.b);
     const gptr<double> c = gbl(g.c);
-    for (int e = threadIdx.x; e < g.m * g.n; e += blockDim.x) {
-        const int i = e % g.m, j = e / g.m;
-        double s = 0.0;
-        if (cp) {
-            s = ta ? a[j + (int64_t)i * g.lda] : a[i + (int64_t)j * g.lda];
-        } else {
-            for (int l = 0; l < g.k; ++l) {
-                const double av = ta ? a[l + (int64_t)i * g.lda] : a[i + (int64_t)l * g.lda];
-                const double bv = tb ? b[j + (int64_t)l * g.ldb] : b[l + (int64_t)j * g.ldb];
-                s = fma(av, bv, s);
+    const int t0 = blockIdx.y * blockDim.x + threadIdx.x, ts = gridDim.y * blockDim.x;
+    if (cp) {
+        for (int e = t0; e < g.m * g.n; e += ts) {
+            const int i = e % g.m, j = e / g.m;
+            const double s = ta ? a[j + (int64_t)i * g.lda] : a[i + (int64_t)j * g.lda];
+            const int64_t o = i + (int64_t)j * g.ldc;
+            c[o] = acc ? c[o] + s : s;
+        }
+        return;
+    }
+    const int mb = (g.m + 3) / 4, nb = (g.n + 3) / 4;
+    for (int e = t0; e < mb * nb; e += ts) {
+        const int i0 = (e % mb) * 4, j0 = (e / mb) * 4;
+        int ii[4], jj[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            ii[r] = min(i0 + r, g.m - 1);
+            jj[r] = min(j0 + r, g.n - 1);
+        }
+        double s[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s[r][q] = 0.0;
+        for (int l = 0; l < g.k; ++l) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                av[r] = ta ? a[l + (int64_t)ii[r] * g.lda] : a[ii[r] + (int64_t)l * g.lda];
+                bv[r] = tb ? b[jj[r] + (int64_t)l * g.ldb] : b[l + (int64_t)jj[r] * g.ldb];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) s[r][q] = fma(av[r], bv[q], s[r][q]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (j0 + q >= g.n) break;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (i0 + r >= g.m) break;
+                const int64_t o = (i0 + r) + (int64_t)(j0 + q) * g.ldc;
+                c[o] = acc ? c[o] + s[r][q] : s[r][q];
             }
         }
-        const int64_t o = i + (int64_t)j * g.ldc;
-        c[o] = acc ? c[o] + s : s;
     }
 }
 
 hipError_t launch_gemm_tasks(const GemmTask* tasks, int count, hipStream_t st) {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gemm_tasks, dim3((unsigned)count), dim3(256), 0, st, tasks);
+    hipLaunchKernelGGL(k_gemm_tasks, dim3((unsigned)count, GT_SPLIT), dim3(256), 0, st, tasks);
     return hipGetLastError();
 }
 
