@@ -589,19 +589,11 @@ static int64_t plan_mv_operators(hss_mat_s* M, std::vector<OpPiece>& pc) {
     return len + 16;  // whole-vector reads past the last operator stay in bounds
 }
 
-static void fill_mv_operators(const std::vector<OpPiece>& pc, const double* blob, double* op) {
-    for (const OpPiece& q : pc)
-        for (int c = 0; c < q.cols; ++c)
-            for (int r = 0; r < q.rows; ++r)
-                op[q.dst + r + (int64_t)c * q.rows] =
-                    q.trans ? blob[q.src + c + (int64_t)r * q.ld] : blob[q.src + r + (int64_t)c * q.ld];
-}
-
-// The matrix of a validated node table whose generators are either on the host (hblob) or
-// already on the device (*dblob, an allocation of dalloc on ctx's device).  On success the
-// matrix owns *dblob and *dblob is set to null; on failure the caller still owns it.  A
-// device blob's operators are gathered on the device (k_gemm_tasks copies), so the
-// generators never cross PCIe.
+// The matrix of a validated node table whose generators are either on the host (hblob,
+// uploaded once) or already on the device (*dblob, an allocation of dalloc on ctx's device).
+// On success the matrix owns *dblob and *dblob is set to null; on failure the caller still
+// owns it.  The matvec operators are gathered from the device blob by copy tasks
+// (k_gemm_tasks), so only the generators themselves cross PCIe, and a device blob not at all.
 static hss_status mat_create_impl(hss_ctx_t ctx, const hss_node_desc* nodes, int32_t nn, const double* hblob,
                                   double** dblob, int64_t blob_len, hss_mat_t* out) {
     *out = nullptr;
@@ -687,8 +679,11 @@ static hss_status mat_create_impl(hss_ctx_t ctx, const hss_node_desc* nodes, int
     };
     hipStream_t st = ctx->stream;
     GemmTask* dT = nullptr;
-    if (dblob) {
-        M->d_blob = *dblob;
+    if (dblob) M->d_blob = *dblob;
+    else if (blob_len > 0 &&
+             (e = hipMemcpyAsync(M->d_blob, hblob, sizeof(double) * blob_len, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return bail(e);
+    {
         std::vector<GemmTask> tasks;
         tasks.reserve(pieces.size());
         for (const OpPiece& q : pieces) {
@@ -709,16 +704,6 @@ static hss_status mat_create_impl(hss_ctx_t ctx, const hss_node_desc* nodes, int
                 return bail(e);
             }
         }
-    } else {
-        std::vector<double> op((size_t)M->op_len, 0.0);
-        fill_mv_operators(pieces, hblob, op.data());
-        if ((blob_len > 0 &&
-             (e = hipMemcpyAsync(M->d_blob, hblob, sizeof(double) * blob_len, hipMemcpyHostToDevice, st)) !=
-                 hipSuccess) ||
-            (e = hipMemcpyAsync(M->d_op, op.data(), sizeof(double) * op.size(), hipMemcpyHostToDevice, st)) !=
-                hipSuccess ||
-            (e = hipStreamSynchronize(st)) != hipSuccess)
-            return bail(e);
     }
     if ((e = hipMemcpyAsync(M->d_mvwork, mvwork.data(), sizeof(MvRec) * mvwork.size(), hipMemcpyHostToDevice, st)) !=
             hipSuccess ||
