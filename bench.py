@@ -69,7 +69,7 @@ def load_generators(pkg):
 
 
 # ---------------------------------------------------------------------------
-# algorithmic byte / flop models (per launch), see DESIGN.md
+# algorithmic byte / flop models (per launch): every operator, input and output counted once
 # ---------------------------------------------------------------------------
 def matvec_leaf_bytes(nodes, nrhs, beta_nonzero=False):
     """k_mv_leaf*: D, U, B_seg, F read once, C written once (8 B each)."""
@@ -430,7 +430,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              f"the sharding design is described in DESIGN.md", file=sys.stderr)
+              f"a run over several GPUs would shard the tree by subtrees, one per GPU", file=sys.stderr)
         sys.exit(2)
 
     import __graft_entry__ as G

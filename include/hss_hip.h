@@ -5,7 +5,7 @@
  * Drop-in boundary for bonevbs/HssMatrices.jl 0.1.6.  The reference has no
  * FFI: its boundary is Julia dispatch on HssMatrix{T}.  Each entry point below
  * replaces one reference method; the Julia binding a maintainer would add is
- * in INTEGRATION.md (and hssmatrices.jl_amd/julia/HssHip.jl).
+ * hssmatrices.jl_amd/julia/HssHip.jl.
  *
  *   hss_matmul / hss_matmul_dev   mul!(C, hssA, B, alpha, beta)  src/matmul.jl:18
  *                                 *(hssA, B) / *(hssA, x)        src/matmul.jl:13,15

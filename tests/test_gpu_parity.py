@@ -3,8 +3,8 @@
 Tolerances (Float64):
   * matvec: relative difference <= 1e-14 (a sum of the same products in a
     different order);
-  * ULV solve: relative difference <= 1e-12 on config C1 (the north star's
-    bound; cond(A) ~ 3.1e3) and <= 50 * cond(A) * eps on the golden fixtures
+  * ULV solve: relative difference <= 1e-12 on config C1 (n = 2001 Cauchy
+    kernel, leaf 64; cond(A) ~ 3.1e3) and <= 50 * cond(A) * eps on the golden fixtures
     (two backward-stable solvers differ by at most ~cond * eps);
   * mixed precision: refinement must reach the requested residual 1e-10 and
     the result is compared with the Float64 oracle.
