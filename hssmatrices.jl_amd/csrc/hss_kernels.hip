@@ -23,6 +23,7 @@
 //
 // Every kernel takes a list of (instance, node) work items, so one launch can
 // cover the same tree level of many HSS matrices.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -32,6 +33,19 @@
 #include "hss_launch.h"
 
 namespace hss {
+
+// Every kernel of this file is launched here.  Inside a timed scope (g_prof_launch set) the
+// scope's first launch also records the scope's kernel events at its own start and end.
+template <typename... KArgs, typename... Args>
+static inline void klaunch(void (*k)(KArgs...), dim3 grid, dim3 block, size_t lds, hipStream_t st, Args... args) {
+    static_assert(sizeof...(KArgs) == sizeof...(Args), "kernel argument count");
+    ProfLaunch* p = g_prof_launch;
+    if (p && p->launches++ == 0) {
+        hipExtLaunchKernelGGL(k, grid, block, (std::uint32_t)lds, st, p->k0, p->k1, 0u, static_cast<KArgs>(args)...);
+        return;
+    }
+    hipLaunchKernelGGL(k, grid, block, lds, st, static_cast<KArgs>(args)...);
+}
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -2198,7 +2212,7 @@ static hipError_t launch_factor_nt(const FInst<T>* insts, const WorkItem* work, 
     const void* fn = reinterpret_cast<const void*>(&k_ulv_factor<T, FACT_CPT, NT>);
     hipError_t e = set_lds_attr<T>(fn, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_factor<T, FACT_CPT, NT>), dim3(nwork), dim3(NT), lds, st, insts, work);
+    klaunch((k_ulv_factor<T, FACT_CPT, NT>), dim3(nwork), dim3(NT), lds, st, insts, work);
     return hipGetLastError();
 }
 template <typename T>
@@ -2212,13 +2226,13 @@ hipError_t launch_ulv_factor(const FInst<T>* insts, const WorkItem* work, int nw
 template <typename T>
 hipError_t launch_ulv_wy(const FInst<T>* insts, const WorkItem* work, int nwork, hipStream_t st) {
     if (nwork <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_ulv_wy<T>), dim3(nwork), dim3(256), 0, st, insts, work);
+    klaunch((k_ulv_wy<T>), dim3(nwork), dim3(256), 0, st, insts, work);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_ulv_root(const FInst<T>* insts, const WorkItem* work, int nwork, hipStream_t st) {
     if (nwork <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_ulv_root<T>), dim3(nwork), dim3(1024), 0, st, insts, work);
+    klaunch((k_ulv_root<T>), dim3(nwork), dim3(1024), 0, st, insts, work);
     return hipGetLastError();
 }
 template <typename T>
@@ -2228,7 +2242,7 @@ hipError_t launch_ulv_gform(const FInst<T>* insts, const WorkItem* work, int nwo
     const void* fn = reinterpret_cast<const void*>(&k_ulv_gform<T>);
     hipError_t e = set_lds_attr<T>(fn, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_gform<T>), dim3(nwork, rhs_tiles(max_kx)), dim3(SWEEP_THREADS), lds, st, insts, work);
+    klaunch((k_ulv_gform<T>), dim3(nwork, rhs_tiles(max_kx)), dim3(SWEEP_THREADS), lds, st, insts, work);
     return hipGetLastError();
 }
 template <typename T>
@@ -2238,7 +2252,7 @@ hipError_t launch_ulv_qform(const FInst<T>* insts, const WorkItem* work, int nwo
     const void* fn = reinterpret_cast<const void*>(&k_ulv_qform<T>);
     hipError_t e = set_lds_attr<T>(fn, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_qform<T>), dim3(nwork, rhs_tiles(max_n)), dim3(SWEEP_THREADS), lds, st, insts, work);
+    klaunch((k_ulv_qform<T>), dim3(nwork, rhs_tiles(max_n)), dim3(SWEEP_THREADS), lds, st, insts, work);
     return hipGetLastError();
 }
 template <typename T>
@@ -2248,7 +2262,7 @@ hipError_t launch_ulv_rinv(const FInst<T>* insts, const WorkItem* work, int nwor
     const void* fn = reinterpret_cast<const void*>(&k_ulv_rinv<T>);
     hipError_t e = set_lds_attr<T>(fn, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_rinv<T>), dim3(nwork, rhs_tiles(max_cols)), dim3(SWEEP_THREADS), lds, st, insts, work);
+    klaunch((k_ulv_rinv<T>), dim3(nwork, rhs_tiles(max_cols)), dim3(SWEEP_THREADS), lds, st, insts, work);
     return hipGetLastError();
 }
 template <typename T>
@@ -2261,8 +2275,8 @@ hipError_t launch_ulv_groot2(const SInst<T>* insts, const SweepRec* work, int nw
                             : reinterpret_cast<const void*>(&k_ulv_groot2<T, false>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    if (staged) hipLaunchKernelGGL((k_ulv_groot2<T, true>), dim3(nwork * tn), dim3(512), lds, st, insts, work, tn);
-    else hipLaunchKernelGGL((k_ulv_groot2<T, false>), dim3(nwork * tn), dim3(512), lds, st, insts, work, tn);
+    if (staged) klaunch((k_ulv_groot2<T, true>), dim3(nwork * tn), dim3(512), lds, st, insts, work, tn);
+    else klaunch((k_ulv_groot2<T, false>), dim3(nwork * tn), dim3(512), lds, st, insts, work, tn);
     return hipGetLastError();
 }
 template <typename T>
@@ -2273,7 +2287,7 @@ hipError_t launch_ulv_groot(const SInst<T>* insts, const SweepRec* work, int nwo
     const void* fn = reinterpret_cast<const void*>(&k_ulv_groot<T>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_groot<T>), dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st, insts,
+    klaunch((k_ulv_groot<T>), dim3(nwork, rhs_tiles(max_nrhs)), dim3(SWEEP_THREADS), lds, st, insts,
                        work, resident);
     return hipGetLastError();
 }
@@ -2302,7 +2316,7 @@ hipError_t launch_ulv_gup(const SInst<T>* insts, const SweepRec* work, int nwork
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gup<T>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_gup<T>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), lds, st, insts, work, nwork,
+    klaunch((k_ulv_gup<T>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), lds, st, insts, work, nwork,
                        tn);
     return hipGetLastError();
 }
@@ -2314,7 +2328,7 @@ static hipError_t launch_gup_s_nt(const SInst<T>* insts, const SweepRec* work, i
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gup_s<T, AR, NT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_gup_s<T, AR, NT>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(NT), lds, st, insts,
+    klaunch((k_ulv_gup_s<T, AR, NT>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(NT), lds, st, insts,
                        work, nwork, tn, tw);
     return hipGetLastError();
 }
@@ -2342,7 +2356,7 @@ static hipError_t launch_gdown_s_nt(const SInst<T>* insts, const SweepRec* work,
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ulv_gdown_s<T, AR, NT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ulv_gdown_s<T, AR, NT>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(NT), lds, st, insts,
+    klaunch((k_ulv_gdown_s<T, AR, NT>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(NT), lds, st, insts,
                        work, nwork, tn, tw);
     return hipGetLastError();
 }
@@ -2371,22 +2385,22 @@ hipError_t launch_ulv_gdown(const SInst<T>* insts, const SweepRec* work, int nwo
     int tm, tn;
     if (max_nrhs > 32) {
         if (hipError_t e = sweep_grid<T>(nwork, max_m, max_nrhs, 64, 64, grid, tm, tn)) return e;
-        hipLaunchKernelGGL((k_ulv_gdown<T, 2, 2>), grid, dim3(256), 0, st, insts, work, nwork, tm, tn);
+        klaunch((k_ulv_gdown<T, 2, 2>), grid, dim3(256), 0, st, insts, work, nwork, tm, tn);
     } else {
         if (hipError_t e = sweep_grid<T>(nwork, max_m, max_nrhs, 128, 32, grid, tm, tn)) return e;
-        hipLaunchKernelGGL((k_ulv_gdown<T, 4, 1>), grid, dim3(256), 0, st, insts, work, nwork, tm, tn);
+        klaunch((k_ulv_gdown<T, 4, 1>), grid, dim3(256), 0, st, insts, work, nwork, tm, tn);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_resid_batch(const BatchVec* tab, int cnt, int maxc, double* norms, hipStream_t st) {
     if (cnt <= 0 || maxc <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_resid_batch, dim3(cnt, maxc), dim3(256), 0, st, tab, maxc, norms);
+    klaunch(k_resid_batch, dim3(cnt, maxc), dim3(256), 0, st, tab, maxc, norms);
     return hipGetLastError();
 }
 hipError_t launch_axpy_batch(const BatchVec* tab, int cnt, int maxc, hipStream_t st) {
     if (cnt <= 0 || maxc <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_axpy_batch, dim3(cnt, maxc), dim3(256), 0, st, tab);
+    klaunch(k_axpy_batch, dim3(cnt, maxc), dim3(256), 0, st, tab);
     return hipGetLastError();
 }
 
@@ -2454,7 +2468,7 @@ __global__ __launch_bounds__(256) void k_gemm_tasks(const GemmTask* __restrict__
 
 hipError_t launch_gemm_tasks(const GemmTask* tasks, int count, hipStream_t st) {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gemm_tasks, dim3((unsigned)count, GT_SPLIT), dim3(256), 0, st, tasks);
+    klaunch(k_gemm_tasks, dim3((unsigned)count, GT_SPLIT), dim3(256), 0, st, tasks);
     return hipGetLastError();
 }
 
@@ -2463,7 +2477,7 @@ hipError_t launch_residual(const double* B, int64_t ldb, double* R, int64_t ldr,
     int blocks = (int)((total + 255) / 256);
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) return hipSuccess;
-    hipLaunchKernelGGL(k_residual, dim3(blocks), dim3(256), 0, st, B, ldb, R, ldr, n, nrhs);
+    klaunch(k_residual, dim3(blocks), dim3(256), 0, st, B, ldb, R, ldr, n, nrhs);
     return hipGetLastError();
 }
 hipError_t launch_axpy(double* X, int64_t ldx, const double* D, int64_t ldd, int64_t n, int nrhs, hipStream_t st) {
@@ -2471,12 +2485,12 @@ hipError_t launch_axpy(double* X, int64_t ldx, const double* D, int64_t ldd, int
     int blocks = (int)((total + 255) / 256);
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) return hipSuccess;
-    hipLaunchKernelGGL(k_axpy, dim3(blocks), dim3(256), 0, st, X, ldx, D, ldd, n, nrhs);
+    klaunch(k_axpy, dim3(blocks), dim3(256), 0, st, X, ldx, D, ldd, n, nrhs);
     return hipGetLastError();
 }
 hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, double* out, hipStream_t st) {
     if (nrhs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_colnorm2, dim3(nrhs), dim3(256), 0, st, A, lda, n, out);
+    klaunch(k_colnorm2, dim3(nrhs), dim3(256), 0, st, A, lda, n, out);
     return hipGetLastError();
 }
 
@@ -2487,7 +2501,7 @@ static hipError_t launch_mv_s_nt(const MvInst* insts, const MvRec* work, int nwo
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mv_s<KIND, AR, NT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_mv_s<KIND, AR, NT>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(NT), lds, st, insts,
+    klaunch((k_mv_s<KIND, AR, NT>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(NT), lds, st, insts,
                        work, nwork, tn, tw);
     return hipGetLastError();
 }
@@ -2513,10 +2527,10 @@ static hipError_t launch_mv_kind(const MvInst* insts, const MvRec* work, int nwo
     int tm, tn;
     if (max_nrhs > 32) {
         if (hipError_t e = sweep_grid<double>(nwork, max_m, max_nrhs, 64, 64, grid, tm, tn)) return e;
-        hipLaunchKernelGGL((k_mv_t<KIND, 2, 2>), grid, dim3(256), 0, st, insts, work, nwork, tm, tn);
+        klaunch((k_mv_t<KIND, 2, 2>), grid, dim3(256), 0, st, insts, work, nwork, tm, tn);
     } else {
         if (hipError_t e = sweep_grid<double>(nwork, max_m, max_nrhs, 128, 32, grid, tm, tn)) return e;
-        hipLaunchKernelGGL((k_mv_t<KIND, 4, 1>), grid, dim3(256), 0, st, insts, work, nwork, tm, tn);
+        klaunch((k_mv_t<KIND, 4, 1>), grid, dim3(256), 0, st, insts, work, nwork, tm, tn);
     }
     return hipGetLastError();
 }

@@ -6,6 +6,15 @@
 
 namespace hss {
 
+// Profiling hook.  While the runtime times a scope, g_prof_launch points at the scope's
+// pair of kernel events: the scope's first launch records them at its own start and end
+// (hipExtLaunchKernelGGL), and every launch in the scope counts itself.
+struct ProfLaunch {
+    hipEvent_t k0 = nullptr, k1 = nullptr;
+    int launches = 0;
+};
+extern ProfLaunch* g_prof_launch;
+
 // Column width of the LDS-resident steps (16 or 32): 16 when a level has too few 32-column
 // tiles to occupy the CUs (halves each workgroup's MFMA chain) or at most 16 columns.
 // A level is wide when it has at least 2048 (node, 32-column) tiles, eight per CU: its

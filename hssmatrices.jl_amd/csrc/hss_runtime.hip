@@ -49,14 +49,21 @@ static hss_status fail(hss_status st, const std::string& msg) {
     } while (0)
 
 // ---------------------------------------------------------------------------
-// optional per-launch timing (hss_prof_*): a HIP event pair around every
-// kernel launch on the library stream, resolved when the caller asks.
+// optional per-launch timing (hss_prof_*).  Each timed scope records HIP events on the
+// library stream around itself (e0, e1) and hands the kernel launcher a second pair (k0,
+// k1) that its first launch records at the kernel's own start and end.  A scope with
+// exactly one launch reports the kernel's duration; any other scope the time between its
+// two events.  Resolved when the caller asks.
 // ---------------------------------------------------------------------------
+namespace hss {
+ProfLaunch* g_prof_launch = nullptr;
+}
 struct ProfRec {
     const char* name;
     int level;
     int nwork;
-    hipEvent_t e0, e1;
+    hipEvent_t e0, e1, k0, k1;
+    int launches;
     float ms;
 };
 static bool g_prof = false;
@@ -77,15 +84,24 @@ static hipEvent_t prof_event() {
 struct ProfScope {
     int idx = -1;
     hipStream_t st;
+    hss::ProfLaunch pl;
+    hss::ProfLaunch* prev = nullptr;
     ProfScope(const char* name, int level, int nwork, hipStream_t s) : st(s) {
         if (!g_prof || nwork <= 0) return;
-        ProfRec r{name, level, nwork, prof_event(), prof_event(), -1.f};
+        ProfRec r{name, level, nwork, prof_event(), prof_event(), prof_event(), prof_event(), 0, -1.f};
         (void)hipEventRecord(r.e0, st);
         g_prof_recs.push_back(r);
         idx = (int)g_prof_recs.size() - 1;
+        pl.k0 = r.k0;
+        pl.k1 = r.k1;
+        prev = hss::g_prof_launch;
+        hss::g_prof_launch = &pl;
     }
     ~ProfScope() {
-        if (idx >= 0) (void)hipEventRecord(g_prof_recs[idx].e1, st);
+        if (idx < 0) return;
+        hss::g_prof_launch = prev;
+        (void)hipEventRecord(g_prof_recs[idx].e1, st);
+        g_prof_recs[idx].launches = pl.launches;
     }
 };
 
@@ -2994,10 +3010,8 @@ extern "C" const char* hss_last_error(void) { return g_err.c_str(); }
 extern "C" void hss_prof_enable(int on) { g_prof = on != 0; }
 
 extern "C" void hss_prof_reset(void) {
-    for (auto& r : g_prof_recs) {
-        g_prof_pool.push_back(r.e0);
-        g_prof_pool.push_back(r.e1);
-    }
+    for (auto& r : g_prof_recs)
+        for (hipEvent_t e : {r.e0, r.e1, r.k0, r.k1}) g_prof_pool.push_back(e);
     g_prof_recs.clear();
 }
 
@@ -3007,7 +3021,8 @@ extern "C" int32_t hss_prof_count(void) {
         if (r.ms < 0.f) {
             (void)hipEventSynchronize(r.e1);
             float ms = 0.f;
-            if (hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) r.ms = ms;
+            if (r.launches == 1 && hipEventElapsedTime(&ms, r.k0, r.k1) == hipSuccess) r.ms = ms;
+            else if (hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) r.ms = ms;
         }
     return (int32_t)g_prof_recs.size();
 }

@@ -242,8 +242,11 @@ hss_status hss_ulv_factor_batch(int32_t count, const hss_mat_t* mats, hss_prec p
 const char* hss_last_error(void);
 /* Per-launch timing of the library's kernels (no counterpart in the reference;
  * for profiling and the roofline report): when enabled, every kernel launch is
- * bracketed by HIP events on the context stream.  hss_prof_count() waits for
- * the recorded launches and returns how many there are. */
+ * one record.  Its time is the kernel's own duration, from the start and end
+ * events the launch records itself (hipExtLaunchKernelGGL); a record that
+ * covers several launches is timed by HIP events recorded around them on the
+ * context stream.  hss_prof_count() waits for the recorded launches and
+ * returns how many there are. */
 void hss_prof_enable(int on);
 void hss_prof_reset(void);
 int32_t hss_prof_count(void);
