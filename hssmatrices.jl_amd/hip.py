@@ -175,7 +175,8 @@ def last_device_ms() -> float:
 
 
 def prof_enable(on: bool = True):
-    """Bracket every library kernel launch with HIP events (``hss_prof_enable``)."""
+    """Time every library kernel launch (``hss_prof_enable``): one record per launch, timed by
+    the kernel's own start and end events."""
     load_library().hss_prof_enable(1 if on else 0)
 
 

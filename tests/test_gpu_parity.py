@@ -669,3 +669,35 @@ def test_server_alone_equals_mixed(hip, oracle):
     with pytest.raises(hip.HssError):
         small.admit(mats[0])
     srv.close()
+
+
+def test_launch_profiling_records(hip, c1_dev):
+    """hss_prof_*: a profiled solve gives the unprofiled solve's result bit for bit; every
+    sweep level is one record with a positive kernel time; the records of one solve add up
+    to less than the solve's wall time (each is a kernel's own duration)."""
+    import time
+    F = c1_dev.ulvfactor("fp64")
+    rng = np.random.default_rng(21)
+    B = rng.standard_normal((2001, 32))
+    Z = F.solve(B)
+    hip.prof_reset()
+    hip.prof_enable(True)
+    try:
+        t0 = time.perf_counter()
+        Zp = F.solve(B)
+        wall_ms = (time.perf_counter() - t0) * 1e3
+        recs = hip.prof_records()
+    finally:
+        hip.prof_enable(False)
+        hip.prof_reset()
+    assert np.array_equal(Zp, Z)
+    assert recs, "no launch recorded"
+    assert all(r["ms"] > 0 and r["nwork"] > 0 for r in recs)
+    up = [r for r in recs if r["kernel"].startswith("k_ulv_gup")]
+    down = [r for r in recs if r["kernel"].startswith("k_ulv_gdown")]
+    roots = [r for r in recs if r["kernel"].startswith("k_ulv_groot")]
+    assert len(roots) == 1 and len(up) >= 1 and len(down) >= 1
+    # one record per up-sweep height (0, 1, ...) and per down-sweep depth
+    assert sorted(r["level"] for r in up) == list(range(len(up)))
+    assert len({r["level"] for r in down}) == len(down)
+    assert sum(r["ms"] for r in recs) < wall_ms
