@@ -2411,6 +2411,67 @@ hipError_t launch_axpy_batch(const BatchVec* tab, int cnt, int maxc, hipStream_t
 // result does not depend on the tiling.  Loads past the task's rows or columns are clamped
 // to its last row or column and their sums are not stored.
 constexpr int GT_SPLIT = 4;
+constexpr int GT_TILE = 64;  // output tile of the LDS path (256 threads x 4 x 4)
+constexpr int GT_KC = 32;    // k-chunk staged per step
+
+// A large product task (m, n >= 64, k >= 32): the task's 64 x 64 output tiles go round robin
+// over its gridDim.y workgroups; each stages 64 x 32 chunks of op(a) and op(b) in LDS and
+// gives each thread 4 x 4 outputs (rows tx*4.., columns ty*4..).  The per-output fma chain
+// runs over l = 0..k-1 in order, as in the register path.  Loads past the task's rows,
+// columns or k are clamped in bounds; a chunk's rows past k are staged but never summed
+// (the inner loop stops at k).
+__device__ __forceinline__ void gemm_task_tiled(const GemmTask& g, bool ta, bool tb, bool acc,
+                                                gptr<const double> a, gptr<const double> b, gptr<double> c) {
+    __shared__ double As[GT_KC][GT_TILE + 1];  // As[l][i] = op(a)(i0 + i, k0 + l)
+    __shared__ double Bs[GT_KC][GT_TILE + 1];  // Bs[l][j] = op(b)(k0 + l, j0 + j)
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int mt = (g.m + GT_TILE - 1) / GT_TILE, nt = (g.n + GT_TILE - 1) / GT_TILE;
+    for (int tile = blockIdx.y; tile < mt * nt; tile += gridDim.y) {
+        const int i0 = (tile % mt) * GT_TILE, j0 = (tile / mt) * GT_TILE;
+        double s[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s[r][q] = 0.0;
+        for (int k0 = 0; k0 < g.k; k0 += GT_KC) {
+            // 64 x 32 elements of each operand, 8 per thread
+            for (int e = tid; e < GT_TILE * GT_KC; e += blockDim.x) {
+                const int x = e % GT_TILE, l = e / GT_TILE;  // x: row of op(a) / column of op(b)
+                const int kk = min(k0 + l, g.k - 1);
+                const int ii = min(i0 + x, g.m - 1), jj = min(j0 + x, g.n - 1);
+                As[l][x] = ta ? a[kk + (int64_t)ii * g.lda] : a[ii + (int64_t)kk * g.lda];
+                Bs[l][x] = tb ? b[jj + (int64_t)kk * g.ldb] : b[kk + (int64_t)jj * g.ldb];
+            }
+            __syncthreads();
+            const int kn = min(GT_KC, g.k - k0);
+            for (int l = 0; l < kn; ++l) {
+                double av[4], bv[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    av[r] = As[l][tx * 4 + r];
+                    bv[r] = Bs[l][ty * 4 + r];
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) s[r][q] = fma(av[r], bv[q], s[r][q]);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = j0 + ty * 4 + q;
+            if (j >= g.n) break;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + tx * 4 + r;
+                if (i >= g.m) break;
+                const int64_t o = i + (int64_t)j * g.ldc;
+                c[o] = acc ? c[o] + s[r][q] : s[r][q];
+            }
+        }
+    }
+}
 __global__ __launch_bounds__(256) void k_gemm_tasks(const GemmTask* __restrict__ tasks) {
     const GemmTask g = tasks[blockIdx.x];
     const bool ta = g.flags & GT_TA, tb = g.flags & GT_TB, acc = g.flags & GT_ACC, cp = g.flags & GT_COPY;
@@ -2425,6 +2486,10 @@ __global__ __launch_bounds__(256) void k_gemm_tasks(const GemmTask* __restrict__
             const int64_t o = i + (int64_t)j * g.ldc;
             c[o] = acc ? c[o] + s : s;
         }
+        return;
+    }
+    if (g.m >= GT_TILE && g.n >= GT_TILE && g.k >= GT_KC) {
+        gemm_task_tiled(g, ta, tb, acc, a, b, c);
         return;
     }
     const int mb = (g.m + 3) / 4, nb = (g.n + 3) / 4;

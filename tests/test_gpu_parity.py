@@ -332,6 +332,25 @@ def test_hss_product_rectangular_and_mismatch(hip, tmp_path):
         A @ A
 
 
+def test_hss_product_large_leaves(hip, tmp_path):
+    """Leaf blocks of 64-130 rows and columns: the leaf products D_A D_B take the task
+    kernel's LDS-tiled path (64 x 64 output tiles, ragged edges); generators against the
+    oracle's product, the dense product and the product's matvec."""
+    from oracle import hss_product as P
+    rng = np.random.default_rng(17)
+    ns = [80, 66, 96, 72]
+    a = rect_hss(rng, [70, 90, 65, 100], ns, 3, 5)
+    b = rect_hss(rng, ns, [75, 64, 130, 68], 4, 2)
+    A, B = hip.HipHssMatrix(a), hip.HipHssMatrix(b)
+    C = A @ B
+    assert C.shape == (325, 337)
+    tc = _device_tree(C, tmp_path)
+    assert _tree_rel(tc, P.product(a, b, H)) < 1e-12
+    assert rel(H.full(tc), H.full(a) @ H.full(b)) < 1e-12
+    X = rng.standard_normal((337, 5))
+    assert rel(C @ X, A @ (B @ X)) < 1e-12
+
+
 def test_adjoint_rectangular(hip, oracle):
     """A rectangular tree (73 x 71, ku = 3, kv = 5): the adjoint exchanges rows and columns,
     sizes and offsets; A' X and X A match the oracle and the dense transpose."""
