@@ -2404,35 +2404,27 @@ hipError_t launch_axpy_batch(const BatchVec* tab, int cnt, int maxc, hipStream_t
     return hipGetLastError();
 }
 
-// The small products that build an HSS x HSS product's generators (src/matmul.jl:65-118),
-// and the copies that gather a matrix's matvec operators.  Task blockIdx.x runs on
-// gridDim.y workgroups.  A product gives each thread 4 x 4 outputs (consecutive threads take
-// consecutive 4-row groups); every output is one fma chain over l = 0..k-1 in order, so the
-// result does not depend on the tiling.  Loads past the task's rows or columns are clamped
-// to its last row or column and their sums are not stored.
 constexpr int GT_SPLIT = 4;
-constexpr int GT_TILE = 64;  // output tile of the LDS path (256 threads x 4 x 4)
+constexpr int GT_TILE = 64;  // output tile of the LDS path (four waves of 16 x 64)
 constexpr int GT_KC = 32;    // k-chunk staged per step
 
-// A large product task (m, n >= 64, k >= 32): the task's 64 x 64 output tiles go round robin
-// over its gridDim.y workgroups; each stages 64 x 32 chunks of op(a) and op(b) in LDS and
-// gives each thread 4 x 4 outputs (rows tx*4.., columns ty*4..).  The per-output fma chain
-// runs over l = 0..k-1 in order, as in the register path.  Loads past the task's rows,
-// columns or k are clamped in bounds; a chunk's rows past k are staged but never summed
-// (the inner loop stops at k).
+// A large product task (m, n >= 64, k >= 32) on FP64 MFMA: the task's 64 x 64 output tiles
+// go round robin over its gridDim.y workgroups; each stages 64 x 32 chunks of op(a) and
+// op(b) in LDS, and wave w accumulates rows [16 w, 16 w + 16) of the tile in four 16 x 16
+// column blocks (16x16x4 steps).  Loads past the task's rows, columns or k are clamped in
+// bounds; a chunk's rows past k are staged but enter the MFMAs as zeros.
 __device__ __forceinline__ void gemm_task_tiled(const GemmTask& g, bool ta, bool tb, bool acc,
                                                 gptr<const double> a, gptr<const double> b, gptr<double> c) {
+    using V4 = Mfma<double>::v4;
     __shared__ double As[GT_KC][GT_TILE + 1];  // As[l][i] = op(a)(i0 + i, k0 + l)
     __shared__ double Bs[GT_KC][GT_TILE + 1];  // Bs[l][j] = op(b)(k0 + l, j0 + j)
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
     const int mt = (g.m + GT_TILE - 1) / GT_TILE, nt = (g.n + GT_TILE - 1) / GT_TILE;
     for (int tile = blockIdx.y; tile < mt * nt; tile += gridDim.y) {
         const int i0 = (tile % mt) * GT_TILE, j0 = (tile / mt) * GT_TILE;
-        double s[4][4];
+        V4 s[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) s[r][q] = 0.0;
+        for (int cb = 0; cb < 4; ++cb) s[cb] = V4{0.0, 0.0, 0.0, 0.0};
         for (int k0 = 0; k0 < g.k; k0 += GT_KC) {
             // 64 x 32 elements of each operand, 8 per thread
             for (int e = tid; e < GT_TILE * GT_KC; e += blockDim.x) {
@@ -2444,34 +2436,37 @@ __device__ __forceinline__ void gemm_task_tiled(const GemmTask& g, bool ta, bool
             }
             __syncthreads();
             const int kn = min(GT_KC, g.k - k0);
-            for (int l = 0; l < kn; ++l) {
-                double av[4], bv[4];
+            for (int st = 0; st < (kn + 3) / 4; ++st) {
+                const int l = 4 * st + lk;  // < GT_KC
+                const bool in = l < kn;
+                const double av = in ? As[l][16 * wave + lr] : 0.0;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    av[r] = As[l][tx * 4 + r];
-                    bv[r] = Bs[l][ty * 4 + r];
-                }
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) s[r][q] = fma(av[r], bv[q], s[r][q]);
+                for (int cb = 0; cb < 4; ++cb) s[cb] = Mfma<double>::mac(av, in ? Bs[l][16 * cb + lr] : 0.0, s[cb]);
             }
             __syncthreads();
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int j = j0 + ty * 4 + q;
-            if (j >= g.n) break;
+        for (int cb = 0; cb < 4; ++cb) {
+            const int j = j0 + 16 * cb + lr;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = i0 + tx * 4 + r;
-                if (i >= g.m) break;
-                const int64_t o = i + (int64_t)j * g.ldc;
-                c[o] = acc ? c[o] + s[r][q] : s[r][q];
+                const int i = i0 + 16 * wave + Mfma<double>::row(lk, r);
+                if (i < g.m && j < g.n) {
+                    const int64_t o = i + (int64_t)j * g.ldc;
+                    c[o] = acc ? c[o] + s[cb][r] : s[cb][r];
+                }
             }
         }
     }
 }
+
+// The small products that build an HSS x HSS product's generators (src/matmul.jl:65-118),
+// and the copies that gather a matrix's matvec operators.  Task blockIdx.x runs on
+// gridDim.y workgroups.  Products of at least 64 rows and columns over at least 32 terms go
+// through gemm_task_tiled (FP64 MFMA); any other gives each thread 4 x 4 outputs
+// (consecutive threads take consecutive 4-row groups), each one fma chain over l = 0..k-1
+// in order.  Loads past the task's rows or columns are clamped to its last row or column and
+// their sums are not stored.
 __global__ __launch_bounds__(256) void k_gemm_tasks(const GemmTask* __restrict__ tasks) {
     const GemmTask g = tasks[blockIdx.x];
     const bool ta = g.flags & GT_TA, tb = g.flags & GT_TB, acc = g.flags & GT_ACC, cp = g.flags & GT_COPY;
