@@ -60,7 +60,8 @@ def load_generators(pkg):
     x = np.sort(np.random.default_rng(0).uniform(-1, 1, N_POINTS))
     K = pkg.KernelMatrix(x, "log", 1.0)
     rcl = pkg.bisection_cluster(N_POINTS, LEAF)
-    h = pkg.compress(K, rcl, rcl, TOL, TOL)
+    h = pkg.compress(K, rcl, rcl, TOL, TOL,
+                     progress=lambda d, n: log(f"[bench] compressed {d}/{n} leaves ({time.time() - t:.0f} s)"))
     nodes, blob = pkg.pack(h)
     DATA.parent.mkdir(exist_ok=True)
     np.savez(DATA, nodes=nodes, blob=blob, x=x)

@@ -242,11 +242,30 @@ def _vcat(Am, pieces, ncols):
     return np.asfortranarray(np.vstack(mats))
 
 
-def compress(A, rcl: ClusterTree, ccl: ClusterTree, atol: float = 1e-9, rtol: float = 1e-9) -> HssMatrix:
+class _Progress:
+    """Counts compressed leaves; calls ``cb(done, total)`` every ``every`` leaves and at the last."""
+
+    def __init__(self, cb, total: int, every: int = 32):
+        self.cb, self.total, self.every, self.done = cb, total, every, 0
+
+    def leaf(self):
+        self.done += 1
+        if self.done % self.every == 0 or self.done == self.total:
+            self.cb(self.done, self.total)
+
+
+def _nleaves(c: ClusterTree) -> int:
+    return 1 if c.isleaf() else _nleaves(c.left) + _nleaves(c.right)
+
+
+def compress(A, rcl: ClusterTree, ccl: ClusterTree, atol: float = 1e-9, rtol: float = 1e-9,
+             progress=None) -> HssMatrix:
     """Direct HSS compression, reference ``compression.jl:47-63``.
 
     ``A`` is a dense array or a ``KernelMatrix``.  Defaults atol = rtol = 1e-9
-    follow ``src/HssMatrices.jl:62-63``.
+    follow ``src/HssMatrices.jl:62-63``.  ``progress(done, total)``, if given, is
+    called every 32 compressed leaves and after the last (long compressions report
+    as they go).
     """
     Am = A if hasattr(A, "block") else _Dense(A)
     if atol < 0:
@@ -260,11 +279,12 @@ def compress(A, rcl: ClusterTree, ccl: ClusterTree, atol: float = 1e-9, rtol: fl
         raise ValueError("size of row- and column-cluster-trees must match")
     if rcl.isleaf():
         return HssMatrix(D=Am.block(rcl.start, rcl.stop, ccl.start, ccl.stop), rootnode=True)
-    h, _, _ = _compress(Am, [], [], m, n, rcl, ccl, atol, rtol, True)
+    prog = _Progress(progress, _nleaves(rcl)) if progress is not None else None
+    h, _, _ = _compress(Am, [], [], m, n, rcl, ccl, atol, rtol, True, prog)
     return h
 
 
-def _compress(Am, Brow, Bcol, nbrow_rows, nbcol_cols, rcl, ccl, atol, rtol, rootnode):
+def _compress(Am, Brow, Bcol, nbrow_rows, nbcol_cols, rcl, ccl, atol, rtol, rootnode, prog=None):
     """Post-order compression; ``Brow``/``Bcol`` are piece lists.
 
     Returns (node, Brow, Bcol) with the compressed block row (k x W) and
@@ -278,6 +298,8 @@ def _compress(Am, Brow, Bcol, nbrow_rows, nbcol_cols, rcl, ccl, atol, rtol, root
         bcol = _vcat(Am, Bcol, n)
         U, brow_c = compress_block(brow, atol, rtol)
         V, bcolT_c = compress_block(np.asfortranarray(bcol.T), atol, rtol)
+        if prog is not None:
+            prog.leaf()
         return HssMatrix(D=D, U=U, V=V), brow_c, np.asfortranarray(bcolT_c.T)
 
     # branch, compression.jl:77-133
@@ -288,11 +310,11 @@ def _compress(Am, Brow, Bcol, nbrow_rows, nbcol_cols, rcl, ccl, atol, rtol, root
 
     Brow1 = [("k", rows1[0], rows1[1], cols2[0], cols2[1])] + _rows(Brow, 0, m1)
     Bcol1 = [("k", rows2[0], rows2[1], cols1[0], cols1[1])] + _cols(Bcol, 0, n1)
-    A11, Brow1c, Bcol1c = _compress(Am, Brow1, Bcol1, m1, n1, rcl.left, ccl.left, atol, rtol, False)
+    A11, Brow1c, Bcol1c = _compress(Am, Brow1, Bcol1, m1, n1, rcl.left, ccl.left, atol, rtol, False, prog)
 
     Brow2 = [Bcol1c[0:m2, :]] + _rows(Brow, m1, m1 + m2)
     Bcol2 = [Brow1c[:, 0:n2]] + _cols(Bcol, n1, n1 + n2)
-    A22, Brow2c, Bcol2c = _compress(Am, Brow2, Bcol2, m2, n2, rcl.right, ccl.right, atol, rtol, False)
+    A22, Brow2c, Bcol2c = _compress(Am, Brow2, Bcol2, m2, n2, rcl.right, ccl.right, atol, rtol, False, prog)
 
     rm1 = Brow1c.shape[0]; rn1 = Bcol1c.shape[1]
     B12 = Bcol2c[0:rm1, :]

@@ -67,6 +67,21 @@ def test_lazy_kernel_equals_dense_compression():
     assert np.allclose(a, b, rtol=0, atol=1e-12)
 
 
+def test_compression_progress_reports():
+    """compress(..., progress=cb): cb(done, total) every 32 leaves and after the last; the
+    generators are those of a compression without it."""
+    rng = np.random.default_rng(4)
+    x = np.sort(rng.uniform(-1, 1, 640))
+    K = H.KernelMatrix(x, "log", 1.0)
+    cl = H.bisection_cluster(640, 8)
+    calls = []
+    h = H.compress(K, cl, cl, 1e-9, 1e-9, progress=lambda d, n: calls.append((d, n)))
+    nleaves = len([c for c in H.pack(h)[0] if c["left"] < 0])
+    assert calls == [(d, nleaves) for d in range(32, nleaves + 1, 32)] + (
+        [] if nleaves % 32 == 0 else [(nleaves, nleaves)])
+    assert np.array_equal(H.full(h), H.full(H.compress(K, cl, cl, 1e-9, 1e-9)))
+
+
 def test_full_matches_hssleaf():
     """_full! (hssmatrix.jl:284-305) and _hssleaf (:338-346) expand to the same matrix."""
     rng = np.random.default_rng(2)
