@@ -30,6 +30,9 @@ using LinearAlgebra
 using HssMatrices
 import Base: *, \, /, adjoint, size
 import LinearAlgebra: mul!
+# ulvfactsolve is HssMatrices' own generic function (exported at src/HssMatrices.jl:44):
+# the methods below extend it for HipHss, so `using HssMatrices, HssHip` sees one function
+import HssMatrices: ulvfactsolve
 
 const libhss = joinpath(@__DIR__, "..", "libhss_hip.so")
 
@@ -206,6 +209,7 @@ end
 *(A::HipHss, B::AbstractMatrix) = mul!(zeros(A.m, size(B, 2)), A, Matrix{Float64}(B), 1.0, 0.0)
 *(A::HipHss, x::AbstractVector) = reshape(A * reshape(x, length(x), 1), length(x))
 
+# a method of HssMatrices.ulvfactsolve (src/ulvfactor.jl:10), not a new function
 function ulvfactsolve(A::HipHss, b::Matrix{Float64})
     A.m == A.n == size(b, 1) || throw(DimensionMismatch("matrix is $(size(A)), right-hand side has $(size(b, 1)) rows"))
     X = zeros(A.n, size(b, 2))
@@ -257,6 +261,7 @@ function \(F::HipUlv, B::Matrix{Float64})
     return X
 end
 
-export HipHss, ulvfactsolve, ulvfactor
+# ulvfactsolve is not exported again: HssMatrices exports it, and the methods above extend it
+export HipHss, ulvfactor
 
 end # module
