@@ -2603,6 +2603,17 @@ hipError_t launch_mv(int kind, const MvInst* insts, const MvRec* work, int nwork
     }
 }
 
+// A solve record written on the device from the kernel's by-value argument: the
+// allocation-free solve places its record in the library's fixed device array without a
+// host-to-device copy (stream-ordered with the sweeps that read it).
+__global__ void k_put_sinst(SInst<double>* dst, SInst<double> v) {
+    if (threadIdx.x == 0) *gbl(dst) = v;
+}
+hipError_t launch_put_sinst(SInst<double>* dst, const SInst<double>& v, hipStream_t st) {
+    klaunch(k_put_sinst, dim3(1), dim3(64), 0, st, dst, v);
+    return hipGetLastError();
+}
+
 // explicit instantiations
 template hipError_t launch_ulv_factor<double>(const FInst<double>*, const WorkItem*, int, size_t, int, hipStream_t);
 template hipError_t launch_ulv_factor<float>(const FInst<float>*, const WorkItem*, int, size_t, int, hipStream_t);

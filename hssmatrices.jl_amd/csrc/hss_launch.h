@@ -77,6 +77,8 @@ template <typename T>
 hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, int tw,
                               size_t lds, int resident, int nt, hipStream_t st);
 
+// writes one solve record to device memory from a kernel argument (no host copy)
+hipError_t launch_put_sinst(SInst<double>* dst, const SInst<double>& v, hipStream_t st);
 // one workgroup per task (HSS x HSS product generators)
 hipError_t launch_gemm_tasks(const GemmTask* tasks, int count, hipStream_t st);
 hipError_t launch_resid_batch(const BatchVec* tab, int cnt, int maxc, double* norms, hipStream_t st);

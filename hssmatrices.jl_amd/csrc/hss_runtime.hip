@@ -319,6 +319,33 @@ struct hss_fact_s {
 };
 
 // ---------------------------------------------------------------------------
+// fixed device workspace of the allocation-free solve (hss_ulv_solve_dev_noalloc).  Each
+// GPU the library is loaded on holds one copy until the process ends.  Host code never
+// uses the array's name as an address: hss_ctx_create resolves the device address once
+// per device (hipGetSymbolAddress, with that device current) and keeps it in
+// g_scratch_ptr, and only that address is passed to kernels.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr size_t kScratchRec = 256;  // the solve record at the start of the array
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// once per device, with that device current
+static hss_status scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(HSS_ERR_ARG, "device index beyond the library's table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return HSS_OK;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(HSS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+    return HSS_OK;
+}
+
+// ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
 extern "C" hss_status hss_ctx_create(int device, hss_ctx_t* out) {
@@ -327,6 +354,8 @@ extern "C" hss_status hss_ctx_create(int device, hss_ctx_t* out) {
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(HSS_ERR_ARG, "device index out of range");
     HIP_TRY(hipSetDevice(device));
+    hss_status ss = scratch_init(device);
+    if (ss != HSS_OK) return ss;
     auto* c = new (std::nothrow) hss_ctx_s;
     if (!c) return fail(HSS_ERR_OOM, "host allocation");
     c->device = device;
@@ -1493,6 +1522,8 @@ static void free_fact(hss_fact_s* F) {
     delete F;
 }
 
+static hss_status ensure_schedule(hss_fact_s* F);
+
 // Factor a batch of matrices that share one context: one launch per height
 // across all of them, then one root launch.
 template <typename TF>
@@ -1663,6 +1694,12 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
         F->d_fac = res;
         F->fac_len = F->res_len;
         F->dev_bytes = (int64_t)(sizeof(TF) * F->fac_len + sizeof(FNode) * F->hfn.size() + 4 * (F->hfn[0].m + 1));
+    }
+    // the single-request solve schedule, built here so that solves (the allocation-free one
+    // included) find it ready
+    for (int b = 0; b < cnt; ++b) {
+        hss_status s2 = ensure_schedule(out[b]);
+        if (s2 != HSS_OK) { cleanup(); return s2; }
     }
     for (auto* F : out) F->mat->refs++;
     return HSS_OK;
@@ -2089,6 +2126,57 @@ extern "C" hss_status hss_ulv_solve_dev(hss_fact_t F, int32_t nrhs, const double
     if (!F) return fail(HSS_ERR_ARG, "null factor");
     HIP_TRY(hipSetDevice(F->mat->ctx->device));
     return solve_dev_impl(F, nrhs, dB, ldb, dX, ldx, opts, rep);
+}
+
+// X = A \ B for a Float64 factor without refinement, allocating nothing: the y/z blocks of
+// the solve and its instance record live in the fixed device array (scratch, 64 MiB per
+// GPU).  The columns run in chunks that fit it; a matrix whose single column needs more
+// workspace than the array holds is refused.  Calls on one GPU share the array: calls that
+// overlap in time (other streams, other threads) race.  Nothing in the array has to start
+// at zero: every sweep writes its outputs before a later sweep reads them.
+extern "C" hss_status hss_ulv_solve_dev_noalloc(hss_fact_t F, int32_t nrhs, const double* dB, int64_t ldb,
+                                                double* dX, int64_t ldx) {
+    if (!F) return fail(HSS_ERR_ARG, "null factor");
+    if (nrhs < 0) return fail(HSS_ERR_ARG, "nrhs < 0");
+    hss_mat_s* M = F->mat;
+    if (F->prec != HSS_PREC_FP64)
+        return fail(HSS_ERR_UNSUPPORTED, "the allocation-free solve takes Float64 factors (a Float32 factor needs "
+                                         "refinement buffers)");
+    if (nrhs == 0 || M->rows == 0) return HSS_OK;
+    if (!dB || !dX) return fail(HSS_ERR_ARG, "null argument");
+    if (ldb < M->rows || ldx < M->rows) return fail(HSS_ERR_DIM, "leading dimension smaller than the matrix size");
+    const int dev = M->ctx->device;
+    HIP_TRY(hipSetDevice(dev));
+    void* base = (dev >= 0 && dev < kMaxDevices) ? g_scratch_ptr[dev] : nullptr;
+    if (!base) return fail(HSS_ERR_ARG, "device workspace not resolved (context not created on this device)");
+    if (!F->d_swork) return fail(HSS_ERR_ARG, "factor has no solve schedule");
+    static_assert(sizeof(SInst<double>) <= kScratchRec, "solve record larger than its slot");
+    const size_t col_bytes = sizeof(double) * (size_t)(F->yrows + F->zrows);
+    const size_t avail = kScratchBytes - kScratchRec;
+    if (col_bytes > avail) {
+        char buf[200];
+        std::snprintf(buf, sizeof buf,
+                      "one right-hand side needs %zu bytes of solve workspace, more than the %zu bytes of the "
+                      "allocation-free solve's device array",
+                      col_bytes, avail);
+        return fail(HSS_ERR_UNSUPPORTED, buf);
+    }
+    const int chunk = (int)std::min<size_t>((size_t)nrhs, avail / std::max<size_t>(col_bytes, 1));
+    hipStream_t st = M->ctx->stream;
+    auto* rec = reinterpret_cast<SInst<double>*>(base);
+    double* ws = reinterpret_cast<double*>(reinterpret_cast<char*>(base) + kScratchRec);
+    for (int j0 = 0; j0 < nrhs; j0 += chunk) {
+        const int c = std::min(chunk, nrhs - j0);
+        // stream order: this record and workspace are overwritten only after the previous
+        // chunk's sweeps, enqueued before, have run
+        SInst<double> I = make_sinst<double>(SolveReq{F, c, dB + (int64_t)j0 * ldb, ldb, dX + (int64_t)j0 * ldx, ldx},
+                                             ws);
+        HIP_TRY(launch_put_sinst(rec, I, st));
+        hss_status s = launch_schedule<double>(rec, F->d_swork, F->sched, c, F->lds_groot, F->lds_groot_res,
+                                               F->lds_groot2, F->lds_groot2s, st);
+        if (s != HSS_OK) return s;
+    }
+    return HSS_OK;
 }
 
 extern "C" hss_status hss_ulv_solve(hss_fact_t F, int32_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx,

@@ -51,7 +51,8 @@ EXPORTS = [
     "hss_gen_write", "hss_gen_read", "hss_mat_load", "hss_mat_save",
     "hss_matmul", "hss_matmul_dev",
     "hss_ulvfactsolve", "hss_ulv_factor", "hss_fact_destroy", "hss_fact_device_bytes",
-    "hss_ulv_solve", "hss_ulv_solve_dev", "hss_ulv_solve_batch_dev", "hss_ulv_factor_batch",
+    "hss_ulv_solve", "hss_ulv_solve_dev", "hss_ulv_solve_dev_noalloc", "hss_ulv_solve_batch_dev",
+    "hss_ulv_factor_batch",
     "hss_last_error", "hss_last_device_ms",
     "hss_prof_enable", "hss_prof_reset", "hss_prof_count", "hss_prof_get", "hss_fact_node_info",
     "hss_plan_create", "hss_plan_execute", "hss_plan_destroy", "hss_plan_device_bytes",
@@ -118,6 +119,7 @@ def load_library(path: pathlib.Path | str | None = None):
                           ctypes.c_int),
         "hss_ulv_solve_dev": ([V, I32, V, I64, V, I64, ctypes.POINTER(RefineOpts), ctypes.POINTER(SolveReport)],
                               ctypes.c_int),
+        "hss_ulv_solve_dev_noalloc": ([V, I32, V, I64, V, I64], ctypes.c_int),
         "hss_ulv_solve_batch_dev": ([I32, V, V, V, V, V, V], ctypes.c_int),
         "hss_ulv_factor_batch": ([I32, V, ctypes.c_int, V], ctypes.c_int),
         "hss_last_error": ([], ctypes.c_char_p),

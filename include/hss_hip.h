@@ -183,6 +183,17 @@ hss_status hss_ulv_solve(hss_fact_t fact, int32_t nrhs, const double* B, int64_t
 hss_status hss_ulv_solve_dev(hss_fact_t fact, int32_t nrhs, const double* dB, int64_t ldb, double* dX, int64_t ldx,
                              const hss_refine_opts* opts, hss_solve_report* report);
 
+/* Allocation-free solve.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  X = A \ B with device pointers on the context stream, for a Float64
+ * factor, without refinement; the same kernels as hss_ulv_solve_dev.  The solve workspace
+ * is a fixed 64 MiB device array of the library (one per GPU, held until the process
+ * ends); columns run in chunks that fit it.  HSS_ERR_UNSUPPORTED for a Float32 factor or a
+ * matrix whose one column needs more workspace than the array holds.  Calls on one GPU share
+ * the array: calls that overlap in time (other streams or threads) race. */
+hss_status hss_ulv_solve_dev_noalloc(hss_fact_t fact, int32_t nrhs, const double* dB, int64_t ldb, double* dX,
+                                     int64_t ldx);
+
 /* ---- batched solves over many (uneven) matrices: one launch per tree level ----
  * Each request i solves fact[i] with nrhs[i] columns, device pointers. */
 hss_status hss_ulv_solve_batch_dev(int32_t count, const hss_fact_t* facts, const int32_t* nrhs,
