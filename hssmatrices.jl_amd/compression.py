@@ -111,6 +111,27 @@ class KernelMatrix:
         n = self.shape[0]
         return self.block(0, n, 0, n)
 
+    def device_dense(self, device="cuda:0"):
+        """The dense matrix in device memory, built there by torch: a tensor T of shape
+        (n, n) with T[j, i] = A[i, j], i.e. A column-major with leading dimension n (the
+        input of ``hip.compress_dev``).  The same entries as ``dense`` up to the rounding of
+        the device's division and logarithm."""
+        import torch
+        x = torch.from_numpy(self.x).to(device)
+        d = x[None, :] - x[:, None]  # d[j, i] = x_i - x_j
+        if self.kind == "cauchy":
+            T = torch.reciprocal(d)
+        elif self.kind == "cauchy_test":
+            T = torch.where(d > 0, 0.001 / d, torch.full_like(d, 2.0))
+        elif self.kind == "log":
+            T = torch.log(torch.abs(d))
+        else:
+            raise ValueError(f"unknown kernel {self.kind}")
+        del d
+        if self.kind != "cauchy_test":
+            T.fill_diagonal_(self.diag)
+        return T
+
 
 _POOL = None
 

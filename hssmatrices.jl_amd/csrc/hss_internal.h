@@ -174,6 +174,18 @@ struct GemmTask {
     double* c;
 };
 
+// One truncated column-pivoted QR of the direct compression (k_rrqr): Y (r x s, ld r) is
+// overwritten; the rank goes to *rank and the first rank columns of Q (ld r, capacity
+// kmax = min(r, s) columns) to Q.  scale multiplies the column norms in the truncation
+// test (1 / sqrt(s) for a Gaussian sample of s columns, 1 for the block itself).
+struct RrqrTask {
+    double* Y;
+    double* Q;
+    int32_t* rank;
+    int32_t r, s, kmax, pad;
+    double scale;
+};
+
 // Per-request vectors of a batched refinement step.
 struct BatchVec {
     const double* b;  // right-hand side (n x nrhs, ldb)

@@ -26,6 +26,8 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #define HSS_KERNELS_TU
@@ -2601,6 +2603,180 @@ hipError_t launch_mv(int kind, const MvInst* insts, const MvRec* work, int nwork
         case 1: return launch_mv_kind<1>(insts, work, nwork, max_m, max_nrhs, tw, lds, resident, st);
         default: return launch_mv_kind<2>(insts, work, nwork, max_m, max_nrhs, tw, lds, resident, st);
     }
+}
+
+// ---------------------------------------------------------------------------
+// direct compression (reference src/compression.jl:22-30, 47-133): Gaussian test
+// matrices and the truncated column-pivoted QR of each node's sampled block row
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    // SplitMix64 finalizer (Steele, Lea, Flood 2014)
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// om[i + j * rows] ~ N(0, 1) for i < rows, j < cols: Box-Muller on two uniforms hashed from
+// (seed, i, j), so an entry does not depend on the matrix's size or on the launch
+__global__ __launch_bounds__(256) void k_gauss(double* __restrict__ om, int64_t rows, int cols, uint64_t seed) {
+    const int64_t total = rows * cols;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % rows, j = e / rows;
+        const uint64_t h1 = mix64(seed ^ mix64((uint64_t)i * 0x100000001B3ull + (uint64_t)j));
+        const uint64_t h2 = mix64(h1 ^ 0xD1B54A32D192ED03ull);
+        const double u1 = ((double)(h1 >> 11) + 0.5) * 0x1.0p-53;  // (0, 1)
+        const double u2 = (double)(h2 >> 11) * 0x1.0p-53;          // [0, 1)
+        gbl(om)[e] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    }
+}
+hipError_t launch_gauss(double* om, int64_t rows, int cols, uint64_t seed, hipStream_t st) {
+    const int64_t total = rows * cols;
+    if (total <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((total + 255) / 256, 8192);
+    klaunch(k_gauss, dim3((unsigned)blocks), dim3(256), 0, st, om, rows, cols, seed);
+    return hipGetLastError();
+}
+
+// Truncated column-pivoted Householder QR of one r x s block Y per workgroup (LAPACK
+// dgeqp3 pivoting with dlarfg reflectors), in place in global memory: step i takes the
+// column of largest remaining norm; the factorization stops before step i when that norm
+// (times t.scale) is not above max(atol, rtol * the first pivot's norm (times t.scale)), the
+// truncation rule of the reference's rank-revealing QR (compression.jl:22-30).  The rank k
+// goes to *t.rank and Q's first k columns (orthonormal, r x k, ld r) to t.Q.  The column
+// norms below the current row are recomputed at every step (no downdating), so small
+// trailing norms keep full accuracy.  A NaN norm ends the factorization.
+// LDS: s + kmax + 8 doubles.
+__global__ __launch_bounds__(256) void k_rrqr(const RrqrTask* __restrict__ tasks, double atol, double rtol) {
+    const RrqrTask t = tasks[blockIdx.x];
+    const int r = t.r, s = t.s, kmax = t.kmax;
+    const gptr<double> Y = gbl(t.Y);
+    const gptr<double> Q = gbl(t.Q);
+    extern __shared__ double lds_d[];
+    double* nrm2 = lds_d;          // s: squared norms of the columns below the current row
+    double* tauv = nrm2 + s;       // kmax
+    double* sc = tauv + kmax;      // scalars
+    int* sci = reinterpret_cast<int*>(sc + 6);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    for (int j = wave; j < s; j += nw) {
+        double part = 0.0;
+        for (int i = lane; i < r; i += 64) {
+            const double v = Y[i + (int64_t)j * r];
+            part += v * v;
+        }
+        part = wave_reduce(part);
+        if (lane == 0) nrm2[j] = part;
+    }
+    __syncthreads();
+    int k = 0;
+    double tol = 0.0;
+    for (int i = 0; i < kmax; ++i) {
+        if (wave == 0) {
+            double best = -1.0;
+            int bj = s;
+            for (int j = i + lane; j < s; j += 64) {
+                const double v = nrm2[j];
+                if (v > best) { best = v; bj = j; }
+            }
+            for (int off = 32; off >= 1; off >>= 1) {
+                const double ob = shfl_xor(best, off);
+                const int oj = __shfl_xor(bj, off, 64);
+                if (ob > best || (ob == best && oj < bj)) { best = ob; bj = oj; }
+            }
+            if (lane == 0) {
+                sci[0] = bj < s ? bj : i;  // all norms NaN: stays in range, the test below stops
+                sc[0] = best;
+            }
+        }
+        __syncthreads();
+        const int p = sci[0];
+        const double nrm = sqrt(fmax(sc[0], 0.0)) * t.scale;
+        if (i == 0) tol = fmax(atol, rtol * nrm);
+        if (!(nrm > tol)) break;  // uniform across the workgroup (LDS values, same arithmetic)
+        if (p != i) {
+            for (int e = tid; e < r; e += blockDim.x) {
+                const double a = Y[e + (int64_t)i * r];
+                Y[e + (int64_t)i * r] = Y[e + (int64_t)p * r];
+                Y[e + (int64_t)p * r] = a;
+            }
+            if (tid == 0) {
+                const double a = nrm2[i];
+                nrm2[i] = nrm2[p];
+                nrm2[p] = a;
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            double part = 0.0;
+            for (int l = i + 1 + lane; l < r; l += 64) {
+                const double v = Y[l + (int64_t)i * r];
+                part += v * v;
+            }
+            part = wave_reduce(part);
+            if (lane == 0) {
+                double beta, tau, scl;
+                larfg(Y[i + (int64_t)i * r], part, beta, tau, scl);
+                sc[1] = beta;
+                sc[2] = tau;
+                sc[3] = scl;
+                tauv[i] = tau;
+            }
+        }
+        __syncthreads();
+        const double tau = sc[2], scl = sc[3];
+        if (tau != 0.0)
+            for (int l = i + 1 + tid; l < r; l += blockDim.x) Y[l + (int64_t)i * r] *= scl;
+        __syncthreads();
+        if (tid == 0) Y[i + (int64_t)i * r] = sc[1];
+        // H = I - tau v v', v = [1; Y[i+1:r, i]], on the remaining columns; then each
+        // column's squared norm below row i
+        for (int j = i + 1 + wave; j < s; j += nw) {
+            const gptr<double> yj = Y + (int64_t)j * r;
+            double part = 0.0;
+            for (int l = i + 1 + lane; l < r; l += 64) part += Y[l + (int64_t)i * r] * yj[l];
+            const double w = tau * (wave_reduce(part) + yj[i]);
+            double n2 = 0.0;
+            for (int l = i + 1 + lane; l < r; l += 64) {
+                const double v = yj[l] - w * Y[l + (int64_t)i * r];
+                yj[l] = v;
+                n2 += v * v;
+            }
+            n2 = wave_reduce(n2);
+            if (lane == 0) {
+                yj[i] -= w;
+                nrm2[j] = n2;
+            }
+        }
+        __syncthreads();
+        k = i + 1;
+    }
+    // Q[:, 0:k] = H_0 ... H_{k-1} [I_k; 0] (LAPACK dorg2r order)
+    for (int e = tid; e < r * k; e += blockDim.x) Q[e] = (e % r == e / r) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int i = k - 1; i >= 0; --i) {
+        const double tau = tauv[i];
+        for (int j = i + wave; j < k; j += nw) {
+            const gptr<double> qj = Q + (int64_t)j * r;
+            double part = 0.0;
+            for (int l = i + 1 + lane; l < r; l += 64) part += Y[l + (int64_t)i * r] * qj[l];
+            const double w = tau * (wave_reduce(part) + qj[i]);
+            for (int l = i + 1 + lane; l < r; l += 64) qj[l] -= w * Y[l + (int64_t)i * r];
+            if (lane == 0) qj[i] -= w;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *gbl(t.rank) = k;
+}
+hipError_t launch_rrqr(const RrqrTask* tasks, int count, int max_s, int max_kmax, double atol, double rtol,
+                       hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    const size_t lds = sizeof(double) * ((size_t)max_s + (size_t)max_kmax + 8);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rrqr),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    klaunch(k_rrqr, dim3((unsigned)count), dim3(256), lds, st, tasks, atol, rtol);
+    return hipGetLastError();
 }
 
 // A solve record written on the device from the kernel's by-value argument: the

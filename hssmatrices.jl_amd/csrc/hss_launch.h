@@ -77,6 +77,10 @@ template <typename T>
 hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const SweepRec* work, int nwork, int max_nrhs, int tw,
                               size_t lds, int resident, int nt, hipStream_t st);
 
+// direct compression: Gaussian test matrix (rows x cols, ld rows) and truncated pivoted QRs
+hipError_t launch_gauss(double* om, int64_t rows, int cols, uint64_t seed, hipStream_t st);
+hipError_t launch_rrqr(const RrqrTask* tasks, int count, int max_s, int max_kmax, double atol, double rtol,
+                       hipStream_t st);
 // writes one solve record to device memory from a kernel argument (no host copy)
 hipError_t launch_put_sinst(SInst<double>* dst, const SInst<double>& v, hipStream_t st);
 // one workgroup per task (HSS x HSS product generators)

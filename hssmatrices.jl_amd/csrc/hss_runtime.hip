@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/hss_hip.h"
+#include "hss_host.h"
 #include "hss_internal.h"
 #include "hss_launch.h"
 
@@ -770,6 +771,28 @@ static hss_status mat_create_impl(hss_ctx_t ctx, const hss_node_desc* nodes, int
     return HSS_OK;
 }
 
+namespace hss {
+hss_status host_fail(hss_status st, const std::string& msg) { return fail(st, msg); }
+hss_status host_validate_tree(const hss_node_desc* nodes, int nn) {
+    if (nn <= 0 || !nodes) return fail(HSS_ERR_ARG, "empty node table");
+    std::vector<hss_node_desc> t(nodes, nodes + nn);
+    for (auto& x : t) {
+        x.ku = x.kv = 0;
+        x.off_D = x.off_U = x.off_V = x.off_B12 = x.off_B21 = x.off_R = x.off_W = -1;
+    }
+    for (int i = 0; i < nn; ++i)
+        if (t[i].left < 0 && (int64_t)t[i].m * t[i].n > 0) t[i].off_D = 0;  // D is checked against a blob of m*n
+    int64_t big = 0;
+    for (auto& x : t) big = std::max<int64_t>(big, (int64_t)x.m * x.n);
+    return validate(t.data(), nn, big);
+}
+hss_status host_mat_from_device(hss_ctx_t ctx, const hss_node_desc* nodes, int nn, double** dblob, int64_t blob_len,
+                                hss_mat_t* out) {
+    return mat_create_impl(ctx, nodes, nn, nullptr, dblob, blob_len, out);
+}
+int host_ctx_device(hss_ctx_t ctx) { return ctx ? ctx->device : -1; }
+}  // namespace hss
+
 extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, int32_t nn, const double* blob,
                                      int64_t blob_len, hss_mat_t* out) {
     if (!ctx || !nodes || !out || (blob_len > 0 && !blob)) return fail(HSS_ERR_ARG, "null argument");
@@ -948,6 +971,22 @@ extern "C" hss_status hss_mat_load(hss_ctx_t ctx, const char* path, hss_mat_t* o
     s = hss_gen_read(path, nodes.data(), nn, &nn, blob.data(), bl, &bl);
     if (s != HSS_OK) return s;
     return hss_mat_create(ctx, nodes.data(), nn, blob.data(), bl, out);
+}
+
+extern "C" hss_status hss_mat_generators(hss_mat_t A, hss_node_desc* nodes, int32_t nodes_cap, int32_t* num_nodes,
+                                         double* blob, int64_t blob_cap, int64_t* blob_len) {
+    if (!A || !num_nodes || !blob_len) return fail(HSS_ERR_ARG, "null argument");
+    *num_nodes = A->nn;
+    *blob_len = A->blob_len;
+    if ((nodes && nodes_cap < A->nn) || (blob && blob_cap < A->blob_len))
+        return fail(HSS_ERR_DIM, "output buffers too small");
+    if (nodes) std::memcpy(nodes, A->desc.data(), sizeof(hss_node_desc) * A->nn);
+    if (blob && A->blob_len > 0) {
+        HIP_TRY(hipSetDevice(A->ctx->device));
+        HIP_TRY(hipStreamSynchronize(A->ctx->stream));
+        HIP_TRY(hipMemcpy(blob, A->d_blob, sizeof(double) * A->blob_len, hipMemcpyDeviceToHost));
+    }
+    return HSS_OK;
 }
 
 extern "C" hss_status hss_mat_save(hss_mat_t A, const char* path) {

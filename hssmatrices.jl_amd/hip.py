@@ -36,7 +36,7 @@ import pathlib
 
 import numpy as np
 
-from .hssmatrix import NODE_DTYPE, DimensionMismatch, HssMatrix, SingularException, pack
+from .hssmatrix import NODE_DTYPE, DimensionMismatch, HssMatrix, SingularException, cluster_nodes, pack
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libhss_hip.so"
 
@@ -48,7 +48,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 EXPORTS = [
     "hss_ctx_create", "hss_ctx_destroy", "hss_ctx_stream",
     "hss_mat_create", "hss_mat_destroy", "hss_mat_adjoint", "hss_mat_mul", "hss_mat_size", "hss_mat_device_bytes",
-    "hss_gen_write", "hss_gen_read", "hss_mat_load", "hss_mat_save",
+    "hss_gen_write", "hss_gen_read", "hss_mat_load", "hss_mat_save", "hss_compress_dev", "hss_mat_generators",
     "hss_matmul", "hss_matmul_dev",
     "hss_ulvfactsolve", "hss_ulv_factor", "hss_fact_destroy", "hss_fact_device_bytes",
     "hss_ulv_solve", "hss_ulv_solve_dev", "hss_ulv_solve_dev_noalloc", "hss_ulv_solve_batch_dev",
@@ -109,6 +109,8 @@ def load_library(path: pathlib.Path | str | None = None):
         "hss_gen_read": ([ctypes.c_char_p, V, I32, ctypes.POINTER(I32), V, I64, ctypes.POINTER(I64)], ctypes.c_int),
         "hss_mat_load": ([V, ctypes.c_char_p, PV], ctypes.c_int),
         "hss_mat_save": ([V, ctypes.c_char_p], ctypes.c_int),
+        "hss_compress_dev": ([V, V, I64, V, I32, D, D, PV], ctypes.c_int),
+        "hss_mat_generators": ([V, V, I32, ctypes.POINTER(I32), V, I64, ctypes.POINTER(I64)], ctypes.c_int),
         "hss_matmul": ([V, I32, D, V, I64, D, V, I64], ctypes.c_int),
         "hss_matmul_dev": ([V, I32, D, V, I64, D, V, I64], ctypes.c_int),
         "hss_ulvfactsolve": ([V, I32, V, I64, V, I64], ctypes.c_int),
@@ -297,6 +299,17 @@ class HipHssMatrix:
         _check(lib.hss_mat_load(ctx.handle, str(path).encode(), ctypes.byref(h)))
         return cls(_Adopt(h, nodes), ctx)
 
+    def generators(self):
+        """The matrix's ``(nodes, blob)`` in host memory (``hss_mat_generators``): the layout
+        ``pack`` produces and ``unpack`` turns back into an ``HssMatrix``."""
+        nn, bl = ctypes.c_int32(), ctypes.c_int64()
+        _check(_lib.hss_mat_generators(self._h, None, 0, ctypes.byref(nn), None, 0, ctypes.byref(bl)))
+        nodes = np.zeros(nn.value, dtype=NODE_DTYPE)
+        blob = np.zeros(max(bl.value, 1), dtype=np.float64)
+        _check(_lib.hss_mat_generators(self._h, nodes.ctypes.data, nn.value, ctypes.byref(nn), blob.ctypes.data,
+                                       blob.size, ctypes.byref(bl)))
+        return nodes, blob[:bl.value]
+
     def save(self, path) -> None:
         """The matrix's generators, read back from HBM, to a generator file (``hss_mat_save``)."""
         _check(_lib.hss_mat_save(self._h, str(path).encode()))
@@ -438,6 +451,39 @@ def mul(C, hssA: HipHssMatrix, B, alpha: float = 1.0, beta: float = 0.0):
     _check(_lib.hss_matmul(hssA.handle, nrhs, float(alpha), Bf.ctypes.data, max(Bf.shape[0], 1), float(beta),
                            C2.ctypes.data, max(C2.shape[0], 1)))
     return C
+
+
+def compress_dev(dA: int, lda: int, rcl, ccl=None, atol: float = 1e-9, rtol: float = 1e-9,
+                 ctx: Context | None = None) -> HipHssMatrix:
+    """Direct HSS compression of a dense device matrix (``hss_compress_dev``), the device
+    counterpart of ``compress(A, rcl, ccl; atol, rtol)`` (reference ``compression.jl:47-63``).
+
+    ``dA`` is the address of the column-major matrix (``lda`` >= its rows; a torch tensor
+    ``T`` of shape (cols, rows) in row-major order is such a matrix with ``lda = rows``).
+    ``rcl`` / ``ccl`` are ``ClusterTree``s (``ccl`` defaults to ``rcl``).  The device work is
+    enqueued on the context's stream after a device-wide synchronization, so buffers that
+    torch filled on its own stream are complete."""
+    lib = load_library()
+    ctx = ctx or default_context()
+    ccl = rcl if ccl is None else ccl
+    nodes = cluster_nodes(rcl, ccl)
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:  # pragma: no cover
+        pass
+    h = ctypes.c_void_p()
+    _check(lib.hss_compress_dev(ctx.handle, dA, lda, nodes.ctypes.data, len(nodes), float(atol), float(rtol),
+                                ctypes.byref(h)))
+    # the node table with the ranks and offsets found; the blob stays on the device
+    found = np.zeros(len(nodes), dtype=NODE_DTYPE)
+    nn, bl = ctypes.c_int32(), ctypes.c_int64()
+    st = lib.hss_mat_generators(h, found.ctypes.data, len(found), ctypes.byref(nn), None, 0, ctypes.byref(bl))
+    if st != HSS_OK:
+        lib.hss_mat_destroy(h)
+        _check(st)
+    return HipHssMatrix(_Adopt(h, found), ctx)
 
 
 def ulvfactsolve(hssA: HipHssMatrix, b):

@@ -354,6 +354,43 @@ NODE_DTYPE = np.dtype([
 assert NODE_DTYPE.itemsize == 112
 
 
+def cluster_nodes(rcl: ClusterTree, ccl: ClusterTree | None = None) -> np.ndarray:
+    """The node table of a pair of compatible cluster trees (row tree ``rcl``, column tree
+    ``ccl``, reference ``binarytree.jl:68-74``): ``NODE_DTYPE`` records in the breadth-first
+    order ``pack`` uses, with the links, depths, heights, sizes and offsets filled, ranks 0
+    and no generators.  The tree argument of ``hss_compress_dev``."""
+    ccl = rcl if ccl is None else ccl
+    if not compatible(rcl, ccl):
+        raise ValueError("row and column clusters are not compatible")
+    order = [(rcl, ccl, -1, 0)]
+    i = 0
+    while i < len(order):
+        r, c, _, d = order[i]
+        if r.isbranch():
+            order.append((r.left, c.left, i, d + 1))
+            order.append((r.right, c.right, i, d + 1))
+        i += 1
+    nodes = np.zeros(len(order), dtype=NODE_DTYPE)
+    for f in ("off_D", "off_U", "off_V", "off_B12", "off_B21", "off_R", "off_W", "left", "right"):
+        nodes[f] = -1
+    for i, (r, c, parent, d) in enumerate(order):
+        rec = nodes[i]
+        rec["parent"], rec["depth"] = parent, d
+        rec["m"], rec["n"] = r.size, c.size
+        rec["row_off"], rec["col_off"] = r.start - rcl.start, c.start - ccl.start
+        if parent >= 0:
+            p = nodes[parent]
+            if p["left"] < 0:
+                p["left"] = i
+            else:
+                p["right"] = i
+    for i in range(len(order) - 1, -1, -1):
+        l, r = nodes["left"][i], nodes["right"][i]
+        if l >= 0:
+            nodes["height"][i] = 1 + max(nodes["height"][l], nodes["height"][r])
+    return nodes
+
+
 def pack(h: HssMatrix):
     """Pack a (root) HSS matrix into ``(nodes, blob)``.
 

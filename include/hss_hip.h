@@ -159,6 +159,25 @@ hss_status hss_mat_load(hss_ctx_t ctx, const char* path, hss_mat_t* out);
 /* The matrix's generators (read back from HBM) and node table to a generator file */
 hss_status hss_mat_save(hss_mat_t mat, const char* path);
 
+/* ---- direct compression on the device: compress(A, rcl, ccl; atol, rtol), src/compression.jl:47-63 ----
+ * dA: the dense rows(tree[0]) x cols(tree[0]) matrix, column-major (lda), in device memory.
+ * tree: the cluster trees as a node table in breadth-first order (parent, left, right, m, n,
+ * row_off, col_off of every node: the row tree rcl and column tree ccl of the reference,
+ * which must be compatible, src/binarytree.jl:68-74); ranks and generator offsets are not
+ * read.  Each node's basis is the truncated column-pivoted QR of its HSS block row (column),
+ * cut at the first pivot whose norm is not above max(atol, rtol * the first pivot's norm),
+ * the reference's rank rule (src/compression.jl:22-30); wide blocks are sampled with
+ * Gaussian columns first.  The result is a new matrix (ranks as found) on ctx.  The
+ * reference's default tolerances are atol = rtol = 1e-9 (src/HssMatrices.jl:62-63). */
+hss_status hss_compress_dev(hss_ctx_t ctx, const double* dA, int64_t lda, const hss_node_desc* tree, int32_t num_nodes,
+                            double atol, double rtol, hss_mat_t* out);
+/* The matrix's node table and generator blob in host memory (the layout hss_mat_create
+ * takes): the HssMatrix{Float64} itself (src/hssmatrix.jl:11-68), e.g. after
+ * hss_compress_dev.  *num_nodes and *blob_len are always set; nodes and blob are each
+ * filled when non-null (HSS_ERR_DIM if nodes_cap or blob_cap is too small). */
+hss_status hss_mat_generators(hss_mat_t mat, hss_node_desc* nodes, int32_t nodes_cap, int32_t* num_nodes,
+                              double* blob, int64_t blob_cap, int64_t* blob_len);
+
 /* ---- matvec: C = alpha*A*B + beta*C,  mul!(C, hssA, B, alpha, beta), src/matmul.jl:18-62 ----
  * Host pointers; B is cols(A) x nrhs (ldb), C is rows(A) x nrhs (ldc). */
 hss_status hss_matmul(hss_mat_t mat, int32_t nrhs, double alpha, const double* B, int64_t ldb, double beta,
