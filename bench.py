@@ -3,8 +3,10 @@
 
 Workload (the metric's configuration): n = 65536 points sorted uniform on
 [-1, 1] (numpy default_rng(0)), kernel log|x_i - x_j| with unit diagonal,
-bisection leaf size 128, direct compression at rtol = atol = 1e-9
-(512 leaves, depth 9), 64 right-hand sides N(0,1) (default_rng(1)).
+bisection leaf size 128, direct compression at rtol = atol = 1e-9 on the device
+(hss_compress_dev; 512 leaves, depth 9), 64 right-hand sides N(0,1)
+(default_rng(1)).  The batched and serving sections build their compressed
+Cauchy and log-kernel instances on the device the same way (untimed).
 
 A "step" is one ULV solve of the 64-column block with a factorization resident
 in HBM (inputs and outputs already on the device):
@@ -39,34 +41,21 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, GB/s
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector = matrix)
 FP32_MFMA_PEAK_TFLOPS = 157.3
-DATA = ROOT / "data" / "log_n65536_leaf128_tol1e-9.npz"
 N_POINTS, LEAF, TOL, NRHS = 65536, 128, 1e-9, 64
-# diagonal shift of the synthetic batched/serving instances: leaf blocks N(0, 1/m) + 2 I with
-# orthonormal nested bases give condition numbers of about 20-40 (a shift of 1 puts the
-# circular-law spectrum of the leaf blocks against 0 and some instances become singular)
-SYNTH_DIAG = 2.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def load_generators(pkg):
-    if DATA.exists():
-        d = np.load(DATA)
-        return d["nodes"], d["blob"]
-    log(f"[bench] {DATA.name} missing: compressing the n={N_POINTS} log kernel on the host (untimed)")
+def build_matrix(hip, W):
+    """The n = 65536 log-kernel matrix, compressed on the device (untimed), and its
+    generators read back for the CPU baseline and the byte models."""
     t = time.time()
-    x = np.sort(np.random.default_rng(0).uniform(-1, 1, N_POINTS))
-    K = pkg.KernelMatrix(x, "log", 1.0)
-    rcl = pkg.bisection_cluster(N_POINTS, LEAF)
-    h = pkg.compress(K, rcl, rcl, TOL, TOL,
-                     progress=lambda d, n: log(f"[bench] compressed {d}/{n} leaves ({time.time() - t:.0f} s)"))
-    nodes, blob = pkg.pack(h)
-    DATA.parent.mkdir(exist_ok=True)
-    np.savez(DATA, nodes=nodes, blob=blob, x=x)
-    log(f"[bench] compression took {time.time() - t:.1f} s")
-    return nodes, blob
+    A = W.log_kernel(hip, N_POINTS, LEAF, TOL, 0)
+    nodes, blob = A.generators()
+    log(f"[bench] n={N_POINTS} log kernel compressed on the device in {time.time() - t:.1f} s (untimed)")
+    return A, nodes, blob
 
 
 # ---------------------------------------------------------------------------
@@ -225,10 +214,8 @@ def solve_ref_total(nodes, fact, nrhs, wf=8):
 
 
 # ---------------------------------------------------------------------------
-# batched (C4) and serving (C5) measurements.  Both use synthetic HSS
-# generators of the configs' shapes (``random_hss``): compressing hundreds of
-# kernel matrices on the host would take longer than the whole benchmark.  The
-# same kernels are pinned by the GPU parity tests on compressed kernel matrices.
+# batched (C4) and serving (C5) measurements on kernel matrices compressed on the device
+# (hssmatrices.jl_amd.workloads; generation is untimed)
 # ---------------------------------------------------------------------------
 def _dev_rhs(torch, rng, n, r, dev):
     """B (r x n torch tensor = column-major n x r) and a zero X; synchronized, because torch
@@ -239,28 +226,40 @@ def _dev_rhs(torch, rng, n, r, dev):
     return B, X
 
 
-def batched(pkg, hip, torch, steps=10, warmup=2, count=256, seed=3, log=print):
-    """C4: `count` instances, n log-uniform in [1024, 16384], leaf 64, ranks 10-40,
-    1-32 RHS each.  Compares one batched plan (one launch per tree level for all
-    instances) against the plain path (one FP64 solve call per instance)."""
-    rng = np.random.default_rng(seed)
+def _timed(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, out
+
+
+def batched(hip, W, torch, steps=10, warmup=2, count=256, seed=3, log=print):
+    """C4: `count` compressed Cauchy matrices (n log-uniform in [1024, 16384], leaf 64,
+    rtol 1e-8), 1-32 RHS each, every solve to relative residual <= 1e-10.  One batched plan
+    (one launch per tree level for all instances; Float32 factors + Float64 refinement in
+    lockstep, or Float64 factors + refinement) against the plain path: one Float64 solve
+    call per instance, refined to the same 1e-10 where its first solve is not there."""
     dev = torch.device("cuda:0")
-    ns = np.round(np.exp(rng.uniform(np.log(1024), np.log(16384), count))).astype(int)
-    rs = rng.integers(1, 33, count)
     t = time.time()
-    mats, packs = [], []
-    for i, n in enumerate(ns):
-        h = pkg.random_hss(int(n), 64, (10, 40), rng, diag=SYNTH_DIAG)
-        packs.append(pkg.pack(h))
-        mats.append(hip.HipHssMatrix(packs[-1]))
-    log(f"[bench] batch: {count} synthetic instances generated in {time.time() - t:.1f} s "
-        f"(n {ns.min()}-{ns.max()}, total RHS {int(rs.sum())})")
+    mats, ns, shifts, rs = W.cauchy_instances(hip, count, seed=seed)
+    ks = [hip_summary(m) for m in mats]
+    log(f"[bench] batch: {count} compressed Cauchy instances built on the device in {time.time() - t:.1f} s "
+        f"(n {ns.min()}-{ns.max()}, ranks {min(k[0] for k in ks)}-{max(k[1] for k in ks)}, total RHS {int(rs.sum())})")
+    rng = np.random.default_rng(seed + 100)
     Bs, Xs = [], []
     for n, r in zip(ns, rs):
         B, X = _dev_rhs(torch, rng, int(n), int(r), dev)
         Bs.append(B)
         Xs.append(X)
-    out = {"instances": count, "n_range": [int(ns.min()), int(ns.max())], "rhs_total": int(rs.sum())}
+    out = {"workload": f"{count} compressed Cauchy matrices 1/(x_i-x_j) (unit diagonal), n {int(ns.min())}-"
+                       f"{int(ns.max())} log-uniform, leaf 64, rtol 1e-8, ranks {min(k[0] for k in ks)}-"
+                       f"{max(k[1] for k in ks)}, 1-32 RHS each (seed {seed})",
+           "instances": count, "n_range": [int(ns.min()), int(ns.max())], "rhs_total": int(rs.sum())}
     for prec in ("fp64", "fp32"):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -268,52 +267,28 @@ def batched(pkg, hip, torch, steps=10, warmup=2, count=256, seed=3, log=print):
         torch.cuda.synchronize()
         fms = (time.perf_counter() - t0) * 1e3
         plan = hip.Plan(facts, [int(r) for r in rs], [b.data_ptr() for b in Bs], [int(b.shape[1]) for b in Bs],
-                        [x.data_ptr() for x in Xs], [int(x.shape[1]) for x in Xs], refinement=(prec == "fp32"))
-        tol = 1e-10 if prec == "fp32" else None
+                        [x.data_ptr() for x in Xs], [int(x.shape[1]) for x in Xs], refinement=True)
         try:
-            for _ in range(warmup):
-                plan.execute(tol=tol)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            rep = None
-            for _ in range(steps):
-                rep = plan.execute(tol=tol)
-            torch.cuda.synchronize()
+            ms, rep = _timed(torch, lambda: plan.execute(tol=1e-10, max_iter=30), steps, warmup)
         except hip.NotConverged as e:
             log(f"[bench] batch {prec}: {e}")
             out[prec] = {"error": str(e)}
             plan.close()
-            for f in facts:
-                f.close()
             continue
-        ms = (time.perf_counter() - t0) * 1e3 / steps
-        recs = profile_launches(hip, lambda: plan.execute(tol=tol), 1)
-        tot, byk = summarize(recs, 1)
-        levels = sorted(((k, lev, v[0], v[2]) for (k, lev), v in tot.items()), key=lambda t: -t[2])[:12]
-        log(f"[bench] batch {prec} busiest launches (kernel, level, ms, nodes): "
-            f"{[(k, lev, round(ms, 3), nw) for k, lev, ms, nw in levels]}")
+        recs = profile_launches(hip, lambda: plan.execute(tol=1e-10, max_iter=30), 1)
+        _, byk = summarize(recs, 1)
         out[prec] = {"batch_ms": round(ms, 3), "solves_per_s": round(count * 1e3 / ms, 1),
-                     "factor_batch_ms": round(fms, 2),
+                     "factor_batch_ms": round(fms, 2), "refine_steps": rep["iterations"],
+                     "worst_rel_residual": rep["rel_residual"],
                      "worst_rel_residual_fp64_check": _batch_residual(mats, Bs, Xs, rs, torch),
                      "per_kernel_ms": {k: round(v, 4) for k, v in byk.items()}}
-        log(f"[bench] batch {prec} per kernel (ms per batch): {out[prec]['per_kernel_ms']}")
-        if rep is not None and prec == "fp32":
-            out[prec]["refine_steps"] = rep["iterations"]
-            out[prec]["worst_rel_residual"] = rep["rel_residual"]
         log(f"[bench] batch {prec}: {ms:.3f} ms per batch -> {count * 1e3 / ms:.1f} solves/s "
-            f"(factor {fms:.1f} ms)")
+            f"({rep['iterations']} refinement steps, factor {fms:.1f} ms); per kernel {out[prec]['per_kernel_ms']}")
         # the same plan after the autotuner chose each level's launch (hss_plan_tune)
         t0 = time.perf_counter()
         tune = plan.tune(reps=3)
         tune_wall = (time.perf_counter() - t0) * 1e3
-        for _ in range(warmup):
-            plan.execute(tol=tol)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            plan.execute(tol=tol)
-        torch.cuda.synchronize()
-        tms = (time.perf_counter() - t0) * 1e3 / steps
+        tms, _ = _timed(torch, lambda: plan.execute(tol=1e-10, max_iter=30), steps, warmup)
         out[prec]["tuned"] = {"batch_ms": round(tms, 3), "solves_per_s": round(count * 1e3 / tms, 1),
                               "tuning_wall_ms": round(tune_wall, 1),
                               "one_solve_ms_before_after": [round(tune["ms_before"], 3), round(tune["ms_after"], 3)],
@@ -321,27 +296,39 @@ def batched(pkg, hip, torch, steps=10, warmup=2, count=256, seed=3, log=print):
         out[prec]["best_solves_per_s"] = max(out[prec]["solves_per_s"], out[prec]["tuned"]["solves_per_s"])
         log(f"[bench] batch {prec} tuned (in {tune_wall:.0f} ms): {tms:.3f} ms per batch -> "
             f"{count * 1e3 / tms:.1f} solves/s")
-        if prec == "fp64":
-            # plain path: one resident-factor FP64 solve call per instance
-            for _ in range(warmup):
-                for f, b, x, r in zip(facts, Bs, Xs, rs):
-                    f.solve_dev(int(r), b.data_ptr(), b.shape[1], x.data_ptr(), x.shape[1])
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(max(1, steps // 2)):
-                for f, b, x, r in zip(facts, Bs, Xs, rs):
-                    f.solve_dev(int(r), b.data_ptr(), b.shape[1], x.data_ptr(), x.shape[1])
-            torch.cuda.synchronize()
-            pms = (time.perf_counter() - t0) * 1e3 / max(1, steps // 2)
-            out["plain_fp64"] = {"ms": round(pms, 3), "solves_per_s": round(count * 1e3 / pms, 1)}
-            log(f"[bench] plain per-instance fp64: {pms:.3f} ms -> {count * 1e3 / pms:.1f} solves/s")
         plan.close()
+        if prec == "fp64":
+            # plain path: one resident-factor FP64 solve call per instance; then the same calls
+            # with refinement to 1e-10 (the FP64 factor's own IR), which the metric's residual
+            # asks of every solve
+            def plain():
+                for f, b, x, r in zip(facts, Bs, Xs, rs):
+                    f.solve_dev(int(r), b.data_ptr(), b.shape[1], x.data_ptr(), x.shape[1])
+            pms, _ = _timed(torch, plain, max(1, steps // 2), warmup)
+            res_plain = _batch_residual(mats, Bs, Xs, rs, torch)
+
+            def plain_ir():
+                for f, b, x, r in zip(facts, Bs, Xs, rs):
+                    f.solve_dev(int(r), b.data_ptr(), b.shape[1], x.data_ptr(), x.shape[1], tol=1e-10, max_iter=10)
+            pims, _ = _timed(torch, plain_ir, max(1, steps // 2), warmup)
+            out["plain_fp64"] = {"ms": round(pms, 3), "solves_per_s": round(count * 1e3 / pms, 1),
+                                 "worst_rel_residual": res_plain,
+                                 "refined_to_1e-10": {"ms": round(pims, 3), "solves_per_s": round(count * 1e3 / pims, 1),
+                                                      "worst_rel_residual": _batch_residual(mats, Bs, Xs, rs, torch)}}
+            log(f"[bench] plain per-instance fp64: {pms:.3f} ms -> {count * 1e3 / pms:.1f} solves/s (worst residual "
+                f"{res_plain:.2e}); refined to 1e-10: {pims:.3f} ms -> {count * 1e3 / pims:.1f} solves/s")
         for f in facts:
             f.close()
+    base = out["plain_fp64"]["refined_to_1e-10"]["solves_per_s"]
     for prec, key in (("fp32", "mixed_batched_vs_plain_fp64"), ("fp64", "fp64_batched_vs_plain_fp64")):
         if "solves_per_s" in out[prec]:
-            out[key] = round(out[prec]["best_solves_per_s"] / out["plain_fp64"]["solves_per_s"], 2)
+            out[key] = round(out[prec]["best_solves_per_s"] / base, 2)
     return out
+
+
+def hip_summary(m):
+    k = np.concatenate([m.nodes["ku"][1:], m.nodes["kv"][1:]])
+    return int(k.min()), int(k.max())
 
 
 def _batch_residual(mats, Bs, Xs, rs, torch):
@@ -358,38 +345,24 @@ def _batch_residual(mats, Bs, Xs, rs, torch):
     return worst
 
 
-def serving(pkg, hip, torch, count=32, requests=400, rate=2000.0, seed=4, log=print, budget=200 * 2 ** 30):
-    """C5: `count` resident FP32 factorizations (n log-uniform in [4096, 65536], leaf 128),
-    Poisson arrivals at `rate` requests/s picking a matrix uniformly, RHS ~ geometric(mean 8)
-    capped at 128, refinement to 1e-10.  Reports p50/p99 latency and solves/s."""
-    rng = np.random.default_rng(seed)
-    dev = torch.device("cuda:0")
-    ns = np.round(np.exp(rng.uniform(np.log(4096), np.log(65536), count))).astype(int)
-    t = time.time()
-    mats = [hip.HipHssMatrix(pkg.pack(pkg.random_hss(int(n), 128, (10, 40), rng, diag=SYNTH_DIAG))) for n in ns]
-    srv = hip.Server(budget_bytes=budget, prec="fp32", tol=1e-10, max_iter=20, max_inflight=64)
-    ids = [srv.admit(m) for m in mats]
-    log(f"[bench] serve: {count} factors resident ({srv.used_bytes() / 2**20:.1f} MiB) in {time.time() - t:.1f} s")
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, requests))
-    rhs = np.minimum(rng.geometric(1.0 / 8.0, requests), 128)
-    which = rng.integers(0, count, requests)
-    bufs = [_dev_rhs(torch, rng, int(ns[w]), int(r), dev) for w, r in zip(which, rhs)]
-    torch.cuda.synchronize()
+def _serve_run(hip, torch, srv, ids, ns, arrivals, rhs, which, bufs, log):
+    """Submits request k at arrivals[k] seconds (all at once when arrivals is None), steps the
+    server until every request is done; returns latencies (ms), refinement steps, worst
+    residual, failures and the wall time."""
     tickets = {}
-    lat = []
-    iters = []
+    lat, iters = [], []
     worst, failed = 0.0, 0
+    requests = len(rhs)
     t0 = time.perf_counter()
-    nxt = 0
-    done = 0
+    nxt = done = 0
     while done < requests:
         now = time.perf_counter() - t0
-        while nxt < requests and arrivals[nxt] <= now:
+        while nxt < requests and (arrivals is None or arrivals[nxt] <= now):
             B, X = bufs[nxt]
             tk = srv.submit(ids[which[nxt]], int(rhs[nxt]), B.data_ptr(), B.shape[1], X.data_ptr(), X.shape[1])
             tickets[tk] = nxt
             nxt += 1
-        if srv.step() == 0 and nxt < requests:
+        if srv.step() == 0 and nxt < requests and arrivals is not None:
             time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 0.001)))
         for tk in list(tickets):
             r = srv.poll(tk)
@@ -400,13 +373,47 @@ def serving(pkg, hip, torch, count=32, requests=400, rate=2000.0, seed=4, log=pr
                 failed += int(r["status"] != 1)
                 del tickets[tk]
                 done += 1
-    wall = time.perf_counter() - t0
+    return lat, iters, worst, failed, time.perf_counter() - t0
+
+
+def serving(hip, W, torch, count=32, requests=2000, seed=4, log=print, budget=200 * 2 ** 30):
+    """C5: `count` resident FP32 factorizations of compressed log-kernel matrices (n
+    log-uniform in [4096, 65536], leaf 128, rtol 1e-9); requests pick a matrix uniformly,
+    RHS ~ geometric(mean 8) capped at 128, refinement to 1e-10.  First the saturation
+    throughput (every request submitted at once), then Poisson arrivals at 70 % of it:
+    p50/p99 latency and solves/s."""
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    t = time.time()
+    mats, ns = W.log_kernel_instances(hip, count, seed=seed)
+    ks = [hip_summary(m) for m in mats]
+    srv = hip.Server(budget_bytes=budget, prec="fp32", tol=1e-10, max_iter=20, max_inflight=64)
+    ids = [srv.admit(m) for m in mats]
+    log(f"[bench] serve: {count} compressed log-kernel matrices built and their FP32 factors resident "
+        f"({srv.used_bytes() / 2**20:.1f} MiB) in {time.time() - t:.1f} s")
+    rhs = np.minimum(rng.geometric(1.0 / 8.0, requests), 128)
+    which = rng.integers(0, count, requests)
+    bufs = [_dev_rhs(torch, rng, int(ns[w]), int(r), dev) for w, r in zip(which, rhs)]
+    # saturation: all requests at once
+    lat_s, it_s, worst_s, fail_s, wall_s = _serve_run(hip, torch, srv, ids, ns, None, rhs, which, bufs, log)
+    sat = requests / wall_s
+    rate = 0.7 * sat
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, requests))
+    lat, iters, worst, failed, wall = _serve_run(hip, torch, srv, ids, ns, arrivals, rhs, which, bufs, log)
     srv.close()
-    out = {"factors": count, "n_range": [int(ns.min()), int(ns.max())], "requests": requests,
-           "arrival_rate_per_s": rate, "solves_per_s": round(requests / wall, 1),
+    out = {"workload": f"{count} compressed log-kernel matrices, n {int(ns.min())}-{int(ns.max())} log-uniform, "
+                       f"leaf 128, rtol 1e-9, ranks {min(k[0] for k in ks)}-{max(k[1] for k in ks)}; FP32 factors, "
+                       f"refinement to 1e-10 (seed {seed})",
+           "factors": count, "n_range": [int(ns.min()), int(ns.max())], "requests": requests,
+           "rhs_mean": round(float(rhs.mean()), 2),
+           "saturation": {"solves_per_s": round(sat, 1), "wall_s": round(wall_s, 3),
+                          "latency_ms": {"p50": round(float(np.percentile(lat_s, 50)), 3),
+                                         "p99": round(float(np.percentile(lat_s, 99)), 3)},
+                          "worst_rel_residual": worst_s, "not_converged": fail_s},
+           "arrival_rate_per_s": round(rate, 1), "solves_per_s": round(requests / wall, 1),
            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                           "p99": round(float(np.percentile(lat, 99)), 3)},
-           "refine_steps": [int(min(iters)), int(max(iters))], "rhs_mean": round(float(rhs.mean()), 2),
+           "refine_steps": [int(min(iters)), int(max(iters))],
            "worst_rel_residual": worst, "not_converged": failed}
     log(f"[bench] serve: {out}")
     return out
@@ -506,16 +513,16 @@ def main():
     hip = importlib.import_module("hssmatrices_jl_amd.hip")
     import torch
 
-    nodes, blob = load_generators(pkg)
+    W = importlib.import_module("hssmatrices_jl_amd.workloads")
+    torch.cuda.init()
+    ctx = hip.default_context()
+    A, nodes, blob = build_matrix(hip, W)
     summ = pkg.tree_summary(nodes)
     n = int(nodes["n"][0])
     nrhs = args.nrhs
     log(f"[bench] HSS n={n} leaves={summ['leaves']} depth={summ['max_depth']} ranks {summ['rank_min']}-"
         f"{summ['rank_max']} G={summ['G']}")
     dev = torch.device("cuda:0")
-    torch.cuda.init()
-    ctx = hip.default_context()
-    A = hip.HipHssMatrix((nodes, blob), ctx)
 
     # one untimed factorization of each precision first: the first launch of a kernel in a
     # process pays for loading its code object
@@ -849,6 +856,15 @@ def main():
             O.ulvfactsolve(nodes, blob, B_host)
             tall = time.perf_counter() - t0
         blas = [i for i in threadpool_info() if i.get("internal_api") == "openblas"]
+        model = "?"
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+        cpu_fs = float(np.mean(times))
         cpu = {
             "value": round(1.0 / float(np.mean(times)), 4),
             "unit": "solves/s",
@@ -858,13 +874,18 @@ def main():
                        f"over OpenBLAS {blas[0].get('version') if blas else '?'} via scipy), n={n}, {nrhs} RHS, "
                        f"1 BLAS thread; each call factors and solves as the reference's hssA\\B does"),
             "all_threads": {"threads": ncores, "seconds_per_solve": round(tall, 3)},
-            "seconds_per_solve": round(float(np.mean(times)), 3),
+            "seconds_per_solve": round(cpu_fs, 3),
+            "cpu_model": model,
+            # like for like: the reference's hssA\B factors and solves on every call, so the GPU's
+            # factor + solve is what the CPU's call compares with
+            "gpu_factor_plus_solve_per_s": round(1e3 / (f64_ms + fp64_ms), 3),
+            "gpu_vs_cpu_factor_plus_solve": round(cpu_fs * 1e3 / (f64_ms + fp64_ms), 1),
         }
         log(f"[bench] cpu baseline {cpu}")
     log(f"[bench] parity {parity}")
 
-    batch = None if args.no_batch else batched(pkg, hip, torch, log=log)
-    serve = None if args.no_serve else serving(pkg, hip, torch, log=log)
+    batch = None if args.no_batch else batched(hip, W, torch, log=log)
+    serve = None if args.no_serve else serving(hip, W, torch, log=log)
 
     value = 1e3 / fp64_ms
     if parity["fp64_rel_residual"] > 1e-10:
@@ -881,7 +902,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: log|x_i-x_j| kernel on sorted uniform points, seeds 0/1",
+        "data": "synthetic: kernel matrices on seeded uniform points, compressed on the device (log|x_i-x_j| "
+                "n=65536 seeds 0/1; 256 Cauchy instances seed 3; 32 log-kernel instances seed 4)",
         "config": {
             "workload": f"HSS n={n} log kernel, leaf {LEAF}, rtol {TOL:g}, {summ['leaves']} leaves, ranks "
                         f"{summ['rank_min']}-{summ['rank_max']}, {nrhs} RHS, resident Float64 ULV factor "

@@ -387,52 +387,6 @@ def test_c1_left_multiply_and_rdiv(c1_dev, oracle, c1):
     assert rel(Z @ c1["F"], X) < 1e-12
 
 
-C2 = ROOT / "data" / "log_n65536_leaf128_tol1e-9.npz"
-
-
-@pytest.mark.skipif(not C2.exists(), reason="n=65536 generators not present (bench.py writes them)")
-def test_c2_full_size(hip, oracle):
-    """Config C2 at full size: parity with the oracle plus size-independent properties."""
-    d = np.load(C2)
-    nodes, blob = d["nodes"], d["blob"]
-    n = int(nodes["n"][0])
-    A = hip.HipHssMatrix((nodes, blob))
-    rng = np.random.default_rng(16)
-    X = rng.standard_normal((n, 8))
-    Y = A @ X
-    assert rel(Y, oracle.matmul(nodes, blob, X)) < 1e-13
-    # linearity
-    Y2 = A @ (3.0 * X[:, :2] - X[:, 2:4])
-    assert rel(Y2, 3.0 * Y[:, :2] - Y[:, 2:4]) < 1e-13
-    F = A.ulvfactor("fp64")
-    B = rng.standard_normal((n, 4))
-    Z = F.solve(B)
-
-    def resid(X):
-        R = oracle.matmul(nodes, blob, X) - B
-        return np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0))
-
-    from threadpoolctl import threadpool_limits
-    with threadpool_limits(1):
-        Zo = oracle.ulvfactsolve(nodes, blob, B)
-    # as accurate as the reference (residual within 10x the oracle's own, which is about
-    # 1e-11 here: cond ~ 1e7) and within the 1e-10 residual the workload is quoted at
-    assert resid(Z) <= 10 * resid(Zo) + 1e-14
-    assert resid(Z) <= 1e-10
-    assert rel(Z, Zo) < 1e-9
-    # adjoint identity <W, A X> = <A' W, X>, and X / A through the adjoint solves X = Z A
-    At = A.adjoint()
-    W = rng.standard_normal((n, 8))
-    lhs, rhs = np.sum(W * Y), np.sum((At @ W) * X)
-    assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(W) * np.linalg.norm(Y)
-    Zr = B.T / A
-    Rr = (Zr @ A) - B.T
-    assert np.max(np.linalg.norm(Rr, axis=1) / np.linalg.norm(B, axis=0)) <= 1e-10
-    # the HSS product A A (ranks doubled) applies A twice
-    C = A @ A
-    assert rel(C @ X[:, :2], A @ Y[:, :2]) < 1e-12
-
-
 def _batch_inputs(hip, rng, specs, prec):
     import torch
     mats, packs = [], []
