@@ -28,6 +28,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <utility>
 #include <stdint.h>
 
 #define HSS_KERNELS_TU
@@ -143,6 +144,47 @@ __device__ __forceinline__ T wave_reduce(T v) {
     v += shfl_xor(v, 16);
     v += shfl_xor(v, 32);
     return v;
+}
+
+// Sum over the four 16-lane rows of a wave (lanes l, l^16, l^32, l^48), on the VALU:
+// v_permlane16_swap then v_permlane32_swap (gfx950), each adding the value the swap brings
+// to the lane's own; every lane ends with the same sum, added in the same order.
+__device__ __forceinline__ float rows_sum(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const int x = __float_as_int(v);
+    auto a = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    v = __int_as_float(a[0]) + __int_as_float(a[1]);
+    const int y = __float_as_int(v);
+    auto b = __builtin_amdgcn_permlane32_swap(y, y, false, false);
+    v = __int_as_float(b[0]) + __int_as_float(b[1]);
+#endif
+    return v;
+}
+__device__ __forceinline__ double rows_sum(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    auto pair = [](int lo, int hi) {
+        return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+    };
+    long long bits = __double_as_longlong(v);
+    auto a = __builtin_amdgcn_permlane16_swap((int)bits, (int)bits, false, false);
+    auto c = __builtin_amdgcn_permlane16_swap((int)(bits >> 32), (int)(bits >> 32), false, false);
+    v = pair(a[0], c[0]) + pair(a[1], c[1]);
+    bits = __double_as_longlong(v);
+    a = __builtin_amdgcn_permlane32_swap((int)bits, (int)bits, false, false);
+    c = __builtin_amdgcn_permlane32_swap((int)(bits >> 32), (int)(bits >> 32), false, false);
+    v = pair(a[0], c[0]) + pair(a[1], c[1]);
+#endif
+    return v;
+}
+// lane l's value (l uniform across the wave) to every lane (v_readlane)
+__device__ __forceinline__ float lane_read(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_read(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // Householder reflector as LAPACK dlarfg: given alpha and ||x||^2, returns
@@ -310,6 +352,28 @@ __device__ __forceinline__ void unstage(Tdst* dst, int64_t ld, const T* src, int
     }
 }
 
+// 16x16x4 MFMA of the element type: lane l supplies A(l%16, l/16) and B(l/16, l%16) and
+// receives 4 outputs of column l%16, rows row(l/16, r) in register r (4r + l/16 for the
+// Float64 instruction, 4(l/16) + r for the Float32 one).
+template <typename TF>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * r + lk; }
+};
+template <>
+struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
+
 // ---------------------------------------------------------------------------
 // matvec
 // ---------------------------------------------------------------------------
@@ -389,29 +453,24 @@ __device__ void merge_products(const FInst<T>& I, const DevNode& nd, const FNode
 template <int CPT>
 __device__ __forceinline__ int vpad(int c) { return c + (c / CPT) * 2; }
 
-// NT = threads per workgroup (the launch bound).  FP64 levels whose nodes need at most 768
-// threads run 768: the register budget per thread grows from 128 to 168 VGPRs and the
-// 1024-thread build's spills go (the leaves' LQ 28 % faster).  FP32 fits 1024 threads
-// without spills and keeps them.
-template <typename T, int CPT, int NT>
-__global__ __launch_bounds__(NT) void k_ulv_factor(const FInst<T>* __restrict__ insts,
-                                                             const WorkItem* __restrict__ work) {
-    const WorkItem wi = work[blockIdx.x];
-    const FInst<T>& I = insts[wi.inst];
-    const DevNode nd = I.nodes[wi.node];
-    const FNode f = I.fn[wi.node];
+// Shared front of the factor kernels (one workgroup per node, any thread count): the merged
+// U in LDS and the source of W = [D; V'] (phase 0), the exit of a pass-through node, the QL
+// of U with its outputs (phase 1), and M = T (Y' D) in LDS (phase 2), so that
+// Q_ql' D = D - Y M.  LDS: Us (m x ku) | G (k x k) | Tq (k x k) | M (k rows of vstride) |
+// vb (2 x vstride) | taus | sc, with vstride = S (CPT + 2) + 2.  Wm is set to the
+// row-major merge buffer of a branch (null at a leaf, whose D and V are read from the
+// generator blob).  Returns false for a pass-through node, whose outputs it has written.
+template <typename T, int CPT>
+__device__ __forceinline__ bool ulv_front(const FInst<T>& I, const DevNode& nd, const FNode& f,
+                                          unsigned char* lds_raw, const T*& Wm_out) {
     const int m = f.m, n = f.n, k = f.k, nk = f.nk, ku = nd.ku, kv = nd.kv;
     const int S = f.S;
     const int tid = threadIdx.x;
-    const int r = tid / S, s = tid % S;
-    const int c0 = s * CPT;
     const int R = m + kv;
-    const bool row_on = r < R;
     const int wave = tid / 64, lane = tid % 64;
     const int nwaves = blockDim.x / 64;
 
     // LDS: Us (m x ku) | G (k x k) | Tq (k x k) | M (k rows of vstride) | vb (2 x vstride) | taus | sc
-    extern __shared__ unsigned char lds_raw[];
     // every row chunk of the S threads is CPT wide (columns >= n are zero padding)
     const int vstride = S * (CPT + 2) + 2;
     T* Us = reinterpret_cast<T*>(lds_raw);
@@ -584,7 +643,7 @@ __global__ __launch_bounds__(NT) void k_ulv_factor(const FInst<T>* __restrict__ 
         for (int e = tid; e < n * kv; e += blockDim.x) Vr[e] = Wsrc(m + e / n, e % n);
         T* Ur = I.rw + f.rwU;
         for (int e = tid; e < m * ku; e += blockDim.x) Ur[e] = Us[e];
-        return;
+        return false;
     }
 
     // ---- phase 1: QL of U (m x k) in LDS, LAPACK dgeqlf semantics (ulvfactor.jl:39) ----
@@ -722,6 +781,44 @@ __global__ __launch_bounds__(NT) void k_ulv_factor(const FInst<T>* __restrict__ 
         }
     }
     __syncthreads();
+    Wm_out = Wm;
+    return true;
+}
+
+// NT = threads per workgroup (the launch bound).  FP64 levels whose nodes need at most 768
+// threads run 768: the register budget per thread grows from 128 to 168 VGPRs and the
+// 1024-thread build's spills go (the leaves' LQ 28 % faster).  FP32 fits 1024 threads
+// without spills and keeps them.
+template <typename T, int CPT, int NT>
+__global__ __launch_bounds__(NT) void k_ulv_factor(const FInst<T>* __restrict__ insts,
+                                                             const WorkItem* __restrict__ work) {
+    const WorkItem wi = work[blockIdx.x];
+    const FInst<T>& I = insts[wi.inst];
+    const DevNode nd = I.nodes[wi.node];
+    const FNode f = I.fn[wi.node];
+    const int m = f.m, n = f.n, k = f.k, nk = f.nk, kv = nd.kv;
+    const int S = f.S;
+    const int tid = threadIdx.x;
+    const int r = tid / S, s = tid % S;
+    const int c0 = s * CPT;
+    const int R = m + kv;
+    const bool row_on = r < R;
+    extern __shared__ unsigned char lds_raw[];
+    const T* Wm = nullptr;
+    if (!ulv_front<T, CPT>(I, nd, f, lds_raw, Wm)) return;
+    const int vstride = S * (CPT + 2) + 2;
+    T* Us = reinterpret_cast<T*>(lds_raw);
+    T* Mq = Us + (size_t)m * nd.ku + 2 * (size_t)k * k;
+    T* vbuf = Mq + (size_t)k * vstride;
+    T* taus = vbuf + 2 * vstride;
+    auto Wsrc = [&](int rr, int c) -> T {
+        if (Wm) return Wm[(int64_t)rr * n + c];
+        return rr < m ? gen<T>(I.blob, nd.D, rr + (int64_t)c * m) : gen<T>(I.blob, nd.V, c + (int64_t)(rr - m) * n);
+    };
+    auto yql = [&](int i, int j) -> T {
+        const int p = m - k + j;
+        return i < p ? Us[i + (int64_t)j * m] : (i == p ? T(1) : T(0));
+    };
 
     // ---- phase 3: rows of W in registers: D rows updated by -Y M, V' rows as is ----
     T w[CPT];
@@ -834,6 +931,258 @@ __global__ __launch_bounds__(NT) void k_ulv_factor(const FInst<T>* __restrict__ 
     }
     // LQ reflector scalars for the compact-WY blocks (built by k_ulv_wy)
     for (int e = tid; e < nk; e += blockDim.x) I.fac[f.taulq + e] = taus[e];
+}
+
+// Blocked factor of one node per workgroup, on MFMA (NT threads, 16 rows of W = [D; V']
+// per wave, n <= 16 NTILE).  Phases 0-2 as ulv_front.  Then each wave keeps its 16 rows of W
+// in registers, transposed, in the accumulator layout of the 16x16x4 MFMA: lane l holds row
+// 16 w + l%16 of W, columns 16 t + Mfma<T>::row(l/16, r) in a[t][r].  In that layout the
+// rows serve directly as the B operand of a product over W's columns, so every update is
+// an MFMA chain with the small operand (M, Y, V, T) read from LDS:
+//   Q_ql' D = D - Y M           (ulvfactor.jl:41; M = T (Y' D) from ulv_front)
+//   LQ of the first nk rows in panels of 16 reflectors (ulvfactor.jl:44,47,50): the panel's
+//   wave factors its own rows (dlarfg reflectors, each applied to the panel's later rows;
+//   cross-lane shuffles, no workgroup barrier), writes the reflectors V (16 x n) to LDS and
+//   forms their compact-WY T (dlarft forward, from the Gram V V' on MFMA); then every wave
+//   below applies the block, W <- W (I - V' T V) = W - ((W V') T) V, as three MFMA chains.
+//   One workgroup barrier per panel (V and T double-buffered).
+// Outputs as k_ulv_factor (L1, the LQ reflectors, L2a, V_top, the reduced D and V, taus)
+// plus the T blocks.
+template <typename T, int NTILE, int NT>
+__global__ __launch_bounds__(NT) void k_ulv_factor_mf(const FInst<T>* __restrict__ insts,
+                                                      const WorkItem* __restrict__ work) {
+    using MF = Mfma<T>;
+    using V4 = typename MF::v4;
+    constexpr int NC = 16 * NTILE;  // columns held per row (n <= NC)
+    const WorkItem wi = work[blockIdx.x];
+    const FInst<T>& I = insts[wi.inst];
+    const DevNode nd = I.nodes[wi.node];
+    const FNode f = I.fn[wi.node];
+    extern __shared__ unsigned char lds_raw[];
+    const T* Wm = nullptr;
+    if (!ulv_front<T, FACT_CPT>(I, nd, f, lds_raw, Wm)) return;
+    const int m = f.m, n = f.n, k = f.k, nk = f.nk, kv = nd.kv, R = m + kv;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int row = 16 * wave + li;
+    const int vstride = f.S * (FACT_CPT + 2) + 2;
+    const T* Us = reinterpret_cast<const T*>(lds_raw);
+    const T* Mq = Us + (size_t)m * nd.ku + 2 * (size_t)k * k;
+    auto colof = [&](int t, int r) { return 16 * t + MF::row(lk, r); };
+
+    // ---- rows of W; Q_ql' D = D - Y M as Wt -= M' Y' ----
+    V4 a[NTILE];
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int c = colof(t, r);
+            T x = T(0);
+            if (row < R && c < n) {
+                if (Wm) x = Wm[(int64_t)row * n + c];
+                else x = row < m ? gen<T>(I.blob, nd.D, row + (int64_t)c * m) : gen<T>(I.blob, nd.V, c + (int64_t)(row - m) * n);
+            }
+            a[t][r] = x;
+        }
+    for (int s4 = 0; s4 < (k + 3) / 4; ++s4) {
+        const int kk = 4 * s4 + lk;  // the k index this lane supplies in the step
+        T yb = T(0);                 // B(kk, i) = Y(row, kk), reflector kk of the QL
+        if (kk < k && row < m) {
+            const int pp = m - k + kk;
+            yb = row < pp ? Us[row + (int64_t)kk * m] : (row == pp ? T(1) : T(0));
+        }
+#pragma unroll
+        for (int t = 0; t < NTILE; ++t) {
+            const int c = 16 * t + li;  // A(c, kk) = -M(kk, c)
+            const T am = (kk < k && c < n) ? -Mq[kk * vstride + vpad<FACT_CPT>(c)] : T(0);
+            a[t] = MF::mac(am, yb, a[t]);
+        }
+    }
+    __syncthreads();  // Us and M are dead from here; the panels' buffers reuse the LDS
+
+    // ---- blocked LQ ----
+    T* Vs = reinterpret_cast<T*>(lds_raw);  // [2][16][NC]: the panel's reflectors (row-major)
+    T* Ts = Vs + 2 * 16 * NC;               // [2][16][16]: T(a, q) at a * 16 + q
+    T* tl = Ts + 2 * 256;                   // nk taus (nk <= n <= 128)
+    T* Gs = tl + 128;                       // [16]: the panel step's Gram column v_b . v_j
+    const int npan = (nk + 15) / 16;
+    for (int p = 0; p < npan; ++p) {
+        const int i0 = 16 * p, nb = min(16, nk - i0), buf = p & 1;
+        T* Vp = Vs + buf * 16 * NC;
+        T* Tp = Ts + buf * 256;
+        if (wave == p) {
+            // the panel's reflectors, one row at a time (a loop, not unrolled), each applied
+            // to the wave's later rows.  Rows live in the 4 lanes l%16 of the wave's 16-lane
+            // rows, so a row's sums are rows_sum; row j's norm and pivot reach every lane by
+            // readlane, and its reflector v_j reaches the other rows through row j of the
+            // panel's V buffer in LDS (written by row j's lanes, read back by every lane; one
+            // wave, so LDS order is program order).  Columns left of the panel (tiles t < p)
+            // are already reduced and skipped.  The same dot loop gives v_b . v_j for the
+            // panel's earlier reflectors b < j (their masked rows), which builds column j of
+            // T (dlarft forward) through the 16-entry Gram column Gs.  Row ra of T lives in
+            // LDS (Tp[ra * 16 + q]); the 4 lanes of row ra write the same values.
+            const int ra = li;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) Tp[ra * 16 + q] = T(0);
+            // Only tile p (the panel's own 16 columns) needs column masks; in tiles t > p every
+            // column is right of the reflector (c > i) and of every panel row's diagonal.  In
+            // tile p, column 16 p + cr with cr = Mfma<T>::row(lk, r).  Row j's reflector is
+            // stored lane-contiguously (slot 16 t + 4 lk + r) so each lane reads its 4 values
+            // of a tile as one vector.
+            for (int j = 0; j < nb; ++j) {
+                const int i = i0 + j;
+                T part = T(0), al = T(0);
+#pragma unroll
+                for (int t = 0; t < NTILE; ++t) {
+                    if (t < p) continue;
+                    if (t == p) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int cr = MF::row(lk, r);
+                            const T x = a[t][r];
+                            part += cr > j ? x * x : T(0);
+                            al += cr == j ? x : T(0);
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) part += a[t][r] * a[t][r];
+                    }
+                }
+                const T xn2 = lane_read(rows_sum(part), j);
+                const T alpha = lane_read(rows_sum(al), j);
+                T beta, tau, scl;
+                larfg(alpha, xn2, beta, tau, scl);
+                // row j becomes [L, beta, v_j]; its lanes also store v_j = [0, 1, x scl]
+                const bool own = li == j;
+                T* vrow = Vp + j * NC;
+#pragma unroll
+                for (int t = 0; t < NTILE; ++t) {
+                    if (t < p) continue;
+                    V4 v;
+                    if (t == p) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int cr = MF::row(lk, r);
+                            const T x = a[t][r];
+                            v[r] = cr > j ? x * scl : (cr == j ? T(1) : T(0));
+                            a[t][r] = own ? (cr == j ? beta : (cr > j ? v[r] : x)) : x;
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            v[r] = a[t][r] * scl;
+                            a[t][r] = own ? v[r] : a[t][r];
+                        }
+                    }
+                    if (own) *reinterpret_cast<V4*>(vrow + 16 * t + 4 * lk) = v;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                // d = row . v_j for the rows below j; v_li . v_j for the reflectors above
+                // (row li < j reads as [0, 1, v] about its diagonal column 16 p + li)
+                T d = T(0);
+#pragma unroll
+                for (int t = 0; t < NTILE; ++t) {
+                    if (t < p) continue;
+                    const V4 v = *reinterpret_cast<const V4*>(vrow + 16 * t + 4 * lk);
+                    if (t == p) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int cr = MF::row(lk, r);
+                            const T x = a[t][r];
+                            const T left = li > j ? x : (cr < li ? T(0) : (cr == li ? T(1) : x));
+                            d += left * v[r];
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) d += a[t][r] * v[r];
+                    }
+                }
+                d = rows_sum(d);
+                const T dt = li > j ? d * tau : T(0);
+#pragma unroll
+                for (int t = 0; t < NTILE; ++t) {
+                    if (t < p) continue;
+                    const V4 v = *reinterpret_cast<const V4*>(vrow + 16 * t + 4 * lk);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) a[t][r] -= dt * v[r];
+                }
+                // T(a, j) = -tau_j sum_{a <= b < j} T(a, b) (v_b . v_j) for a < j
+                Gs[li] = d;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                T acc = T(0);
+                for (int b2 = 0; b2 < j; ++b2) acc += Tp[ra * 16 + b2] * Gs[b2];
+                Tp[ra * 16 + j] = ra < j ? -tau * acc : (ra == j ? tau : T(0));
+                if (lane == 0) tl[i] = tau;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            // V (rows beyond nb zero) and T to LDS; T also to the factor
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int c = colof(t, r), ii = i0 + li;
+                    Vp[li * NC + c] = li < nb ? (c < ii ? T(0) : (c == ii ? T(1) : a[t][r])) : T(0);
+                }
+            if (lane < 16) {
+                T* Tg = I.fac + f.Tlq + (int64_t)p * NB * NB;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) Tg[ra + q * NB] = (q < nb && ra < nb) ? Tp[ra * 16 + q] : T(0);
+            }
+        }
+        __syncthreads();
+        if (wave > p) {
+            // W <- W - ((W V') T) V for this wave's rows, as Wt -= V' (T' (V Wt))
+            V4 P = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) P = MF::mac(Vp[li * NC + colof(t, r)], a[t][r], P);
+            V4 Q = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Q = MF::mac(Tp[MF::row(lk, r) * 16 + li], P[r], Q);
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) a[t] = MF::mac(-Vp[MF::row(lk, r) * NC + 16 * t + li], Q[r], a[t]);
+        }
+    }
+    __syncthreads();
+
+    // ---- outputs (the same blocks and layouts as k_ulv_factor's phase 5) ----
+    // (the row index passes through an empty asm so that the output addresses are formed
+    // here, not hoisted to the top of the kernel and kept live through the panels)
+    int orow = row;
+    asm volatile("" : "+v"(orow));
+    if (orow < R) {
+        const int row = orow;
+#pragma unroll
+        for (int t = 0; t < NTILE; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int c = colof(t, r);
+                if (c >= n) continue;
+                const T x = a[t][r];
+                if (row < nk) {
+                    if (c < nk) I.fac[f.L1 + row + (int64_t)c * nk] = (c <= row) ? x : T(0);
+                    I.fac[f.Ylq + c + (int64_t)row * n] = (c < row) ? T(0) : (c == row ? T(1) : x);
+                } else if (row < m) {
+                    const int rr = row - nk;
+                    if (c < nk) I.fac[f.L2a + rr + (int64_t)c * k] = x;
+                    else I.rw[f.rwD + rr + (int64_t)(c - nk) * f.mr] = x;
+                } else {
+                    const int j = row - m;
+                    if (c < nk) I.fac[f.Vt + c + (int64_t)j * nk] = x;
+                    else I.rw[f.rwV + (c - nk) + (int64_t)j * f.nr] = x;
+                }
+            }
+    }
+    for (int e = tid; e < nk; e += blockDim.x) I.fac[f.taulq + e] = tl[e];
 }
 
 // Compact-WY T blocks of the LQ reflectors (dlarft, forward columnwise) and the
@@ -1334,24 +1683,6 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_rinv(const FInst<TF>* __r
 // and receives 4 outputs of column l%16: rows 4r + l/16 in register r for the
 // Float64 instruction, rows 4(l/16) + r for the Float32 one).  K advances in chunks
 // of GK, staged through double-buffered LDS with one barrier per chunk.
-template <typename TF>
-struct Mfma;
-template <>
-struct Mfma<double> {
-    typedef double v4 __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lk, int r) { return 4 * r + lk; }
-};
-template <>
-struct Mfma<float> {
-    typedef float v4 __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
-};
 
 // x operand: rows [0, r0) from segment 0, [r0, r1) from segment 1 (global,
 // column-major, Float64 or the factor type), rows [r1, K) from an LDS block zs
@@ -2217,6 +2548,37 @@ static hipError_t launch_factor_nt(const FInst<T>* insts, const WorkItem* work, 
     klaunch((k_ulv_factor<T, FACT_CPT, NT>), dim3(nwork), dim3(NT), lds, st, insts, work);
     return hipGetLastError();
 }
+template <typename T, int NTILE, int NT>
+static hipError_t launch_factor_mf_nt(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds,
+                                      hipStream_t st) {
+    const void* fn = reinterpret_cast<const void*>(&k_ulv_factor_mf<T, NTILE, NT>);
+    hipError_t e = set_lds_attr<T>(fn, lds);
+    if (e != hipSuccess) return e;
+    klaunch((k_ulv_factor_mf<T, NTILE, NT>), dim3(nwork), dim3(NT), lds, st, insts, work);
+    return hipGetLastError();
+}
+template <typename T, int NTILE>
+static hipError_t launch_factor_mf_tile(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds, int waves,
+                                        hipStream_t st) {
+    if (waves <= 8) return launch_factor_mf_nt<T, NTILE, 512>(insts, work, nwork, lds, st);
+    if (waves <= 12) return launch_factor_mf_nt<T, NTILE, 768>(insts, work, nwork, lds, st);
+    if (sizeof(T) == 4) return launch_factor_mf_nt<T, NTILE, 1024>(insts, work, nwork, lds, st);
+    return hipErrorInvalidValue;
+}
+int factor_mf_max_rows(int elem_bytes) { return elem_bytes == 8 ? 192 : 256; }
+template <typename T>
+hipError_t launch_ulv_factor_mf(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds, int max_rows,
+                                int max_n, hipStream_t st) {
+    if (nwork <= 0) return hipSuccess;
+    // nodes whose children eliminated everything have an empty (0 x 0) block: one wave
+    if (max_rows < 0 || max_rows > factor_mf_max_rows(sizeof(T)) || max_n < 0 || max_n > 128 || lds > 160 * 1024)
+        return hipErrorInvalidValue;
+    // at least 8 waves: the rows need ceil(max_rows / 16), and the phases before the
+    // blocked LQ (merge, QL, M) are parallel over all threads
+    const int waves = std::max(8, (max_rows + 15) / 16);
+    return max_n <= 64 ? launch_factor_mf_tile<T, 4>(insts, work, nwork, lds, waves, st)
+                       : launch_factor_mf_tile<T, 8>(insts, work, nwork, lds, waves, st);
+}
 template <typename T>
 hipError_t launch_ulv_factor(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds, int threads,
                              hipStream_t st) {
@@ -2793,6 +3155,10 @@ hipError_t launch_put_sinst(SInst<double>* dst, const SInst<double>& v, hipStrea
 // explicit instantiations
 template hipError_t launch_ulv_factor<double>(const FInst<double>*, const WorkItem*, int, size_t, int, hipStream_t);
 template hipError_t launch_ulv_factor<float>(const FInst<float>*, const WorkItem*, int, size_t, int, hipStream_t);
+template hipError_t launch_ulv_factor_mf<double>(const FInst<double>*, const WorkItem*, int, size_t, int, int,
+                                                 hipStream_t);
+template hipError_t launch_ulv_factor_mf<float>(const FInst<float>*, const WorkItem*, int, size_t, int, int,
+                                                hipStream_t);
 template hipError_t launch_ulv_wy<double>(const FInst<double>*, const WorkItem*, int, hipStream_t);
 template hipError_t launch_ulv_wy<float>(const FInst<float>*, const WorkItem*, int, hipStream_t);
 template hipError_t launch_ulv_root<double>(const FInst<double>*, const WorkItem*, int, hipStream_t);

@@ -37,6 +37,12 @@ hipError_t launch_mv(int kind, const MvInst* insts, const MvRec* work, int nwork
 template <typename T>
 hipError_t launch_ulv_factor(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds, int threads,
                              hipStream_t st);
+// blocked factor on MFMA: every node of the launch has n <= 128 and m + kv <= max_rows, at
+// most factor_mf_max_rows(sizeof(T)) (192 rows for Float64, 256 for Float32)
+template <typename T>
+hipError_t launch_ulv_factor_mf(const FInst<T>* insts, const WorkItem* work, int nwork, size_t lds, int max_rows,
+                                int max_n, hipStream_t st);
+int factor_mf_max_rows(int elem_bytes);
 template <typename T>
 hipError_t launch_ulv_wy(const FInst<T>* insts, const WorkItem* work, int nwork, hipStream_t st);
 template <typename T>

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -68,6 +69,16 @@ struct ProfRec {
     float ms;
 };
 static bool g_prof = false;
+// HSS_FACTOR_KERNEL=vector (or =mfma) puts every level on the per-reflector factor kernel
+// (or every level that fits on the blocked MFMA kernel), for comparing the two; the results
+// agree to rounding.  Unset: by precision (see use_mf in factor_batch).
+enum class FactorKernel { Auto, Vector, Mfma };
+static const FactorKernel g_factor_kernel = [] {
+    const char* e = std::getenv("HSS_FACTOR_KERNEL");
+    if (e && std::strcmp(e, "vector") == 0) return FactorKernel::Vector;
+    if (e && std::strcmp(e, "mfma") == 0) return FactorKernel::Mfma;
+    return FactorKernel::Auto;
+}();
 static std::vector<ProfRec> g_prof_recs;
 static std::vector<hipEvent_t> g_prof_pool;
 
@@ -1629,11 +1640,11 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
             lds_rinv = std::max(lds_rinv, sizeof(TF) * (size_t)(r.m + NB) * RHS_TILE);
         }
     }
-    // work lists across the batch
+    // work lists across the batch; per level the launch shape of either factor kernel
     std::vector<WorkItem> work;
     std::vector<Range> lev(maxh);
-    std::vector<size_t> lev_lds(maxh, 0);
-    std::vector<int> lev_thr(maxh, 1);
+    std::vector<size_t> lev_lds(maxh, 0), lev_lds_mf(maxh, 0);
+    std::vector<int> lev_thr(maxh, 1), lev_rows(maxh, 0), lev_n(maxh, 0);
     for (int h = 0; h < maxh; ++h) {
         lev[h].off = (int)work.size();
         for (int b = 0; b < cnt; ++b) {
@@ -1644,10 +1655,25 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
                     const FNode& fi = out[b]->hfn[i];
                     lev_lds[h] = std::max(lev_lds[h], factor_lds<TF>(fi, M->hn[i]));
                     lev_thr[h] = std::max(lev_thr[h], (fi.m + M->hn[i].kv) * fi.S);
+                    lev_rows[h] = std::max(lev_rows[h], fi.m + M->hn[i].kv);
+                    lev_n[h] = std::max(lev_n[h], fi.n);
+                    // Vs 2 x 16 x 128 | Ts 2 x 256 | taus 128
+                    lev_lds_mf[h] = std::max(lev_lds_mf[h], sizeof(TF) * (size_t)(2 * 16 * 128 + 512 + 128 + 16));
                 }
         }
         lev[h].count = (int)work.size() - lev[h].off;
+        lev_lds_mf[h] = std::max(lev_lds_mf[h], lev_lds[h]);
     }
+    // Float64 levels run the blocked MFMA kernel when their nodes have n <= 128 and at most
+    // factor_mf_max_rows rows of [D; V'] (others run the per-reflector kernel).  Float32
+    // levels run the per-reflector kernel: measured on the n = 65536 log-kernel matrix
+    // (leaf 128, ranks 11-35), the blocked kernel is 10 % faster per level in Float64 and
+    // 0-20 % slower in Float32, whose per-reflector steps are cheaper.
+    auto use_mf = [&](int h) {
+        const bool want = g_factor_kernel == FactorKernel::Auto ? sizeof(TF) == 8 : g_factor_kernel == FactorKernel::Mfma;
+        return want && lev_n[h] <= 128 && lev_rows[h] <= factor_mf_max_rows((int)sizeof(TF)) &&
+               lev_lds_mf[h] <= 160 * 1024;
+    };
     const int wy_count = (int)work.size();  // every non-root node of every matrix
     Range roots{(int)work.size(), cnt};
     for (int b = 0; b < cnt; ++b) work.push_back(WorkItem{b, 0});
@@ -1675,8 +1701,14 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
         HIP_TRY(hipMemcpyAsync(d_work, work.data(), sizeof(WorkItem) * work.size(), hipMemcpyHostToDevice, st));
         HIP_TRY(hipEventRecord(ctx->ev0, st));
         for (int h = 0; h < maxh; ++h) {
-            ProfScope ps("k_ulv_factor", h, lev[h].count, st);
-            HIP_TRY((launch_ulv_factor<TF>(d_inst, d_work + lev[h].off, lev[h].count, lev_lds[h], lev_thr[h], st)));
+            if (use_mf(h)) {
+                ProfScope ps("k_ulv_factor_mf", h, lev[h].count, st);
+                HIP_TRY((launch_ulv_factor_mf<TF>(d_inst, d_work + lev[h].off, lev[h].count, lev_lds_mf[h],
+                                                  lev_rows[h], lev_n[h], st)));
+            } else {
+                ProfScope ps("k_ulv_factor", h, lev[h].count, st);
+                HIP_TRY((launch_ulv_factor<TF>(d_inst, d_work + lev[h].off, lev[h].count, lev_lds[h], lev_thr[h], st)));
+            }
         }
         {
             ProfScope ps("k_ulv_wy", 0, wy_count, st);
