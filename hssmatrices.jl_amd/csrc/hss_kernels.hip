@@ -1306,9 +1306,10 @@ __global__ __launch_bounds__(1024) void k_ulv_root(const FInst<T>* __restrict__ 
     int info = 0;
     for (int i = 0; i < m; ++i) {
         if (wave == 0) {
-            // idamax over A[i:m, i] (first index of the largest magnitude)
+            // idamax over A[i:m, i] (first index of the largest magnitude).  bi starts at
+            // i, so an all-NaN column pivots in range (and is reported below)
             T best = T(-1);
-            int bi = m;
+            int bi = i;
             for (int rr = i + lane; rr < m; rr += 64) {
                 const T v = fabs(A[rr + (int64_t)i * m]);
                 if (v > best) { best = v; bi = rr; }
@@ -1327,7 +1328,7 @@ __global__ __launch_bounds__(1024) void k_ulv_root(const FInst<T>* __restrict__ 
         __syncthreads();
         const int p = s_piv;
         const T pv = s_val;
-        if (pv == T(0)) {
+        if (pv == T(0) || pv != pv) {  // a zero or NaN pivot: singular (status i + 1)
             if (info == 0) info = i + 1;
         } else {
             if (p != i)

@@ -2691,6 +2691,9 @@ extern "C" hss_status hss_plan_execute(hss_plan_t P, const hss_refine_opts* opts
     if (rep) { rep->iterations = 0; rep->rel_residual = -1.0; rep->seconds = 0.0; }
     HIP_TRY(hipEventRecord(P->ctx->ev0, st));
     const bool refine = P->has_ir && opts && opts->mode == 1;
+    if (fp32 && !refine)
+        return fail(HSS_ERR_ARG, "a plan of Float32 factors executes with refinement only (with_refinement at "
+                                 "creation and opts->mode 1): a Float32 solve alone is accurate to about 1e-7");
     if (!refine && !g_prof && !fp32) {
         // Float64 solve: replay one captured graph of all level launches
         if (!P->gexec) {
@@ -2811,6 +2814,11 @@ extern "C" hss_status hss_ulv_solve_batch_dev(int32_t count, const hss_fact_t* f
         lx.push_back(ldx[i]);
     }
     if (f.empty()) return HSS_OK;
+    for (hss_fact_t F : f)
+        if (F && F->prec == HSS_PREC_FP32)
+            return fail(HSS_ERR_ARG, "hss_ulv_solve_batch_dev solves Float64 factors only: a Float32 factor alone "
+                                     "is accurate to about 1e-7; refine it (hss_plan_create with_refinement, "
+                                     "hss_plan_execute with opts->mode 1)");
     hss_plan_t P = nullptr;
     hss_status s = hss_plan_create((int32_t)f.size(), f.data(), nr.data(), b.data(), lb.data(), x.data(), lx.data(), 0, &P);
     if (s != HSS_OK) return s;
@@ -2885,6 +2893,10 @@ static hss_status grow(char** p, size_t* cap, size_t need) {
 extern "C" hss_status hss_server_create(hss_ctx_t ctx, int64_t hbm_budget_bytes, hss_prec prec,
                                         const hss_refine_opts* opts, int32_t max_inflight, hss_server_t* out) {
     if (!ctx || !out || hbm_budget_bytes <= 0 || max_inflight <= 0) return fail(HSS_ERR_ARG, "bad server arguments");
+    if (prec != HSS_PREC_FP64 && prec != HSS_PREC_FP32) return fail(HSS_ERR_ARG, "unknown precision");
+    if (prec == HSS_PREC_FP32 && opts && opts->mode != 1)
+        return fail(HSS_ERR_ARG, "a Float32 server needs refinement (opts->mode 1): a Float32 solve alone is "
+                                 "accurate to about 1e-7");
     auto* S = new (std::nothrow) hss_server_s;
     if (!S) return fail(HSS_ERR_OOM, "host allocation");
     S->ctx = ctx;

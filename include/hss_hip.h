@@ -190,7 +190,15 @@ hss_status hss_matmul_dev(hss_mat_t mat, int32_t nrhs, double alpha, const doubl
  * Factor + solve + release in one call, as the reference does on every "\". */
 hss_status hss_ulvfactsolve(hss_mat_t mat, int32_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx);
 
-/* Resident factorization (HBM) and solves that reuse it. */
+/* Resident factorization (HBM) and solves that reuse it.
+ * Node limits of the factor (this and hss_ulvfactsolve, the batched factor, the server; the
+ * reference has none): every node's block W = [D; V'] after its children's elimination, m
+ * rows of D and n columns, with kv rows of V', must have m <= 256, n <= 256 and
+ * (m + kv) * S <= 1024 for S the power of two >= ceil(n / 32), that is
+ *   n <= 32: m + kv <= 1024;  n <= 64: m + kv <= 512;  n <= 128: m + kv <= 256;  n <= 256: m + kv <= 128.
+ * Leaves of at most 128 points (the reference's default leaf size is 64, src/HssMatrices.jl:64)
+ * meet them whenever the ranks stay at or below 128.  A larger node is refused with
+ * HSS_ERR_UNSUPPORTED before any device work, naming the node and its sizes. */
 hss_status hss_ulv_factor(hss_mat_t mat, hss_prec prec, hss_fact_t* out);
 hss_status hss_fact_destroy(hss_fact_t fact);
 int64_t hss_fact_device_bytes(hss_fact_t fact);
@@ -214,7 +222,9 @@ hss_status hss_ulv_solve_dev_noalloc(hss_fact_t fact, int32_t nrhs, const double
                                      int64_t ldx);
 
 /* ---- batched solves over many (uneven) matrices: one launch per tree level ----
- * Each request i solves fact[i] with nrhs[i] columns, device pointers. */
+ * Each request i solves fact[i] with nrhs[i] columns, device pointers.  Float64 factors only
+ * (HSS_ERR_ARG for a Float32 factor, whose unrefined solve is accurate to about 1e-7: batch
+ * Float32 factors in a plan with refinement). */
 hss_status hss_ulv_solve_batch_dev(int32_t count, const hss_fact_t* facts, const int32_t* nrhs,
                                    const double* const* dB, const int64_t* ldb, double* const* dX,
                                    const int64_t* ldx);
@@ -223,7 +233,9 @@ hss_status hss_ulv_solve_batch_dev(int32_t count, const hss_fact_t* facts, const
  * execution issues one launch per tree level across all requests; without refinement the launch
  * sequence is captured once in a HIP graph and replayed.  With with_refinement != 0 an execution
  * may refine (opts->mode == 1) in lockstep: one batched Float64 HSS matvec per step, requests
- * leave the batch as they reach opts->tol.  No counterpart in the reference (configs C4/C5). */
+ * leave the batch as they reach opts->tol.  A plan of Float32 factors executes only with
+ * refinement (with_refinement != 0 and opts->mode == 1; HSS_ERR_ARG otherwise).  No counterpart
+ * in the reference (batched and served solves). */
 typedef struct hss_plan_s* hss_plan_t;
 hss_status hss_plan_create(int32_t count, const hss_fact_t* facts, const int32_t* nrhs, const double* const* dB,
                            const int64_t* ldb, double* const* dX, const int64_t* ldx, int32_t with_refinement,
@@ -251,7 +263,8 @@ int64_t hss_plan_device_bytes(hss_plan_t plan);
  * together -- their first solve or their next refinement correction in one launch per tree
  * level, and one batched Float64 matvec + residual -- and requests at opts->tol retire.
  * Device pointers dB/dX must stay valid until the ticket is reported finished by
- * hss_server_poll.  Not thread-safe: one caller drives a server. */
+ * hss_server_poll.  opts NULL: refinement to 1e-10, at most 10 steps.  A Float32 server needs
+ * opts->mode == 1 (HSS_ERR_ARG otherwise).  Not thread-safe: one caller drives a server. */
 typedef struct hss_server_s* hss_server_t;
 hss_status hss_server_create(hss_ctx_t ctx, int64_t hbm_budget_bytes, hss_prec prec, const hss_refine_opts* opts,
                              int32_t max_inflight, hss_server_t* out);

@@ -225,6 +225,54 @@ def test_errors_map_to_reference_exceptions(hip, c1_dev):
     assert (c1_dev @ np.zeros((2001, 0))).shape == (2001, 0)
 
 
+def test_refusals(hip):
+    """What the factor and the batched entry points refuse, with the documented status:
+    * a root block with a NaN column: pivoted in range and reported singular (LAPACK's
+      idamax never returns an index outside the column either);
+    * nodes beyond the factor's size limits (leaves of 250 points: n = 250 needs
+      m + kv <= 128): HSS_ERR_UNSUPPORTED naming the node, before any device work;
+    * Float32 factors where nothing refines them (hss_ulv_solve_batch_dev, a plan executed
+      without refinement, a server without refinement): HSS_ERR_ARG, not a 1e-7 answer."""
+    import ctypes
+
+    import torch
+    from hssmatrices_jl_amd.hssmatrix import SingularException
+    D = np.eye(6)
+    D[:, 2] = np.nan
+    Nn = hip.HipHssMatrix(H.pack(H.HssMatrix(D=D, rootnode=True)))
+    with pytest.raises(SingularException):
+        Nn.solve(np.ones(6))
+
+    _, h = random_hss(np.random.default_rng(3), 1000, 256, 0, "cauchy", 1e-8)
+    nodes, blob = H.pack(h)
+    assert nodes["n"][nodes["left"] < 0].max() == 250
+    big = hip.HipHssMatrix((nodes, blob))
+    for call in (lambda: big.ulvfactor("fp64"), lambda: big.ulvfactor("fp32"), lambda: big.solve(np.ones((1000, 1)))):
+        with pytest.raises(hip.HssError) as ei:
+            call()
+        assert ei.value.status == hip.HSS_ERR_UNSUPPORTED and "exceeds" in str(ei.value)
+
+    _, h = random_hss(np.random.default_rng(4), 600, 64, 0, "cauchy", 1e-8)
+    A = hip.HipHssMatrix(H.pack(h))
+    F32 = A.ulvfactor("fp32")
+    dev = torch.device("cuda:0")
+    B = torch.ones((2, 600), dtype=torch.float64, device=dev)
+    X = torch.zeros((2, 600), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    with pytest.raises(ValueError, match="Float64 factors only"):
+        hip.solve_batch_dev([F32], [2], [B.data_ptr()], [600], [X.data_ptr()], [600])
+    for refinement in (False, True):
+        plan = hip.Plan([F32], [2], [B.data_ptr()], [600], [X.data_ptr()], [600], refinement=refinement)
+        with pytest.raises(ValueError, match="refinement only"):
+            plan.execute()
+    lib = hip.load_library()
+    ctx = hip.default_context()
+    hs = ctypes.c_void_p()
+    st = lib.hss_server_create(ctx.handle, 1 << 30, hip.PREC_FP32, ctypes.byref(hip.RefineOpts(1e-10, 10, 0)), 4,
+                               ctypes.byref(hs))
+    assert st == hip.HSS_ERR_ARG and not hs.value
+
+
 @pytest.mark.parametrize("name", NAMES)
 def test_adjoint_golden(hip, oracle, name):
     """hss_mat_adjoint (hssmatrix.jl:165-171): A' X, X A (matmul.jl:14) and X / A
