@@ -41,6 +41,7 @@
 #include "hss_host.h"
 #include "hss_internal.h"
 #include "hss_launch.h"
+#include "hss_tasks.h"
 
 using namespace hss;
 
@@ -48,32 +49,6 @@ namespace {
 
 constexpr int kOversample = 32;                  // Gaussian columns beyond the input's rows
 constexpr uint64_t kSeed = 0x4853534350524553ull;  // fixed: compressions are reproducible
-
-// device buffers of one compression, freed together after the stream has drained
-struct Buffers {
-    hipStream_t st = nullptr;
-    std::vector<void*> ptrs;
-    ~Buffers() {
-        if (st) (void)hipStreamSynchronize(st);
-        for (void* p : ptrs) (void)hipFree(p);
-    }
-    template <typename T>
-    hipError_t get(T** p, size_t count) {
-        void* q = nullptr;
-        hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
-        if (e != hipSuccess) return e;
-        ptrs.push_back(q);
-        *p = reinterpret_cast<T*>(q);
-        return hipSuccess;
-    }
-    void release(void* p) {
-        auto it = std::find(ptrs.begin(), ptrs.end(), p);
-        if (it == ptrs.end()) return;
-        (void)hipStreamSynchronize(st);
-        (void)hipFree(p);
-        ptrs.erase(it);
-    }
-};
 
 // per-node state of one side (rows: U/R and Phi; columns: V/W and Psi)
 struct Side {
@@ -85,21 +60,6 @@ struct Side {
     int k = 0;             // rank
     double* Phi = nullptr; // k x (the other dimension): compressed block row / column
     double* big = nullptr; // expanded basis (size x k), columns only
-};
-
-struct TaskList {
-    std::vector<GemmTask> t;
-    void gemm(int m, int n, int k, int flags, const double* a, int lda, const double* b, int ldb, double* c, int ldc) {
-        if (m <= 0 || n <= 0) return;
-        GemmTask g{};
-        g.m = m; g.n = n; g.k = k; g.flags = flags;
-        g.lda = std::max(lda, 1); g.ldb = std::max(ldb, 1); g.ldc = std::max(ldc, 1);
-        g.a = a; g.b = b; g.c = c;
-        t.push_back(g);
-    }
-    void copy(int m, int n, bool trans, const double* a, int lda, double* c, int ldc) {
-        gemm(m, n, 0, GT_COPY | (trans ? GT_TA : 0), a, lda, nullptr, 1, c, ldc);
-    }
 };
 
 }  // namespace

@@ -165,7 +165,7 @@ struct SInst {
 //   c = op(a) op(b)  (+ c when GT_ACC), op = transpose when GT_TA / GT_TB;
 //   GT_COPY: c = op(a) (b, k unused).
 // Column-major, absolute device pointers; m x n output.
-enum { GT_TA = 1, GT_TB = 2, GT_ACC = 4, GT_COPY = 8 };
+enum { GT_TA = 1, GT_TB = 2, GT_ACC = 4, GT_COPY = 8, GT_NEG = 16 };  // GT_NEG: c = -op(a) op(b) (+ c)
 struct GemmTask {
     int32_t m, n, k, flags;
     int32_t lda, ldb, ldc, pad;
@@ -180,10 +180,33 @@ struct GemmTask {
 // test (1 / sqrt(s) for a Gaussian sample of s columns, 1 for the block itself).
 struct RrqrTask {
     double* Y;
-    double* Q;
+    double* Q;        // may be null: no basis formed
     int32_t* rank;
-    int32_t r, s, kmax, pad;
+    int32_t* perm;    // may be null; else perm[j] = the original index of column j after pivoting
+    int32_t r, s, kmax;
+    int32_t id_rule;  // 0: stop when the pivot norm <= max(atol, rtol |R11|) (compression.jl:22-30);
+                      // 1: keep |R_ii| > min(atol, rtol |R11|) (_interpolate, compression.jl:433-444)
     double scale;
+};
+
+// Interpolation matrix of one interpolative decomposition (compression.jl:441-443): R is the
+// r x s output of k_rrqr (R in rows 0..k-1, columns in pivoted order, ld r) and perm its
+// pivots; Xt (s x k, ld ldx) = X' with X(:, perm) = [I_k, R11^-1 R12].
+struct IdTask {
+    const double* R;
+    const int32_t* perm;
+    double* Xt;
+    int32_t r, s, k, ldx;
+};
+
+// dst (m x n, ld ldd) = src[ri[i] + ci[j] * ld] (ri / ci null: i / j)
+struct GatherTask {
+    const double* src;
+    const int32_t* ri;
+    const int32_t* ci;
+    double* dst;
+    int64_t ld;
+    int32_t m, n, ldd, pad;
 };
 
 // Per-request vectors of a batched refinement step.

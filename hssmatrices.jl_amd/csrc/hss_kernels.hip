@@ -2778,7 +2778,7 @@ constexpr int GT_KC = 32;    // k-chunk staged per step
 // op(b) in LDS, and wave w accumulates rows [16 w, 16 w + 16) of the tile in four 16 x 16
 // column blocks (16x16x4 steps).  Loads past the task's rows, columns or k are clamped in
 // bounds; a chunk's rows past k are staged but enter the MFMAs as zeros.
-__device__ __forceinline__ void gemm_task_tiled(const GemmTask& g, bool ta, bool tb, bool acc,
+__device__ __forceinline__ void gemm_task_tiled(const GemmTask& g, bool ta, bool tb, bool acc, double sg,
                                                 gptr<const double> a, gptr<const double> b, gptr<double> c) {
     using V4 = Mfma<double>::v4;
     __shared__ double As[GT_KC][GT_TILE + 1];  // As[l][i] = op(a)(i0 + i, k0 + l)
@@ -2818,7 +2818,7 @@ __device__ __forceinline__ void gemm_task_tiled(const GemmTask& g, bool ta, bool
                 const int i = i0 + 16 * wave + Mfma<double>::row(lk, r);
                 if (i < g.m && j < g.n) {
                     const int64_t o = i + (int64_t)j * g.ldc;
-                    c[o] = acc ? c[o] + s[cb][r] : s[cb][r];
+                    c[o] = acc ? c[o] + sg * s[cb][r] : sg * s[cb][r];
                 }
             }
         }
@@ -2835,6 +2835,7 @@ __device__ __forceinline__ void gemm_task_tiled(const GemmTask& g, bool ta, bool
 __global__ __launch_bounds__(256) void k_gemm_tasks(const GemmTask* __restrict__ tasks) {
     const GemmTask g = tasks[blockIdx.x];
     const bool ta = g.flags & GT_TA, tb = g.flags & GT_TB, acc = g.flags & GT_ACC, cp = g.flags & GT_COPY;
+    const double sg = (g.flags & GT_NEG) ? -1.0 : 1.0;
     const gptr<const double> a = gbl(g.a);
     const gptr<const double> b = gbl(g.b);
     const gptr<double> c = gbl(g.c);
@@ -2842,14 +2843,14 @@ __global__ __launch_bounds__(256) void k_gemm_tasks(const GemmTask* __restrict__
     if (cp) {
         for (int e = t0; e < g.m * g.n; e += ts) {
             const int i = e % g.m, j = e / g.m;
-            const double s = ta ? a[j + (int64_t)i * g.lda] : a[i + (int64_t)j * g.lda];
+            const double s = sg * (ta ? a[j + (int64_t)i * g.lda] : a[i + (int64_t)j * g.lda]);
             const int64_t o = i + (int64_t)j * g.ldc;
             c[o] = acc ? c[o] + s : s;
         }
         return;
     }
     if (g.m >= GT_TILE && g.n >= GT_TILE && g.k >= GT_KC) {
-        gemm_task_tiled(g, ta, tb, acc, a, b, c);
+        gemm_task_tiled(g, ta, tb, acc, sg, a, b, c);
         return;
     }
     const int mb = (g.m + 3) / 4, nb = (g.n + 3) / 4;
@@ -2885,7 +2886,7 @@ __global__ __launch_bounds__(256) void k_gemm_tasks(const GemmTask* __restrict__
             for (int r = 0; r < 4; ++r) {
                 if (i0 + r >= g.m) break;
                 const int64_t o = (i0 + r) + (int64_t)(j0 + q) * g.ldc;
-                c[o] = acc ? c[o] + s[r][q] : s[r][q];
+                c[o] = acc ? c[o] + sg * s[r][q] : sg * s[r][q];
             }
         }
     }
@@ -3005,8 +3006,11 @@ hipError_t launch_gauss(double* om, int64_t rows, int cols, uint64_t seed, hipSt
 // dgeqp3 pivoting with dlarfg reflectors), in place in global memory: step i takes the
 // column of largest remaining norm; the factorization stops before step i when that norm
 // (times t.scale) is not above max(atol, rtol * the first pivot's norm (times t.scale)), the
-// truncation rule of the reference's rank-revealing QR (compression.jl:22-30).  The rank k
-// goes to *t.rank and Q's first k columns (orthonormal, r x k, ld r) to t.Q.  The column
+// truncation rule of the reference's rank-revealing QR (compression.jl:22-30), or, with
+// t.id_rule, min(atol, rtol * the first pivot's norm), the rule of its interpolative
+// decomposition (compression.jl:439-440).  The rank k goes to *t.rank, Q's first k columns
+// (orthonormal, r x k, ld r) to t.Q (if not null), R stays in Y's first k rows (columns in
+// pivoted order) and the pivots to t.perm (if not null).  The column
 // norms below the current row are recomputed at every step (no downdating), so small
 // trailing norms keep full accuracy.  A NaN norm ends the factorization.
 // LDS: s + kmax + 8 doubles.
@@ -3021,6 +3025,9 @@ __global__ __launch_bounds__(256) void k_rrqr(const RrqrTask* __restrict__ tasks
     double* sc = tauv + kmax;      // scalars
     int* sci = reinterpret_cast<int*>(sc + 6);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    const gptr<int32_t> perm = gbl(t.perm);
+    if (t.perm)
+        for (int j = tid; j < s; j += blockDim.x) perm[j] = j;
     for (int j = wave; j < s; j += nw) {
         double part = 0.0;
         for (int i = lane; i < r; i += 64) {
@@ -3054,7 +3061,7 @@ __global__ __launch_bounds__(256) void k_rrqr(const RrqrTask* __restrict__ tasks
         __syncthreads();
         const int p = sci[0];
         const double nrm = sqrt(fmax(sc[0], 0.0)) * t.scale;
-        if (i == 0) tol = fmax(atol, rtol * nrm);
+        if (i == 0) tol = t.id_rule ? fmin(atol, rtol * nrm) : fmax(atol, rtol * nrm);
         if (!(nrm > tol)) break;  // uniform across the workgroup (LDS values, same arithmetic)
         if (p != i) {
             for (int e = tid; e < r; e += blockDim.x) {
@@ -3066,6 +3073,11 @@ __global__ __launch_bounds__(256) void k_rrqr(const RrqrTask* __restrict__ tasks
                 const double a = nrm2[i];
                 nrm2[i] = nrm2[p];
                 nrm2[p] = a;
+                if (t.perm) {
+                    const int32_t q = perm[i];
+                    perm[i] = perm[p];
+                    perm[p] = q;
+                }
             }
         }
         __syncthreads();
@@ -3113,6 +3125,10 @@ __global__ __launch_bounds__(256) void k_rrqr(const RrqrTask* __restrict__ tasks
         __syncthreads();
         k = i + 1;
     }
+    if (!t.Q) {
+        if (tid == 0) *gbl(t.rank) = k;
+        return;
+    }
     // Q[:, 0:k] = H_0 ... H_{k-1} [I_k; 0] (LAPACK dorg2r order)
     for (int e = tid; e < r * k; e += blockDim.x) Q[e] = (e % r == e / r) ? 1.0 : 0.0;
     __syncthreads();
@@ -3139,6 +3155,55 @@ hipError_t launch_rrqr(const RrqrTask* tasks, int count, int max_s, int max_kmax
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     klaunch(k_rrqr, dim3((unsigned)count), dim3(256), lds, st, tasks, atol, rtol);
+    return hipGetLastError();
+}
+
+// Interpolation matrices (one IdTask per workgroup; compression.jl:441-443): column j of
+// the pivoted R is column perm[j] of the original, so Xt[perm[j], :] = e_j for j < k and
+// Xt[perm[j], :] = (R11^-1 R(:, j))' for j >= k, by back substitution (one thread per
+// column; the solution is built in place in Xt, which only that thread touches).
+__global__ __launch_bounds__(256) void k_id_form(const IdTask* __restrict__ tasks) {
+    const IdTask t = tasks[blockIdx.x];
+    const gptr<const double> R = gbl(t.R);
+    const gptr<const int32_t> perm = gbl(t.perm);
+    const gptr<double> Xt = gbl(t.Xt);
+    for (int j = threadIdx.x; j < t.s; j += blockDim.x) {
+        const int64_t row = perm[j];
+        if (j < t.k) {
+            for (int c = 0; c < t.k; ++c) Xt[row + (int64_t)c * t.ldx] = c == j ? 1.0 : 0.0;
+            continue;
+        }
+        for (int c = t.k - 1; c >= 0; --c) {
+            double v = R[c + (int64_t)j * t.r];
+            for (int l = c + 1; l < t.k; ++l) v -= R[c + (int64_t)l * t.r] * Xt[row + (int64_t)l * t.ldx];
+            Xt[row + (int64_t)c * t.ldx] = v / R[c + (int64_t)c * t.r];
+        }
+    }
+}
+hipError_t launch_id_form(const IdTask* tasks, int count, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    klaunch(k_id_form, dim3((unsigned)count), dim3(256), 0, st, tasks);
+    return hipGetLastError();
+}
+
+// Gathers (one GatherTask per blockIdx.x, gridDim.y workgroups each): the skeleton rows of a
+// sample and the couplings A[J_rows, J_cols] of randomized compression.
+__global__ __launch_bounds__(256) void k_gather_tasks(const GatherTask* __restrict__ tasks) {
+    const GatherTask g = tasks[blockIdx.x];
+    const gptr<const double> src = gbl(g.src);
+    const gptr<const int32_t> ri = gbl(g.ri);
+    const gptr<const int32_t> ci = gbl(g.ci);
+    const gptr<double> dst = gbl(g.dst);
+    const int64_t total = (int64_t)g.m * g.n;
+    for (int64_t e = blockIdx.y * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.y * blockDim.x) {
+        const int i = (int)(e % g.m), j = (int)(e / g.m);
+        const int64_t r = g.ri ? ri[i] : i, c = g.ci ? ci[j] : j;
+        dst[i + (int64_t)j * g.ldd] = src[r + c * g.ld];
+    }
+}
+hipError_t launch_gather_tasks(const GatherTask* tasks, int count, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    klaunch(k_gather_tasks, dim3((unsigned)count, GT_SPLIT), dim3(256), 0, st, tasks);
     return hipGetLastError();
 }
 

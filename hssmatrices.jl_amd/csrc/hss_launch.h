@@ -87,6 +87,9 @@ hipError_t launch_ulv_gdown_s(const SInst<T>* insts, const SweepRec* work, int n
 hipError_t launch_gauss(double* om, int64_t rows, int cols, uint64_t seed, hipStream_t st);
 hipError_t launch_rrqr(const RrqrTask* tasks, int count, int max_s, int max_kmax, double atol, double rtol,
                        hipStream_t st);
+// interpolation matrices of k_rrqr outputs (IdTask) and gathers by index lists (GatherTask)
+hipError_t launch_id_form(const IdTask* tasks, int count, hipStream_t st);
+hipError_t launch_gather_tasks(const GatherTask* tasks, int count, hipStream_t st);
 // writes one solve record to device memory from a kernel argument (no host copy)
 hipError_t launch_put_sinst(SInst<double>* dst, const SInst<double>& v, hipStream_t st);
 // one workgroup per task (HSS x HSS product generators)

@@ -48,7 +48,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 EXPORTS = [
     "hss_ctx_create", "hss_ctx_destroy", "hss_ctx_stream",
     "hss_mat_create", "hss_mat_destroy", "hss_mat_adjoint", "hss_mat_mul", "hss_mat_size", "hss_mat_device_bytes",
-    "hss_gen_write", "hss_gen_read", "hss_mat_load", "hss_mat_save", "hss_compress_dev", "hss_mat_generators",
+    "hss_gen_write", "hss_gen_read", "hss_mat_load", "hss_mat_save", "hss_compress_dev", "hss_randcompress_dev", "hss_mat_generators",
     "hss_matmul", "hss_matmul_dev",
     "hss_ulvfactsolve", "hss_ulv_factor", "hss_fact_destroy", "hss_fact_device_bytes",
     "hss_ulv_solve", "hss_ulv_solve_dev", "hss_ulv_solve_dev_noalloc", "hss_ulv_solve_batch_dev",
@@ -110,6 +110,7 @@ def load_library(path: pathlib.Path | str | None = None):
         "hss_mat_load": ([V, ctypes.c_char_p, PV], ctypes.c_int),
         "hss_mat_save": ([V, ctypes.c_char_p], ctypes.c_int),
         "hss_compress_dev": ([V, V, I64, V, I32, D, D, PV], ctypes.c_int),
+        "hss_randcompress_dev": ([V, V, I64, V, I32, D, D, I32, I32, I32, PV, V], ctypes.c_int),
         "hss_mat_generators": ([V, V, I32, ctypes.POINTER(I32), V, I64, ctypes.POINTER(I64)], ctypes.c_int),
         "hss_matmul": ([V, I32, D, V, I64, D, V, I64], ctypes.c_int),
         "hss_matmul_dev": ([V, I32, D, V, I64, D, V, I64], ctypes.c_int),
@@ -484,6 +485,38 @@ def compress_dev(dA: int, lda: int, rcl, ccl=None, atol: float = 1e-9, rtol: flo
         lib.hss_mat_destroy(h)
         _check(st)
     return HipHssMatrix(_Adopt(h, found), ctx)
+
+
+def randcompress_dev(dA: int, lda: int, rcl, ccl=None, atol: float = 1e-9, rtol: float = 1e-9, kest: int = 10,
+                     noversampling: int = 10, stepsize: int = 20, ctx: Context | None = None,
+                     report: bool = False):
+    """Randomized adaptive HSS compression of a dense device matrix (``hss_randcompress_dev``),
+    the device counterpart of ``randcompress_adaptive(A, rcl, ccl; kest, atol, rtol,
+    noversampling, stepsize)`` (reference ``compression.jl:312-430``; defaults as the
+    reference's ``HssOptions``).  Arguments as ``compress_dev``.  With ``report`` returns
+    (matrix, the sample width it was built with)."""
+    lib = load_library()
+    ctx = ctx or default_context()
+    ccl = rcl if ccl is None else ccl
+    nodes = cluster_nodes(rcl, ccl)
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:  # pragma: no cover
+        pass
+    h = ctypes.c_void_p()
+    smp = ctypes.c_int32()
+    _check(lib.hss_randcompress_dev(ctx.handle, dA, lda, nodes.ctypes.data, len(nodes), float(atol), float(rtol),
+                                    int(kest), int(noversampling), int(stepsize), ctypes.byref(h), ctypes.byref(smp)))
+    found = np.zeros(len(nodes), dtype=NODE_DTYPE)
+    nn, bl = ctypes.c_int32(), ctypes.c_int64()
+    st = lib.hss_mat_generators(h, found.ctypes.data, len(found), ctypes.byref(nn), None, 0, ctypes.byref(bl))
+    if st != HSS_OK:
+        lib.hss_mat_destroy(h)
+        _check(st)
+    A = HipHssMatrix(_Adopt(h, found), ctx)
+    return (A, int(smp.value)) if report else A
 
 
 def ulvfactsolve(hssA: HipHssMatrix, b):

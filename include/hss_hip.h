@@ -171,6 +171,22 @@ hss_status hss_mat_save(hss_mat_t mat, const char* path);
  * reference's default tolerances are atol = rtol = 1e-9 (src/HssMatrices.jl:62-63). */
 hss_status hss_compress_dev(hss_ctx_t ctx, const double* dA, int64_t lda, const hss_node_desc* tree, int32_t num_nodes,
                             double atol, double rtol, hss_mat_t* out);
+
+/* Randomized adaptive compression of a dense matrix in device memory on one GPU:
+ * randcompress_adaptive(A, rcl, ccl; kest, atol, rtol, noversampling, stepsize)
+ * (src/compression.jl:312-355, recursion :358-430, interpolative decomposition :433-444).
+ * dA is column-major (lda >= rows), tree as for hss_compress_dev.  The matrix is sampled with
+ * kest + noversampling Gaussian columns from each side; every node's basis is an
+ * interpolative decomposition of its samples (rank = #{|R_ii| > min(atol, rtol |R_11|)}) and
+ * the couplings are entries of A at the skeleton rows and columns.  The result is tested on
+ * stepsize further columns, ||A Om - hssA Om||_F <= max(atol, rtol ||A Om||_F); on failure
+ * the sample grows by stepsize columns and the compression is redone, while kest < columns.
+ * samples (may be null) receives the sample width of the returned matrix.  The reference's
+ * defaults: kest 10, noversampling 10, stepsize 20 (src/HssMatrices.jl:62-66). */
+hss_status hss_randcompress_dev(hss_ctx_t ctx, const double* dA, int64_t lda, const hss_node_desc* tree,
+                                int32_t num_nodes, double atol, double rtol, int32_t kest, int32_t noversampling,
+                                int32_t stepsize, hss_mat_t* out, int32_t* samples);
+
 /* The matrix's node table and generator blob in host memory (the layout hss_mat_create
  * takes): the HssMatrix{Float64} itself (src/hssmatrix.jl:11-68), e.g. after
  * hss_compress_dev.  *num_nodes and *blob_len are always set; nodes and blob are each
