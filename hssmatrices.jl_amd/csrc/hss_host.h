@@ -17,5 +17,10 @@ hss_status host_validate_tree(const hss_node_desc* nodes, int nn);
 hss_status host_mat_from_device(hss_ctx_t ctx, const hss_node_desc* nodes, int nn, double** dblob, int64_t blob_len,
                                 hss_mat_t* out);
 int host_ctx_device(hss_ctx_t ctx);
+// a matrix's context, its node table (as created: ranks and blob offsets) and its
+// generator blob in device memory
+hss_ctx_t host_mat_ctx(hss_mat_t mat);
+const hss_node_desc* host_mat_nodes(hss_mat_t mat, int* nn);
+const double* host_mat_blob(hss_mat_t mat);
 
 }  // namespace hss

@@ -802,6 +802,12 @@ hss_status host_mat_from_device(hss_ctx_t ctx, const hss_node_desc* nodes, int n
     return mat_create_impl(ctx, nodes, nn, nullptr, dblob, blob_len, out);
 }
 int host_ctx_device(hss_ctx_t ctx) { return ctx ? ctx->device : -1; }
+hss_ctx_t host_mat_ctx(hss_mat_t mat) { return mat ? mat->ctx : nullptr; }
+const hss_node_desc* host_mat_nodes(hss_mat_t mat, int* nn) {
+    if (nn) *nn = mat ? mat->nn : 0;
+    return mat ? mat->desc.data() : nullptr;
+}
+const double* host_mat_blob(hss_mat_t mat) { return mat ? mat->d_blob : nullptr; }
 }  // namespace hss
 
 extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, int32_t nn, const double* blob,

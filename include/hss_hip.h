@@ -187,6 +187,13 @@ hss_status hss_randcompress_dev(hss_ctx_t ctx, const double* dA, int64_t lda, co
                                 int32_t num_nodes, double atol, double rtol, int32_t kest, int32_t noversampling,
                                 int32_t stepsize, hss_mat_t* out, int32_t* samples);
 
+/* Recompression: recompress!(hssA; atol, rtol) (src/compression.jl:148-260, with
+ * orthonormalize_generators!, src/generators.jl:64-125) as a new matrix on mat's context:
+ * the bases are made orthonormal bottom-up, then every branch's couplings and its children's
+ * bases are truncated top-down by column-pivoted QR at max(atol, rtol * the first pivot's
+ * norm).  Ranks never grow; D is copied.  For products (hss_mat_mul), whose ranks add. */
+hss_status hss_mat_recompress(hss_mat_t mat, double atol, double rtol, hss_mat_t* out);
+
 /* The matrix's node table and generator blob in host memory (the layout hss_mat_create
  * takes): the HssMatrix{Float64} itself (src/hssmatrix.jl:11-68), e.g. after
  * hss_compress_dev.  *num_nodes and *blob_len are always set; nodes and blob are each
