@@ -2914,6 +2914,17 @@ hipError_t launch_axpy(double* X, int64_t ldx, const double* D, int64_t ldd, int
     klaunch(k_axpy, dim3(blocks), dim3(256), 0, st, X, ldx, D, ldd, n, nrhs);
     return hipGetLastError();
 }
+// E[(j0 + c) + c * ld] = v for c < w (the unit columns j0 .. j0 + w - 1 of an identity,
+// written into a zeroed rows x w block, or cleared again with v = 0)
+__global__ void k_unit_cols(double* __restrict__ E, int64_t ld, int64_t j0, int w, int64_t rows, double v) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < w && j0 + c < rows) E[(j0 + c) + (int64_t)c * ld] = v;
+}
+hipError_t launch_unit_cols(double* E, int64_t ld, int64_t j0, int w, int64_t rows, double v, hipStream_t st) {
+    if (w <= 0) return hipSuccess;
+    klaunch(k_unit_cols, dim3((unsigned)((w + 255) / 256)), dim3(256), 0, st, E, ld, j0, w, rows, v);
+    return hipGetLastError();
+}
 hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, double* out, hipStream_t st) {
     if (nrhs <= 0) return hipSuccess;
     klaunch(k_colnorm2, dim3(nrhs), dim3(256), 0, st, A, lda, n, out);

@@ -98,6 +98,8 @@ hipError_t launch_resid_batch(const BatchVec* tab, int cnt, int maxc, double* no
 hipError_t launch_axpy_batch(const BatchVec* tab, int cnt, int maxc, hipStream_t st);
 hipError_t launch_residual(const double* B, int64_t ldb, double* R, int64_t ldr, int64_t n, int nrhs, hipStream_t st);
 hipError_t launch_axpy(double* X, int64_t ldx, const double* D, int64_t ldd, int64_t n, int nrhs, hipStream_t st);
+// unit columns j0 .. j0 + w - 1 of an identity into a zeroed rows x w block (v = 1), or cleared (v = 0)
+hipError_t launch_unit_cols(double* E, int64_t ld, int64_t j0, int w, int64_t rows, double v, hipStream_t st);
 hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, double* out, hipStream_t st);
 
 }  // namespace hss

@@ -48,7 +48,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 EXPORTS = [
     "hss_ctx_create", "hss_ctx_destroy", "hss_ctx_stream",
     "hss_mat_create", "hss_mat_destroy", "hss_mat_adjoint", "hss_mat_mul", "hss_mat_size", "hss_mat_device_bytes",
-    "hss_gen_write", "hss_gen_read", "hss_mat_load", "hss_mat_save", "hss_compress_dev", "hss_randcompress_dev", "hss_mat_recompress", "hss_mat_generators",
+    "hss_gen_write", "hss_gen_read", "hss_mat_load", "hss_mat_save", "hss_compress_dev", "hss_randcompress_dev", "hss_mat_recompress", "hss_mat_ldiv", "hss_mat_rdiv", "hss_mat_generators",
     "hss_matmul", "hss_matmul_dev",
     "hss_ulvfactsolve", "hss_ulv_factor", "hss_fact_destroy", "hss_fact_device_bytes",
     "hss_ulv_solve", "hss_ulv_solve_dev", "hss_ulv_solve_dev_noalloc", "hss_ulv_solve_batch_dev",
@@ -112,6 +112,8 @@ def load_library(path: pathlib.Path | str | None = None):
         "hss_compress_dev": ([V, V, I64, V, I32, D, D, PV], ctypes.c_int),
         "hss_randcompress_dev": ([V, V, I64, V, I32, D, D, I32, I32, I32, PV, V], ctypes.c_int),
         "hss_mat_recompress": ([V, D, D, PV], ctypes.c_int),
+        "hss_mat_ldiv": ([V, V, D, D, PV], ctypes.c_int),
+        "hss_mat_rdiv": ([V, V, D, D, PV], ctypes.c_int),
         "hss_mat_generators": ([V, V, I32, ctypes.POINTER(I32), V, I64, ctypes.POINTER(I64)], ctypes.c_int),
         "hss_matmul": ([V, I32, D, V, I64, D, V, I64], ctypes.c_int),
         "hss_matmul_dev": ([V, I32, D, V, I64, D, V, I64], ctypes.c_int),
@@ -374,6 +376,28 @@ class HipHssMatrix:
             _lib.hss_mat_destroy(h)
             _check(st)
         return HipHssMatrix(_Adopt(h, nodes), self.ctx)
+
+    def _adopted(self, h) -> "HipHssMatrix":
+        nodes = np.zeros(len(self.nodes), dtype=NODE_DTYPE)
+        nn, bl = ctypes.c_int32(), ctypes.c_int64()
+        st = _lib.hss_mat_generators(h, nodes.ctypes.data, len(nodes), ctypes.byref(nn), None, 0, ctypes.byref(bl))
+        if st != HSS_OK:
+            _lib.hss_mat_destroy(h)
+            _check(st)
+        return HipHssMatrix(_Adopt(h, nodes), self.ctx)
+
+    def hss_ldiv(self, B: "HipHssMatrix", atol: float = 1e-9, rtol: float = 1e-9) -> "HipHssMatrix":
+        """``hssA \\ hssB`` (``hssmatrix.jl:235``, ``ulvdivide.jl``): A^-1 B as an HSS matrix
+        (``hss_mat_ldiv``)."""
+        h = ctypes.c_void_p()
+        _check(_lib.hss_mat_ldiv(self._h, B._h, float(atol), float(rtol), ctypes.byref(h)))
+        return self._adopted(h)
+
+    def hss_rdiv(self, B: "HipHssMatrix", atol: float = 1e-9, rtol: float = 1e-9) -> "HipHssMatrix":
+        """``hssA / hssB`` (``hssmatrix.jl:237``): A B^-1 as an HSS matrix (``hss_mat_rdiv``)."""
+        h = ctypes.c_void_p()
+        _check(_lib.hss_mat_rdiv(self._h, B._h, float(atol), float(rtol), ctypes.byref(h)))
+        return self._adopted(h)
 
     def matmul_dev(self, nrhs: int, alpha: float, dB: int, ldb: int, beta: float, dC: int, ldc: int):
         """``hss_matmul_dev`` on device pointers (e.g. ``torch.Tensor.data_ptr()``)."""

@@ -194,6 +194,16 @@ hss_status hss_randcompress_dev(hss_ctx_t ctx, const double* dA, int64_t lda, co
  * norm).  Ranks never grow; D is copied.  For products (hss_mat_mul), whose ranks add. */
 hss_status hss_mat_recompress(hss_mat_t mat, double atol, double rtol, hss_mat_t* out);
 
+/* HSS right-hand sides: \(hssA, hssB) and /(hssA, hssB) (src/hssmatrix.jl:235-237; the
+ * reference's ULV divide, ldiv! / rdiv!, src/ulvdivide.jl:13-54) as new matrices on a's
+ * context.  ldiv: A^-1 B, compressed at (atol, rtol) over A's column tree and B's column
+ * tree; A square, its column tree the shape of B's row tree (HSS_ERR_DIM otherwise).  A is
+ * factored once (Float64, resident) and A^-1 B is formed densely in device memory
+ * (8 rows(B) cols(B) bytes, HSS_ERR_OOM if it does not fit) from B's matvec and the ULV
+ * solve, then compressed as by hss_compress_dev.  rdiv: A B^-1 = (B' \ A')'. */
+hss_status hss_mat_ldiv(hss_mat_t a, hss_mat_t b, double atol, double rtol, hss_mat_t* out);
+hss_status hss_mat_rdiv(hss_mat_t a, hss_mat_t b, double atol, double rtol, hss_mat_t* out);
+
 /* The matrix's node table and generator blob in host memory (the layout hss_mat_create
  * takes): the HssMatrix{Float64} itself (src/hssmatrix.jl:11-68), e.g. after
  * hss_compress_dev.  *num_nodes and *blob_len are always set; nodes and blob are each
