@@ -241,6 +241,33 @@ function *(A::HipHss, B::HipHss)
 end
 /(X::Matrix{Float64}, A::HipHss) = copy(ulvfactsolve(A', collect(X'))')
 
+# a device call that returns a new matrix handle of size m x n
+function adopt(A::HipHss, r::Ref{Ptr{Cvoid}}, m::Integer, n::Integer)
+    C = HipHss(r[], m, n, A.ctx)
+    finalizer(C -> ccall((:hss_mat_destroy, libhss), Cint, (Ptr{Cvoid},), C.h), C)
+    return C
+end
+# hssA \ hssB and hssA / hssB (src/hssmatrix.jl:235-237), at the tolerances of HssOptions
+function \(A::HipHss, B::HipHss; atol::Float64=HssOptions(Float64).atol, rtol::Float64=HssOptions(Float64).rtol)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hss_mat_ldiv, libhss), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Float64, Float64, Ptr{Ptr{Cvoid}}),
+                A.h, B.h, atol, rtol, r))
+    return adopt(A, r, A.n, B.n)
+end
+function /(A::HipHss, B::HipHss; atol::Float64=HssOptions(Float64).atol, rtol::Float64=HssOptions(Float64).rtol)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hss_mat_rdiv, libhss), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Float64, Float64, Ptr{Ptr{Cvoid}}),
+                A.h, B.h, atol, rtol, r))
+    return adopt(A, r, A.m, B.m)
+end
+# recompress!(hssA; atol, rtol) (src/compression.jl:148-203) as a new device matrix
+function recompress(A::HipHss; atol::Float64=HssOptions(Float64).atol, rtol::Float64=HssOptions(Float64).rtol)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hss_mat_recompress, libhss), Cint, (Ptr{Cvoid}, Float64, Float64, Ptr{Ptr{Cvoid}}),
+                A.h, atol, rtol, r))
+    return adopt(A, r, A.m, A.n)
+end
+
 mutable struct HipUlv
     h::Ptr{Cvoid}
     A::HipHss
@@ -262,6 +289,6 @@ function \(F::HipUlv, B::Matrix{Float64})
 end
 
 # ulvfactsolve is not exported again: HssMatrices exports it, and the methods above extend it
-export HipHss, ulvfactor
+export HipHss, ulvfactor, recompress
 
 end # module
