@@ -326,6 +326,60 @@ def batched(hip, W, torch, steps=10, warmup=2, count=256, seed=3, log=print):
     return out
 
 
+def construction(hip, W, torch, log=print, n_div=16384):
+    """Building HSS matrices on the device: the n = 65536 log-kernel matrix compressed directly
+    (hss_compress_dev) and by randomized adaptive compression (hss_randcompress_dev, the
+    reference's defaults kest 10, noversampling 10, stepsize 20), each timed from the dense
+    matrix in HBM (its formation by torch excluded) and checked on 4 random vectors against the
+    dense matrix; and the HSS divide A \\ A (hss_mat_ldiv) of an n = n_div log-kernel matrix,
+    which should be the identity."""
+    from hssmatrices_jl_amd.compression import KernelMatrix
+    from hssmatrices_jl_amd.hssmatrix import bisection_cluster
+    out = {}
+    x = W.log_points(N_POINTS, 0)
+    K = KernelMatrix(x, "log", 1.0)
+    T = K.device_dense()
+    cl = bisection_cluster(N_POINTS, LEAF)
+    g = torch.Generator(device="cuda:0").manual_seed(5)
+    Xv = torch.randn((4, N_POINTS), dtype=torch.float64, device="cuda:0", generator=g)
+    Yref = (Xv @ T).cpu().numpy().T  # T holds A column-major as a row-major (n, n) tensor: x' A' = (A x)'
+    Xh = Xv.cpu().numpy().T
+    hssrank = lambda nd: int(max(nd["ku"][1:].max(), nd["kv"][1:].max()))
+    for name, fn in (("direct", lambda: hip.compress_dev(T.data_ptr(), N_POINTS, cl, cl, TOL, TOL)),
+                     ("randomized", lambda: hip.randcompress_dev(T.data_ptr(), N_POINTS, cl, cl, TOL, TOL,
+                                                                 report=True))):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        M, smp = (r if isinstance(r, tuple) else (r, None))
+        nd, _ = M.generators()
+        err = float(np.linalg.norm(M @ Xh - Yref) / np.linalg.norm(Yref))
+        out[name] = {"ms": round(ms, 1), "hssrank": hssrank(nd), "rel_err_on_4_vectors": err}
+        if smp is not None:
+            out[name]["samples"] = smp
+        log(f"[bench] {name} compression n={N_POINTS}: {ms:.0f} ms, hssrank {hssrank(nd)}, rel err {err:.2e}"
+            + (f", samples {smp}" if smp is not None else ""))
+        M.close()
+    del T, Xv
+    torch.cuda.empty_cache()
+    Ad = W.log_kernel(hip, n_div, LEAF, TOL, 7)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Q = Ad.hss_ldiv(Ad, TOL, TOL)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    Xd = np.random.default_rng(8).standard_normal((n_div, 4))
+    e = float(np.linalg.norm(Q @ Xd - Xd) / np.linalg.norm(Xd))
+    out["divide"] = {"case": f"A \\ A, log kernel n={n_div} (the identity)", "ms": round(ms, 1),
+                     "hssrank": hssrank(Q.generators()[0]), "rel_err_vs_identity": e}
+    log(f"[bench] divide {out['divide']}")
+    Q.close()
+    Ad.close()
+    return out
+
+
 def hip_summary(m):
     k = np.concatenate([m.nodes["ku"][1:], m.nodes["kv"][1:]])
     return int(k.min()), int(k.max())
@@ -498,6 +552,8 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=8, help="1-thread CPU oracle calls in the baseline sample")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances (C4) measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (C5) measurement")
+    ap.add_argument("--no-construct", action="store_true",
+                    help="skip the construction measurements (device compression, randomized compression, divide)")
     ap.add_argument("--profile-only", action="store_true",
                     help="for rocprofv3: a few fp64 solves, matvecs and one plain 256 MiB device copy "
                          "(the copy calibrates FETCH_SIZE/WRITE_SIZE), then exit")
@@ -783,10 +839,20 @@ def main():
     Yc = Cp.cpu().numpy().T
     Yaa = A @ (A @ B_host)
     prod_rel = float(np.linalg.norm(Yc - Yaa) / np.linalg.norm(Yaa))
+    # recompression of the product (ranks doubled) at the compression tolerance
+    hssrank = lambda nd: int(max(nd["ku"][1:].max(), nd["kv"][1:].max()))
+    t0 = time.perf_counter()
+    Crec = C.recompress(TOL, TOL)
+    rec_ms = (time.perf_counter() - t0) * 1e3
+    rec = {"ms": round(rec_ms, 1), "hssrank_before": hssrank(C.nodes), "hssrank_after": hssrank(Crec.nodes),
+           "hssrank_A": hssrank(nodes),
+           "rel_diff_vs_A_A_x": float(np.linalg.norm(Crec @ B_host[:, :4] - Yaa[:, :4]) /
+                                      np.linalg.norm(Yaa[:, :4]))}
+    Crec.close()
     C.close()
     del Cp
     log(f"[bench] HSS product A A: {prod_ms:.1f} ms wall ({prod_dev_ms:.2f} ms of device tasks); its matvec "
-        f"{prod_mv_ms:.3f} ms; vs A (A X): {prod_rel:.1e}")
+        f"{prod_mv_ms:.3f} ms; vs A (A X): {prod_rel:.1e}; recompressed {rec}")
 
     # ---- per-kernel timing (HIP events around each launch on the library stream) ----
     prof_reps = 3
@@ -884,6 +950,7 @@ def main():
         log(f"[bench] cpu baseline {cpu}")
     log(f"[bench] parity {parity}")
 
+    construct = None if args.no_construct else construction(hip, W, torch, log=log)
     batch = None if args.no_batch else batched(hip, W, torch, log=log)
     serve = None if args.no_serve else serving(hip, W, torch, log=log)
 
@@ -941,7 +1008,8 @@ def main():
                     "fp64_solve_ms": round(adj_solve_ms, 4), "fp64_solve_rel_residual": adj_res},
         "hss_product": {"case": "A A (ranks doubled)", "wall_ms": round(prod_ms, 1),
                         "device_task_ms": round(prod_dev_ms, 3), "product_matvec_ms": round(prod_mv_ms, 4),
-                        "rel_diff_vs_A_A_x": prod_rel},
+                        "rel_diff_vs_A_A_x": prod_rel, "recompressed": rec},
+        "construction": construct,
         "generator_file": {"bytes": int(gen_bytes), "save_ms": round(gen_save_ms, 1),
                            "load_ms": round(gen_load_ms, 1), "loaded_matvec_identical": gen_same},
         "roofline": {"kernel": f"{dk} (level {dlev}, {dnw} nodes)", "bound": "hbm",
