@@ -2758,6 +2758,56 @@ hipError_t launch_ulv_gdown(const SInst<T>* insts, const SweepRec* work, int nwo
     return hipGetLastError();
 }
 
+// The convergence test of one lockstep refinement step, on the device (one workgroup):
+// request i is active while tab[i].nrhs > 0; its relative residual is
+// max_j ||r_j|| / ||b_j|| from norms (squared, as k_resid_batch leaves them), written to rel[i];
+// at or below tol the request is retired by setting its column count to 0 in every table
+// the step's launches read (the two solve tables, the matvec table and tab), so they skip it.
+// out[0] = the requests still active.
+__global__ __launch_bounds__(256) void k_refine_check(BatchVec* __restrict__ tab, char* __restrict__ sinst0,
+                                                       char* __restrict__ sinst1, int64_t sinst_size, int nrhs_off,
+                                                       MvInst* __restrict__ mv, int cnt, int maxc,
+                                                       const double* __restrict__ norms, double tol,
+                                                       double* __restrict__ rel, int* __restrict__ out) {
+    int active = 0;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const int nr = tab[i].nrhs;
+        if (nr <= 0) continue;
+        double worst = 0.0;
+        for (int j = 0; j < nr; ++j) {
+            const double nb = sqrt(norms[2 * ((int64_t)i * maxc + j)]);
+            const double rr = sqrt(norms[2 * ((int64_t)i * maxc + j) + 1]);
+            const double q = nb > 0.0 ? rr / nb : rr;
+            worst = q > worst || q != q ? q : worst;  // a NaN residual stays the worst
+        }
+        rel[i] = worst;
+        if (worst <= tol) {
+            tab[i].nrhs = 0;
+            mv[i].nrhs = 0;
+            if (sinst0) *reinterpret_cast<int32_t*>(sinst0 + (int64_t)i * sinst_size + nrhs_off) = 0;
+            if (sinst1) *reinterpret_cast<int32_t*>(sinst1 + (int64_t)i * sinst_size + nrhs_off) = 0;
+        } else {
+            ++active;
+        }
+    }
+    __shared__ int red[16];
+    for (int o = 32; o >= 1; o >>= 1) active += __shfl_xor(active, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = active;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+        out[0] = t;
+    }
+}
+hipError_t launch_refine_check(BatchVec* tab, char* sinst0, char* sinst1, int64_t sinst_size, int nrhs_off,
+                               MvInst* mv, int cnt, int maxc, const double* norms, double tol, double* rel, int* out,
+                               hipStream_t st) {
+    if (cnt <= 0) return hipSuccess;
+    klaunch(k_refine_check, dim3(1), dim3(256), 0, st, tab, sinst0, sinst1, sinst_size, nrhs_off, mv, cnt, maxc,
+            norms, tol, rel, out);
+    return hipGetLastError();
+}
 hipError_t launch_resid_batch(const BatchVec* tab, int cnt, int maxc, double* norms, hipStream_t st) {
     if (cnt <= 0 || maxc <= 0) return hipSuccess;
     klaunch(k_resid_batch, dim3(cnt, maxc), dim3(256), 0, st, tab, maxc, norms);

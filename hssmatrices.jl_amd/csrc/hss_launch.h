@@ -94,6 +94,10 @@ hipError_t launch_gather_tasks(const GatherTask* tasks, int count, hipStream_t s
 hipError_t launch_put_sinst(SInst<double>* dst, const SInst<double>& v, hipStream_t st);
 // one workgroup per task (HSS x HSS product generators)
 hipError_t launch_gemm_tasks(const GemmTask* tasks, int count, hipStream_t st);
+// the convergence test of a lockstep refinement step on the device (k_refine_check)
+hipError_t launch_refine_check(BatchVec* tab, char* sinst0, char* sinst1, int64_t sinst_size, int nrhs_off,
+                               MvInst* mv, int cnt, int maxc, const double* norms, double tol, double* rel, int* out,
+                               hipStream_t st);
 hipError_t launch_resid_batch(const BatchVec* tab, int cnt, int maxc, double* norms, hipStream_t st);
 hipError_t launch_axpy_batch(const BatchVec* tab, int cnt, int maxc, hipStream_t st);
 hipError_t launch_residual(const double* B, int64_t ldb, double* R, int64_t ldr, int64_t n, int nrhs, hipStream_t st);

@@ -2325,6 +2325,12 @@ struct hss_plan_s {
     bool all_groot2 = true;
     double* d_norms = nullptr;
     std::vector<double> h_norms;
+    // device-side convergence test of lockstep refinement: per-request residuals, the
+    // active count of each step (a ring, copied to pinned host memory) and its events
+    double* d_rel = nullptr;
+    int* d_count = nullptr;
+    int* h_count = nullptr;  // pinned
+    std::vector<hipEvent_t> count_ev;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int64_t dev_bytes = 0;
@@ -2343,6 +2349,10 @@ static void free_plan(hss_plan_s* P) {
     (void)hipFree(P->d_work);
     (void)hipFree(P->d_rec);
     (void)hipFree(P->d_norms);
+    (void)hipFree(P->d_rel);
+    (void)hipFree(P->d_count);
+    if (P->h_count) (void)hipHostFree(P->h_count);
+    for (hipEvent_t e : P->count_ev) (void)hipEventDestroy(e);
     delete P;
 }
 
@@ -2722,38 +2732,69 @@ extern "C" hss_status hss_plan_execute(hss_plan_t P, const hss_refine_opts* opts
         HIP_TRY(hipEventRecord(P->ctx->ev1, st));
         return HSS_OK;
     }
-    std::vector<char> active(P->cnt, 1);
-    double worst = 0.0;
-    int it = 0;
-    for (;;) {
-        s = plan_matvec(P);  // r = A x
-        if (s != HSS_OK) return s;
+    // Lockstep refinement with the convergence test on the device: each step is
+    //   check k:  r = b - A x (batched Float64 matvec), norms, k_refine_check (retires the
+    //             requests at tol in the device tables, counts the others)
+    //   solve k:  d = A \ r for the active requests, x += d
+    // The host reads check k's count (pinned memory, event) only after enqueuing check k + 1,
+    // so the device never waits for the host; a step enqueued after the last request
+    // retired finds every column count 0 and its launches do nothing.
+    constexpr int kRing = 64;
+    if (!P->d_rel) {
+        HIP_TRY(dalloc(&P->d_rel, (size_t)P->cnt));
+        HIP_TRY(hipMemsetAsync(P->d_rel, 0, sizeof(double) * P->cnt, st));
+        HIP_TRY(dalloc(&P->d_count, (size_t)kRing));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&P->h_count), sizeof(int) * kRing, hipHostMallocDefault));
+        P->count_ev.resize(kRing);
+        for (auto& e : P->count_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    const size_t nrhs_off = fp32 ? offsetof(SInst<float>, nrhs) : offsetof(SInst<double>, nrhs);
+    auto check = [&](int k) -> hss_status {
+        hss_status c = plan_matvec(P);  // r = A x
+        if (c != HSS_OK) return c;
         HIP_TRY(launch_resid_batch(P->d_vec, P->cnt, P->maxnrhs, P->d_norms, st));
-        HIP_TRY(hipMemcpyAsync(P->h_norms.data(), P->d_norms, sizeof(double) * P->h_norms.size(),
-                               hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        worst = 0.0;
-        bool any = false;
-        for (int i = 0; i < P->cnt; ++i) {
-            if (!active[i]) continue;
-            double rel = 0.0;
-            for (int j = 0; j < P->reqs[i].nrhs; ++j) {
-                const double nb = std::sqrt(P->h_norms[2 * ((size_t)i * P->maxnrhs + j)]);
-                const double nr = std::sqrt(P->h_norms[2 * ((size_t)i * P->maxnrhs + j) + 1]);
-                rel = std::max(rel, nb > 0 ? nr / nb : nr);
+        const int slot = k % kRing;
+        HIP_TRY(launch_refine_check(P->d_vec, P->d_sinst[0], P->d_sinst[1], (int64_t)P->sinst_size, (int)nrhs_off,
+                                    P->d_mv, P->cnt, P->maxnrhs, P->d_norms, opts->tol, P->d_rel,
+                                    P->d_count + slot, st));
+        HIP_TRY(hipMemcpyAsync(P->h_count + slot, P->d_count + slot, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipEventRecord(P->count_ev[slot], st));
+        return HSS_OK;
+    };
+    auto count_of = [&](int k, int& c) -> hss_status {
+        HIP_TRY(hipEventSynchronize(P->count_ev[k % kRing]));
+        c = P->h_count[k % kRing];
+        return HSS_OK;
+    };
+    int it = 0, remaining = -1;
+    for (int k = 0;; ++k) {
+        if ((s = check(k)) != HSS_OK) return s;
+        if (k >= 1) {
+            int c = 0;
+            if ((s = count_of(k - 1, c)) != HSS_OK) return s;
+            if (c == 0) {  // retired by check k - 1: solve k - 1 and check k did nothing
+                it = k - 1;
+                remaining = 0;
+                break;
             }
-            worst = std::max(worst, rel);
-            if (rel <= opts->tol) active[i] = 0;
-            else any = true;
         }
-        if (!any || it >= opts->max_iter) break;
-        s = plan_set_active(P, active);
-        if (s != HSS_OK) return s;
+        if (k >= opts->max_iter) {
+            if ((s = count_of(k, remaining)) != HSS_OK) return s;
+            it = k;
+            break;
+        }
         s = fp32 ? plan_solve<float>(P, 1) : plan_solve<double>(P, 1);  // d = A \ r
         if (s != HSS_OK) return s;
         HIP_TRY(launch_axpy_batch(P->d_vec, P->cnt, P->maxnrhs, st));  // x += d
-        ++it;
     }
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<double> hrel(P->cnt);
+    HIP_TRY(hipMemcpy(hrel.data(), P->d_rel, sizeof(double) * P->cnt, hipMemcpyDeviceToHost));
+    double worst = 0.0;
+    for (int i = 0; i < P->cnt; ++i)
+        if (P->reqs[i].nrhs > 0) worst = hrel[i] > worst || hrel[i] != hrel[i] ? hrel[i] : worst;
+    // the device tables retired requests on their own: upload the full activity again
+    P->uploaded_active.assign(P->cnt, 2);
     // restore all requests as active for the next execution
     std::vector<char> all(P->cnt, 1);
     s = plan_set_active(P, all);
@@ -2767,7 +2808,7 @@ extern "C" hss_status hss_plan_execute(hss_plan_t P, const hss_refine_opts* opts
         rep->rel_residual = worst;
         rep->seconds = ms * 1e-3;
     }
-    if (worst > opts->tol) {
+    if (remaining != 0 || !(worst <= opts->tol)) {
         char buf[160];
         std::snprintf(buf, sizeof buf, "batched refinement stopped at relative residual %.3e > tol %.3e after %d steps",
                       worst, opts->tol, it);
