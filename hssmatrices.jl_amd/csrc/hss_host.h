@@ -22,5 +22,7 @@ int host_ctx_device(hss_ctx_t ctx);
 hss_ctx_t host_mat_ctx(hss_mat_t mat);
 const hss_node_desc* host_mat_nodes(hss_mat_t mat, int* nn);
 const double* host_mat_blob(hss_mat_t mat);
+// rows of a factor's matrix
+int64_t host_fact_rows(hss_fact_t fact);
 
 }  // namespace hss

@@ -808,6 +808,7 @@ const hss_node_desc* host_mat_nodes(hss_mat_t mat, int* nn) {
     return mat ? mat->desc.data() : nullptr;
 }
 const double* host_mat_blob(hss_mat_t mat) { return mat ? mat->d_blob : nullptr; }
+int64_t host_fact_rows(hss_fact_t fact) { return fact && fact->mat ? fact->mat->rows : 0; }
 }  // namespace hss
 
 extern "C" hss_status hss_mat_create(hss_ctx_t ctx, const hss_node_desc* nodes, int32_t nn, const double* blob,

@@ -49,6 +49,7 @@ EXPORTS = [
     "hss_ctx_create", "hss_ctx_destroy", "hss_ctx_stream",
     "hss_mat_create", "hss_mat_destroy", "hss_mat_adjoint", "hss_mat_mul", "hss_mat_size", "hss_mat_device_bytes",
     "hss_gen_write", "hss_gen_read", "hss_mat_load", "hss_mat_save", "hss_compress_dev", "hss_randcompress_dev", "hss_mat_recompress", "hss_mat_ldiv", "hss_mat_rdiv", "hss_mat_generators",
+    "hss_mat_create_f32", "hss_matmul_f32", "hss_ulvfactsolve_f32", "hss_ulv_solve_f32",
     "hss_matmul", "hss_matmul_dev",
     "hss_ulvfactsolve", "hss_ulv_factor", "hss_fact_destroy", "hss_fact_device_bytes",
     "hss_ulv_solve", "hss_ulv_solve_dev", "hss_ulv_solve_dev_noalloc", "hss_ulv_solve_batch_dev",
@@ -113,6 +114,10 @@ def load_library(path: pathlib.Path | str | None = None):
         "hss_randcompress_dev": ([V, V, I64, V, I32, D, D, I32, I32, I32, PV, V], ctypes.c_int),
         "hss_mat_recompress": ([V, D, D, PV], ctypes.c_int),
         "hss_mat_ldiv": ([V, V, D, D, PV], ctypes.c_int),
+        "hss_mat_create_f32": ([V, V, I32, V, I64, PV], ctypes.c_int),
+        "hss_matmul_f32": ([V, I32, ctypes.c_float, V, I64, ctypes.c_float, V, I64], ctypes.c_int),
+        "hss_ulvfactsolve_f32": ([V, I32, V, I64, V, I64], ctypes.c_int),
+        "hss_ulv_solve_f32": ([V, I32, V, I64, V, I64], ctypes.c_int),
         "hss_mat_rdiv": ([V, V, D, D, PV], ctypes.c_int),
         "hss_mat_generators": ([V, V, I32, ctypes.POINTER(I32), V, I64, ctypes.POINTER(I64)], ctypes.c_int),
         "hss_matmul": ([V, I32, D, V, I64, D, V, I64], ctypes.c_int),
@@ -281,10 +286,16 @@ class HipHssMatrix:
         else:
             nodes, blob = hss
         self.nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
-        blob = np.ascontiguousarray(blob, dtype=np.float64)
         h = ctypes.c_void_p()
-        _check(lib.hss_mat_create(self.ctx.handle, self.nodes.ctypes.data, len(self.nodes), blob.ctypes.data,
-                                  blob.size, ctypes.byref(h)))
+        if isinstance(blob, np.ndarray) and blob.dtype == np.float32:
+            # HssMatrix{Float32} generators (hss_mat_create_f32: widened to Float64 on creation)
+            blob = np.ascontiguousarray(blob)
+            _check(lib.hss_mat_create_f32(self.ctx.handle, self.nodes.ctypes.data, len(self.nodes),
+                                          blob.ctypes.data, blob.size, ctypes.byref(h)))
+        else:
+            blob = np.ascontiguousarray(blob, dtype=np.float64)
+            _check(lib.hss_mat_create(self.ctx.handle, self.nodes.ctypes.data, len(self.nodes), blob.ctypes.data,
+                                      blob.size, ctypes.byref(h)))
         self._h = h
         self.shape = (int(self.nodes["m"][0]), int(self.nodes["n"][0]))
 
@@ -341,6 +352,8 @@ class HipHssMatrix:
         product ``hssA * hssB`` (``matmul.jl:65-118``, ``hss_mat_mul``) as a new one."""
         if isinstance(B, HipHssMatrix):
             return self.hss_mul(B)
+        if isinstance(B, np.ndarray) and B.dtype == np.float32:
+            return _f32_call(self, B, "matmul")
         Bf, vec = _as_f64_fortran(B)
         if Bf.shape[0] != self.shape[1]:
             raise DimensionMismatch(
@@ -451,7 +464,35 @@ class HipHssMatrix:
 
     def solve(self, B):
         """``hssA \\ B`` (reference ``hssmatrix.jl:234`` -> ``ulvfactsolve``)."""
+        if isinstance(B, np.ndarray) and B.dtype == np.float32:
+            return _f32_call(self, B, "solve")
         return ulvfactsolve(self, B)
+
+
+def _f32_call(A, B, what: str, fact=None):
+    """Float32 data at the boundary (``hss_matmul_f32`` / ``hss_ulvfactsolve_f32`` /
+    ``hss_ulv_solve_f32``): the device computes in Float64, the result is Float32."""
+    vec = B.ndim == 1
+    Bf = np.asfortranarray(B.reshape(-1, 1) if vec else B, dtype=np.float32)
+    rows = A.shape[0]
+    if what == "matmul":
+        if Bf.shape[0] != A.shape[1]:
+            raise DimensionMismatch(f"First dimension of B does not match second dimension of A. Expected "
+                                    f"{A.shape[1]}, got {Bf.shape[0]}")
+        C = np.zeros((rows, Bf.shape[1]), dtype=np.float32, order="F")
+        _check(_lib.hss_matmul_f32(A.handle, Bf.shape[1], 1.0, Bf.ctypes.data, max(Bf.shape[0], 1), 0.0,
+                                   C.ctypes.data, max(rows, 1)))
+    else:
+        if Bf.shape[0] != rows:
+            raise DimensionMismatch(f"matrix is {A.shape}, right-hand side has {Bf.shape[0]} rows")
+        C = np.zeros((A.shape[1], Bf.shape[1]), dtype=np.float32, order="F")
+        if fact is None:
+            _check(_lib.hss_ulvfactsolve_f32(A.handle, Bf.shape[1], Bf.ctypes.data, max(rows, 1), C.ctypes.data,
+                                             max(A.shape[1], 1)))
+        else:
+            _check(_lib.hss_ulv_solve_f32(fact, Bf.shape[1], Bf.ctypes.data, max(rows, 1), C.ctypes.data,
+                                          max(A.shape[1], 1)))
+    return C[:, 0] if vec else C
 
 
 def gen_write(path, nodes, blob) -> None:
@@ -619,7 +660,10 @@ class HipUlvFactor:
             pass
 
     def solve(self, b, tol: float | None = None, max_iter: int | None = None, report: bool = False):
-        """X = A \\ b with the resident factor; FP32 factors refine to ``tol`` (default 1e-10)."""
+        """X = A \\ b with the resident factor; FP32 factors refine to ``tol`` (default 1e-10).
+        Float32 data (a numpy float32 array) go through ``hss_ulv_solve_f32``."""
+        if isinstance(b, np.ndarray) and b.dtype == np.float32 and tol is None and max_iter is None and not report:
+            return _f32_call(self.mat, b, "solve", fact=self._h)
         bf, vec = _as_f64_fortran(b)
         n = self.mat.shape[0]
         if bf.shape[0] != n:

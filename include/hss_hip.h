@@ -314,6 +314,17 @@ hss_status hss_server_destroy(hss_server_t server);
 /* Batched factorization of many matrices (one launch per level across all of them). */
 hss_status hss_ulv_factor_batch(int32_t count, const hss_mat_t* mats, hss_prec prec, hss_fact_t* out);
 
+/* ---- Float32 at the boundary (the reference's hot path for T = Float32, test/runtests.jl:16) ----
+ * HssMatrix{Float32} generators (the same node table, a Float32 blob) and Float32 host data.
+ * The device computes in Float64: generators and inputs are widened, results rounded once
+ * to Float32.  Otherwise as the Float64 entry points. */
+hss_status hss_mat_create_f32(hss_ctx_t ctx, const hss_node_desc* nodes, int32_t num_nodes, const float* blob,
+                              int64_t blob_len, hss_mat_t* out);
+hss_status hss_matmul_f32(hss_mat_t mat, int32_t nrhs, float alpha, const float* B, int64_t ldb, float beta, float* C,
+                          int64_t ldc);
+hss_status hss_ulvfactsolve_f32(hss_mat_t mat, int32_t nrhs, const float* B, int64_t ldb, float* X, int64_t ldx);
+hss_status hss_ulv_solve_f32(hss_fact_t fact, int32_t nrhs, const float* B, int64_t ldb, float* X, int64_t ldx);
+
 /* ---- diagnostics ---- */
 const char* hss_last_error(void);
 /* Per-launch timing of the library's kernels (no counterpart in the reference;
