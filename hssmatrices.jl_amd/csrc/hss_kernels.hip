@@ -2442,6 +2442,148 @@ __global__ __launch_bounds__(256) void k_mv_t(const MvInst* __restrict__ insts, 
 }
 
 // ---------------------------------------------------------------------------
+// tree cut: one dense product in place of the sweep steps above a depth
+// ---------------------------------------------------------------------------
+struct CutArgs {
+    int row0, rows, K, lda;
+    int64_t T, xrow, yrow;
+};
+__device__ __forceinline__ CutArgs cut_args(const MvRec& r) { return CutArgs{r.r0, r.M, r.K, r.ld0, r.A, r.x0, r.x1}; }
+__device__ __forceinline__ CutArgs cut_args(const SweepRec& r) {
+    return CutArgs{r.r0, r.My, r.K, r.l0, r.A, r.x0, r.x1};
+}
+__device__ __forceinline__ double* cut_ws(const MvInst& I) { return I.Z; }
+template <typename TF>
+__device__ __forceinline__ TF* cut_ws(const SInst<TF>& I) { return I.wy; }
+
+constexpr int CUT_TW = 16;       // right-hand-side columns per workgroup
+constexpr int CUT_THREADS = 512;
+constexpr int CUT_MAXK = 1024;   // largest K (the stacked input is held in LDS)
+constexpr int CUT_MAXS = (CUT_MAXK / 4 + CUT_THREADS / 64 - 1) / (CUT_THREADS / 64);  // k-steps per wave
+constexpr int CUT_RPT = (CUT_MAXK + CUT_THREADS - 1) / CUT_THREADS;                   // input rows per thread
+
+// One workgroup per (16-row strip of T, 16-column tile).  Every wave owns a contiguous
+// share of the k-steps of the strip: its slice of T goes into registers at once, in the
+// same round of loads as the row tables, then the stacked input (K x 16) is gathered into
+// LDS through the input row table; after the barrier each wave accumulates its share on
+// MFMA and the partial sums meet in LDS.  The record carries T and both row tables, so
+// the loads start from the record alone.  LDS: K x 17 input, then 8 waves x 256 sums.
+template <typename TF, typename Inst, typename Rec>
+__global__ __launch_bounds__(CUT_THREADS, 1) void k_cut(const Inst* __restrict__ insts, const Rec* __restrict__ work,
+                                                     int nwork, int tn) {
+    int item, ti, tj;
+    if (!tile_of(nwork * tn, 1, tn, item, ti, tj)) return;
+    const Rec rc = work[item];
+    const CutArgs c = cut_args(rc);
+    const int K = c.K;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    using V4 = typename Mfma<TF>::v4;
+    // this wave's k-steps [s0, s0 + ns) of the strip; lane (lr, lk) holds row lr, column
+    // 4 s + lk of each (rows past the strip repeat its last row and are never stored)
+    const int S = (K + 3) / 4;
+    const int s0 = S * wave / nw, ns = S * (wave + 1) / nw - s0;
+    const gptr<const TF> T = gbl(reinterpret_cast<const TF*>(c.T) + c.row0 + (lr < c.rows ? lr : c.rows - 1));
+    TF a[CUT_MAXS];
+#pragma unroll
+    for (int u = 0; u < CUT_MAXS; ++u) {
+        const int k = 4 * (s0 + u) + lk;
+        a[u] = (u < ns && k < K) ? T[(int64_t)k * c.lda] : TF(0);
+    }
+    const gptr<const CutRow> xrow = gbl(reinterpret_cast<const CutRow*>(c.xrow));
+    CutRow xr[CUT_RPT];
+#pragma unroll
+    for (int q = 0; q < CUT_RPT; ++q) {
+        const int k = threadIdx.x + q * CUT_THREADS;
+        xr[q] = CutRow{0, 0, 0};
+        if (k < K) xr[q] = xrow[k];
+    }
+    CutRow orow[4];
+    if (wave == 0) {
+        const gptr<const CutRow> yrow = gbl(reinterpret_cast<const CutRow*>(c.yrow));
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int ii = Mfma<TF>::row(lk, r);
+            orow[r] = CutRow{0, 0, 0};
+            if (ii < c.rows) orow[r] = yrow[c.row0 + ii];
+        }
+    }
+    const Inst& I = insts[rc.inst];
+    const int nrhs = I.nrhs, j0 = tj * CUT_TW;
+    if (j0 >= nrhs) return;  // the same for every thread of the workgroup
+    const int nc = min(CUT_TW, nrhs - j0);
+    const gptr<TF> W = gbl(cut_ws(I));
+    constexpr int xp = CUT_TW + 1;
+    extern __shared__ unsigned char lds_raw[];
+    TF* xs = reinterpret_cast<TF*>(lds_raw);
+    TF* red = xs + (((int64_t)K * xp + 15) & ~int64_t(15));
+    TF v[CUT_RPT][CUT_TW];
+#pragma unroll
+    for (int q = 0; q < CUT_RPT; ++q) {
+        const int k = threadIdx.x + q * CUT_THREADS;
+        const int64_t base = xr[q].off * nrhs + xr[q].inner + (int64_t)j0 * xr[q].ld;
+#pragma unroll
+        for (int j = 0; j < CUT_TW; ++j) v[q][j] = (k < K && j < nc) ? W[base + (int64_t)j * xr[q].ld] : TF(0);
+    }
+#pragma unroll
+    for (int q = 0; q < CUT_RPT; ++q) {
+        const int k = threadIdx.x + q * CUT_THREADS;
+        if (k < K)
+#pragma unroll
+            for (int j = 0; j < CUT_TW; ++j) xs[k * xp + j] = v[q][j];
+    }
+    __syncthreads();
+    V4 a0 = V4{TF(0), TF(0), TF(0), TF(0)}, a1 = a0;
+#pragma unroll
+    for (int u = 0; u < CUT_MAXS; ++u) {
+        if (u < ns) {
+            const int k = 4 * (s0 + u) + lk;
+            const TF b = xs[(k < K ? k : K - 1) * xp + lr];  // a[u] is 0 past K
+            if (u & 1) a1 = Mfma<TF>::mac(a[u], b, a1);
+            else a0 = Mfma<TF>::mac(a[u], b, a0);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) red[wave * 256 + r * 64 + lane] = a0[r] + a1[r];
+    __syncthreads();
+    if (wave != 0) return;
+    V4 sum = V4{TF(0), TF(0), TF(0), TF(0)};
+    for (int w = 0; w < nw; ++w)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sum[r] += red[w * 256 + r * 64 + lane];
+    if (lr >= nc) return;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int ii = Mfma<TF>::row(lk, r);
+        if (ii < c.rows) W[orow[r].off * nrhs + orow[r].inner + (int64_t)(j0 + lr) * orow[r].ld] = sum[r];
+    }
+}
+
+// The identity seed of a cut's build (nrhs columns): row k < K of a stacked input gets a 1
+// in column col0 + k.
+template <typename TF>
+__global__ void k_cut_seed(TF* __restrict__ W, const CutRow* __restrict__ map, int K, int64_t nrhs, int col0) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const CutRow r = map[k];
+    W[r.off * nrhs + r.inner + (int64_t)(col0 + k) * r.ld] = TF(1);
+}
+
+// T (M x cols, column-major) = columns [0, cols) of the stacked rows of a cut's build
+// (nrhs columns).
+template <typename TF>
+__global__ void k_cut_gather(TF* __restrict__ T, int M, int cols, const CutRow* __restrict__ map,
+                             const TF* __restrict__ W, int64_t nrhs) {
+    const int64_t total = (int64_t)M * cols;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int g = (int)(e % M);
+        const int64_t k = e / M;
+        const CutRow r = map[g];
+        T[e] = W[r.off * nrhs + r.inner + k * r.ld];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // vector helpers for refinement
 // ---------------------------------------------------------------------------
 // R = B - R  (R holds A*X on entry), per column
@@ -2975,6 +3117,48 @@ hipError_t launch_unit_cols(double* E, int64_t ld, int64_t j0, int w, int64_t ro
     klaunch(k_unit_cols, dim3((unsigned)((w + 255) / 256)), dim3(256), 0, st, E, ld, j0, w, rows, v);
     return hipGetLastError();
 }
+int cut_max_k() { return CUT_MAXK; }
+size_t cut_lds(size_t w, int max_k) {
+    return w * (size_t)((((int64_t)max_k * (CUT_TW + 1) + 15) & ~int64_t(15)) + (CUT_THREADS / 64) * 256);
+}
+template <typename TF, typename Inst, typename Rec>
+hipError_t launch_cut(const Inst* insts, const Rec* work, int nwork, int max_nrhs, int max_k, hipStream_t st) {
+    if (nwork <= 0 || max_nrhs <= 0 || max_k <= 0) return hipSuccess;
+    if (max_k > CUT_MAXK) return hipErrorInvalidValue;
+    const size_t lds = cut_lds(sizeof(TF), max_k);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cut<TF, Inst, Rec>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    const int tn = (max_nrhs + CUT_TW - 1) / CUT_TW;
+    const int64_t total = (int64_t)nwork * tn;
+    klaunch((k_cut<TF, Inst, Rec>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(CUT_THREADS), lds, st, insts, work,
+            nwork, tn);
+    return hipGetLastError();
+}
+template hipError_t launch_cut<double, MvInst, MvRec>(const MvInst*, const MvRec*, int, int, int, hipStream_t);
+template hipError_t launch_cut<double, SInst<double>, SweepRec>(const SInst<double>*, const SweepRec*, int, int, int,
+                                                                 hipStream_t);
+template hipError_t launch_cut<float, SInst<float>, SweepRec>(const SInst<float>*, const SweepRec*, int, int, int,
+                                                               hipStream_t);
+template <typename TF>
+hipError_t launch_cut_seed(TF* W, const CutRow* map, int K, int64_t nrhs, int col0, hipStream_t st) {
+    if (K <= 0) return hipSuccess;
+    klaunch(k_cut_seed<TF>, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, st, W, map, K, nrhs, col0);
+    return hipGetLastError();
+}
+template <typename TF>
+hipError_t launch_cut_gather(TF* T, int M, int cols, const CutRow* map, const TF* W, int64_t nrhs, hipStream_t st) {
+    const int64_t total = (int64_t)M * cols;
+    if (total <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 4096);
+    klaunch(k_cut_gather<TF>, dim3(blocks), dim3(256), 0, st, T, M, cols, map, W, nrhs);
+    return hipGetLastError();
+}
+template hipError_t launch_cut_seed<double>(double*, const CutRow*, int, int64_t, int, hipStream_t);
+template hipError_t launch_cut_seed<float>(float*, const CutRow*, int, int64_t, int, hipStream_t);
+template hipError_t launch_cut_gather<double>(double*, int, int, const CutRow*, const double*, int64_t, hipStream_t);
+template hipError_t launch_cut_gather<float>(float*, int, int, const CutRow*, const float*, int64_t, hipStream_t);
+
 hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, double* out, hipStream_t st) {
     if (nrhs <= 0) return hipSuccess;
     klaunch(k_colnorm2, dim3(nrhs), dim3(256), 0, st, A, lda, n, out);

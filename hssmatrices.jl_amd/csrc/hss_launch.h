@@ -105,5 +105,17 @@ hipError_t launch_axpy(double* X, int64_t ldx, const double* D, int64_t ldd, int
 // unit columns j0 .. j0 + w - 1 of an identity into a zeroed rows x w block (v = 1), or cleared (v = 0)
 hipError_t launch_unit_cols(double* E, int64_t ld, int64_t j0, int w, int64_t rows, double v, hipStream_t st);
 hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, double* out, hipStream_t st);
+// tree cut (k_cut): one workgroup per (cut record, 16-column tile); max_k = the largest K
+// of the records, at most cut_max_k().  cut_lds = its LDS bytes for element size w.
+int cut_max_k();
+size_t cut_lds(size_t w, int max_k);
+template <typename TF, typename Inst, typename Rec>
+hipError_t launch_cut(const Inst* insts, const Rec* work, int nwork, int max_nrhs, int max_k, hipStream_t st);
+// a cut's build (nrhs columns): the identity seed of a stacked input (row k -> column
+// col0 + k), and T (M x cols) gathered from stacked output rows
+template <typename TF>
+hipError_t launch_cut_seed(TF* W, const CutRow* map, int K, int64_t nrhs, int col0, hipStream_t st);
+template <typename TF>
+hipError_t launch_cut_gather(TF* T, int M, int cols, const CutRow* map, const TF* W, int64_t nrhs, hipStream_t st);
 
 }  // namespace hss

@@ -79,6 +79,12 @@ static const FactorKernel g_factor_kernel = [] {
     if (e && std::strcmp(e, "mfma") == 0) return FactorKernel::Mfma;
     return FactorKernel::Auto;
 }();
+// HSS_CUT=0 runs every level of the matvec and solve sweeps as a launch of its own (no tree
+// cut), for comparing the two; the results agree to rounding.
+static const bool g_cut = [] {
+    const char* e = std::getenv("HSS_CUT");
+    return !(e && std::strcmp(e, "0") == 0);
+}();
 static std::vector<ProfRec> g_prof_recs;
 static std::vector<hipEvent_t> g_prof_pool;
 
@@ -129,6 +135,15 @@ static hipError_t dalloc(T** p, size_t count) {
 // ---------------------------------------------------------------------------
 struct Range {
     int off = 0, count = 0;
+};
+
+// One dense product of a tree cut (host copy; T and the row tables are device memory):
+// y = T x, T M x K column-major.
+struct CutInfo {
+    const void* T = nullptr;
+    int M = 0, K = 0;
+    const CutRow* xrow = nullptr;
+    const CutRow* yrow = nullptr;
 };
 
 // Device copies of per-call instance records.  A record is uploaded once and
@@ -189,6 +204,10 @@ struct Schedule {
     Range roots;
     // per-level choices made by the autotuner (hss_plan_tune); empty = built-in rules
     std::vector<LevelCfg> up_cfg, down_cfg;
+    // the tree cuts of the instances that have one (k_cut records in rec): cut_up (product
+    // 0) runs before the roots, cut (product 1) after them; *_k = the largest Kc
+    Range cut_up, cut;
+    int cut_up_k = 0, cut_k = 0;
 };
 // Largest LDS of a workgroup (bytes); levels above it use the streamed kernels.
 constexpr size_t kLdsResidentMax = 160 * 1024;
@@ -234,6 +253,10 @@ struct MvSched {
     int leaf_m = 0;
     std::vector<int64_t> up_na, down_na, up_k, down_k;
     int64_t leaf_na = 0, leaf_k = 0;
+    // the tree cuts of the instances that have one (k_cut records), run after the up-sweep;
+    // cut_k = the largest Kc
+    Range cut;
+    int cut_k = 0;
 };
 
 struct PlanKey {
@@ -289,11 +312,16 @@ struct hss_mat_s {
     MvRec* d_mvwork = nullptr;
     // per-call scratch
     SlotCache mvinst;
-    double *d_Z = nullptr, *d_F = nullptr;
-    size_t zf_cap = 0;  // nrhs capacity of Z/F
+    double* d_Z = nullptr;  // Z, then F at Z + zrows * nrhs
+    size_t zf_cap = 0;      // nrhs capacity of Z/F
     double *d_io_b = nullptr, *d_io_c = nullptr;
     size_t io_cap = 0;  // elements
     int64_t dev_bytes = 0;
+    // tree cut of the matvec (cut_depth 0: none): its product, T and the row tables
+    int cut_depth = 0;
+    std::vector<CutInfo> cut;
+    double* d_cutT = nullptr;
+    char* d_cut = nullptr;
 };
 
 struct hss_fact_s {
@@ -314,8 +342,8 @@ struct hss_fact_s {
     size_t lds_groot2s = 0;            // the same with D and Mb staged too
     size_t lds_gform = 0, lds_qform = 0;
     int max_kx = 0, max_nq = 0;        // widest Gr / Qt (columns formed per node)
-    void *d_wy = nullptr, *d_wz = nullptr;
-    size_t ws_cap = 0;  // nrhs capacity
+    void* d_wy = nullptr;  // y blocks, then the z blocks at wy + yrows * nrhs
+    size_t ws_cap = 0;     // nrhs capacity
     SlotCache sinst;
     // single-request solve schedule (instance 0)
     SweepRec* d_swork = nullptr;
@@ -328,6 +356,12 @@ struct hss_fact_s {
     int64_t dev_bytes = 0;
     // captured single-request plans keyed by their buffers (repeated solves replay a graph)
     std::vector<std::pair<PlanKey, hss_plan_t>> plan_cache;
+    // tree cut of the solve (cut_depth 0: none): its products [0] T_up and [1] T_down
+    // (factor type, one allocation) and their row tables
+    int cut_depth = 0;
+    std::vector<CutInfo> cut;
+    void* d_cutT = nullptr;
+    char* d_cut = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -446,21 +480,22 @@ static void free_mat(hss_mat_s* M) {
     (void)hipFree(M->d_nodes);
     (void)hipFree(M->d_work);
     (void)hipFree(M->d_Z);
-    (void)hipFree(M->d_F);
     (void)hipFree(M->d_io_b);
     (void)hipFree(M->d_io_c);
+    (void)hipFree(M->d_cutT);
+    (void)hipFree(M->d_cut);
     delete M;
 }
 
 // Operator shape of node i in a matvec step (kind 0 up, 1 down, 2 leaf): rows M, columns K.
-static void mv_shape(const hss_mat_s* M, int i, int kind, int& rows, int& cols) {
-    const DevNode& d = M->hn[i];
+static void mv_shape(const std::vector<DevNode>& hn, int i, int kind, int& rows, int& cols) {
+    const DevNode& d = hn[i];
     if (kind == 0) {
         rows = d.kv;
-        cols = d.is_leaf ? d.n : M->hn[d.left].kv + M->hn[d.right].kv;
+        cols = d.is_leaf ? d.n : hn[d.left].kv + hn[d.right].kv;
     } else if (kind == 1) {
-        const DevNode& p = M->hn[d.parent];
-        const DevNode& sb = M->hn[d.is_left ? p.right : p.left];
+        const DevNode& p = hn[d.parent];
+        const DevNode& sb = hn[d.is_left ? p.right : p.left];
         rows = d.ku;
         cols = sb.kv + p.ku;
     } else {
@@ -469,13 +504,14 @@ static void mv_shape(const hss_mat_s* M, int i, int kind, int& rows, int& cols) 
     }
 }
 
-// The matvec record of node i (kind 0 up, 1 down, 2 leaf) of instance b.
-static MvRec mv_record(const hss_mat_s* M, int b, int i, int kind) {
-    const DevNode& d = M->hn[i];
+// The matvec record of node i (kind 0 up, 1 down, 2 leaf) of instance b, with the Z / F
+// rows of the node table hn.
+static MvRec mv_record(const std::vector<DevNode>& hn, int b, int i, int kind) {
+    const DevNode& d = hn[i];
     MvRec q{};
     q.inst = b;
     int rows, cols;
-    mv_shape(M, i, kind, rows, cols);
+    mv_shape(hn, i, kind, rows, cols);
     q.M = rows;
     q.K = cols;
     if (kind == 0) {
@@ -488,7 +524,7 @@ static MvRec mv_record(const hss_mat_s* M, int b, int i, int kind) {
             q.r0 = d.n;
             q.ld1 = 1;
         } else {
-            const DevNode &L = M->hn[d.left], &R = M->hn[d.right];
+            const DevNode &L = hn[d.left], &R = hn[d.right];
             q.x0 = L.zrow;
             q.ld0 = std::max(L.kv, 1);
             q.r0 = L.kv;
@@ -496,8 +532,8 @@ static MvRec mv_record(const hss_mat_s* M, int b, int i, int kind) {
             q.ld1 = std::max(R.kv, 1);
         }
     } else if (kind == 1) {
-        const DevNode& p = M->hn[d.parent];
-        const DevNode& sb = M->hn[d.is_left ? p.right : p.left];
+        const DevNode& p = hn[d.parent];
+        const DevNode& sb = hn[d.is_left ? p.right : p.left];
         q.A = d.opF;
         q.x0 = sb.zrow;
         q.ld0 = std::max(sb.kv, 1);
@@ -518,7 +554,9 @@ static MvRec mv_record(const hss_mat_s* M, int b, int i, int kind) {
     return q;
 }
 
-// Appends the matvec work lists of `mats` (instance b = position in mats) to `work`.
+// Appends the matvec work lists of `mats` (instance b = position in mats) to `work`.  A
+// matrix with a tree cut at depth c runs no step of a node above depth c, nor the down step
+// of a depth-c node: the cut product gives the depth-c nodes' F from their Z.
 static void build_mv_schedule(const std::vector<hss_mat_s*>& mats, std::vector<MvRec>& work, MvSched& S) {
     int maxh = 0, maxd = 0;
     for (auto* M : mats) {
@@ -531,9 +569,9 @@ static void build_mv_schedule(const std::vector<hss_mat_s*>& mats, std::vector<M
             hss_mat_s* M = mats[b];
             for (int i = 0; i < M->nn; ++i) {
                 if (!sel(M, i)) continue;
-                work.push_back(mv_record(M, b, i, kind));
+                work.push_back(mv_record(M->hn, b, i, kind));
                 int rows, cols;
-                mv_shape(M, i, kind, rows, cols);
+                mv_shape(M->hn, i, kind, rows, cols);
                 mmax = std::max(mmax, rows);
                 na = std::max(na, ((int64_t)rows * cols + 15) / 16 * 16);
                 kk = std::max(kk, (int64_t)cols);
@@ -546,18 +584,40 @@ static void build_mv_schedule(const std::vector<hss_mat_s*>& mats, std::vector<M
     S.up_na.assign(maxh, 0);
     S.up_k.assign(maxh, 0);
     for (int h = 0; h < maxh; ++h)
-        level(0, [&](const hss_mat_s* M, int i) { return i != 0 && M->hn[i].height == h; }, S.up[h], S.up_m[h],
-              S.up_na[h], S.up_k[h]);
+        level(0, [&](const hss_mat_s* M, int i) { return i != 0 && M->hn[i].height == h && M->hn[i].depth >= M->cut_depth; },
+              S.up[h], S.up_m[h], S.up_na[h], S.up_k[h]);
     S.down.assign(maxd + 1, Range{});
     S.down_m.assign(maxd + 1, 0);
     S.down_na.assign(maxd + 1, 0);
     S.down_k.assign(maxd + 1, 0);
     for (int d = 1; d <= maxd; ++d)
-        level(1, [&](const hss_mat_s* M, int i) { return i != 0 && M->hn[i].depth == d; }, S.down[d], S.down_m[d],
-              S.down_na[d], S.down_k[d]);
+        level(1, [&](const hss_mat_s* M, int i) { return i != 0 && M->hn[i].depth == d && d > M->cut_depth; },
+              S.down[d], S.down_m[d], S.down_na[d], S.down_k[d]);
     S.leaf_m = 0;
     S.leaf_na = S.leaf_k = 0;
     level(2, [&](const hss_mat_s* M, int i) { return M->hn[i].is_leaf != 0; }, S.leaf, S.leaf_m, S.leaf_na, S.leaf_k);
+    // cut products: 16-row strips of T
+    S.cut.off = (int)work.size();
+    S.cut_k = 0;
+    for (int b = 0; b < (int)mats.size(); ++b) {
+        const hss_mat_s* M = mats[b];
+        if (M->cut_depth <= 0) continue;
+        const CutInfo& ci = M->cut[0];
+        S.cut_k = std::max(S.cut_k, ci.K);
+        for (int r0 = 0; r0 < ci.M; r0 += 16) {
+            MvRec q{};
+            q.inst = b;
+            q.r0 = r0;
+            q.M = std::min(16, ci.M - r0);
+            q.K = ci.K;
+            q.ld0 = ci.M;
+            q.A = reinterpret_cast<int64_t>(ci.T);
+            q.x0 = reinterpret_cast<int64_t>(ci.xrow);
+            q.x1 = reinterpret_cast<int64_t>(ci.yrow);
+            work.push_back(q);
+        }
+    }
+    S.cut.count = (int)work.size() - S.cut.off;
 }
 
 // Each step: operator and input staged in LDS (pick_resident), else the operator
@@ -568,6 +628,7 @@ static hss_status launch_mv_schedule(const MvInst* di, const MvRec* dw, const Mv
                                       {"k_mv_down_s", "k_mv_down_x", "k_mv_down"},
                                       {"k_mv_leaf_s", "k_mv_leaf_x", "k_mv_leaf"}};
     auto step = [&](int kind, int level, const Range& r, int m, int64_t na, int64_t k) -> hss_status {
+        if (r.count == 0) return HSS_OK;
         int tw = 32, form = 2;
         size_t lds = 0;
         if (pick_resident(sizeof(double), na, k, r.count, maxnrhs, tw, lds)) form = 0;
@@ -581,11 +642,223 @@ static hss_status launch_mv_schedule(const MvInst* di, const MvRec* dw, const Mv
         hss_status s = step(0, (int)h, S.up[h], S.up_m[h], S.up_na[h], S.up_k[h]);
         if (s != HSS_OK) return s;
     }
+    if (S.cut.count > 0) {
+        ProfScope ps("k_cut", 0, S.cut.count, st);
+        HIP_TRY((launch_cut<double, MvInst, MvRec>(di, dw + S.cut.off, S.cut.count, maxnrhs, S.cut_k, st)));
+    }
     for (size_t d = 1; d < S.down.size(); ++d) {
         hss_status s = step(1, (int)d, S.down[d], S.down_m[d], S.down_na[d], S.down_k[d]);
         if (s != HSS_OK) return s;
     }
     return step(2, 0, S.leaf, S.leaf_m, S.leaf_na, S.leaf_k);
+}
+
+// ---------------------------------------------------------------------------
+// tree cuts: the sweep steps above one depth collapsed into one dense product
+// ---------------------------------------------------------------------------
+// Depth of a tree cut.  sizes(c, k, m, t) gives, for a cut at depth c, the largest stacked
+// input k and output m of its products and their total elements t.  Returns the deepest
+// c >= 3 at or above every leaf (so at least the steps of depths 1 and 2 go) whose k and m
+// are within cut_max_k() (a k_cut workgroup holds its stacked input in LDS) and whose
+// products hold at most `budget` elements; 0 when there is none.  The budget (a 32nd of
+// the operators the cut's own sweeps stream) keeps a cut small next to the work it saves
+// launches on: a level of a batched launch serves many matrices at once and saves no
+// launch by one matrix's cut, so there a cut only adds its own traffic.
+template <typename Sizes>
+static int choose_cut(const hss_mat_s* M, int64_t budget, Sizes sizes) {
+    if (!g_cut) return 0;
+    int lmin = M->max_depth;
+    for (const DevNode& d : M->hn)
+        if (d.is_leaf) lmin = std::min(lmin, d.depth);
+    for (int c = lmin; c >= 3; --c) {
+        int64_t k = 0, m = 0, t = 0;
+        sizes(c, k, m, t);
+        if (k >= 1 && m >= 1 && k <= cut_max_k() && m <= cut_max_k() && t <= budget) return c;
+    }
+    return 0;
+}
+
+// Device buffers of a build, freed (after the stream drains) when it goes out of scope.
+struct TmpBufs {
+    hipStream_t st;
+    std::vector<void*> p;
+    explicit TmpBufs(hipStream_t s) : st(s) {}
+    ~TmpBufs() {
+        (void)hipStreamSynchronize(st);
+        for (void* q : p) (void)hipFree(q);
+    }
+    template <typename T>
+    hipError_t alloc(T** q, size_t n) {
+        hipError_t e = dalloc(q, n);
+        if (e == hipSuccess) p.push_back(*q);
+        return e;
+    }
+    template <typename T>
+    hipError_t put(T** q, const std::vector<T>& v) {
+        hipError_t e = alloc(q, v.size());
+        if (e == hipSuccess && !v.empty()) e = hipMemcpy(*q, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+        return e;
+    }
+};
+
+// One dense product of a cut: T (M x K) and its row tables (relative to the call's
+// workspace).
+struct CutSpec {
+    const void* T;
+    int M, K;
+    std::vector<CutRow> xr, yr;
+};
+// The row tables of a cut's products in one device allocation (*out); info = the products
+// with their device addresses.
+static hipError_t upload_cuts(const std::vector<CutSpec>& cs, char** out, std::vector<CutInfo>& info, int64_t* bytes) {
+    size_t tot = 0;
+    for (const CutSpec& c : cs) tot += sizeof(CutRow) * (c.xr.size() + c.yr.size());
+    char* d = nullptr;
+    hipError_t e = dalloc(&d, tot);
+    if (e != hipSuccess) return e;
+    std::vector<char> h(std::max<size_t>(tot, 1), 0);
+    info.clear();
+    size_t o = 0;
+    for (const CutSpec& c : cs) {
+        const size_t ox = o, oy = ox + sizeof(CutRow) * c.xr.size();
+        o = oy + sizeof(CutRow) * c.yr.size();
+        CutInfo ci;
+        ci.T = c.T;
+        ci.M = c.M;
+        ci.K = c.K;
+        ci.xrow = reinterpret_cast<const CutRow*>(d + ox);
+        ci.yrow = reinterpret_cast<const CutRow*>(d + oy);
+        info.push_back(ci);
+        std::memcpy(h.data() + ox, c.xr.data(), sizeof(CutRow) * c.xr.size());
+        std::memcpy(h.data() + oy, c.yr.data(), sizeof(CutRow) * c.yr.size());
+    }
+    e = hipMemcpy(d, h.data(), tot, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        info.clear();
+        return e;
+    }
+    *out = d;
+    *bytes = (int64_t)tot;
+    return hipSuccess;
+}
+
+// Tree cut of the matvec at depth c: input = the depth-c nodes' Z (kv rows each), output =
+// their F (ku rows each), F = T Z, one product.  T is built by running the sweep steps above the cut on
+// the identity -- Z of the depth-c nodes = unit columns, Kc right-hand sides, on Z / F rows
+// numbered for the nodes at depth <= c only -- and gathering the depth-c nodes' F.  The
+// matrix's schedule is then rebuilt without the steps the cut replaces.
+static hss_status mv_cut_build(hss_mat_s* M) {
+    const int c = choose_cut(M, M->op_len / 32, [&](int c, int64_t& k, int64_t& m, int64_t& t) {
+        for (const DevNode& d : M->hn)
+            if (d.depth == c) {
+                k += d.kv;
+                m += d.ku;
+            }
+        t = k * m;
+    });
+    if (c == 0) return HSS_OK;
+    int Kc = 0, Mc = 0;
+    for (const DevNode& d : M->hn)
+        if (d.depth == c) {
+            Kc += d.kv;
+            Mc += d.ku;
+        }
+    hipStream_t st = M->ctx->stream;
+    std::vector<DevNode> hn = M->hn;
+    int64_t zt = 0, ft = 0;
+    for (DevNode& d : hn) {
+        if (d.depth > c) continue;
+        d.zrow = zt;
+        zt += d.kv;
+        d.frow = ft;
+        ft += d.ku;
+    }
+    // row tables of the build (compact rows) and of the matvec workspaces
+    std::vector<CutRow> xb, yb, xr, yr;
+    for (int i = 0; i < M->nn; ++i) {
+        const DevNode &d = hn[i], &o = M->hn[i];
+        if (d.depth != c) continue;
+        for (int r = 0; r < d.kv; ++r) {
+            xb.push_back(CutRow{d.zrow, r, d.kv});
+            xr.push_back(CutRow{o.zrow, r, d.kv});
+        }
+        for (int r = 0; r < d.ku; ++r) {
+            yb.push_back(CutRow{d.frow, r, d.ku});
+            yr.push_back(CutRow{M->zrows + o.frow, r, d.ku});  // F follows Z in the workspace
+        }
+    }
+    // the steps above the cut: up from depth c - 1 to 1, then down from depth 1 to c
+    std::vector<MvRec> work;
+    MvSched S;
+    S.up.assign(c - 1, Range{});
+    S.up_m.assign(c - 1, 0);
+    S.up_na.assign(c - 1, 0);
+    S.up_k.assign(c - 1, 0);
+    S.down.assign(c + 1, Range{});
+    S.down_m.assign(c + 1, 0);
+    S.down_na.assign(c + 1, 0);
+    S.down_k.assign(c + 1, 0);
+    auto add = [&](int kind, int depth, Range& r, int& mm, int64_t& na, int64_t& kk) {
+        r.off = (int)work.size();
+        for (int i = 1; i < M->nn; ++i) {
+            if (hn[i].depth != depth) continue;
+            work.push_back(mv_record(hn, 0, i, kind));
+            int rows, cols;
+            mv_shape(hn, i, kind, rows, cols);
+            mm = std::max(mm, rows);
+            na = std::max(na, ((int64_t)rows * cols + 15) / 16 * 16);
+            kk = std::max(kk, (int64_t)cols);
+        }
+        r.count = (int)work.size() - r.off;
+    };
+    for (int h = 0; h < c - 1; ++h) add(0, c - 1 - h, S.up[h], S.up_m[h], S.up_na[h], S.up_k[h]);
+    for (int d = 1; d <= c; ++d) add(1, d, S.down[d], S.down_m[d], S.down_na[d], S.down_k[d]);
+    {
+        TmpBufs tb(st);
+        double *Zt = nullptr, *Ft = nullptr;
+        MvRec* dw = nullptr;
+        MvInst* di = nullptr;
+        CutRow *dxb = nullptr, *dyb = nullptr;
+        HIP_TRY(tb.alloc(&Zt, (size_t)zt * Kc));
+        HIP_TRY(tb.alloc(&Ft, (size_t)std::max<int64_t>(ft, 1) * Kc));
+        HIP_TRY(hipMemsetAsync(Zt, 0, sizeof(double) * (size_t)zt * Kc, st));
+        HIP_TRY(hipMemsetAsync(Ft, 0, sizeof(double) * (size_t)std::max<int64_t>(ft, 1) * Kc, st));
+        HIP_TRY(tb.put(&dw, work));
+        HIP_TRY(tb.put(&dxb, xb));
+        HIP_TRY(tb.put(&dyb, yb));
+        std::vector<MvInst> I(1);
+        std::memset(I.data(), 0, sizeof(MvInst));
+        I[0].nodes = M->d_nodes;
+        I[0].op = M->d_op;
+        I[0].Z = Zt;
+        I[0].F = Ft;
+        I[0].alpha = 1.0;
+        I[0].nrhs = Kc;
+        HIP_TRY(tb.put(&di, I));
+        HIP_TRY(launch_cut_seed<double>(Zt, dxb, Kc, Kc, 0, st));
+        hss_status s = launch_mv_schedule(di, dw, S, Kc, st);
+        if (s != HSS_OK) return s;
+        HIP_TRY(dalloc(&M->d_cutT, (size_t)Mc * Kc));
+        HIP_TRY(launch_cut_gather<double>(M->d_cutT, Mc, Kc, dyb, Ft, Kc, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    int64_t tab = 0;
+    HIP_TRY(upload_cuts(std::vector<CutSpec>{CutSpec{M->d_cutT, Mc, Kc, xr, yr}}, &M->d_cut, M->cut, &tab));
+    M->cut_depth = c;
+    std::vector<MvRec> mw;
+    build_mv_schedule(std::vector<hss_mat_s*>{M}, mw, M->mvs);
+    MvRec* nw = nullptr;
+    HIP_TRY(dalloc(&nw, mw.size()));
+    hipError_t e = hipMemcpy(nw, mw.data(), sizeof(MvRec) * mw.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(nw);
+        return fail(HSS_ERR_HIP, std::string("matvec cut: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(M->d_mvwork);
+    M->d_mvwork = nw;
+    M->dev_bytes += (int64_t)(sizeof(double) * (size_t)Mc * Kc) + tab;
+    return HSS_OK;
 }
 
 // One piece of a matvec operator, copied from the generator blob:
@@ -774,9 +1047,16 @@ static hss_status mat_create_impl(hss_ctx_t ctx, const hss_node_desc* nodes, int
         return bail(e);
     }
     (void)hipFree(dT);
-    if (dblob) *dblob = nullptr;
     M->dev_bytes = (int64_t)(sizeof(double) * (blob_len + M->op_len) + sizeof(DevNode) * nn +
                              sizeof(WorkItem) * M->hwork.size() + sizeof(MvRec) * mvwork.size());
+    hss_status cs = mv_cut_build(M);
+    if (cs != HSS_OK) {
+        (void)hipStreamSynchronize(st);
+        if (dblob) M->d_blob = nullptr;
+        free_mat(M);
+        return cs;
+    }
+    if (dblob) *dblob = nullptr;
     ctx->refs++;
     *out = M;
     return HSS_OK;
@@ -1293,11 +1573,9 @@ extern "C" int64_t hss_mat_device_bytes(hss_mat_t M) { return M ? M->dev_bytes :
 static hss_status ensure_zf(hss_mat_s* M, int nrhs) {
     if ((size_t)nrhs <= M->zf_cap) return HSS_OK;
     (void)hipFree(M->d_Z);
-    (void)hipFree(M->d_F);
-    M->d_Z = M->d_F = nullptr;
+    M->d_Z = nullptr;
     M->zf_cap = 0;
-    HIP_TRY(dalloc(&M->d_Z, (size_t)std::max<int64_t>(M->zrows, 1) * nrhs));
-    HIP_TRY(dalloc(&M->d_F, (size_t)std::max<int64_t>(M->frows, 1) * nrhs));
+    HIP_TRY(dalloc(&M->d_Z, (size_t)std::max<int64_t>(M->zrows + M->frows, 1) * nrhs));
     M->zf_cap = nrhs;
     return HSS_OK;
 }
@@ -1320,7 +1598,7 @@ static hss_status matmul_enqueue(hss_mat_s* M, int nrhs, double alpha, const dou
     I.ldb = ldb;
     I.ldc = ldc;
     I.Z = M->d_Z;
-    I.F = M->d_F;
+    I.F = I.Z + M->zrows * nrhs;
     I.alpha = alpha;
     I.beta = beta;
     I.nrhs = nrhs;
@@ -1569,17 +1847,20 @@ static void free_fact(hss_fact_s* F) {
     (void)hipFree(F->d_piv);
     (void)hipFree(F->d_status);
     (void)hipFree(F->d_wy);
-    (void)hipFree(F->d_wz);
     (void)hipFree(F->d_swork);
     (void)hipFree(F->d_r);
     (void)hipFree(F->d_d);
     (void)hipFree(F->d_nrm);
     (void)hipFree(F->d_io_b);
     (void)hipFree(F->d_io_x);
+    (void)hipFree(F->d_cutT);
+    (void)hipFree(F->d_cut);
     delete F;
 }
 
 static hss_status ensure_schedule(hss_fact_s* F);
+template <typename TF>
+static hss_status solve_cut_build(const std::vector<hss_fact_s*>& facts, hipStream_t st);
 
 // Factor a batch of matrices that share one context: one launch per height
 // across all of them, then one root launch.
@@ -1773,8 +2054,12 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
         F->fac_len = F->res_len;
         F->dev_bytes = (int64_t)(sizeof(TF) * F->fac_len + sizeof(FNode) * F->hfn.size() + 4 * (F->hfn[0].m + 1));
     }
-    // the single-request solve schedule, built here so that solves (the allocation-free one
-    // included) find it ready
+    // tree cuts of the solves, then the single-request solve schedule, built here so that
+    // solves (the allocation-free one included) find it ready
+    {
+        hss_status s2 = solve_cut_build<TF>(out, st);
+        if (s2 != HSS_OK) { cleanup(); return s2; }
+    }
     for (int b = 0; b < cnt; ++b) {
         hss_status s2 = ensure_schedule(out[b]);
         if (s2 != HSS_OK) { cleanup(); return s2; }
@@ -1824,11 +2109,9 @@ template <typename TF>
 static hss_status ensure_solve_ws(hss_fact_s* F, int nrhs) {
     if ((size_t)nrhs <= F->ws_cap) return HSS_OK;
     (void)hipFree(F->d_wy);
-    (void)hipFree(F->d_wz);
-    F->d_wy = F->d_wz = nullptr;
+    F->d_wy = nullptr;
     F->ws_cap = 0;
-    HIP_TRY(dalloc(reinterpret_cast<TF**>(&F->d_wy), (size_t)std::max<int64_t>(F->yrows, 1) * nrhs));
-    HIP_TRY(dalloc(reinterpret_cast<TF**>(&F->d_wz), (size_t)std::max<int64_t>(F->zrows, 1) * nrhs));
+    HIP_TRY(dalloc(reinterpret_cast<TF**>(&F->d_wy), (size_t)std::max<int64_t>(F->yrows + F->zrows, 1) * nrhs));
     F->ws_cap = nrhs;
     return HSS_OK;
 }
@@ -1864,12 +2147,93 @@ static SInst<TF> make_sinst(const SolveReq& q, TF* ws = nullptr) {
         I.wz = ws + F->yrows * q.nrhs;
     } else {
         I.wy = reinterpret_cast<TF*>(F->d_wy);
-        I.wz = reinterpret_cast<TF*>(F->d_wz);
+        I.wz = I.wy + F->yrows * q.nrhs;
     }
     I.nrhs = q.nrhs;
     return I;
 }
 
+// Appends the up-sweep record of `node` (instance b, solve rows of fn) to level h of S.
+static void add_up_rec(Schedule& S, size_t h, const hss_mat_s* M, const std::vector<FNode>& fn, int b, int node) {
+    auto r16 = [](int64_t x) { return (x + 15) & ~int64_t(15); };
+    S.work.push_back(WorkItem{b, node});
+    const FNode& f = fn[node];
+    const DevNode& nd = M->hn[node];
+    SweepRec q{};
+    q.inst = b;
+    q.node = node;
+    q.leaf = nd.is_leaf;
+    q.nk = f.nk;
+    q.K = f.Kx;
+    q.My = f.My;
+    q.A = f.nk > 0 ? f.Gz : f.Gr;  // Gr follows Gz
+    q.y = f.yrow;
+    if (nd.is_leaf) {
+        q.x0 = nd.row_off;
+    } else {
+        const FNode &fl = fn[nd.left], &fr = fn[nd.right];
+        q.x0 = fl.yrow;
+        q.add0 = fl.nk;
+        q.l0 = fl.My;
+        q.r0 = fl.My - fl.nk;
+        q.x1 = fr.yrow;
+        q.add1 = fr.nk;
+        q.l1 = fr.My;
+    }
+    S.rec.push_back(q);
+    const int64_t na = (int64_t)f.nk * f.Kx + (int64_t)(f.My - f.nk) * (f.Kx + f.nk);
+    S.up_m[h] = std::max(S.up_m[h], f.nk);
+    S.up_na[h] = std::max(S.up_na[h], r16(na));
+    S.up_k[h] = std::max(S.up_k[h], (int64_t)(f.Kx + f.nk));
+    S.up_rows[h] = std::max(S.up_rows[h], std::max(f.nk, f.My - f.nk));
+}
+
+// Appends the down-sweep record of `node` (instance b, solve rows of fn) to level d of S.
+static void add_down_rec(Schedule& S, size_t d, const hss_mat_s* M, const std::vector<FNode>& fn, int b, int node) {
+    auto r16 = [](int64_t x) { return (x + 15) & ~int64_t(15); };
+    S.work.push_back(WorkItem{b, node});
+    const FNode& f = fn[node];
+    const DevNode& nd = M->hn[node];
+    const FNode& fp = fn[nd.parent];
+    SweepRec q{};
+    q.inst = b;
+    q.node = node;
+    q.leaf = nd.is_leaf;
+    q.nk = f.nk;
+    q.K = f.n;
+    q.My = f.My;
+    q.A = f.Qt;
+    q.x0 = f.yrow;
+    q.x1 = fp.zrow;
+    q.add1 = nd.is_left ? 0 : fn[M->hn[nd.parent].left].nr;
+    q.l1 = fp.n;
+    q.y = nd.is_leaf ? nd.col_off : f.zrow;
+    S.rec.push_back(q);
+    const int64_t na = f.Qt >= 0 ? (int64_t)f.n * f.n : 0;
+    S.down_m[d] = std::max(S.down_m[d], f.n);
+    S.down_na[d] = std::max(S.down_na[d], r16(na));
+    S.down_k[d] = std::max(S.down_k[d], (int64_t)f.n);
+}
+
+static void schedule_levels(Schedule& S, int nup, int ndown) {
+    S.work.clear();
+    S.rec.clear();
+    S.up.assign(nup, Range{});
+    S.up_m.assign(nup, 0);
+    S.up_na.assign(nup, 0);
+    S.up_k.assign(nup, 0);
+    S.up_rows.assign(nup, 0);
+    S.down.assign(ndown, Range{});
+    S.down_m.assign(ndown, 0);
+    S.down_na.assign(ndown, 0);
+    S.down_k.assign(ndown, 0);
+    S.cut_up = S.cut = Range{};
+    S.cut_up_k = S.cut_k = 0;
+}
+
+// A factor with a tree cut at depth c runs the up steps of the nodes at depth >= c, cut
+// product 0 (the depth-1 nodes' reduced b and u), the root step, cut product 1 (the z of the
+// depth c - 1 nodes) and the down steps of the nodes at depth >= c.
 static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
     const int cnt = (int)facts.size();
     int maxh = 0, maxd = 0;
@@ -1877,59 +2241,17 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
         maxh = std::max(maxh, F->mat->max_height);
         maxd = std::max(maxd, F->mat->max_depth);
     }
-    S.work.clear();
-    S.rec.clear();
-    S.up.assign(maxh, Range{});
-    S.up_m.assign(maxh, 0);
-    S.up_na.assign(maxh, 0);
-    S.up_k.assign(maxh, 0);
-    S.up_rows.assign(maxh, 0);
-    S.down.assign(maxd + 1, Range{});
-    S.down_m.assign(maxd + 1, 0);
-    S.down_na.assign(maxd + 1, 0);
-    S.down_k.assign(maxd + 1, 0);
+    schedule_levels(S, maxh, maxd + 1);
     S.w = (!facts.empty() && facts[0]->prec == HSS_PREC_FP32) ? 4 : 8;
-    auto r16 = [](int64_t x) { return (x + 15) & ~int64_t(15); };
     for (int h = 0; h < maxh; ++h) {
         S.up[h].off = (int)S.work.size();
         for (int b = 0; b < cnt; ++b) {
             hss_mat_s* M = facts[b]->mat;
-            if (h < M->max_height) {
-                const Range& r = M->by_height[h];
-                for (int t = 0; t < r.count; ++t) {
-                    const int node = M->hwork[r.off + t].node;
-                    S.work.push_back(WorkItem{b, node});
-                    const std::vector<FNode>& fn = facts[b]->hfn;
-                    const FNode& f = fn[node];
-                    const DevNode& nd = M->hn[node];
-                    SweepRec q{};
-                    q.inst = b;
-                    q.node = node;
-                    q.leaf = nd.is_leaf;
-                    q.nk = f.nk;
-                    q.K = f.Kx;
-                    q.My = f.My;
-                    q.A = f.nk > 0 ? f.Gz : f.Gr;  // Gr follows Gz
-                    q.y = f.yrow;
-                    if (nd.is_leaf) {
-                        q.x0 = nd.row_off;
-                    } else {
-                        const FNode &fl = fn[nd.left], &fr = fn[nd.right];
-                        q.x0 = fl.yrow;
-                        q.add0 = fl.nk;
-                        q.l0 = fl.My;
-                        q.r0 = fl.My - fl.nk;
-                        q.x1 = fr.yrow;
-                        q.add1 = fr.nk;
-                        q.l1 = fr.My;
-                    }
-                    S.rec.push_back(q);
-                    const int64_t na = (int64_t)f.nk * f.Kx + (int64_t)(f.My - f.nk) * (f.Kx + f.nk);
-                    S.up_m[h] = std::max(S.up_m[h], f.nk);
-                    S.up_na[h] = std::max(S.up_na[h], r16(na));
-                    S.up_k[h] = std::max(S.up_k[h], (int64_t)(f.Kx + f.nk));
-                    S.up_rows[h] = std::max(S.up_rows[h], std::max(f.nk, f.My - f.nk));
-                }
+            if (h >= M->max_height) continue;
+            const Range& r = M->by_height[h];
+            for (int t = 0; t < r.count; ++t) {
+                const int node = M->hwork[r.off + t].node;
+                if (M->hn[node].depth >= facts[b]->cut_depth) add_up_rec(S, h, M, facts[b]->hfn, b, node);
             }
         }
         S.up[h].count = (int)S.work.size() - S.up[h].off;
@@ -1941,39 +2263,39 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
         q.inst = b;
         S.rec.push_back(q);
     }
+    // cut products: 16-row strips of T_up and T_down
+    auto strips = [&](Range& r, int& kmax, int which) {
+        r.off = (int)S.work.size();
+        for (int b = 0; b < cnt; ++b) {
+            const hss_fact_s* F = facts[b];
+            if (F->cut_depth <= 0) continue;
+            const CutInfo& ci = F->cut[which];
+            kmax = std::max(kmax, ci.K);
+            for (int r0 = 0; r0 < ci.M; r0 += 16) {
+                S.work.push_back(WorkItem{b, -1});
+                SweepRec q{};
+                q.inst = b;
+                q.r0 = r0;
+                q.My = std::min(16, ci.M - r0);
+                q.K = ci.K;
+                q.l0 = ci.M;
+                q.A = reinterpret_cast<int64_t>(ci.T);
+                q.x0 = reinterpret_cast<int64_t>(ci.xrow);
+                q.x1 = reinterpret_cast<int64_t>(ci.yrow);
+                S.rec.push_back(q);
+            }
+        }
+        r.count = (int)S.work.size() - r.off;
+    };
+    strips(S.cut_up, S.cut_up_k, 0);
+    strips(S.cut, S.cut_k, 1);
     for (int d = 1; d <= maxd; ++d) {
         S.down[d].off = (int)S.work.size();
         for (int b = 0; b < cnt; ++b) {
             hss_mat_s* M = facts[b]->mat;
-            if (d <= M->max_depth) {
-                const Range& r = M->by_depth[d];
-                for (int t = 0; t < r.count; ++t) {
-                    const int node = M->hwork[r.off + t].node;
-                    S.work.push_back(WorkItem{b, node});
-                    const std::vector<FNode>& fn = facts[b]->hfn;
-                    const FNode& f = fn[node];
-                    const DevNode& nd = M->hn[node];
-                    const FNode& fp = fn[nd.parent];
-                    SweepRec q{};
-                    q.inst = b;
-                    q.node = node;
-                    q.leaf = nd.is_leaf;
-                    q.nk = f.nk;
-                    q.K = f.n;
-                    q.My = f.My;
-                    q.A = f.Qt;
-                    q.x0 = f.yrow;
-                    q.x1 = fp.zrow;
-                    q.add1 = nd.is_left ? 0 : fn[M->hn[nd.parent].left].nr;
-                    q.l1 = fp.n;
-                    q.y = nd.is_leaf ? nd.col_off : f.zrow;
-                    S.rec.push_back(q);
-                    const int64_t na = f.Qt >= 0 ? (int64_t)f.n * f.n : 0;
-                    S.down_m[d] = std::max(S.down_m[d], f.n);
-                    S.down_na[d] = std::max(S.down_na[d], r16(na));
-                    S.down_k[d] = std::max(S.down_k[d], (int64_t)f.n);
-                }
-            }
+            if (d > M->max_depth || d < facts[b]->cut_depth) continue;
+            const Range& r = M->by_depth[d];
+            for (int t = 0; t < r.count; ++t) add_down_rec(S, d, M, facts[b]->hfn, b, M->hwork[r.off + t].node);
         }
         S.down[d].count = (int)S.work.size() - S.down[d].off;
     }
@@ -2002,6 +2324,7 @@ template <typename TF>
 static hss_status launch_up_level(const SInst<TF>* di, const SweepRec* dw, const Schedule& S, size_t h, int maxnrhs,
                                   LevelCfg c, hipStream_t st) {
     const int cnt = S.up[h].count;
+    if (cnt == 0) return HSS_OK;
     if (c.form < 0) c = default_cfg(S.w, S.up_na[h], S.up_k[h], S.up_rows[h], cnt, maxnrhs);
     if (c.form == 2) {
         ProfScope ps("k_ulv_gup", (int)h, cnt, st);
@@ -2020,6 +2343,7 @@ template <typename TF>
 static hss_status launch_down_level(const SInst<TF>* di, const SweepRec* dw, const Schedule& S, size_t d, int maxnrhs,
                                     LevelCfg c, hipStream_t st) {
     const int cnt = S.down[d].count;
+    if (cnt == 0) return HSS_OK;
     if (c.form < 0) c = default_cfg(S.w, S.down_na[d], S.down_k[d], S.down_m[d], cnt, maxnrhs);
     if (c.form == 2) {
         ProfScope ps("k_ulv_gdown", (int)d, cnt, st);
@@ -2042,7 +2366,11 @@ static hss_status launch_schedule(const SInst<TF>* di, const SweepRec* dw, const
         hss_status s = launch_up_level<TF>(di, dw, S, h, maxnrhs, h < S.up_cfg.size() ? S.up_cfg[h] : LevelCfg{}, st);
         if (s != HSS_OK) return s;
     }
-    {
+    if (S.cut_up.count > 0) {
+        ProfScope ps("k_cut", 0, S.cut_up.count, st);
+        HIP_TRY((launch_cut<TF, SInst<TF>, SweepRec>(di, dw + S.cut_up.off, S.cut_up.count, maxnrhs, S.cut_up_k, st)));
+    }
+    if (S.roots.count > 0) {
         if (lgroot2 > 0 && lgroot2s > 0 && lgroot2s <= kLdsResidentMax) {
             ProfScope ps("k_ulv_groot2", 0, S.roots.count, st);
             HIP_TRY((launch_ulv_groot2<TF>(di, dw + S.roots.off, S.roots.count, maxnrhs, lgroot2s, 1, st)));
@@ -2056,10 +2384,158 @@ static hss_status launch_schedule(const SInst<TF>* di, const SweepRec* dw, const
                                           res ? 1 : 0, st)));
         }
     }
+    if (S.cut.count > 0) {
+        ProfScope ps("k_cut", 1, S.cut.count, st);
+        HIP_TRY((launch_cut<TF, SInst<TF>, SweepRec>(di, dw + S.cut.off, S.cut.count, maxnrhs, S.cut_k, st)));
+    }
     for (size_t d = 1; d < S.down.size(); ++d) {
         hss_status s =
             launch_down_level<TF>(di, dw, S, d, maxnrhs, d < S.down_cfg.size() ? S.down_cfg[d] : LevelCfg{}, st);
         if (s != HSS_OK) return s;
+    }
+    return HSS_OK;
+}
+
+// Tree cut of the solve at depth c, two products around the root step (which keeps its
+// refinement):
+//   product 0: the depth-1 nodes' reduced b and u (rows [nk, My) of their y, what the root
+//              reads) from the depth-c nodes' (what their parents' up steps read), T_up;
+//   product 1: the z of the depth c - 1 nodes (n rows each, what the depth-c down steps
+//              read) from the depth-c nodes' reduced b and u and the root's z, T_down.
+// Both are built by running the sweeps above the cut without the root on the identity:
+// Kc + n_root right-hand sides, the first Kc with unit columns in the depth-c nodes' reduced
+// rows and a zero root z, the last n_root with zero inputs and unit columns in the root's z,
+// on y / z rows numbered for the nodes at depth <= c only.  Each factor builds in a launch
+// sequence of its own, so T (and every solve through it) is the same whether the factor was
+// made alone or in a batch.
+template <typename TF>
+static hss_status solve_cut_build(const std::vector<hss_fact_s*>& facts, hipStream_t st) {
+    for (hss_fact_s* F : facts) {
+        const hss_mat_s* M = F->mat;
+        const std::vector<FNode>& of = F->hfn;
+        const int nroot = of[0].n;
+        // products: T_up (Mu x Ku) and T_down (Md x Ku + nroot)
+        auto dims = [&](int c, int64_t& ku, int64_t& mu, int64_t& md) {
+            ku = mu = md = 0;
+            for (int i = 0; i < M->nn; ++i) {
+                const int d = M->hn[i].depth;
+                if (d == c) ku += of[i].My - of[i].nk;
+                if (d == c - 1) md += of[i].n;
+                if (d == 1) mu += of[i].My - of[i].nk;
+            }
+        };
+        const int c = choose_cut(M, F->res_len / 32, [&](int c, int64_t& k, int64_t& m, int64_t& t) {
+            int64_t ku, mu, md;
+            dims(c, ku, mu, md);
+            k = ku > 0 ? ku + nroot : 0;
+            m = std::max(mu, md);
+            t = mu * ku + md * (ku + nroot);
+        });
+        if (c == 0) continue;
+        int64_t ku64, mu64, md64;
+        dims(c, ku64, mu64, md64);
+        const int Ku = (int)ku64, Kd = Ku + nroot;
+        // compact rows: y for depths 1..c, z for depths 0..c-1 (after the y rows)
+        std::vector<FNode> fn = of;
+        int64_t yt = 0, zt = 0;
+        for (int i = 0; i < M->nn; ++i) {
+            FNode& f = fn[i];
+            const int d = M->hn[i].depth;
+            f.yrow = f.zrow = 0;
+            if (d >= 1 && d <= c) {
+                f.yrow = yt;
+                yt += f.My;
+            }
+            if (d <= c - 1) {
+                f.zrow = zt;
+                zt += f.n;
+            }
+        }
+        // row tables: b* of the build (workspace y then z, Kd columns), r* of the solves
+        std::vector<CutRow> bx, broot, bup, bdown, rx, rup, rdown;
+        for (int i = 0; i < M->nn; ++i) {
+            const int d = M->hn[i].depth;
+            const FNode &f = fn[i], &o = of[i];
+            if (d == c)
+                for (int r = 0; r < f.My - f.nk; ++r) {
+                    bx.push_back(CutRow{f.yrow, f.nk + r, f.My});
+                    rx.push_back(CutRow{o.yrow, f.nk + r, f.My});
+                }
+            if (d == 1)
+                for (int r = 0; r < f.My - f.nk; ++r) {
+                    bup.push_back(CutRow{f.yrow, f.nk + r, f.My});
+                    rup.push_back(CutRow{o.yrow, f.nk + r, f.My});
+                }
+            if (d == c - 1)
+                for (int r = 0; r < f.n; ++r) {
+                    bdown.push_back(CutRow{yt + f.zrow, r, f.n});
+                    rdown.push_back(CutRow{F->yrows + o.zrow, r, f.n});
+                }
+        }
+        for (int r = 0; r < nroot; ++r) broot.push_back(CutRow{yt + fn[0].zrow, r, nroot});
+        std::vector<CutRow> rxd = rx;
+        for (int r = 0; r < nroot; ++r) rxd.push_back(CutRow{F->yrows + of[0].zrow, r, nroot});
+        const int Mu = (int)bup.size(), Md = (int)bdown.size();
+        // up steps by depth (c - 1 first), no root, down steps by depth (1 .. c - 1)
+        Schedule S;
+        schedule_levels(S, c - 1, c);
+        S.w = sizeof(TF);
+        for (int h = 0; h < c - 1; ++h) {
+            const int d = c - 1 - h;
+            S.up[h].off = (int)S.work.size();
+            for (int i = 1; i < M->nn; ++i)
+                if (M->hn[i].depth == d) add_up_rec(S, h, M, fn, 0, i);
+            S.up[h].count = (int)S.work.size() - S.up[h].off;
+        }
+        S.roots = Range{(int)S.work.size(), 0};
+        for (int d = 1; d <= c - 1; ++d) {
+            S.down[d].off = (int)S.work.size();
+            for (int i = 1; i < M->nn; ++i)
+                if (M->hn[i].depth == d) add_down_rec(S, d, M, fn, 0, i);
+            S.down[d].count = (int)S.work.size() - S.down[d].off;
+        }
+        const size_t wsz = (size_t)(yt + zt) * Kd;
+        TmpBufs tb(st);
+        TF* ws = nullptr;
+        FNode* dfn = nullptr;
+        CutRow *dbx = nullptr, *dbroot = nullptr, *dbup = nullptr, *dbdown = nullptr;
+        SInst<TF>* di = nullptr;
+        SweepRec* dw = nullptr;
+        HIP_TRY(tb.alloc(&ws, wsz));
+        HIP_TRY(hipMemsetAsync(ws, 0, sizeof(TF) * wsz, st));
+        HIP_TRY(tb.put(&dfn, fn));
+        HIP_TRY(tb.put(&dbx, bx));
+        HIP_TRY(tb.put(&dbroot, broot));
+        HIP_TRY(tb.put(&dbup, bup));
+        HIP_TRY(tb.put(&dbdown, bdown));
+        std::vector<SInst<TF>> I(1);
+        std::memset(I.data(), 0, sizeof(SInst<TF>));
+        I[0].nodes = M->d_nodes;
+        I[0].fn = dfn;
+        I[0].fac = reinterpret_cast<const TF*>(F->d_fac);
+        I[0].ipiv = F->d_piv;
+        I[0].wy = ws;
+        I[0].wz = ws + yt * Kd;
+        I[0].nrhs = Kd;
+        HIP_TRY(tb.put(&di, I));
+        HIP_TRY(tb.put(&dw, S.rec));
+        HIP_TRY(launch_cut_seed<TF>(ws, dbx, Ku, Kd, 0, st));
+        HIP_TRY(launch_cut_seed<TF>(ws, dbroot, nroot, Kd, Ku, st));
+        hss_status s = launch_schedule<TF>(di, dw, S, Kd, F->lds_groot, F->lds_groot_res, F->lds_groot2,
+                                           F->lds_groot2s, st);
+        if (s != HSS_OK) return s;
+        TF* T = nullptr;
+        HIP_TRY(dalloc(&T, (size_t)Mu * Ku + (size_t)Md * Kd));
+        F->d_cutT = T;
+        TF* Td = T + (size_t)Mu * Ku;
+        HIP_TRY(launch_cut_gather<TF>(T, Mu, Ku, dbup, ws, Kd, st));
+        HIP_TRY(launch_cut_gather<TF>(Td, Md, Kd, dbdown, ws, Kd, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        int64_t tab = 0;
+        HIP_TRY(upload_cuts(std::vector<CutSpec>{CutSpec{T, Mu, Ku, rx, rup}, CutSpec{Td, Md, Kd, rxd, rdown}},
+                            &F->d_cut, F->cut, &tab));
+        F->cut_depth = c;
+        F->dev_bytes += (int64_t)(sizeof(TF) * ((size_t)Mu * Ku + (size_t)Md * Kd)) + tab;
     }
     return HSS_OK;
 }
