@@ -2467,7 +2467,8 @@ constexpr int CUT_RPT = (CUT_MAXK + CUT_THREADS - 1) / CUT_THREADS;             
 // same round of loads as the row tables, then the stacked input (K x 16) is gathered into
 // LDS through the input row table; after the barrier each wave accumulates its share on
 // MFMA and the partial sums meet in LDS.  The record carries T and both row tables, so
-// the loads start from the record alone.  LDS: K x 17 input, then 8 waves x 256 sums.
+// the loads start from the record alone.  LDS: 1024 x 17 input (rows past K unused), then
+// 8 waves x 256 sums.
 template <typename TF, typename Inst, typename Rec>
 __global__ __launch_bounds__(CUT_THREADS, 1) void k_cut(const Inst* __restrict__ insts, const Rec* __restrict__ work,
                                                      int nwork, int tn) {
@@ -2484,30 +2485,22 @@ __global__ __launch_bounds__(CUT_THREADS, 1) void k_cut(const Inst* __restrict__
     const int S = (K + 3) / 4;
     const int s0 = S * wave / nw, ns = S * (wave + 1) / nw - s0;
     const gptr<const TF> T = gbl(reinterpret_cast<const TF*>(c.T) + c.row0 + (lr < c.rows ? lr : c.rows - 1));
-    TF a[CUT_MAXS];
-#pragma unroll
-    for (int u = 0; u < CUT_MAXS; ++u) {
-        const int k = 4 * (s0 + u) + lk;
-        a[u] = (u < ns && k < K) ? T[(int64_t)k * c.lda] : TF(0);
-    }
+    // Every global load below is unconditional, its index clamped into range, and no branch
+    // depends on its value, so the loads of one round are all in flight at once; the row
+    // table goes first, as the input gather waits for it.  Steps past the wave's share or
+    // past K get a zero x row at the MFMA instead; columns past nrhs repeat the last one and
+    // are never stored.
     const gptr<const CutRow> xrow = gbl(reinterpret_cast<const CutRow*>(c.xrow));
     CutRow xr[CUT_RPT];
 #pragma unroll
-    for (int q = 0; q < CUT_RPT; ++q) {
-        const int k = threadIdx.x + q * CUT_THREADS;
-        xr[q] = CutRow{0, 0, 0};
-        if (k < K) xr[q] = xrow[k];
-    }
-    CutRow orow[4];
-    if (wave == 0) {
-        const gptr<const CutRow> yrow = gbl(reinterpret_cast<const CutRow*>(c.yrow));
+    for (int q = 0; q < CUT_RPT; ++q) xr[q] = xrow[min((int)threadIdx.x + q * CUT_THREADS, K - 1)];
+    TF a[CUT_MAXS];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int ii = Mfma<TF>::row(lk, r);
-            orow[r] = CutRow{0, 0, 0};
-            if (ii < c.rows) orow[r] = yrow[c.row0 + ii];
-        }
-    }
+    for (int u = 0; u < CUT_MAXS; ++u) a[u] = T[(int64_t)min(4 * (s0 + u) + lk, K - 1) * c.lda];
+    const gptr<const CutRow> yrow = gbl(reinterpret_cast<const CutRow*>(c.yrow));
+    CutRow orow[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) orow[r] = yrow[c.row0 + min(Mfma<TF>::row(lk, r), c.rows - 1)];
     const Inst& I = insts[rc.inst];
     const int nrhs = I.nrhs, j0 = tj * CUT_TW;
     if (j0 >= nrhs) return;  // the same for every thread of the workgroup
@@ -2515,32 +2508,37 @@ __global__ __launch_bounds__(CUT_THREADS, 1) void k_cut(const Inst* __restrict__
     const gptr<TF> W = gbl(cut_ws(I));
     constexpr int xp = CUT_TW + 1;
     extern __shared__ unsigned char lds_raw[];
-    TF* xs = reinterpret_cast<TF*>(lds_raw);
-    TF* red = xs + (((int64_t)K * xp + 15) & ~int64_t(15));
+    TF* xs = reinterpret_cast<TF*>(lds_raw);  // CUT_RPT * CUT_THREADS rows (those past K unused)
+    TF* red = xs + CUT_RPT * CUT_THREADS * xp;
     TF v[CUT_RPT][CUT_TW];
 #pragma unroll
     for (int q = 0; q < CUT_RPT; ++q) {
-        const int k = threadIdx.x + q * CUT_THREADS;
         const int64_t base = xr[q].off * nrhs + xr[q].inner + (int64_t)j0 * xr[q].ld;
 #pragma unroll
-        for (int j = 0; j < CUT_TW; ++j) v[q][j] = (k < K && j < nc) ? W[base + (int64_t)j * xr[q].ld] : TF(0);
+        for (int j = 0; j < CUT_TW; ++j) v[q][j] = W[base + (int64_t)min(j, nc - 1) * xr[q].ld];
     }
 #pragma unroll
-    for (int q = 0; q < CUT_RPT; ++q) {
-        const int k = threadIdx.x + q * CUT_THREADS;
-        if (k < K)
+    for (int q = 0; q < CUT_RPT; ++q)
 #pragma unroll
-            for (int j = 0; j < CUT_TW; ++j) xs[k * xp + j] = v[q][j];
-    }
+        for (int j = 0; j < CUT_TW; ++j) xs[(threadIdx.x + q * CUT_THREADS) * xp + j] = v[q][j];
     __syncthreads();
+    // groups of 8 k-steps: the group's LDS reads first, then its MFMAs
     V4 a0 = V4{TF(0), TF(0), TF(0), TF(0)}, a1 = a0;
 #pragma unroll
-    for (int u = 0; u < CUT_MAXS; ++u) {
-        if (u < ns) {
-            const int k = 4 * (s0 + u) + lk;
-            const TF b = xs[(k < K ? k : K - 1) * xp + lr];  // a[u] is 0 past K
-            if (u & 1) a1 = Mfma<TF>::mac(a[u], b, a1);
-            else a0 = Mfma<TF>::mac(a[u], b, a0);
+    for (int g = 0; g < CUT_MAXS; g += 8) {
+        if (g < ns) {
+            TF b[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = 4 * (s0 + g + u) + lk;
+                const TF t = xs[min(k, K - 1) * xp + lr];
+                b[u] = (g + u < ns && k < K) ? t : TF(0);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (u & 1) a1 = Mfma<TF>::mac(a[g + u], b[u], a1);
+                else a0 = Mfma<TF>::mac(a[g + u], b[u], a0);
+            }
         }
     }
 #pragma unroll
@@ -2553,10 +2551,9 @@ __global__ __launch_bounds__(CUT_THREADS, 1) void k_cut(const Inst* __restrict__
         for (int r = 0; r < 4; ++r) sum[r] += red[w * 256 + r * 64 + lane];
     if (lr >= nc) return;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int ii = Mfma<TF>::row(lk, r);
-        if (ii < c.rows) W[orow[r].off * nrhs + orow[r].inner + (int64_t)(j0 + lr) * orow[r].ld] = sum[r];
-    }
+    for (int r = 0; r < 4; ++r)
+        if (Mfma<TF>::row(lk, r) < c.rows)
+            W[orow[r].off * nrhs + orow[r].inner + (int64_t)(j0 + lr) * orow[r].ld] = sum[r];
 }
 
 // The identity seed of a cut's build (nrhs columns): row k < K of a stacked input gets a 1
@@ -3119,7 +3116,8 @@ hipError_t launch_unit_cols(double* E, int64_t ld, int64_t j0, int w, int64_t ro
 }
 int cut_max_k() { return CUT_MAXK; }
 size_t cut_lds(size_t w, int max_k) {
-    return w * (size_t)((((int64_t)max_k * (CUT_TW + 1) + 15) & ~int64_t(15)) + (CUT_THREADS / 64) * 256);
+    (void)max_k;
+    return w * (size_t)(CUT_RPT * CUT_THREADS * (CUT_TW + 1) + (CUT_THREADS / 64) * 256);
 }
 template <typename TF, typename Inst, typename Rec>
 hipError_t launch_cut(const Inst* insts, const Rec* work, int nwork, int max_nrhs, int max_k, hipStream_t st) {
