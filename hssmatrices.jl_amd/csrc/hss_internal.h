@@ -99,10 +99,6 @@ struct WorkItem {
 //                     x1 = parent's zrow, rows from add1, pitch l1 = the parent's n;
 //                     y = own zrow, or col_off into X at a leaf
 //   root: inst and node only
-//   cut (k_cut): inst; A = T, x0 = its input row table, x1 = its output row table
-//                (device addresses), l0 = T's rows (lda), K = its columns, r0 = the first
-//                output row of the strip, My = the strip's rows (<= 16); the workspace is
-//                wy, with the z blocks at wy + yrows * nrhs
 struct SweepRec {
     int32_t inst, node, leaf, nk;
     int32_t K, My, r0, l0;
@@ -127,12 +123,12 @@ struct MvRec {
     int64_t A, x0, x1, y;
 };
 
-// A tree cut: the sweep steps above the nodes of one depth (the cut nodes) collapsed into
-// dense products y = T x (T M x K, column-major, lda M).  Row k of the stacked input x is
-// read from W + xrow[k].off * nrhs + xrow[k].inner (column pitch xrow[k].ld) and row g of
-// the output y is written to W + yrow[g].off * nrhs + yrow[g].inner (pitch yrow[g].ld),
-// where W is the call's sweep workspace (matvec: Z followed by F; solve: the y blocks
-// followed by the z blocks) and row bases are in units of nrhs.
+// A tree cut of the matvec: the sweep steps above the nodes of one depth (the cut nodes)
+// collapsed into one dense product y = T x (T M x K, column-major, lda M).  Row k of the
+// stacked input x is read from W + xrow[k].off * nrhs + xrow[k].inner (column pitch
+// xrow[k].ld) and row g of the output y is written to W + yrow[g].off * nrhs +
+// yrow[g].inner (pitch yrow[g].ld), where W is the call's matvec workspace (Z followed by F)
+// and row bases are in units of nrhs.
 struct CutRow {
     int64_t off;
     int32_t inner, ld;

@@ -2449,12 +2449,7 @@ struct CutArgs {
     int64_t T, xrow, yrow;
 };
 __device__ __forceinline__ CutArgs cut_args(const MvRec& r) { return CutArgs{r.r0, r.M, r.K, r.ld0, r.A, r.x0, r.x1}; }
-__device__ __forceinline__ CutArgs cut_args(const SweepRec& r) {
-    return CutArgs{r.r0, r.My, r.K, r.l0, r.A, r.x0, r.x1};
-}
 __device__ __forceinline__ double* cut_ws(const MvInst& I) { return I.Z; }
-template <typename TF>
-__device__ __forceinline__ TF* cut_ws(const SInst<TF>& I) { return I.wy; }
 
 constexpr int CUT_TW = 16;       // right-hand-side columns per workgroup
 constexpr int CUT_THREADS = 512;
@@ -2667,11 +2662,6 @@ __global__ void k_axpy_batch(const BatchVec* __restrict__ tab) {
 // host-side launchers
 // ---------------------------------------------------------------------------
 static inline int rhs_tiles(int nrhs) { return (nrhs + RHS_TILE - 1) / RHS_TILE; }
-
-static hipError_t lds_attr(const void* fn, size_t lds) {
-    if (lds > 65536) return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    return hipSuccess;
-}
 
 template <typename T>
 static hipError_t set_lds_attr(const void* fn, size_t lds) {
@@ -3134,10 +3124,6 @@ hipError_t launch_cut(const Inst* insts, const Rec* work, int nwork, int max_nrh
     return hipGetLastError();
 }
 template hipError_t launch_cut<double, MvInst, MvRec>(const MvInst*, const MvRec*, int, int, int, hipStream_t);
-template hipError_t launch_cut<double, SInst<double>, SweepRec>(const SInst<double>*, const SweepRec*, int, int, int,
-                                                                 hipStream_t);
-template hipError_t launch_cut<float, SInst<float>, SweepRec>(const SInst<float>*, const SweepRec*, int, int, int,
-                                                               hipStream_t);
 template <typename TF>
 hipError_t launch_cut_seed(TF* W, const CutRow* map, int K, int64_t nrhs, int col0, hipStream_t st) {
     if (K <= 0) return hipSuccess;
@@ -3153,9 +3139,7 @@ hipError_t launch_cut_gather(TF* T, int M, int cols, const CutRow* map, const TF
     return hipGetLastError();
 }
 template hipError_t launch_cut_seed<double>(double*, const CutRow*, int, int64_t, int, hipStream_t);
-template hipError_t launch_cut_seed<float>(float*, const CutRow*, int, int64_t, int, hipStream_t);
 template hipError_t launch_cut_gather<double>(double*, int, int, const CutRow*, const double*, int64_t, hipStream_t);
-template hipError_t launch_cut_gather<float>(float*, int, int, const CutRow*, const float*, int64_t, hipStream_t);
 
 hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, double* out, hipStream_t st) {
     if (nrhs <= 0) return hipSuccess;

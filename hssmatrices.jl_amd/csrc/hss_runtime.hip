@@ -79,8 +79,8 @@ static const FactorKernel g_factor_kernel = [] {
     if (e && std::strcmp(e, "mfma") == 0) return FactorKernel::Mfma;
     return FactorKernel::Auto;
 }();
-// HSS_CUT=0 runs every level of the matvec and solve sweeps as a launch of its own (no tree
-// cut), for comparing the two; the results agree to rounding.
+// HSS_CUT=0 runs every level of the matvec sweeps as a launch of its own (no tree cut), for
+// comparing the two; the results agree to rounding.
 static const bool g_cut = [] {
     const char* e = std::getenv("HSS_CUT");
     return !(e && std::strcmp(e, "0") == 0);
@@ -204,10 +204,6 @@ struct Schedule {
     Range roots;
     // per-level choices made by the autotuner (hss_plan_tune); empty = built-in rules
     std::vector<LevelCfg> up_cfg, down_cfg;
-    // the tree cuts of the instances that have one (k_cut records in rec): cut_up (product
-    // 0) runs before the roots, cut (product 1) after them; *_k = the largest Kc
-    Range cut_up, cut;
-    int cut_up_k = 0, cut_k = 0;
 };
 // Largest LDS of a workgroup (bytes); levels above it use the streamed kernels.
 constexpr size_t kLdsResidentMax = 160 * 1024;
@@ -356,12 +352,6 @@ struct hss_fact_s {
     int64_t dev_bytes = 0;
     // captured single-request plans keyed by their buffers (repeated solves replay a graph)
     std::vector<std::pair<PlanKey, hss_plan_t>> plan_cache;
-    // tree cut of the solve (cut_depth 0: none): its products [0] T_up and [1] T_down
-    // (factor type, one allocation) and their row tables
-    int cut_depth = 0;
-    std::vector<CutInfo> cut;
-    void* d_cutT = nullptr;
-    char* d_cut = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -1853,14 +1843,10 @@ static void free_fact(hss_fact_s* F) {
     (void)hipFree(F->d_nrm);
     (void)hipFree(F->d_io_b);
     (void)hipFree(F->d_io_x);
-    (void)hipFree(F->d_cutT);
-    (void)hipFree(F->d_cut);
     delete F;
 }
 
 static hss_status ensure_schedule(hss_fact_s* F);
-template <typename TF>
-static hss_status solve_cut_build(const std::vector<hss_fact_s*>& facts, hipStream_t st);
 
 // Factor a batch of matrices that share one context: one launch per height
 // across all of them, then one root launch.
@@ -2054,12 +2040,8 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
         F->fac_len = F->res_len;
         F->dev_bytes = (int64_t)(sizeof(TF) * F->fac_len + sizeof(FNode) * F->hfn.size() + 4 * (F->hfn[0].m + 1));
     }
-    // tree cuts of the solves, then the single-request solve schedule, built here so that
-    // solves (the allocation-free one included) find it ready
-    {
-        hss_status s2 = solve_cut_build<TF>(out, st);
-        if (s2 != HSS_OK) { cleanup(); return s2; }
-    }
+    // the single-request solve schedule, built here so that solves (the allocation-free one
+    // included) find it ready
     for (int b = 0; b < cnt; ++b) {
         hss_status s2 = ensure_schedule(out[b]);
         if (s2 != HSS_OK) { cleanup(); return s2; }
@@ -2227,13 +2209,8 @@ static void schedule_levels(Schedule& S, int nup, int ndown) {
     S.down_m.assign(ndown, 0);
     S.down_na.assign(ndown, 0);
     S.down_k.assign(ndown, 0);
-    S.cut_up = S.cut = Range{};
-    S.cut_up_k = S.cut_k = 0;
 }
 
-// A factor with a tree cut at depth c runs the up steps of the nodes at depth >= c, cut
-// product 0 (the depth-1 nodes' reduced b and u), the root step, cut product 1 (the z of the
-// depth c - 1 nodes) and the down steps of the nodes at depth >= c.
 static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
     const int cnt = (int)facts.size();
     int maxh = 0, maxd = 0;
@@ -2249,10 +2226,7 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
             hss_mat_s* M = facts[b]->mat;
             if (h >= M->max_height) continue;
             const Range& r = M->by_height[h];
-            for (int t = 0; t < r.count; ++t) {
-                const int node = M->hwork[r.off + t].node;
-                if (M->hn[node].depth >= facts[b]->cut_depth) add_up_rec(S, h, M, facts[b]->hfn, b, node);
-            }
+            for (int t = 0; t < r.count; ++t) add_up_rec(S, h, M, facts[b]->hfn, b, M->hwork[r.off + t].node);
         }
         S.up[h].count = (int)S.work.size() - S.up[h].off;
     }
@@ -2263,37 +2237,11 @@ static void build_schedule(const std::vector<hss_fact_s*>& facts, Schedule& S) {
         q.inst = b;
         S.rec.push_back(q);
     }
-    // cut products: 16-row strips of T_up and T_down
-    auto strips = [&](Range& r, int& kmax, int which) {
-        r.off = (int)S.work.size();
-        for (int b = 0; b < cnt; ++b) {
-            const hss_fact_s* F = facts[b];
-            if (F->cut_depth <= 0) continue;
-            const CutInfo& ci = F->cut[which];
-            kmax = std::max(kmax, ci.K);
-            for (int r0 = 0; r0 < ci.M; r0 += 16) {
-                S.work.push_back(WorkItem{b, -1});
-                SweepRec q{};
-                q.inst = b;
-                q.r0 = r0;
-                q.My = std::min(16, ci.M - r0);
-                q.K = ci.K;
-                q.l0 = ci.M;
-                q.A = reinterpret_cast<int64_t>(ci.T);
-                q.x0 = reinterpret_cast<int64_t>(ci.xrow);
-                q.x1 = reinterpret_cast<int64_t>(ci.yrow);
-                S.rec.push_back(q);
-            }
-        }
-        r.count = (int)S.work.size() - r.off;
-    };
-    strips(S.cut_up, S.cut_up_k, 0);
-    strips(S.cut, S.cut_k, 1);
     for (int d = 1; d <= maxd; ++d) {
         S.down[d].off = (int)S.work.size();
         for (int b = 0; b < cnt; ++b) {
             hss_mat_s* M = facts[b]->mat;
-            if (d > M->max_depth || d < facts[b]->cut_depth) continue;
+            if (d > M->max_depth) continue;
             const Range& r = M->by_depth[d];
             for (int t = 0; t < r.count; ++t) add_down_rec(S, d, M, facts[b]->hfn, b, M->hwork[r.off + t].node);
         }
@@ -2366,10 +2314,6 @@ static hss_status launch_schedule(const SInst<TF>* di, const SweepRec* dw, const
         hss_status s = launch_up_level<TF>(di, dw, S, h, maxnrhs, h < S.up_cfg.size() ? S.up_cfg[h] : LevelCfg{}, st);
         if (s != HSS_OK) return s;
     }
-    if (S.cut_up.count > 0) {
-        ProfScope ps("k_cut", 0, S.cut_up.count, st);
-        HIP_TRY((launch_cut<TF, SInst<TF>, SweepRec>(di, dw + S.cut_up.off, S.cut_up.count, maxnrhs, S.cut_up_k, st)));
-    }
     if (S.roots.count > 0) {
         if (lgroot2 > 0 && lgroot2s > 0 && lgroot2s <= kLdsResidentMax) {
             ProfScope ps("k_ulv_groot2", 0, S.roots.count, st);
@@ -2384,158 +2328,10 @@ static hss_status launch_schedule(const SInst<TF>* di, const SweepRec* dw, const
                                           res ? 1 : 0, st)));
         }
     }
-    if (S.cut.count > 0) {
-        ProfScope ps("k_cut", 1, S.cut.count, st);
-        HIP_TRY((launch_cut<TF, SInst<TF>, SweepRec>(di, dw + S.cut.off, S.cut.count, maxnrhs, S.cut_k, st)));
-    }
     for (size_t d = 1; d < S.down.size(); ++d) {
         hss_status s =
             launch_down_level<TF>(di, dw, S, d, maxnrhs, d < S.down_cfg.size() ? S.down_cfg[d] : LevelCfg{}, st);
         if (s != HSS_OK) return s;
-    }
-    return HSS_OK;
-}
-
-// Tree cut of the solve at depth c, two products around the root step (which keeps its
-// refinement):
-//   product 0: the depth-1 nodes' reduced b and u (rows [nk, My) of their y, what the root
-//              reads) from the depth-c nodes' (what their parents' up steps read), T_up;
-//   product 1: the z of the depth c - 1 nodes (n rows each, what the depth-c down steps
-//              read) from the depth-c nodes' reduced b and u and the root's z, T_down.
-// Both are built by running the sweeps above the cut without the root on the identity:
-// Kc + n_root right-hand sides, the first Kc with unit columns in the depth-c nodes' reduced
-// rows and a zero root z, the last n_root with zero inputs and unit columns in the root's z,
-// on y / z rows numbered for the nodes at depth <= c only.  Each factor builds in a launch
-// sequence of its own, so T (and every solve through it) is the same whether the factor was
-// made alone or in a batch.
-template <typename TF>
-static hss_status solve_cut_build(const std::vector<hss_fact_s*>& facts, hipStream_t st) {
-    for (hss_fact_s* F : facts) {
-        const hss_mat_s* M = F->mat;
-        const std::vector<FNode>& of = F->hfn;
-        const int nroot = of[0].n;
-        // products: T_up (Mu x Ku) and T_down (Md x Ku + nroot)
-        auto dims = [&](int c, int64_t& ku, int64_t& mu, int64_t& md) {
-            ku = mu = md = 0;
-            for (int i = 0; i < M->nn; ++i) {
-                const int d = M->hn[i].depth;
-                if (d == c) ku += of[i].My - of[i].nk;
-                if (d == c - 1) md += of[i].n;
-                if (d == 1) mu += of[i].My - of[i].nk;
-            }
-        };
-        const int c = choose_cut(M, F->res_len / 32, [&](int c, int64_t& k, int64_t& m, int64_t& t) {
-            int64_t ku, mu, md;
-            dims(c, ku, mu, md);
-            k = ku > 0 ? ku + nroot : 0;
-            m = std::max(mu, md);
-            t = mu * ku + md * (ku + nroot);
-        });
-        if (c == 0) continue;
-        int64_t ku64, mu64, md64;
-        dims(c, ku64, mu64, md64);
-        const int Ku = (int)ku64, Kd = Ku + nroot;
-        // compact rows: y for depths 1..c, z for depths 0..c-1 (after the y rows)
-        std::vector<FNode> fn = of;
-        int64_t yt = 0, zt = 0;
-        for (int i = 0; i < M->nn; ++i) {
-            FNode& f = fn[i];
-            const int d = M->hn[i].depth;
-            f.yrow = f.zrow = 0;
-            if (d >= 1 && d <= c) {
-                f.yrow = yt;
-                yt += f.My;
-            }
-            if (d <= c - 1) {
-                f.zrow = zt;
-                zt += f.n;
-            }
-        }
-        // row tables: b* of the build (workspace y then z, Kd columns), r* of the solves
-        std::vector<CutRow> bx, broot, bup, bdown, rx, rup, rdown;
-        for (int i = 0; i < M->nn; ++i) {
-            const int d = M->hn[i].depth;
-            const FNode &f = fn[i], &o = of[i];
-            if (d == c)
-                for (int r = 0; r < f.My - f.nk; ++r) {
-                    bx.push_back(CutRow{f.yrow, f.nk + r, f.My});
-                    rx.push_back(CutRow{o.yrow, f.nk + r, f.My});
-                }
-            if (d == 1)
-                for (int r = 0; r < f.My - f.nk; ++r) {
-                    bup.push_back(CutRow{f.yrow, f.nk + r, f.My});
-                    rup.push_back(CutRow{o.yrow, f.nk + r, f.My});
-                }
-            if (d == c - 1)
-                for (int r = 0; r < f.n; ++r) {
-                    bdown.push_back(CutRow{yt + f.zrow, r, f.n});
-                    rdown.push_back(CutRow{F->yrows + o.zrow, r, f.n});
-                }
-        }
-        for (int r = 0; r < nroot; ++r) broot.push_back(CutRow{yt + fn[0].zrow, r, nroot});
-        std::vector<CutRow> rxd = rx;
-        for (int r = 0; r < nroot; ++r) rxd.push_back(CutRow{F->yrows + of[0].zrow, r, nroot});
-        const int Mu = (int)bup.size(), Md = (int)bdown.size();
-        // up steps by depth (c - 1 first), no root, down steps by depth (1 .. c - 1)
-        Schedule S;
-        schedule_levels(S, c - 1, c);
-        S.w = sizeof(TF);
-        for (int h = 0; h < c - 1; ++h) {
-            const int d = c - 1 - h;
-            S.up[h].off = (int)S.work.size();
-            for (int i = 1; i < M->nn; ++i)
-                if (M->hn[i].depth == d) add_up_rec(S, h, M, fn, 0, i);
-            S.up[h].count = (int)S.work.size() - S.up[h].off;
-        }
-        S.roots = Range{(int)S.work.size(), 0};
-        for (int d = 1; d <= c - 1; ++d) {
-            S.down[d].off = (int)S.work.size();
-            for (int i = 1; i < M->nn; ++i)
-                if (M->hn[i].depth == d) add_down_rec(S, d, M, fn, 0, i);
-            S.down[d].count = (int)S.work.size() - S.down[d].off;
-        }
-        const size_t wsz = (size_t)(yt + zt) * Kd;
-        TmpBufs tb(st);
-        TF* ws = nullptr;
-        FNode* dfn = nullptr;
-        CutRow *dbx = nullptr, *dbroot = nullptr, *dbup = nullptr, *dbdown = nullptr;
-        SInst<TF>* di = nullptr;
-        SweepRec* dw = nullptr;
-        HIP_TRY(tb.alloc(&ws, wsz));
-        HIP_TRY(hipMemsetAsync(ws, 0, sizeof(TF) * wsz, st));
-        HIP_TRY(tb.put(&dfn, fn));
-        HIP_TRY(tb.put(&dbx, bx));
-        HIP_TRY(tb.put(&dbroot, broot));
-        HIP_TRY(tb.put(&dbup, bup));
-        HIP_TRY(tb.put(&dbdown, bdown));
-        std::vector<SInst<TF>> I(1);
-        std::memset(I.data(), 0, sizeof(SInst<TF>));
-        I[0].nodes = M->d_nodes;
-        I[0].fn = dfn;
-        I[0].fac = reinterpret_cast<const TF*>(F->d_fac);
-        I[0].ipiv = F->d_piv;
-        I[0].wy = ws;
-        I[0].wz = ws + yt * Kd;
-        I[0].nrhs = Kd;
-        HIP_TRY(tb.put(&di, I));
-        HIP_TRY(tb.put(&dw, S.rec));
-        HIP_TRY(launch_cut_seed<TF>(ws, dbx, Ku, Kd, 0, st));
-        HIP_TRY(launch_cut_seed<TF>(ws, dbroot, nroot, Kd, Ku, st));
-        hss_status s = launch_schedule<TF>(di, dw, S, Kd, F->lds_groot, F->lds_groot_res, F->lds_groot2,
-                                           F->lds_groot2s, st);
-        if (s != HSS_OK) return s;
-        TF* T = nullptr;
-        HIP_TRY(dalloc(&T, (size_t)Mu * Ku + (size_t)Md * Kd));
-        F->d_cutT = T;
-        TF* Td = T + (size_t)Mu * Ku;
-        HIP_TRY(launch_cut_gather<TF>(T, Mu, Ku, dbup, ws, Kd, st));
-        HIP_TRY(launch_cut_gather<TF>(Td, Md, Kd, dbdown, ws, Kd, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        int64_t tab = 0;
-        HIP_TRY(upload_cuts(std::vector<CutSpec>{CutSpec{T, Mu, Ku, rx, rup}, CutSpec{Td, Md, Kd, rxd, rdown}},
-                            &F->d_cut, F->cut, &tab));
-        F->cut_depth = c;
-        F->dev_bytes += (int64_t)(sizeof(TF) * ((size_t)Mu * Ku + (size_t)Md * Kd)) + tab;
     }
     return HSS_OK;
 }
