@@ -1394,6 +1394,10 @@ __global__ __launch_bounds__(1024) void k_ulv_root(const FInst<T>* __restrict__ 
 // The reduction of a node (ulvfactor.jl:42,48-51,74,89: Q_ql' b, the forward
 // substitution with L1, the reduced b and u, the children's merges) is linear in
 // its input x = the leaf's rows of B, or [b_1; u_1; b_2; u_2] for a branch.
+template <bool TA, typename TF, typename Store>
+__device__ __forceinline__ void mfma_gemm(int M, int K, const TF* __restrict__ A, int64_t lda, const TF* xs, int ncols,
+                                          Store store, int xp);
+
 // k_ulv_gform runs its steps on columns of the identity:
 //   Gz (nk x Kx):   column c = zloc for x = e_c
 //   Gr (R x (Kx + nk)), R = mr + kv: column c < Kx = [(Q_ql' b)[nk:m]; W_1'u_1 + W_2'u_2]
@@ -1452,31 +1456,31 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_gform(const FInst<TF>* __
     if (!nd.is_leaf) {
         // b = [b_1 - P12 u_2; b_2 - P21 u_1]   (ulvfactor.jl:74)
         const int mrr = m - mrl;
-        gemm<false>(mrl, nt, kvr, I.fac + f.P12, mrl, us + kvl * NTP,
-                    [&](int i, int j, TF v) { bv[i * NTP + j] -= v; });
-        gemm<false>(mrr, nt, kvl, I.fac + f.P21, mrr, us,
-                    [&](int i, int j, TF v) { bv[(mrl + i) * NTP + j] -= v; });
+        mfma_gemm<false, TF>(mrl, kvr, I.fac + f.P12, mrl, us + kvl * NTP, nt,
+                         [&](int i, int j, TF v) { bv[i * NTP + j] -= v; }, NTP);
+        mfma_gemm<false, TF>(mrr, kvl, I.fac + f.P21, mrr, us, nt,
+                         [&](int i, int j, TF v) { bv[(mrl + i) * NTP + j] -= v; }, NTP);
         __syncthreads();
     }
     TF* Grc = Gr + (int64_t)j0 * R;  // column j0 of Gr
     if (!f.pass) {
         // b <- Q_ql' b = b - Y (T (Y' b))   (ulvfactor.jl:42)
         if (k > 0) {
-            gemm<true>(k, nt, m, I.fac + f.Yql, m, bv, [&](int i, int j, TF v) { t1[i * NTP + j] = v; });
+            mfma_gemm<true, TF>(k, m, I.fac + f.Yql, m, bv, nt, [&](int i, int j, TF v) { t1[i * NTP + j] = v; }, NTP);
             __syncthreads();
-            gemm<false>(k, nt, k, I.fac + f.Tql, k, t1, [&](int i, int j, TF v) { t2[i * NTP + j] = v; });
+            mfma_gemm<false, TF>(k, k, I.fac + f.Tql, k, t1, nt, [&](int i, int j, TF v) { t2[i * NTP + j] = v; }, NTP);
             __syncthreads();
-            gemm<false>(m, nt, k, I.fac + f.Yql, m, t2, [&](int i, int j, TF v) { bv[i * NTP + j] -= v; });
+            mfma_gemm<false, TF>(m, k, I.fac + f.Yql, m, t2, nt, [&](int i, int j, TF v) { bv[i * NTP + j] -= v; }, NTP);
             __syncthreads();
         }
         // zloc = L1 \ b[0:nk]  (ulvfactor.jl:48), blocked with inverted diagonal blocks
         for (int bb = 0; bb < f.nblk; ++bb) {
             const int b0 = bb * NB, nb = min(NB, nk - b0);
-            gemm<false>(nb, nt, b0, I.fac + f.L1 + b0, nk, zl,
-                        [&](int i, int j, TF v) { t1[i * NTP + j] = bv[(b0 + i) * NTP + j] - v; });
+            mfma_gemm<false, TF>(nb, b0, I.fac + f.L1 + b0, nk, zl, nt,
+                             [&](int i, int j, TF v) { t1[i * NTP + j] = bv[(b0 + i) * NTP + j] - v; }, NTP);
             __syncthreads();
-            gemm<false>(nb, nt, nb, I.fac + f.L1inv + (int64_t)bb * NB * NB, NB, t1,
-                        [&](int i, int j, TF v) { zl[(b0 + i) * NTP + j] = v; });
+            mfma_gemm<false, TF>(nb, nb, I.fac + f.L1inv + (int64_t)bb * NB * NB, NB, t1, nt,
+                             [&](int i, int j, TF v) { zl[(b0 + i) * NTP + j] = v; }, NTP);
             __syncthreads();
         }
         unstage(I.fac + f.Gz + (int64_t)j0 * nk, nk, zl, 0, nk, nt);
@@ -1526,12 +1530,12 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_qform(const FInst<TF>* __
     for (int bb = f.nblk - 1; bb >= 0; --bb) {
         const int b0 = bb * NB, nb = min(NB, nk - b0);
         const TF* Yb = I.fac + f.Ylq + (int64_t)b0 * n + b0;  // rows b0.., columns b0..b0+nb
-        gemm<true>(nb, nt, n - b0, Yb, n, z + b0 * NTP, [&](int i, int j, TF v) { t1[i * NTP + j] = v; });
+        mfma_gemm<true, TF>(nb, n - b0, Yb, n, z + b0 * NTP, nt, [&](int i, int j, TF v) { t1[i * NTP + j] = v; }, NTP);
         __syncthreads();
-        gemm<false>(nb, nt, nb, I.fac + f.Tlq + (int64_t)bb * NB * NB, NB, t1,
-                    [&](int i, int j, TF v) { t2[i * NTP + j] = v; });
+        mfma_gemm<false, TF>(nb, nb, I.fac + f.Tlq + (int64_t)bb * NB * NB, NB, t1, nt,
+                         [&](int i, int j, TF v) { t2[i * NTP + j] = v; }, NTP);
         __syncthreads();
-        gemm<false>(n - b0, nt, nb, Yb, n, t2, [&](int i, int j, TF v) { z[(b0 + i) * NTP + j] -= v; });
+        mfma_gemm<false, TF>(n - b0, nb, Yb, n, t2, nt, [&](int i, int j, TF v) { z[(b0 + i) * NTP + j] -= v; }, NTP);
         __syncthreads();
     }
     unstage(I.fac + f.Qt + (int64_t)j0 * n, n, z, 0, n, nt);
@@ -1951,13 +1955,16 @@ __device__ __forceinline__ void lds_gemm(int M, int K, const TF* As, const TF* x
 // (lda) in LDS (AL) or in memory, xs row-major in LDS (pitch xp).  A strip covers all
 // (one or two) 16-column blocks, so every A element is read by one wave and feeds one
 // MFMA per column block; with one block the k-steps alternate between acc0 and acc1.
+// TA (A in memory only): the strip's rows are columns of A, i.e. the product is A' xs with
+// A (K x M) column-major (lda).
 // A in memory is loaded straight into registers U k-steps ahead of the MFMAs; the loads
 // are unconditional (columns past K are clamped in bounds and k past K is zeroed at the
 // MFMA), so no wait is forced before a value is used.  Rows past M read row M - 1 and
 // are never stored.
-template <bool AL, bool two, typename TF>
+template <bool AL, bool two, typename TF, bool TA = false>
 __device__ __forceinline__ void strip_acc(int M, int K, const TF* __restrict__ A, int64_t lda, const TF* xs, int xp,
                                           int rb, typename Mfma<TF>::v4& acc0, typename Mfma<TF>::v4& acc1) {
+    static_assert(!(AL && TA), "a transposed A is read from memory only");
     if (K <= 0) return;
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
     const int i = rb * 16 + lr;
@@ -2015,13 +2022,15 @@ __device__ __forceinline__ void strip_acc(int M, int K, const TF* __restrict__ A
     // VGPRs per step.  Every access is inside A: the lane's column is clamped to K - 1
     // (full steps never need it) and steps at or past Sf are loaded by the edge code.
     const int kl = lk < K ? lk : K - 1;
-    const int voff = (ic + kl * (int)lda) * (int)sizeof(TF);
-    const int step = 4 * (int)lda * (int)sizeof(TF);
-    const auto rsrc = buf_rsrc(A, (int)((lda * (K - 1) + M) * (int64_t)sizeof(TF)));
+    // element (row i, column k) of the strip's operator: A[i + k lda], or A[k + i lda] (TA)
+    auto at = [&](int i, int k) { return TA ? i * (int)lda + k : i + k * (int)lda; };
+    const int voff = at(ic, kl) * (int)sizeof(TF);
+    const int step = 4 * (TA ? 1 : (int)lda) * (int)sizeof(TF);
+    const auto rsrc = buf_rsrc(A, (int)((TA ? lda * (M - 1) + K : lda * (K - 1) + M) * (int64_t)sizeof(TF)));
     auto ld = [&](int st) { return buf_load<TF>(rsrc, voff, (st < Sf ? st : (Sf > 0 ? Sf - 1 : 0)) * step); };
     auto ld_edge = [&](int st) {
         const int kk = 4 * st + lk;
-        return buf_load<TF>(rsrc, (ic + (kk < K ? kk : K - 1) * (int)lda) * (int)sizeof(TF), 0);
+        return buf_load<TF>(rsrc, at(ic, kk < K ? kk : K - 1) * (int)sizeof(TF), 0);
     };
     TF a[U];
 #pragma unroll
@@ -2110,6 +2119,25 @@ __device__ __forceinline__ void glb_gemm(int M, int K, const TF* __restrict__ A,
                                          Store store, int xp = XS) {
     if (ncols > 16) glb_gemm_cb<true>(M, K, A, lda, xs, ncols, store, xp);
     else glb_gemm_cb<false>(M, K, A, lda, xs, ncols, store, xp);
+}
+
+// out = op(A) xs on MFMA (M x ncols, ncols <= 32; K terms): op(A) = A (M x K, lda), or A'
+// with A (K x M, lda) when TA; xs row-major in LDS (pitch xp).  No barrier inside.
+template <bool TA, typename TF, typename Store>
+__device__ __forceinline__ void mfma_gemm(int M, int K, const TF* __restrict__ A, int64_t lda, const TF* xs, int ncols,
+                                          Store store, int xp) {
+    using V4 = typename Mfma<TF>::v4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int rb = wave; rb < (M + 15) / 16; rb += nw) {
+        V4 a0 = V4{TF(0), TF(0), TF(0), TF(0)}, a1 = a0;
+        if (ncols > 16) {
+            strip_acc<false, true, TF, TA>(M, K, A, lda, xs, xp, rb, a0, a1);
+            strip_put<true, TF>(M, ncols, rb, lane, a0, a1, store);
+        } else {
+            strip_acc<false, false, TF, TA>(M, K, A, lda, xs, xp, rb, a0, a1);
+            strip_put<false, TF>(M, ncols, rb, lane, a0, a1, store);
+        }
+    }
 }
 
 // The two products of a node's up-sweep step with the input x in xs rows [0, Kx):
