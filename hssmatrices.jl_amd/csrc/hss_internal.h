@@ -33,14 +33,16 @@ struct DevNode {
     int32_t m, n, ku, kv;       // generator sizes; ku/kv = basis ranks (0 at the root)
     int32_t left, right, parent;
     int32_t is_leaf, is_left;   // is_left: this node is its parent's first child
-    int32_t depth, height, pad;
+    int32_t depth, height;
+    int32_t fold;               // leaf: its down step is folded into its leaf product (opC below)
     int64_t row_off, col_off;
     int64_t D, U, V, B12, B21, R, W;  // element offsets into the generator blob (-1 = none)
     int64_t zrow, frow;         // matvec workspace rows: Z (kv rows), F (ku rows)
     // matvec operators (column-major, element offsets into the operator buffer, -1 = none):
     //   opZ kv x (n | kv_1 + kv_2): V' at a leaf, [W_1' W_2'] at a branch
     //   opF ku x (kv_sibling + ku_parent): [B_sibling R] (B12 for a left child, B21 for a right one)
-    //   opC m x (n + ku) at a leaf: [D U]
+    //   opC m x (n + ku) at a leaf: [D U]; a folded leaf's is m x (n + kv_sibling + ku_parent):
+    //       [D, U B_sibling, U R], so C = D x + U (B_sibling Z_sibling + R F_parent) in one product
     int64_t opZ, opF, opC;
 };
 
@@ -111,16 +113,22 @@ struct SweepRec {
 // element offset A of the operator buffer and rows [0, r0) of x from segment 0:
 //   kind 0 (up):   Z_node = opZ [B rows (leaf) | Z_left; Z_right]
 //   kind 1 (down): F_node = opF [Z_sibling; F_parent]
-//   kind 2 (leaf): C rows = alpha opC [B rows; F_node] + beta C rows
+//   kind 2 (leaf): C rows = alpha opC [B rows; F_node] + beta C rows, or for a folded leaf
+//                  alpha opC [B rows; Z_sibling; F_parent] + beta C rows
 // B / C offsets (x0 with from_b, y of kind 2) are row offsets (pitch: the call's ldb /
 // ldc); Z / F offsets are row bases in units of nrhs elements, with pitches ld0, ld1, ldy.
 // A cut record (k_cut): inst; A = T, x0 = its input row table, x1 = its output row table
 // (device addresses), ld0 = T's rows (lda), K = its columns, r0 = the first output row of
 // the strip, M = the strip's rows (<= 16); the workspace is Z, with F at Z + zrows * nrhs.
+// Rows [r0, r1) of x come from segment 1 and rows [r1, K) from segment 2 (x2, pitch ld2;
+// folded leaves only).  A leaf's segments 1 and 2 are row bases into Z, whose F part
+// follows at Z + zrows * nrhs.
 struct MvRec {
     int32_t inst, M, K, r0;
     int32_t ld0, ld1, ldy, from_b;
     int64_t A, x0, x1, y;
+    int32_t r1, ld2;
+    int64_t x2;
 };
 
 // A tree cut of the matvec: the sweep steps above the nodes of one depth (the cut nodes)

@@ -1692,10 +1692,13 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_ulv_rinv(const FInst<TF>* __r
 // x operand: rows [0, r0) from segment 0, [r0, r1) from segment 1 (global,
 // column-major, Float64 or the factor type), rows [r1, K) from an LDS block zs
 // (row-major, pitch zp, columns local to the workgroup tile).
+// Rows [0, r0) of x from p0, [r0, r1) from p1, then from p2 (Float64, pitch ld2) when it
+// is set, else from the LDS rows zs.
 struct SegOp {
     const void* p0;
     const void* p1;
-    int64_t ld0, ld1;
+    const void* p2;
+    int64_t ld0, ld1, ld2;
     int r0, r1, dbl0, dbl1;
 };
 
@@ -1711,6 +1714,7 @@ __device__ __forceinline__ TF seg_load(const SegOp& x, const TF* zs, int zp, int
         return x.dbl1 ? (TF)gbl(static_cast<const double*>(x.p1))[r + j * x.ld1]
                       : gbl(static_cast<const TF*>(x.p1))[r + j * x.ld1];
     }
+    if (x.p2) return (TF)gbl(static_cast<const double*>(x.p2))[(r - x.r1) + j * x.ld2];
     return zs[(r - x.r1) * zp + jl];
 }
 
@@ -1845,6 +1849,8 @@ __global__ __launch_bounds__(256) void k_ulv_gup(const SInst<TF>* __restrict__ i
     TF* zs = reinterpret_cast<TF*>(lds_raw);
     const int My = rc.My, nk = rc.nk, Kx = rc.K, R = My - nk;
     SegOp x;
+    x.p2 = nullptr;
+    x.ld2 = 0;
     if (rc.leaf) {
         x.p0 = I.B + rc.x0;
         x.ld0 = I.ldb;
@@ -1892,6 +1898,8 @@ __global__ __launch_bounds__(256) void k_ulv_gdown(const SInst<TF>* __restrict__
     const int i0 = ti * 32 * WM, j0 = tj * 32 * WN;
     if (i0 >= n || j0 >= nrhs) return;
     SegOp x;
+    x.p2 = nullptr;
+    x.ld2 = 0;
     x.p0 = I.wy + rc.x0 * nrhs;
     x.ld0 = rc.My;
     x.dbl0 = 0;
@@ -2359,6 +2367,7 @@ __global__ __launch_bounds__(512) void k_ulv_groot2(const SInst<TF>* __restrict_
 //   up   (per height):  Z = V' B_seg (leaf) | [W_1' W_2'] [Z_1; Z_2]       (_matmatup, matmul.jl:32-42)
 //   down (per depth):   F = [B_sibling R] [Z_sibling; F_parent]             (_matmatdown!, matmul.jl:51-57)
 //   leaf:               C_seg = alpha [D U] [B_seg; F] + beta C_seg         (matmul.jl:46-47)
+//                       or, down step folded in, alpha [D, U B_sibling, U R] [B_seg; Z_sibling; F_parent]
 struct MvOp {
     const double* A;
     int M, K;
@@ -2376,7 +2385,7 @@ __device__ __forceinline__ void mv_op(const MvInst& I, const MvRec& rc, MvOp& o)
     o.K = rc.K;
     o.x.dbl0 = o.x.dbl1 = 1;
     o.x.r0 = rc.r0;
-    o.x.r1 = rc.K;
+    o.x.r1 = rc.r1;
     if (rc.from_b) {
         o.x.p0 = I.B + rc.x0;
         o.x.ld0 = I.ldb;
@@ -2384,8 +2393,10 @@ __device__ __forceinline__ void mv_op(const MvInst& I, const MvRec& rc, MvOp& o)
         o.x.p0 = I.Z + rc.x0 * nrhs;
         o.x.ld0 = rc.ld0;
     }
-    o.x.p1 = (KIND == 0 ? I.Z : I.F) + rc.x1 * nrhs;
+    o.x.p1 = (KIND == 1 ? I.F : I.Z) + rc.x1 * nrhs;  // a leaf's rows are Z-based (F follows Z)
     o.x.ld1 = rc.ld1;
+    o.x.p2 = (KIND == 2 && rc.r1 < rc.K) ? I.Z + rc.x2 * nrhs : nullptr;
+    o.x.ld2 = rc.ld2;
     if (KIND == 2) {
         o.y = I.C + rc.y;
         o.ldy = I.ldc;
@@ -2426,13 +2437,15 @@ __global__ __launch_bounds__(NT, 4) void k_mv_s(const MvInst* __restrict__ insts
     const int r0 = o.x.r0;
     const gptr<const double> x0 = gbl(static_cast<const double*>(o.x.p0) + (int64_t)j0 * o.x.ld0);
     const gptr<const double> x1 = gbl(static_cast<const double*>(o.x.p1) + (int64_t)j0 * o.x.ld1);
-    const int64_t l0 = o.x.ld0, l1 = o.x.ld1;
+    const gptr<const double> x2 = gbl(static_cast<const double*>(o.x.p2) + (int64_t)j0 * o.x.ld2);
+    const int64_t l0 = o.x.ld0, l1 = o.x.ld1, l2 = o.x.ld2;
+    const int r1 = o.x.r1;
     stage2(
         As, o.A, AR ? na : 0, K * tw,
         [&](int e) {
             const int k = e % K, j = e / K;
             if (j >= nc) return 0.0;
-            return k < r0 ? x0[k + j * l0] : x1[(k - r0) + j * l1];
+            return k < r0 ? x0[k + j * l0] : k < r1 ? x1[(k - r0) + j * l1] : x2[(k - r1) + j * l2];
         },
         [&](int e, double v) { xs[(e % K) * xp + e / K] = v; });
     __syncthreads();

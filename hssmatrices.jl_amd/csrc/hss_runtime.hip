@@ -488,6 +488,10 @@ static void mv_shape(const std::vector<DevNode>& hn, int i, int kind, int& rows,
         const DevNode& sb = hn[d.is_left ? p.right : p.left];
         rows = d.ku;
         cols = sb.kv + p.ku;
+    } else if (d.fold) {
+        const DevNode& p = hn[d.parent];
+        rows = d.m;
+        cols = d.n + hn[d.is_left ? p.right : p.left].kv + p.ku;
     } else {
         rows = d.m;
         cols = d.n + d.ku;
@@ -495,8 +499,8 @@ static void mv_shape(const std::vector<DevNode>& hn, int i, int kind, int& rows,
 }
 
 // The matvec record of node i (kind 0 up, 1 down, 2 leaf) of instance b, with the Z / F
-// rows of the node table hn.
-static MvRec mv_record(const std::vector<DevNode>& hn, int b, int i, int kind) {
+// rows of the node table hn (a leaf's F rows are offset by zrows: F follows Z).
+static MvRec mv_record(const std::vector<DevNode>& hn, int64_t zrows, int b, int i, int kind) {
     const DevNode& d = hn[i];
     MvRec q{};
     q.inst = b;
@@ -504,6 +508,7 @@ static MvRec mv_record(const std::vector<DevNode>& hn, int b, int i, int kind) {
     mv_shape(hn, i, kind, rows, cols);
     q.M = rows;
     q.K = cols;
+    q.r1 = cols;
     if (kind == 0) {
         q.A = d.opZ;
         q.y = d.zrow;
@@ -537,9 +542,19 @@ static MvRec mv_record(const std::vector<DevNode>& hn, int b, int i, int kind) {
         q.from_b = 1;
         q.x0 = d.col_off;
         q.r0 = d.n;
-        q.x1 = d.frow;
-        q.ld1 = std::max(d.ku, 1);
         q.y = d.row_off;
+        if (d.fold) {
+            const DevNode& p = hn[d.parent];
+            const DevNode& sb = hn[d.is_left ? p.right : p.left];
+            q.x1 = sb.zrow;
+            q.ld1 = std::max(sb.kv, 1);
+            q.r1 = d.n + sb.kv;
+            q.x2 = zrows + p.frow;
+            q.ld2 = std::max(p.ku, 1);
+        } else {
+            q.x1 = zrows + d.frow;
+            q.ld1 = std::max(d.ku, 1);
+        }
     }
     return q;
 }
@@ -559,7 +574,7 @@ static void build_mv_schedule(const std::vector<hss_mat_s*>& mats, std::vector<M
             hss_mat_s* M = mats[b];
             for (int i = 0; i < M->nn; ++i) {
                 if (!sel(M, i)) continue;
-                work.push_back(mv_record(M->hn, b, i, kind));
+                work.push_back(mv_record(M->hn, M->zrows, b, i, kind));
                 int rows, cols;
                 mv_shape(M->hn, i, kind, rows, cols);
                 mmax = std::max(mmax, rows);
@@ -581,7 +596,10 @@ static void build_mv_schedule(const std::vector<hss_mat_s*>& mats, std::vector<M
     S.down_na.assign(maxd + 1, 0);
     S.down_k.assign(maxd + 1, 0);
     for (int d = 1; d <= maxd; ++d)
-        level(1, [&](const hss_mat_s* M, int i) { return i != 0 && M->hn[i].depth == d && d > M->cut_depth; },
+        level(1,
+              [&](const hss_mat_s* M, int i) {
+                  return i != 0 && M->hn[i].depth == d && d > M->cut_depth && !M->hn[i].fold;
+              },
               S.down[d], S.down_m[d], S.down_na[d], S.down_k[d]);
     S.leaf_m = 0;
     S.leaf_na = S.leaf_k = 0;
@@ -733,13 +751,14 @@ static hipError_t upload_cuts(const std::vector<CutSpec>& cs, char** out, std::v
     return hipSuccess;
 }
 
-// Tree cut of the matvec at depth c: input = the depth-c nodes' Z (kv rows each), output =
-// their F (ku rows each), F = T Z, one product.  T is built by running the sweep steps above the cut on
-// the identity -- Z of the depth-c nodes = unit columns, Kc right-hand sides, on Z / F rows
-// numbered for the nodes at depth <= c only -- and gathering the depth-c nodes' F.  The
-// matrix's schedule is then rebuilt without the steps the cut replaces.
-static hss_status mv_cut_build(hss_mat_s* M) {
-    const int c = choose_cut(M, M->op_len / 32, [&](int c, int64_t& k, int64_t& m, int64_t& t) {
+// Tree cut of the matvec at depth c (mv_cut_depth chooses c, 0 = none): input = the depth-c
+// nodes' Z (kv rows each), output = their F (ku rows each), F = T Z, one product.
+// mv_cut_build builds T by running the sweep steps above the cut on the identity -- Z of the
+// depth-c nodes = unit columns, Kc right-hand sides, on Z / F rows numbered for the nodes at
+// depth <= c only -- and gathering the depth-c nodes' F; the matrix's schedule is then
+// rebuilt without the steps the cut replaces.
+static int mv_cut_depth(const hss_mat_s* M) {
+    return choose_cut(M, M->op_len / 32, [&](int c, int64_t& k, int64_t& m, int64_t& t) {
         for (const DevNode& d : M->hn)
             if (d.depth == c) {
                 k += d.kv;
@@ -747,6 +766,8 @@ static hss_status mv_cut_build(hss_mat_s* M) {
             }
         t = k * m;
     });
+}
+static hss_status mv_cut_build(hss_mat_s* M, int c) {
     if (c == 0) return HSS_OK;
     int Kc = 0, Mc = 0;
     for (const DevNode& d : M->hn)
@@ -793,7 +814,7 @@ static hss_status mv_cut_build(hss_mat_s* M) {
         r.off = (int)work.size();
         for (int i = 1; i < M->nn; ++i) {
             if (hn[i].depth != depth) continue;
-            work.push_back(mv_record(hn, 0, i, kind));
+            work.push_back(mv_record(hn, 0, 0, i, kind));
             int rows, cols;
             mv_shape(hn, i, kind, rows, cols);
             mm = std::max(mm, rows);
@@ -851,6 +872,13 @@ static hss_status mv_cut_build(hss_mat_s* M) {
     return HSS_OK;
 }
 
+// One product of a matvec operator: op[dst + r + c * m] = sum_l blob[a + r + l * m] *
+// blob[b + l + c * k] for r < m, c < n, l < k (a folded leaf's U B_sibling and U R).
+struct OpProduct {
+    int64_t a, b, dst;
+    int m, n, k;
+};
+
 // One piece of a matvec operator, copied from the generator blob:
 //   op[dst + r + c * rows] = blob[src + r + c * ld]   (blob[src + c + r * ld] when trans)
 // for r < rows, c < cols.
@@ -864,7 +892,7 @@ struct OpPiece {
 // 16-element boundary (gaps zero): sets DevNode::opZ/opF/opC, appends the pieces that fill
 // it and returns its length.  Leaf opZ = V'; branch opZ = [W_l' W_r']; opF = [B_sibling R];
 // leaf opC = [D U].
-static int64_t plan_mv_operators(hss_mat_s* M, std::vector<OpPiece>& pc) {
+static int64_t plan_mv_operators(hss_mat_s* M, std::vector<OpPiece>& pc, std::vector<OpProduct>& pr) {
     int64_t len = 0;
     auto a16 = [&]() { len = (len + 15) & ~int64_t(15); };
     auto piece = [&](int64_t src, int rows, int cols, int ld, bool trans, int64_t dst) {
@@ -887,22 +915,34 @@ static int64_t plan_mv_operators(hss_mat_s* M, std::vector<OpPiece>& pc) {
                 piece(R.W, kv, R.kv, R.kv, true, len + (int64_t)L.kv * kv);
                 len += (int64_t)kv * (L.kv + R.kv);
             }
-            a16();
-            d.opF = len;
             const DevNode& p = M->hn[d.parent];
             const DevNode& sb = M->hn[d.is_left ? p.right : p.left];
             const int ku = d.ku;
-            piece(d.is_left ? p.B12 : p.B21, ku, sb.kv, ku, false, len);
-            piece(d.R, ku, p.ku, ku, false, len + (int64_t)sb.kv * ku);
-            len += (int64_t)ku * (sb.kv + p.ku);
+            if (!d.fold) {
+                a16();
+                d.opF = len;
+                piece(d.is_left ? p.B12 : p.B21, ku, sb.kv, ku, false, len);
+                piece(d.R, ku, p.ku, ku, false, len + (int64_t)sb.kv * ku);
+                len += (int64_t)ku * (sb.kv + p.ku);
+            }
         }
         if (d.is_leaf) {
             a16();
             d.opC = len;
             piece(d.D, d.m, d.n, d.m, false, len);
             len += (int64_t)d.m * d.n;
-            piece(d.U, d.m, d.ku, d.m, false, len);
-            len += (int64_t)d.m * d.ku;
+            if (d.fold) {
+                // [U B_sibling, U R]: U (m x ku) times B12 / B21 (ku x kv_sibling) and R (ku x ku_parent)
+                const DevNode& p = M->hn[d.parent];
+                const DevNode& sb = M->hn[d.is_left ? p.right : p.left];
+                pr.push_back(OpProduct{d.U, d.is_left ? p.B12 : p.B21, len, d.m, sb.kv, d.ku});
+                len += (int64_t)d.m * sb.kv;
+                pr.push_back(OpProduct{d.U, d.R, len, d.m, p.ku, d.ku});
+                len += (int64_t)d.m * p.ku;
+            } else {
+                piece(d.U, d.m, d.ku, d.m, false, len);
+                len += (int64_t)d.m * d.ku;
+            }
         }
     }
     a16();
@@ -976,8 +1016,16 @@ static hss_status mat_create_impl(hss_ctx_t ctx, const hss_node_desc* nodes, int
     M->root.off = (int)M->hwork.size();
     M->hwork.push_back(WorkItem{0, 0});
     M->root.count = 1;
+    // The matvec tree cut's depth comes first (from the unfolded operator size): the leaves
+    // below it fold their down step into their leaf product.
     std::vector<OpPiece> pieces;
-    M->op_len = plan_mv_operators(M, pieces);
+    std::vector<OpProduct> prods;
+    M->op_len = plan_mv_operators(M, pieces, prods);
+    const int cut = mv_cut_depth(M);
+    for (DevNode& d : M->hn) d.fold = (d.is_leaf && d.parent >= 0 && d.depth > cut && d.ku > 0) ? 1 : 0;
+    pieces.clear();
+    prods.clear();
+    M->op_len = plan_mv_operators(M, pieces, prods);
     std::vector<MvRec> mvwork;
     build_mv_schedule(std::vector<hss_mat_s*>{M}, mvwork, M->mvs);
 
@@ -1013,6 +1061,14 @@ static hss_status mat_create_impl(hss_ctx_t ctx, const hss_node_desc* nodes, int
             t.a = M->d_blob + q.src; t.c = M->d_op + q.dst;
             tasks.push_back(t);
         }
+        for (const OpProduct& q : prods) {
+            if (q.m <= 0 || q.n <= 0 || q.k <= 0) continue;
+            GemmTask t{};
+            t.m = q.m; t.n = q.n; t.k = q.k; t.flags = 0;
+            t.lda = q.m; t.ldb = q.k; t.ldc = q.m;
+            t.a = M->d_blob + q.a; t.b = M->d_blob + q.b; t.c = M->d_op + q.dst;
+            tasks.push_back(t);
+        }
         if ((e = hipMemsetAsync(M->d_op, 0, sizeof(double) * (size_t)M->op_len, st)) != hipSuccess) return bail(e);
         if (!tasks.empty()) {
             if ((e = dalloc(&dT, tasks.size())) != hipSuccess ||
@@ -1039,7 +1095,7 @@ static hss_status mat_create_impl(hss_ctx_t ctx, const hss_node_desc* nodes, int
     (void)hipFree(dT);
     M->dev_bytes = (int64_t)(sizeof(double) * (blob_len + M->op_len) + sizeof(DevNode) * nn +
                              sizeof(WorkItem) * M->hwork.size() + sizeof(MvRec) * mvwork.size());
-    hss_status cs = mv_cut_build(M);
+    hss_status cs = mv_cut_build(M, cut);
     if (cs != HSS_OK) {
         (void)hipStreamSynchronize(st);
         if (dblob) M->d_blob = nullptr;
