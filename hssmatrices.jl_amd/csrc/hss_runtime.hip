@@ -3592,4 +3592,11 @@ extern "C" hss_status hss_fact_node_info(hss_fact_t F, int32_t node, int32_t* in
     info[8] = f.My; info[9] = f.Kx;
     return HSS_OK;
 }
+extern "C" hss_status hss_mat_matvec_info(hss_mat_t M, int32_t* cut_depth, int32_t* folded) {
+    if (!M || !cut_depth || !folded) return fail(HSS_ERR_ARG, "null argument");
+    *cut_depth = M->cut_depth;
+    *folded = 0;
+    for (const DevNode& d : M->hn) *folded += d.fold ? 1 : 0;
+    return HSS_OK;
+}
 extern "C" double hss_last_device_ms(void) { return g_last_ms; }

@@ -56,6 +56,7 @@ EXPORTS = [
     "hss_ulv_factor_batch",
     "hss_last_error", "hss_last_device_ms",
     "hss_prof_enable", "hss_prof_reset", "hss_prof_count", "hss_prof_get", "hss_fact_node_info",
+    "hss_mat_matvec_info",
     "hss_plan_create", "hss_plan_execute", "hss_plan_destroy", "hss_plan_device_bytes",
     "hss_plan_tune", "hss_plan_tune_choice",
     "hss_server_create", "hss_server_admit", "hss_server_evict", "hss_server_used_bytes", "hss_server_submit",
@@ -141,6 +142,7 @@ def load_library(path: pathlib.Path | str | None = None):
         "hss_prof_get": ([I32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(I32), ctypes.POINTER(I32),
                           ctypes.POINTER(D)], ctypes.c_int),
         "hss_fact_node_info": ([V, I32, ctypes.POINTER(I32)], ctypes.c_int),
+        "hss_mat_matvec_info": ([V, ctypes.POINTER(I32), ctypes.POINTER(I32)], ctypes.c_int),
         "hss_plan_create": ([I32, V, V, V, V, V, V, I32, PV], ctypes.c_int),
         "hss_plan_execute": ([V, ctypes.POINTER(RefineOpts), ctypes.POINTER(SolveReport)], ctypes.c_int),
         "hss_plan_destroy": ([V], ctypes.c_int),
@@ -331,6 +333,12 @@ class HipHssMatrix:
 
     def device_bytes(self) -> int:
         return int(_lib.hss_mat_device_bytes(self._h))
+
+    def matvec_info(self) -> dict:
+        """The matvec's tree cut depth (0: none) and folded leaves (``hss_mat_matvec_info``)."""
+        c, f = ctypes.c_int32(), ctypes.c_int32()
+        _check(_lib.hss_mat_matvec_info(self._h, ctypes.byref(c), ctypes.byref(f)))
+        return {"cut_depth": c.value, "folded_leaves": f.value}
 
     def size(self, dim=None):
         return self.shape if dim is None else self.shape[dim - 1]

@@ -342,6 +342,10 @@ hss_status hss_prof_get(int32_t i, const char** name, int32_t* level, int32_t* n
  * up-sweep reads Kx input rows and writes My = nk + mr + kv output rows through two operators,
  * nk x Kx and (My - nk) x (Kx + nk); its down-sweep operator is n x n. */
 hss_status hss_fact_node_info(hss_fact_t fact, int32_t node, int32_t* info);
+/* The matvec's launch structure (no counterpart in the reference): *cut_depth = the depth c
+ * whose up and down steps above collapse into one dense product (0: none), *folded = the
+ * leaves whose down step is folded into their leaf product. */
+hss_status hss_mat_matvec_info(hss_mat_t mat, int32_t* cut_depth, int32_t* folded);
 /* Per-call device timing of the last matvec/solve on this thread (milliseconds, -1 if none). */
 double hss_last_device_ms(void);
 

@@ -75,6 +75,28 @@ def test_c1_matvec_vector_and_alpha_beta(hip, c1_dev, oracle, c1):
     assert rel(C, ref) < 1e-14
 
 
+def test_matvec_tree_cut_at_leaf_depth(hip, oracle):
+    """A tree whose only depth of leaves is the cut: 8 rectangular leaves (row / column ranks
+    3 / 2) make the cut product hold every coupling above the leaves, and no leaf folds its
+    down step.  Against the oracle for 1, 20 and 40 columns, and with alpha / beta."""
+    rng = np.random.default_rng(31)
+    ms, ns = [64, 60, 70, 58, 66, 61, 72, 63], [50, 71, 66, 59, 64, 69, 57, 62]
+    h = rect_hss(rng, ms, ns, 3, 2)
+    nodes, blob = H.pack(h)
+    A = hip.HipHssMatrix((nodes, blob))
+    assert A.matvec_info() == {"cut_depth": 3, "folded_leaves": 0}
+    F = H.full(h)
+    for r in (1, 20, 40):
+        X = rng.standard_normal((sum(ns), r))
+        assert rel(A @ X, oracle.matmul(nodes, blob, X)) < 1e-14
+        assert rel(A @ X, F @ X) < 1e-13
+    B = rng.standard_normal((sum(ns), 3))
+    C0 = np.asfortranarray(rng.standard_normal((sum(ms), 3)))
+    C = C0.copy(order="F")
+    hip.mul(C, A, B, 2.5, -0.5)
+    assert rel(C, oracle.matmul(nodes, blob, B, alpha=2.5, beta=-0.5, C=C0)) < 1e-14
+
+
 @pytest.mark.parametrize("nrhs", [1, 5, 33, 64])
 def test_c1_solve_parity(c1_dev, oracle, c1, nrhs):
     """North star: the FP64 GPU path matches the reference's hssA\\x to 1e-12 relative."""

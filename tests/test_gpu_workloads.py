@@ -47,11 +47,33 @@ def log65536(hip, W):
     return A, nodes, blob
 
 
+def test_matvec_uneven_leaves_below_the_cut(hip, oracle, W):
+    """n = 16580 bisected to leaves of at most 64 points leaves 60 leaves at depth 8 and 392
+    at depth 9; the log kernel at 1e-8 keeps the ranks small enough for a tree cut above
+    them, so every leaf folds its down step into its leaf product, at two depths.  Against
+    the oracle for 1, 16 and 37 columns, and through the adjoint."""
+    n = 16580
+    A = W.log_kernel(hip, n, 64, 1e-8, 5)
+    nodes, blob = A.generators()
+    leaf_depths = {int(nodes["depth"][i]) for i in range(len(nodes)) if nodes["left"][i] < 0}
+    assert leaf_depths == {8, 9}
+    info = A.matvec_info()
+    assert info["cut_depth"] >= 3 and info["folded_leaves"] == 452
+    rng = np.random.default_rng(17)
+    for r in (1, 16, 37):
+        X = rng.standard_normal((n, r))
+        assert rel(A @ X, oracle.matmul(nodes, blob, X)) < 1e-13
+    X = rng.standard_normal((n, 5))
+    nt, bt = H.pack(H.unpack(nodes, blob).adjoint())
+    assert rel(A.adjoint() @ X, oracle.matmul(nt, bt, X)) < 1e-13
+
+
 def test_log65536_matvec_and_fp64_solve(hip, oracle, log65536):
     A, nodes, blob = log65536
     n = 65536
     s = H.tree_summary(nodes)
     assert s["leaves"] == 512 and s["max_depth"] == 9 and 8 <= s["rank_min"] and s["rank_max"] <= 48
+    assert A.matvec_info() == {"cut_depth": 4, "folded_leaves": 512}
     rng = np.random.default_rng(16)
     X = rng.standard_normal((n, 8))
     Y = A @ X
