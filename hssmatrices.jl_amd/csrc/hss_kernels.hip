@@ -4,7 +4,9 @@
 // operators, one MFMA product per node and step:
 //   k_mv_s<0> / k_mv_t<0>   Z = V' B_seg (leaf) | [W1' W2'] [Z1; Z2]     (_matmatup, matmul.jl:32-42)
 //   k_mv_s<1> / k_mv_t<1>   F = [B_sibling R] [Z_sibling; F_parent]      (_matmatdown!, matmul.jl:51-57)
-//   k_mv_s<2> / k_mv_t<2>   C_seg = alpha [D U] [B_seg; F] + beta C_seg  (matmul.jl:46-47)
+//   k_mv_s<2> / k_mv_t<2>   C_seg = alpha [D U] [B_seg; F] + beta C_seg  (matmul.jl:46-47), or with
+//                           the leaf's down step folded in: alpha [D, U B_sib, U R] [B_seg; Z_sib; F_parent]
+//   k_cut                   the steps above one depth as one dense product F_cut = T Z_cut (tree cut)
 //
 // ULV (reference src/ulvfactor.jl:10-107), split into a resident factor and a solve:
 //   k_ulv_factor  per node: merge of the children (ulvfactor.jl:74-79), QL of U,
