@@ -2662,12 +2662,28 @@ struct hss_plan_s {
     std::vector<hipEvent_t> count_ev;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // captured lockstep refinement: the check step for each of the two count slots, and the
+    // correction step (solve + axpy); ir_tol = the tolerance the checks were captured with
+    hipGraph_t ir_graph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t ir_exec[3] = {nullptr, nullptr, nullptr};
+    double ir_tol = -1.0;
     int64_t dev_bytes = 0;
     std::vector<char> uploaded_active;  // request activity the device tables currently hold
 };
 
+static void drop_ir_graphs(hss_plan_s* P) {
+    for (int i = 0; i < 3; ++i) {
+        if (P->ir_exec[i]) (void)hipGraphExecDestroy(P->ir_exec[i]);
+        if (P->ir_graph[i]) (void)hipGraphDestroy(P->ir_graph[i]);
+        P->ir_exec[i] = nullptr;
+        P->ir_graph[i] = nullptr;
+    }
+    P->ir_tol = -1.0;
+}
+
 static void free_plan(hss_plan_s* P) {
     if (!P) return;
+    drop_ir_graphs(P);
     if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
     if (P->graph) (void)hipGraphDestroy(P->graph);
     (void)hipFree(P->arena);
@@ -2982,6 +2998,7 @@ extern "C" hss_status hss_plan_tune(hss_plan_t P, int32_t reps, double* ms_befor
     if (P->graph) (void)hipGraphDestroy(P->graph);
     P->gexec = nullptr;
     P->graph = nullptr;
+    drop_ir_graphs(P);
     return whole(ms_after);
 }
 
@@ -3067,8 +3084,12 @@ extern "C" hss_status hss_plan_execute(hss_plan_t P, const hss_refine_opts* opts
     //   solve k:  d = A \ r for the active requests, x += d
     // The host reads check k's count (pinned memory, event) only after enqueuing check k + 1,
     // so the device never waits for the host; a step enqueued after the last request
-    // retired finds every column count 0 and its launches do nothing.
-    constexpr int kRing = 64;
+    // retired finds every column count 0 and its launches do nothing.  Check k writes count
+    // slot k % 2: the host has read slot k % 2 (check k - 2's) before it enqueues check k.
+    // Outside profiling, the two check steps (one per slot) and the correction step are
+    // captured once (per tolerance) and replayed as graphs; every kernel reads its tables
+    // from device memory, which the checks and plan_set_active update in place.
+    constexpr int kRing = 2;
     if (!P->d_rel) {
         HIP_TRY(dalloc(&P->d_rel, (size_t)P->cnt));
         HIP_TRY(hipMemsetAsync(P->d_rel, 0, sizeof(double) * P->cnt, st));
@@ -3078,15 +3099,55 @@ extern "C" hss_status hss_plan_execute(hss_plan_t P, const hss_refine_opts* opts
         for (auto& e : P->count_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     const size_t nrhs_off = fp32 ? offsetof(SInst<float>, nrhs) : offsetof(SInst<double>, nrhs);
-    auto check = [&](int k) -> hss_status {
+    auto enqueue_check = [&](int slot) -> hss_status {
         hss_status c = plan_matvec(P);  // r = A x
         if (c != HSS_OK) return c;
         HIP_TRY(launch_resid_batch(P->d_vec, P->cnt, P->maxnrhs, P->d_norms, st));
-        const int slot = k % kRing;
         HIP_TRY(launch_refine_check(P->d_vec, P->d_sinst[0], P->d_sinst[1], (int64_t)P->sinst_size, (int)nrhs_off,
                                     P->d_mv, P->cnt, P->maxnrhs, P->d_norms, opts->tol, P->d_rel,
                                     P->d_count + slot, st));
         HIP_TRY(hipMemcpyAsync(P->h_count + slot, P->d_count + slot, sizeof(int), hipMemcpyDeviceToHost, st));
+        return HSS_OK;
+    };
+    auto enqueue_solve = [&]() -> hss_status {
+        hss_status c = fp32 ? plan_solve<float>(P, 1) : plan_solve<double>(P, 1);  // d = A \ r
+        if (c != HSS_OK) return c;
+        HIP_TRY(launch_axpy_batch(P->d_vec, P->cnt, P->maxnrhs, st));  // x += d
+        return HSS_OK;
+    };
+    const bool graphs = !g_prof;
+    if (graphs && P->ir_tol != opts->tol) drop_ir_graphs(P);
+    auto capture = [&](int i, auto&& enqueue) -> hss_status {
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hss_status c = enqueue();
+        hipGraph_t g = nullptr;
+        hipError_t ee = hipStreamEndCapture(st, &g);
+        if (c != HSS_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return c;
+        }
+        HIP_TRY(ee);
+        P->ir_graph[i] = g;
+        HIP_TRY(hipGraphInstantiate(&P->ir_exec[i], g, nullptr, nullptr, 0));
+        return HSS_OK;
+    };
+    if (graphs && !P->ir_exec[0]) {
+        hss_status c = capture(0, [&] { return enqueue_check(0); });
+        if (c == HSS_OK) c = capture(1, [&] { return enqueue_check(1); });
+        if (c == HSS_OK) c = capture(2, enqueue_solve);
+        if (c != HSS_OK) {
+            drop_ir_graphs(P);
+            return c;
+        }
+        P->ir_tol = opts->tol;
+    }
+    auto check = [&](int k) -> hss_status {
+        const int slot = k % kRing;
+        if (graphs) HIP_TRY(hipGraphLaunch(P->ir_exec[slot], st));
+        else {
+            hss_status c = enqueue_check(slot);
+            if (c != HSS_OK) return c;
+        }
         HIP_TRY(hipEventRecord(P->count_ev[slot], st));
         return HSS_OK;
     };
@@ -3112,9 +3173,8 @@ extern "C" hss_status hss_plan_execute(hss_plan_t P, const hss_refine_opts* opts
             it = k;
             break;
         }
-        s = fp32 ? plan_solve<float>(P, 1) : plan_solve<double>(P, 1);  // d = A \ r
-        if (s != HSS_OK) return s;
-        HIP_TRY(launch_axpy_batch(P->d_vec, P->cnt, P->maxnrhs, st));  // x += d
+        if (graphs) HIP_TRY(hipGraphLaunch(P->ir_exec[2], st));
+        else if ((s = enqueue_solve()) != HSS_OK) return s;
     }
     HIP_TRY(hipStreamSynchronize(st));
     std::vector<double> hrel(P->cnt);

@@ -533,6 +533,43 @@ def test_plan_mixed_precision_lockstep_refinement(hip, oracle):
     plan.close()
 
 
+def test_plan_refinement_graph_replays_match_eager(hip, oracle):
+    """Lockstep refinement replays captured graphs (a check step per count slot, the
+    correction step): every replay, with the program's own eager work and synchronizations
+    in between, gives the launch-by-launch result (profiling on runs the steps eagerly) bit
+    for bit; a new tolerance captures again and meets it against the oracle's matvec."""
+    import torch
+    rng = np.random.default_rng(23)
+    mats, packs, facts = _batch_inputs(hip, rng, [(1500, 64), (900, 64)], "fp32")
+    Bs, Xs = _plan_inputs(rng, packs, [4, 2])
+    plan = hip.Plan(facts, [4, 2], [b.data_ptr() for b in Bs], [int(b.shape[1]) for b in Bs],
+                    [x.data_ptr() for x in Xs], [int(x.shape[1]) for x in Xs], refinement=True)
+    hip.prof_enable(True)
+    try:
+        eager = plan.execute(tol=1e-10, max_iter=20)
+    finally:
+        hip.prof_enable(False)
+        hip.prof_reset()
+    X0 = [x.cpu().numpy().copy() for x in Xs]
+    assert eager["rel_residual"] <= 1e-10
+    for rep in range(3):
+        junk = torch.randn(1 << 20, dtype=torch.float64, device="cuda:0").sum().item()  # eager work between replays
+        assert np.isfinite(junk)
+        got = plan.execute(tol=1e-10, max_iter=20)
+        torch.cuda.synchronize()
+        assert got["iterations"] == eager["iterations"] and got["rel_residual"] == eager["rel_residual"]
+        for x, x0 in zip(Xs, X0):
+            assert np.array_equal(x.cpu().numpy(), x0), f"replay {rep} differs from the eager run"
+    got = plan.execute(tol=1e-11, max_iter=30)
+    assert got["rel_residual"] <= 1e-11
+    for i, (nodes, blob) in enumerate(packs):
+        B = Bs[i].cpu().numpy().T
+        X = Xs[i].cpu().numpy().T
+        R = oracle.matmul(nodes, blob, X) - B
+        assert np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)) <= 1e-11
+    plan.close()
+
+
 def _plan_inputs(rng, packs, rs):
     import torch
     dev = torch.device("cuda:0")
