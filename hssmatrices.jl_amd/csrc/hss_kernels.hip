@@ -3190,6 +3190,67 @@ hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, do
     return hipGetLastError();
 }
 
+// Gram-Schmidt steps of a lockstep GMRES with one Krylov space per column.  Basis block j
+// sits at V + j * vs (n x nrhs, ld n); W is n x nrhs (ld n); coefficients h[j + c * hld].
+//   k_gs_dots:   h[j + c * hld] = V_j(:, c)' W(:, c), j < kc   (one workgroup per (c, j))
+//   k_gs_update: W(:, c) = (acc ? W(:, c) : 0) + sg * sum_{j < kc} h[j + c * hld] V_j(:, c)
+//   k_col_scale: Y(:, c) = W(:, c) / sqrt(s[c]), or 0 where s[c] == 0
+__global__ void k_gs_dots(const double* __restrict__ V, int64_t vs, const double* __restrict__ W, int64_t n,
+                          double* __restrict__ h, int hld) {
+    const int c = blockIdx.x, j = blockIdx.y;
+    const double* v = V + j * vs + (int64_t)c * n;
+    const double* w = W + (int64_t)c * n;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += v[i] * w[i];
+    __shared__ double red[16];
+    s = wave_reduce(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w8 = 0; w8 < (int)(blockDim.x >> 6); ++w8) t += red[w8];
+        h[j + (int64_t)c * hld] = t;
+    }
+}
+__global__ void k_gs_update(double* __restrict__ W, const double* __restrict__ V, int64_t vs, int kc, int64_t n,
+                            int nrhs, const double* __restrict__ h, int hld, double sg, int acc) {
+    const int64_t total = n * nrhs;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / n;
+        double t = 0.0;
+        for (int j = 0; j < kc; ++j) t += h[j + c * hld] * V[j * vs + e];
+        W[e] = (acc ? W[e] : 0.0) + sg * t;
+    }
+}
+__global__ void k_col_scale(double* __restrict__ Y, const double* __restrict__ W, int64_t n, int nrhs,
+                            const double* __restrict__ s) {
+    const int64_t total = n * nrhs;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const double q = s[e / n];
+        Y[e] = q > 0.0 ? W[e] / sqrt(q) : 0.0;
+    }
+}
+static inline int stream_blocks(int64_t total) {
+    return (int)std::min<int64_t>(std::max<int64_t>((total + 255) / 256, 1), 4096);
+}
+hipError_t launch_gs_dots(const double* V, int64_t vs, int kc, const double* W, int64_t n, int nrhs, double* h, int hld,
+                          hipStream_t st) {
+    if (nrhs <= 0 || kc <= 0) return hipSuccess;
+    klaunch(k_gs_dots, dim3(nrhs, kc), dim3(256), 0, st, V, vs, W, n, h, hld);
+    return hipGetLastError();
+}
+hipError_t launch_gs_update(double* W, const double* V, int64_t vs, int kc, int64_t n, int nrhs, const double* h,
+                            int hld, double sg, int acc, hipStream_t st) {
+    if (nrhs <= 0 || n <= 0) return hipSuccess;
+    klaunch(k_gs_update, dim3(stream_blocks(n * nrhs)), dim3(256), 0, st, W, V, vs, kc, n, nrhs, h, hld, sg, acc);
+    return hipGetLastError();
+}
+hipError_t launch_col_scale(double* Y, const double* W, int64_t n, int nrhs, const double* s, hipStream_t st) {
+    if (nrhs <= 0 || n <= 0) return hipSuccess;
+    klaunch(k_col_scale, dim3(stream_blocks(n * nrhs)), dim3(256), 0, st, Y, W, n, nrhs, s);
+    return hipGetLastError();
+}
+
 template <int KIND, bool AR, int NT>
 static hipError_t launch_mv_s_nt(const MvInst* insts, const MvRec* work, int nwork, int tn, int tw, size_t lds,
                                  hipStream_t st) {

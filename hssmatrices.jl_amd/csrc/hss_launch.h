@@ -105,6 +105,14 @@ hipError_t launch_axpy(double* X, int64_t ldx, const double* D, int64_t ldd, int
 // unit columns j0 .. j0 + w - 1 of an identity into a zeroed rows x w block (v = 1), or cleared (v = 0)
 hipError_t launch_unit_cols(double* E, int64_t ld, int64_t j0, int w, int64_t rows, double v, hipStream_t st);
 hipError_t launch_colnorm2(const double* A, int64_t lda, int64_t n, int nrhs, double* out, hipStream_t st);
+// Gram-Schmidt steps of the lockstep GMRES (basis block j at V + j * vs, n x nrhs, ld n):
+// h[j + c * hld] = V_j(:, c)' W(:, c) for j < kc; W = (acc ? W : 0) + sg * sum_j h_j V_j per
+// column; Y(:, c) = W(:, c) / sqrt(s[c]) (0 where s[c] == 0)
+hipError_t launch_gs_dots(const double* V, int64_t vs, int kc, const double* W, int64_t n, int nrhs, double* h, int hld,
+                          hipStream_t st);
+hipError_t launch_gs_update(double* W, const double* V, int64_t vs, int kc, int64_t n, int nrhs, const double* h,
+                            int hld, double sg, int acc, hipStream_t st);
+hipError_t launch_col_scale(double* Y, const double* W, int64_t n, int nrhs, const double* s, hipStream_t st);
 // tree cut (k_cut): one workgroup per (cut record, 16-column tile); max_k = the largest K
 // of the records, at most cut_max_k().  cut_lds = its LDS bytes for element size w.
 int cut_max_k();

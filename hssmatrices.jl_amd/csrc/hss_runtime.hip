@@ -347,6 +347,9 @@ struct hss_fact_s {
     // refinement scratch (Float64): residual and correction
     double *d_r = nullptr, *d_d = nullptr, *d_nrm = nullptr;
     size_t rf_cap = 0;
+    // GMRES-IR workspace (Krylov basis, the preconditioned vector, coefficients), gm_cap doubles
+    double* d_gm = nullptr;
+    size_t gm_cap = 0;
     double *d_io_b = nullptr, *d_io_x = nullptr;
     size_t io_cap = 0;
     int64_t dev_bytes = 0;
@@ -1897,6 +1900,7 @@ static void free_fact(hss_fact_s* F) {
     (void)hipFree(F->d_r);
     (void)hipFree(F->d_d);
     (void)hipFree(F->d_nrm);
+    (void)hipFree(F->d_gm);
     (void)hipFree(F->d_io_b);
     (void)hipFree(F->d_io_x);
     delete F;
@@ -2449,6 +2453,125 @@ static hss_status rel_residual(hss_fact_s* F, int nrhs, const double* dB, int64_
     return HSS_OK;
 }
 
+// GMRES-IR correction (Carson and Higham, SIAM J. Sci. Comput. 40(2), 2018): d = A^{-1} r
+// by GMRES in Float64, right-preconditioned by the resident factor M (Float32 or Float64):
+// A M^{-1} u = r, d = M^{-1} u.  Each column runs its own Krylov space, all in lockstep.
+// Step k: z = M^{-1} v_k, w = A z (Float64 HSS matvec), two passes of classical Gram-Schmidt
+// against v_0 .. v_k (the second restores the orthogonality the first loses in rounding),
+// v_{k+1} = w / ||w||.  The host keeps each column's Hessenberg matrix reduced by Givens
+// rotations and retires a column once its GMRES residual is at most kGmresTol ||r_c|| (or
+// the space stops growing); at most kGmresDim steps.  Then y = R^{-1} g per column and
+// d = M^{-1} (V y).  Input r = F->d_r, output d = F->d_d (n x nrhs, ld n).
+constexpr int kGmresDim = 20;
+constexpr double kGmresTol = 1e-6;
+
+static hss_status gmres_correction(hss_fact_s* F, int nrhs, hipStream_t st) {
+    hss_mat_s* M = F->mat;
+    const int64_t n = M->rows, blk = n * nrhs;
+    const int m = kGmresDim, hld = m + 1;
+    const bool fp32 = F->prec == HSS_PREC_FP32;
+    // V (m + 1 blocks) | W | h1 | h2 | s (squared norms) | y
+    const size_t need = (size_t)(m + 2) * blk + (size_t)3 * hld * nrhs + nrhs;
+    if (need > F->gm_cap) {
+        (void)hipFree(F->d_gm);
+        F->d_gm = nullptr;
+        F->gm_cap = 0;
+        HIP_TRY(dalloc(&F->d_gm, need));
+        F->gm_cap = need;
+    }
+    double* V = F->d_gm;
+    double* W = V + (size_t)(m + 1) * blk;
+    double* h1 = W + blk;
+    double* h2 = h1 + (size_t)hld * nrhs;
+    double* sq = h2 + (size_t)hld * nrhs;
+    double* dy = sq + nrhs;
+    auto precond = [&](const double* b, double* x) -> hss_status {
+        return fp32 ? solve_one<float>(F, nrhs, b, n, x, n, st) : solve_one<double>(F, nrhs, b, n, x, n, st);
+    };
+    // v_0 = r / ||r|| per column
+    HIP_TRY(launch_colnorm2(F->d_r, n, n, nrhs, sq, st));
+    HIP_TRY(launch_col_scale(V, F->d_r, n, nrhs, sq, st));
+    std::vector<double> hs((size_t)2 * hld * nrhs + nrhs);  // h1 | h2 | s, as on the device
+    HIP_TRY(hipMemcpyAsync(hs.data(), sq, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // per column c: R column k at R[(c m + k) hld], rotations cs / sn, right-hand side g
+    std::vector<double> beta(nrhs), R((size_t)hld * m * nrhs, 0.0), cs((size_t)m * nrhs), sn((size_t)m * nrhs),
+        g((size_t)hld * nrhs, 0.0);
+    std::vector<int> kc(nrhs, 0);
+    std::vector<char> active(nrhs, 0);
+    int nact = 0;
+    for (int c = 0; c < nrhs; ++c) {
+        beta[c] = std::sqrt(hs[c]);
+        g[(size_t)c * hld] = beta[c];
+        active[c] = beta[c] > 0.0;
+        nact += active[c];
+    }
+    for (int k = 0; k < m && nact > 0; ++k) {
+        hss_status s = precond(V + (size_t)k * blk, F->d_d);  // z = M^{-1} v_k
+        if (s != HSS_OK) return s;
+        s = matmul_enqueue(M, nrhs, 1.0, F->d_d, n, 0.0, W, n);  // w = A z
+        if (s != HSS_OK) return s;
+        HIP_TRY(launch_gs_dots(V, blk, k + 1, W, n, nrhs, h1, hld, st));
+        HIP_TRY(launch_gs_update(W, V, blk, k + 1, n, nrhs, h1, hld, -1.0, 1, st));
+        HIP_TRY(launch_gs_dots(V, blk, k + 1, W, n, nrhs, h2, hld, st));
+        HIP_TRY(launch_gs_update(W, V, blk, k + 1, n, nrhs, h2, hld, -1.0, 1, st));
+        HIP_TRY(launch_colnorm2(W, n, n, nrhs, sq, st));
+        HIP_TRY(launch_col_scale(V + (size_t)(k + 1) * blk, W, n, nrhs, sq, st));
+        HIP_TRY(hipMemcpyAsync(hs.data(), h1, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (int c = 0; c < nrhs; ++c) {
+            if (!active[c]) continue;
+            double* r = &R[((size_t)c * m + k) * hld];
+            for (int i = 0; i <= k; ++i) r[i] = hs[i + (size_t)c * hld] + hs[(size_t)hld * nrhs + i + (size_t)c * hld];
+            const double hk1 = std::sqrt(hs[(size_t)2 * hld * nrhs + c]);
+            double* csc = &cs[(size_t)c * m];
+            double* snc = &sn[(size_t)c * m];
+            for (int i = 0; i < k; ++i) {
+                const double t = csc[i] * r[i] + snc[i] * r[i + 1];
+                r[i + 1] = -snc[i] * r[i] + csc[i] * r[i + 1];
+                r[i] = t;
+            }
+            const double den = std::hypot(r[k], hk1);
+            if (!(den > 0.0)) {  // no new direction: the column keeps its first k steps
+                active[c] = 0;
+                --nact;
+                continue;
+            }
+            double* gc = &g[(size_t)c * hld];
+            csc[k] = r[k] / den;
+            snc[k] = hk1 / den;
+            r[k] = den;
+            gc[k + 1] = -snc[k] * gc[k];
+            gc[k] = csc[k] * gc[k];
+            kc[c] = k + 1;
+            if (std::fabs(gc[k + 1]) <= kGmresTol * beta[c] || hk1 == 0.0) {
+                active[c] = 0;
+                --nact;
+            }
+        }
+    }
+    // y = R^{-1} g over each column's kc steps (zero beyond), u = V y, d = M^{-1} u
+    std::vector<double> y((size_t)hld * nrhs, 0.0);
+    int kmax = 0;
+    for (int c = 0; c < nrhs; ++c) {
+        const int q = kc[c];
+        kmax = std::max(kmax, q);
+        double* yc = &y[(size_t)c * hld];
+        const double* gc = &g[(size_t)c * hld];
+        for (int i = q - 1; i >= 0; --i) {
+            double t = gc[i];
+            for (int l = i + 1; l < q; ++l) t -= R[((size_t)c * m + l) * hld + i] * yc[l];
+            yc[i] = t / R[((size_t)c * m + i) * hld + i];
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(dy, y.data(), sizeof(double) * y.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_gs_update(W, V, blk, kmax, n, nrhs, dy, hld, 1.0, 0, st));
+    hss_status s = precond(W, F->d_d);
+    if (s != HSS_OK) return s;
+    HIP_TRY(hipStreamSynchronize(st));  // y is read before it goes out of scope
+    return HSS_OK;
+}
+
 static hss_status solve_dev_impl(hss_fact_s* F, int nrhs, const double* dB, int64_t ldb, double* dX, int64_t ldx,
                                  const hss_refine_opts* opts, hss_solve_report* rep) {
     hss_mat_s* M = F->mat;
@@ -2461,10 +2584,13 @@ static hss_status solve_dev_impl(hss_fact_s* F, int nrhs, const double* dB, int6
     hss_refine_opts o;
     if (opts) o = *opts;
     else { o.tol = fp32 ? 1e-10 : 0.0; o.max_iter = fp32 ? 10 : 0; o.mode = fp32 ? 1 : 0; }
+    if (o.mode < 0 || o.mode > 2)
+        return fail(HSS_ERR_ARG, "opts->mode must be 0 (none), 1 (iterative refinement) or 2 (GMRES-IR)");
+    const bool refine = o.mode != 0 && o.max_iter >= 0;
     auto solve = [&](const double* b, int64_t lb, double* x, int64_t lx) -> hss_status {
         return fp32 ? solve_one<float>(F, nrhs, b, lb, x, lx, st) : solve_one<double>(F, nrhs, b, lb, x, lx, st);
     };
-    if (o.mode != 1 && !g_prof && !fp32) {
+    if (!refine && !g_prof && !fp32) {
         // repeated unrefined solves on the same buffers replay a captured graph
         const PlanKey key{nrhs, dB, ldb, dX, ldx};
         hss_plan_t P = nullptr;
@@ -2487,7 +2613,7 @@ static hss_status solve_dev_impl(hss_fact_s* F, int nrhs, const double* dB, int6
     HIP_TRY(hipEventRecord(M->ctx->ev0, st));
     hss_status s = solve(dB, ldb, dX, ldx);
     if (s != HSS_OK) return s;
-    if (o.mode == 1 && o.max_iter >= 0) {
+    if (refine) {
         s = ensure_refine(F, nrhs);
         if (s != HSS_OK) return s;
         const int64_t n = M->rows;
@@ -2501,8 +2627,9 @@ static hss_status solve_dev_impl(hss_fact_s* F, int nrhs, const double* dB, int6
             s = rel_residual(F, nrhs, dB, ldb, F->d_r, n, st, &rel);
             if (s != HSS_OK) return s;
             if (rel <= o.tol || it >= o.max_iter) break;
-            // d = A \ r with the resident factor, x += d
-            s = solve(F->d_r, n, F->d_d, n);
+            // d = A \ r with the resident factor (mode 1) or by GMRES preconditioned with it
+            // (mode 2), x += d
+            s = o.mode == 2 ? gmres_correction(F, nrhs, st) : solve(F->d_r, n, F->d_d, n);
             if (s != HSS_OK) return s;
             HIP_TRY(launch_axpy(dX, ldx, F->d_d, n, n, nrhs, st));
             ++it;
@@ -3052,6 +3179,8 @@ extern "C" hss_status hss_plan_execute(hss_plan_t P, const hss_refine_opts* opts
     const bool fp32 = P->prec == HSS_PREC_FP32;
     if (rep) { rep->iterations = 0; rep->rel_residual = -1.0; rep->seconds = 0.0; }
     HIP_TRY(hipEventRecord(P->ctx->ev0, st));
+    if (opts && opts->mode == 2)
+        return fail(HSS_ERR_ARG, "GMRES-IR (opts->mode 2) runs on single solves (hss_ulv_solve, hss_ulv_solve_dev)");
     const bool refine = P->has_ir && opts && opts->mode == 1;
     if (fp32 && !refine)
         return fail(HSS_ERR_ARG, "a plan of Float32 factors executes with refinement only (with_refinement at "
@@ -3330,6 +3459,8 @@ extern "C" hss_status hss_server_create(hss_ctx_t ctx, int64_t hbm_budget_bytes,
                                         const hss_refine_opts* opts, int32_t max_inflight, hss_server_t* out) {
     if (!ctx || !out || hbm_budget_bytes <= 0 || max_inflight <= 0) return fail(HSS_ERR_ARG, "bad server arguments");
     if (prec != HSS_PREC_FP64 && prec != HSS_PREC_FP32) return fail(HSS_ERR_ARG, "unknown precision");
+    if (opts && opts->mode == 2)
+        return fail(HSS_ERR_ARG, "GMRES-IR (opts->mode 2) runs on single solves (hss_ulv_solve, hss_ulv_solve_dev)");
     if (prec == HSS_PREC_FP32 && opts && opts->mode != 1)
         return fail(HSS_ERR_ARG, "a Float32 server needs refinement (opts->mode 1): a Float32 solve alone is "
                                  "accurate to about 1e-7");

@@ -185,6 +185,17 @@ def _check(status: int):
     raise HssError(status, msg)
 
 
+def _refine_opts(tol, max_iter, method):
+    """hss_refine_opts of a single solve: none unless tol or max_iter is given; mode 1
+    (iterative refinement) or 2 (GMRES-IR) when tol > 0."""
+    if method not in ("ir", "gmres"):
+        raise ValueError(f"method must be 'ir' or 'gmres', not {method!r}")
+    if tol is None and max_iter is None:
+        return None
+    mode = (2 if method == "gmres" else 1) if (tol is not None and tol > 0) else 0
+    return RefineOpts(tol if tol is not None else 0.0, max_iter if max_iter is not None else 10, mode)
+
+
 def last_device_ms() -> float:
     return float(load_library().hss_last_device_ms())
 
@@ -667,8 +678,11 @@ class HipUlvFactor:
         except Exception:
             pass
 
-    def solve(self, b, tol: float | None = None, max_iter: int | None = None, report: bool = False):
+    def solve(self, b, tol: float | None = None, max_iter: int | None = None, report: bool = False,
+              method: str = "ir"):
         """X = A \\ b with the resident factor; FP32 factors refine to ``tol`` (default 1e-10).
+        ``method``: "ir" (each correction is one solve with the factor) or "gmres" (GMRES-IR:
+        each correction by Float64 GMRES preconditioned with the factor).
         Float32 data (a numpy float32 array) go through ``hss_ulv_solve_f32``."""
         if isinstance(b, np.ndarray) and b.dtype == np.float32 and tol is None and max_iter is None and not report:
             return _f32_call(self.mat, b, "solve", fact=self._h)
@@ -677,10 +691,7 @@ class HipUlvFactor:
         if bf.shape[0] != n:
             raise DimensionMismatch(f"matrix is {self.mat.shape}, right-hand side has {bf.shape[0]} rows")
         X = np.zeros((n, bf.shape[1]), order="F")
-        opts = None
-        if tol is not None or max_iter is not None:
-            opts = RefineOpts(tol if tol is not None else 0.0, max_iter if max_iter is not None else 10,
-                              1 if (tol is not None and tol > 0) else 0)
+        opts = _refine_opts(tol, max_iter, method)
         rep = SolveReport()
         _check(_lib.hss_ulv_solve(self._h, bf.shape[1], bf.ctypes.data, max(n, 1), X.ctypes.data, max(n, 1),
                                   ctypes.byref(opts) if opts is not None else None, ctypes.byref(rep)))
@@ -690,11 +701,8 @@ class HipUlvFactor:
         return out
 
     def solve_dev(self, nrhs: int, dB: int, ldb: int, dX: int, ldx: int, tol: float | None = None,
-                  max_iter: int | None = None):
-        opts = None
-        if tol is not None or max_iter is not None:
-            opts = RefineOpts(tol if tol is not None else 0.0, max_iter if max_iter is not None else 10,
-                              1 if (tol is not None and tol > 0) else 0)
+                  max_iter: int | None = None, method: str = "ir"):
+        opts = _refine_opts(tol, max_iter, method)
         rep = SolveReport()
         _check(_lib.hss_ulv_solve_dev(self._h, nrhs, dB, ldb, dX, ldx,
                                       ctypes.byref(opts) if opts is not None else None, ctypes.byref(rep)))

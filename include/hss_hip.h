@@ -89,7 +89,10 @@ typedef enum hss_prec {
 typedef struct hss_refine_opts {
     double tol;   /* target ||b - A x|| / ||b|| per column (max over columns); <= 0: no refinement */
     int max_iter; /* refinement steps after the first solve */
-    int mode;     /* 0 = none, 1 = iterative refinement with the HSS matvec residual */
+    int mode;     /* 0 = none, 1 = iterative refinement with the HSS matvec residual,
+                     2 = GMRES-IR: each correction by Float64 GMRES preconditioned with the
+                     factor (hss_ulv_solve / hss_ulv_solve_dev only; plans and servers
+                     return HSS_ERR_ARG) */
 } hss_refine_opts;
 
 typedef struct hss_solve_report {

@@ -44,3 +44,17 @@ def test_oracle_node_record_matches():
     src = (ROOT / "oracle" / "hss_oracle.c").read_text()
     assert "int32_t parent, left, right, depth, height, m, n, ku, kv, pad;" in src
     assert "int64_t row_off, col_off, off_D, off_U, off_V, off_B12, off_B21, off_R, off_W;" in src
+
+
+def test_refine_opts_modes(hip):
+    """Single solves map tol / max_iter / method onto hss_refine_opts: mode 1 is iterative
+    refinement, mode 2 GMRES-IR, mode 0 when tol <= 0; no options without tol or max_iter."""
+    assert hip._refine_opts(None, None, "ir") is None
+    o = hip._refine_opts(1e-10, 5, "ir")
+    assert (o.tol, o.max_iter, o.mode) == (1e-10, 5, 1)
+    o = hip._refine_opts(1e-9, None, "gmres")
+    assert (o.tol, o.max_iter, o.mode) == (1e-9, 10, 2)
+    assert hip._refine_opts(None, 3, "gmres").mode == 0
+    import pytest
+    with pytest.raises(ValueError):
+        hip._refine_opts(1e-10, 5, "cg")
