@@ -288,6 +288,23 @@ function \(F::HipUlv, B::Matrix{Float64})
     return X
 end
 
+# hss_refine_opts: mode 1 = iterative refinement, 2 = GMRES-IR (Float64 GMRES preconditioned
+# with the factor); the solve stops at max_j ||b_j - A x_j|| / ||b_j|| <= tol
+struct RefineOpts
+    tol::Float64
+    max_iter::Cint
+    mode::Cint
+end
+function solve(F::HipUlv, B::Matrix{Float64}; tol::Float64=1e-10, maxiter::Integer=10, method::Symbol=:ir)
+    method in (:ir, :gmres) || throw(ArgumentError("method must be :ir or :gmres"))
+    X = zeros(F.A.n, size(B, 2))
+    opts = Ref(RefineOpts(tol, Cint(maxiter), method === :gmres ? Cint(2) : Cint(1)))
+    check(ccall((:hss_ulv_solve, libhss), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ref{RefineOpts}, Ptr{Cvoid}),
+                F.h, size(B, 2), B, stride(B, 2), X, stride(X, 2), opts, C_NULL))
+    return X
+end
+
 # ulvfactsolve is not exported again: HssMatrices exports it, and the methods above extend it
 export HipHss, ulvfactor, recompress
 
