@@ -649,9 +649,16 @@ __device__ __forceinline__ bool ulv_front(const FInst<T>& I, const DevNode& nd, 
     }
 
     // ---- phase 1: QL of U (m x k) in LDS, LAPACK dgeqlf semantics (ulvfactor.jl:39) ----
+    // Two barriers per reflector.  While reflector jj is applied its vector stays unscaled in
+    // LDS and each use forms col[i] * scale, the value dgeqlf stores; the stored vector is
+    // scaled in place during the next reflector's norm (another column).  beta goes to col[p]
+    // at once: the update never reads it (the vector's unit entry is implicit).  The scalars
+    // alternate between sc[0..2] and sc[4..6], so the deferred scaling reads the previous
+    // reflector's while the current one's are written.
     for (int jj = k - 1; jj >= 0; --jj) {
         const int p = m - k + jj;
         T* col = Us + (int64_t)jj * m;
+        T* scj = sc + 4 * (jj & 1);
         if (wave == 0) {
             T part = T(0);
             for (int i = lane; i < p; i += 64) part += col[i] * col[i];
@@ -659,31 +666,37 @@ __device__ __forceinline__ bool ulv_front(const FInst<T>& I, const DevNode& nd, 
             if (lane == 0) {
                 T beta, tau, scale;
                 larfg(col[p], part, beta, tau, scale);
-                sc[0] = beta;
-                sc[1] = tau;
-                sc[2] = scale;
+                scj[0] = beta;
+                scj[1] = tau;
+                scj[2] = scale;
                 taus[jj] = tau;
+                col[p] = beta;
             }
         }
+        if (jj + 1 < k) {  // reflector jj + 1's vector: rows [0, p + 1) of the next column
+            const T* scp = sc + 4 * ((jj + 1) & 1);
+            if (scp[1] != T(0))
+                for (int i = tid; i < p + 1; i += blockDim.x) col[m + i] *= scp[2];
+        }
         __syncthreads();
-        const T tau = sc[1], scale = sc[2];
-        if (tau != T(0))
-            for (int i = tid; i < p; i += blockDim.x) col[i] *= scale;
-        __syncthreads();
-        // apply H = I - tau v v' to U[0:p+1, 0:jj], v = [col[0:p]; 1]
+        // apply H = I - tau v v' to U[0:p+1, 0:jj], v = [col[0:p] * scale; 1]
+        const T tau = scj[1], scale = scj[2];
         if (tau != T(0)) {
             for (int c = wave; c < jj; c += nwaves) {
                 T* uc = Us + (int64_t)c * m;
                 T part = T(0);
-                for (int i = lane; i < p; i += 64) part += col[i] * uc[i];
+                for (int i = lane; i < p; i += 64) part += (col[i] * scale) * uc[i];
                 part = wave_reduce(part) + uc[p];
                 const T t = tau * part;
-                for (int i = lane; i < p; i += 64) uc[i] -= t * col[i];
+                for (int i = lane; i < p; i += 64) uc[i] -= t * (col[i] * scale);
                 if (lane == 0) uc[p] -= t;
             }
         }
         __syncthreads();
-        if (tid == 0) col[p] = sc[0];
+    }
+    if (k > 0) {  // reflector 0's vector: rows [0, m - k) of column 0
+        if (sc[1] != T(0))
+            for (int i = tid; i < m - k; i += blockDim.x) Us[i] *= sc[2];
         __syncthreads();
     }
     // reflector j: y_j[i] = Us[i, j] (i < p_j), 1 (i = p_j), 0 below
