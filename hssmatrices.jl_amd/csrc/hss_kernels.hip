@@ -1016,14 +1016,18 @@ __global__ __launch_bounds__(NT) void k_ulv_factor_mf(const FInst<T>* __restrict
     __syncthreads();  // Us and M are dead from here; the panels' buffers reuse the LDS
 
     // ---- blocked LQ ----
-    T* Vs = reinterpret_cast<T*>(lds_raw);  // [2][16][NC]: the panel's reflectors (row-major)
-    T* Ts = Vs + 2 * 16 * NC;               // [2][16][16]: T(a, q) at a * 16 + q
+    // [2][16][NCP]: the panel's reflectors (row-major).  The row pitch NC + 2 doubles puts the
+    // 16 rows a column read touches (lanes li) in distinct LDS banks (a pitch of NC = 64 or
+    // 128 maps them all to one) and keeps every row 16-byte aligned for the vector reads.
+    constexpr int NCP = NC + 2;
+    T* Vs = reinterpret_cast<T*>(lds_raw);
+    T* Ts = Vs + 2 * 16 * NCP;  // [2][16][16]: T(a, q) at a * 16 + q
     T* tl = Ts + 2 * 256;                   // nk taus (nk <= n <= 128)
     T* Gs = tl + 128;                       // [16]: the panel step's Gram column v_b . v_j
     const int npan = (nk + 15) / 16;
     for (int p = 0; p < npan; ++p) {
         const int i0 = 16 * p, nb = min(16, nk - i0), buf = p & 1;
-        T* Vp = Vs + buf * 16 * NC;
+        T* Vp = Vs + buf * 16 * NCP;
         T* Tp = Ts + buf * 256;
         if (wave == p) {
             // the panel's reflectors, one row at a time (a loop, not unrolled), each applied
@@ -1069,7 +1073,7 @@ __global__ __launch_bounds__(NT) void k_ulv_factor_mf(const FInst<T>* __restrict
                 larfg(alpha, xn2, beta, tau, scl);
                 // row j becomes [L, beta, v_j]; its lanes also store v_j = [0, 1, x scl]
                 const bool own = li == j;
-                T* vrow = Vp + j * NC;
+                T* vrow = Vp + j * NCP;
 #pragma unroll
                 for (int t = 0; t < NTILE; ++t) {
                     if (t < p) continue;
@@ -1142,7 +1146,7 @@ __global__ __launch_bounds__(NT) void k_ulv_factor_mf(const FInst<T>* __restrict
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int c = colof(t, r), ii = i0 + li;
-                    Vp[li * NC + c] = li < nb ? (c < ii ? T(0) : (c == ii ? T(1) : a[t][r])) : T(0);
+                    Vp[li * NCP + c] = li < nb ? (c < ii ? T(0) : (c == ii ? T(1) : a[t][r])) : T(0);
                 }
             if (lane < 16) {
                 T* Tg = I.fac + f.Tlq + (int64_t)p * NB * NB;
@@ -1157,14 +1161,14 @@ __global__ __launch_bounds__(NT) void k_ulv_factor_mf(const FInst<T>* __restrict
 #pragma unroll
             for (int t = 0; t < NTILE; ++t)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) P = MF::mac(Vp[li * NC + colof(t, r)], a[t][r], P);
+                for (int r = 0; r < 4; ++r) P = MF::mac(Vp[li * NCP + colof(t, r)], a[t][r], P);
             V4 Q = {T(0), T(0), T(0), T(0)};
 #pragma unroll
             for (int r = 0; r < 4; ++r) Q = MF::mac(Tp[MF::row(lk, r) * 16 + li], P[r], Q);
 #pragma unroll
             for (int t = 0; t < NTILE; ++t)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) a[t] = MF::mac(-Vp[MF::row(lk, r) * NC + 16 * t + li], Q[r], a[t]);
+                for (int r = 0; r < 4; ++r) a[t] = MF::mac(-Vp[MF::row(lk, r) * NCP + 16 * t + li], Q[r], a[t]);
         }
     }
     __syncthreads();
