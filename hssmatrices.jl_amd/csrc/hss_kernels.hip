@@ -1021,14 +1021,17 @@ __global__ __launch_bounds__(NT) void k_ulv_factor_mf(const FInst<T>* __restrict
     // 128 maps them all to one) and keeps every row 16-byte aligned for the vector reads.
     constexpr int NCP = NC + 2;
     T* Vs = reinterpret_cast<T*>(lds_raw);
-    T* Ts = Vs + 2 * 16 * NCP;  // [2][16][16]: T(a, q) at a * 16 + q
-    T* tl = Ts + 2 * 256;                   // nk taus (nk <= n <= 128)
+    // [2][16][TPP]: T(a, q) at a * TPP + q; the pitch 17 puts the 16 rows (lanes li) in
+    // distinct LDS banks
+    constexpr int TPP = 17;
+    T* Ts = Vs + 2 * 16 * NCP;
+    T* tl = Ts + 2 * 16 * TPP;              // nk taus (nk <= n <= 128)
     T* Gs = tl + 128;                       // [16]: the panel step's Gram column v_b . v_j
     const int npan = (nk + 15) / 16;
     for (int p = 0; p < npan; ++p) {
         const int i0 = 16 * p, nb = min(16, nk - i0), buf = p & 1;
         T* Vp = Vs + buf * 16 * NCP;
-        T* Tp = Ts + buf * 256;
+        T* Tp = Ts + buf * 16 * TPP;
         if (wave == p) {
             // the panel's reflectors, one row at a time (a loop, not unrolled), each applied
             // to the wave's later rows.  Rows live in the 4 lanes l%16 of the wave's 16-lane
@@ -1039,10 +1042,10 @@ __global__ __launch_bounds__(NT) void k_ulv_factor_mf(const FInst<T>* __restrict
             // are already reduced and skipped.  The same dot loop gives v_b . v_j for the
             // panel's earlier reflectors b < j (their masked rows), which builds column j of
             // T (dlarft forward) through the 16-entry Gram column Gs.  Row ra of T lives in
-            // LDS (Tp[ra * 16 + q]); the 4 lanes of row ra write the same values.
+            // LDS (Tp[ra * TPP + q]); the 4 lanes of row ra write the same values.
             const int ra = li;
 #pragma unroll
-            for (int q = 0; q < 16; ++q) Tp[ra * 16 + q] = T(0);
+            for (int q = 0; q < 16; ++q) Tp[ra * TPP + q] = T(0);
             // Only tile p (the panel's own 16 columns) needs column masks; in tiles t > p every
             // column is right of the reflector (c > i) and of every panel row's diagonal.  In
             // tile p, column 16 p + cr with cr = Mfma<T>::row(lk, r).  Row j's reflector is
@@ -1133,8 +1136,8 @@ __global__ __launch_bounds__(NT) void k_ulv_factor_mf(const FInst<T>* __restrict
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 T acc = T(0);
-                for (int b2 = 0; b2 < j; ++b2) acc += Tp[ra * 16 + b2] * Gs[b2];
-                Tp[ra * 16 + j] = ra < j ? -tau * acc : (ra == j ? tau : T(0));
+                for (int b2 = 0; b2 < j; ++b2) acc += Tp[ra * TPP + b2] * Gs[b2];
+                Tp[ra * TPP + j] = ra < j ? -tau * acc : (ra == j ? tau : T(0));
                 if (lane == 0) tl[i] = tau;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -1151,7 +1154,7 @@ __global__ __launch_bounds__(NT) void k_ulv_factor_mf(const FInst<T>* __restrict
             if (lane < 16) {
                 T* Tg = I.fac + f.Tlq + (int64_t)p * NB * NB;
 #pragma unroll
-                for (int q = 0; q < 16; ++q) Tg[ra + q * NB] = (q < nb && ra < nb) ? Tp[ra * 16 + q] : T(0);
+                for (int q = 0; q < 16; ++q) Tg[ra + q * NB] = (q < nb && ra < nb) ? Tp[ra * TPP + q] : T(0);
             }
         }
         __syncthreads();
@@ -1164,7 +1167,7 @@ __global__ __launch_bounds__(NT) void k_ulv_factor_mf(const FInst<T>* __restrict
                 for (int r = 0; r < 4; ++r) P = MF::mac(Vp[li * NCP + colof(t, r)], a[t][r], P);
             V4 Q = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-            for (int r = 0; r < 4; ++r) Q = MF::mac(Tp[MF::row(lk, r) * 16 + li], P[r], Q);
+            for (int r = 0; r < 4; ++r) Q = MF::mac(Tp[MF::row(lk, r) * TPP + li], P[r], Q);
 #pragma unroll
             for (int t = 0; t < NTILE; ++t)
 #pragma unroll

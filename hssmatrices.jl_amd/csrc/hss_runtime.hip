@@ -1991,8 +1991,8 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
                     lev_thr[h] = std::max(lev_thr[h], (fi.m + M->hn[i].kv) * fi.S);
                     lev_rows[h] = std::max(lev_rows[h], fi.m + M->hn[i].kv);
                     lev_n[h] = std::max(lev_n[h], fi.n);
-                    // Vs 2 x 16 x 130 (rows padded by 2) | Ts 2 x 256 | taus 128 | Gram column 16
-                    lev_lds_mf[h] = std::max(lev_lds_mf[h], sizeof(TF) * (size_t)(2 * 16 * 130 + 512 + 128 + 16));
+                    // Vs 2 x 16 x 130 | Ts 2 x 16 x 17 (rows padded) | taus 128 | Gram column 16
+                    lev_lds_mf[h] = std::max(lev_lds_mf[h], sizeof(TF) * (size_t)(2 * 16 * 130 + 2 * 16 * 17 + 128 + 16));
                 }
         }
         lev[h].count = (int)work.size() - lev[h].off;
