@@ -234,6 +234,39 @@ def test_gmres_ir_well_conditioned(hip, oracle):
     assert rel(Xg, Xo) < 1e-7 and rel(Xi, Xo) < 1e-7
 
 
+def test_refinement_mode_checks(hip):
+    """GMRES-IR (refinement mode 2) runs on single solves only: a plan or a server given it
+    returns HSS_ERR_ARG instead of refining another way, and a single solve refuses an
+    unknown mode; the plan still refines with mode 1 afterwards."""
+    import ctypes
+
+    import torch
+    rng = np.random.default_rng(23)
+    _, h = cauchy_c1()
+    nodes, blob = H.pack(h)
+    F = hip.HipHssMatrix((nodes, blob)).ulvfactor("fp32")
+    lib = hip.load_library()
+    n = 2001
+    B = np.asfortranarray(rng.standard_normal((n, 2)))
+    X = np.zeros((n, 2), order="F")
+    rep = hip.SolveReport()
+    bad = hip.RefineOpts(1e-10, 5, 3)
+    assert lib.hss_ulv_solve(F._h, 2, B.ctypes.data, n, X.ctypes.data, n, ctypes.byref(bad),
+                             ctypes.byref(rep)) == hip.HSS_ERR_ARG
+    gm = hip.RefineOpts(1e-10, 10, 2)
+    srv = ctypes.c_void_p()
+    assert lib.hss_server_create(hip.default_context().handle, 1 << 30, hip.PREC_FP32, ctypes.byref(gm), 8,
+                                 ctypes.byref(srv)) == hip.HSS_ERR_ARG
+    dev = torch.device("cuda:0")
+    Bt = torch.from_numpy(np.ascontiguousarray(B.T)).to(dev)
+    Xt = torch.zeros_like(Bt)
+    torch.cuda.synchronize()
+    plan = hip.Plan([F], [2], [Bt.data_ptr()], [n], [Xt.data_ptr()], [n], refinement=True)
+    assert lib.hss_plan_execute(plan._h, ctypes.byref(gm), ctypes.byref(rep)) == hip.HSS_ERR_ARG
+    assert plan.execute(tol=1e-10, max_iter=20)["rel_residual"] <= 1e-10
+    plan.close()
+
+
 def test_batched_factor_and_solve_match_single(hip, oracle):
     """Uneven instances in one launch per level give the same answers as one at a time."""
     import torch
