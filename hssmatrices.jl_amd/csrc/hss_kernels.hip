@@ -464,7 +464,7 @@ __device__ __forceinline__ int vpad(int c) { return c + (c / CPT) * 2; }
 // generator blob).  Returns false for a pass-through node, whose outputs it has written.
 template <typename T, int CPT>
 __device__ __forceinline__ bool ulv_front(const FInst<T>& I, const DevNode& nd, const FNode& f,
-                                          unsigned char* lds_raw, const T*& Wm_out) {
+                                          unsigned char* lds_raw, const T*& Wm_out, int64_t lds_elems) {
     const int m = f.m, n = f.n, k = f.k, nk = f.nk, ku = nd.ku, kv = nd.kv;
     const int S = f.S;
     const int tid = threadIdx.x;
@@ -497,10 +497,11 @@ __device__ __forceinline__ bool ulv_front(const FInst<T>& I, const DevNode& nd, 
         T* Wmw = I.rw + f.rwM;
         // The merge's inner products run over the children's ranks, so they are staged:
         // the LDS after Us (used by phases 1-5 only later) holds the rank-sized operands,
-        // loaded with many loads in flight, and the products read LDS.  This node's own
-        // LDS size (the formula of the launch's, which is the level's largest) bounds
-        // what fits; a merge that does not fit reads its operands from memory.
-        const int64_t cap = 2 * (int64_t)k * k + (int64_t)(k + 2) * vstride + (k > nk ? k : nk) + 1 + 8;
+        // loaded with many loads in flight, and the products read LDS.  The launch's
+        // dynamic LDS (lds_elems elements, at least this node's own formula) bounds what
+        // fits; a merge that does not fit reads its operands from memory.
+        const int64_t own = 2 * (int64_t)k * k + (int64_t)(k + 2) * vstride + (k > nk ? k : nk) + 1 + 8;
+        const int64_t cap = lds_elems - (int64_t)m * ku > own ? lds_elems - (int64_t)m * ku : own;
         auto mx = [](int64_t x, int64_t y) { return x > y ? x : y; };
         const int64_t nP12 = (int64_t)a.mr * b.kv, nP21 = (int64_t)b.mr * a.kv;
         const int64_t nU1 = (int64_t)a.mr * a.ku, nU2 = (int64_t)b.mr * b.ku;
@@ -806,7 +807,7 @@ __device__ __forceinline__ bool ulv_front(const FInst<T>& I, const DevNode& nd, 
 // without spills and keeps them.
 template <typename T, int CPT, int NT>
 __global__ __launch_bounds__(NT) void k_ulv_factor(const FInst<T>* __restrict__ insts,
-                                                             const WorkItem* __restrict__ work) {
+                                                             const WorkItem* __restrict__ work, int64_t lds_elems) {
     const WorkItem wi = work[blockIdx.x];
     const FInst<T>& I = insts[wi.inst];
     const DevNode nd = I.nodes[wi.node];
@@ -820,7 +821,7 @@ __global__ __launch_bounds__(NT) void k_ulv_factor(const FInst<T>* __restrict__ 
     const bool row_on = r < R;
     extern __shared__ unsigned char lds_raw[];
     const T* Wm = nullptr;
-    if (!ulv_front<T, CPT>(I, nd, f, lds_raw, Wm)) return;
+    if (!ulv_front<T, CPT>(I, nd, f, lds_raw, Wm, lds_elems)) return;
     const int vstride = S * (CPT + 2) + 2;
     T* Us = reinterpret_cast<T*>(lds_raw);
     T* Mq = Us + (size_t)m * nd.ku + 2 * (size_t)k * k;
@@ -965,7 +966,7 @@ __global__ __launch_bounds__(NT) void k_ulv_factor(const FInst<T>* __restrict__ 
 // plus the T blocks.
 template <typename T, int NTILE, int NT>
 __global__ __launch_bounds__(NT) void k_ulv_factor_mf(const FInst<T>* __restrict__ insts,
-                                                      const WorkItem* __restrict__ work) {
+                                                      const WorkItem* __restrict__ work, int64_t lds_elems) {
     using MF = Mfma<T>;
     using V4 = typename MF::v4;
     constexpr int NC = 16 * NTILE;  // columns held per row (n <= NC)
@@ -975,7 +976,7 @@ __global__ __launch_bounds__(NT) void k_ulv_factor_mf(const FInst<T>* __restrict
     const FNode f = I.fn[wi.node];
     extern __shared__ unsigned char lds_raw[];
     const T* Wm = nullptr;
-    if (!ulv_front<T, FACT_CPT>(I, nd, f, lds_raw, Wm)) return;
+    if (!ulv_front<T, FACT_CPT>(I, nd, f, lds_raw, Wm, lds_elems)) return;
     const int m = f.m, n = f.n, k = f.k, nk = f.nk, kv = nd.kv, R = m + kv;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -2738,7 +2739,7 @@ static hipError_t launch_factor_nt(const FInst<T>* insts, const WorkItem* work, 
     const void* fn = reinterpret_cast<const void*>(&k_ulv_factor<T, FACT_CPT, NT>);
     hipError_t e = set_lds_attr<T>(fn, lds);
     if (e != hipSuccess) return e;
-    klaunch((k_ulv_factor<T, FACT_CPT, NT>), dim3(nwork), dim3(NT), lds, st, insts, work);
+    klaunch((k_ulv_factor<T, FACT_CPT, NT>), dim3(nwork), dim3(NT), lds, st, insts, work, (int64_t)(lds / sizeof(T)));
     return hipGetLastError();
 }
 template <typename T, int NTILE, int NT>
@@ -2747,7 +2748,7 @@ static hipError_t launch_factor_mf_nt(const FInst<T>* insts, const WorkItem* wor
     const void* fn = reinterpret_cast<const void*>(&k_ulv_factor_mf<T, NTILE, NT>);
     hipError_t e = set_lds_attr<T>(fn, lds);
     if (e != hipSuccess) return e;
-    klaunch((k_ulv_factor_mf<T, NTILE, NT>), dim3(nwork), dim3(NT), lds, st, insts, work);
+    klaunch((k_ulv_factor_mf<T, NTILE, NT>), dim3(nwork), dim3(NT), lds, st, insts, work, (int64_t)(lds / sizeof(T)));
     return hipGetLastError();
 }
 template <typename T, int NTILE>

@@ -1997,6 +1997,12 @@ static hss_status factor_batch(const std::vector<hss_mat_s*>& mats, hss_prec pre
         }
         lev[h].count = (int)work.size() - lev[h].off;
         lev_lds_mf[h] = std::max(lev_lds_mf[h], lev_lds[h]);
+        // a level of at most one workgroup per CU (256 CUs) gets all 160 KB of LDS: the
+        // occupancy does not change, and the merge stages operands beyond the node's formula
+        if (lev[h].count <= 256) {
+            lev_lds[h] = std::max(lev_lds[h], (size_t)160 * 1024);
+            lev_lds_mf[h] = std::max(lev_lds_mf[h], (size_t)160 * 1024);
+        }
     }
     // Float64 levels run the blocked MFMA kernel when their nodes have n <= 128 and at most
     // factor_mf_max_rows rows of [D; V'] (others run the per-reflector kernel).  Float32
